@@ -1,0 +1,217 @@
+/*
+ * q2d_gpu.h — C-ABI of the MI355X explicit finite-volume step.
+ *
+ * The library replaces one call of the Q2D Fortran host:
+ *
+ *     call Explicit_Step ( domain(level), External_Function )
+ *
+ * made from Q2D_solve (src/lib/driver/Wrap_Solve.f90:19-32, body in
+ * src/lib/numerics/time/explicit/Mod_Explicit.f90:10-174).  Everything that
+ * step does on the hot path — Compute_dt / Set_Global_dt (Mod_dt.f90:10-128),
+ * Copy_State (Mod_Explicit.f90:177-197), and per RK stage Fill_Ghost_Cell
+ * (Lib_Ghost.f90:16-140), Fluxes (Mod_Fluxes.f90:97-234), BC_Fluxes
+ * (Mod_BC_Fluxes.f90:49-271), Ghost_Wall_Extrapolation (Lib_Ghost.f90:366),
+ * Compute_Sources (Mod_Sources.f90:79-91) and RK_Newstate
+ * (Lib_Newstate.f90:14-161), then Compute_Residual
+ * (Mod_Diagnostic.f90:438-469) — runs on the GPU behind these entry points.
+ *
+ * Memory layouts accepted at the boundary are the reference's own Fortran
+ * allocations (Mod_Allocate_Data.f90:100-126), so a Fortran host passes
+ * c_loc() of its arrays without repacking:
+ *
+ *   P      (nprim, 1-gc:ni+gc, 1-gc:nj+gc)        nprim = 4 : rho, u, v, p
+ *   dl     (1-gc:ni+gc, 1-gc:nj+gc) % c(2)        -> 2 doubles per cell
+ *   M      (1-gc:ni+gc, 1-gc:nj+gc) % c(2,2)      -> 4 doubles per cell, column major
+ *   sur    (1-gc:ni+gc, 1-gc:nj+gc)
+ *   dir(1)%f(0:ni, 1:nj) {N(2), int_l}            -> 3 doubles per i-face
+ *   dir(2)%f(1:ni, 0:nj) {N(2), int_l}            -> 3 doubles per j-face
+ *   node   (0:ni, 0:nj) % c(2)                    -> 2 doubles per node (viscous / axisymmetric only)
+ *   src_a, src_b (1:ni, 1:nj)                     axis_r_ave / axis_inv_r, or area_geom_term_x / _y
+ *
+ * All functions return Q2D_OK (0) or a negative Q2D_E_* code; no C++
+ * exception crosses the boundary.  q2d_last_error() returns a message for the
+ * last failing call on the calling thread.
+ *
+ * Gas model: single species, calorically perfect (constant cp, gas constant R).
+ * This is the closed form of the FLINT thermodynamic calls the reference makes
+ * (co_rotot_Rtot, f_ss, H0, E0, prim2cons, cons2prim); FLINT itself is not part
+ * of the reference tree.
+ */
+#ifndef Q2D_GPU_H
+#define Q2D_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define Q2D_GC    2
+#define Q2D_NPRIM 4
+
+/* status codes */
+#define Q2D_OK               0
+#define Q2D_E_ARG           -1   /* invalid argument / inconsistent problem description */
+#define Q2D_E_HIP           -2   /* HIP runtime error */
+#define Q2D_E_NOMEM         -3   /* device allocation failed or budget exceeded */
+#define Q2D_E_STATE         -4   /* NaN or p<0 after an update (Lib_Newstate.f90:327-331) */
+#define Q2D_E_INTERFACE     -5   /* non-physical reconstructed interface state (Lib_Convective.f90:49-54) */
+#define Q2D_E_UNSUPPORTED   -6   /* option not implemented on this path */
+#define Q2D_E_NODEVICE      -7   /* no usable GPU */
+
+/* limiter ids — Mod_Space.f90:44-78 ('minmod','vanleer',...; default IORD) */
+#define Q2D_LIM_IORD      0
+#define Q2D_LIM_MINMOD    1
+#define Q2D_LIM_VANLEER   2
+#define Q2D_LIM_VANALBADA 3
+#define Q2D_LIM_MC        4
+#define Q2D_LIM_SUPERBEE  5
+#define Q2D_LIM_LIMO3     6
+
+/* Riemann solver ids — Mod_Riemann.f90:25-109 */
+#define Q2D_RIEMANN_HLLC  0
+#define Q2D_RIEMANN_HLLE  1
+
+/* source ids — Mod_Sources.f90:16-91 ('axisymmetric' | 'area_variation') */
+#define Q2D_SRC_NONE      0
+#define Q2D_SRC_AXI       1
+#define Q2D_SRC_AREA      2
+
+/* arithmetic forms of the face computation */
+#define Q2D_PREC_FP64     0   /* plain: every operation in FP64, reference operation order */
+#define Q2D_PREC_MIXED    1   /* reconstruction, limiter and Riemann flux in FP32; state, residual
+                                 accumulation, RK update, dt and norms in FP64 */
+
+/* BC codes handled on this path (Mod_BC_Fluxes.f90:93-265) */
+#define Q2D_BC_CONNECT    101
+#define Q2D_BC_PERIODIC   201
+#define Q2D_BC_SYMMETRY   300
+#define Q2D_BC_WALL_T     302
+#define Q2D_BC_EXTRAP     400
+#define Q2D_BC_SUPERSONIC 405
+#define Q2D_BC_OUTFLOW    406
+
+/* Numerical scheme of one case: the [Q2D-Numerics]/[Q2D-Physics]/[Q2D-Sources]
+ * keys of input.ini after Assign_Setup (Assign_Setup.f90:11-62). */
+typedef struct q2d_scheme {
+  int32_t n_rk;            /* 1 euler, 2 RK2, 3 RK3  (Lib_RK.f90:8-10) */
+  int32_t limiter;         /* Q2D_LIM_* ; 'first-order' reconstruction == IORD */
+  int32_t riemann;         /* Q2D_RIEMANN_* */
+  int32_t time_accurate;   /* 1: global dt (Set_Global_dt), 0: local dt */
+  int32_t model;           /* 0 Euler, 1 laminar Navier-Stokes */
+  int32_t source;          /* Q2D_SRC_* */
+  int32_t precision;       /* Q2D_PREC_* */
+  int32_t ramp_iter;       /* cfl-rise-threshold */
+  double  cfl, vnn;
+  double  gas_R, gas_cp;   /* gas constant and constant cp */
+  double  mu_ref, T_ref, S_suth, Pr;   /* Sutherland law mu = mu_ref (T/T_ref)^1.5 (T_ref+S)/(T+S); k = mu cp / Pr */
+} q2d_scheme;
+
+/* One structured block, in the reference's allocation layout (see top). */
+typedef struct q2d_block_desc {
+  int32_t ni, nj;
+  int32_t case_id;          /* index of the independent solve this block belongs to */
+  int32_t reserved;
+  const double *P;          /* initial state incl. ghost layers */
+  const double *dl;
+  const double *M;
+  const double *sur;
+  const double *face_i;
+  const double *face_j;
+  const double *node;       /* viscous/axisymmetric only (may be NULL otherwise) */
+  const double *src_a;      /* SRC_AXI: axis_r_ave ; SRC_AREA: area_geom_term_x */
+  const double *src_b;      /* SRC_AXI: axis_inv_r ; SRC_AREA: area_geom_term_y */
+} q2d_block_desc;
+
+/* One boundary-cell record of domain%bc (Advanced_Types_m.f90:78-122, read by
+ * IO_BC.f90:163-...).  b/bs are 1-based block numbers inside the record's case;
+ * i, j, is, js are 1-based cell indices; f, fs in 1..4.  Records of one case
+ * are ordered by block then face, as BC_Fluxes assumes (Mod_BC_Fluxes.f90:79-86). */
+typedef struct q2d_bc_entry {
+  int32_t b, i, j, f, type;
+  int32_t bs, is, js, fs, d11, d12, d21, d22;
+  int32_t case_id;
+  int32_t reserved[2];
+  double  mach;             /* 405: inflow Mach number */
+  double  T;                /* 405: static temperature ; 302: wall temperature Tw */
+  double  p;                /* 405: static pressure ; 406: exit (back) pressure */
+  double  rel_fac;          /* 405/406 relaxation factor */
+  double  alpha;            /* 405: flow angle [rad]; >= 0.5*DBL_MAX means 'normal' */
+} q2d_bc_entry;
+
+/* One independent solve. */
+typedef struct q2d_case_desc {
+  q2d_scheme scheme;
+  int32_t    itermax;       /* iter-threshold */
+  int32_t    reserved;
+  double     res_threshold; /* res-threshold (absolute, on residual(1)) */
+  double     time_threshold;
+} q2d_case_desc;
+
+typedef struct q2d_problem {
+  int32_t nblocks, nbc, ncases, reserved;
+  const q2d_block_desc *blocks;    /* blocks of all cases; block k of case c is blocks[case_first_block(c)+k-1] */
+  const q2d_bc_entry   *bc;        /* all records, grouped by case, then block, then face */
+  const q2d_case_desc  *cases;
+} q2d_problem;
+
+/* Per-case result of one explicit step (Mod_Explicit.f90:36-174). */
+typedef struct q2d_step_out {
+  double  dtglobal;         /* domain%dtglobal */
+  double  time;             /* domain%time */
+  double  residual[4];      /* residuotot(1:nres): sqrt of the summed squares */
+  int32_t iter;             /* domain%iter after the step */
+  int32_t todo;             /* 1 continue, 3 end of simulation (iter/res/time thresholds) */
+  int32_t status;           /* Q2D_OK or Q2D_E_STATE / Q2D_E_INTERFACE */
+  int32_t fail_b, fail_i, fail_j;   /* first failing cell (block 1-based within case) */
+  int32_t reserved[2];
+} q2d_step_out;
+
+typedef struct q2d_handle q2d_handle;
+
+/* Library / device information. */
+const char *q2d_version(void);
+const char *q2d_last_error(void);
+int q2d_device_count(int *count);
+
+/* Create the device-resident solver for a problem (all cases). Copies every
+ * array to device memory; the caller may free its buffers afterwards. */
+int q2d_gpu_create(const q2d_problem *prob, q2d_handle **out);
+int q2d_gpu_destroy(q2d_handle *h);
+
+/* Exactly one Explicit_Step for every active case; out[] has ncases entries
+ * (may be NULL).  Synchronises with the device. */
+int q2d_gpu_step(q2d_handle *h, q2d_step_out *out);
+
+/* nsteps explicit steps without host synchronisation in between (each step a
+ * replay of one captured HIP graph).  Per-case stop conditions are evaluated on
+ * the device: a case whose todo became 3 is frozen.  out[] as for step. */
+int q2d_gpu_run(q2d_handle *h, int32_t nsteps, q2d_step_out *out);
+
+/* Copy block `block` (0-based global index) state to/from P in the reference
+ * layout P(nprim,1-gc:ni+gc,1-gc:nj+gc). */
+int q2d_gpu_download_state(q2d_handle *h, int32_t block, double *P);
+int q2d_gpu_upload_state(q2d_handle *h, int32_t block, const double *P);
+
+/* Local time step of the last step, dtlocal(1:ni,1:nj). */
+int q2d_gpu_download_dt(q2d_handle *h, int32_t block, double *dtlocal);
+
+/* Kernel-time accounting: when enabled, every stage-kernel launch is bracketed
+ * by HIP events on the solver's stream; stats returns the number of timed
+ * stage launches and their summed duration in milliseconds. */
+int q2d_gpu_set_timing(q2d_handle *h, int32_t enable);
+int q2d_gpu_timing_stats(q2d_handle *h, int64_t *launches, double *ms_total);
+
+/* Launch-shape control of the stage kernel (rows per work item, waves per
+ * workgroup).  0 keeps the current value.  Used by the tile autotuner. */
+int q2d_gpu_set_tiling(q2d_handle *h, int32_t rows_per_item, int32_t waves_per_group);
+int q2d_gpu_get_tiling(q2d_handle *h, int32_t *rows_per_item, int32_t *waves_per_group);
+
+/* Work-item count and cell count of the current launch. */
+int q2d_gpu_info(q2d_handle *h, int64_t *cells, int64_t *work_items);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* Q2D_GPU_H */

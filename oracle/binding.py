@@ -1,0 +1,122 @@
+"""ctypes binding of oracle/build/libq2d_oracle.so (test infrastructure)."""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+_LIB = None
+
+
+def lib_path():
+    return HERE / "build" / "libq2d_oracle.so"
+
+
+def build():
+    """Compile the oracle (gcc, see oracle/Makefile)."""
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return lib_path()
+
+
+def load():
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    p = lib_path()
+    if not p.exists():
+        build()
+    lib = C.CDLL(str(p))
+    from q2d_amd.capi import ProblemC, StepOut
+    lib.q2do_create.argtypes = [C.POINTER(ProblemC)]
+    lib.q2do_create.restype = C.c_void_p
+    lib.q2do_destroy.argtypes = [C.c_void_p]
+    lib.q2do_step.argtypes = [C.c_void_p, C.POINTER(StepOut)]
+    lib.q2do_step.restype = C.c_int
+    for f in ("q2do_get_state", "q2do_set_state", "q2do_get_dt"):
+        getattr(lib, f).argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+        getattr(lib, f).restype = C.c_int
+    lib.q2do_set_threads.argtypes = [C.c_int]
+    lib.q2do_set_threads.restype = C.c_int
+    lib.q2do_get_threads.restype = C.c_int
+    lib.q2do_limiter.argtypes = [C.c_int, C.c_double, C.c_double]
+    lib.q2do_limiter.restype = C.c_double
+    dp = C.POINTER(C.c_double)
+    lib.q2do_reconstruct.argtypes = [C.c_int] + [C.c_double] * 10 + [dp, dp]
+    lib.q2do_riemann.argtypes = [C.c_int, C.c_double, C.c_double, dp, dp, C.c_double, C.c_double, dp]
+    lib.q2do_roe_averages.argtypes = [C.c_double, C.c_double, dp, dp, dp]
+    lib.q2do_convective_flux.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, dp, C.c_double, C.c_double,
+                                         C.c_double, dp, dp]
+    lib.q2do_convective_flux.restype = C.c_int
+    _LIB = lib
+    return lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Oracle:
+    """The CPU restatement run on a cases.Problem."""
+
+    def __init__(self, problem, threads=None):
+        from q2d_amd.capi import StepOut, to_problem_c
+        self.lib = load()
+        if threads:
+            self.lib.q2do_set_threads(int(threads))
+        self.problem = problem
+        pc, self._keep = to_problem_c(problem)
+        self.h = self.lib.q2do_create(C.byref(pc))
+        if not self.h:
+            raise RuntimeError("q2do_create failed")
+        self._out = (StepOut * len(problem.cases))()
+
+    def close(self):
+        if self.h:
+            self.lib.q2do_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self):
+        rc = self.lib.q2do_step(self.h, self._out)
+        return rc, [o.as_dict() for o in self._out]
+
+    def run(self, nsteps, stop_on_todo=True):
+        outs = None
+        for _ in range(nsteps):
+            rc, outs = self.step()
+            if rc != 0:
+                return rc, outs
+            if stop_on_todo and all(o["todo"] == 3 for o in outs):
+                break
+        return 0, outs
+
+    def state(self, block):
+        b = self.problem.blocks[block]
+        P = np.empty((b.nj + 4, b.ni + 4, 4))
+        self.lib.q2do_get_state(self.h, block, P.ctypes.data_as(C.c_void_p))
+        return P
+
+    def dtlocal(self, block):
+        b = self.problem.blocks[block]
+        d = np.empty((b.nj, b.ni))
+        self.lib.q2do_get_dt(self.h, block, d.ctypes.data_as(C.c_void_p))
+        return d
+
+    # unit functions
+    def limiter(self, lid, x, y):
+        return self.lib.q2do_limiter(lid, x, y)
+
+    def riemann(self, rid, R, cp, L, Rs, nx, ny):
+        L = np.ascontiguousarray(L, dtype=np.float64)
+        Rs = np.ascontiguousarray(Rs, dtype=np.float64)
+        F = np.zeros(4)
+        self.lib.q2do_riemann(rid, R, cp, _dp(L), _dp(Rs), nx, ny, _dp(F))
+        return F

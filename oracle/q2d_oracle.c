@@ -1,0 +1,1038 @@
+/*
+ * q2d_oracle.c — CPU restatement of Q2D's explicit step (TEST INFRASTRUCTURE).
+ *
+ * Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg use this
+ * file.  Built with -ffp-contract=off and without fast-math: each expression
+ * below keeps the evaluation order of the Fortran it restates (Fortran and C
+ * both evaluate a*b*c as (a*b)*c and a+b-c as (a+b)-c), so the FP64 results
+ * are those of the reference source read literally.
+ *
+ * Gas closure (the reference calls the absent FLINT library for these):
+ * single species, constant cp, gas constant R, cv = cp - R, gamma = cp/cv.
+ *   T(p,rho)   = p / (rho*R)
+ *   h(T)       = cp*T            (FLINT h_tab interpolation for constant cp)
+ *   a(p,rho)   = sqrt(gamma*p/rho)                                (f_ss)
+ *   H0         = cp*T + 0.5*|V|*|V|                               (H0)
+ *   E0         = cv*T + 0.5*|V|*|V|                               (E0)
+ *   prim2cons  = (rho, rho*u, rho*v, rho*(cv*T + 0.5*(u*u+v*v)))
+ *   cons2prim  : u=U2/rho, v=U3/rho, e=U4/rho-0.5*(u*u+v*v), T=e/cv, p=rho*R*T
+ * Parity of these closures with FLINT is unpinned: FLINT is an un-vendored
+ * submodule (.gitmodules:4-6) with no pinned revision.
+ */
+#include "q2d_oracle.h"
+
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <float.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define GC 2
+#define NP 4
+
+/* ------------------------------------------------------------------------ */
+/* gas closure                                                               */
+/* ------------------------------------------------------------------------ */
+typedef struct { double R, cp, cv, gam; } gas_t;
+
+static gas_t mkgas(double R, double cp) {
+  gas_t g; g.R = R; g.cp = cp; g.cv = cp - R; g.gam = cp / g.cv; return g;
+}
+static double f_ss(const gas_t *g, double p, double rho) { return sqrt(g->gam * p / rho); }
+static double eos_T(const gas_t *g, double p, double rho) { return p / (rho * g->R); }
+static double f_H0(const gas_t *g, double p, double rho, double vmag) {
+  double T = eos_T(g, p, rho);
+  return g->cp * T + 0.5 * vmag * vmag;
+}
+static double f_E0(const gas_t *g, double p, double rho, double vmag) {
+  double T = eos_T(g, p, rho);
+  return g->cv * T + 0.5 * vmag * vmag;
+}
+static void prim2cons(const gas_t *g, const double *pr, double *u) {
+  double rho = pr[0];
+  double T = eos_T(g, pr[3], rho);
+  u[0] = rho;
+  u[1] = rho * pr[1];
+  u[2] = rho * pr[2];
+  u[3] = rho * (g->cv * T + 0.5 * (pr[1] * pr[1] + pr[2] * pr[2]));
+}
+static void cons2prim(const gas_t *g, const double *u, double *pr) {
+  double rho = u[0];
+  double vx = u[1] / rho, vy = u[2] / rho;
+  double e = u[3] / rho - 0.5 * (vx * vx + vy * vy);
+  double T = e / g->cv;
+  pr[0] = rho; pr[1] = vx; pr[2] = vy; pr[3] = rho * g->R * T;
+}
+/* check_gas_state: a reconstructed state is physical when rho>0 and p>0. */
+static int check_gas_state(double rho, double p) { return (rho > 0.0 && p > 0.0) ? 0 : 1; }
+
+/* Fortran MIN/MAX of three reals, evaluated left to right. */
+static double min3(double a, double b, double c) { double m = a; if (b < m) m = b; if (c < m) m = c; return m; }
+static double max3(double a, double b, double c) { double m = a; if (b > m) m = b; if (c > m) m = c; return m; }
+
+/* ------------------------------------------------------------------------ */
+/* limiters — Lib_Limiters.f90:26-112                                        */
+/* ------------------------------------------------------------------------ */
+static double lim_minmod(double x, double y) {          /* :36-46 */
+  double r = 0.0;
+  if (x * y < 0.0) return r;
+  r = x;
+  if (fabs(x) > fabs(y)) r = y;
+  return r;
+}
+static double lim_vanleer(double x, double y) {         /* :49-58 */
+  double ru = y / (x + 1e-30);
+  double phi = (ru + fabs(ru)) / (1.0 + fabs(ru));
+  return phi * x;
+}
+static double lim_vanalbada(double x, double y) {       /* :61-70 */
+  double ru = y / (x + 1e-30);
+  double phi = (ru * ru + ru) / (1.0 + ru * ru);
+  return phi * x;
+}
+static double lim_mc(double x, double y) {              /* :73-82: maxval([0, minval([2r, (1+r)/2, 2])]) */
+  double ru = y / (x + 1e-30);
+  double m = min3(2.0 * ru, 0.5 * (1.0 + ru), 2.0);
+  double phi = 0.0; if (m > phi) phi = m;
+  return phi * x;
+}
+static double lim_sb(double x, double y) {              /* :85-94: maxval([0, min(2r,1), min(r,2)]) */
+  double ru = y / (x + 1e-30);
+  double a = 2.0 * ru; if (1.0 < a) a = 1.0;
+  double b = ru; if (2.0 < b) b = 2.0;
+  return max3(0.0, a, b) * x;
+}
+static double lim_limo3(double x, double y) {           /* :98-110 */
+  double ru = y / (x + 1e-30);
+  double t = (2.0 + ru) / 3.0;
+  double inner = min3(2.0 * ru, t, 1.6);
+  double mid = max3(-0.5 * ru, 0.0, inner);
+  double outer = t; if (mid < outer) outer = mid;
+  double phi = 0.0; if (outer > phi) phi = outer;
+  return phi * x;
+}
+static double limiter(int id, double x, double y) {
+  switch (id) {
+    case Q2D_LIM_MINMOD: return lim_minmod(x, y);
+    case Q2D_LIM_VANLEER: return lim_vanleer(x, y);
+    case Q2D_LIM_VANALBADA: return lim_vanalbada(x, y);
+    case Q2D_LIM_MC: return lim_mc(x, y);
+    case Q2D_LIM_SUPERBEE: return lim_sb(x, y);
+    case Q2D_LIM_LIMO3: return lim_limo3(x, y);
+    default: return 0.0;                                /* rlimiter_IORD :141-148 */
+  }
+}
+
+/* Reconstruction — Lib_Reconstruction.f90:10-29 */
+static void reconstruct(int lim, double prev, double local, double next, double next2,
+                        double dl0, double dl1, double dl2, double dll, double dlr, double beta,
+                        double *pl, double *pr) {
+  double s0 = (local - prev) / dl0;
+  double s1 = (next - local) / dl1;
+  double s2 = (next2 - next) / dl2;
+  double sl = limiter(lim, s1, s0);
+  double sr = limiter(lim, s2, s1);
+  *pl = local + beta * sl * dll;
+  *pr = next - beta * sr * dlr;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Riemann solvers — Lib_Riemann_HLL.f90                                      */
+/* ------------------------------------------------------------------------ */
+typedef struct { double h0l, h0r, rho, u, v, a, h0; } roe_t;
+
+/* roe_averages :229-317 (one species, constant-cp tables: h = cp*T, cp(T) = cp) */
+static roe_t roe_averages(const gas_t *g, double dl, double dr, double dltot, double drtot,
+                          double ul, double vl, double ur, double vr, double pl, double pr) {
+  roe_t o;
+  double Rl = g->R, Rr = g->R;
+  double inv_dltot = 1.0 / dltot;
+  double inv_drtot = 1.0 / drtot;
+  double Tl = pl * inv_dltot / Rl;
+  double Tr = pr * inv_drtot / Rr;
+  double srL = sqrt(dltot);
+  double srR = sqrt(drtot);
+  double inv_sr = 1.0 / (srL + srR);
+  o.rho = srL * srR;
+  o.u = (srR * ur + srL * ul) * inv_sr;
+  o.v = (srR * vr + srL * vl) * inv_sr;
+  double R_roe = (srR * Rr + srL * Rl) * inv_sr;
+  double T_roe = (srR * Tr + srL * Tl) * inv_sr;
+  double h0l = 0.0, h0r = 0.0, cv_roe = 0.0, sum_ei = 0.0;
+  {
+    double invW = g->R;
+    double hl = g->cp * Tl;
+    double hr = g->cp * Tr;
+    double el = hl - invW * Tl;
+    double er = hr - invW * Tr;
+    h0l = h0l + hl * dl * inv_dltot;
+    h0r = h0r + hr * dr * inv_drtot;
+    double d_roe = (srR * dr * inv_drtot + srL * dl * inv_dltot) * inv_sr;
+    cv_roe = cv_roe + d_roe * (0.5 * (g->cp + g->cp) - invW);
+    sum_ei = sum_ei + d_roe * (srR * er + srL * el) * inv_sr;
+  }
+  h0l = h0l + 0.5 * (ul * ul + vl * vl);
+  h0r = h0r + 0.5 * (ur * ur + vr * vr);
+  o.h0 = (srR * h0r + srL * h0l) * inv_sr;
+  double gam_roe = 1.0 + R_roe / cv_roe;
+  double vel2 = o.u * o.u + o.v * o.v;
+  o.a = sqrt((gam_roe - 1.0) * (o.h0 - 0.5 * vel2 + cv_roe * T_roe - sum_ei));
+  o.h0l = h0l; o.h0r = h0r;
+  return o;
+}
+
+/* riemann_HLLC :34-123 */
+static void riemann_hllc(const gas_t *g, double dl, double ul, double vl, double pl, double al, double dltot,
+                         double dr, double ur, double vr, double pr, double ar, double drtot,
+                         double nx, double ny, double *F) {
+  double unl = ul * nx + vl * ny;
+  double unr = ur * nx + vr * ny;
+  roe_t ra = roe_averages(g, dl, dr, dltot, drtot, ul, vl, ur, vr, pl, pr);
+  double h0l = ra.h0l, h0r = ra.h0r;
+  double un_roe = ra.u * nx + ra.v * ny;
+  double e0l = h0l - pl / dltot;
+  double e0r = h0r - pr / drtot;
+  double S1 = min3(0.0, unl - al, un_roe - ra.a);
+  double S4 = max3(0.0, unr + ar, un_roe + ra.a);
+  double inv_denom = 1.0 / (dltot * (S1 - unl) - drtot * (S4 - unr));
+  double Sstar = (pr - pl + dltot * unl * (S1 - unl) - drtot * unr * (S4 - unr)) * inv_denom;
+  double pstar = dltot * (unl - S1) * (unl - Sstar) + pl;
+  double F_r, F_u, F_v, F_E;
+  if (S1 >= 0.0) {
+    F_r = dltot * unl;
+    F_u = F_r * ul + pl * nx;
+    F_v = F_r * vl + pl * ny;
+    F_E = F_r * h0l;
+  } else if (Sstar >= 0.0) {
+    F_r = dltot * unl;
+    F_u = F_r * ul + pl * nx;
+    F_v = F_r * vl + pl * ny;
+    F_E = F_r * h0l;
+    inv_denom = 1.0 / (S1 - Sstar);
+    double U1S = dltot * (S1 - unl) * inv_denom;
+    double U2S = ((S1 - unl) * dltot * ul + (pstar - pl) * nx) * inv_denom;
+    double U3S = ((S1 - unl) * dltot * vl + (pstar - pl) * ny) * inv_denom;
+    double U5S = ((S1 - unl) * dltot * e0l - pl * unl + pstar * Sstar) * inv_denom;
+    F_r = F_r + S1 * (U1S - dltot);
+    F_u = F_u + S1 * (U2S - dltot * ul);
+    F_v = F_v + S1 * (U3S - dltot * vl);
+    F_E = F_E + S1 * (U5S - dltot * e0l);
+  } else if (S4 > 0.0) {
+    F_r = drtot * unr;
+    F_u = F_r * ur + pr * nx;
+    F_v = F_r * vr + pr * ny;
+    F_E = F_r * h0r;
+    inv_denom = 1.0 / (S4 - Sstar);
+    double U1S = drtot * (S4 - unr) * inv_denom;
+    double U2S = ((S4 - unr) * drtot * ur + (pstar - pr) * nx) * inv_denom;
+    double U3S = ((S4 - unr) * drtot * vr + (pstar - pr) * ny) * inv_denom;
+    double U5S = ((S4 - unr) * drtot * e0r - pr * unr + pstar * Sstar) * inv_denom;
+    F_r = F_r + S4 * (U1S - drtot);
+    F_u = F_u + S4 * (U2S - drtot * ur);
+    F_v = F_v + S4 * (U3S - drtot * vr);
+    F_E = F_E + S4 * (U5S - drtot * e0r);
+  } else {
+    F_r = drtot * unr;
+    F_u = F_r * ur + pr * nx;
+    F_v = F_r * vr + pr * ny;
+    F_E = F_r * h0r;
+  }
+  F[0] = F_r; F[1] = F_u; F[2] = F_v; F[3] = F_E;
+}
+
+/* riemann_HLLE :126-169 */
+static void riemann_hlle(const gas_t *g, double dl, double ul, double vl, double pl, double al, double dltot,
+                         double dr, double ur, double vr, double pr, double ar, double drtot,
+                         double nx, double ny, double *F) {
+  double unl = ul * nx + vl * ny;
+  double unr = ur * nx + vr * ny;
+  roe_t ra = roe_averages(g, dl, dr, dltot, drtot, ul, vl, ur, vr, pl, pr);
+  double h0l = ra.h0l, h0r = ra.h0r;
+  double un_roe = ra.u * nx + ra.v * ny;
+  double S1 = min3(0.0, unl - al, un_roe - ra.a);
+  double S4 = max3(0.0, unr + ar, un_roe + ra.a);
+  double inv_denom = 1.0 / (S4 - S1);
+  double FrL = dltot * unl;
+  double FrR = drtot * unr;
+  F[0] = (S4 * FrL - S1 * FrR + S1 * S4 * (drtot - dltot)) * inv_denom;
+  F[1] = (S4 * (FrL * ul + pl * nx) - S1 * (FrR * ur + pr * nx) + S1 * S4 * (drtot * ur - dltot * ul)) * inv_denom;
+  F[2] = (S4 * (FrL * vl + pl * ny) - S1 * (FrR * vr + pr * ny) + S1 * S4 * (drtot * vr - dltot * vl)) * inv_denom;
+  F[3] = (S4 * (FrL * h0l) - S1 * (FrR * h0r)
+          + S1 * S4 * (drtot * (h0r - pr / drtot) - dltot * (h0l - pl / dltot))) * inv_denom;
+}
+
+/* Riemann pointer bound by Assign_Riemann_Solver (Mod_Riemann.f90:25-109).
+ * Argument list of riemann_if: (dl,ul,vl,pl,al,dltot, dr,ur,vr,pr,ar,drtot, beta, nx,ny). */
+static void riemann(int id, const gas_t *g, double dl, double ul, double vl, double pl, double al, double dltot,
+                    double dr, double ur, double vr, double pr, double ar, double drtot,
+                    double nx, double ny, double *F) {
+  if (id == Q2D_RIEMANN_HLLE)
+    riemann_hlle(g, dl, ul, vl, pl, al, dltot, dr, ur, vr, pr, ar, drtot, nx, ny, F);
+  else
+    riemann_hllc(g, dl, ul, vl, pl, al, dltot, dr, ur, vr, pr, ar, drtot, nx, ny, F);
+}
+
+/* ------------------------------------------------------------------------ */
+/* block storage in the reference layout                                     */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+  int ni, nj, ngi, case_id;
+  double *P, *PO, *R;                 /* P(4,1-gc:ni+gc,1-gc:nj+gc) */
+  double *dl, *M, *sur;               /* ghost extents */
+  double *fi, *fj;                    /* dir(1)%f(0:ni,1:nj), dir(2)%f(1:ni,0:nj): {N1,N2,int_l} */
+  double *node;                       /* node(0:ni,0:nj)%c(2) (optional) */
+  double *sa, *sb;                    /* (1:ni,1:nj) source geometry (optional) */
+  double *dtlocal;                    /* (1:ni,1:nj) */
+} oblk;
+
+static inline int cidx(const oblk *b, int i, int j) { return (j + 1) * b->ngi + (i + 1); }
+#define PV(b, v, i, j)  ((b)->P[cidx((b), (i), (j)) * NP + (v)])
+#define POV(b, v, i, j) ((b)->PO[cidx((b), (i), (j)) * NP + (v)])
+#define RV(b, v, i, j)  ((b)->R[cidx((b), (i), (j)) * NP + (v)])
+#define DL(b, d, i, j)  ((b)->dl[cidx((b), (i), (j)) * 2 + (d)])
+/* M(i,j)%c(r,c) with r,c in 1..2, column major */
+#define MM(b, r, c, i, j) ((b)->M[cidx((b), (i), (j)) * 4 + ((r) - 1) + 2 * ((c) - 1)])
+#define SUR(b, i, j)    ((b)->sur[cidx((b), (i), (j))])
+#define FI(b, k, i, j)  ((b)->fi[(((j) - 1) * ((b)->ni + 1) + (i)) * 3 + (k)])
+#define FJ(b, k, i, j)  ((b)->fj[((j) * (b)->ni + ((i) - 1)) * 3 + (k)])
+#define IN(b, i, j)     (((j) - 1) * (b)->ni + ((i) - 1))
+#define NODE(b, k, i, j) ((b)->node[((j) * ((b)->ni + 1) + (i)) * 2 + (k)])
+
+/* guide(6,3) — Parameters_m.f90:13-18 */
+static const int guide[5][2] = { {0, 0}, {1, 0}, {-1, 0}, {0, 1}, {0, -1} };
+
+typedef struct {
+  q2d_case_desc d;
+  gas_t gas;
+  int first_block, nblocks, first_bc, nbc;
+  int iter, iter_from_call;
+  double time, dtglobal;
+  int ndir;
+} ocase;
+
+typedef struct {
+  q2d_bc_entry e;
+  double cos_a, sin_a;                 /* 405 inflow direction (Compute_Inflow_Direction) */
+} obc;
+
+struct q2do {
+  int nblocks, nbc, ncases;
+  oblk *blk;
+  obc *bc;
+  ocase *cs;
+};
+
+static double *dupd(const double *src, size_t n) {
+  double *d = (double *)malloc(n * sizeof(double));
+  if (!d) return NULL;
+  if (src) memcpy(d, src, n * sizeof(double)); else memset(d, 0, n * sizeof(double));
+  return d;
+}
+
+q2do *q2do_create(const q2d_problem *prob) {
+  if (!prob || prob->nblocks <= 0 || prob->ncases <= 0) return NULL;
+  q2do *o = (q2do *)calloc(1, sizeof(q2do));
+  o->nblocks = prob->nblocks; o->nbc = prob->nbc; o->ncases = prob->ncases;
+  o->blk = (oblk *)calloc((size_t)o->nblocks, sizeof(oblk));
+  o->bc = (obc *)calloc((size_t)(o->nbc > 0 ? o->nbc : 1), sizeof(obc));
+  o->cs = (ocase *)calloc((size_t)o->ncases, sizeof(ocase));
+  for (int b = 0; b < o->nblocks; b++) {
+    const q2d_block_desc *d = &prob->blocks[b];
+    oblk *k = &o->blk[b];
+    k->ni = d->ni; k->nj = d->nj; k->ngi = d->ni + 2 * GC; k->case_id = d->case_id;
+    size_t ncg = (size_t)(d->ni + 2 * GC) * (size_t)(d->nj + 2 * GC);
+    k->P = dupd(d->P, ncg * NP);
+    k->PO = dupd(d->P, ncg * NP);
+    k->R = dupd(NULL, ncg * NP);
+    k->dl = dupd(d->dl, ncg * 2);
+    k->M = dupd(d->M, ncg * 4);
+    k->sur = dupd(d->sur, ncg);
+    k->fi = dupd(d->face_i, (size_t)(d->ni + 1) * d->nj * 3);
+    k->fj = dupd(d->face_j, (size_t)d->ni * (d->nj + 1) * 3);
+    k->node = d->node ? dupd(d->node, (size_t)(d->ni + 1) * (d->nj + 1) * 2) : NULL;
+    k->sa = d->src_a ? dupd(d->src_a, (size_t)d->ni * d->nj) : NULL;
+    k->sb = d->src_b ? dupd(d->src_b, (size_t)d->ni * d->nj) : NULL;
+    k->dtlocal = dupd(NULL, (size_t)d->ni * d->nj);
+  }
+  for (int c = 0; c < o->ncases; c++) {
+    ocase *cs = &o->cs[c];
+    cs->d = prob->cases[c];
+    cs->gas = mkgas(cs->d.scheme.gas_R, cs->d.scheme.gas_cp);
+    cs->first_block = -1; cs->first_bc = -1;
+    cs->dtglobal = 0.0; cs->time = 0.0;
+  }
+  for (int b = 0; b < o->nblocks; b++) {
+    ocase *cs = &o->cs[o->blk[b].case_id];
+    if (cs->first_block < 0) cs->first_block = b;
+    cs->nblocks++;
+  }
+  for (int i = 0; i < o->nbc; i++) {
+    o->bc[i].e = prob->bc[i];
+    ocase *cs = &o->cs[prob->bc[i].case_id];
+    if (cs->first_bc < 0) cs->first_bc = i;
+    cs->nbc++;
+  }
+  for (int c = 0; c < o->ncases; c++) {
+    ocase *cs = &o->cs[c];
+    /* Check_Mesh_Type (Lib_Metrics.f90:24-37): ndir from the first block */
+    cs->ndir = (o->blk[cs->first_block].nj == 1) ? 1 : 2;
+  }
+  /* Compute_Inflow_Direction (Lib_BC_Fluxes_Inflow.f90:548-560), evaluated once per record. */
+  for (int i = 0; i < o->nbc; i++) {
+    q2d_bc_entry *e = &o->bc[i].e;
+    if (e->type != Q2D_BC_SUPERSONIC) continue;
+    oblk *k = &o->blk[o->cs[e->case_id].first_block + e->b - 1];
+    double n1, n2;
+    if (e->f <= 2) { int fi = e->i - (e->f % 2); n1 = FI(k, 0, fi, e->j); n2 = FI(k, 1, fi, e->j); }
+    else { int fj = e->j - (e->f % 2); n1 = FJ(k, 0, e->i, fj); n2 = FJ(k, 1, e->i, fj); }
+    double a = (e->alpha >= 0.5 * DBL_MAX) ? atan(n2 / (n1 + 1e-20)) : e->alpha;
+    o->bc[i].cos_a = cos(a); o->bc[i].sin_a = sin(a);
+  }
+  return o;
+}
+
+void q2do_destroy(q2do *o) {
+  if (!o) return;
+  for (int b = 0; b < o->nblocks; b++) {
+    oblk *k = &o->blk[b];
+    free(k->P); free(k->PO); free(k->R); free(k->dl); free(k->M); free(k->sur);
+    free(k->fi); free(k->fj); free(k->node); free(k->sa); free(k->sb); free(k->dtlocal);
+  }
+  free(o->blk); free(o->bc); free(o->cs); free(o);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Compute_dt — Mod_dt.f90:10-104 ; Set_Global_dt :109-128                   */
+/* ------------------------------------------------------------------------ */
+static void compute_dt(q2do *o, ocase *cs) {
+  const q2d_scheme *s = &cs->d.scheme;
+  double dtglobal = cs->dtglobal;
+  for (int bb = 0; bb < cs->nblocks; bb++) {
+    oblk *b = &o->blk[cs->first_block + bb];
+    double dtmin_blk = dtglobal;
+#ifdef _OPENMP
+#pragma omp parallel for reduction(min : dtmin_blk) schedule(static)
+#endif
+    for (int j = 1; j <= b->nj; j++) {
+      for (int i = 1; i <= b->ni; i++) {
+        double rho = PV(b, 0, i, j);
+        double sound = f_ss(&cs->gas, PV(b, 3, i, j), rho);
+        double dtmin = 1e8;
+        for (int d = 1; d <= cs->ndir; d++) {
+          double m1 = MM(b, d, 1, i, j), m2 = MM(b, d, 2, i, j);
+          double nrm = sqrt(m1 * m1 + m2 * m2);
+          double v1 = m1 / nrm, v2 = m2 / nrm;
+          double vel_ = fabs(PV(b, 1, i, j) * v1 + PV(b, 2, i, j) * v2);
+          double lambda = vel_ + sound;
+          double dt = DL(b, d - 1, i, j) / lambda * s->cfl;
+          if (dt < dtmin) dtmin = dt;
+        }
+        /* model>0: the VNN condition of Mod_dt.f90:88-102 is not part of this oracle */
+        double dtcell = dtmin;
+        if (cs->iter < s->ramp_iter) dtcell = dtcell * cs->iter / s->ramp_iter;
+        b->dtlocal[IN(b, i, j)] = dtcell;
+        if (dtcell < dtmin_blk) dtmin_blk = dtcell;
+      }
+    }
+    if (dtmin_blk < dtglobal) dtglobal = dtmin_blk;
+  }
+  cs->dtglobal = dtglobal;
+}
+
+static void set_global_dt(q2do *o, ocase *cs) {
+  for (int bb = 0; bb < cs->nblocks; bb++) {
+    oblk *b = &o->blk[cs->first_block + bb];
+    for (int k = 0; k < b->ni * b->nj; k++) b->dtlocal[k] = cs->dtglobal;
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Fill_Ghost_Cell — Lib_Ghost.f90:16-140                                    */
+/* ------------------------------------------------------------------------ */
+static void ghost_connection(oblk *bm, int Im, int Jm, int Fm, const oblk *bs, int Is, int Js, int Fs) {
+  /* Ghost_Connection :143-160 */
+  for (int g = 1; g <= GC; g++) {
+    int Ig = Im - guide[Fm][0] * g, Jg = Jm - guide[Fm][1] * g;
+    int Id = Is + guide[Fs][0] * (g - 1), Jd = Js + guide[Fs][1] * (g - 1);
+    for (int v = 0; v < NP; v++) PV(bm, v, Ig, Jg) = bs->P[cidx(bs, Id, Jd) * NP + v];
+  }
+}
+static void ghost_symmetry(oblk *b, int Im, int Jm, int Fm) {
+  /* Ghost_Symmetry :163-196 */
+  int Ig = Im - guide[Fm][0], Jg = Jm - guide[Fm][1];
+  double n1, n2;
+  if (Fm <= 2) { int Ic = Im - (Fm % 2); n1 = FI(b, 0, Ic, Jm); n2 = FI(b, 1, Ic, Jm); }
+  else { int Jc = Jm - (Fm % 2); n1 = FJ(b, 0, Im, Jc); n2 = FJ(b, 1, Im, Jc); }
+  double u = PV(b, 1, Im, Jm), v = PV(b, 2, Im, Jm);
+  double udotn = u * n1 + v * n2;
+  u = u - 2.0 * udotn * n1;
+  v = v - 2.0 * udotn * n2;
+  PV(b, 1, Ig, Jg) = u; PV(b, 2, Ig, Jg) = v;
+  PV(b, 0, Ig, Jg) = PV(b, 0, Im, Jm);
+  PV(b, 3, Ig, Jg) = PV(b, 3, Im, Jm);
+}
+static void ghost_zg(oblk *b, int Im, int Jm, int Fm) {
+  /* Ghost_ZG_Extrapolate :199-219 */
+  for (int g = 1; g <= GC; g++) {
+    int Ig = Im - guide[Fm][0] * g, Jg = Jm - guide[Fm][1] * g;
+    for (int v = 0; v < NP; v++) PV(b, v, Ig, Jg) = PV(b, v, Im, Jm);
+  }
+}
+static void ghost_extrap(oblk *b, int Im, int Jm, int Fm) {
+  /* Ghost_Extrapolate :270-299 (x_g = 3 x_{g-1} - 3 x_{g-2} + x_{g-3}) */
+  int di = -guide[Fm][0], dj = -guide[Fm][1];          /* outward step */
+  for (int g = 1; g <= GC; g++) {
+    int I0 = Im + di * g, J0 = Jm + dj * g;
+    int I1 = Im + di * (g - 1), J1 = Jm + dj * (g - 1);
+    int I2 = Im + di * (g - 2), J2 = Jm + dj * (g - 2);
+    int I3 = Im + di * (g - 3), J3 = Jm + dj * (g - 3);
+    for (int v = 0; v < NP; v++)
+      PV(b, v, I0, J0) = 3.0 * PV(b, v, I1, J1) - 3.0 * PV(b, v, I2, J2) + PV(b, v, I3, J3);
+  }
+}
+
+static void fill_ghost(q2do *o, ocase *cs) {
+  for (int k = 0; k < cs->nbc; k++) {
+    const q2d_bc_entry *e = &o->bc[cs->first_bc + k].e;
+    oblk *bm = &o->blk[cs->first_block + e->b - 1];
+    switch (e->type) {
+      case 101: case 201: {
+        const oblk *bs = &o->blk[cs->first_block + e->bs - 1];
+        ghost_connection(bm, e->i, e->j, e->f, bs, e->is, e->js, e->fs);
+        break;
+      }
+      case 300: ghost_symmetry(bm, e->i, e->j, e->f); break;
+      case 0: case 401: case 402: case 403: case 404: case 405: case 406: case 407: case 420:
+        ghost_zg(bm, e->i, e->j, e->f); break;
+      default: ghost_extrap(bm, e->i, e->j, e->f); break;
+    }
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Convective_Flux — Lib_Convective.f90:15-82                                */
+/* ------------------------------------------------------------------------ */
+/* dl[-1..2] -> dl[0..3], prim[c][v] for cells -1..2 -> c = 0..3. Returns 0 or Q2D_E_INTERFACE. */
+static int convective_flux(int lim, int rid, const gas_t *g, const double *dl, double nx, double ny,
+                           double int_l, const double (*prim)[NP], double *Flux) {
+  double l0 = 0.5 * (dl[0] + dl[1]);
+  double l1 = 0.5 * (dl[1] + dl[2]);
+  double l2 = 0.5 * (dl[2] + dl[3]);
+  double lm = 0.5 * dl[1];
+  double lp = 0.5 * dl[2];
+  double beta_ = 1.0;
+  double P1[NP], P4[NP];
+  for (int v = 0; v < NP; v++)
+    reconstruct(lim, prim[0][v], prim[1][v], prim[2][v], prim[3][v], l0, l1, l2, lm, lp, beta_, &P1[v], &P4[v]);
+  if (check_gas_state(P1[0], P1[3]) || check_gas_state(P4[0], P4[3])) return Q2D_E_INTERFACE;
+  double rotot1 = P1[0], rotot4 = P4[0];
+  double a1 = f_ss(g, P1[3], rotot1);
+  double a4 = f_ss(g, P4[3], rotot4);
+  double F[4];
+  riemann(rid, g, P1[0], P1[1], P1[2], P1[3], a1, rotot1, P4[0], P4[1], P4[2], P4[3], a4, rotot4, nx, ny, F);
+  double su = copysign(0.5, F[0]);
+  double sp = 0.5 + su;
+  double sm = su - 0.5;
+  Flux[0] = F[0] * int_l * (sp * (P1[0] / rotot1) - sm * (P4[0] / rotot4));
+  Flux[1] = F[1] * int_l;
+  Flux[2] = F[2] * int_l;
+  Flux[3] = F[3] * int_l;
+  return 0;
+}
+
+/* Fluxes_blk_omp_do — Mod_Fluxes.f90:97-234 (Euler part) */
+static int fluxes_block(oblk *b, const ocase *cs) {
+  const q2d_scheme *s = &cs->d.scheme;
+  int err = 0;
+  for (int j = 1; j <= b->nj; j++)
+    for (int i = 1; i <= b->ni; i++)
+      for (int v = 0; v < NP; v++) RV(b, v, i, j) = 0.0;
+  /* i-faces: rows in parallel, each face once (:140-185) */
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) reduction(| : err)
+#endif
+  for (int j = 1; j <= b->nj; j++) {
+    for (int i = 1; i <= b->ni - 1; i++) {
+      double dl[4], prim[4][NP], F[NP];
+      for (int c = 0; c < 4; c++) {
+        dl[c] = DL(b, 0, i - 1 + c, j);
+        for (int v = 0; v < NP; v++) prim[c][v] = PV(b, v, i - 1 + c, j);
+      }
+      err |= -convective_flux(s->limiter, s->riemann, &cs->gas, dl, FI(b, 0, i, j), FI(b, 1, i, j),
+                              FI(b, 2, i, j), (const double(*)[NP])prim, F);
+      for (int v = 0; v < NP; v++) { RV(b, v, i, j) = RV(b, v, i, j) + F[v]; RV(b, v, i + 1, j) = RV(b, v, i + 1, j) - F[v]; }
+    }
+  }
+  /* j-faces: columns in parallel (:187-232) */
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) reduction(| : err)
+#endif
+  for (int i = 1; i <= b->ni; i++) {
+    for (int j = 1; j <= b->nj - 1; j++) {
+      double dl[4], prim[4][NP], F[NP];
+      for (int c = 0; c < 4; c++) {
+        dl[c] = DL(b, 1, i, j - 1 + c);
+        for (int v = 0; v < NP; v++) prim[c][v] = PV(b, v, i, j - 1 + c);
+      }
+      err |= -convective_flux(s->limiter, s->riemann, &cs->gas, dl, FJ(b, 0, i, j), FJ(b, 1, i, j),
+                              FJ(b, 2, i, j), (const double(*)[NP])prim, F);
+      for (int v = 0; v < NP; v++) { RV(b, v, i, j) = RV(b, v, i, j) + F[v]; RV(b, v, i, j + 1) = RV(b, v, i, j + 1) - F[v]; }
+    }
+  }
+  return err ? Q2D_E_INTERFACE : 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* BC fluxes — Mod_BC_Fluxes.f90:49-271                                      */
+/* ------------------------------------------------------------------------ */
+/* Face_Index — Lib_BC_Fluxes.f90:34-50 */
+static void face_geom(const oblk *b, int Fm, int Im, int Jm, int *dir, double *n1, double *n2, double *il) {
+  if (Fm <= 2) {
+    int fi = Im - (Fm % 2);
+    *dir = 1; *n1 = FI(b, 0, fi, Jm); *n2 = FI(b, 1, fi, Jm); *il = FI(b, 2, fi, Jm);
+  } else {
+    int fj = Jm - (Fm % 2);
+    *dir = 2; *n1 = FJ(b, 0, Im, fj); *n2 = FJ(b, 1, Im, fj); *il = FJ(b, 2, Im, fj);
+  }
+}
+
+/* BC_Connection_Eul — Lib_BC_Fluxes_Connection.f90:24-133 */
+static int bc_connection_eul(oblk *b, const ocase *cs, int Im, int Jm, int Fm) {
+  const q2d_scheme *s = &cs->d.scheme;
+  int modfm = Fm % 2, modfm1 = 1 - modfm, modfm2 = 1 - 2 * modfm;
+  int g1 = guide[Fm][0], g2 = guide[Fm][1];
+  int I1 = modfm * (Im - g1 * GC) + modfm1 * (Im + g1);
+  int J1 = modfm * (Jm - g2 * GC) + modfm1 * (Jm + g2);
+  int I2 = modfm1 * Im + modfm * (Im - g1);
+  int J2 = modfm1 * Jm + modfm * (Jm - g2);
+  int I3 = modfm1 * (Im - g1) + modfm * Im;
+  int J3 = modfm1 * (Jm - g2) + modfm * Jm;
+  int I4 = modfm1 * (Im - g1 * GC) + modfm * (Im + g1);
+  int J4 = modfm1 * (Jm - g2 * GC) + modfm * (Jm + g2);
+  int dir; double n1, n2, int_l;
+  face_geom(b, Fm, Im, Jm, &dir, &n1, &n2, &int_l);
+  double Dl1 = DL(b, dir - 1, I1, J1) * 0.5;
+  double Dl2 = DL(b, dir - 1, I2, J2) * 0.5;
+  double Dl3 = DL(b, dir - 1, I3, J3) * 0.5;
+  double Dl4 = DL(b, dir - 1, I4, J4) * 0.5;
+  double Dist23 = Dl2 + Dl3;
+  double Dist12 = Dl1 + Dl2;
+  double Dist34 = Dl3 + Dl4;
+  double PL[NP], PR[NP];
+  for (int v = 0; v < NP; v++)
+    reconstruct(s->limiter, PV(b, v, I1, J1), PV(b, v, I2, J2), PV(b, v, I3, J3), PV(b, v, I4, J4),
+                Dist12, Dist23, Dist34, Dl2, Dl3, 1.0, &PL[v], &PR[v]);
+  if (check_gas_state(PL[0], PL[3]) || check_gas_state(PR[0], PR[3])) return Q2D_E_INTERFACE;
+  double rho_L = PL[0], rho_R = PR[0];
+  double a_L = f_ss(&cs->gas, PL[3], rho_L);
+  double a_R = f_ss(&cs->gas, PR[3], rho_R);
+  double F[4];
+  riemann(s->riemann, &cs->gas, PL[0], PL[1], PL[2], PL[3], a_L, rho_L, PR[0], PR[1], PR[2], PR[3], a_R, rho_R, n1, n2, F);
+  double su = copysign(0.5, F[0]);
+  double sel_L = 0.5 + su, sel_R = su - 0.5;
+  double Fmass = F[0] * int_l;
+  double Flux[NP];
+  Flux[0] = Fmass * (sel_L * PL[0] / rho_L - sel_R * PR[0] / rho_R);
+  Flux[1] = F[1] * int_l;
+  Flux[2] = F[2] * int_l;
+  Flux[3] = F[3] * int_l;
+  for (int v = 0; v < NP; v++) RV(b, v, Im, Jm) = RV(b, v, Im, Jm) + Flux[v] * (double)modfm2;
+  return 0;
+}
+
+/* BC_Symmetry_Eul — Lib_BC_Fluxes_Symmetry.f90:28-124 (Un_wall = 0) */
+static void bc_symmetry_eul(oblk *b, const ocase *cs, int Im, int Jm, int Fm) {
+  const q2d_scheme *s = &cs->d.scheme;
+  int modfm = Fm % 2, modfm1 = 1 - modfm, modfm2 = 1 - 2 * modfm;
+  int Int_i = Im + guide[Fm][0], Int_j = Jm + guide[Fm][1];
+  int dir; double n1, n2, int_l;
+  face_geom(b, Fm, Im, Jm, &dir, &n1, &n2, &int_l);
+  double Bound_Dl = 0.5 * DL(b, dir - 1, Im, Jm);
+  double Int_Dl = 0.5 * DL(b, dir - 1, Int_i, Int_j);
+  double Bp[NP], Ip[NP], Gp[NP], P1[NP], P3[NP], Slope[NP], Fp[NP];
+  for (int v = 0; v < NP; v++) { Bp[v] = PV(b, v, Im, Jm); Ip[v] = PV(b, v, Int_i, Int_j); }
+  double Un = Bp[1] * n1 + Bp[2] * n2;
+  for (int v = 0; v < NP; v++) Gp[v] = Bp[v];
+  Gp[1] = Gp[1] - 2.0 * Un * n1;
+  Gp[2] = Gp[2] - 2.0 * Un * n2;
+  double dm = (double)modfm, dm1 = (double)modfm1, dm2 = (double)modfm2;
+  for (int v = 0; v < NP; v++) P1[v] = dm * Gp[v] + dm1 * Ip[v];
+  double Dist12 = 2.0 * Bound_Dl * dm + (Bound_Dl + Int_Dl) * dm1;
+  for (int v = 0; v < NP; v++) P3[v] = dm * Ip[v] + dm1 * Gp[v];
+  double Dist23 = (Bound_Dl + Int_Dl) * dm + 2.0 * Bound_Dl * dm1;
+  for (int v = 0; v < NP; v++) {
+    double s32 = (P3[v] - Bp[v]) / Dist23;
+    double s21 = (Bp[v] - P1[v]) / Dist12;
+    Slope[v] = limiter(s->limiter, s32, s21);
+  }
+  double Dist = Bound_Dl * dm2;
+  for (int v = 0; v < NP; v++) Fp[v] = Bp[v] + Slope[v] * Dist;
+  double rotot = Fp[0];
+  double sound = f_ss(&cs->gas, Fp[3], rotot);
+  Un = Fp[1] * n1 + Fp[2] * n2;
+  double vt1 = Fp[1] - Un * n1, vt2 = Fp[2] - Un * n2;
+  double Un1 = Un * dm2;
+  double U1x = Un1 * n1 + vt1, U1y = Un1 * n2 + vt2;
+  double U4x = -Un1 * n1 + vt1, U4y = -Un1 * n2 + vt2;
+  double F[4];
+  riemann(s->riemann, &cs->gas, Fp[0], U1x, U1y, Fp[3], sound, rotot, Fp[0], U4x, U4y, Fp[3], sound, rotot, n1, n2, F);
+  double fu = F[1] * int_l, fv = F[2] * int_l;
+  RV(b, 1, Im, Jm) = RV(b, 1, Im, Jm) + dm2 * fu;
+  RV(b, 2, Im, Jm) = RV(b, 2, Im, Jm) + dm2 * fv;
+}
+
+/* BC_Extrapolation — Lib_BC_Fluxes_Extrapolation.f90:18-49 */
+static void bc_extrapolation(oblk *b, const ocase *cs, int Im, int Jm, int Fm) {
+  int dir; double n1, n2, int_l;
+  face_geom(b, Fm, Im, Jm, &dir, &n1, &n2, &int_l);
+  double rho = PV(b, 0, Im, Jm), u = PV(b, 1, Im, Jm), v = PV(b, 2, Im, Jm), p = PV(b, 3, Im, Jm);
+  double Un = u * n1 + v * n2;
+  double Flux[NP];
+  Flux[0] = rho * Un * int_l;
+  double Fmass = Flux[0];
+  Flux[1] = Fmass * u + p * int_l * n1;
+  Flux[2] = Fmass * v + p * int_l * n2;
+  Flux[3] = Fmass * f_H0(&cs->gas, p, rho, sqrt(u * u + v * v));
+  double sgn = (double)(1 - 2 * (Fm % 2));
+  for (int k = 0; k < NP; k++) RV(b, k, Im, Jm) = RV(b, k, Im, Jm) + sgn * Flux[k];
+}
+
+/* BC_Inlet_Supersonic_Static + Supersonic_Inflow — Lib_BC_Fluxes_Inflow.f90:272-403 */
+static void bc_supersonic_inflow(oblk *b, const ocase *cs, const obc *rec, int Im, int Jm, int Fm) {
+  const q2d_scheme *s = &cs->d.scheme;
+  const gas_t *g = &cs->gas;
+  const q2d_bc_entry *e = &rec->e;
+  int modfm = Fm % 2, modfm1 = 1 - modfm, modfm2 = 1 - 2 * modfm;
+  int Int_i = Im + guide[Fm][0], Int_j = Jm + guide[Fm][1];
+  int dir; double n1, n2, int_l;
+  face_geom(b, Fm, Im, Jm, &dir, &n1, &n2, &int_l);
+  double P2[NP], Ip[NP];
+  for (int v = 0; v < NP; v++) { P2[v] = PV(b, v, Im, Jm); Ip[v] = PV(b, v, Int_i, Int_j); }
+  double BC_Sign = (double)modfm2;
+  double Un = P2[1] * n1 + P2[2] * n2;
+  if (BC_Sign * Un >= 0.0) {                           /* :296-299 -> BC_Symmetry_Eul (Mod_BC_Fluxes.f90:221) */
+    bc_symmetry_eul(b, cs, Im, Jm, Fm);
+    return;
+  }
+  double flow_Sound = f_ss(g, P2[3], P2[0]);
+  double flow_Mach = sqrt(P2[1] * P2[1] + P2[2] * P2[2]) / flow_Sound;
+  double Mach_l = e->rel_fac * e->mach + (1.0 - e->rel_fac) * flow_Mach;
+  double p0_l = e->rel_fac * e->p + (1.0 - e->rel_fac) * P2[3];
+  double Sup[NP];
+  Sup[3] = p0_l;
+  double Sup_T = e->T;
+  double Sup_Rgas = 1.0 * g->R;
+  double Sup_rho = Sup[3] / (Sup_Rgas * Sup_T);
+  Sup[0] = Sup_rho * 1.0;
+  double Sup_Sound = f_ss(g, Sup[3], Sup_rho);
+  Sup[1] = Mach_l * Sup_Sound * rec->cos_a;
+  Sup[2] = Mach_l * Sup_Sound * rec->sin_a;
+  double dm = (double)modfm, dm1 = (double)modfm1, dm2 = (double)modfm2;
+  double P1[NP], P3[NP], Face[NP], PL[NP], PR[NP];
+  for (int v = 0; v < NP; v++) { P1[v] = dm * Sup[v] + dm1 * Ip[v]; P3[v] = dm * Ip[v] + dm1 * Sup[v]; }
+  for (int v = 0; v < NP; v++) {
+    double d32 = P3[v] - P2[v], d21 = P2[v] - P1[v];
+    double sl = limiter(s->limiter, d32, d21);
+    Face[v] = P2[v] + 0.5 * sl * dm2;
+  }
+  for (int v = 0; v < NP; v++) { PL[v] = dm * Sup[v] + dm1 * Face[v]; PR[v] = dm * Face[v] + dm1 * Sup[v]; }
+  double rho_L = PL[0], rho_R = PR[0];
+  double a_L = f_ss(g, PL[3], rho_L), a_R = f_ss(g, PR[3], rho_R);
+  double F[4];
+  riemann(s->riemann, g, PL[0], PL[1], PL[2], PL[3], a_L, rho_L, PR[0], PR[1], PR[2], PR[3], a_R, rho_R, n1, n2, F);
+  double su = copysign(1.0, F[0]);
+  double Sel_L = (1.0 + su) / 2.0, Sel_R = (su - 1.0) / 2.0;
+  double Fmass = F[0] * int_l;
+  double Flux[NP];
+  Flux[0] = Fmass * (Sel_L * PL[0] / rho_L - Sel_R * PR[0] / rho_R);
+  Flux[1] = F[1] * int_l;
+  Flux[2] = F[2] * int_l;
+  Flux[3] = F[3] * int_l;
+  for (int v = 0; v < NP; v++) RV(b, v, Im, Jm) = RV(b, v, Im, Jm) + dm2 * Flux[v];
+}
+
+/* BC_Outflow + Outflow — Lib_BC_Fluxes_Outflow.f90:36-165 */
+static void bc_outflow(oblk *b, const ocase *cs, const obc *rec, int Im, int Jm, int Fm) {
+  const gas_t *g = &cs->gas;
+  const q2d_bc_entry *e = &rec->e;
+  int modfm2 = 1 - 2 * (Fm % 2);
+  int dir; double n1, n2, int_l;
+  face_geom(b, Fm, Im, Jm, &dir, &n1, &n2, &int_l);
+  double Bp[NP];
+  for (int v = 0; v < NP; v++) Bp[v] = PV(b, v, Im, Jm);
+  double BC_Sign = (double)modfm2;
+  double Un = Bp[1] * n1 + Bp[2] * n2;
+  if (BC_Sign * Un < 0.0) { bc_symmetry_eul(b, cs, Im, Jm, Fm); return; }
+  double Bound_rho = Bp[0], Bound_Rgas = g->R;
+  double Bound_Sound = f_ss(g, Bp[3], Bound_rho);
+  double Bound_Gamma = g->gam;
+  double Riem = Bound_Sound + BC_Sign * 0.5 * (Bound_Gamma - 1.0) * Un;
+  double VN1 = Un * n1, VN2 = Un * n2;
+  double VT1 = Bp[1] - VN1, VT2 = Bp[2] - VN2;
+  double Mach_N = fabs(Un / Bound_Sound);
+  double Fp[NP], Face_Un, Face_rho, Face_Rgas;
+  if (Mach_N < 1.0 && e->p > 0.0) {
+    double p_exit = e->p;
+    double delta = 0.5 * (Bound_Gamma - 1.0);
+    double ex = Bound_Gamma / (Bound_Gamma - 1.0);
+    double den = 1.0 / pow(1.0 + delta, ex);
+    double pstar = Bp[3] * den * pow(1.0 + delta * (Mach_N * Mach_N), ex);
+    if (pstar >= p_exit) p_exit = pstar;
+    double Isentropic = (Bound_Gamma - 1.0) / (2.0 * Bound_Gamma);
+    Fp[3] = p_exit * e->rel_fac + Bp[3] * (1.0 - e->rel_fac);
+    double Face_Sound = Bound_Sound * pow(Fp[3] / Bp[3], Isentropic);
+    Face_Un = (Riem - Face_Sound) / (BC_Sign * 0.5 * (Bound_Gamma - 1.0));
+    Face_rho = Bound_Gamma * Fp[3] / (Face_Sound * Face_Sound);
+    Fp[0] = Bp[0] / Bound_rho * Face_rho;
+    Fp[1] = VT1 + Face_Un * n1;
+    Fp[2] = VT2 + Face_Un * n2;
+    Face_Rgas = Bound_Rgas;
+  } else {
+    for (int v = 0; v < NP; v++) Fp[v] = Bp[v];
+    Face_Un = Un; Face_rho = Bound_rho; Face_Rgas = Bound_Rgas;
+  }
+  double Face_T = Fp[3] / (Face_Rgas * Face_rho);
+  double Face_Enthalpy = 0.0 + Fp[0] / Face_rho * (g->cp * Face_T);
+  double Flux[NP];
+  Flux[0] = Fp[0] * int_l * Face_Un;
+  double mdot = Face_rho * int_l * Face_Un;
+  Flux[1] = Fp[1] * mdot + Fp[3] * int_l * n1;
+  Flux[2] = Fp[2] * mdot + Fp[3] * int_l * n2;
+  Flux[3] = mdot * (0.5 * (Fp[1] * Fp[1] + Fp[2] * Fp[2]) + Face_Enthalpy);
+  for (int v = 0; v < NP; v++) RV(b, v, Im, Jm) = RV(b, v, Im, Jm) + (double)modfm2 * Flux[v];
+}
+
+static int bc_fluxes(q2do *o, ocase *cs) {
+  int err = 0;
+  for (int k = 0; k < cs->nbc; k++) {
+    const obc *rec = &o->bc[cs->first_bc + k];
+    const q2d_bc_entry *e = &rec->e;
+    oblk *b = &o->blk[cs->first_block + e->b - 1];
+    switch (e->type) {
+      case 101: case 201: { int r = bc_connection_eul(b, cs, e->i, e->j, e->f); if (r) err = r; break; }
+      case 300: bc_symmetry_eul(b, cs, e->i, e->j, e->f); break;
+      case 302: bc_symmetry_eul(b, cs, e->i, e->j, e->f); break;   /* Euler part; viscous wall not in this oracle */
+      case 400: bc_extrapolation(b, cs, e->i, e->j, e->f); break;
+      case 405: bc_supersonic_inflow(b, cs, rec, e->i, e->j, e->f); break;
+      case 406: bc_outflow(b, cs, rec, e->i, e->j, e->f); break;
+      default: break;
+    }
+  }
+  return err;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Sources — Lib_Source_Axisymmetry.f90:40-78, Lib_Source_Areavariation.f90:104-144 */
+/* ------------------------------------------------------------------------ */
+static void sources(q2do *o, ocase *cs) {
+  const q2d_scheme *s = &cs->d.scheme;
+  if (s->source == Q2D_SRC_NONE) return;
+  for (int bb = 0; bb < cs->nblocks; bb++) {
+    oblk *b = &o->blk[cs->first_block + bb];
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+    for (int j = 1; j <= b->nj; j++) {
+      for (int i = 1; i <= b->ni; i++) {
+        double rho = PV(b, 0, i, j), u = PV(b, 1, i, j), v = PV(b, 2, i, j), p = PV(b, 3, i, j);
+        double mod_v = sqrt(u * u + v * v);
+        double S[NP];
+        if (s->source == Q2D_SRC_AXI) {
+          double invr = b->sb[IN(b, i, j)];
+          S[0] = -(rho * v * invr);
+          S[1] = -(rho * u * v * invr);
+          S[2] = -(rho * v * v * invr);
+          S[3] = -((rho * f_E0(&cs->gas, p, rho, mod_v) + p) * v * invr);
+        } else {
+          if (!b->sa) continue;
+          double Gx = b->sa[IN(b, i, j)], Gy = b->sb[IN(b, i, j)];
+          double VdG = u * Gx + v * Gy;
+          S[0] = rho * VdG;
+          S[1] = rho * u * VdG;
+          S[2] = rho * v * VdG;
+          S[3] = (rho * f_E0(&cs->gas, p, rho, mod_v) + p) * VdG;
+        }
+        double sr = SUR(b, i, j);
+        for (int k = 0; k < NP; k++) RV(b, k, i, j) = RV(b, k, i, j) - S[k] * sr;
+      }
+    }
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Newstate_Conservative / Compute_Cons_NoIRS — Lib_Newstate.f90:14-161      */
+/* ------------------------------------------------------------------------ */
+static const double RKcoeff[3][3] = { {1.0, 0.0, 0.0}, {1.0, 0.5, 0.0}, {1.0, 0.25, 2.0 / 3.0} };
+
+static int newstate(q2do *o, ocase *cs, int irk, int *fb, int *fi, int *fj) {
+  const q2d_scheme *s = &cs->d.scheme;
+  double c = RKcoeff[s->n_rk - 1][irk - 1];
+  double strang = 1.0;
+  int err = 0;
+  for (int bb = 0; bb < cs->nblocks; bb++) {
+    oblk *b = &o->blk[cs->first_block + bb];
+    for (int j = 1; j <= b->nj; j++) {
+      for (int i = 1; i <= b->ni; i++) {
+        double dt = b->dtlocal[IN(b, i, j)], sr = SUR(b, i, j);
+        double res[NP], prim[NP], primold[NP], cons[NP], consold[NP];
+        for (int v = 0; v < NP; v++) {
+          res[v] = -RV(b, v, i, j) / sr * dt * strang;
+          prim[v] = PV(b, v, i, j); primold[v] = POV(b, v, i, j);
+        }
+        prim2cons(&cs->gas, primold, consold);
+        prim2cons(&cs->gas, prim, cons);
+        for (int v = 0; v < NP; v++) {                 /* RK_stage — Lib_RK.f90:15-27 */
+          double r_ = (cons[v] - consold[v] + res[v]) * c;
+          cons[v] = consold[v] + r_;
+        }
+        cons2prim(&cs->gas, cons, prim);
+        /* Check_And_Fix_State — Lib_Newstate.f90:316-350 */
+        if (isnan(prim[0]) || isnan(prim[1]) || isnan(prim[2]) || isnan(prim[3]) || prim[3] < 0.0) {
+          if (!err) { err = Q2D_E_STATE; *fb = bb + 1; *fi = i; *fj = j; }
+        }
+        if (prim[0] < 1e-20) prim[0] = 1e-20;
+        for (int v = 0; v < NP; v++) PV(b, v, i, j) = prim[v];
+      }
+    }
+  }
+  return err;
+}
+
+/* Compute_Residual — Mod_Diagnostic.f90:438-469 (row partial sums, rows summed in order) */
+static void residual(q2do *o, ocase *cs, double *tot) {
+  for (int k = 0; k < 4; k++) tot[k] = 0.0;
+  for (int bb = 0; bb < cs->nblocks; bb++) {
+    oblk *b = &o->blk[cs->first_block + bb];
+    double blk[4] = {0, 0, 0, 0};
+    for (int j = 1; j <= b->nj; j++) {
+      double row[4] = {0, 0, 0, 0};
+      for (int i = 1; i <= b->ni; i++) {
+        for (int v = 0; v < 4; v++) {
+          double r = fabs(PV(b, v, i, j) - POV(b, v, i, j));
+          row[v] = row[v] + r * r;
+        }
+      }
+      for (int v = 0; v < 4; v++) blk[v] = blk[v] + row[v];
+    }
+    for (int v = 0; v < 4; v++) tot[v] = tot[v] + blk[v];
+  }
+  for (int v = 0; v < 4; v++) tot[v] = sqrt(tot[v]);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Explicit_Step — Mod_Explicit.f90:10-174                                   */
+/* ------------------------------------------------------------------------ */
+static int step_case(q2do *o, ocase *cs, q2d_step_out *out) {
+  const q2d_scheme *s = &cs->d.scheme;
+  int status = 0, fb = 0, fi = 0, fj = 0;
+  cs->iter += 1;
+  cs->iter_from_call += 1;
+  cs->dtglobal = 1e5;
+  compute_dt(o, cs);
+  if (s->time_accurate) {
+    set_global_dt(o, cs);
+    cs->time = cs->time + cs->dtglobal;
+  }
+  for (int bb = 0; bb < cs->nblocks; bb++) {          /* Copy_State :177-197 */
+    oblk *b = &o->blk[cs->first_block + bb];
+    for (int j = 1; j <= b->nj; j++)
+      for (int i = 1; i <= b->ni; i++)
+        for (int v = 0; v < NP; v++) POV(b, v, i, j) = PV(b, v, i, j);
+  }
+  for (int irk = 1; irk <= s->n_rk; irk++) {
+    fill_ghost(o, cs);
+    for (int bb = 0; bb < cs->nblocks; bb++) {
+      int r = fluxes_block(&o->blk[cs->first_block + bb], cs);
+      if (r && !status) status = r;
+    }
+    { int r = bc_fluxes(o, cs); if (r && !status) status = r; }
+    sources(o, cs);
+    { int r = newstate(o, cs, irk, &fb, &fi, &fj); if (r && !status) status = r; }
+  }
+  double res[4];
+  residual(o, cs, res);
+  int endsim = (cs->iter_from_call >= cs->d.itermax) || (res[0] <= cs->d.res_threshold) ||
+               (cs->time >= cs->d.time_threshold);
+  if (out) {
+    out->dtglobal = cs->dtglobal; out->time = cs->time;
+    for (int k = 0; k < 4; k++) out->residual[k] = res[k];
+    out->iter = cs->iter; out->todo = endsim ? 3 : 1;
+    out->status = status; out->fail_b = fb; out->fail_i = fi; out->fail_j = fj;
+  }
+  return status;
+}
+
+int q2do_step(q2do *o, q2d_step_out *out) {
+  int st = 0;
+  for (int c = 0; c < o->ncases; c++) {
+    int r = step_case(o, &o->cs[c], out ? &out[c] : NULL);
+    if (r && !st) st = r;
+  }
+  return st;
+}
+
+int q2do_get_state(q2do *o, int block, double *P) {
+  if (block < 0 || block >= o->nblocks) return Q2D_E_ARG;
+  oblk *b = &o->blk[block];
+  memcpy(P, b->P, sizeof(double) * NP * (size_t)(b->ni + 2 * GC) * (size_t)(b->nj + 2 * GC));
+  return 0;
+}
+int q2do_set_state(q2do *o, int block, const double *P) {
+  if (block < 0 || block >= o->nblocks) return Q2D_E_ARG;
+  oblk *b = &o->blk[block];
+  memcpy(b->P, P, sizeof(double) * NP * (size_t)(b->ni + 2 * GC) * (size_t)(b->nj + 2 * GC));
+  return 0;
+}
+int q2do_get_dt(q2do *o, int block, double *dt) {
+  if (block < 0 || block >= o->nblocks) return Q2D_E_ARG;
+  oblk *b = &o->blk[block];
+  memcpy(dt, b->dtlocal, sizeof(double) * (size_t)b->ni * (size_t)b->nj);
+  return 0;
+}
+
+int q2do_set_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+  return omp_get_max_threads();
+#else
+  (void)n; return 1;
+#endif
+}
+int q2do_get_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+/* ---- unit-level entry points ---- */
+double q2do_limiter(int id, double x, double y) { return limiter(id, x, y); }
+
+void q2do_reconstruct(int lim, double prev, double local, double next, double next2,
+                      double dl0, double dl1, double dl2, double dll, double dlr, double beta,
+                      double *priml, double *primr) {
+  reconstruct(lim, prev, local, next, next2, dl0, dl1, dl2, dll, dlr, beta, priml, primr);
+}
+
+void q2do_riemann(int id, double gas_R, double gas_cp, const double *L, const double *Rr,
+                  double nx, double ny, double *F) {
+  gas_t g = mkgas(gas_R, gas_cp);
+  double aL = f_ss(&g, L[3], L[0]), aR = f_ss(&g, Rr[3], Rr[0]);
+  riemann(id, &g, L[0], L[1], L[2], L[3], aL, L[0], Rr[0], Rr[1], Rr[2], Rr[3], aR, Rr[0], nx, ny, F);
+}
+
+void q2do_roe_averages(double gas_R, double gas_cp, const double *L, const double *Rr, double *out) {
+  gas_t g = mkgas(gas_R, gas_cp);
+  roe_t r = roe_averages(&g, L[0], Rr[0], L[0], Rr[0], L[1], L[2], Rr[1], Rr[2], L[3], Rr[3]);
+  out[0] = r.h0l; out[1] = r.h0r; out[2] = r.rho; out[3] = r.u; out[4] = r.v; out[5] = r.a; out[6] = r.h0;
+}
+
+int q2do_convective_flux(int lim, int rid, double gas_R, double gas_cp, const double *dl,
+                         double nx, double ny, double int_l, const double *prim, double *flux) {
+  gas_t g = mkgas(gas_R, gas_cp);
+  return convective_flux(lim, rid, &g, dl, nx, ny, int_l, (const double(*)[NP])prim, flux);
+}

@@ -1,0 +1,62 @@
+/*
+ * q2d_oracle.h — CPU restatement of Q2D's explicit finite-volume step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is the checker the HIP path is compared
+ * against; nothing in the product links, loads or calls it.  It is used by
+ * tests/, by __graft_entry__.smoke() and by bench.py's CPU-baseline leg.
+ *
+ * The restatement follows the reference Fortran in operation order (every
+ * function cites the file:line it restates), in FP64 with floating-point
+ * contraction disabled, so that a bit-faithful GPU implementation can be
+ * compared to it exactly.
+ */
+#ifndef Q2D_ORACLE_H
+#define Q2D_ORACLE_H
+
+#include "../include/q2d_gpu.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct q2do q2do;
+
+/* Build the oracle state from the same problem description the C-ABI takes. */
+q2do *q2do_create(const q2d_problem *prob);
+void  q2do_destroy(q2do *o);
+
+/* One Explicit_Step for every case (Mod_Explicit.f90:10-174). Returns Q2D_OK or
+ * the first failure code; out has ncases entries (may be NULL). */
+int q2do_step(q2do *o, q2d_step_out *out);
+
+/* State of block `block` (0-based global) in the reference layout. */
+int q2do_get_state(q2do *o, int block, double *P);
+int q2do_set_state(q2do *o, int block, const double *P);
+int q2do_get_dt(q2do *o, int block, double *dtlocal);
+
+/* Number of OpenMP threads the oracle uses (1 when built without OpenMP). */
+int q2do_set_threads(int n);
+int q2do_get_threads(void);
+
+/* ---- unit-level entry points (tests pin these against closed forms) ---- */
+double q2do_limiter(int id, double x, double y);
+/* Reconstruction (Lib_Reconstruction.f90:10-29) of one scalar. */
+void q2do_reconstruct(int limiter, double prev, double local, double next, double next2,
+                      double dl0, double dl1, double dl2, double dll, double dlr, double beta,
+                      double *priml, double *primr);
+/* Riemann flux (Lib_Riemann_HLL.f90) for primitive L/R states; F[4] = F_r,F_u,F_v,F_E. */
+void q2do_riemann(int id, double gas_R, double gas_cp,
+                  const double *primL, const double *primR, double nx, double ny, double *F);
+/* Roe averages (Lib_Riemann_HLL.f90:229-317): out = h0l,h0r,rho,u,v,a,h0. */
+void q2do_roe_averages(double gas_R, double gas_cp, const double *primL, const double *primR, double *out);
+/* Convective_Flux (Lib_Convective.f90:15-82) for one face: stencil prim[4][4]
+ * (cells -1..2), dl[4]; returns the flux added to cell 0 (and subtracted from cell 1). */
+int q2do_convective_flux(int limiter, int riemann, double gas_R, double gas_cp,
+                         const double *dl, double nx, double ny, double int_l,
+                         const double *prim, double *flux);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

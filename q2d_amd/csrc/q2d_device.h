@@ -1,0 +1,197 @@
+// q2d_device.h — per-face and per-cell arithmetic of the explicit step, for CDNA4.
+//
+// Every function keeps the evaluation order of the Fortran it restates
+// (file:line in each comment), so that with T = double and floating-point
+// contraction disabled the results are bit-identical to a literal FP64
+// reading of the reference.  With T = float the same code is the FP32 face
+// computation of the mixed-precision form, where division and square root use
+// the hardware reciprocal / square-root instructions.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace q2d {
+
+template <typename T> struct Gas { T R, cp, cv, gam; };
+
+// ---- arithmetic primitives ----------------------------------------------
+template <typename T> __device__ __forceinline__ T qdiv(T a, T b);
+template <> __device__ __forceinline__ double qdiv<double>(double a, double b) { return a / b; }
+template <> __device__ __forceinline__ float qdiv<float>(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+
+template <typename T> __device__ __forceinline__ T qsqrt(T a);
+template <> __device__ __forceinline__ double qsqrt<double>(double a) { return __builtin_sqrt(a); }
+template <> __device__ __forceinline__ float qsqrt<float>(float a) { return __builtin_amdgcn_sqrtf(a); }
+
+template <typename T> __device__ __forceinline__ T qabs(T a) { return __builtin_fabs(a); }
+template <> __device__ __forceinline__ float qabs<float>(float a) { return __builtin_fabsf(a); }
+template <typename T> __device__ __forceinline__ T qcopysign(T a, T b) { return __builtin_copysign(a, b); }
+template <> __device__ __forceinline__ float qcopysign<float>(float a, float b) { return __builtin_copysignf(a, b); }
+
+// Fortran MIN/MAX of three reals, left to right.
+template <typename T> __device__ __forceinline__ T min3(T a, T b, T c) { T m = a; if (b < m) m = b; if (c < m) m = c; return m; }
+template <typename T> __device__ __forceinline__ T max3(T a, T b, T c) { T m = a; if (b > m) m = b; if (c > m) m = c; return m; }
+
+// ---- gas closure (single species, constant cp) ---------------------------
+// a = sqrt(gamma*p/rho)  (f_ss)
+template <typename T> __device__ __forceinline__ T sound(const Gas<T>& g, T p, T rho) { return qsqrt<T>(qdiv<T>(g.gam * p, rho)); }
+
+// ---- limiters — Lib_Limiters.f90:26-112 ----------------------------------
+template <typename T> __device__ __forceinline__ T limiter(int id, T x, T y) {
+  const T tiny = T(1e-30);
+  switch (id) {
+    case 1: {                                   // minmod :36-46
+      T r = T(0);
+      if (x * y < T(0)) return r;
+      r = x;
+      if (qabs<T>(x) > qabs<T>(y)) r = y;
+      return r;
+    }
+    case 2: {                                   // vanleer :49-58
+      T ru = qdiv<T>(y, x + tiny);
+      T phi = qdiv<T>(ru + qabs<T>(ru), T(1) + qabs<T>(ru));
+      return phi * x;
+    }
+    case 3: {                                   // vanalbada :61-70
+      T ru = qdiv<T>(y, x + tiny);
+      T phi = qdiv<T>(ru * ru + ru, T(1) + ru * ru);
+      return phi * x;
+    }
+    case 4: {                                   // MC :73-82
+      T ru = qdiv<T>(y, x + tiny);
+      T m = min3<T>(T(2) * ru, T(0.5) * (T(1) + ru), T(2));
+      T phi = T(0); if (m > phi) phi = m;
+      return phi * x;
+    }
+    case 5: {                                   // superbee :85-94
+      T ru = qdiv<T>(y, x + tiny);
+      T a = T(2) * ru; if (T(1) < a) a = T(1);
+      T b = ru; if (T(2) < b) b = T(2);
+      return max3<T>(T(0), a, b) * x;
+    }
+    case 6: {                                   // LIMO3 :98-110
+      T ru = qdiv<T>(y, x + tiny);
+      T t = qdiv<T>(T(2) + ru, T(3));
+      T inner = min3<T>(T(2) * ru, t, T(1.6));
+      T mid = max3<T>(T(-0.5) * ru, T(0), inner);
+      T outer = t; if (mid < outer) outer = mid;
+      T phi = T(0); if (outer > phi) phi = outer;
+      return phi * x;
+    }
+    default: return T(0);                       // IORD :141-148
+  }
+}
+
+// ---- Riemann solvers — Lib_Riemann_HLL.f90 --------------------------------
+// roe_averages :229-317, one species, h = cp*T, cp(T) = cp.
+template <typename T>
+__device__ __forceinline__ void roe_averages(const Gas<T>& g, T dltot, T drtot, T ul, T vl, T ur, T vr, T pl, T pr,
+                                             T& h0l_o, T& h0r_o, T& u_roe, T& v_roe, T& a_roe) {
+  const T dl = dltot, dr = drtot;
+  T Rl = g.R, Rr = g.R;
+  T inv_dltot = qdiv<T>(T(1), dltot);
+  T inv_drtot = qdiv<T>(T(1), drtot);
+  T Tl = qdiv<T>(pl * inv_dltot, Rl);
+  T Tr = qdiv<T>(pr * inv_drtot, Rr);
+  T srL = qsqrt<T>(dltot);
+  T srR = qsqrt<T>(drtot);
+  T inv_sr = qdiv<T>(T(1), srL + srR);
+  u_roe = (srR * ur + srL * ul) * inv_sr;
+  v_roe = (srR * vr + srL * vl) * inv_sr;
+  T R_roe = (srR * Rr + srL * Rl) * inv_sr;
+  T T_roe = (srR * Tr + srL * Tl) * inv_sr;
+  T invW = g.R;
+  T hl = g.cp * Tl;
+  T hr = g.cp * Tr;
+  T el = hl - invW * Tl;
+  T er = hr - invW * Tr;
+  T h0l = T(0) + hl * dl * inv_dltot;
+  T h0r = T(0) + hr * dr * inv_drtot;
+  T d_roe = (srR * dr * inv_drtot + srL * dl * inv_dltot) * inv_sr;
+  T cv_roe = T(0) + d_roe * (T(0.5) * (g.cp + g.cp) - invW);
+  T sum_ei = T(0) + d_roe * (srR * er + srL * el) * inv_sr;
+  h0l = h0l + T(0.5) * (ul * ul + vl * vl);
+  h0r = h0r + T(0.5) * (ur * ur + vr * vr);
+  T h0_roe = (srR * h0r + srL * h0l) * inv_sr;
+  T gam_roe = T(1) + qdiv<T>(R_roe, cv_roe);
+  T vel2 = u_roe * u_roe + v_roe * v_roe;
+  a_roe = qsqrt<T>((gam_roe - T(1)) * (h0_roe - T(0.5) * vel2 + cv_roe * T_roe - sum_ei));
+  h0l_o = h0l; h0r_o = h0r;
+}
+
+// riemann_HLLC :34-123 and riemann_HLLE :126-169.  F = (F_r, F_u, F_v, F_E).
+template <typename T, int RIE>
+__device__ __forceinline__ void riemann(const Gas<T>& g, T ul, T vl, T pl, T al, T dltot,
+                                        T ur, T vr, T pr, T ar, T drtot, T nx, T ny, T F[4]) {
+  T unl = ul * nx + vl * ny;
+  T unr = ur * nx + vr * ny;
+  T h0l, h0r, u_roe, v_roe, a_roe;
+  roe_averages<T>(g, dltot, drtot, ul, vl, ur, vr, pl, pr, h0l, h0r, u_roe, v_roe, a_roe);
+  T un_roe = u_roe * nx + v_roe * ny;
+  if (RIE == 1) {                               // HLLE
+    T S1 = min3<T>(T(0), unl - al, un_roe - a_roe);
+    T S4 = max3<T>(T(0), unr + ar, un_roe + a_roe);
+    T inv_denom = qdiv<T>(T(1), S4 - S1);
+    T FrL = dltot * unl;
+    T FrR = drtot * unr;
+    T S1S4 = S1 * S4;
+    F[0] = (S4 * FrL - S1 * FrR + S1S4 * (drtot - dltot)) * inv_denom;
+    F[1] = (S4 * (FrL * ul + pl * nx) - S1 * (FrR * ur + pr * nx) + S1S4 * (drtot * ur - dltot * ul)) * inv_denom;
+    F[2] = (S4 * (FrL * vl + pl * ny) - S1 * (FrR * vr + pr * ny) + S1S4 * (drtot * vr - dltot * vl)) * inv_denom;
+    F[3] = (S4 * (FrL * h0l) - S1 * (FrR * h0r)
+            + S1S4 * (drtot * (h0r - qdiv<T>(pr, drtot)) - dltot * (h0l - qdiv<T>(pl, dltot)))) * inv_denom;
+    return;
+  }
+  T e0l = h0l - qdiv<T>(pl, dltot);
+  T e0r = h0r - qdiv<T>(pr, drtot);
+  T S1 = min3<T>(T(0), unl - al, un_roe - a_roe);
+  T S4 = max3<T>(T(0), unr + ar, un_roe + a_roe);
+  T inv_denom = qdiv<T>(T(1), dltot * (S1 - unl) - drtot * (S4 - unr));
+  T Sstar = (pr - pl + dltot * unl * (S1 - unl) - drtot * unr * (S4 - unr)) * inv_denom;
+  T pstar = dltot * (unl - S1) * (unl - Sstar) + pl;
+  // The two star branches share one code path: side s = L when Sstar >= 0, else R.
+  bool left = (S1 >= T(0)) || (Sstar >= T(0));
+  bool star = (S1 < T(0)) && ((Sstar >= T(0)) || (S4 > T(0)));
+  T Sk = left ? S1 : S4;
+  T dk = left ? dltot : drtot;
+  T uk = left ? ul : ur;
+  T vk = left ? vl : vr;
+  T pk = left ? pl : pr;
+  T unk = left ? unl : unr;
+  T h0k = left ? h0l : h0r;
+  T e0k = left ? e0l : e0r;
+  T F_r = dk * unk;
+  T F_u = F_r * uk + pk * nx;
+  T F_v = F_r * vk + pk * ny;
+  T F_E = F_r * h0k;
+  if (star) {
+    T inv_d = qdiv<T>(T(1), Sk - Sstar);
+    T U1S = dk * (Sk - unk) * inv_d;
+    T U2S = ((Sk - unk) * dk * uk + (pstar - pk) * nx) * inv_d;
+    T U3S = ((Sk - unk) * dk * vk + (pstar - pk) * ny) * inv_d;
+    T U5S = ((Sk - unk) * dk * e0k - pk * unk + pstar * Sstar) * inv_d;
+    F_r = F_r + Sk * (U1S - dk);
+    F_u = F_u + Sk * (U2S - dk * uk);
+    F_v = F_v + Sk * (U3S - dk * vk);
+    F_E = F_E + Sk * (U5S - dk * e0k);
+  }
+  F[0] = F_r; F[1] = F_u; F[2] = F_v; F[3] = F_E;
+}
+
+// ---- conservative <-> primitive (FP64 always) ------------------------------
+__device__ __forceinline__ void prim2cons(const Gas<double>& g, double rho, double u, double v, double p, double U[4]) {
+  double T = p / (rho * g.R);
+  U[0] = rho;
+  U[1] = rho * u;
+  U[2] = rho * v;
+  U[3] = rho * (g.cv * T + 0.5 * (u * u + v * v));
+}
+__device__ __forceinline__ void cons2prim(const Gas<double>& g, const double U[4], double q[4]) {
+  double rho = U[0];
+  double vx = U[1] / rho, vy = U[2] / rho;
+  double e = U[3] / rho - 0.5 * (vx * vx + vy * vy);
+  double T = e / g.cv;
+  q[0] = rho; q[1] = vx; q[2] = vy; q[3] = rho * g.R * T;
+}
+
+}  // namespace q2d

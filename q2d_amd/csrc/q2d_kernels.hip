@@ -1,0 +1,1331 @@
+// q2d_kernels.hip — MI355X (gfx950) kernels and C-ABI of the explicit FV step.
+//
+// Device layout (per block): every per-cell or per-face field is a plane of
+// (nj+4) rows x pitch doubles, i fastest; cell (i,j) of the Fortran arrays
+// (i,j in 1-gc..n+gc) lives at (j+1)*pitch + (i+1).  The i-face between cells
+// i and i+1 (dir(1)%f(i,j)) is stored at the position of cell i, the j-face
+// between rows j and j+1 (dir(2)%f(i,j)) at the position of cell (i,j).
+// State is SoA (rho,u,v,p planes) in three rotating buffers so that no RK
+// stage reads what it writes; Copy_State disappears.
+//
+// Work decomposition of the stage kernel: one wavefront owns a strip of 60
+// interior columns (lanes 2..61; lanes 0,1,62,63 are the i-halo) over a
+// segment of rows, and marches up the rows.  The j-direction stencil is kept
+// in registers (each row is loaded once), the i-direction stencil is shared
+// across lanes with wavefront shuffles.  Each face flux is evaluated once, and
+// each cell's limited slope is evaluated once per direction and shared by the
+// two faces that use it (the reference evaluates it twice with identical
+// operands, so the value is bit-identical).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/q2d_gpu.h"
+#include "q2d_device.h"
+
+using namespace q2d;
+
+namespace {
+
+constexpr int kWave = 64;
+constexpr int kOwn = 60;          // owned columns per strip
+constexpr int kMaxCases = 1 << 20;
+
+// ---------------------------------------------------------------------------
+// device-side descriptors
+// ---------------------------------------------------------------------------
+struct DevCase {
+  int n_rk, limiter, riemann, time_accurate;
+  int model, source, precision, ramp_iter;
+  int ndir, itermax, item_begin, item_end;
+  double cfl, vnn, R, cp, cv, gam;
+  double res_threshold, time_threshold;
+};
+
+struct DevCtrl {
+  unsigned long long dtmin_bits;
+  unsigned long long fail_key;
+  double dtglobal, time;
+  double residual[4];
+  int iter, iter_from_call, todo, active;
+  int cur, status, stage_count, pad;
+};
+
+struct DevBlock {
+  int ni, nj, pitch, case_id;
+  int blk_in_case, pad0;
+  int bcf[4];                    // offset of the per-cell BC-record index list of faces 1..4 (-1: none)
+  long long plane;               // doubles per plane
+  double* P[3];                  // three state buffers, 4 planes each
+  const double *dlx, *dly, *sur;
+  const double *fix, *fiy, *fil; // i-face N(1), N(2), int_l
+  const double *fjx, *fjy, *fjl; // j-face
+  const double *m11, *m21, *m12, *m22;
+  const double *sa, *sb;
+  double* dtl;
+};
+
+struct DevBC {
+  int type, blk, i, j;           // blk: global block index
+  int f, dblk, is, js;           // donor (101/201)
+  int fs, pad0, pad1, pad2;
+  double mach, T, p, rel_fac, cos_a, sin_a;
+};
+
+struct Item { int blk, x0, y0, y1; };
+
+__device__ __forceinline__ int stage_in(int cur, int irk) { return irk == 1 ? cur : (cur + 1 + ((irk - 2) & 1)) % 3; }
+__device__ __forceinline__ int stage_out(int cur, int irk) { return (cur + 1 + ((irk - 1) & 1)) % 3; }
+
+__device__ __forceinline__ double shdn(double v) { return __shfl_down(v, 1, kWave); }
+__device__ __forceinline__ double shup(double v) { return __shfl_up(v, 1, kWave); }
+__device__ __forceinline__ float shdn(float v) { return __shfl_down(v, 1, kWave); }
+__device__ __forceinline__ float shup(float v) { return __shfl_up(v, 1, kWave); }
+
+__device__ __forceinline__ unsigned long long fail_key(int stage, int kind, int b, int j, int i) {
+  return ((unsigned long long)stage << 42) | ((unsigned long long)kind << 41) | ((unsigned long long)(b & 0xff) << 32) |
+         ((unsigned long long)(j & 0xffff) << 16) | (unsigned long long)(i & 0xffff);
+}
+
+__device__ __forceinline__ void record_fail(DevCtrl* c, int code, unsigned long long key) {
+  atomicMin(&c->fail_key, key);
+  atomicMin(&c->status, code);   // codes are negative; min keeps a failure
+}
+
+// ---------------------------------------------------------------------------
+// step bookkeeping kernels (one thread per case)
+// ---------------------------------------------------------------------------
+__global__ void k_begin(DevCtrl* ctrl, int ncases) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncases) return;
+  DevCtrl& k = ctrl[c];
+  if (!k.active) return;
+  k.iter += 1;                                // Mod_Explicit.f90:38-40
+  k.iter_from_call += 1;
+  k.dtmin_bits = (unsigned long long)__double_as_longlong(1e5);   // :49
+}
+
+__global__ void k_dt_finish(DevCtrl* ctrl, const DevCase* cases, int ncases) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncases) return;
+  DevCtrl& k = ctrl[c];
+  if (!k.active) return;
+  k.dtglobal = __longlong_as_double((long long)k.dtmin_bits);     // Mod_dt.f90:54-55
+  if (cases[c].time_accurate) k.time = k.time + k.dtglobal;      // Mod_Explicit.f90:51-54
+}
+
+// ---------------------------------------------------------------------------
+// Compute_dt — Mod_dt.f90:10-104 (one wavefront per work item)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_dt(const DevBlock* __restrict__ blocks, const DevCase* __restrict__ cases,
+                                            DevCtrl* __restrict__ ctrl, const Item* __restrict__ items, int n_items) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wpg = blockDim.x / kWave;
+  const int wave0 = blockIdx.x * wpg + threadIdx.x / kWave;
+  const int nwaves = gridDim.x * wpg;
+  for (int it = wave0; it < n_items; it += nwaves) {
+    const Item item = items[it];
+    const DevBlock& B = blocks[item.blk];
+    DevCtrl& K = ctrl[B.case_id];
+    if (!K.active) continue;
+    const DevCase& C = cases[B.case_id];
+    const int x = item.x0 - 2 + lane;
+    const bool own = lane >= 2 && lane < 2 + kOwn && x <= B.ni;
+    const double* P = B.P[K.cur];
+    const long long pl = B.plane;
+    double dmin = 1e300;
+    if (own) {
+      for (int r = item.y0; r <= item.y1; ++r) {
+        const long long o = (long long)(r + 1) * B.pitch + (x + 1);
+        const double rho = P[o], u = P[pl + o], v = P[2 * pl + o], p = P[3 * pl + o];
+        const double snd = sqrt(C.gam * p / rho);
+        double dtmin = 1e8;
+        {
+          const double m1 = B.m11[o], m2 = B.m12[o];
+          const double nrm = sqrt(m1 * m1 + m2 * m2);
+          const double v1 = m1 / nrm, v2 = m2 / nrm;
+          const double vel_ = fabs(u * v1 + v * v2);
+          const double lambda = vel_ + snd;
+          const double dt = B.dlx[o] / lambda * C.cfl;
+          if (dt < dtmin) dtmin = dt;
+        }
+        if (C.ndir == 2) {
+          const double m1 = B.m21[o], m2 = B.m22[o];
+          const double nrm = sqrt(m1 * m1 + m2 * m2);
+          const double v1 = m1 / nrm, v2 = m2 / nrm;
+          const double vel_ = fabs(u * v1 + v * v2);
+          const double lambda = vel_ + snd;
+          const double dt = B.dly[o] / lambda * C.cfl;
+          if (dt < dtmin) dtmin = dt;
+        }
+        double dtcell = dtmin;
+        if (K.iter < C.ramp_iter) dtcell = dtcell * (double)K.iter / (double)C.ramp_iter;
+        if (!C.time_accurate) B.dtl[o] = dtcell;
+        if (dtcell < dmin) dmin = dtcell;
+      }
+    }
+    // wavefront min (order-independent), then one atomic per wavefront
+    for (int s = 32; s >= 1; s >>= 1) {
+      double o = __shfl_xor(dmin, s, kWave);
+      if (o < dmin) dmin = o;
+    }
+    if (lane == 0 && dmin < 1e300) atomicMin(&K.dtmin_bits, (unsigned long long)__double_as_longlong(dmin));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Fill_Ghost_Cell — Lib_Ghost.f90:16-140 (one thread per boundary record)
+// ---------------------------------------------------------------------------
+__constant__ int c_guide[5][2] = {{0, 0}, {1, 0}, {-1, 0}, {0, 1}, {0, -1}};   // Parameters_m.f90:13-18
+
+__global__ void k_ghost(const DevBlock* __restrict__ blocks, const DevCase* __restrict__ cases, DevCtrl* __restrict__ ctrl,
+                        const DevBC* __restrict__ bcs, int nbc, int irk) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nbc) return;
+  const DevBC e = bcs[t];
+  const DevBlock& B = blocks[e.blk];
+  const DevCtrl& K = ctrl[B.case_id];
+  if (!K.active || irk > cases[B.case_id].n_rk) return;
+  const int buf = stage_in(K.cur, irk);
+  double* P = B.P[buf];
+  const long long pl = B.plane;
+  const int pitch = B.pitch;
+  auto off = [pitch](int i, int j) { return (long long)(j + 1) * pitch + (i + 1); };
+  const int g1 = c_guide[e.f][0], g2 = c_guide[e.f][1];
+  switch (e.type) {
+    case 101:
+    case 201: {                                   // Ghost_Connection :143-160
+      const DevBlock& S = blocks[e.dblk];
+      const double* Ps = S.P[buf];
+      const int s1 = c_guide[e.fs][0], s2 = c_guide[e.fs][1];
+      for (int g = 1; g <= 2; ++g) {
+        long long od = (long long)(e.js + s2 * (g - 1) + 1) * S.pitch + (e.is + s1 * (g - 1) + 1);
+        long long om = off(e.i - g1 * g, e.j - g2 * g);
+        for (int v = 0; v < 4; ++v) P[v * pl + om] = Ps[v * S.plane + od];
+      }
+      break;
+    }
+    case 300: {                                   // Ghost_Symmetry :163-196
+      long long om = off(e.i, e.j), og = off(e.i - g1, e.j - g2);
+      double n1, n2;
+      if (e.f <= 2) { long long of = off(e.i - (e.f % 2), e.j); n1 = B.fix[of]; n2 = B.fiy[of]; }
+      else { long long of = off(e.i, e.j - (e.f % 2)); n1 = B.fjx[of]; n2 = B.fjy[of]; }
+      double u = P[pl + om], v = P[2 * pl + om];
+      double udotn = u * n1 + v * n2;
+      u = u - 2.0 * udotn * n1;
+      v = v - 2.0 * udotn * n2;
+      P[pl + og] = u; P[2 * pl + og] = v;
+      P[og] = P[om]; P[3 * pl + og] = P[3 * pl + om];
+      break;
+    }
+    case 0: case 401: case 402: case 403: case 404: case 405: case 406: case 407: case 420: {
+      long long om = off(e.i, e.j);              // Ghost_ZG_Extrapolate :199-219
+      for (int g = 1; g <= 2; ++g) {
+        long long og = off(e.i - g1 * g, e.j - g2 * g);
+        for (int v = 0; v < 4; ++v) P[v * pl + og] = P[v * pl + om];
+      }
+      break;
+    }
+    default: {                                    // Ghost_Extrapolate :270-299
+      const int di = -g1, dj = -g2;
+      for (int g = 1; g <= 2; ++g) {
+        long long o0 = off(e.i + di * g, e.j + dj * g);
+        long long o1 = off(e.i + di * (g - 1), e.j + dj * (g - 1));
+        long long o2 = off(e.i + di * (g - 2), e.j + dj * (g - 2));
+        long long o3 = off(e.i + di * (g - 3), e.j + dj * (g - 3));
+        for (int v = 0; v < 4; ++v)
+          P[v * pl + o0] = 3.0 * P[v * pl + o1] - 3.0 * P[v * pl + o2] + P[v * pl + o3];
+      }
+      break;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Boundary-face fluxes — BC_Fluxes (Mod_BC_Fluxes.f90:49-271), one thread per
+// boundary record.  The stage kernel adds the stored values to the cells'
+// residuals in record order, exactly where the reference adds them.
+// ---------------------------------------------------------------------------
+struct BCFlux { double F[4]; int mask; int pad; };   // F already carries the face sign modfm2
+
+template <typename FT>
+__device__ __forceinline__ void riemann_rt(int rid, const Gas<FT>& g, const FT L[4], const FT R[4], FT nx, FT ny, FT F[4]) {
+  const FT aL = sound<FT>(g, L[3], L[0]);
+  const FT aR = sound<FT>(g, R[3], R[0]);
+  if (rid == Q2D_RIEMANN_HLLE) riemann<FT, 1>(g, L[1], L[2], L[3], aL, L[0], R[1], R[2], R[3], aR, R[0], nx, ny, F);
+  else riemann<FT, 0>(g, L[1], L[2], L[3], aL, L[0], R[1], R[2], R[3], aR, R[0], nx, ny, F);
+}
+
+template <typename FT>
+__global__ __launch_bounds__(256) void k_bcflux(const DevBlock* __restrict__ blocks, const DevCase* __restrict__ cases,
+                                                const DevCtrl* __restrict__ ctrl, const DevBC* __restrict__ bcs, int nbc,
+                                                int irk, BCFlux* __restrict__ out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nbc) return;
+  const DevBC& E = bcs[t];
+  const DevBlock& B = blocks[E.blk];
+  const DevCtrl& K = ctrl[B.case_id];
+  const DevCase& C = cases[B.case_id];
+  if (!K.active || irk > C.n_rk) return;
+  const double* P = B.P[stage_in(K.cur, irk)];
+  const long long pl = B.plane;
+  const int pitch = B.pitch;
+  auto off = [pitch](int i, int j) { return (long long)(j + 1) * pitch + (i + 1); };
+  const int Im = E.i, Jm = E.j, Fm = E.f;
+  const int modfm = Fm % 2, modfm1 = 1 - modfm, modfm2 = 1 - 2 * modfm;
+  const double sgn = (double)modfm2;
+  const int g1 = c_guide[Fm][0], g2 = c_guide[Fm][1];
+  const int lim = C.limiter, rid = C.riemann;
+  const Gas<FT> gf{(FT)C.R, (FT)C.cp, (FT)C.cv, (FT)C.gam};
+  const Gas<double> gd{C.R, C.cp, C.cv, C.gam};
+  // Face_Index — Lib_BC_Fluxes.f90:34-50
+  const bool idir = Fm <= 2;
+  const long long ofc = idir ? off(Im - modfm, Jm) : off(Im, Jm - modfm);
+  const double n1 = idir ? B.fix[ofc] : B.fjx[ofc];
+  const double n2 = idir ? B.fiy[ofc] : B.fjy[ofc];
+  const double il = idir ? B.fil[ofc] : B.fjl[ofc];
+  const double* dlp = idir ? B.dlx : B.dly;
+  const long long oB = off(Im, Jm);
+  double Bp[4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) Bp[v] = P[v * pl + oB];
+  BCFlux res;
+  res.mask = 0xF; res.pad = 0;
+  int ty = E.type;
+
+  if (ty == 101 || ty == 201) {
+    // BC_Connection_Eul — Lib_BC_Fluxes_Connection.f90:24-133
+    const int I1 = modfm * (Im - g1 * 2) + modfm1 * (Im + g1), J1 = modfm * (Jm - g2 * 2) + modfm1 * (Jm + g2);
+    const int I2 = modfm1 * Im + modfm * (Im - g1), J2 = modfm1 * Jm + modfm * (Jm - g2);
+    const int I3 = modfm1 * (Im - g1) + modfm * Im, J3 = modfm1 * (Jm - g2) + modfm * Jm;
+    const int I4 = modfm1 * (Im - g1 * 2) + modfm * (Im + g1), J4 = modfm1 * (Jm - g2 * 2) + modfm * (Jm + g2);
+    const long long o1 = off(I1, J1), o2 = off(I2, J2), o3 = off(I3, J3), o4 = off(I4, J4);
+    const double Dl1 = dlp[o1] * 0.5, Dl2 = dlp[o2] * 0.5, Dl3 = dlp[o3] * 0.5, Dl4 = dlp[o4] * 0.5;
+    const FT Dist23 = (FT)(Dl2 + Dl3), Dist12 = (FT)(Dl1 + Dl2), Dist34 = (FT)(Dl3 + Dl4);
+    FT L[4], R[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const double p1 = P[v * pl + o1], p2 = P[v * pl + o2], p3 = P[v * pl + o3], p4 = P[v * pl + o4];
+      const FT s0 = qdiv<FT>((FT)(p2 - p1), Dist12);
+      const FT s1 = qdiv<FT>((FT)(p3 - p2), Dist23);
+      const FT s2 = qdiv<FT>((FT)(p4 - p3), Dist34);
+      const FT sl = limiter<FT>(lim, s1, s0);
+      const FT sr = limiter<FT>(lim, s2, s1);
+      L[v] = (FT)p2 + sl * (FT)Dl2;
+      R[v] = (FT)p3 - sr * (FT)Dl3;
+    }
+    const bool bad = !(L[0] > FT(0) && L[3] > FT(0) && R[0] > FT(0) && R[3] > FT(0));
+    if (bad) record_fail(const_cast<DevCtrl*>(&K), Q2D_E_INTERFACE, fail_key(K.stage_count + irk, 0, B.blk_in_case, Jm, Im));
+    FT F[4];
+    riemann_rt<FT>(rid, gf, L, R, (FT)n1, (FT)n2, F);
+    // one species: Sel_L*rho_L/rho_L - Sel_R*rho_R/rho_R == 1, so Flux(1) = F_r*int_l
+#pragma unroll
+    for (int v = 0; v < 4; ++v) res.F[v] = ((double)F[v] * il) * sgn;
+    out[t] = res;
+    return;
+  }
+  const int Ii = Im + g1, Ij = Jm + g2;                 // interior neighbour
+  const long long oI = off(Ii, Ij);
+  double Ip[4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) Ip[v] = P[v * pl + oI];
+
+  if (ty == 400) {
+    // BC_Extrapolation — Lib_BC_Fluxes_Extrapolation.f90:18-49
+    const double rho = Bp[0], u = Bp[1], v = Bp[2], p = Bp[3];
+    const double Un = u * n1 + v * n2;
+    double Fb[4];
+    Fb[0] = rho * Un * il;
+    const double Fmass = Fb[0];
+    Fb[1] = Fmass * u + p * il * n1;
+    Fb[2] = Fmass * v + p * il * n2;
+    const double vmag = sqrt(u * u + v * v);
+    const double Tt = p / (rho * gd.R);
+    Fb[3] = Fmass * (gd.cp * Tt + 0.5 * vmag * vmag);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) res.F[k] = sgn * Fb[k];
+    out[t] = res;
+    return;
+  }
+  if (ty == 406) {
+    // BC_Outflow + Outflow — Lib_BC_Fluxes_Outflow.f90:36-165
+    const double Un = Bp[1] * n1 + Bp[2] * n2;
+    if (sgn * Un < 0.0) {
+      ty = 300;                                          // error=1 -> BC_Symmetry_Eul (Mod_BC_Fluxes.f90:227)
+    } else {
+      const double Bound_Sound = sqrt(gd.gam * Bp[3] / Bp[0]);
+      const double G = gd.gam;
+      const double Riem = Bound_Sound + sgn * 0.5 * (G - 1.0) * Un;
+      const double VT1 = Bp[1] - Un * n1, VT2 = Bp[2] - Un * n2;
+      const double Mach_N = fabs(Un / Bound_Sound);
+      double Fp[4], Face_Un, Face_rho;
+      if (Mach_N < 1.0 && E.p > 0.0) {
+        double p_exit = E.p;
+        const double delta = 0.5 * (G - 1.0);
+        const double ex = G / (G - 1.0);
+        const double den = 1.0 / pow(1.0 + delta, ex);
+        const double pstar = Bp[3] * den * pow(1.0 + delta * (Mach_N * Mach_N), ex);
+        if (pstar >= p_exit) p_exit = pstar;
+        const double Isen = (G - 1.0) / (2.0 * G);
+        Fp[3] = p_exit * E.rel_fac + Bp[3] * (1.0 - E.rel_fac);
+        const double Face_Sound = Bound_Sound * pow(Fp[3] / Bp[3], Isen);
+        Face_Un = (Riem - Face_Sound) / (sgn * 0.5 * (G - 1.0));
+        Face_rho = G * Fp[3] / (Face_Sound * Face_Sound);
+        Fp[0] = Bp[0] / Bp[0] * Face_rho;
+        Fp[1] = VT1 + Face_Un * n1;
+        Fp[2] = VT2 + Face_Un * n2;
+      } else {
+        Fp[0] = Bp[0]; Fp[1] = Bp[1]; Fp[2] = Bp[2]; Fp[3] = Bp[3];
+        Face_Un = Un; Face_rho = Bp[0];
+      }
+      const double Face_T = Fp[3] / (gd.R * Face_rho);
+      const double Hf = 0.0 + Fp[0] / Face_rho * (gd.cp * Face_T);
+      double Fb[4];
+      Fb[0] = Fp[0] * il * Face_Un;
+      const double mdot = Face_rho * il * Face_Un;
+      Fb[1] = Fp[1] * mdot + Fp[3] * il * n1;
+      Fb[2] = Fp[2] * mdot + Fp[3] * il * n2;
+      Fb[3] = mdot * (0.5 * (Fp[1] * Fp[1] + Fp[2] * Fp[2]) + Hf);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) res.F[k] = sgn * Fb[k];
+      out[t] = res;
+      return;
+    }
+  }
+  FT L[4], R[4];
+  if (ty == 405) {
+    // BC_Inlet_Supersonic_Static + Supersonic_Inflow — Lib_BC_Fluxes_Inflow.f90:272-403
+    const double Un = Bp[1] * n1 + Bp[2] * n2;
+    if (sgn * Un >= 0.0) {
+      ty = 300;                                          // error=1 -> BC_Symmetry_Eul (Mod_BC_Fluxes.f90:221)
+    } else {
+      const double dm = (double)modfm, dm1 = (double)modfm1;
+      const double flow_Sound = sqrt(gd.gam * Bp[3] / Bp[0]);
+      const double flow_Mach = sqrt(Bp[1] * Bp[1] + Bp[2] * Bp[2]) / flow_Sound;
+      const double Mach_l = E.rel_fac * E.mach + (1.0 - E.rel_fac) * flow_Mach;
+      double Sup[4];
+      Sup[3] = E.rel_fac * E.p + (1.0 - E.rel_fac) * Bp[3];
+      const double Sup_rho = Sup[3] / (1.0 * gd.R * E.T);
+      Sup[0] = Sup_rho * 1.0;
+      const double Sup_Sound = sqrt(gd.gam * Sup[3] / Sup_rho);
+      Sup[1] = Mach_l * Sup_Sound * E.cos_a;
+      Sup[2] = Mach_l * Sup_Sound * E.sin_a;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const double P1 = dm * Sup[v] + dm1 * Ip[v];
+        const double P3 = dm * Ip[v] + dm1 * Sup[v];
+        const FT d32 = (FT)(P3 - Bp[v]), d21 = (FT)(Bp[v] - P1);
+        const FT sl = limiter<FT>(lim, d32, d21);
+        const FT Fc = (FT)Bp[v] + FT(0.5) * sl * (FT)sgn;
+        L[v] = (FT)dm * (FT)Sup[v] + (FT)dm1 * Fc;
+        R[v] = (FT)dm * Fc + (FT)dm1 * (FT)Sup[v];
+      }
+    }
+  }
+  if (ty == 300 || ty == 302) {
+    // BC_Symmetry_Eul — Lib_BC_Fluxes_Symmetry.f90:28-124 (Un_wall = 0); momentum only
+    res.mask = 0x6;
+    const double Bound_Dl = 0.5 * dlp[oB], Int_Dl = 0.5 * dlp[oI];
+    double Gp[4] = {Bp[0], Bp[1], Bp[2], Bp[3]};
+    double Un = Bp[1] * n1 + Bp[2] * n2;
+    Gp[1] = Gp[1] - 2.0 * Un * n1;
+    Gp[2] = Gp[2] - 2.0 * Un * n2;
+    const double dm = (double)modfm, dm1 = (double)modfm1;
+    const FT Dist12 = (FT)(2.0 * Bound_Dl * dm + (Bound_Dl + Int_Dl) * dm1);
+    const FT Dist23 = (FT)((Bound_Dl + Int_Dl) * dm + 2.0 * Bound_Dl * dm1);
+    const FT Dist = (FT)(Bound_Dl * sgn);
+    FT Fp[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const double P1 = dm * Gp[v] + dm1 * Ip[v];
+      const double P3 = dm * Ip[v] + dm1 * Gp[v];
+      const FT s32 = qdiv<FT>((FT)(P3 - Bp[v]), Dist23);
+      const FT s21 = qdiv<FT>((FT)(Bp[v] - P1), Dist12);
+      const FT sl = limiter<FT>(lim, s32, s21);
+      Fp[v] = (FT)Bp[v] + sl * Dist;
+    }
+    const FT fn1 = (FT)n1, fn2 = (FT)n2;
+    const FT Unf = Fp[1] * fn1 + Fp[2] * fn2;
+    const FT vt1 = Fp[1] - Unf * fn1, vt2 = Fp[2] - Unf * fn2;
+    const FT Un1 = Unf * (FT)sgn;
+    L[0] = Fp[0]; L[1] = Un1 * fn1 + vt1; L[2] = Un1 * fn2 + vt2; L[3] = Fp[3];
+    R[0] = Fp[0]; R[1] = -Un1 * fn1 + vt1; R[2] = -Un1 * fn2 + vt2; R[3] = Fp[3];
+  }
+  FT F[4];
+  riemann_rt<FT>(rid, gf, L, R, (FT)n1, (FT)n2, F);
+#pragma unroll
+  for (int v = 0; v < 4; ++v) res.F[v] = sgn * ((double)F[v] * il);
+  if (res.mask == 0x6) { res.F[0] = 0.0; res.F[3] = 0.0; }
+  out[t] = res;
+}
+
+// ---------------------------------------------------------------------------
+// the fused RK-stage kernel
+// ---------------------------------------------------------------------------
+template <typename FT>
+__device__ __forceinline__ void limiter4(int lim, const FT x[4], const FT y[4], FT o[4]) {
+  switch (lim) {
+    case 1:
+#pragma unroll
+      for (int v = 0; v < 4; ++v) o[v] = limiter<FT>(1, x[v], y[v]);
+      break;
+    case 2:
+#pragma unroll
+      for (int v = 0; v < 4; ++v) o[v] = limiter<FT>(2, x[v], y[v]);
+      break;
+    case 3:
+#pragma unroll
+      for (int v = 0; v < 4; ++v) o[v] = limiter<FT>(3, x[v], y[v]);
+      break;
+    case 4:
+#pragma unroll
+      for (int v = 0; v < 4; ++v) o[v] = limiter<FT>(4, x[v], y[v]);
+      break;
+    case 5:
+#pragma unroll
+      for (int v = 0; v < 4; ++v) o[v] = limiter<FT>(5, x[v], y[v]);
+      break;
+    case 6:
+#pragma unroll
+      for (int v = 0; v < 4; ++v) o[v] = limiter<FT>(6, x[v], y[v]);
+      break;
+    default:
+#pragma unroll
+      for (int v = 0; v < 4; ++v) o[v] = FT(0);
+      break;
+  }
+}
+
+// Convective_Flux — Lib_Convective.f90:46-80 for one face whose left/right
+// cell states and limited slopes are known.  Returns true on a non-physical
+// interface state (check_gas_state).
+template <typename FT, int RIE>
+__device__ __forceinline__ bool face_flux(const Gas<FT>& g, const double qa[4], const FT pa[4], FT lm, const double qb[4],
+                                          const FT pb[4], FT lp, FT nx, FT ny, FT il, FT out[4]) {
+  FT qL[4], qR[4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    qL[v] = (FT)qa[v] + pa[v] * lm;                   // priml = local + beta*slopel*dll
+    qR[v] = (FT)qb[v] - pb[v] * lp;                   // primr = next  - beta*sloper*dlr
+  }
+  const bool bad = !(qL[0] > FT(0) && qL[3] > FT(0) && qR[0] > FT(0) && qR[3] > FT(0));
+  const FT aL = sound<FT>(g, qL[3], qL[0]);
+  const FT aR = sound<FT>(g, qR[3], qR[0]);
+  FT F[4];
+  riemann<FT, RIE>(g, qL[1], qL[2], qL[3], aL, qL[0], qR[1], qR[2], qR[3], aR, qR[0], nx, ny, F);
+  // one species: sp*(rho1/rotot1) - sm*(rho4/rotot4) == 1 exactly, so Flux(1) = F_r*int_l
+#pragma unroll
+  for (int v = 0; v < 4; ++v) out[v] = F[v] * il;
+  return bad;
+}
+
+template <typename FT, int RIE>
+__global__ __launch_bounds__(256) void k_stage(const DevBlock* __restrict__ blocks, const DevCase* __restrict__ cases,
+                                               DevCtrl* __restrict__ ctrl, const Item* __restrict__ items, int item_lo,
+                                               int item_hi, const int* __restrict__ bcface,
+                                               const BCFlux* __restrict__ bcflux, double* __restrict__ partials, int irk,
+                                               int xcd_remap) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wpg = blockDim.x / kWave;
+  int grp = blockIdx.x;
+  if (xcd_remap) {                                     // consecutive logical groups on one XCD (8 XCDs, round-robin dispatch)
+    const int ng = gridDim.x;
+    const int per = ng / 8;
+    if (per > 0 && grp < per * 8) grp = (grp % 8) * per + grp / 8;
+  }
+  const int wave0 = grp * wpg + threadIdx.x / kWave;
+  const int nwaves = gridDim.x * wpg;
+
+  for (int it = item_lo + wave0; it < item_hi; it += nwaves) {
+    const Item item = items[it];
+    const DevBlock& B = blocks[item.blk];
+    DevCtrl& K = ctrl[B.case_id];
+    const DevCase& C = cases[B.case_id];
+    if (!K.active || irk > C.n_rk) continue;
+    const bool last = (irk == C.n_rk);
+    const int ni = B.ni, nj = B.nj, pitch = B.pitch;
+    const long long pl = B.plane;
+    const int bin = stage_in(K.cur, irk), bout = stage_out(K.cur, irk);
+    const double* __restrict__ Pin = B.P[bin];
+    const double* __restrict__ PO = B.P[K.cur];
+    double* __restrict__ Pout = B.P[bout];
+    const int lim = C.limiter;
+    const Gas<FT> gf{(FT)C.R, (FT)C.cp, (FT)C.cv, (FT)C.gam};
+    const double c_rk = (C.n_rk == 1) ? 1.0 : (C.n_rk == 2 ? (irk == 1 ? 1.0 : 0.5)
+                                                            : (irk == 1 ? 1.0 : (irk == 2 ? 0.25 : 2.0 / 3.0)));
+    const int stage_id = K.stage_count + irk;
+
+    const int x = item.x0 - 2 + lane;                  // Fortran column of this lane
+    const int xc = x <= ni + 2 ? x : ni + 2;           // clamped column for loads
+    const bool own = lane >= 2 && lane < 2 + kOwn && x <= ni;
+    const bool xdo = lane >= 1 && lane <= 61 && x >= 1 && x <= ni - 1;   // interior face (x|x+1)
+    const int rec3 = (own && B.bcf[2] >= 0) ? bcface[B.bcf[2] + x - 1] : -1;
+    const int rec4 = (own && B.bcf[3] >= 0) ? bcface[B.bcf[3] + x - 1] : -1;
+
+    auto at = [&](int r) { return (long long)(r + 1) * pitch + (xc + 1); };
+
+    // ---- pipeline warm-up: rows y0-2, y0-1, y0 ----------------------------
+    double qA[4], qB[4], qC[4];
+    double dlyA, dlyB, dlyC;
+    FT sAB[4], phiA[4], Gprev[4];
+    {
+      double qm[4];
+      const long long o0 = at(item.y0 - 2), o1 = at(item.y0 - 1), o2 = at(item.y0);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) { qm[v] = Pin[v * pl + o0]; qA[v] = Pin[v * pl + o1]; qB[v] = Pin[v * pl + o2]; }
+      const double dlm = B.dly[o0];
+      dlyA = B.dly[o1]; dlyB = B.dly[o2];
+      const FT l0 = (FT)(0.5 * (dlm + dlyA)), l1 = (FT)(0.5 * (dlyA + dlyB));
+      FT s0[4];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        s0[v] = qdiv<FT>((FT)(qA[v] - qm[v]), l0);
+        sAB[v] = qdiv<FT>((FT)(qB[v] - qA[v]), l1);
+        Gprev[v] = FT(0);
+      }
+      limiter4<FT>(lim, sAB, s0, phiA);
+    }
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+
+    for (int r = item.y0 - 1; r <= item.y1; ++r) {
+      // row r+2 of the j-stencil
+      const long long oC = at(r + 2);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) qC[v] = Pin[v * pl + oC];
+      dlyC = B.dly[oC];
+      FT sBC[4], phiB[4], Gcur[4];
+      {
+        const FT lBC = (FT)(0.5 * (dlyB + dlyC));
+#pragma unroll
+        for (int v = 0; v < 4; ++v) sBC[v] = qdiv<FT>((FT)(qC[v] - qB[v]), lBC);
+        limiter4<FT>(lim, sBC, sAB, phiB);            // phi_{r+1} = rlimiter(s(r+1|r+2), s(r|r+1))
+      }
+      // interior j-face (r | r+1) — Mod_Fluxes.f90:187-232
+      if (own && r >= 1 && r <= nj - 1) {
+        const long long oA = at(r);
+        if (face_flux<FT, RIE>(gf, qA, phiA, (FT)(0.5 * dlyA), qB, phiB, (FT)(0.5 * dlyB), (FT)B.fjx[oA],
+                               (FT)B.fjy[oA], (FT)B.fjl[oA], Gcur))
+          record_fail(&K, Q2D_E_INTERFACE, fail_key(stage_id, 0, B.blk_in_case, r, x));
+      } else {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) Gcur[v] = FT(0);
+      }
+
+      if (r >= item.y0) {
+        // ---- i-direction on row r (lanes share the stencil) ---------------
+        const long long oA = at(r);
+        const double dlx = B.dlx[oA];
+        double qn[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) qn[v] = shdn(qA[v]);
+        const double dlxn = shdn(dlx);
+        FT sx[4], sxp[4], phx[4], phxn[4];
+        {
+          const FT l1 = (FT)(0.5 * (dlx + dlxn));
+#pragma unroll
+          for (int v = 0; v < 4; ++v) sx[v] = qdiv<FT>((FT)(qn[v] - qA[v]), l1);
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) sxp[v] = shup(sx[v]);
+        limiter4<FT>(lim, sx, sxp, phx);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) phxn[v] = shdn(phx[v]);
+        FT Fx[4];
+        if (xdo) {
+          if (face_flux<FT, RIE>(gf, qA, phx, (FT)(0.5 * dlx), qn, phxn, (FT)(0.5 * dlxn), (FT)B.fix[oA],
+                                 (FT)B.fiy[oA], (FT)B.fil[oA], Fx))
+            record_fail(&K, Q2D_E_INTERFACE, fail_key(stage_id, 0, B.blk_in_case, r, x));
+        } else {
+#pragma unroll
+          for (int v = 0; v < 4; ++v) Fx[v] = FT(0);
+        }
+        FT Fl[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) Fl[v] = shup(Fx[v]);
+
+        if (own) {
+          // ---- residual in the reference's accumulation order ------------
+          double res[4];
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            double rv = 0.0;
+            if (x >= 2) rv = 0.0 - (double)Fl[v];           // face x-1 (Mod_Fluxes.f90:140-185)
+            if (x <= ni - 1) rv = rv + (double)Fx[v];       // face x
+            if (r >= 2) rv = rv - (double)Gprev[v];         // face r-1 (:187-232)
+            if (r <= nj - 1) rv = rv + (double)Gcur[v];     // face r
+            res[v] = rv;
+          }
+          // ---- BC faces 1..4 in record order (Mod_BC_Fluxes.f90:79-86) ----
+          if (x == 1 || x == ni || r == 1 || r == nj) {
+#pragma unroll 1
+            for (int f = 0; f < 4; ++f) {
+              int rec = -1;
+              if (f == 0 && x == 1 && B.bcf[0] >= 0) rec = bcface[B.bcf[0] + r - 1];
+              if (f == 1 && x == ni && B.bcf[1] >= 0) rec = bcface[B.bcf[1] + r - 1];
+              if (f == 2 && r == 1) rec = rec3;
+              if (f == 3 && r == nj) rec = rec4;
+              if (rec < 0) continue;
+              const BCFlux& q = bcflux[rec];
+#pragma unroll
+              for (int v = 0; v < 4; ++v)
+                if ((q.mask >> v) & 1) res[v] = res[v] + q.F[v];
+            }
+          }
+          // ---- geometric sources — Lib_Source_Axisymmetry.f90:40-78, Lib_Source_Areavariation.f90:104-144
+          const double sr = B.sur[oA];
+          const Gas<double> gd{C.R, C.cp, C.cv, C.gam};
+          if (C.source != Q2D_SRC_NONE) {
+            const double rho = qA[0], u = qA[1], v = qA[2], p = qA[3];
+            const double mod_v = sqrt(u * u + v * v);
+            const double E0 = gd.cv * (p / (rho * gd.R)) + 0.5 * mod_v * mod_v;
+            double S[4];
+            if (C.source == Q2D_SRC_AXI) {
+              const double invr = B.sb[oA];
+              S[0] = -(rho * v * invr);
+              S[1] = -(rho * u * v * invr);
+              S[2] = -(rho * v * v * invr);
+              S[3] = -((rho * E0 + p) * v * invr);
+            } else {
+              const double VdG = u * B.sa[oA] + v * B.sb[oA];
+              S[0] = rho * VdG;
+              S[1] = rho * u * VdG;
+              S[2] = rho * v * VdG;
+              S[3] = (rho * E0 + p) * VdG;
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) res[k] = res[k] - S[k] * sr;
+          }
+          // ---- Compute_Cons_NoIRS — Lib_Newstate.f90:131-159 -------------
+          const double dt = C.time_accurate ? K.dtglobal : B.dtl[oA];
+          double po[4];
+#pragma unroll
+          for (int v = 0; v < 4; ++v) po[v] = PO[v * pl + oA];
+          double Uo[4], U[4], qn2[4];
+          prim2cons(gd, po[0], po[1], po[2], po[3], Uo);
+          prim2cons(gd, qA[0], qA[1], qA[2], qA[3], U);
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const double rr = -res[v] / sr * dt;
+            U[v] = Uo[v] + (U[v] - Uo[v] + rr) * c_rk;     // RK_stage — Lib_RK.f90:15-27
+          }
+          cons2prim(gd, U, qn2);
+          // Check_And_Fix_State — Lib_Newstate.f90:316-350
+          if (__builtin_isnan(qn2[0]) || __builtin_isnan(qn2[1]) || __builtin_isnan(qn2[2]) ||
+              __builtin_isnan(qn2[3]) || qn2[3] < 0.0)
+            record_fail(&K, Q2D_E_STATE, fail_key(stage_id, 1, B.blk_in_case, r, x));
+          if (qn2[0] < 1e-20) qn2[0] = 1e-20;
+#pragma unroll
+          for (int v = 0; v < 4; ++v) Pout[v * pl + oA] = qn2[v];
+          if (last) {                                        // Compute_Residual — Mod_Diagnostic.f90:452-458
+#pragma unroll
+            for (int v = 0; v < 4; ++v) { const double d = fabs(qn2[v] - po[v]); acc[v] = acc[v] + d * d; }
+          }
+        }
+      }
+      // ---- roll the j-stencil -------------------------------------------
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        qA[v] = qB[v]; qB[v] = qC[v];
+        sAB[v] = sBC[v]; phiA[v] = phiB[v]; Gprev[v] = Gcur[v];
+      }
+      dlyA = dlyB; dlyB = dlyC;
+    }
+    if (last) {
+      // fixed-pattern wavefront sum -> one partial per work item (deterministic)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        double s = acc[v];
+        for (int k = 32; k >= 1; k >>= 1) s = s + __shfl_xor(s, k, kWave);
+        if (lane == 0) partials[(long long)it * 4 + v] = s;
+      }
+    }
+  }
+}
+
+// Per-case residual reduction and end-of-step control — Mod_Explicit.f90:120-168.
+__global__ __launch_bounds__(256) void k_finish(DevCtrl* ctrl, const DevCase* cases, const double* partials, int ncases) {
+  const int c = blockIdx.x;
+  if (c >= ncases) return;
+  DevCtrl& K = ctrl[c];
+  const DevCase& C = cases[c];
+  if (!K.active) return;
+  __shared__ double sh[4][256];
+  double s[4] = {0, 0, 0, 0};
+  for (int it = C.item_begin + threadIdx.x; it < C.item_end; it += blockDim.x)
+    for (int v = 0; v < 4; ++v) s[v] = s[v] + partials[(long long)it * 4 + v];
+  for (int v = 0; v < 4; ++v) sh[v][threadIdx.x] = s[v];
+  __syncthreads();
+  for (int w = blockDim.x / 2; w >= 1; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int v = 0; v < 4; ++v) sh[v][threadIdx.x] = sh[v][threadIdx.x] + sh[v][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    for (int v = 0; v < 4; ++v) K.residual[v] = sqrt(sh[v][0]);
+    const bool endsim = (K.iter_from_call >= C.itermax) || (K.residual[0] <= C.res_threshold) ||
+                        (K.time >= C.time_threshold);
+    K.todo = endsim ? 3 : 1;
+    K.cur = (K.cur + 1 + ((C.n_rk - 1) & 1)) % 3;          // buffer written by the last stage
+    K.stage_count += 4;
+    if (endsim || K.status != 0) K.active = 0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) { g_err = msg; return code; }
+
+#define HIPCHK(x)                                                                 \
+  do {                                                                            \
+    hipError_t e_ = (x);                                                          \
+    if (e_ != hipSuccess) return fail(Q2D_E_HIP, std::string(#x ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+struct HostBlock {
+  int ni, nj, pitch, case_id, blk_in_case;
+  long long plane;
+  double* arena = nullptr;       // one allocation for everything of the block
+  size_t arena_bytes = 0;
+};
+
+}  // namespace
+
+struct q2d_handle {
+  int nblocks = 0, ncases = 0, nbc = 0, max_rk = 1;
+  std::vector<HostBlock> hb;
+  std::vector<DevBlock> db;
+  std::vector<DevCase> dc;
+  std::vector<DevBC> dbc;
+  std::vector<int> bcface;
+  std::vector<Item> items;
+  DevBlock* d_blocks = nullptr;
+  DevCase* d_cases = nullptr;
+  DevCtrl* d_ctrl = nullptr;
+  DevBC* d_bc = nullptr;
+  int* d_bcface = nullptr;
+  Item* d_items = nullptr;
+  double* d_partials = nullptr;
+  BCFlux* d_bcflux = nullptr;
+  int n_hllc_items = 0;          // items are sorted: HLLC cases first, then HLLE cases
+  hipStream_t stream = nullptr;
+  hipGraphExec_t graph = nullptr;
+  int rows_per_item = 16, waves_per_group = 4, max_groups = 0, xcd_remap = 1;
+  int precision_mixed = 0;
+  bool timing = false;
+  long long timed_launches = 0;
+  double timed_ms = 0.0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int cu_count = 256;
+};
+
+namespace {
+
+int build_items(q2d_handle* h) {
+  h->items.clear();
+  // items grouped by Riemann solver (one kernel instantiation each), then by case
+  for (int pass = 0; pass < 2; ++pass) {
+    for (int c = 0; c < h->ncases; ++c) {
+      if (h->dc[c].riemann != pass) continue;
+      h->dc[c].item_begin = (int)h->items.size();
+      for (int b = 0; b < h->nblocks; ++b) {
+        if (h->hb[b].case_id != c) continue;
+        const int ni = h->hb[b].ni, nj = h->hb[b].nj, H = h->rows_per_item;
+        for (int y0 = 1; y0 <= nj; y0 += H)
+          for (int x0 = 1; x0 <= ni; x0 += kOwn) h->items.push_back(Item{b, x0, y0, std::min(nj, y0 + H - 1)});
+      }
+      h->dc[c].item_end = (int)h->items.size();
+    }
+    if (pass == 0) h->n_hllc_items = (int)h->items.size();
+  }
+  if (h->d_items) { (void)hipFree(h->d_items); h->d_items = nullptr; }
+  if (h->d_partials) { (void)hipFree(h->d_partials); h->d_partials = nullptr; }
+  HIPCHK(hipMalloc(&h->d_items, sizeof(Item) * std::max<size_t>(1, h->items.size())));
+  HIPCHK(hipMalloc(&h->d_partials, sizeof(double) * 4 * std::max<size_t>(1, h->items.size())));
+  HIPCHK(hipMemcpy(h->d_items, h->items.data(), sizeof(Item) * h->items.size(), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(h->d_cases, h->dc.data(), sizeof(DevCase) * h->dc.size(), hipMemcpyHostToDevice));
+  if (h->graph) { (void)hipGraphExecDestroy(h->graph); h->graph = nullptr; }
+  return Q2D_OK;
+}
+
+template <typename FT>
+void launch_stage(q2d_handle* h, int irk) {
+  const int tpb = kWave * h->waves_per_group;
+  const int ranges[2][2] = {{0, h->n_hllc_items}, {h->n_hllc_items, (int)h->items.size()}};
+  for (int rie = 0; rie < 2; ++rie) {
+    const int lo = ranges[rie][0], hi = ranges[rie][1];
+    if (hi <= lo) continue;
+    int groups = (hi - lo + h->waves_per_group - 1) / h->waves_per_group;
+    if (h->max_groups > 0) groups = std::min(groups, h->max_groups);
+    groups = std::max(groups, 1);
+    if (rie == 0)
+      hipLaunchKernelGGL((k_stage<FT, 0>), dim3(groups), dim3(tpb), 0, h->stream, h->d_blocks, h->d_cases, h->d_ctrl,
+                         h->d_items, lo, hi, h->d_bcface, h->d_bcflux, h->d_partials, irk, h->xcd_remap);
+    else
+      hipLaunchKernelGGL((k_stage<FT, 1>), dim3(groups), dim3(tpb), 0, h->stream, h->d_blocks, h->d_cases, h->d_ctrl,
+                         h->d_items, lo, hi, h->d_bcface, h->d_bcflux, h->d_partials, irk, h->xcd_remap);
+  }
+}
+
+int launch_step(q2d_handle* h) {
+  const int n_items = (int)h->items.size();
+  const int tpb = kWave * h->waves_per_group;
+  int groups = (n_items + h->waves_per_group - 1) / h->waves_per_group;
+  if (h->max_groups > 0) groups = std::min(groups, h->max_groups);
+  groups = std::max(groups, 1);
+  const int cthreads = 64, cgroups = (h->ncases + cthreads - 1) / cthreads;
+  const int bgroups = (h->nbc + 255) / 256;
+  hipLaunchKernelGGL(k_begin, dim3(cgroups), dim3(cthreads), 0, h->stream, h->d_ctrl, h->ncases);
+  hipLaunchKernelGGL(k_dt, dim3(groups), dim3(tpb), 0, h->stream, h->d_blocks, h->d_cases, h->d_ctrl, h->d_items, n_items);
+  hipLaunchKernelGGL(k_dt_finish, dim3(cgroups), dim3(cthreads), 0, h->stream, h->d_ctrl, h->d_cases, h->ncases);
+  for (int irk = 1; irk <= h->max_rk; ++irk) {
+    if (h->nbc > 0) {
+      hipLaunchKernelGGL(k_ghost, dim3(bgroups), dim3(256), 0, h->stream, h->d_blocks, h->d_cases, h->d_ctrl, h->d_bc,
+                         h->nbc, irk);
+      if (h->precision_mixed)
+        hipLaunchKernelGGL(k_bcflux<float>, dim3(bgroups), dim3(256), 0, h->stream, h->d_blocks, h->d_cases, h->d_ctrl,
+                           h->d_bc, h->nbc, irk, h->d_bcflux);
+      else
+        hipLaunchKernelGGL(k_bcflux<double>, dim3(bgroups), dim3(256), 0, h->stream, h->d_blocks, h->d_cases, h->d_ctrl,
+                           h->d_bc, h->nbc, irk, h->d_bcflux);
+    }
+    if (h->timing) HIPCHK(hipEventRecord(h->ev0, h->stream));
+    if (h->precision_mixed) launch_stage<float>(h, irk);
+    else launch_stage<double>(h, irk);
+    if (h->timing) {
+      HIPCHK(hipEventRecord(h->ev1, h->stream));
+      HIPCHK(hipEventSynchronize(h->ev1));
+      float ms = 0.f;
+      HIPCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+      h->timed_ms += ms;
+      h->timed_launches += 1;
+    }
+  }
+  hipLaunchKernelGGL(k_finish, dim3(h->ncases), dim3(256), 0, h->stream, h->d_ctrl, h->d_cases, h->d_partials, h->ncases);
+  HIPCHK(hipGetLastError());
+  return Q2D_OK;
+}
+
+int read_out(q2d_handle* h, q2d_step_out* out, int* first_err) {
+  std::vector<DevCtrl> ctl(h->ncases);
+  HIPCHK(hipMemcpyAsync(ctl.data(), h->d_ctrl, sizeof(DevCtrl) * h->ncases, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  *first_err = Q2D_OK;
+  for (int c = 0; c < h->ncases; ++c) {
+    const DevCtrl& k = ctl[c];
+    if (k.status != 0 && *first_err == Q2D_OK) *first_err = k.status;
+    if (!out) continue;
+    q2d_step_out& o = out[c];
+    memset(&o, 0, sizeof(o));
+    o.dtglobal = k.dtglobal; o.time = k.time;
+    for (int v = 0; v < 4; ++v) o.residual[v] = k.residual[v];
+    o.iter = k.iter; o.todo = k.todo; o.status = k.status;
+    if (k.status != 0) {
+      o.fail_b = (int)((k.fail_key >> 32) & 0xff);
+      o.fail_j = (int)((k.fail_key >> 16) & 0xffff);
+      o.fail_i = (int)(k.fail_key & 0xffff);
+    }
+  }
+  return Q2D_OK;
+}
+
+// Fortran layout (nprim fastest) -> SoA planes of the device layout
+void pack_state(const HostBlock& b, const double* P, std::vector<double>& out) {
+  const int ngi = b.ni + 4, ngj = b.nj + 4;
+  out.assign((size_t)b.plane * 4, 0.0);
+  for (int j = 0; j < ngj; ++j)
+    for (int i = 0; i < ngi; ++i)
+      for (int v = 0; v < 4; ++v) out[(size_t)v * b.plane + (size_t)j * b.pitch + i] = P[((size_t)j * ngi + i) * 4 + v];
+}
+
+}  // namespace
+
+// ===========================================================================
+// C-ABI
+// ===========================================================================
+extern "C" {
+
+const char* q2d_version(void) { return "q2d-mi355x 0.1 (gfx950)"; }
+const char* q2d_last_error(void) { return g_err.c_str(); }
+
+int q2d_device_count(int* count) {
+  if (!count) return fail(Q2D_E_ARG, "count is NULL");
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess) { *count = 0; return fail(Q2D_E_NODEVICE, hipGetErrorString(e)); }
+  *count = n;
+  return Q2D_OK;
+}
+
+int q2d_gpu_destroy(q2d_handle* h) {
+  if (!h) return Q2D_OK;
+  if (h->graph) (void)hipGraphExecDestroy(h->graph);
+  for (auto& b : h->hb) if (b.arena) (void)hipFree(b.arena);
+  (void)hipFree(h->d_blocks); (void)hipFree(h->d_cases); (void)hipFree(h->d_ctrl); (void)hipFree(h->d_bc);
+  (void)hipFree(h->d_bcface); (void)hipFree(h->d_items); (void)hipFree(h->d_partials); (void)hipFree(h->d_bcflux);
+  if (h->ev0) (void)hipEventDestroy(h->ev0);
+  if (h->ev1) (void)hipEventDestroy(h->ev1);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+  return Q2D_OK;
+}
+
+int q2d_gpu_create(const q2d_problem* prob, q2d_handle** out) {
+  if (!prob || !out) return fail(Q2D_E_ARG, "NULL argument");
+  *out = nullptr;
+  if (prob->nblocks <= 0 || prob->ncases <= 0 || !prob->blocks || !prob->cases)
+    return fail(Q2D_E_ARG, "empty problem");
+  if (prob->ncases > kMaxCases) return fail(Q2D_E_ARG, "too many cases");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(Q2D_E_NODEVICE, "no HIP device");
+
+  // ---- validation ----
+  std::vector<int> first_block(prob->ncases, -1), nblk(prob->ncases, 0);
+  for (int b = 0; b < prob->nblocks; ++b) {
+    const q2d_block_desc& d = prob->blocks[b];
+    if (d.case_id < 0 || d.case_id >= prob->ncases) return fail(Q2D_E_ARG, "block case_id out of range");
+    if (d.ni < 1 || d.nj < 1 || d.ni > 60000 || d.nj > 60000) return fail(Q2D_E_ARG, "block dimensions out of range");
+    if (!d.P || !d.dl || !d.M || !d.sur || !d.face_i || !d.face_j) return fail(Q2D_E_ARG, "missing block array");
+    if (first_block[d.case_id] < 0) first_block[d.case_id] = b;
+    else if (b != first_block[d.case_id] + nblk[d.case_id]) return fail(Q2D_E_ARG, "blocks of a case must be contiguous");
+    nblk[d.case_id]++;
+  }
+  for (int c = 0; c < prob->ncases; ++c) {
+    const q2d_scheme& s = prob->cases[c].scheme;
+    if (nblk[c] == 0) return fail(Q2D_E_ARG, "case without blocks");
+    if (nblk[c] > 255) return fail(Q2D_E_ARG, "more than 255 blocks in a case");
+    if (s.n_rk < 1 || s.n_rk > 3) return fail(Q2D_E_ARG, "n_rk must be 1, 2 or 3");
+    if (s.limiter < 0 || s.limiter > 6) return fail(Q2D_E_ARG, "unknown limiter");
+    if (s.riemann < 0 || s.riemann > 1) return fail(Q2D_E_ARG, "unknown Riemann solver");
+    if (s.model != 0) return fail(Q2D_E_UNSUPPORTED, "viscous model not available on this path");
+    if (s.source < 0 || s.source > 2) return fail(Q2D_E_ARG, "unknown source");
+    if (s.precision < 0 || s.precision > 1) return fail(Q2D_E_ARG, "unknown precision");
+    if (!(s.gas_R > 0) || !(s.gas_cp > s.gas_R)) return fail(Q2D_E_ARG, "gas constants invalid");
+    if (s.precision != prob->cases[0].scheme.precision) return fail(Q2D_E_ARG, "all cases must use one precision");
+    for (int k = 0; k < nblk[c]; ++k) {
+      const q2d_block_desc& d = prob->blocks[first_block[c] + k];
+      if (s.source == Q2D_SRC_AXI && !d.src_b) return fail(Q2D_E_ARG, "axisymmetric source needs src_b (axis_inv_r)");
+      if (s.source == Q2D_SRC_AREA && (!d.src_a || !d.src_b)) return fail(Q2D_E_ARG, "area source needs src_a/src_b");
+    }
+  }
+
+  q2d_handle* h = new q2d_handle();
+  h->nblocks = prob->nblocks; h->ncases = prob->ncases; h->nbc = prob->nbc;
+  h->precision_mixed = prob->cases[0].scheme.precision == Q2D_PREC_MIXED;
+  {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, 0) == hipSuccess) h->cu_count = prop.multiProcessorCount;
+  }
+  auto bail = [&](int code) { q2d_gpu_destroy(h); return code; };
+
+  // ---- per-case constants ----
+  h->dc.resize(h->ncases);
+  for (int c = 0; c < h->ncases; ++c) {
+    const q2d_case_desc& cd = prob->cases[c];
+    const q2d_scheme& s = cd.scheme;
+    DevCase& k = h->dc[c];
+    memset(&k, 0, sizeof(k));
+    k.n_rk = s.n_rk; k.limiter = s.limiter; k.riemann = s.riemann; k.time_accurate = s.time_accurate;
+    k.model = s.model; k.source = s.source; k.precision = s.precision; k.ramp_iter = s.ramp_iter;
+    k.ndir = (prob->blocks[first_block[c]].nj == 1) ? 1 : 2;     // Check_Mesh_Type (Lib_Metrics.f90:24-37)
+    k.itermax = cd.itermax; k.cfl = s.cfl; k.vnn = s.vnn;
+    k.R = s.gas_R; k.cp = s.gas_cp; k.cv = s.gas_cp - s.gas_R; k.gam = s.gas_cp / k.cv;
+    k.res_threshold = cd.res_threshold; k.time_threshold = cd.time_threshold;
+    h->max_rk = std::max(h->max_rk, s.n_rk);
+  }
+
+  // ---- blocks: SoA planes in one arena per block ----
+  h->hb.resize(h->nblocks);
+  h->db.resize(h->nblocks);
+  const int kPlanes = 12 + 18;   // 3 state buffers x 4 + geometry/dt/source planes
+  for (int b = 0; b < h->nblocks; ++b) {
+    const q2d_block_desc& d = prob->blocks[b];
+    HostBlock& hb = h->hb[b];
+    hb.ni = d.ni; hb.nj = d.nj; hb.case_id = d.case_id; hb.blk_in_case = b - first_block[d.case_id] + 1;
+    hb.pitch = ((d.ni + 4) + 15) / 16 * 16;
+    hb.plane = (long long)hb.pitch * (d.nj + 4);
+    hb.arena_bytes = sizeof(double) * (size_t)hb.plane * kPlanes;
+    if (hipMalloc(&hb.arena, hb.arena_bytes) != hipSuccess) return bail(fail(Q2D_E_NOMEM, "block allocation failed"));
+    if (hipMemset(hb.arena, 0, hb.arena_bytes) != hipSuccess) return bail(fail(Q2D_E_HIP, "memset failed"));
+    const long long pl = hb.plane;
+    const int ngi = d.ni + 4, ngj = d.nj + 4, pitch = hb.pitch;
+    std::vector<double> host((size_t)pl * kPlanes, 0.0);
+    auto plane = [&](int k) { return host.data() + (size_t)k * pl; };
+    // state buffer 0 (planes 0..3), buffers 1, 2 initialised to the same values
+    for (int j = 0; j < ngj; ++j)
+      for (int i = 0; i < ngi; ++i)
+        for (int v = 0; v < 4; ++v) {
+          double val = d.P[((size_t)j * ngi + i) * 4 + v];
+          for (int s = 0; s < 3; ++s) plane(4 * s + v)[(size_t)j * pitch + i] = val;
+        }
+    // geometry over the ghost extent
+    for (int j = 0; j < ngj; ++j)
+      for (int i = 0; i < ngi; ++i) {
+        const size_t src = (size_t)j * ngi + i, dst = (size_t)j * pitch + i;
+        plane(12)[dst] = d.dl[src * 2 + 0];
+        plane(13)[dst] = d.dl[src * 2 + 1];
+        plane(14)[dst] = d.sur[src];
+        plane(15)[dst] = d.M[src * 4 + 0];   // c(1,1)
+        plane(16)[dst] = d.M[src * 4 + 1];   // c(2,1)
+        plane(17)[dst] = d.M[src * 4 + 2];   // c(1,2)
+        plane(18)[dst] = d.M[src * 4 + 3];   // c(2,2)
+      }
+    // i-faces f(0:ni,1:nj) at cell (i,j); j-faces f(1:ni,0:nj) at cell (i,j)
+    for (int j = 1; j <= d.nj; ++j)
+      for (int i = 0; i <= d.ni; ++i) {
+        const size_t src = ((size_t)(j - 1) * (d.ni + 1) + i) * 3, dst = (size_t)(j + 1) * pitch + (i + 1);
+        plane(19)[dst] = d.face_i[src]; plane(20)[dst] = d.face_i[src + 1]; plane(21)[dst] = d.face_i[src + 2];
+      }
+    for (int j = 0; j <= d.nj; ++j)
+      for (int i = 1; i <= d.ni; ++i) {
+        const size_t src = ((size_t)j * d.ni + (i - 1)) * 3, dst = (size_t)(j + 1) * pitch + (i + 1);
+        plane(22)[dst] = d.face_j[src]; plane(23)[dst] = d.face_j[src + 1]; plane(24)[dst] = d.face_j[src + 2];
+      }
+    for (int j = 1; j <= d.nj; ++j)
+      for (int i = 1; i <= d.ni; ++i) {
+        const size_t src = (size_t)(j - 1) * d.ni + (i - 1), dst = (size_t)(j + 1) * pitch + (i + 1);
+        if (d.src_a) plane(26)[dst] = d.src_a[src];
+        if (d.src_b) plane(27)[dst] = d.src_b[src];
+      }
+    if (hipMemcpy(hb.arena, host.data(), hb.arena_bytes, hipMemcpyHostToDevice) != hipSuccess)
+      return bail(fail(Q2D_E_HIP, "upload failed"));
+    DevBlock& db = h->db[b];
+    memset(&db, 0, sizeof(db));
+    db.ni = d.ni; db.nj = d.nj; db.pitch = pitch; db.case_id = d.case_id; db.blk_in_case = hb.blk_in_case;
+    db.plane = pl;
+    for (int s = 0; s < 3; ++s) db.P[s] = hb.arena + (size_t)4 * s * pl;
+    double* a = hb.arena;
+    db.dlx = a + 12 * pl; db.dly = a + 13 * pl; db.sur = a + 14 * pl;
+    db.m11 = a + 15 * pl; db.m21 = a + 16 * pl; db.m12 = a + 17 * pl; db.m22 = a + 18 * pl;
+    db.fix = a + 19 * pl; db.fiy = a + 20 * pl; db.fil = a + 21 * pl;
+    db.fjx = a + 22 * pl; db.fjy = a + 23 * pl; db.fjl = a + 24 * pl;
+    db.dtl = a + 25 * pl; db.sa = a + 26 * pl; db.sb = a + 27 * pl;
+    for (int f = 0; f < 4; ++f) db.bcf[f] = -1;
+  }
+
+  // ---- boundary records ----
+  std::vector<std::vector<int>> face_list(h->nblocks * 4);
+  for (int b = 0; b < h->nblocks; ++b) {
+    face_list[b * 4 + 0].assign(prob->blocks[b].nj, -1);
+    face_list[b * 4 + 1].assign(prob->blocks[b].nj, -1);
+    face_list[b * 4 + 2].assign(prob->blocks[b].ni, -1);
+    face_list[b * 4 + 3].assign(prob->blocks[b].ni, -1);
+  }
+  h->dbc.resize(h->nbc);
+  int prev_case = -1, prev_blk = 0, prev_face = 0;
+  for (int k = 0; k < h->nbc; ++k) {
+    const q2d_bc_entry& e = prob->bc[k];
+    if (e.case_id < 0 || e.case_id >= h->ncases) return bail(fail(Q2D_E_ARG, "bc case_id out of range"));
+    const int c = e.case_id;
+    if (e.b < 1 || e.b > nblk[c] || e.f < 1 || e.f > 4) return bail(fail(Q2D_E_ARG, "bc block/face out of range"));
+    // records ordered by case, block, face (Mod_BC_Fluxes.f90:79-86)
+    if (c < prev_case || (c == prev_case && (e.b < prev_blk || (e.b == prev_blk && e.f < prev_face))))
+      return bail(fail(Q2D_E_ARG, "bc records must be ordered by case, block and face"));
+    prev_case = c; prev_blk = e.b; prev_face = e.f;
+    const int gb = first_block[c] + e.b - 1;
+    const q2d_block_desc& d = prob->blocks[gb];
+    const bool iface = e.f <= 2;
+    if (iface) {
+      if (e.j < 1 || e.j > d.nj || e.i != (e.f == 1 ? 1 : d.ni)) return bail(fail(Q2D_E_ARG, "bc cell not on its face"));
+    } else {
+      if (e.i < 1 || e.i > d.ni || e.j != (e.f == 3 ? 1 : d.nj)) return bail(fail(Q2D_E_ARG, "bc cell not on its face"));
+    }
+    switch (e.type) {
+      case 101: case 201: case 300: case 302: case 400: case 405: case 406:
+      case 0: case 401: case 402: case 403: case 404: case 407: case 420:
+        break;
+      default: return bail(fail(Q2D_E_UNSUPPORTED, "bc type not available on this path"));
+    }
+    if (e.type != 101 && e.type != 201 && e.type != 300 && e.type != 302 && e.type != 400 && e.type != 405 &&
+        e.type != 406)
+      return bail(fail(Q2D_E_UNSUPPORTED, "bc flux type not available on this path"));
+    DevBC& r = h->dbc[k];
+    memset(&r, 0, sizeof(r));
+    r.type = e.type; r.blk = gb; r.i = e.i; r.j = e.j; r.f = e.f;
+    if (e.type == 101 || e.type == 201) {
+      if (e.bs < 1 || e.bs > nblk[c] || e.fs < 1 || e.fs > 4) return bail(fail(Q2D_E_ARG, "bc donor out of range"));
+      const q2d_block_desc& s = prob->blocks[first_block[c] + e.bs - 1];
+      // donor cells Is + guide(Fs)*(g-1), g = 1..gc, must be interior cells of the donor
+      static const int gd[5][2] = {{0, 0}, {1, 0}, {-1, 0}, {0, 1}, {0, -1}};
+      for (int g = 1; g <= 2; ++g) {
+        int ii = e.is + gd[e.fs][0] * (g - 1), jj = e.js + gd[e.fs][1] * (g - 1);
+        if (ii < 1 || ii > s.ni || jj < 1 || jj > s.nj) return bail(fail(Q2D_E_ARG, "bc donor cell outside donor block"));
+      }
+      r.dblk = first_block[c] + e.bs - 1; r.is = e.is; r.js = e.js; r.fs = e.fs;
+    }
+    r.mach = e.mach; r.T = e.T; r.p = e.p; r.rel_fac = e.rel_fac;
+    if (e.type == 405) {                                // Compute_Inflow_Direction (Lib_BC_Fluxes_Inflow.f90:548-560)
+      double n1, n2;
+      if (iface) { int fi = e.i - (e.f % 2); size_t s = ((size_t)(e.j - 1) * (d.ni + 1) + fi) * 3; n1 = d.face_i[s]; n2 = d.face_i[s + 1]; }
+      else { int fj = e.j - (e.f % 2); size_t s = ((size_t)fj * d.ni + (e.i - 1)) * 3; n1 = d.face_j[s]; n2 = d.face_j[s + 1]; }
+      double a = (e.alpha >= 0.5 * DBL_MAX) ? atan(n2 / (n1 + 1e-20)) : e.alpha;
+      r.cos_a = cos(a); r.sin_a = sin(a);
+    }
+    std::vector<int>& fl = face_list[gb * 4 + (e.f - 1)];
+    const int pos = iface ? e.j - 1 : e.i - 1;
+    if (fl[pos] >= 0) return bail(fail(Q2D_E_ARG, "two bc records for one boundary face"));
+    fl[pos] = k;
+  }
+  // every boundary face must have a record
+  for (int b = 0; b < h->nblocks; ++b)
+    for (int f = 0; f < 4; ++f) {
+      std::vector<int>& fl = face_list[b * 4 + f];
+      bool any = false;
+      for (int v : fl) any |= (v >= 0);
+      if (!any) continue;
+      for (int v : fl) if (v < 0) return bail(fail(Q2D_E_ARG, "boundary face partially covered by bc records"));
+      h->db[b].bcf[f] = (int)h->bcface.size();
+      h->bcface.insert(h->bcface.end(), fl.begin(), fl.end());
+    }
+
+  // ---- device copies ----
+  if (hipStreamCreate(&h->stream) != hipSuccess) return bail(fail(Q2D_E_HIP, "stream creation failed"));
+  if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess)
+    return bail(fail(Q2D_E_HIP, "event creation failed"));
+  if (hipMalloc(&h->d_blocks, sizeof(DevBlock) * h->nblocks) != hipSuccess ||
+      hipMalloc(&h->d_cases, sizeof(DevCase) * h->ncases) != hipSuccess ||
+      hipMalloc(&h->d_ctrl, sizeof(DevCtrl) * h->ncases) != hipSuccess ||
+      hipMalloc(&h->d_bc, sizeof(DevBC) * std::max(1, h->nbc)) != hipSuccess ||
+      hipMalloc(&h->d_bcflux, sizeof(BCFlux) * std::max(1, h->nbc)) != hipSuccess ||
+      hipMalloc(&h->d_bcface, sizeof(int) * std::max<size_t>(1, h->bcface.size())) != hipSuccess)
+    return bail(fail(Q2D_E_NOMEM, "descriptor allocation failed"));
+  std::vector<DevCtrl> ctl(h->ncases);
+  for (auto& k : ctl) { memset(&k, 0, sizeof(k)); k.active = 1; k.todo = 1; k.fail_key = ~0ull; }
+  if (hipMemcpy(h->d_blocks, h->db.data(), sizeof(DevBlock) * h->nblocks, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(h->d_ctrl, ctl.data(), sizeof(DevCtrl) * h->ncases, hipMemcpyHostToDevice) != hipSuccess ||
+      (h->nbc > 0 && hipMemcpy(h->d_bc, h->dbc.data(), sizeof(DevBC) * h->nbc, hipMemcpyHostToDevice) != hipSuccess) ||
+      (!h->bcface.empty() &&
+       hipMemcpy(h->d_bcface, h->bcface.data(), sizeof(int) * h->bcface.size(), hipMemcpyHostToDevice) != hipSuccess))
+    return bail(fail(Q2D_E_HIP, "descriptor upload failed"));
+  int rc = build_items(h);
+  if (rc != Q2D_OK) return bail(rc);
+  *out = h;
+  return Q2D_OK;
+}
+
+int q2d_gpu_step(q2d_handle* h, q2d_step_out* out) {
+  if (!h) return fail(Q2D_E_ARG, "NULL handle");
+  int rc = launch_step(h);
+  if (rc != Q2D_OK) return rc;
+  int err = Q2D_OK;
+  rc = read_out(h, out, &err);
+  if (rc != Q2D_OK) return rc;
+  if (err != Q2D_OK) return fail(err, err == Q2D_E_STATE ? "NaN or p<0 detected" : "Non-physical state detected at the interface");
+  return Q2D_OK;
+}
+
+int q2d_gpu_run(q2d_handle* h, int32_t nsteps, q2d_step_out* out) {
+  if (!h) return fail(Q2D_E_ARG, "NULL handle");
+  if (nsteps < 0) return fail(Q2D_E_ARG, "nsteps < 0");
+  if (h->timing) {                                     // per-launch events cannot live inside a graph
+    for (int s = 0; s < nsteps; ++s) { int rc = launch_step(h); if (rc != Q2D_OK) return rc; }
+  } else if (nsteps > 0) {
+    if (!h->graph) {
+      hipGraph_t g;
+      HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+      int rc = launch_step(h);
+      hipError_t ce = hipStreamEndCapture(h->stream, &g);
+      if (rc != Q2D_OK) return rc;
+      if (ce != hipSuccess) return fail(Q2D_E_HIP, std::string("graph capture: ") + hipGetErrorString(ce));
+      HIPCHK(hipGraphInstantiate(&h->graph, g, nullptr, nullptr, 0));
+      (void)hipGraphDestroy(g);
+    }
+    for (int s = 0; s < nsteps; ++s) HIPCHK(hipGraphLaunch(h->graph, h->stream));
+  }
+  int err = Q2D_OK;
+  int rc = read_out(h, out, &err);
+  if (rc != Q2D_OK) return rc;
+  if (err != Q2D_OK) return fail(err, err == Q2D_E_STATE ? "NaN or p<0 detected" : "Non-physical state detected at the interface");
+  return Q2D_OK;
+}
+
+int q2d_gpu_download_state(q2d_handle* h, int32_t block, double* P) {
+  if (!h || !P || block < 0 || block >= h->nblocks) return fail(Q2D_E_ARG, "bad argument");
+  HostBlock& b = h->hb[block];
+  DevCtrl k;
+  HIPCHK(hipMemcpyAsync(&k, h->d_ctrl + b.case_id, sizeof(DevCtrl), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  std::vector<double> tmp((size_t)b.plane * 4);
+  HIPCHK(hipMemcpy(tmp.data(), b.arena + (size_t)4 * k.cur * b.plane, sizeof(double) * tmp.size(), hipMemcpyDeviceToHost));
+  const int ngi = b.ni + 4, ngj = b.nj + 4;
+  for (int j = 0; j < ngj; ++j)
+    for (int i = 0; i < ngi; ++i)
+      for (int v = 0; v < 4; ++v) P[((size_t)j * ngi + i) * 4 + v] = tmp[(size_t)v * b.plane + (size_t)j * b.pitch + i];
+  return Q2D_OK;
+}
+
+int q2d_gpu_upload_state(q2d_handle* h, int32_t block, const double* P) {
+  if (!h || !P || block < 0 || block >= h->nblocks) return fail(Q2D_E_ARG, "bad argument");
+  HostBlock& b = h->hb[block];
+  DevCtrl k;
+  HIPCHK(hipMemcpyAsync(&k, h->d_ctrl + b.case_id, sizeof(DevCtrl), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  std::vector<double> tmp;
+  pack_state(b, P, tmp);
+  HIPCHK(hipMemcpy(b.arena + (size_t)4 * k.cur * b.plane, tmp.data(), sizeof(double) * tmp.size(), hipMemcpyHostToDevice));
+  return Q2D_OK;
+}
+
+int q2d_gpu_download_dt(q2d_handle* h, int32_t block, double* dtlocal) {
+  if (!h || !dtlocal || block < 0 || block >= h->nblocks) return fail(Q2D_E_ARG, "bad argument");
+  HostBlock& b = h->hb[block];
+  DevCtrl k;
+  HIPCHK(hipMemcpy(&k, h->d_ctrl + b.case_id, sizeof(DevCtrl), hipMemcpyDeviceToHost));
+  const DevCase& c = h->dc[b.case_id];
+  if (c.time_accurate) {
+    for (int i = 0; i < b.ni * b.nj; ++i) dtlocal[i] = k.dtglobal;
+    return Q2D_OK;
+  }
+  std::vector<double> tmp((size_t)b.plane);
+  HIPCHK(hipMemcpy(tmp.data(), b.arena + (size_t)25 * b.plane, sizeof(double) * tmp.size(), hipMemcpyDeviceToHost));
+  for (int j = 1; j <= b.nj; ++j)
+    for (int i = 1; i <= b.ni; ++i) dtlocal[(size_t)(j - 1) * b.ni + (i - 1)] = tmp[(size_t)(j + 1) * b.pitch + (i + 1)];
+  return Q2D_OK;
+}
+
+int q2d_gpu_set_timing(q2d_handle* h, int32_t enable) {
+  if (!h) return fail(Q2D_E_ARG, "NULL handle");
+  h->timing = enable != 0;
+  h->timed_launches = 0; h->timed_ms = 0.0;
+  return Q2D_OK;
+}
+
+int q2d_gpu_timing_stats(q2d_handle* h, int64_t* launches, double* ms_total) {
+  if (!h || !launches || !ms_total) return fail(Q2D_E_ARG, "NULL argument");
+  *launches = h->timed_launches; *ms_total = h->timed_ms;
+  return Q2D_OK;
+}
+
+int q2d_gpu_set_tiling(q2d_handle* h, int32_t rows_per_item, int32_t waves_per_group) {
+  if (!h) return fail(Q2D_E_ARG, "NULL handle");
+  if (rows_per_item < 0 || rows_per_item > 4096) return fail(Q2D_E_ARG, "rows_per_item out of range");
+  if (waves_per_group != 0 && waves_per_group != 1 && waves_per_group != 2 && waves_per_group != 4)
+    return fail(Q2D_E_ARG, "waves_per_group must be 1, 2 or 4");
+  if (rows_per_item > 0) h->rows_per_item = rows_per_item;
+  if (waves_per_group > 0) h->waves_per_group = waves_per_group;
+  return build_items(h);
+}
+
+int q2d_gpu_get_tiling(q2d_handle* h, int32_t* rows_per_item, int32_t* waves_per_group) {
+  if (!h || !rows_per_item || !waves_per_group) return fail(Q2D_E_ARG, "NULL argument");
+  *rows_per_item = h->rows_per_item; *waves_per_group = h->waves_per_group;
+  return Q2D_OK;
+}
+
+int q2d_gpu_info(q2d_handle* h, int64_t* cells, int64_t* work_items) {
+  if (!h || !cells || !work_items) return fail(Q2D_E_ARG, "NULL argument");
+  long long n = 0;
+  for (auto& b : h->hb) n += (long long)b.ni * b.nj;
+  *cells = n; *work_items = (int64_t)h->items.size();
+  return Q2D_OK;
+}
+
+}  // extern "C"
