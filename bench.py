@@ -100,7 +100,16 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--form", choices=["both", "fp64", "mixed"], default="both",
+                    help="profiling aid: run only one arithmetic form (no JSON line)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="run only the HBM reference copy kernel (1 GiB, 10 launches) and print GB/s")
     args = ap.parse_args()
+    if args.copy_probe:
+        from q2d_amd.capi import hbm_copy_gbps
+        gbs, ms = hbm_copy_gbps()
+        print(json.dumps({"copy_GBps": gbs, "ms_per_copy": ms, "bytes_per_copy": 2 << 30}))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is designed (DESIGN.md) "
                                    "but not built", "n_gpus": args.gpus}))
@@ -112,6 +121,10 @@ def main():
         return cases.planar_shock(seed=args.seed, n=args.n, steps=1 << 30, precision=precision)
 
     log(f"bench: planar shock 4 x {args.n}^2, SSP-RK3, warmup {args.warmup}, steps {args.steps}")
+    if args.form != "both":
+        r, _ = time_form(problem_fn, args.form, args.steps, args.warmup)
+        log(f"bench: {args.form} {r['cell_updates_per_s'] / 1e9:.3f} G cell-updates/s, stage kernel {r['stage_kernel_ms']:.4f} ms")
+        return
     plain, rho64 = time_form(problem_fn, "fp64", args.steps, args.warmup)
     log(f"bench: plain fp64  {plain['cell_updates_per_s'] / 1e9:.3f} G cell-updates/s")
     mixed, rho32 = time_form(problem_fn, "mixed", args.steps, args.warmup)

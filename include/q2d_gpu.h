@@ -207,6 +207,10 @@ int q2d_gpu_timing_stats(q2d_handle *h, int64_t *launches, double *ms_total);
 int q2d_gpu_set_tiling(q2d_handle *h, int32_t rows_per_item, int32_t waves_per_group);
 int q2d_gpu_get_tiling(q2d_handle *h, int32_t *rows_per_item, int32_t *waves_per_group);
 
+/* HBM reference: time `reps` launches of a plain device copy of nbytes (read nbytes + write
+ * nbytes); used to measure the box's achievable bandwidth and to calibrate HBM byte counters. */
+int q2d_hbm_copy(int64_t nbytes, int32_t reps, double *ms_per_copy);
+
 /* Work-item count and cell count of the current launch. */
 int q2d_gpu_info(q2d_handle *h, int64_t *cells, int64_t *work_items);
 

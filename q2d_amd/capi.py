@@ -120,7 +120,7 @@ def to_problem_c(problem):
 
 
 _LIB = None
-LIB_PATH = Path(__file__).resolve().parent / "lib" / "libq2d_gpu.so"
+LIB_PATH = Path(os.environ.get("Q2D_GPU_LIB", Path(__file__).resolve().parent / "lib" / "libq2d_gpu.so"))
 
 # exported symbols and their signatures (declared in include/q2d_gpu.h)
 SIGNATURES = {
@@ -139,7 +139,16 @@ SIGNATURES = {
     "q2d_gpu_set_tiling": ([C.c_void_p, C.c_int32, C.c_int32], C.c_int),
     "q2d_gpu_get_tiling": ([C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)], C.c_int),
     "q2d_gpu_info": ([C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)], C.c_int),
+    "q2d_hbm_copy": ([C.c_int64, C.c_int32, C.POINTER(C.c_double)], C.c_int),
 }
+
+
+def hbm_copy_gbps(nbytes=1 << 30, reps=10):
+    """Achieved bandwidth of the library's plain copy kernel: (read + write bytes) / time."""
+    lib = load_library()
+    ms = C.c_double(0.0)
+    _check(lib, lib.q2d_hbm_copy(int(nbytes), int(reps), C.byref(ms)))
+    return 2.0 * nbytes / (ms.value * 1e-3) / 1e9, ms.value
 
 
 def load_library(path=None):

@@ -178,6 +178,9 @@ __device__ __forceinline__ void riemann(const Gas<T>& g, T ul, T vl, T pl, T al,
   F[0] = F_r; F[1] = F_u; F[2] = F_v; F[3] = F_E;
 }
 
+// reciprocal used by the mixed form's FP64 update (one division, then products)
+__device__ __forceinline__ double rcp64(double a) { return 1.0 / a; }
+
 // ---- conservative <-> primitive (FP64 always) ------------------------------
 __device__ __forceinline__ void prim2cons(const Gas<double>& g, double rho, double u, double v, double p, double U[4]) {
   double T = p / (rho * g.R);

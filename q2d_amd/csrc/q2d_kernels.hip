@@ -36,6 +36,19 @@ namespace {
 constexpr int kWave = 64;
 constexpr int kOwn = 60;          // owned columns per strip
 constexpr int kMaxCases = 1 << 20;
+// Fuse the next step's Compute_dt into the last RK stage (saves the separate dt pass over the
+// state).  The FP64 stage kernel keeps a separate dt pass: the fused version costs it registers.
+#ifndef Q2D_FUSE_DT_FP64
+#define Q2D_FUSE_DT_FP64 0
+#endif
+constexpr bool kFuseDtFp64 = Q2D_FUSE_DT_FP64 != 0;
+#ifndef Q2D_FUSE_DT_MIXED
+#define Q2D_FUSE_DT_MIXED 0
+#endif
+constexpr bool kFuseDtMixed = Q2D_FUSE_DT_MIXED != 0;
+#ifndef Q2D_STAGE_MIN_WAVES
+#define Q2D_STAGE_MIN_WAVES 1
+#endif
 
 // ---------------------------------------------------------------------------
 // device-side descriptors
@@ -50,11 +63,12 @@ struct DevCase {
 
 struct DevCtrl {
   unsigned long long dtmin_bits;
+  unsigned long long dtmin_next_bits;   // next step's dt, reduced by the last RK stage
   unsigned long long fail_key;
   double dtglobal, time;
   double residual[4];
   int iter, iter_from_call, todo, active;
-  int cur, status, stage_count, pad;
+  int cur, status, stage_count, dtsel;
 };
 
 struct DevBlock {
@@ -68,7 +82,7 @@ struct DevBlock {
   const double *fjx, *fjy, *fjl; // j-face
   const double *m11, *m21, *m12, *m22;
   const double *sa, *sb;
-  double* dtl;
+  double* dtl[2];                // local dt, double-buffered: [dtsel] holds the current step's dt
 };
 
 struct DevBC {
@@ -83,10 +97,54 @@ struct Item { int blk, x0, y0, y1; };
 __device__ __forceinline__ int stage_in(int cur, int irk) { return irk == 1 ? cur : (cur + 1 + ((irk - 2) & 1)) % 3; }
 __device__ __forceinline__ int stage_out(int cur, int irk) { return (cur + 1 + ((irk - 1) & 1)) % 3; }
 
+// Global-address-space views of device pointers: loads/stores compile to global_* instead of flat_*
+// (flat accesses also count against the LDS counter, which would make every cross-lane shuffle wait
+// for outstanding HBM loads).
+typedef const __attribute__((address_space(1))) double* GP;
+typedef __attribute__((address_space(1))) double* GPW;
+__device__ __forceinline__ GP gp(const double* p) { return (GP)p; }
+__device__ __forceinline__ GPW gpw(double* p) { return (GPW)p; }
+
+// Neighbour-lane exchange within the wavefront.  With Q2D_DPP the wave-wide DPP shifts of GFX9
+// (wave_shl:1 = 0x130: lane l reads lane l+1; wave_shr:1 = 0x138: lane l reads lane l-1) move the
+// data inside the VALU with no LDS round trip; the edge lanes read their own value (unused).
+#ifndef Q2D_DPP
+#define Q2D_DPP 1
+#endif
+#if Q2D_DPP
+__device__ __forceinline__ int dpp_dn(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x130, 0xF, 0xF, false); }
+__device__ __forceinline__ int dpp_up(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x138, 0xF, 0xF, false); }
+__device__ __forceinline__ double shdn(double v) {
+  long long b = __double_as_longlong(v);
+  int lo = dpp_dn((int)(b & 0xffffffffll)), hi = dpp_dn((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ double shup(double v) {
+  long long b = __double_as_longlong(v);
+  int lo = dpp_up((int)(b & 0xffffffffll)), hi = dpp_up((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ float shdn(float v) { return __int_as_float(dpp_dn(__float_as_int(v))); }
+__device__ __forceinline__ float shup(float v) { return __int_as_float(dpp_up(__float_as_int(v))); }
+#else
 __device__ __forceinline__ double shdn(double v) { return __shfl_down(v, 1, kWave); }
 __device__ __forceinline__ double shup(double v) { return __shfl_up(v, 1, kWave); }
 __device__ __forceinline__ float shdn(float v) { return __shfl_down(v, 1, kWave); }
 __device__ __forceinline__ float shup(float v) { return __shfl_up(v, 1, kWave); }
+#endif
+
+// Per-row data the stage kernel needs for the row it finalises.
+struct RowGeo { double dlx, fix, fiy, fil, fjx, fjy, fjl, sur, dtl, po[4]; };
+__device__ __forceinline__ void load_geo(RowGeo& g, long long o, GP dlx, GP fix, GP fiy, GP fil, GP fjx, GP fjy,
+                                         GP fjl, GP sur, GP dtl, GP PO, long long pl, bool want_dtl, GP m11,
+                                         bool want_m) {
+  g.dlx = dlx[o]; g.fix = fix[o]; g.fiy = fiy[o]; g.fil = fil[o];
+  g.fjx = fjx[o]; g.fjy = fjy[o]; g.fjl = fjl[o]; g.sur = sur[o];
+  g.dtl = want_dtl ? dtl[o] : 0.0;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) g.po[v] = PO[v * pl + o];
+  (void)m11; (void)want_m;
+}
 
 __device__ __forceinline__ unsigned long long fail_key(int stage, int kind, int b, int j, int i) {
   return ((unsigned long long)stage << 42) | ((unsigned long long)kind << 41) | ((unsigned long long)(b & 0xff) << 32) |
@@ -101,14 +159,19 @@ __device__ __forceinline__ void record_fail(DevCtrl* c, int code, unsigned long 
 // ---------------------------------------------------------------------------
 // step bookkeeping kernels (one thread per case)
 // ---------------------------------------------------------------------------
-__global__ void k_begin(DevCtrl* ctrl, int ncases) {
+// use_next = 1: the previous step's last stage already reduced this step's dt (from the same
+// state Compute_dt reads); use_next = 0: k_dt computes it (first step, or after an upload).
+__global__ void k_begin(DevCtrl* ctrl, int ncases, int use_next) {
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncases) return;
   DevCtrl& k = ctrl[c];
   if (!k.active) return;
   k.iter += 1;                                // Mod_Explicit.f90:38-40
   k.iter_from_call += 1;
-  k.dtmin_bits = (unsigned long long)__double_as_longlong(1e5);   // :49
+  const unsigned long long b1e5 = (unsigned long long)__double_as_longlong(1e5);   // :49
+  k.dtmin_bits = use_next ? (k.dtmin_next_bits < b1e5 ? k.dtmin_next_bits : b1e5) : b1e5;
+  if (use_next) k.dtsel ^= 1;                 // the plane the last stage wrote becomes current
+  k.dtmin_next_bits = b1e5;
 }
 
 __global__ void k_dt_finish(DevCtrl* ctrl, const DevCase* cases, int ncases) {
@@ -123,6 +186,28 @@ __global__ void k_dt_finish(DevCtrl* ctrl, const DevCase* cases, int ncases) {
 // ---------------------------------------------------------------------------
 // Compute_dt — Mod_dt.f90:10-104 (one wavefront per work item)
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ double cell_dt(const DevCase& C, const DevCtrl& K, double rho, double u, double v, double p,
+                                          double m11, double m21, double m12, double m22, double dlx, double dly) {
+  const double snd = sqrt(C.gam * p / rho);
+  double dtmin = 1e8;
+  {
+    const double nrm = sqrt(m11 * m11 + m12 * m12);
+    const double v1 = m11 / nrm, v2 = m12 / nrm;
+    const double lambda = fabs(u * v1 + v * v2) + snd;
+    const double dt = dlx / lambda * C.cfl;
+    if (dt < dtmin) dtmin = dt;
+  }
+  if (C.ndir == 2) {
+    const double nrm = sqrt(m21 * m21 + m22 * m22);
+    const double v1 = m21 / nrm, v2 = m22 / nrm;
+    const double lambda = fabs(u * v1 + v * v2) + snd;
+    const double dt = dly / lambda * C.cfl;
+    if (dt < dtmin) dtmin = dt;
+  }
+  if (K.iter < C.ramp_iter) dtmin = dtmin * (double)K.iter / (double)C.ramp_iter;   // Mod_dt.f90:43
+  return dtmin;
+}
+
 __global__ __launch_bounds__(256) void k_dt(const DevBlock* __restrict__ blocks, const DevCase* __restrict__ cases,
                                             DevCtrl* __restrict__ ctrl, const Item* __restrict__ items, int n_items) {
   const int lane = threadIdx.x & (kWave - 1);
@@ -137,37 +222,31 @@ __global__ __launch_bounds__(256) void k_dt(const DevBlock* __restrict__ blocks,
     const DevCase& C = cases[B.case_id];
     const int x = item.x0 - 2 + lane;
     const bool own = lane >= 2 && lane < 2 + kOwn && x <= B.ni;
-    const double* P = B.P[K.cur];
+    GP P = gp(B.P[K.cur]);
+    GP m11 = gp(B.m11), m21 = gp(B.m21), m12 = gp(B.m12), m22 = gp(B.m22), dlx = gp(B.dlx), dly = gp(B.dly);
+    GPW dtl = gpw(B.dtl[K.dtsel]);
     const long long pl = B.plane;
+    const bool local = !C.time_accurate;
     double dmin = 1e300;
     if (own) {
-      for (int r = item.y0; r <= item.y1; ++r) {
+      // four rows per trip: 40 independent loads in flight per lane
+      int r = item.y0;
+      for (; r + 3 <= item.y1; r += 4) {
+        double d[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const long long o = (long long)(r + k + 1) * B.pitch + (x + 1);
+          d[k] = cell_dt(C, K, P[o], P[pl + o], P[2 * pl + o], P[3 * pl + o], m11[o], m21[o], m12[o], m22[o], dlx[o], dly[o]);
+          if (local) dtl[o] = d[k];
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) if (d[k] < dmin) dmin = d[k];
+      }
+      for (; r <= item.y1; ++r) {
         const long long o = (long long)(r + 1) * B.pitch + (x + 1);
-        const double rho = P[o], u = P[pl + o], v = P[2 * pl + o], p = P[3 * pl + o];
-        const double snd = sqrt(C.gam * p / rho);
-        double dtmin = 1e8;
-        {
-          const double m1 = B.m11[o], m2 = B.m12[o];
-          const double nrm = sqrt(m1 * m1 + m2 * m2);
-          const double v1 = m1 / nrm, v2 = m2 / nrm;
-          const double vel_ = fabs(u * v1 + v * v2);
-          const double lambda = vel_ + snd;
-          const double dt = B.dlx[o] / lambda * C.cfl;
-          if (dt < dtmin) dtmin = dt;
-        }
-        if (C.ndir == 2) {
-          const double m1 = B.m21[o], m2 = B.m22[o];
-          const double nrm = sqrt(m1 * m1 + m2 * m2);
-          const double v1 = m1 / nrm, v2 = m2 / nrm;
-          const double vel_ = fabs(u * v1 + v * v2);
-          const double lambda = vel_ + snd;
-          const double dt = B.dly[o] / lambda * C.cfl;
-          if (dt < dtmin) dtmin = dt;
-        }
-        double dtcell = dtmin;
-        if (K.iter < C.ramp_iter) dtcell = dtcell * (double)K.iter / (double)C.ramp_iter;
-        if (!C.time_accurate) B.dtl[o] = dtcell;
-        if (dtcell < dmin) dmin = dtcell;
+        const double d = cell_dt(C, K, P[o], P[pl + o], P[2 * pl + o], P[3 * pl + o], m11[o], m21[o], m12[o], m22[o], dlx[o], dly[o]);
+        if (local) dtl[o] = d;
+        if (d < dmin) dmin = d;
       }
     }
     // wavefront min (order-independent), then one atomic per wavefront
@@ -184,15 +263,10 @@ __global__ __launch_bounds__(256) void k_dt(const DevBlock* __restrict__ blocks,
 // ---------------------------------------------------------------------------
 __constant__ int c_guide[5][2] = {{0, 0}, {1, 0}, {-1, 0}, {0, 1}, {0, -1}};   // Parameters_m.f90:13-18
 
-__global__ void k_ghost(const DevBlock* __restrict__ blocks, const DevCase* __restrict__ cases, DevCtrl* __restrict__ ctrl,
-                        const DevBC* __restrict__ bcs, int nbc, int irk) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nbc) return;
-  const DevBC e = bcs[t];
+// Ghost layers of one boundary record.  A record reads interior cells (its own block's or the donor's)
+// and writes only its own ghost cells, so all records can be filled concurrently.
+__device__ __forceinline__ void ghost_fill(const DevBlock* blocks, const DevBC& e, int buf) {
   const DevBlock& B = blocks[e.blk];
-  const DevCtrl& K = ctrl[B.case_id];
-  if (!K.active || irk > cases[B.case_id].n_rk) return;
-  const int buf = stage_in(K.cur, irk);
   double* P = B.P[buf];
   const long long pl = B.plane;
   const int pitch = B.pitch;
@@ -248,8 +322,8 @@ __global__ void k_ghost(const DevBlock* __restrict__ blocks, const DevCase* __re
 }
 
 // ---------------------------------------------------------------------------
-// Boundary-face fluxes — BC_Fluxes (Mod_BC_Fluxes.f90:49-271), one thread per
-// boundary record.  The stage kernel adds the stored values to the cells'
+// Ghost fill and boundary-face fluxes — Fill_Ghost_Cell + BC_Fluxes (Mod_BC_Fluxes.f90:49-271),
+// one thread per boundary record.  The stage kernel adds the stored values to the cells'
 // residuals in record order, exactly where the reference adds them.
 // ---------------------------------------------------------------------------
 struct BCFlux { double F[4]; int mask; int pad; };   // F already carries the face sign modfm2
@@ -268,12 +342,16 @@ __global__ __launch_bounds__(256) void k_bcflux(const DevBlock* __restrict__ blo
                                                 int irk, BCFlux* __restrict__ out) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nbc) return;
-  const DevBC& E = bcs[t];
+  const DevBC E = bcs[t];
   const DevBlock& B = blocks[E.blk];
   const DevCtrl& K = ctrl[B.case_id];
   const DevCase& C = cases[B.case_id];
   if (!K.active || irk > C.n_rk) return;
-  const double* P = B.P[stage_in(K.cur, irk)];
+  const int buf = stage_in(K.cur, irk);
+  // Fill_Ghost_Cell for this record first: the only ghost cells this record's flux reads
+  // (the connection stencil, Lib_BC_Fluxes_Connection.f90:49-62) are its own, written here.
+  ghost_fill(blocks, E, buf);
+  const double* P = B.P[buf];
   const long long pl = B.plane;
   const int pitch = B.pitch;
   auto off = [pitch](int i, int j) { return (long long)(j + 1) * pitch + (i + 1); };
@@ -525,8 +603,8 @@ __device__ __forceinline__ bool face_flux(const Gas<FT>& g, const double qa[4], 
   return bad;
 }
 
-template <typename FT, int RIE>
-__global__ __launch_bounds__(256) void k_stage(const DevBlock* __restrict__ blocks, const DevCase* __restrict__ cases,
+template <typename FT, int RIE, bool FUSE>
+__global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlock* __restrict__ blocks, const DevCase* __restrict__ cases,
                                                DevCtrl* __restrict__ ctrl, const Item* __restrict__ items, int item_lo,
                                                int item_hi, const int* __restrict__ bcface,
                                                const BCFlux* __restrict__ bcflux, double* __restrict__ partials, int irk,
@@ -552,13 +630,18 @@ __global__ __launch_bounds__(256) void k_stage(const DevBlock* __restrict__ bloc
     const int ni = B.ni, nj = B.nj, pitch = B.pitch;
     const long long pl = B.plane;
     const int bin = stage_in(K.cur, irk), bout = stage_out(K.cur, irk);
-    const double* __restrict__ Pin = B.P[bin];
-    const double* __restrict__ PO = B.P[K.cur];
-    double* __restrict__ Pout = B.P[bout];
+    GP Pin = gp(B.P[bin]);
+    GP PO = gp(B.P[K.cur]);
+    GPW Pout = gpw(B.P[bout]);
+    GP dlxp = gp(B.dlx), dlyp = gp(B.dly), surp = gp(B.sur), dtlp = gp(B.dtl[K.dtsel]);
+    GP fixp = gp(B.fix), fiyp = gp(B.fiy), filp = gp(B.fil), fjxp = gp(B.fjx), fjyp = gp(B.fjy), fjlp = gp(B.fjl);
+    GP m11p = gp(B.m11);
     const int lim = C.limiter;
+    const bool tacc = C.time_accurate != 0;
     const Gas<FT> gf{(FT)C.R, (FT)C.cp, (FT)C.cv, (FT)C.gam};
     const double c_rk = (C.n_rk == 1) ? 1.0 : (C.n_rk == 2 ? (irk == 1 ? 1.0 : 0.5)
                                                             : (irk == 1 ? 1.0 : (irk == 2 ? 0.25 : 2.0 / 3.0)));
+    const double dtg = K.dtglobal;
     const int stage_id = K.stage_count + irk;
 
     const int x = item.x0 - 2 + lane;                  // Fortran column of this lane
@@ -567,20 +650,25 @@ __global__ __launch_bounds__(256) void k_stage(const DevBlock* __restrict__ bloc
     const bool xdo = lane >= 1 && lane <= 61 && x >= 1 && x <= ni - 1;   // interior face (x|x+1)
     const int rec3 = (own && B.bcf[2] >= 0) ? bcface[B.bcf[2] + x - 1] : -1;
     const int rec4 = (own && B.bcf[3] >= 0) ? bcface[B.bcf[3] + x - 1] : -1;
+    const int rmax = nj + 2;
 
     auto at = [&](int r) { return (long long)(r + 1) * pitch + (xc + 1); };
 
     // ---- pipeline warm-up: rows y0-2, y0-1, y0 ----------------------------
-    double qA[4], qB[4], qC[4];
-    double dlyA, dlyB, dlyC;
+    double qA[4], qB[4], qC[4], qN[4];
+    double dlyA, dlyB, dlyC, dlyN;
     FT sAB[4], phiA[4], Gprev[4];
+    RowGeo gcur, gnext;
     {
       double qm[4];
-      const long long o0 = at(item.y0 - 2), o1 = at(item.y0 - 1), o2 = at(item.y0);
+      const long long o0 = at(item.y0 - 2), o1 = at(item.y0 - 1), o2 = at(item.y0), o3 = at(item.y0 + 1);
 #pragma unroll
-      for (int v = 0; v < 4; ++v) { qm[v] = Pin[v * pl + o0]; qA[v] = Pin[v * pl + o1]; qB[v] = Pin[v * pl + o2]; }
-      const double dlm = B.dly[o0];
-      dlyA = B.dly[o1]; dlyB = B.dly[o2];
+      for (int v = 0; v < 4; ++v) {
+        qm[v] = Pin[v * pl + o0]; qA[v] = Pin[v * pl + o1]; qB[v] = Pin[v * pl + o2]; qN[v] = Pin[v * pl + o3];
+      }
+      const double dlm = dlyp[o0];
+      dlyA = dlyp[o1]; dlyB = dlyp[o2]; dlyN = dlyp[o3];
+      load_geo(gnext, at(item.y0 - 1), dlxp, fixp, fiyp, filp, fjxp, fjyp, fjlp, surp, dtlp, PO, pl, false, m11p, false);
       const FT l0 = (FT)(0.5 * (dlm + dlyA)), l1 = (FT)(0.5 * (dlyA + dlyB));
       FT s0[4];
 #pragma unroll
@@ -592,13 +680,23 @@ __global__ __launch_bounds__(256) void k_stage(const DevBlock* __restrict__ bloc
       limiter4<FT>(lim, sAB, s0, phiA);
     }
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    double dtnext = 1e300;
 
     for (int r = item.y0 - 1; r <= item.y1; ++r) {
-      // row r+2 of the j-stencil
-      const long long oC = at(r + 2);
+      // software pipeline: this iteration's row r+2 was loaded one iteration ago; issue row r+3
+      // and the per-row geometry of row r+1 now, to be consumed by the next iteration.
 #pragma unroll
-      for (int v = 0; v < 4; ++v) qC[v] = Pin[v * pl + oC];
-      dlyC = B.dly[oC];
+      for (int v = 0; v < 4; ++v) qC[v] = qN[v];
+      dlyC = dlyN;
+      gcur = gnext;
+      {
+        const long long oN = at(min(r + 3, rmax));
+#pragma unroll
+        for (int v = 0; v < 4; ++v) qN[v] = Pin[v * pl + oN];
+        dlyN = dlyp[oN];
+        load_geo(gnext, at(min(r + 1, rmax)), dlxp, fixp, fiyp, filp, fjxp, fjyp, fjlp, surp, dtlp, PO, pl,
+                 own && !tacc, m11p, false);
+      }
       FT sBC[4], phiB[4], Gcur[4];
       {
         const FT lBC = (FT)(0.5 * (dlyB + dlyC));
@@ -608,9 +706,8 @@ __global__ __launch_bounds__(256) void k_stage(const DevBlock* __restrict__ bloc
       }
       // interior j-face (r | r+1) — Mod_Fluxes.f90:187-232
       if (own && r >= 1 && r <= nj - 1) {
-        const long long oA = at(r);
-        if (face_flux<FT, RIE>(gf, qA, phiA, (FT)(0.5 * dlyA), qB, phiB, (FT)(0.5 * dlyB), (FT)B.fjx[oA],
-                               (FT)B.fjy[oA], (FT)B.fjl[oA], Gcur))
+        if (face_flux<FT, RIE>(gf, qA, phiA, (FT)(0.5 * dlyA), qB, phiB, (FT)(0.5 * dlyB), (FT)gcur.fjx,
+                               (FT)gcur.fjy, (FT)gcur.fjl, Gcur))
           record_fail(&K, Q2D_E_INTERFACE, fail_key(stage_id, 0, B.blk_in_case, r, x));
       } else {
 #pragma unroll
@@ -618,9 +715,8 @@ __global__ __launch_bounds__(256) void k_stage(const DevBlock* __restrict__ bloc
       }
 
       if (r >= item.y0) {
-        // ---- i-direction on row r (lanes share the stencil) ---------------
-        const long long oA = at(r);
-        const double dlx = B.dlx[oA];
+        // ---- i-direction on row r (lanes share the stencil through DPP shifts) ----
+        const double dlx = gcur.dlx;
         double qn[4];
 #pragma unroll
         for (int v = 0; v < 4; ++v) qn[v] = shdn(qA[v]);
@@ -637,9 +733,9 @@ __global__ __launch_bounds__(256) void k_stage(const DevBlock* __restrict__ bloc
 #pragma unroll
         for (int v = 0; v < 4; ++v) phxn[v] = shdn(phx[v]);
         FT Fx[4];
-        if (xdo) {
-          if (face_flux<FT, RIE>(gf, qA, phx, (FT)(0.5 * dlx), qn, phxn, (FT)(0.5 * dlxn), (FT)B.fix[oA],
-                                 (FT)B.fiy[oA], (FT)B.fil[oA], Fx))
+        if (xdo) {                                     // interior i-face (x|x+1) — Mod_Fluxes.f90:140-185
+          if (face_flux<FT, RIE>(gf, qA, phx, (FT)(0.5 * dlx), qn, phxn, (FT)(0.5 * dlxn), (FT)gcur.fix,
+                                 (FT)gcur.fiy, (FT)gcur.fil, Fx))
             record_fail(&K, Q2D_E_INTERFACE, fail_key(stage_id, 0, B.blk_in_case, r, x));
         } else {
 #pragma unroll
@@ -650,6 +746,7 @@ __global__ __launch_bounds__(256) void k_stage(const DevBlock* __restrict__ bloc
         for (int v = 0; v < 4; ++v) Fl[v] = shup(Fx[v]);
 
         if (own) {
+          const long long oA = at(r);
           // ---- residual in the reference's accumulation order ------------
           double res[4];
 #pragma unroll
@@ -678,7 +775,7 @@ __global__ __launch_bounds__(256) void k_stage(const DevBlock* __restrict__ bloc
             }
           }
           // ---- geometric sources — Lib_Source_Axisymmetry.f90:40-78, Lib_Source_Areavariation.f90:104-144
-          const double sr = B.sur[oA];
+          const double sr = gcur.sur;
           const Gas<double> gd{C.R, C.cp, C.cv, C.gam};
           if (C.source != Q2D_SRC_NONE) {
             const double rho = qA[0], u = qA[1], v = qA[2], p = qA[3];
@@ -686,13 +783,13 @@ __global__ __launch_bounds__(256) void k_stage(const DevBlock* __restrict__ bloc
             const double E0 = gd.cv * (p / (rho * gd.R)) + 0.5 * mod_v * mod_v;
             double S[4];
             if (C.source == Q2D_SRC_AXI) {
-              const double invr = B.sb[oA];
+              const double invr = gp(B.sb)[oA];
               S[0] = -(rho * v * invr);
               S[1] = -(rho * u * v * invr);
               S[2] = -(rho * v * v * invr);
               S[3] = -((rho * E0 + p) * v * invr);
             } else {
-              const double VdG = u * B.sa[oA] + v * B.sb[oA];
+              const double VdG = u * gp(B.sa)[oA] + v * gp(B.sb)[oA];
               S[0] = rho * VdG;
               S[1] = rho * u * VdG;
               S[2] = rho * v * VdG;
@@ -702,19 +799,27 @@ __global__ __launch_bounds__(256) void k_stage(const DevBlock* __restrict__ bloc
             for (int k = 0; k < 4; ++k) res[k] = res[k] - S[k] * sr;
           }
           // ---- Compute_Cons_NoIRS — Lib_Newstate.f90:131-159 -------------
-          const double dt = C.time_accurate ? K.dtglobal : B.dtl[oA];
-          double po[4];
-#pragma unroll
-          for (int v = 0; v < 4; ++v) po[v] = PO[v * pl + oA];
+          const double dt = tacc ? dtg : gcur.dtl;
           double Uo[4], U[4], qn2[4];
-          prim2cons(gd, po[0], po[1], po[2], po[3], Uo);
+          prim2cons(gd, gcur.po[0], gcur.po[1], gcur.po[2], gcur.po[3], Uo);
           prim2cons(gd, qA[0], qA[1], qA[2], qA[3], U);
+          if (sizeof(FT) == 8) {
 #pragma unroll
-          for (int v = 0; v < 4; ++v) {
-            const double rr = -res[v] / sr * dt;
-            U[v] = Uo[v] + (U[v] - Uo[v] + rr) * c_rk;     // RK_stage — Lib_RK.f90:15-27
+            for (int v = 0; v < 4; ++v) {
+              const double rr = -res[v] / sr * dt;
+              U[v] = Uo[v] + (U[v] - Uo[v] + rr) * c_rk;   // RK_stage — Lib_RK.f90:15-27
+            }
+            cons2prim(gd, U, qn2);
+          } else {
+            // mixed form: FP64 update with one reciprocal of sur and of rho
+            const double dts = dt * rcp64(sr);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) U[v] = Uo[v] + (U[v] - Uo[v] - res[v] * dts) * c_rk;
+            const double ir = rcp64(U[0]);
+            const double vx = U[1] * ir, vy = U[2] * ir;
+            const double e = U[3] * ir - 0.5 * (vx * vx + vy * vy);
+            qn2[0] = U[0]; qn2[1] = vx; qn2[2] = vy; qn2[3] = U[0] * (gd.gam - 1.0) * e;
           }
-          cons2prim(gd, U, qn2);
           // Check_And_Fix_State — Lib_Newstate.f90:316-350
           if (__builtin_isnan(qn2[0]) || __builtin_isnan(qn2[1]) || __builtin_isnan(qn2[2]) ||
               __builtin_isnan(qn2[3]) || qn2[3] < 0.0)
@@ -724,7 +829,36 @@ __global__ __launch_bounds__(256) void k_stage(const DevBlock* __restrict__ bloc
           for (int v = 0; v < 4; ++v) Pout[v * pl + oA] = qn2[v];
           if (last) {                                        // Compute_Residual — Mod_Diagnostic.f90:452-458
 #pragma unroll
-            for (int v = 0; v < 4; ++v) { const double d = fabs(qn2[v] - po[v]); acc[v] = acc[v] + d * d; }
+            for (int v = 0; v < 4; ++v) { const double d = fabs(qn2[v] - gcur.po[v]); acc[v] = acc[v] + d * d; }
+          }
+          if (last && FUSE) {
+            // Compute_dt of the next step (Mod_dt.f90:59-104) on the state just written; the
+            // next step's Compute_dt would read exactly these values.
+            const double snd = sqrt(C.gam * qn2[3] / qn2[0]);
+            double dtmin = 1e8;
+            double mm[4];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) mm[v] = m11p[v * pl + oA];   // c11, c21, c12, c22
+            {
+              const double m1 = mm[0], m2 = mm[2];
+              const double nrm = sqrt(m1 * m1 + m2 * m2);
+              const double v1 = m1 / nrm, v2 = m2 / nrm;
+              const double lambda = fabs(qn2[1] * v1 + qn2[2] * v2) + snd;
+              const double dtd = dlx / lambda * C.cfl;
+              if (dtd < dtmin) dtmin = dtd;
+            }
+            if (C.ndir == 2) {
+              const double m1 = mm[1], m2 = mm[3];
+              const double nrm = sqrt(m1 * m1 + m2 * m2);
+              const double v1 = m1 / nrm, v2 = m2 / nrm;
+              const double lambda = fabs(qn2[1] * v1 + qn2[2] * v2) + snd;
+              const double dtd = dlyA / lambda * C.cfl;
+              if (dtd < dtmin) dtmin = dtd;
+            }
+            const int iter_next = K.iter + 1;
+            if (iter_next < C.ramp_iter) dtmin = dtmin * (double)iter_next / (double)C.ramp_iter;
+            if (!tacc) gpw(B.dtl[K.dtsel ^ 1])[oA] = dtmin;
+            if (dtmin < dtnext) dtnext = dtmin;
           }
         }
       }
@@ -744,6 +878,14 @@ __global__ __launch_bounds__(256) void k_stage(const DevBlock* __restrict__ bloc
         for (int k = 32; k >= 1; k >>= 1) s = s + __shfl_xor(s, k, kWave);
         if (lane == 0) partials[(long long)it * 4 + v] = s;
       }
+    }
+    if (last && FUSE) {
+      // next step's global dt: exact min, order independent
+      for (int k = 32; k >= 1; k >>= 1) {
+        const double o = __shfl_xor(dtnext, k, kWave);
+        if (o < dtnext) dtnext = o;
+      }
+      if (lane == 0 && dtnext < 1e300) atomicMin(&K.dtmin_next_bits, (unsigned long long)__double_as_longlong(dtnext));
     }
   }
 }
@@ -775,6 +917,14 @@ __global__ __launch_bounds__(256) void k_finish(DevCtrl* ctrl, const DevCase* ca
     K.stage_count += 4;
     if (endsim || K.status != 0) K.active = 0;
   }
+}
+
+// ---------------------------------------------------------------------------
+// HBM reference copy (bandwidth and counter calibration): dst = src, 16 B per lane per access.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_hbm_copy(const double2* __restrict__ src, double2* __restrict__ dst, long long n) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -820,6 +970,7 @@ struct q2d_handle {
   hipGraphExec_t graph = nullptr;
   int rows_per_item = 16, waves_per_group = 4, max_groups = 0, xcd_remap = 1;
   int precision_mixed = 0;
+  bool dt_fused_valid = false;   // the last stage of the previous step reduced this step's dt
   bool timing = false;
   long long timed_launches = 0;
   double timed_ms = 0.0;
@@ -856,7 +1007,7 @@ int build_items(q2d_handle* h) {
   return Q2D_OK;
 }
 
-template <typename FT>
+template <typename FT, bool FUSE>
 void launch_stage(q2d_handle* h, int irk) {
   const int tpb = kWave * h->waves_per_group;
   const int ranges[2][2] = {{0, h->n_hllc_items}, {h->n_hllc_items, (int)h->items.size()}};
@@ -867,10 +1018,10 @@ void launch_stage(q2d_handle* h, int irk) {
     if (h->max_groups > 0) groups = std::min(groups, h->max_groups);
     groups = std::max(groups, 1);
     if (rie == 0)
-      hipLaunchKernelGGL((k_stage<FT, 0>), dim3(groups), dim3(tpb), 0, h->stream, h->d_blocks, h->d_cases, h->d_ctrl,
+      hipLaunchKernelGGL((k_stage<FT, 0, FUSE>), dim3(groups), dim3(tpb), 0, h->stream, h->d_blocks, h->d_cases, h->d_ctrl,
                          h->d_items, lo, hi, h->d_bcface, h->d_bcflux, h->d_partials, irk, h->xcd_remap);
     else
-      hipLaunchKernelGGL((k_stage<FT, 1>), dim3(groups), dim3(tpb), 0, h->stream, h->d_blocks, h->d_cases, h->d_ctrl,
+      hipLaunchKernelGGL((k_stage<FT, 1, FUSE>), dim3(groups), dim3(tpb), 0, h->stream, h->d_blocks, h->d_cases, h->d_ctrl,
                          h->d_items, lo, hi, h->d_bcface, h->d_bcflux, h->d_partials, irk, h->xcd_remap);
   }
 }
@@ -883,13 +1034,14 @@ int launch_step(q2d_handle* h) {
   groups = std::max(groups, 1);
   const int cthreads = 64, cgroups = (h->ncases + cthreads - 1) / cthreads;
   const int bgroups = (h->nbc + 255) / 256;
-  hipLaunchKernelGGL(k_begin, dim3(cgroups), dim3(cthreads), 0, h->stream, h->d_ctrl, h->ncases);
-  hipLaunchKernelGGL(k_dt, dim3(groups), dim3(tpb), 0, h->stream, h->d_blocks, h->d_cases, h->d_ctrl, h->d_items, n_items);
+  const int use_next = (h->dt_fused_valid && (h->precision_mixed ? kFuseDtMixed : kFuseDtFp64)) ? 1 : 0;
+  hipLaunchKernelGGL(k_begin, dim3(cgroups), dim3(cthreads), 0, h->stream, h->d_ctrl, h->ncases, use_next);
+  if (!use_next)
+    hipLaunchKernelGGL(k_dt, dim3(groups), dim3(tpb), 0, h->stream, h->d_blocks, h->d_cases, h->d_ctrl, h->d_items,
+                       n_items);
   hipLaunchKernelGGL(k_dt_finish, dim3(cgroups), dim3(cthreads), 0, h->stream, h->d_ctrl, h->d_cases, h->ncases);
   for (int irk = 1; irk <= h->max_rk; ++irk) {
     if (h->nbc > 0) {
-      hipLaunchKernelGGL(k_ghost, dim3(bgroups), dim3(256), 0, h->stream, h->d_blocks, h->d_cases, h->d_ctrl, h->d_bc,
-                         h->nbc, irk);
       if (h->precision_mixed)
         hipLaunchKernelGGL(k_bcflux<float>, dim3(bgroups), dim3(256), 0, h->stream, h->d_blocks, h->d_cases, h->d_ctrl,
                            h->d_bc, h->nbc, irk, h->d_bcflux);
@@ -898,8 +1050,8 @@ int launch_step(q2d_handle* h) {
                            h->d_bc, h->nbc, irk, h->d_bcflux);
     }
     if (h->timing) HIPCHK(hipEventRecord(h->ev0, h->stream));
-    if (h->precision_mixed) launch_stage<float>(h, irk);
-    else launch_stage<double>(h, irk);
+    if (h->precision_mixed) launch_stage<float, kFuseDtMixed>(h, irk);
+    else launch_stage<double, kFuseDtFp64>(h, irk);
     if (h->timing) {
       HIPCHK(hipEventRecord(h->ev1, h->stream));
       HIPCHK(hipEventSynchronize(h->ev1));
@@ -911,6 +1063,7 @@ int launch_step(q2d_handle* h) {
   }
   hipLaunchKernelGGL(k_finish, dim3(h->ncases), dim3(256), 0, h->stream, h->d_ctrl, h->d_cases, h->d_partials, h->ncases);
   HIPCHK(hipGetLastError());
+  h->dt_fused_valid = true;
   return Q2D_OK;
 }
 
@@ -1107,7 +1260,7 @@ int q2d_gpu_create(const q2d_problem* prob, q2d_handle** out) {
     db.m11 = a + 15 * pl; db.m21 = a + 16 * pl; db.m12 = a + 17 * pl; db.m22 = a + 18 * pl;
     db.fix = a + 19 * pl; db.fiy = a + 20 * pl; db.fil = a + 21 * pl;
     db.fjx = a + 22 * pl; db.fjy = a + 23 * pl; db.fjl = a + 24 * pl;
-    db.dtl = a + 25 * pl; db.sa = a + 26 * pl; db.sb = a + 27 * pl;
+    db.dtl[0] = a + 25 * pl; db.dtl[1] = a + 28 * pl; db.sa = a + 26 * pl; db.sb = a + 27 * pl;
     for (int f = 0; f < 4; ++f) db.bcf[f] = -1;
   }
 
@@ -1228,7 +1381,12 @@ int q2d_gpu_run(q2d_handle* h, int32_t nsteps, q2d_step_out* out) {
   if (h->timing) {                                     // per-launch events cannot live inside a graph
     for (int s = 0; s < nsteps; ++s) { int rc = launch_step(h); if (rc != Q2D_OK) return rc; }
   } else if (nsteps > 0) {
-    if (!h->graph) {
+    if (!h->dt_fused_valid) {                          // first step after create/upload: eager, with k_dt
+      int rc = launch_step(h);
+      if (rc != Q2D_OK) return rc;
+      --nsteps;
+    }
+    if (!h->graph && nsteps > 0) {
       hipGraph_t g;
       HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
       int rc = launch_step(h);
@@ -1271,6 +1429,7 @@ int q2d_gpu_upload_state(q2d_handle* h, int32_t block, const double* P) {
   std::vector<double> tmp;
   pack_state(b, P, tmp);
   HIPCHK(hipMemcpy(b.arena + (size_t)4 * k.cur * b.plane, tmp.data(), sizeof(double) * tmp.size(), hipMemcpyHostToDevice));
+  h->dt_fused_valid = false;
   return Q2D_OK;
 }
 
@@ -1285,7 +1444,7 @@ int q2d_gpu_download_dt(q2d_handle* h, int32_t block, double* dtlocal) {
     return Q2D_OK;
   }
   std::vector<double> tmp((size_t)b.plane);
-  HIPCHK(hipMemcpy(tmp.data(), b.arena + (size_t)25 * b.plane, sizeof(double) * tmp.size(), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(tmp.data(), b.arena + (size_t)(k.dtsel ? 28 : 25) * b.plane, sizeof(double) * tmp.size(), hipMemcpyDeviceToHost));
   for (int j = 1; j <= b.nj; ++j)
     for (int i = 1; i <= b.ni; ++i) dtlocal[(size_t)(j - 1) * b.ni + (i - 1)] = tmp[(size_t)(j + 1) * b.pitch + (i + 1)];
   return Q2D_OK;
@@ -1317,6 +1476,30 @@ int q2d_gpu_set_tiling(q2d_handle* h, int32_t rows_per_item, int32_t waves_per_g
 int q2d_gpu_get_tiling(q2d_handle* h, int32_t* rows_per_item, int32_t* waves_per_group) {
   if (!h || !rows_per_item || !waves_per_group) return fail(Q2D_E_ARG, "NULL argument");
   *rows_per_item = h->rows_per_item; *waves_per_group = h->waves_per_group;
+  return Q2D_OK;
+}
+
+int q2d_hbm_copy(int64_t nbytes, int32_t reps, double* ms_per_copy) {
+  if (nbytes < 16 || reps < 1 || !ms_per_copy) return fail(Q2D_E_ARG, "bad argument");
+  const long long n = nbytes / 16;
+  double2 *a = nullptr, *b = nullptr;
+  hipEvent_t e0, e1;
+  HIPCHK(hipMalloc(&a, n * 16));
+  if (hipMalloc(&b, n * 16) != hipSuccess) { (void)hipFree(a); return fail(Q2D_E_NOMEM, "copy buffer"); }
+  HIPCHK(hipMemset(a, 0, n * 16));
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  int grid = 256 * 32;
+  hipLaunchKernelGGL(k_hbm_copy, dim3(grid), dim3(256), 0, 0, a, b, n);   // warm-up
+  HIPCHK(hipEventRecord(e0, 0));
+  for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(k_hbm_copy, dim3(grid), dim3(256), 0, 0, a, b, n);
+  HIPCHK(hipEventRecord(e1, 0));
+  HIPCHK(hipEventSynchronize(e1));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  *ms_per_copy = ms / reps;
+  (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
+  (void)hipFree(a); (void)hipFree(b);
   return Q2D_OK;
 }
 
