@@ -36,11 +36,20 @@
 /* ------------------------------------------------------------------------ */
 /* gas closure                                                               */
 /* ------------------------------------------------------------------------ */
-typedef struct { double R, cp, cv, gam; } gas_t;
+typedef struct { double R, cp, cv, gam, mu_ref, T_ref, S, Pr; } gas_t;
 
 static gas_t mkgas(double R, double cp) {
-  gas_t g; g.R = R; g.cp = cp; g.cv = cp - R; g.gam = cp / g.cv; return g;
+  gas_t g; g.R = R; g.cp = cp; g.cv = cp - R; g.gam = cp / g.cv;
+  g.mu_ref = 0.0; g.T_ref = 1.0; g.S = 0.0; g.Pr = 0.72; return g;
 }
+/* Laminar transport (FLINT f_laminarViscosity / co_k_mi_lam_Wilke, closed form):
+ * Sutherland mu = mu_ref * (T/T_ref)^1.5 * (T_ref + S) / (T + S), with (T/T_ref)^1.5 = x*sqrt(x);
+ * k = mu * cp / Pr. */
+static double mu_lam(const gas_t *g, double T) {
+  double x = T / g->T_ref;
+  return g->mu_ref * (x * sqrt(x)) * (g->T_ref + g->S) / (T + g->S);
+}
+static double k_lam(const gas_t *g, double mu) { return mu * g->cp / g->Pr; }
 static double f_ss(const gas_t *g, double p, double rho) { return sqrt(g->gam * p / rho); }
 static double eos_T(const gas_t *g, double p, double rho) { return p / (rho * g->R); }
 static double f_H0(const gas_t *g, double p, double rho, double vmag) {
@@ -361,6 +370,8 @@ q2do *q2do_create(const q2d_problem *prob) {
     ocase *cs = &o->cs[c];
     cs->d = prob->cases[c];
     cs->gas = mkgas(cs->d.scheme.gas_R, cs->d.scheme.gas_cp);
+    cs->gas.mu_ref = cs->d.scheme.mu_ref; cs->gas.T_ref = cs->d.scheme.T_ref;
+    cs->gas.S = cs->d.scheme.S_suth; cs->gas.Pr = cs->d.scheme.Pr;
     cs->first_block = -1; cs->first_bc = -1;
     cs->dtglobal = 0.0; cs->time = 0.0;
   }
@@ -430,7 +441,15 @@ static void compute_dt(q2do *o, ocase *cs) {
           double dt = DL(b, d - 1, i, j) / lambda * s->cfl;
           if (dt < dtmin) dtmin = dt;
         }
-        /* model>0: the VNN condition of Mod_dt.f90:88-102 is not part of this oracle */
+        if (s->model > 0) {                            /* VNN condition :88-102 (laminar: mie = 0) */
+          double mil = mu_lam(&cs->gas, eos_T(&cs->gas, PV(b, 3, i, j), rho));
+          double mi = 0.0 + mil;
+          for (int d = 1; d <= cs->ndir; d++) {
+            double dl = DL(b, d - 1, i, j);
+            double dt = rho * (dl * dl) * s->vnn / mi;
+            if (dt < dtmin) dtmin = dt;
+          }
+        }
         double dtcell = dtmin;
         if (cs->iter < s->ramp_iter) dtcell = dtcell * cs->iter / s->ramp_iter;
         b->dtlocal[IN(b, i, j)] = dtcell;
@@ -543,7 +562,83 @@ static int convective_flux(int lim, int rid, const gas_t *g, const double *dl, d
   return 0;
 }
 
-/* Fluxes_blk_omp_do — Mod_Fluxes.f90:97-234 (Euler part) */
+/* ------------------------------------------------------------------------ */
+/* Laminar viscous terms — Lib_Diffusive.f90, Lib_Fluid.f90 (laminar, mie = 0) */
+/* ------------------------------------------------------------------------ */
+static const double negativetwothird = -2.0 / 3.0;   /* Lib_Fluid.f90:21 */
+
+/* Stress_Vector_Std (:73-99, V_dot_G = 0) / Stress_Vector_Axi (:102-125). gv[a][b] = Gradvel(a+1,b+1). */
+static void stress_vector(int axi, const double gv[2][2], double n1, double n2, double mil, double r_ave, double vr,
+                          double *S1, double *S2) {
+  double mi = mil + 0.0;
+  double lam = negativetwothird * mi;
+  double Divervel;
+  if (axi) {
+    if (r_ave <= 1.0e-6) Divervel = gv[0][0] + gv[1][1] + gv[1][1];
+    else Divervel = gv[0][0] + gv[1][1] + vr / r_ave;
+  } else {
+    Divervel = gv[0][0] + gv[1][1] - 0.0;
+  }
+  double lamDivervel = lam * Divervel;
+  double Tau_xx = lamDivervel + 2.0 * mi * gv[0][0];
+  double Tau_yy = lamDivervel + 2.0 * mi * gv[1][1];
+  double Tau_xy = mi * (gv[0][1] + gv[1][0]);
+  *S1 = Tau_xx * n1 + Tau_xy * n2;
+  *S2 = Tau_xy * n1 + Tau_yy * n2;
+}
+
+/* Gradient(k,:) <- Gradient(k,:) . M   (matmul(Gradient, M), M(l,c) = MM(l,c)) */
+static void grad_times_M(double g[2], const double M[4]) {
+  double a = g[0] * M[0] + g[1] * M[1];           /* M(1,1), M(2,1) */
+  double c = g[0] * M[2] + g[1] * M[3];           /* M(1,2), M(2,2) */
+  g[0] = a; g[1] = c;
+}
+
+/* Diffusive_Flux + Tangential_Gradient + Compute_Diffusive_Flux — Lib_Diffusive.f90:10-199.
+ * P1,P2: the two cells of the face; P3..P6 the tangential stencil; a = direction of the face normal (0/1). */
+static void diffusive_flux(const ocase *cs, double n1, double n2, double int_l, const double *P1, const double *P2,
+                           const double *P3, const double *P4, const double *P5, const double *P6,
+                           const double *M1, const double *M2, double r2, double r1, int a, double *Flux) {
+  const gas_t *g = &cs->gas;
+  int axi = cs->d.scheme.source == Q2D_SRC_AXI;
+  int b = 1 - a;
+  double r_ave = 0.5 * (r1 + r2);
+  double rho1 = P1[0], T1 = P1[3] / (rho1 * g->R);
+  double rho2 = P2[0], T2 = P2[3] / (rho2 * g->R);
+  double Gr[4][2];                                   /* Gradient(nprim,2) */
+  Gr[0][a] = P2[0] / rho2 - P1[0] / rho1;
+  Gr[3][a] = T2 - T1;
+  Gr[1][a] = P2[1] - P1[1];
+  Gr[2][a] = P2[2] - P1[2];
+  {                                                  /* Tangential_Gradient :80-110 */
+    double r3 = P3[0], r4 = P4[0], r5 = P5[0], r6 = P6[0];
+    double T3 = P3[3] / (r3 * g->R), T4 = P4[3] / (r4 * g->R), T5 = P5[3] / (r5 * g->R), T6 = P6[3] / (r6 * g->R);
+    Gr[0][b] = (P4[0] / r4 - P3[0] / r3 + P6[0] / r6 - P5[0] / r5) * 0.25;
+    Gr[3][b] = (T4 - T3 + T6 - T5) * 0.25;
+    Gr[1][b] = (P4[1] - P3[1] + P6[1] - P5[1]) * 0.25;
+    Gr[2][b] = (P4[2] - P3[2] + P6[2] - P5[2]) * 0.25;
+  }
+  double M[4];
+  for (int k = 0; k < 4; k++) M[k] = 0.5 * (M1[k] + M2[k]);
+  for (int k = 0; k < 4; k++) grad_times_M(Gr[k], M);
+  double Pm[4];
+  for (int k = 0; k < 4; k++) Pm[k] = 0.5 * (P1[k] + P2[k]);
+  /* Compute_Diffusive_Flux :113-199 */
+  double rho = Pm[0];
+  double T = Pm[3] / (rho * g->R);
+  double mil = mu_lam(g, T), kl = k_lam(g, mil);
+  double kappa = kl + 0.0;
+  double gv[2][2] = {{Gr[1][0], Gr[1][1]}, {Gr[2][0], Gr[2][1]}};
+  double S1, S2;
+  stress_vector(axi, gv, n1, n2, mil, r_ave, Pm[2], &S1, &S2);
+  /* one species: Dm*grad(Y) = 0, so the species flux and the enthalpy diffusion terms vanish */
+  Flux[0] = 0.0;
+  Flux[1] = int_l * S1;
+  Flux[2] = int_l * S2;
+  Flux[3] = int_l * ((S1 * Pm[1] + S2 * Pm[2]) + kappa * (Gr[3][0] * n1 + Gr[3][1] * n2) + 0.0);
+}
+
+/* Fluxes_blk_omp_do — Mod_Fluxes.f90:97-234 (convective + laminar diffusive) */
 static int fluxes_block(oblk *b, const ocase *cs) {
   const q2d_scheme *s = &cs->d.scheme;
   int err = 0;
@@ -564,6 +659,13 @@ static int fluxes_block(oblk *b, const ocase *cs) {
       err |= -convective_flux(s->limiter, s->riemann, &cs->gas, dl, FI(b, 0, i, j), FI(b, 1, i, j),
                               FI(b, 2, i, j), (const double(*)[NP])prim, F);
       for (int v = 0; v < NP; v++) { RV(b, v, i, j) = RV(b, v, i, j) + F[v]; RV(b, v, i + 1, j) = RV(b, v, i + 1, j) - F[v]; }
+      if (s->model > 0) {
+        double D[NP];
+        diffusive_flux(cs, FI(b, 0, i, j), FI(b, 1, i, j), FI(b, 2, i, j), &PV(b, 0, i, j), &PV(b, 0, i + 1, j),
+                       &PV(b, 0, i, j - 1), &PV(b, 0, i, j + 1), &PV(b, 0, i + 1, j - 1), &PV(b, 0, i + 1, j + 1),
+                       &MM(b, 1, 1, i, j), &MM(b, 1, 1, i + 1, j), NODE(b, 1, i, j), NODE(b, 1, i, j - 1), 0, D);
+        for (int v = 0; v < NP; v++) { RV(b, v, i, j) = RV(b, v, i, j) - D[v]; RV(b, v, i + 1, j) = RV(b, v, i + 1, j) + D[v]; }
+      }
     }
   }
   /* j-faces: columns in parallel (:187-232) */
@@ -580,6 +682,13 @@ static int fluxes_block(oblk *b, const ocase *cs) {
       err |= -convective_flux(s->limiter, s->riemann, &cs->gas, dl, FJ(b, 0, i, j), FJ(b, 1, i, j),
                               FJ(b, 2, i, j), (const double(*)[NP])prim, F);
       for (int v = 0; v < NP; v++) { RV(b, v, i, j) = RV(b, v, i, j) + F[v]; RV(b, v, i, j + 1) = RV(b, v, i, j + 1) - F[v]; }
+      if (s->model > 0) {
+        double D[NP];
+        diffusive_flux(cs, FJ(b, 0, i, j), FJ(b, 1, i, j), FJ(b, 2, i, j), &PV(b, 0, i, j), &PV(b, 0, i, j + 1),
+                       &PV(b, 0, i - 1, j), &PV(b, 0, i + 1, j), &PV(b, 0, i - 1, j + 1), &PV(b, 0, i + 1, j + 1),
+                       &MM(b, 1, 1, i, j), &MM(b, 1, 1, i, j + 1), NODE(b, 1, i, j), NODE(b, 1, i - 1, j), 1, D);
+        for (int v = 0; v < NP; v++) { RV(b, v, i, j) = RV(b, v, i, j) - D[v]; RV(b, v, i, j + 1) = RV(b, v, i, j + 1) + D[v]; }
+      }
     }
   }
   return err ? Q2D_E_INTERFACE : 0;
@@ -805,6 +914,91 @@ static void bc_outflow(oblk *b, const ocase *cs, const obc *rec, int Im, int Jm,
   for (int v = 0; v < NP; v++) RV(b, v, Im, Jm) = RV(b, v, Im, Jm) + (double)modfm2 * Flux[v];
 }
 
+/* r_ave of a boundary face from the nodal radii (Lib_BC_Fluxes_Symmetry.f90:183-192) */
+static double face_r_ave(const oblk *b, int Im, int Jm, int Fm) {
+  switch (Fm) {
+    case 1: return 0.5 * (NODE(b, 1, Im - 1, Jm) + NODE(b, 1, Im - 1, Jm - 1));
+    case 2: return 0.5 * (NODE(b, 1, Im, Jm) + NODE(b, 1, Im, Jm - 1));
+    case 3: return 0.5 * (NODE(b, 1, Im, Jm - 1) + NODE(b, 1, Im - 1, Jm - 1));
+    default: return 0.5 * (NODE(b, 1, Im, Jm) + NODE(b, 1, Im - 1, Jm));
+  }
+}
+
+/* BC_Symmetry_Visc — Lib_BC_Fluxes_Symmetry.f90:127-236 (laminar) */
+static void bc_symmetry_visc(oblk *b, const ocase *cs, int Im, int Jm, int Fm) {
+  const gas_t *g = &cs->gas;
+  int axi = cs->d.scheme.source == Q2D_SRC_AXI;
+  int modfm2 = 1 - 2 * (Fm % 2);
+  int dir; double n1, n2, int_l;
+  face_geom(b, Fm, Im, Jm, &dir, &n1, &n2, &int_l);
+  const double *M = &MM(b, 1, 1, Im, Jm);
+  double rho = PV(b, 0, Im, Jm);
+  double mil = mu_lam(g, eos_T(g, PV(b, 3, Im, Jm), rho));
+  double V1[2], V3[2], V4[2];
+  int a;
+  V1[0] = PV(b, 1, Im, Jm); V1[1] = PV(b, 2, Im, Jm);
+  if (Fm <= 2) {
+    V3[0] = PV(b, 1, Im, Jm - 1); V3[1] = PV(b, 2, Im, Jm - 1);
+    V4[0] = PV(b, 1, Im, Jm + 1); V4[1] = PV(b, 2, Im, Jm + 1);
+    a = 0;
+  } else {
+    V3[0] = PV(b, 1, Im - 1, Jm); V3[1] = PV(b, 2, Im - 1, Jm);
+    V4[0] = PV(b, 1, Im + 1, Jm); V4[1] = PV(b, 2, Im + 1, Jm);
+    a = 1;
+  }
+  int bb = 1 - a;
+  double d1 = V1[0] * n1 + V1[1] * n2, d3 = V3[0] * n1 + V3[1] * n2, d4 = V4[0] * n1 + V4[1] * n2;
+  double V2[2] = {V1[0] - 2.0 * d1 * n1, V1[1] - 2.0 * d1 * n2};
+  double V5[2] = {V3[0] - 2.0 * d3 * n1, V3[1] - 2.0 * d3 * n2};
+  double V6[2] = {V4[0] - 2.0 * d4 * n1, V4[1] - 2.0 * d4 * n2};
+  double Gr[2][2];
+  for (int k = 0; k < 2; k++) {
+    Gr[k][a] = (V2[k] - V1[k]) * (double)modfm2;
+    Gr[k][bb] = (V4[k] - V3[k] + V6[k] - V5[k]) * 0.25;
+  }
+  grad_times_M(Gr[0], M);
+  grad_times_M(Gr[1], M);
+  double r_ave = face_r_ave(b, Im, Jm, Fm);
+  double Vm[2] = {0.5 * (V1[0] + V2[0]), 0.5 * (V1[1] + V2[1])};
+  double S1, S2;
+  (void)axi;
+  stress_vector(axi, (const double(*)[2])Gr, n1, n2, mil, r_ave, 0.5 * (PV(b, 2, Im, Jm) + V2[1]), &S1, &S2);
+  double F1 = int_l * S1, F2 = int_l * S2;
+  double F3 = int_l * (Vm[0] * S1 + Vm[1] * S2);
+  RV(b, 1, Im, Jm) = RV(b, 1, Im, Jm) - (double)modfm2 * F1;
+  RV(b, 2, Im, Jm) = RV(b, 2, Im, Jm) - (double)modfm2 * F2;
+  RV(b, 3, Im, Jm) = RV(b, 3, Im, Jm) - (double)modfm2 * F3;
+}
+
+/* BC_Wall_Temperature — Lib_BC_Fluxes_Wall_Temperature.f90:20-106 (laminar, no frame rotation) */
+static void bc_wall_temperature(oblk *b, const ocase *cs, int Im, int Jm, int Fm, double T_wall) {
+  const gas_t *g = &cs->gas;
+  int axi = cs->d.scheme.source == Q2D_SRC_AXI;
+  int modfm = Fm % 2, modfm2 = 1 - 2 * modfm, modfm3 = 2 * (2 * modfm - 1);
+  int dir; double n1, n2, int_l;
+  face_geom(b, Fm, Im, Jm, &dir, &n1, &n2, &int_l);
+  const double *M = &MM(b, 1, 1, Im, Jm);
+  double rho = PV(b, 0, Im, Jm), p = PV(b, 3, Im, Jm);
+  double T = p / (rho * g->R);
+  double mil = mu_lam(g, T_wall), kl = k_lam(g, mil);
+  double Gr[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+  int d = dir - 1;
+  Gr[1][d] = PV(b, 1, Im, Jm) * (double)modfm3;
+  Gr[2][d] = PV(b, 2, Im, Jm) * (double)modfm3;
+  Gr[3][d] = (T - T_wall) * (double)modfm3;
+  for (int k = 0; k < 4; k++) grad_times_M(Gr[k], M);
+  double r_ave = face_r_ave(b, Im, Jm, Fm);
+  double gv[2][2] = {{Gr[1][0], Gr[1][1]}, {Gr[2][0], Gr[2][1]}};
+  double S1, S2;
+  stress_vector(axi, gv, n1, n2, mil, r_ave, 0.0, &S1, &S2);
+  double Flux[4];
+  Flux[0] = 0.0;
+  Flux[1] = S1 * int_l;
+  Flux[2] = S2 * int_l;
+  Flux[3] = int_l * kl * (Gr[3][0] * n1 + Gr[3][1] * n2);
+  for (int v = 0; v < 4; v++) RV(b, v, Im, Jm) = RV(b, v, Im, Jm) - (double)modfm2 * Flux[v];
+}
+
 static int bc_fluxes(q2do *o, ocase *cs) {
   int err = 0;
   for (int k = 0; k < cs->nbc; k++) {
@@ -813,8 +1007,14 @@ static int bc_fluxes(q2do *o, ocase *cs) {
     oblk *b = &o->blk[cs->first_block + e->b - 1];
     switch (e->type) {
       case 101: case 201: { int r = bc_connection_eul(b, cs, e->i, e->j, e->f); if (r) err = r; break; }
-      case 300: bc_symmetry_eul(b, cs, e->i, e->j, e->f); break;
-      case 302: bc_symmetry_eul(b, cs, e->i, e->j, e->f); break;   /* Euler part; viscous wall not in this oracle */
+      case 300:
+        bc_symmetry_eul(b, cs, e->i, e->j, e->f);
+        if (cs->d.scheme.model > 0) bc_symmetry_visc(b, cs, e->i, e->j, e->f);
+        break;
+      case 302:
+        bc_symmetry_eul(b, cs, e->i, e->j, e->f);
+        if (cs->d.scheme.model > 0) bc_wall_temperature(b, cs, e->i, e->j, e->f, e->T);
+        break;
       case 400: bc_extrapolation(b, cs, e->i, e->j, e->f); break;
       case 405: bc_supersonic_inflow(b, cs, rec, e->i, e->j, e->f); break;
       case 406: bc_outflow(b, cs, rec, e->i, e->j, e->f); break;
@@ -827,6 +1027,70 @@ static int bc_fluxes(q2do *o, ocase *cs) {
 /* ------------------------------------------------------------------------ */
 /* Sources — Lib_Source_Axisymmetry.f90:40-78, Lib_Source_Areavariation.f90:104-144 */
 /* ------------------------------------------------------------------------ */
+/* Ghost_Wall_Extrapolation — Lib_Ghost.f90:366-399 (wall codes 301..304) */
+static void ghost_wall_extrapolation(q2do *o, ocase *cs) {
+  for (int k = 0; k < cs->nbc; k++) {
+    const q2d_bc_entry *e = &o->bc[cs->first_bc + k].e;
+    if (e->type < 301 || e->type > 304) continue;
+    oblk *b = &o->blk[cs->first_block + e->b - 1];
+    int ig = e->i - guide[e->f][0], jg = e->j - guide[e->f][1];
+    PV(b, 1, ig, jg) = -PV(b, 1, e->i, e->j);
+    PV(b, 2, ig, jg) = -PV(b, 2, e->i, e->j);
+    PV(b, 0, ig, jg) = PV(b, 0, e->i, e->j);
+    PV(b, 3, ig, jg) = PV(b, 3, e->i, e->j);
+  }
+}
+
+/* Source_Axi_Visc_Blk — Lib_Source_Axisymmetry.f90:81-247 (laminar, one species) */
+static void source_axi_visc(oblk *b, const ocase *cs) {
+  const gas_t *g = &cs->gas;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+  for (int j = 1; j <= b->nj; j++) {
+    for (int i = 1; i <= b->ni; i++) {
+      double rho = PV(b, 0, i, j);
+      double rho1 = PV(b, 0, i + 1, j), rho2 = PV(b, 0, i - 1, j), rho3 = PV(b, 0, i, j + 1), rho4 = PV(b, 0, i, j - 1);
+      double r_ave = b->sa[IN(b, i, j)], invr = b->sb[IN(b, i, j)];
+      double T = PV(b, 3, i, j) / (rho * g->R);
+      double T1 = PV(b, 3, i + 1, j) / (rho1 * g->R);
+      double T2 = PV(b, 3, i - 1, j) / (rho2 * g->R);
+      double T3 = PV(b, 3, i, j + 1) / (rho3 * g->R);
+      double T4 = PV(b, 3, i, j - 1) / (rho4 * g->R);
+      double mil = mu_lam(g, T), kl = k_lam(g, mil);
+      double Gr[4][2];
+      Gr[0][0] = (PV(b, 0, i + 1, j) / rho1 - PV(b, 0, i - 1, j) / rho2) * 0.5;
+      Gr[0][1] = (PV(b, 0, i, j + 1) / rho3 - PV(b, 0, i, j - 1) / rho4) * 0.5;
+      Gr[1][0] = (PV(b, 1, i + 1, j) - PV(b, 1, i - 1, j)) * 0.5;
+      Gr[2][0] = (PV(b, 2, i + 1, j) - PV(b, 2, i - 1, j)) * 0.5;
+      Gr[1][1] = (PV(b, 1, i, j + 1) - PV(b, 1, i, j - 1)) * 0.5;
+      Gr[2][1] = (PV(b, 2, i, j + 1) - PV(b, 2, i, j - 1)) * 0.5;
+      Gr[3][0] = (T1 - T2) * 0.5;
+      Gr[3][1] = (T3 - T4) * 0.5;
+      const double *M = &MM(b, 1, 1, i, j);
+      for (int k = 0; k < 4; k++) grad_times_M(Gr[k], M);
+      double kappa = kl + 0.0;
+      /* Stress_Tensor_Axi — Lib_Fluid.f90:163-198 */
+      double vr = PV(b, 2, i, j);
+      double mi = mil + 0.0;
+      double lam = negativetwothird * mi;
+      double Divervel = Gr[1][0] + Gr[2][1] + vr / r_ave;
+      double lamDivervel = lam * Divervel;
+      double Tau_yy = lamDivervel + 2.0 * mi * Gr[2][1];
+      double Tau_xy = mi * (Gr[1][1] + Gr[2][0]);
+      double Tau_zz = 2.0 * mi * vr / r_ave + lamDivervel;
+      double qr = -kappa * Gr[3][1];
+      double S[4];
+      S[0] = 0.0;                                       /* one species: Dm*grad(Y) = 0 */
+      S[1] = Tau_xy * invr;
+      S[2] = (Tau_yy - Tau_zz) * invr;
+      S[3] = (-qr + PV(b, 1, i, j) * Tau_xy + PV(b, 2, i, j) * Tau_yy + 0.0) * invr;
+      double sr = SUR(b, i, j);
+      for (int k = 0; k < NP; k++) RV(b, k, i, j) = RV(b, k, i, j) - S[k] * sr;
+    }
+  }
+}
+
 static void sources(q2do *o, ocase *cs) {
   const q2d_scheme *s = &cs->d.scheme;
   if (s->source == Q2D_SRC_NONE) return;
@@ -859,6 +1123,7 @@ static void sources(q2do *o, ocase *cs) {
         for (int k = 0; k < NP; k++) RV(b, k, i, j) = RV(b, k, i, j) - S[k] * sr;
       }
     }
+    if (s->source == Q2D_SRC_AXI && s->model > 0) source_axi_visc(b, cs);
   }
 }
 
@@ -949,6 +1214,7 @@ static int step_case(q2do *o, ocase *cs, q2d_step_out *out) {
       if (r && !status) status = r;
     }
     { int r = bc_fluxes(o, cs); if (r && !status) status = r; }
+    if (s->source != Q2D_SRC_NONE) ghost_wall_extrapolation(o, cs);   /* Mod_Explicit.f90:83 */
     sources(o, cs);
     { int r = newstate(o, cs, irk, &fb, &fi, &fj); if (r && !status) status = r; }
   }

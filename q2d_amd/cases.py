@@ -11,6 +11,7 @@ Generators (all seeded, numpy Generator):
   planar_shock       angled planar shock into a perturbed gas, 2x2 blocks
   batch_uneven       many independent planar / quasi-2D cases of uneven size
   ramp_channel       supersonic compression ramp (steady, local dt)
+  axisymmetric_pipe  laminar axisymmetric Navier-Stokes, supersonic pipe inflow, isothermal wall
 """
 from __future__ import annotations
 
@@ -351,3 +352,65 @@ def ramp_channel(ni=128, nj=64, wedge_deg=10.0, mach=2.5, x_ramp=0.2, length=1.0
     pb = Problem(blocks=[Block(geo, P)], bc=recs, cases=[spec],
                  meta={"name": "ramp_channel", "wedge_deg": wedge_deg, "mach": mach, "res_drop": res_drop})
     return finish(pb, [geo])
+
+
+def _tanh_wall_cluster(nj, first):
+    """Radial node coordinates on [0, 1] clustered at eta = 1 by a tanh law, first cell (at eta = 1) of size
+    `first`:  y(eta) = tanh(beta eta) / tanh(beta), beta solved by bisection."""
+    def first_cell(beta):
+        return 1.0 - math.tanh(beta * (1.0 - 1.0 / nj)) / math.tanh(beta)
+    if first >= 1.0 / nj:
+        return np.arange(nj + 1) / nj
+    lo, hi = 1e-6, 50.0
+    for _ in range(200):
+        mid = 0.5 * (lo + hi)
+        if first_cell(mid) > first:
+            lo = mid
+        else:
+            hi = mid
+    beta = 0.5 * (lo + hi)
+    eta = np.arange(nj + 1) / nj
+    y = np.tanh(beta * eta) / math.tanh(beta)
+    y[0], y[-1] = 0.0, 1.0
+    return y
+
+
+def axisymmetric_pipe(seed=0, ni=2048, nj=512, first_cell=1e-5, reynolds=1e5, mach=None, radius=0.25,
+                      T_wall=None, precision="mixed", n_rk=3, limiter="vanleer", riemann="HLLC", cfl=0.5,
+                      vnn=0.3, steps=2000):
+    """Axisymmetric laminar Navier-Stokes: supersonic inflow (BC 405, face 1) into a pipe of length 1 and
+    radius `radius`; axis r = 0 on j = 0 (BC 300, face 3), isothermal no-slip wall on r = radius (BC 302,
+    face 4), supersonic outflow (BC 400, face 2).  Radial tanh clustering at the wall with a first cell of
+    `first_cell` (lengths in units of the pipe length L = 1).  Non-dimensional gas R = 1, cp = 3.5 at
+    rho = p = T = 1; Sutherland viscosity with S / T_ref = 110.4 / 288.15, Pr = 0.72, mu_inf from Re_L.
+    Mach ~ U[1.5, 3] from the seed unless given.  Local time step."""
+    rng = np.random.default_rng(seed)
+    if mach is None:
+        mach = float(rng.uniform(1.5, 3.0))
+    gam = 1.4
+    a = math.sqrt(gam)
+    U = mach * a
+    mu_inf = U * 1.0 / reynolds
+    if T_wall is None:
+        T_wall = 1.0 + 0.5 * (gam - 1.0) * mach * mach * 0.85      # near the laminar recovery temperature
+    xs = np.arange(ni + 1) / ni
+    rs = radius * _tanh_wall_cluster(nj, first_cell / radius)
+    node = np.empty((nj + 1, ni + 1, 2))
+    node[..., 0] = xs[None, :]
+    node[..., 1] = rs[:, None]
+    geo = BlockGeometry(node)
+    P = np.zeros((nj + 4, ni + 4, 4))
+    P[2:-2, 2:-2] = [1.0, U, 0.0, 1.0]
+    fill_initial_ghosts(P)
+    props = {1: {"mach": mach, "T": 1.0, "p": 1.0, "rel_fac": 1.0, "alpha": 0.0},
+             4: {"T": T_wall}}
+    recs = face_records(1, ni, nj, {1: 405, 2: 400, 3: 300, 4: 302}, props=props)
+    spec = CaseSpec(n_rk=n_rk, limiter=limiter, riemann=riemann, time_accurate=False, model=1,
+                    source="axisymmetric", precision=precision, cfl=cfl, vnn=vnn, mu_ref=mu_inf, T_ref=1.0,
+                    S_suth=110.4 / 288.15, Pr=0.72, itermax=steps)
+    blk = Block(geo, P)
+    pb = Problem(blocks=[blk], bc=recs, cases=[spec],
+                 meta={"name": "axisymmetric_pipe", "mach": mach, "reynolds": reynolds, "T_wall": T_wall})
+    finish(pb, [geo])
+    blk.src_a, blk.src_b = axisymmetric_geometry(geo)
+    return pb

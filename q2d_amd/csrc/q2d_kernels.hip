@@ -17,6 +17,7 @@
 // two faces that use it (the reference evaluates it twice with identical
 // operands, so the value is bit-identical).
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <algorithm>
 #include <cfloat>
@@ -1007,8 +1008,11 @@ int build_items(q2d_handle* h) {
   return Q2D_OK;
 }
 
+// With timing on, each stage launch carries start/stop events that the runtime stamps from the kernel's own
+// dispatch packet (hipExtLaunchKernelGGL), so the measured time is the kernel's execution alone, the same
+// interval a kernel-trace profiler reports.
 template <typename FT, bool FUSE>
-void launch_stage(q2d_handle* h, int irk) {
+int launch_stage(q2d_handle* h, int irk) {
   const int tpb = kWave * h->waves_per_group;
   const int ranges[2][2] = {{0, h->n_hllc_items}, {h->n_hllc_items, (int)h->items.size()}};
   for (int rie = 0; rie < 2; ++rie) {
@@ -1017,13 +1021,24 @@ void launch_stage(q2d_handle* h, int irk) {
     int groups = (hi - lo + h->waves_per_group - 1) / h->waves_per_group;
     if (h->max_groups > 0) groups = std::min(groups, h->max_groups);
     groups = std::max(groups, 1);
+    hipEvent_t e0 = h->timing ? h->ev0 : nullptr, e1 = h->timing ? h->ev1 : nullptr;
     if (rie == 0)
-      hipLaunchKernelGGL((k_stage<FT, 0, FUSE>), dim3(groups), dim3(tpb), 0, h->stream, h->d_blocks, h->d_cases, h->d_ctrl,
-                         h->d_items, lo, hi, h->d_bcface, h->d_bcflux, h->d_partials, irk, h->xcd_remap);
+      hipExtLaunchKernelGGL((k_stage<FT, 0, FUSE>), dim3(groups), dim3(tpb), 0, h->stream, e0, e1, 0, h->d_blocks,
+                            h->d_cases, h->d_ctrl, h->d_items, lo, hi, (const int*)h->d_bcface,
+                            (const BCFlux*)h->d_bcflux, h->d_partials, irk, h->xcd_remap);
     else
-      hipLaunchKernelGGL((k_stage<FT, 1, FUSE>), dim3(groups), dim3(tpb), 0, h->stream, h->d_blocks, h->d_cases, h->d_ctrl,
-                         h->d_items, lo, hi, h->d_bcface, h->d_bcflux, h->d_partials, irk, h->xcd_remap);
+      hipExtLaunchKernelGGL((k_stage<FT, 1, FUSE>), dim3(groups), dim3(tpb), 0, h->stream, e0, e1, 0, h->d_blocks,
+                            h->d_cases, h->d_ctrl, h->d_items, lo, hi, (const int*)h->d_bcface,
+                            (const BCFlux*)h->d_bcflux, h->d_partials, irk, h->xcd_remap);
+    if (h->timing) {
+      HIPCHK(hipEventSynchronize(h->ev1));
+      float ms = 0.f;
+      HIPCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+      h->timed_ms += ms;
+      h->timed_launches += 1;
+    }
   }
+  return Q2D_OK;
 }
 
 int launch_step(q2d_handle* h) {
@@ -1049,17 +1064,8 @@ int launch_step(q2d_handle* h) {
         hipLaunchKernelGGL(k_bcflux<double>, dim3(bgroups), dim3(256), 0, h->stream, h->d_blocks, h->d_cases, h->d_ctrl,
                            h->d_bc, h->nbc, irk, h->d_bcflux);
     }
-    if (h->timing) HIPCHK(hipEventRecord(h->ev0, h->stream));
-    if (h->precision_mixed) launch_stage<float, kFuseDtMixed>(h, irk);
-    else launch_stage<double, kFuseDtFp64>(h, irk);
-    if (h->timing) {
-      HIPCHK(hipEventRecord(h->ev1, h->stream));
-      HIPCHK(hipEventSynchronize(h->ev1));
-      float ms = 0.f;
-      HIPCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
-      h->timed_ms += ms;
-      h->timed_launches += 1;
-    }
+    int rc = h->precision_mixed ? launch_stage<float, kFuseDtMixed>(h, irk) : launch_stage<double, kFuseDtFp64>(h, irk);
+    if (rc != Q2D_OK) return rc;
   }
   hipLaunchKernelGGL(k_finish, dim3(h->ncases), dim3(256), 0, h->stream, h->d_ctrl, h->d_cases, h->d_partials, h->ncases);
   HIPCHK(hipGetLastError());
