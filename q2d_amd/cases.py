@@ -377,7 +377,7 @@ def _tanh_wall_cluster(nj, first):
 
 def axisymmetric_pipe(seed=0, ni=2048, nj=512, first_cell=1e-5, reynolds=1e5, mach=None, radius=0.25,
                       T_wall=None, precision="mixed", n_rk=3, limiter="vanleer", riemann="HLLC", cfl=0.5,
-                      vnn=0.3, steps=2000):
+                      vnn=0.15, steps=2000):
     """Axisymmetric laminar Navier-Stokes: supersonic inflow (BC 405, face 1) into a pipe of length 1 and
     radius `radius`; axis r = 0 on j = 0 (BC 300, face 3), isothermal no-slip wall on r = radius (BC 302,
     face 4), supersonic outflow (BC 400, face 2).  Radial tanh clustering at the wall with a first cell of

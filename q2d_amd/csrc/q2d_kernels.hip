@@ -60,6 +60,7 @@ struct DevCase {
   int ndir, itermax, item_begin, item_end;
   double cfl, vnn, R, cp, cv, gam;
   double res_threshold, time_threshold;
+  double mu_ref, T_ref, S_suth, Pr;     // Sutherland viscosity, Prandtl number (model 1)
 };
 
 struct DevCtrl {
@@ -84,6 +85,9 @@ struct DevBlock {
   const double *m11, *m21, *m12, *m22;
   const double *sa, *sb;
   double* dtl[2];                // local dt, double-buffered: [dtsel] holds the current step's dt
+  const double* nodr;            // node(i,j)%c(2) at cell (i,j), i = 0..ni, j = 0..nj (viscous cases)
+  double *vdx, *vey, *vsv;       // 3 planes each: diffusive flux of i-face (x|x+1), of j-face (r|r+1), and
+                                 // the axisymmetric viscous source, components rho u, rho v, rho E
 };
 
 struct DevBC {
@@ -94,6 +98,14 @@ struct DevBC {
 };
 
 struct Item { int blk, x0, y0, y1; };
+
+// Laminar transport of one species: Sutherland viscosity and Fourier conductivity k = mu cp / Pr, the
+// closed form behind the reference's f_laminarViscosity / co_k_mi_lam_Wilke calls (FP64 in both forms).
+__device__ __forceinline__ double mu_lam(const DevCase& C, double T) {
+  const double x = T / C.T_ref;
+  return C.mu_ref * (x * sqrt(x)) * (C.T_ref + C.S_suth) / (T + C.S_suth);
+}
+__device__ __forceinline__ double k_lam(const DevCase& C, double mu) { return mu * C.cp / C.Pr; }
 
 __device__ __forceinline__ int stage_in(int cur, int irk) { return irk == 1 ? cur : (cur + 1 + ((irk - 2) & 1)) % 3; }
 __device__ __forceinline__ int stage_out(int cur, int irk) { return (cur + 1 + ((irk - 1) & 1)) % 3; }
@@ -204,6 +216,16 @@ __device__ __forceinline__ double cell_dt(const DevCase& C, const DevCtrl& K, do
     const double lambda = fabs(u * v1 + v * v2) + snd;
     const double dt = dly / lambda * C.cfl;
     if (dt < dtmin) dtmin = dt;
+  }
+  if (C.model > 0) {                               // VNN limit — Mod_dt.f90:88-102 (laminar: mie = 0)
+    const double mil = mu_lam(C, p / (rho * C.R));
+    const double mi = 0.0 + mil;
+    double dt = rho * (dlx * dlx) * C.vnn / mi;
+    if (dt < dtmin) dtmin = dt;
+    if (C.ndir == 2) {
+      dt = rho * (dly * dly) * C.vnn / mi;
+      if (dt < dtmin) dtmin = dt;
+    }
   }
   if (K.iter < C.ramp_iter) dtmin = dtmin * (double)K.iter / (double)C.ramp_iter;   // Mod_dt.f90:43
   return dtmin;
@@ -327,7 +349,9 @@ __device__ __forceinline__ void ghost_fill(const DevBlock* blocks, const DevBC& 
 // one thread per boundary record.  The stage kernel adds the stored values to the cells'
 // residuals in record order, exactly where the reference adds them.
 // ---------------------------------------------------------------------------
-struct BCFlux { double F[4]; int mask; int pad; };   // F already carries the face sign modfm2
+// F: convective boundary flux, already carrying the face sign modfm2, added where mask has the bit.
+// V: viscous boundary flux as -(modfm2 * Flux), added after F where vmask has the bit (BC 300/302, model 1).
+struct BCFlux { double F[4]; double V[4]; int mask; int vmask; };
 
 template <typename FT>
 __device__ __forceinline__ void riemann_rt(int rid, const Gas<FT>& g, const FT L[4], const FT R[4], FT nx, FT ny, FT F[4]) {
@@ -375,7 +399,8 @@ __global__ __launch_bounds__(256) void k_bcflux(const DevBlock* __restrict__ blo
 #pragma unroll
   for (int v = 0; v < 4; ++v) Bp[v] = P[v * pl + oB];
   BCFlux res;
-  res.mask = 0xF; res.pad = 0;
+  res.mask = 0xF; res.vmask = 0;
+  res.V[0] = res.V[1] = res.V[2] = res.V[3] = 0.0;
   int ty = E.type;
 
   if (ty == 101 || ty == 201) {
@@ -545,6 +570,259 @@ __global__ __launch_bounds__(256) void k_bcflux(const DevBlock* __restrict__ blo
 }
 
 // ---------------------------------------------------------------------------
+// laminar Navier-Stokes terms (model 1), FP64 in both arithmetic forms
+// ---------------------------------------------------------------------------
+constexpr double kNegTwoThird = -2.0 / 3.0;
+
+// Stress_Vector_Std / Stress_Vector_Axi — Lib_Fluid.f90:73-125, laminar (mi = mi_l + 0).  gUX = Gradient(nu,X),
+// gVX = Gradient(nv,X) after the metric transform; the Std form gets V.G = 0 (no area variation).
+__device__ __forceinline__ void stress_vector(bool axi, double gUx, double gUy, double gVx, double gVy, double n1,
+                                              double n2, double mil, double r_ave, double vr, double& S1, double& S2) {
+  const double mi = mil + 0.0;
+  const double lam = kNegTwoThird * mi;
+  double div;
+  if (axi) div = (r_ave <= 1.0e-6) ? (gUx + gVy + gVy) : (gUx + gVy + vr / r_ave);
+  else div = gUx + gVy - 0.0;
+  const double ld = lam * div;
+  const double txx = ld + 2.0 * mi * gUx;
+  const double tyy = ld + 2.0 * mi * gVy;
+  const double txy = mi * (gUy + gVx);
+  S1 = txx * n1 + txy * n2;
+  S2 = txy * n1 + tyy * n2;
+}
+
+// one row of matmul(Gradient, M): (g0, g1) . M, M(l,c) column-major = (m11, m21, m12, m22)
+__device__ __forceinline__ void grad_m(double g0, double g1, const double M[4], double& o0, double& o1) {
+  o0 = g0 * M[0] + g1 * M[1];
+  o1 = g0 * M[2] + g1 * M[3];
+}
+
+// Diffusive_Flux + Tangential_Gradient + Compute_Diffusive_Flux — Lib_Diffusive.f90:10-199 for one interior
+// face.  q1, q2: the face's two cells; q3..q6: the tangential stencil (q4 - q3 + q6 - q5) / 4; a: 0 for an
+// i-face, 1 for a j-face; M: the two cells' metric averaged.  One species: the species flux and the
+// enthalpy-diffusion term vanish (grad Y = 0), so D = (Flux(nu), Flux(nv), Flux(np)).
+__device__ __forceinline__ void visc_face(const DevCase& C, bool axi, int a, double n1, double n2, double il,
+                                          const double q1[4], const double q2[4], const double q3[4],
+                                          const double q4[4], const double q5[4], const double q6[4],
+                                          const double M[4], double r_ave, double D[3]) {
+  const double R = C.R;
+  const double T1 = q1[3] / (q1[0] * R), T2 = q2[3] / (q2[0] * R);
+  const double T3 = q3[3] / (q3[0] * R), T4 = q4[3] / (q4[0] * R);
+  const double T5 = q5[3] / (q5[0] * R), T6 = q6[3] / (q6[0] * R);
+  const double nU = q2[1] - q1[1], nV = q2[2] - q1[2], nT = T2 - T1;
+  const double tU = (q4[1] - q3[1] + q6[1] - q5[1]) * 0.25;
+  const double tV = (q4[2] - q3[2] + q6[2] - q5[2]) * 0.25;
+  const double tT = (T4 - T3 + T6 - T5) * 0.25;
+  double gUx, gUy, gVx, gVy, gTx, gTy;
+  if (a == 0) {
+    grad_m(nU, tU, M, gUx, gUy); grad_m(nV, tV, M, gVx, gVy); grad_m(nT, tT, M, gTx, gTy);
+  } else {
+    grad_m(tU, nU, M, gUx, gUy); grad_m(tV, nV, M, gVx, gVy); grad_m(tT, nT, M, gTx, gTy);
+  }
+  const double pm0 = 0.5 * (q1[0] + q2[0]), pm1 = 0.5 * (q1[1] + q2[1]);
+  const double pm2 = 0.5 * (q1[2] + q2[2]), pm3 = 0.5 * (q1[3] + q2[3]);
+  const double T = pm3 / (pm0 * R);
+  const double mil = mu_lam(C, T);
+  const double kappa = k_lam(C, mil) + 0.0;
+  double S1, S2;
+  stress_vector(axi, gUx, gUy, gVx, gVy, n1, n2, mil, r_ave, pm2, S1, S2);
+  D[0] = il * S1;
+  D[1] = il * S2;
+  D[2] = il * ((S1 * pm1 + S2 * pm2) + kappa * (gTx * n1 + gTy * n2) + 0.0);
+}
+
+// Interior diffusive face fluxes and the axisymmetric viscous source of every cell of the viscous work
+// items (one wavefront per item, one lane per column).  Cell (x,r) stores the flux of its i-face (x|x+1)
+// (Mod_Fluxes.f90:160-183), of its j-face (r|r+1) (:207-230) and its Source_Axi_Visc_Blk term
+// (Lib_Source_Axisymmetry.f90:81-247); the stage kernel adds them to the residual in the reference's order.
+// The faces read the ghost cells as Fill_Ghost_Cell left them; the source reads a wall face's ghost as
+// Ghost_Wall_Extrapolation (Lib_Ghost.f90:366-399) rewrites it just before the sources: the boundary
+// cell's own state with the velocity reversed.
+__global__ __launch_bounds__(256) void k_visc(const DevBlock* __restrict__ blocks, const DevCase* __restrict__ cases,
+                                              const DevCtrl* __restrict__ ctrl, const Item* __restrict__ items,
+                                              int item_lo, int item_hi, const DevBC* __restrict__ bcs,
+                                              const int* __restrict__ bcface, int irk) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wpg = blockDim.x / kWave;
+  const int wave0 = blockIdx.x * wpg + threadIdx.x / kWave;
+  const int nwaves = gridDim.x * wpg;
+  for (int it = item_lo + wave0; it < item_hi; it += nwaves) {
+    const Item item = items[it];
+    const DevBlock& B = blocks[item.blk];
+    const DevCtrl& K = ctrl[B.case_id];
+    const DevCase& C = cases[B.case_id];
+    if (!K.active || irk > C.n_rk) continue;
+    const int x = item.x0 - 2 + lane;
+    if (!(lane >= 2 && lane < 2 + kOwn && x <= B.ni)) continue;
+    const int ni = B.ni, nj = B.nj, pitch = B.pitch;
+    const long long pl = B.plane;
+    const bool axi = C.source == Q2D_SRC_AXI;
+    GP P = gp(B.P[stage_in(K.cur, irk)]);
+    GP nodr = gp(B.nodr);
+    GPW vdx = gpw(B.vdx), vey = gpw(B.vey), vsv = gpw(B.vsv);
+    auto at = [pitch, x](int dx, int r) { return (long long)(r + 1) * pitch + (x + dx + 1); };
+    auto ld = [&](long long o, double q[4]) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) q[v] = P[v * pl + o];
+    };
+    auto ldM = [&](long long o, double M[4]) {
+      M[0] = gp(B.m11)[o]; M[1] = gp(B.m21)[o]; M[2] = gp(B.m12)[o]; M[3] = gp(B.m22)[o];
+    };
+    // wall faces of the boundary cells of this column (BC 301..304)
+    auto is_wall = [&](int f, int pos) {
+      if (B.bcf[f] < 0) return false;
+      const int t = bcs[bcface[B.bcf[f] + pos - 1]].type;
+      return t >= 301 && t <= 304;
+    };
+    const bool wall3 = axi && is_wall(2, x), wall4 = axi && is_wall(3, x);
+    for (int r = item.y0; r <= item.y1; ++r) {
+      const long long o = at(0, r);
+      double q0[4], qE[4], qW[4], qS[4], qN[4], qNE[4], qSE[4], qNW[4], M0[4];
+      ld(o, q0); ld(at(1, r), qE); ld(at(-1, r), qW); ld(at(0, r - 1), qS); ld(at(0, r + 1), qN);
+      ld(at(1, r + 1), qNE); ld(at(1, r - 1), qSE); ld(at(-1, r + 1), qNW);
+      ldM(o, M0);
+      if (x <= ni - 1) {                               // i-face (x | x+1)
+        double M1[4], Ma[4], D[3];
+        ldM(at(1, r), M1);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) Ma[k] = 0.5 * (M0[k] + M1[k]);
+        const double r_ave = 0.5 * (nodr[o] + nodr[at(0, r - 1)]);
+        visc_face(C, axi, 0, gp(B.fix)[o], gp(B.fiy)[o], gp(B.fil)[o], q0, qE, qS, qN, qSE, qNE, Ma, r_ave, D);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) vdx[k * pl + o] = D[k];
+      }
+      if (r <= nj - 1) {                               // j-face (r | r+1)
+        double M1[4], Ma[4], D[3];
+        ldM(at(0, r + 1), M1);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) Ma[k] = 0.5 * (M0[k] + M1[k]);
+        const double r_ave = 0.5 * (nodr[o] + nodr[at(-1, r)]);
+        visc_face(C, axi, 1, gp(B.fjx)[o], gp(B.fjy)[o], gp(B.fjl)[o], q0, qN, qW, qE, qNW, qNE, Ma, r_ave, D);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) vey[k * pl + o] = D[k];
+      }
+      if (axi) {
+        // Source_Axi_Visc_Blk :125-236 with the wall ghosts reflected
+        const bool w1 = (x == 1) && is_wall(0, r), w2 = (x == ni) && is_wall(1, r);
+        const bool w3 = (r == 1) && wall3, w4 = (r == nj) && wall4;
+        auto refl = [&](double q[4]) { q[0] = q0[0]; q[1] = -q0[1]; q[2] = -q0[2]; q[3] = q0[3]; };
+        if (w1) refl(qW);
+        if (w2) refl(qE);
+        if (w3) refl(qS);
+        if (w4) refl(qN);
+        const double R = C.R;
+        const double rho = q0[0];
+        const double T = q0[3] / (rho * R);
+        const double T1 = qE[3] / (qE[0] * R), T2 = qW[3] / (qW[0] * R);
+        const double T3 = qN[3] / (qN[0] * R), T4 = qS[3] / (qS[0] * R);
+        const double mil = mu_lam(C, T);
+        const double kl = k_lam(C, mil);
+        double gUx, gUy, gVx, gVy, gTx, gTy;
+        grad_m((qE[1] - qW[1]) * 0.5, (qN[1] - qS[1]) * 0.5, M0, gUx, gUy);
+        grad_m((qE[2] - qW[2]) * 0.5, (qN[2] - qS[2]) * 0.5, M0, gVx, gVy);
+        grad_m((T1 - T2) * 0.5, (T3 - T4) * 0.5, M0, gTx, gTy);
+        (void)gTx;
+        const double kappa = kl + 0.0;
+        const double r_ave = gp(B.sa)[o], invr = gp(B.sb)[o];
+        // Stress_Tensor_Axi — Lib_Fluid.f90:166-198
+        const double vr = q0[2];
+        const double mi = mil + 0.0;
+        const double lam = kNegTwoThird * mi;
+        const double div = gUx + gVy + vr / r_ave;
+        const double ld2 = lam * div;
+        const double tyy = ld2 + 2.0 * mi * gVy;
+        const double txy = mi * (gUy + gVx);
+        const double tzz = 2.0 * mi * vr / r_ave + ld2;
+        const double qr = -kappa * gTy;
+        vsv[o] = txy * invr;
+        vsv[pl + o] = (tyy - tzz) * invr;
+        vsv[2 * pl + o] = (-qr + q0[1] * txy + q0[2] * tyy + 0.0) * invr;
+      }
+    }
+  }
+}
+
+// BC_Symmetry_Visc (Lib_BC_Fluxes_Symmetry.f90:127-236) and BC_Wall_Temperature
+// (Lib_BC_Fluxes_Wall_Temperature.f90:20-106, static wall) for the boundary records of viscous cases.
+// Runs after every ghost cell of the stage is filled: at a corner the symmetry stencil reads the
+// neighbouring face's ghost.  One thread per listed record; the result goes to BCFlux::V.
+__global__ __launch_bounds__(256) void k_bcvisc(const DevBlock* __restrict__ blocks, const DevCase* __restrict__ cases,
+                                                const DevCtrl* __restrict__ ctrl, const DevBC* __restrict__ bcs,
+                                                const int* __restrict__ vrec, int nvrec, int irk,
+                                                BCFlux* __restrict__ out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nvrec) return;
+  const int k = vrec[t];
+  const DevBC E = bcs[k];
+  const DevBlock& B = blocks[E.blk];
+  const DevCtrl& K = ctrl[B.case_id];
+  const DevCase& C = cases[B.case_id];
+  if (!K.active || irk > C.n_rk) return;
+  const double* P = B.P[stage_in(K.cur, irk)];
+  const long long pl = B.plane;
+  const int pitch = B.pitch;
+  auto off = [pitch](int i, int j) { return (long long)(j + 1) * pitch + (i + 1); };
+  const int Im = E.i, Jm = E.j, Fm = E.f;
+  const int modfm = Fm % 2, modfm2 = 1 - 2 * modfm, modfm3 = 2 * (2 * modfm - 1);
+  const double sgn = (double)modfm2;
+  const bool axi = C.source == Q2D_SRC_AXI;
+  const bool idir = Fm <= 2;
+  const long long ofc = idir ? off(Im - modfm, Jm) : off(Im, Jm - modfm);
+  const double n1 = idir ? B.fix[ofc] : B.fjx[ofc];
+  const double n2 = idir ? B.fiy[ofc] : B.fjy[ofc];
+  const double il = idir ? B.fil[ofc] : B.fjl[ofc];
+  const long long oB = off(Im, Jm);
+  const double M[4] = {B.m11[oB], B.m21[oB], B.m12[oB], B.m22[oB]};
+  // face radius from the nodes of the face (:183-192)
+  double r_ave;
+  switch (Fm) {
+    case 1: r_ave = 0.5 * (B.nodr[off(Im - 1, Jm)] + B.nodr[off(Im - 1, Jm - 1)]); break;
+    case 2: r_ave = 0.5 * (B.nodr[off(Im, Jm)] + B.nodr[off(Im, Jm - 1)]); break;
+    case 3: r_ave = 0.5 * (B.nodr[off(Im, Jm - 1)] + B.nodr[off(Im - 1, Jm - 1)]); break;
+    default: r_ave = 0.5 * (B.nodr[off(Im, Jm)] + B.nodr[off(Im - 1, Jm)]); break;
+  }
+  const double rho = P[oB], u = P[pl + oB], v = P[2 * pl + oB], p = P[3 * pl + oB];
+  double F1, F2, F3;
+  if (E.type == 300) {
+    const double mil = mu_lam(C, p / (rho * C.R));
+    const long long o3 = idir ? off(Im, Jm - 1) : off(Im - 1, Jm);
+    const long long o4 = idir ? off(Im, Jm + 1) : off(Im + 1, Jm);
+    const double V3x = P[pl + o3], V3y = P[2 * pl + o3], V4x = P[pl + o4], V4y = P[2 * pl + o4];
+    const double d1 = u * n1 + v * n2, d3 = V3x * n1 + V3y * n2, d4 = V4x * n1 + V4y * n2;
+    const double V2x = u - 2.0 * d1 * n1, V2y = v - 2.0 * d1 * n2;
+    const double V5x = V3x - 2.0 * d3 * n1, V5y = V3y - 2.0 * d3 * n2;
+    const double V6x = V4x - 2.0 * d4 * n1, V6y = V4y - 2.0 * d4 * n2;
+    const double aU = (V2x - u) * sgn, aV = (V2y - v) * sgn;                  // Gradient(:,a)
+    const double bU = (V4x - V3x + V6x - V5x) * 0.25, bV = (V4y - V3y + V6y - V5y) * 0.25;   // Gradient(:,b)
+    double gUx, gUy, gVx, gVy;
+    if (idir) { grad_m(aU, bU, M, gUx, gUy); grad_m(aV, bV, M, gVx, gVy); }
+    else { grad_m(bU, aU, M, gUx, gUy); grad_m(bV, aV, M, gVx, gVy); }
+    const double Vmx = 0.5 * (u + V2x), Vmy = 0.5 * (v + V2y);
+    double S1, S2;
+    stress_vector(axi, gUx, gUy, gVx, gVy, n1, n2, mil, r_ave, 0.5 * (v + V2y), S1, S2);
+    F1 = il * S1; F2 = il * S2;
+    F3 = il * (Vmx * S1 + Vmy * S2);
+  } else {                                           // 302
+    const double T = p / (rho * C.R);
+    const double mil = mu_lam(C, E.T), kl = k_lam(C, mil);
+    const double m3 = (double)modfm3;
+    double gU0 = 0.0, gU1 = 0.0, gV0 = 0.0, gV1 = 0.0, gT0 = 0.0, gT1 = 0.0;
+    if (idir) { gU0 = u * m3; gV0 = v * m3; gT0 = (T - E.T) * m3; }
+    else { gU1 = u * m3; gV1 = v * m3; gT1 = (T - E.T) * m3; }
+    double gUx, gUy, gVx, gVy, gTx, gTy;
+    grad_m(gU0, gU1, M, gUx, gUy); grad_m(gV0, gV1, M, gVx, gVy); grad_m(gT0, gT1, M, gTx, gTy);
+    double S1, S2;
+    stress_vector(axi, gUx, gUy, gVx, gVy, n1, n2, mil, r_ave, 0.0, S1, S2);
+    F1 = S1 * il; F2 = S2 * il;
+    F3 = il * kl * (gTx * n1 + gTy * n2);
+  }
+  BCFlux& q = out[k];
+  q.V[0] = -(sgn * 0.0);                             // Flux(1) = 0; BC 302 updates R(:), BC 300 R(nu:np)
+  q.V[1] = -(sgn * F1); q.V[2] = -(sgn * F2); q.V[3] = -(sgn * F3);
+  q.vmask = E.type == 302 ? 0xF : 0xE;
+}
+
+// ---------------------------------------------------------------------------
 // the fused RK-stage kernel
 // ---------------------------------------------------------------------------
 template <typename FT>
@@ -604,7 +882,7 @@ __device__ __forceinline__ bool face_flux(const Gas<FT>& g, const double qa[4], 
   return bad;
 }
 
-template <typename FT, int RIE, bool FUSE>
+template <typename FT, int RIE, bool FUSE, bool VISC>
 __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlock* __restrict__ blocks, const DevCase* __restrict__ cases,
                                                DevCtrl* __restrict__ ctrl, const Item* __restrict__ items, int item_lo,
                                                int item_hi, const int* __restrict__ bcface,
@@ -682,6 +960,7 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
     }
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     double dtnext = 1e300;
+    double Eprev[3] = {0.0, 0.0, 0.0};                 // diffusive flux of j-face r-1 (VISC)
 
     for (int r = item.y0 - 1; r <= item.y1; ++r) {
       // software pipeline: this iteration's row r+2 was loaded one iteration ago; issue row r+3
@@ -704,6 +983,12 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
 #pragma unroll
         for (int v = 0; v < 4; ++v) sBC[v] = qdiv<FT>((FT)(qC[v] - qB[v]), lBC);
         limiter4<FT>(lim, sBC, sAB, phiB);            // phi_{r+1} = rlimiter(s(r+1|r+2), s(r|r+1))
+      }
+      double Ecur[3] = {0.0, 0.0, 0.0};
+      if (VISC && own && r >= 1 && r <= nj - 1) {
+        const long long oE = at(r);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) Ecur[k] = gp(B.vey)[k * pl + oE];
       }
       // interior j-face (r | r+1) — Mod_Fluxes.f90:187-232
       if (own && r >= 1 && r <= nj - 1) {
@@ -746,17 +1031,39 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
 #pragma unroll
         for (int v = 0; v < 4; ++v) Fl[v] = shup(Fx[v]);
 
+        double Dx[3] = {0.0, 0.0, 0.0}, Dl[3] = {0.0, 0.0, 0.0};
+        if (VISC) {                                    // diffusive flux of i-face x, and of x-1 from the left lane
+          const long long oD = at(r);
+#pragma unroll
+          for (int k = 0; k < 3; ++k) Dx[k] = gp(B.vdx)[k * pl + oD];
+#pragma unroll
+          for (int k = 0; k < 3; ++k) Dl[k] = shup(Dx[k]);
+        }
         if (own) {
           const long long oA = at(r);
           // ---- residual in the reference's accumulation order ------------
+          // per face: convective flux (Convective_Flux), then diffusive flux (Diffusive_Flux: Res1 -= F,
+          // Res2 += F; its density component is 0)
           double res[4];
 #pragma unroll
           for (int v = 0; v < 4; ++v) {
             double rv = 0.0;
-            if (x >= 2) rv = 0.0 - (double)Fl[v];           // face x-1 (Mod_Fluxes.f90:140-185)
-            if (x <= ni - 1) rv = rv + (double)Fx[v];       // face x
-            if (r >= 2) rv = rv - (double)Gprev[v];         // face r-1 (:187-232)
-            if (r <= nj - 1) rv = rv + (double)Gcur[v];     // face r
+            if (x >= 2) {                                   // face x-1 (Mod_Fluxes.f90:140-185)
+              rv = 0.0 - (double)Fl[v];
+              if (VISC) rv = rv + (v == 0 ? 0.0 : Dl[v - 1]);
+            }
+            if (x <= ni - 1) {                              // face x
+              rv = rv + (double)Fx[v];
+              if (VISC) rv = rv - (v == 0 ? 0.0 : Dx[v - 1]);
+            }
+            if (r >= 2) {                                   // face r-1 (:187-232)
+              rv = rv - (double)Gprev[v];
+              if (VISC) rv = rv + (v == 0 ? 0.0 : Eprev[v - 1]);
+            }
+            if (r <= nj - 1) {                              // face r
+              rv = rv + (double)Gcur[v];
+              if (VISC) rv = rv - (v == 0 ? 0.0 : Ecur[v - 1]);
+            }
             res[v] = rv;
           }
           // ---- BC faces 1..4 in record order (Mod_BC_Fluxes.f90:79-86) ----
@@ -773,6 +1080,11 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
 #pragma unroll
               for (int v = 0; v < 4; ++v)
                 if ((q.mask >> v) & 1) res[v] = res[v] + q.F[v];
+              if (VISC) {
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                  if ((q.vmask >> v) & 1) res[v] = res[v] + q.V[v];
+              }
             }
           }
           // ---- geometric sources — Lib_Source_Axisymmetry.f90:40-78, Lib_Source_Areavariation.f90:104-144
@@ -798,6 +1110,10 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
             }
 #pragma unroll
             for (int k = 0; k < 4; ++k) res[k] = res[k] - S[k] * sr;
+            if (VISC && C.source == Q2D_SRC_AXI) {      // Source_Axi_Visc_Blk, after the Euler source
+#pragma unroll
+              for (int k = 1; k < 4; ++k) res[k] = res[k] - gp(B.vsv)[(k - 1) * pl + oA] * sr;
+            }
           }
           // ---- Compute_Cons_NoIRS — Lib_Newstate.f90:131-159 -------------
           const double dt = tacc ? dtg : gcur.dtl;
@@ -868,6 +1184,10 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
       for (int v = 0; v < 4; ++v) {
         qA[v] = qB[v]; qB[v] = qC[v];
         sAB[v] = sBC[v]; phiA[v] = phiB[v]; Gprev[v] = Gcur[v];
+      }
+      if (VISC) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) Eprev[k] = Ecur[k];
       }
       dlyA = dlyB; dlyB = dlyC;
     }
@@ -966,7 +1286,12 @@ struct q2d_handle {
   Item* d_items = nullptr;
   double* d_partials = nullptr;
   BCFlux* d_bcflux = nullptr;
-  int n_hllc_items = 0;          // items are sorted: HLLC cases first, then HLLE cases
+  int range[5] = {0, 0, 0, 0, 0}; // items sorted into [range[k], range[k+1]): k = 2*viscous + riemann
+  bool any_visc = false;
+  std::vector<int> vrec;         // boundary records with a viscous flux (BC 300/302 of model-1 cases)
+  int* d_vrec = nullptr;
+  std::vector<hipEvent_t> tev;   // per-launch event pairs of the timed stage launches of one step
+  int tslot = 0;
   hipStream_t stream = nullptr;
   hipGraphExec_t graph = nullptr;
   int rows_per_item = 16, waves_per_group = 4, max_groups = 0, xcd_remap = 1;
@@ -975,7 +1300,6 @@ struct q2d_handle {
   bool timing = false;
   long long timed_launches = 0;
   double timed_ms = 0.0;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int cu_count = 256;
 };
 
@@ -983,10 +1307,11 @@ namespace {
 
 int build_items(q2d_handle* h) {
   h->items.clear();
-  // items grouped by Riemann solver (one kernel instantiation each), then by case
-  for (int pass = 0; pass < 2; ++pass) {
+  // items grouped by (viscous, Riemann solver) — one kernel instantiation each — then by case
+  for (int pass = 0; pass < 4; ++pass) {
+    h->range[pass] = (int)h->items.size();
     for (int c = 0; c < h->ncases; ++c) {
-      if (h->dc[c].riemann != pass) continue;
+      if (h->dc[c].riemann != (pass & 1) || (h->dc[c].model > 0) != (pass >= 2)) continue;
       h->dc[c].item_begin = (int)h->items.size();
       for (int b = 0; b < h->nblocks; ++b) {
         if (h->hb[b].case_id != c) continue;
@@ -996,8 +1321,8 @@ int build_items(q2d_handle* h) {
       }
       h->dc[c].item_end = (int)h->items.size();
     }
-    if (pass == 0) h->n_hllc_items = (int)h->items.size();
   }
+  h->range[4] = (int)h->items.size();
   if (h->d_items) { (void)hipFree(h->d_items); h->d_items = nullptr; }
   if (h->d_partials) { (void)hipFree(h->d_partials); h->d_partials = nullptr; }
   HIPCHK(hipMalloc(&h->d_items, sizeof(Item) * std::max<size_t>(1, h->items.size())));
@@ -1014,29 +1339,30 @@ int build_items(q2d_handle* h) {
 template <typename FT, bool FUSE>
 int launch_stage(q2d_handle* h, int irk) {
   const int tpb = kWave * h->waves_per_group;
-  const int ranges[2][2] = {{0, h->n_hllc_items}, {h->n_hllc_items, (int)h->items.size()}};
-  for (int rie = 0; rie < 2; ++rie) {
-    const int lo = ranges[rie][0], hi = ranges[rie][1];
+  for (int pass = 0; pass < 4; ++pass) {
+    const int lo = h->range[pass], hi = h->range[pass + 1];
     if (hi <= lo) continue;
     int groups = (hi - lo + h->waves_per_group - 1) / h->waves_per_group;
     if (h->max_groups > 0) groups = std::min(groups, h->max_groups);
     groups = std::max(groups, 1);
-    hipEvent_t e0 = h->timing ? h->ev0 : nullptr, e1 = h->timing ? h->ev1 : nullptr;
-    if (rie == 0)
-      hipExtLaunchKernelGGL((k_stage<FT, 0, FUSE>), dim3(groups), dim3(tpb), 0, h->stream, e0, e1, 0, h->d_blocks,
-                            h->d_cases, h->d_ctrl, h->d_items, lo, hi, (const int*)h->d_bcface,
-                            (const BCFlux*)h->d_bcflux, h->d_partials, irk, h->xcd_remap);
-    else
-      hipExtLaunchKernelGGL((k_stage<FT, 1, FUSE>), dim3(groups), dim3(tpb), 0, h->stream, e0, e1, 0, h->d_blocks,
-                            h->d_cases, h->d_ctrl, h->d_items, lo, hi, (const int*)h->d_bcface,
-                            (const BCFlux*)h->d_bcflux, h->d_partials, irk, h->xcd_remap);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
     if (h->timing) {
-      HIPCHK(hipEventSynchronize(h->ev1));
-      float ms = 0.f;
-      HIPCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
-      h->timed_ms += ms;
-      h->timed_launches += 1;
+      if ((size_t)(2 * h->tslot + 2) > h->tev.size()) return fail(Q2D_E_HIP, "timing event pool exhausted");
+      e0 = h->tev[2 * h->tslot]; e1 = h->tev[2 * h->tslot + 1]; h->tslot++;
     }
+    const int* bf = h->d_bcface;
+    const BCFlux* bq = h->d_bcflux;
+#define Q2D_STAGE_LAUNCH(RIE, VISC)                                                                               \
+    hipExtLaunchKernelGGL((k_stage<FT, RIE, FUSE, VISC>), dim3(groups), dim3(tpb), 0, h->stream, e0, e1, 0,       \
+                          h->d_blocks, h->d_cases, h->d_ctrl, h->d_items, lo, hi, bf, bq, h->d_partials, irk,    \
+                          h->xcd_remap)
+    switch (pass) {
+      case 0: Q2D_STAGE_LAUNCH(0, false); break;
+      case 1: Q2D_STAGE_LAUNCH(1, false); break;
+      case 2: Q2D_STAGE_LAUNCH(0, true); break;
+      default: Q2D_STAGE_LAUNCH(1, true); break;
+    }
+#undef Q2D_STAGE_LAUNCH
   }
   return Q2D_OK;
 }
@@ -1049,7 +1375,9 @@ int launch_step(q2d_handle* h) {
   groups = std::max(groups, 1);
   const int cthreads = 64, cgroups = (h->ncases + cthreads - 1) / cthreads;
   const int bgroups = (h->nbc + 255) / 256;
-  const int use_next = (h->dt_fused_valid && (h->precision_mixed ? kFuseDtMixed : kFuseDtFp64)) ? 1 : 0;
+  const int use_next =
+      (h->dt_fused_valid && !h->any_visc && (h->precision_mixed ? kFuseDtMixed : kFuseDtFp64)) ? 1 : 0;
+  h->tslot = 0;
   hipLaunchKernelGGL(k_begin, dim3(cgroups), dim3(cthreads), 0, h->stream, h->d_ctrl, h->ncases, use_next);
   if (!use_next)
     hipLaunchKernelGGL(k_dt, dim3(groups), dim3(tpb), 0, h->stream, h->d_blocks, h->d_cases, h->d_ctrl, h->d_items,
@@ -1064,11 +1392,29 @@ int launch_step(q2d_handle* h) {
         hipLaunchKernelGGL(k_bcflux<double>, dim3(bgroups), dim3(256), 0, h->stream, h->d_blocks, h->d_cases, h->d_ctrl,
                            h->d_bc, h->nbc, irk, h->d_bcflux);
     }
+    if (!h->vrec.empty())
+      hipLaunchKernelGGL(k_bcvisc, dim3(((int)h->vrec.size() + 255) / 256), dim3(256), 0, h->stream, h->d_blocks,
+                         h->d_cases, h->d_ctrl, h->d_bc, h->d_vrec, (int)h->vrec.size(), irk, h->d_bcflux);
+    if (h->range[4] > h->range[2]) {
+      int vg = (h->range[4] - h->range[2] + h->waves_per_group - 1) / h->waves_per_group;
+      if (h->max_groups > 0) vg = std::min(vg, h->max_groups);
+      hipLaunchKernelGGL(k_visc, dim3(std::max(vg, 1)), dim3(tpb), 0, h->stream, h->d_blocks, h->d_cases, h->d_ctrl,
+                         h->d_items, h->range[2], h->range[4], h->d_bc, h->d_bcface, irk);
+    }
     int rc = h->precision_mixed ? launch_stage<float, kFuseDtMixed>(h, irk) : launch_stage<double, kFuseDtFp64>(h, irk);
     if (rc != Q2D_OK) return rc;
   }
   hipLaunchKernelGGL(k_finish, dim3(h->ncases), dim3(256), 0, h->stream, h->d_ctrl, h->d_cases, h->d_partials, h->ncases);
   HIPCHK(hipGetLastError());
+  if (h->timing && h->tslot > 0) {                   // stage launches ran back to back; read their stamps now
+    HIPCHK(hipEventSynchronize(h->tev[2 * h->tslot - 1]));
+    for (int k = 0; k < h->tslot; ++k) {
+      float ms = 0.f;
+      HIPCHK(hipEventElapsedTime(&ms, h->tev[2 * k], h->tev[2 * k + 1]));
+      h->timed_ms += ms;
+      h->timed_launches += 1;
+    }
+  }
   h->dt_fused_valid = true;
   return Q2D_OK;
 }
@@ -1130,8 +1476,8 @@ int q2d_gpu_destroy(q2d_handle* h) {
   for (auto& b : h->hb) if (b.arena) (void)hipFree(b.arena);
   (void)hipFree(h->d_blocks); (void)hipFree(h->d_cases); (void)hipFree(h->d_ctrl); (void)hipFree(h->d_bc);
   (void)hipFree(h->d_bcface); (void)hipFree(h->d_items); (void)hipFree(h->d_partials); (void)hipFree(h->d_bcflux);
-  if (h->ev0) (void)hipEventDestroy(h->ev0);
-  if (h->ev1) (void)hipEventDestroy(h->ev1);
+  for (hipEvent_t e : h->tev) if (e) (void)hipEventDestroy(e);
+  (void)hipFree(h->d_vrec);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
   return Q2D_OK;
@@ -1164,7 +1510,12 @@ int q2d_gpu_create(const q2d_problem* prob, q2d_handle** out) {
     if (s.n_rk < 1 || s.n_rk > 3) return fail(Q2D_E_ARG, "n_rk must be 1, 2 or 3");
     if (s.limiter < 0 || s.limiter > 6) return fail(Q2D_E_ARG, "unknown limiter");
     if (s.riemann < 0 || s.riemann > 1) return fail(Q2D_E_ARG, "unknown Riemann solver");
-    if (s.model != 0) return fail(Q2D_E_UNSUPPORTED, "viscous model not available on this path");
+    if (s.model < 0 || s.model > 1) return fail(Q2D_E_UNSUPPORTED, "only the Euler (0) and laminar (1) models are available");
+    if (s.model == 1 && s.source == Q2D_SRC_AREA)
+      return fail(Q2D_E_UNSUPPORTED, "laminar model with the area-variation source not available on this path");
+    if (s.model == 1 && !(s.mu_ref > 0) ) return fail(Q2D_E_ARG, "laminar model needs mu_ref > 0");
+    if (s.model == 1 && (!(s.T_ref > 0) || !(s.Pr > 0) || !(s.S_suth >= 0)))
+      return fail(Q2D_E_ARG, "laminar model needs T_ref > 0, S >= 0, Pr > 0");
     if (s.source < 0 || s.source > 2) return fail(Q2D_E_ARG, "unknown source");
     if (s.precision < 0 || s.precision > 1) return fail(Q2D_E_ARG, "unknown precision");
     if (!(s.gas_R > 0) || !(s.gas_cp > s.gas_R)) return fail(Q2D_E_ARG, "gas constants invalid");
@@ -1173,6 +1524,8 @@ int q2d_gpu_create(const q2d_problem* prob, q2d_handle** out) {
       const q2d_block_desc& d = prob->blocks[first_block[c] + k];
       if (s.source == Q2D_SRC_AXI && !d.src_b) return fail(Q2D_E_ARG, "axisymmetric source needs src_b (axis_inv_r)");
       if (s.source == Q2D_SRC_AREA && (!d.src_a || !d.src_b)) return fail(Q2D_E_ARG, "area source needs src_a/src_b");
+      if (s.source == Q2D_SRC_AXI && s.model == 1 && (!d.src_a || !d.node))
+        return fail(Q2D_E_ARG, "axisymmetric laminar model needs src_a (axis_r_ave) and node");
     }
   }
 
@@ -1198,15 +1551,20 @@ int q2d_gpu_create(const q2d_problem* prob, q2d_handle** out) {
     k.itermax = cd.itermax; k.cfl = s.cfl; k.vnn = s.vnn;
     k.R = s.gas_R; k.cp = s.gas_cp; k.cv = s.gas_cp - s.gas_R; k.gam = s.gas_cp / k.cv;
     k.res_threshold = cd.res_threshold; k.time_threshold = cd.time_threshold;
+    k.mu_ref = s.mu_ref; k.T_ref = s.T_ref; k.S_suth = s.S_suth; k.Pr = s.Pr;
     h->max_rk = std::max(h->max_rk, s.n_rk);
+    if (s.model > 0) h->any_visc = true;
   }
 
   // ---- blocks: SoA planes in one arena per block ----
   h->hb.resize(h->nblocks);
   h->db.resize(h->nblocks);
-  const int kPlanes = 12 + 18;   // 3 state buffers x 4 + geometry/dt/source planes
+  // planes 0-11: 3 state buffers x 4; 12-24 geometry; 25, 28 local dt; 26, 27 source terms; 29 node radius;
+  // viscous blocks add 30-38: diffusive fluxes of i- and j-faces and the viscous source, 3 planes each
   for (int b = 0; b < h->nblocks; ++b) {
     const q2d_block_desc& d = prob->blocks[b];
+    const bool visc = prob->cases[d.case_id].scheme.model > 0;
+    const int kPlanes = visc ? 39 : 30;
     HostBlock& hb = h->hb[b];
     hb.ni = d.ni; hb.nj = d.nj; hb.case_id = d.case_id; hb.blk_in_case = b - first_block[d.case_id] + 1;
     hb.pitch = ((d.ni + 4) + 15) / 16 * 16;
@@ -1254,6 +1612,10 @@ int q2d_gpu_create(const q2d_problem* prob, q2d_handle** out) {
         if (d.src_a) plane(26)[dst] = d.src_a[src];
         if (d.src_b) plane(27)[dst] = d.src_b[src];
       }
+    if (d.node)                                       // node(0:ni, 0:nj)%c(2) at cell position (i, j)
+      for (int j = 0; j <= d.nj; ++j)
+        for (int i = 0; i <= d.ni; ++i)
+          plane(29)[(size_t)(j + 1) * pitch + (i + 1)] = d.node[((size_t)j * (d.ni + 1) + i) * 2 + 1];
     if (hipMemcpy(hb.arena, host.data(), hb.arena_bytes, hipMemcpyHostToDevice) != hipSuccess)
       return bail(fail(Q2D_E_HIP, "upload failed"));
     DevBlock& db = h->db[b];
@@ -1267,6 +1629,8 @@ int q2d_gpu_create(const q2d_problem* prob, q2d_handle** out) {
     db.fix = a + 19 * pl; db.fiy = a + 20 * pl; db.fil = a + 21 * pl;
     db.fjx = a + 22 * pl; db.fjy = a + 23 * pl; db.fjl = a + 24 * pl;
     db.dtl[0] = a + 25 * pl; db.dtl[1] = a + 28 * pl; db.sa = a + 26 * pl; db.sb = a + 27 * pl;
+    db.nodr = a + 29 * pl;
+    if (visc) { db.vdx = a + 30 * pl; db.vey = a + 33 * pl; db.vsv = a + 36 * pl; }
     for (int f = 0; f < 4; ++f) db.bcf[f] = -1;
   }
 
@@ -1306,6 +1670,9 @@ int q2d_gpu_create(const q2d_problem* prob, q2d_handle** out) {
     if (e.type != 101 && e.type != 201 && e.type != 300 && e.type != 302 && e.type != 400 && e.type != 405 &&
         e.type != 406)
       return bail(fail(Q2D_E_UNSUPPORTED, "bc flux type not available on this path"));
+    if (prob->cases[c].scheme.model > 0 && (e.type == 101 || e.type == 201))
+      return bail(fail(Q2D_E_UNSUPPORTED, "block connections of a laminar case (BC_Connection_Visc) not available on this path"));
+    if (prob->cases[c].scheme.model > 0 && (e.type == 300 || e.type == 302)) h->vrec.push_back(k);
     DevBC& r = h->dbc[k];
     memset(&r, 0, sizeof(r));
     r.type = e.type; r.blk = gb; r.i = e.i; r.j = e.j; r.f = e.f;
@@ -1347,15 +1714,20 @@ int q2d_gpu_create(const q2d_problem* prob, q2d_handle** out) {
 
   // ---- device copies ----
   if (hipStreamCreate(&h->stream) != hipSuccess) return bail(fail(Q2D_E_HIP, "stream creation failed"));
-  if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess)
-    return bail(fail(Q2D_E_HIP, "event creation failed"));
+  h->tev.assign(2 * 3 * 4, nullptr);                  // 3 stages x 4 kernel variants, start/stop each
+  for (auto& e : h->tev)
+    if (hipEventCreate(&e) != hipSuccess) return bail(fail(Q2D_E_HIP, "event creation failed"));
   if (hipMalloc(&h->d_blocks, sizeof(DevBlock) * h->nblocks) != hipSuccess ||
       hipMalloc(&h->d_cases, sizeof(DevCase) * h->ncases) != hipSuccess ||
       hipMalloc(&h->d_ctrl, sizeof(DevCtrl) * h->ncases) != hipSuccess ||
       hipMalloc(&h->d_bc, sizeof(DevBC) * std::max(1, h->nbc)) != hipSuccess ||
       hipMalloc(&h->d_bcflux, sizeof(BCFlux) * std::max(1, h->nbc)) != hipSuccess ||
-      hipMalloc(&h->d_bcface, sizeof(int) * std::max<size_t>(1, h->bcface.size())) != hipSuccess)
+      hipMalloc(&h->d_bcface, sizeof(int) * std::max<size_t>(1, h->bcface.size())) != hipSuccess ||
+      hipMalloc(&h->d_vrec, sizeof(int) * std::max<size_t>(1, h->vrec.size())) != hipSuccess)
     return bail(fail(Q2D_E_NOMEM, "descriptor allocation failed"));
+  if (!h->vrec.empty() &&
+      hipMemcpy(h->d_vrec, h->vrec.data(), sizeof(int) * h->vrec.size(), hipMemcpyHostToDevice) != hipSuccess)
+    return bail(fail(Q2D_E_HIP, "descriptor upload failed"));
   std::vector<DevCtrl> ctl(h->ncases);
   for (auto& k : ctl) { memset(&k, 0, sizeof(k)); k.active = 1; k.todo = 1; k.fail_key = ~0ull; }
   if (hipMemcpy(h->d_blocks, h->db.data(), sizeof(DevBlock) * h->nblocks, hipMemcpyHostToDevice) != hipSuccess ||

@@ -1,0 +1,113 @@
+"""Laminar Navier-Stokes on the HIP path against the oracle, through the C-ABI (needs the GPU).
+
+FP64 form: bit-identical state and local time step (interior diffusive fluxes, viscous symmetry and
+isothermal-wall fluxes, axisymmetric viscous source with reflected wall ghosts, VNN limit).
+Mixed form (FP32 convective face flux, FP64 viscous terms and state): relative L1 density error
+<= 1e-5 against the FP64 oracle."""
+import math
+
+import numpy as np
+import pytest
+
+from q2d_amd import GpuSolver, cases
+from q2d_amd.capi import Q2DError
+
+pytestmark = pytest.mark.gpu
+
+TOL_L1 = 1e-5
+
+
+def _planar_viscous(riemann="HLLE", limiter="vanleer", n_rk=2, theta=0.4):
+    """Sheared, rotated planar channel: BC 400 ends, symmetry (300) below, isothermal wall (302) above."""
+    ni, nj = 90, 40
+    node = cases.cartesian_nodes(ni, nj, 0.0, 1.0, 0.0, 0.4)
+    node[..., 1] += 0.05 * np.sin(math.pi * node[..., 0]) * (0.4 - node[..., 1])
+    ct, st = math.cos(theta), math.sin(theta)
+    rot = node.copy()
+    rot[..., 0] = ct * node[..., 0] - st * node[..., 1]
+    rot[..., 1] = st * node[..., 0] + ct * node[..., 1]
+    from q2d_amd.geometry import BlockGeometry
+    geo = BlockGeometry(rot)
+    P = np.zeros((nj + 4, ni + 4, 4))
+    c = 0.25 * (node[1:, 1:] + node[1:, :-1] + node[:-1, 1:] + node[:-1, :-1])
+    x, y = c[..., 0], c[..., 1]
+    u = 1.5 * (1.0 - (y / 0.4) ** 2) + 0.1 * np.sin(6 * x)
+    v = 0.05 * np.cos(5 * x) * y
+    P[2:-2, 2:-2, 0] = 1.0 + 0.1 * np.exp(-((x - 0.4) ** 2 + (y - 0.2) ** 2) / 0.01)
+    P[2:-2, 2:-2, 1] = ct * u - st * v
+    P[2:-2, 2:-2, 2] = st * u + ct * v
+    P[2:-2, 2:-2, 3] = 1.0 + 0.05 * np.cos(3 * x)
+    cases.fill_initial_ghosts(P)
+    recs = cases.face_records(1, ni, nj, {1: 400, 2: 400, 3: 300, 4: 302}, props={4: {"T": 1.2}})
+    spec = cases.CaseSpec(n_rk=n_rk, limiter=limiter, riemann=riemann, time_accurate=True, model=1, source="none",
+                          cfl=0.5, vnn=0.3, mu_ref=2e-3, T_ref=1.0, S_suth=0.38, Pr=0.72, itermax=100)
+    pb = cases.Problem(blocks=[cases.Block(geo, P)], bc=recs, cases=[spec])
+    return cases.finish(pb, [geo])
+
+
+def _run_both(oracle_mod, pb, steps):
+    g = GpuSolver(pb)
+    go = g.run(steps)
+    o = oracle_mod.Oracle(pb)
+    rc, oo = o.run(steps, stop_on_todo=False)
+    assert rc == 0
+    return g, go, o, oo
+
+
+def test_axisymmetric_pipe_fp64_bit_exact(oracle_mod):
+    """Axisymmetric laminar pipe (axis 300, isothermal wall 302, inflow 405, outflow 400), local dt."""
+    pb = cases.axisymmetric_pipe(seed=1, ni=124, nj=40, first_cell=2e-4, reynolds=2e4, precision="fp64",
+                                 steps=150)
+    g, go, o, oo = _run_both(oracle_mod, pb, 150)
+    assert go[0]["iter"] == oo[0]["iter"] == 150
+    assert np.array_equal(g.state(0)[2:-2, 2:-2], o.state(0)[2:-2, 2:-2]), \
+        np.abs(g.state(0)[2:-2, 2:-2] - o.state(0)[2:-2, 2:-2]).max()
+    assert np.array_equal(g.dtlocal(0), o.dtlocal(0))
+    for k in range(4):
+        assert abs(go[0]["residual"][k] - oo[0]["residual"][k]) <= 1e-12 * abs(oo[0]["residual"][k])
+
+
+@pytest.mark.parametrize("rie,lim,n_rk", [("HLLE", "vanleer", 2), ("HLLC", "minmod", 3)])
+def test_planar_viscous_fp64_bit_exact(oracle_mod, rie, lim, n_rk):
+    pb = _planar_viscous(rie, lim, n_rk)
+    g, go, o, oo = _run_both(oracle_mod, pb, 80)
+    assert go[0]["dtglobal"] == oo[0]["dtglobal"]
+    assert np.array_equal(g.state(0)[2:-2, 2:-2], o.state(0)[2:-2, 2:-2]), \
+        np.abs(g.state(0)[2:-2, 2:-2] - o.state(0)[2:-2, 2:-2]).max()
+
+
+def test_axisymmetric_pipe_mixed_within_tolerance(oracle_mod):
+    make = lambda prec: cases.axisymmetric_pipe(seed=2, ni=124, nj=40, first_cell=2e-4, reynolds=2e4,
+                                                precision=prec, steps=200)
+    g = GpuSolver(make("mixed"))
+    go = g.run(200)
+    o = oracle_mod.Oracle(make("fp64"))
+    rc, oo = o.run(200, stop_on_todo=False)
+    assert rc == 0 and go[0]["iter"] == oo[0]["iter"] == 200
+    a, r = g.state(0)[2:-2, 2:-2, 0], o.state(0)[2:-2, 2:-2, 0]
+    err = np.abs(a - r).sum() / np.abs(r).sum()
+    assert 0 < err <= TOL_L1, err
+
+
+def test_viscous_and_inviscid_cases_in_one_batch(oracle_mod):
+    """A laminar and an Euler case share one launch; each equals its own oracle run bit for bit."""
+    pv = cases.axisymmetric_pipe(seed=3, ni=64, nj=24, first_cell=5e-4, reynolds=1e4, precision="fp64", steps=60)
+    pe = cases.riemann_planar(seed=4, ni=128, nj=8, steps=60)
+    batch = cases.axisymmetric_pipe(seed=3, ni=64, nj=24, first_cell=5e-4, reynolds=1e4, precision="fp64", steps=60)
+    batch.add(cases.riemann_planar(seed=4, ni=128, nj=8, steps=60))
+    g = GpuSolver(batch)
+    g.run(60)
+    for k, pb in enumerate((pv, pe)):
+        o = oracle_mod.Oracle(pb)
+        rc, _ = o.run(60, stop_on_todo=False)
+        assert rc == 0
+        assert np.array_equal(g.state(k)[2:-2, 2:-2], o.state(0)[2:-2, 2:-2]), k
+
+
+def test_laminar_connection_rejected():
+    pb = cases.planar_shock(seed=0, n=32, steps=5)
+    for c in pb.cases:
+        c.model, c.mu_ref = 1, 1e-3
+    with pytest.raises(Q2DError) as ei:
+        GpuSolver(pb)
+    assert ei.value.code == -6
