@@ -30,6 +30,20 @@ BYTES_PER_CELL_STAGE = 168      # state in 32 + RK base 32 + state out 32 + geom
 BYTES_DT_PER_CELL = 80          # dt pass: state 32 + M 32 + dl 16
 
 
+PROFILE = ROOT / "profiles" / "k_stage_planar_shock_4x1024_rk3.md"
+
+
+def measured_traffic(n):
+    """HBM bytes per stage launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this
+    workload (copy-kernel calibrated); None for other sizes or when the summary is absent."""
+    if n != 1024 or not PROFILE.exists():
+        return {}
+    for line in PROFILE.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch:"):
+            return json.loads(line.split(":", 1)[1])
+    return {}
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -111,8 +125,8 @@ def main():
         print(json.dumps({"copy_GBps": gbs, "ms_per_copy": ms, "bytes_per_copy": 2 << 30}))
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is designed (DESIGN.md) "
-                                   "but not built", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this build runs on one GPU only: the multi-GPU sharded step is not built",
+                          "n_gpus": args.gpus}))
         sys.exit(2)
 
     from q2d_amd import cases
@@ -137,11 +151,13 @@ def main():
     n_rk = plain["n_rk"]
     bytes_update = n_rk * BYTES_PER_CELL_STAGE + BYTES_DT_PER_CELL
 
-    def roofline(r):
+    traffic = measured_traffic(args.n)
+
+    def roofline(r, form):
         alg = BYTES_PER_CELL_STAGE * r["ncells"]            # bytes of one stage launch
         ach = alg / (r["stage_kernel_ms"] * 1e-3) / 1e9
         return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic.get(form),
                 "kernel": "k_stage", "kernel_ms": round(r["stage_kernel_ms"], 4),
                 "alg_bytes_per_launch": alg}
 
@@ -162,13 +178,13 @@ def main():
                                f"CFL 0.5, BC 101/201/400, device ghost exchange",
                    "cells": plain["ncells"], "parallelism": "single GPU"},
         "plain_fp64": {"value": plain["cell_updates_per_s"], "ms_per_step": plain["ms_per_step"],
-                       "roofline": roofline(plain), "dtype": "f64"},
+                       "roofline": roofline(plain, "fp64"), "dtype": "f64"},
         "mixed_vs_plain": round(mixed["cell_updates_per_s"] / plain["cell_updates_per_s"], 3),
         "mixed_err_vs_fp64": {"rel_L1_rho": l1, "rel_Linf_rho": linf, "steps": args.warmup + args.steps},
         "alg_bytes_per_cell_update": bytes_update,
         "effective_GBps": {"plain": plain["cell_updates_per_s"] * bytes_update / 1e9,
                            "mixed": mixed["cell_updates_per_s"] * bytes_update / 1e9},
-        "roofline": roofline(mixed),
+        "roofline": roofline(mixed, "mixed"),
     }
     if not args.no_cpu:
         log("bench: CPU baseline ...")

@@ -175,43 +175,55 @@ const char *q2d_last_error(void);
 int q2d_device_count(int *count);
 
 /* Create the device-resident solver for a problem (all cases). Copies every
- * array to device memory; the caller may free its buffers afterwards. */
+ * array to device memory; the caller may free its buffers afterwards.
+ * Replaces the device side of the set-up before the time loop: the arrays
+ * Allocate_Data fills (Mod_Allocate_Data.f90:86-180), Setup_Metrics
+ * (Mod_Metrics.f90:11-148) and the BC list read by IO_BC.f90:163-...; the
+ * scheme keys bound by Assign_Setup (Assign_Setup.f90:11-62). */
 int q2d_gpu_create(const q2d_problem *prob, q2d_handle **out);
 int q2d_gpu_destroy(q2d_handle *h);
 
-/* Exactly one Explicit_Step for every active case; out[] has ncases entries
+/* Exactly one Explicit_Step (Mod_Explicit.f90:10-174, called from Q2D_solve,
+ * Wrap_Solve.f90:19-32) for every active case; out[] has ncases entries
  * (may be NULL).  Synchronises with the device. */
 int q2d_gpu_step(q2d_handle *h, q2d_step_out *out);
 
 /* nsteps explicit steps without host synchronisation in between (each step a
- * replay of one captured HIP graph).  Per-case stop conditions are evaluated on
- * the device: a case whose todo became 3 is frozen.  out[] as for step. */
+ * replay of one captured HIP graph): the host's time loop around Explicit_Step
+ * (main.f90:48 -> Wrap_Solve.f90:24) with its TODO test (Mod_Explicit.f90:
+ * 120-168) evaluated on the device: a case whose todo became 3 is frozen.
+ * out[] as for step. */
 int q2d_gpu_run(q2d_handle *h, int32_t nsteps, q2d_step_out *out);
 
 /* Copy block `block` (0-based global index) state to/from P in the reference
- * layout P(nprim,1-gc:ni+gc,1-gc:nj+gc). */
+ * layout P(nprim,1-gc:ni+gc,1-gc:nj+gc): what the host needs for a solution
+ * write or a restart (Wrap_Postprocess.f90:67-113; domain%blk(b)%P). */
 int q2d_gpu_download_state(q2d_handle *h, int32_t block, double *P);
 int q2d_gpu_upload_state(q2d_handle *h, int32_t block, const double *P);
 
-/* Local time step of the last step, dtlocal(1:ni,1:nj). */
+/* Local time step of the last step, domain%blk(b)%dtlocal(1:ni,1:nj)
+ * (Mod_dt.f90:46). */
 int q2d_gpu_download_dt(q2d_handle *h, int32_t block, double *dtlocal);
 
-/* Kernel-time accounting: when enabled, every stage-kernel launch is bracketed
- * by HIP events on the solver's stream; stats returns the number of timed
- * stage launches and their summed duration in milliseconds. */
+/* Kernel-time accounting (no counterpart in the reference): when enabled,
+ * every stage-kernel launch carries start/stop events stamped from its own
+ * dispatch; stats returns the number of timed stage launches and their summed
+ * duration in milliseconds.  Steps run eagerly (not from the graph) while on. */
 int q2d_gpu_set_timing(q2d_handle *h, int32_t enable);
 int q2d_gpu_timing_stats(q2d_handle *h, int64_t *launches, double *ms_total);
 
 /* Launch-shape control of the stage kernel (rows per work item, waves per
- * workgroup).  0 keeps the current value.  Used by the tile autotuner. */
+ * workgroup); no counterpart in the reference.  0 keeps the current value.
+ * Used by the tile autotuner. */
 int q2d_gpu_set_tiling(q2d_handle *h, int32_t rows_per_item, int32_t waves_per_group);
 int q2d_gpu_get_tiling(q2d_handle *h, int32_t *rows_per_item, int32_t *waves_per_group);
 
-/* HBM reference: time `reps` launches of a plain device copy of nbytes (read nbytes + write
- * nbytes); used to measure the box's achievable bandwidth and to calibrate HBM byte counters. */
+/* HBM reference (no counterpart in the reference): time `reps` launches of a plain device copy
+ * of nbytes (read nbytes + write nbytes); measures the achievable bandwidth and calibrates HBM
+ * byte counters. */
 int q2d_hbm_copy(int64_t nbytes, int32_t reps, double *ms_per_copy);
 
-/* Work-item count and cell count of the current launch. */
+/* Work-item count and cell count of the current launch (no counterpart in the reference). */
 int q2d_gpu_info(q2d_handle *h, int64_t *cells, int64_t *work_items);
 
 #ifdef __cplusplus

@@ -16,8 +16,15 @@
  *   E0         = cv*T + 0.5*|V|*|V|                               (E0)
  *   prim2cons  = (rho, rho*u, rho*v, rho*(cv*T + 0.5*(u*u+v*v)))
  *   cons2prim  : u=U2/rho, v=U3/rho, e=U4/rho-0.5*(u*u+v*v), T=e/cv, p=rho*R*T
+ * Laminar transport (model 1), the closure of co_k_mi_lam_Wilke(_expr) /
+ * f_laminarViscosity for one species:
+ *   mu(T)      = mu_ref*(x*sqrt(x))*(T_ref+S)/(T+S),  x = T/T_ref  (Sutherland)
+ *   k(mu)      = mu*cp/Pr
  * Parity of these closures with FLINT is unpinned: FLINT is an un-vendored
- * submodule (.gitmodules:4-6) with no pinned revision.
+ * submodule (.gitmodules:4-6) with no pinned revision; its tables interpolate
+ * linearly between integer kelvins, which this closed form does not.
+ * Everything above the closures is pinned by tests/ (golden Shu-Osher vectors
+ * of the reference binary, closed forms, invariants).
  */
 #include "q2d_oracle.h"
 

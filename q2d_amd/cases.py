@@ -376,14 +376,18 @@ def _tanh_wall_cluster(nj, first):
 
 
 def axisymmetric_pipe(seed=0, ni=2048, nj=512, first_cell=1e-5, reynolds=1e5, mach=None, radius=0.25,
-                      T_wall=None, precision="mixed", n_rk=3, limiter="vanleer", riemann="HLLC", cfl=0.5,
-                      vnn=0.15, steps=2000):
+                      T_wall=1.0, precision="mixed", n_rk=2, limiter="vanleer", riemann="HLLC", cfl=0.5,
+                      vnn=0.2, steps=2000):
     """Axisymmetric laminar Navier-Stokes: supersonic inflow (BC 405, face 1) into a pipe of length 1 and
     radius `radius`; axis r = 0 on j = 0 (BC 300, face 3), isothermal no-slip wall on r = radius (BC 302,
     face 4), supersonic outflow (BC 400, face 2).  Radial tanh clustering at the wall with a first cell of
     `first_cell` (lengths in units of the pipe length L = 1).  Non-dimensional gas R = 1, cp = 3.5 at
-    rho = p = T = 1; Sutherland viscosity with S / T_ref = 110.4 / 288.15, Pr = 0.72, mu_inf from Re_L.
-    Mach ~ U[1.5, 3] from the seed unless given.  Local time step."""
+    rho = p = T = 1; Sutherland viscosity with S / T_ref = 110.4 / 273.15, Pr = 0.72, mu_inf from Re_L;
+    wall at T_wall (default the free-stream temperature).  Mach ~ U[1.5, 3] from the seed unless given.
+    Local time step, CFL 0.5, SSP-RK2.  VNN 0.2 rather than the reference default 0.3
+    (Register_Numerics.f90:35): the VNN limit uses the kinematic viscosity, while heat conduction diffuses
+    at gamma/Pr = 1.94 times that rate, so 0.3 puts the wall cells past SSP-RK2's diffusive stability
+    limit (0.3 * 1.94 > 0.5); the explicit step then fails at the inflow/wall corner in half the seeds."""
     rng = np.random.default_rng(seed)
     if mach is None:
         mach = float(rng.uniform(1.5, 3.0))
@@ -391,8 +395,6 @@ def axisymmetric_pipe(seed=0, ni=2048, nj=512, first_cell=1e-5, reynolds=1e5, ma
     a = math.sqrt(gam)
     U = mach * a
     mu_inf = U * 1.0 / reynolds
-    if T_wall is None:
-        T_wall = 1.0 + 0.5 * (gam - 1.0) * mach * mach * 0.85      # near the laminar recovery temperature
     xs = np.arange(ni + 1) / ni
     rs = radius * _tanh_wall_cluster(nj, first_cell / radius)
     node = np.empty((nj + 1, ni + 1, 2))
@@ -407,7 +409,7 @@ def axisymmetric_pipe(seed=0, ni=2048, nj=512, first_cell=1e-5, reynolds=1e5, ma
     recs = face_records(1, ni, nj, {1: 405, 2: 400, 3: 300, 4: 302}, props=props)
     spec = CaseSpec(n_rk=n_rk, limiter=limiter, riemann=riemann, time_accurate=False, model=1,
                     source="axisymmetric", precision=precision, cfl=cfl, vnn=vnn, mu_ref=mu_inf, T_ref=1.0,
-                    S_suth=110.4 / 288.15, Pr=0.72, itermax=steps)
+                    S_suth=110.4 / 273.15, Pr=0.72, itermax=steps)
     blk = Block(geo, P)
     pb = Problem(blocks=[blk], bc=recs, cases=[spec],
                  meta={"name": "axisymmetric_pipe", "mach": mach, "reynolds": reynolds, "T_wall": T_wall})
