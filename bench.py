@@ -59,11 +59,13 @@ def cpu_info():
     return "unknown"
 
 
-def time_form(problem_fn, precision, steps, warmup):
+def time_form(problem_fn, precision, steps, warmup, tune=False):
     from q2d_amd import GpuSolver
     pb = problem_fn(precision)
     g = GpuSolver(pb)
     ncells = pb.ncells()
+    if tune:
+        g.autotune()                               # tile shape measured on a scratch solver (cached)
     g.run(warmup)                                  # graph capture + warm-up steps (synchronised)
     t0 = time.perf_counter()
     out = g.run(steps)                             # K steps, synchronised at the end
@@ -105,6 +107,67 @@ def cpu_baseline(problem_fn, steps):
                       f"workload, {steps} SSP-RK3 steps ({sec:.1f} s); CPU: {cpu_info()}"}
 
 
+def other_workload(args):
+    """--workload axi | batch: one JSON line for the laminar axisymmetric case (2048x512) or the batch of
+    64 uneven planar / quasi-2D cases, both arithmetic forms from the same initial state."""
+    from q2d_amd import cases
+    if args.workload == "axi":
+        def problem_fn(precision):
+            return cases.axisymmetric_pipe(seed=args.seed, precision=precision, steps=1 << 30)
+        desc = ("axisymmetric laminar Navier-Stokes pipe, 2048x512, tanh wall clustering (first cell 1e-5 L), "
+                "Re_L 1e5, Sutherland, Pr 0.72, isothermal wall, axis BC, MUSCL vanleer, HLLC, SSP-RK3, local dt, "
+                "CFL 0.5, VNN 0.2")
+        tune = False
+    else:
+        def problem_fn(precision):
+            return cases.batch_uneven(seed=args.seed, ncases=64, steps=1 << 30, precision=precision)
+        desc = ("64 independent planar / quasi-2D cases, 128x64 .. 2048x1024 log-uniform, Riemann or uniform "
+                "inflow, HLL or HLLC per case, SSP-RK2, one persistent launch per stage, autotuned tiles")
+        tune = True
+    log(f"bench: {args.workload}: warmup {args.warmup}, steps {args.steps}")
+    plain, rho64 = time_form(problem_fn, "fp64", args.steps, args.warmup, tune)
+    log(f"bench: plain fp64  {plain['cell_updates_per_s'] / 1e9:.3f} G cell-updates/s  tiling {plain['tiling']}")
+    mixed, rho32 = time_form(problem_fn, "mixed", args.steps, args.warmup, tune)
+    log(f"bench: mixed       {mixed['cell_updates_per_s'] / 1e9:.3f} G cell-updates/s  tiling {mixed['tiling']}")
+    num = sum(np.abs(a - b).sum() for a, b in zip(rho32, rho64))
+    den = sum(np.abs(b).sum() for b in rho64)
+    linf = max(np.abs(a - b).max() for a, b in zip(rho32, rho64)) / max(np.abs(b).max() for b in rho64)
+    print(json.dumps({
+        "metric": "cell-updates/s batched-mixed vs plain FP64 at stated err",
+        "value": mixed["cell_updates_per_s"], "unit": "cell-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": mixed["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32 flux / f64 state",
+        "data": "synthetic (seeded)",
+        "config": {"workload": desc, "cells": plain["ncells"], "parallelism": "single GPU"},
+        "plain_fp64": {"value": plain["cell_updates_per_s"], "ms_per_step": plain["ms_per_step"],
+                       "stage_kernel_ms": plain["stage_kernel_ms"], "tiling": plain["tiling"]},
+        "mixed_stage_kernel_ms": mixed["stage_kernel_ms"], "mixed_tiling": mixed["tiling"],
+        "mixed_vs_plain": round(mixed["cell_updates_per_s"] / plain["cell_updates_per_s"], 3),
+        "mixed_err_vs_fp64": {"rel_L1_rho": num / den, "rel_Linf_rho": linf, "steps": args.warmup + args.steps},
+    }))
+
+
+def serve_workload(nreq, seed, log_fn=None):
+    """Poisson stream of steady compression-ramp solves (wedge ~ U[5,15] deg, Mach ~ U[1.5,4], grids
+    log-uniform 64x64 .. 4096x1024), each run in FP64 to a 1e-6 drop of residual(1); continuous batching
+    on one device-resident solver under a 64 GiB HBM budget.  The arrival rate is set for ~70 % occupancy from
+    a cost estimate (1,000 steps at 64x64 growing as cells^0.275, the oracle's measured convergence on
+    this family, at 4e9 cell-updates/s)."""
+    from q2d_amd import serving
+    probe = serving.poisson_stream(nreq, rate=1.0, seed=seed)
+    est = sum(r.problem.ncells() * 1000.0 * (r.problem.ncells() / 4096.0) ** 0.275 for r in probe) / 4e9
+    rate = 0.7 * nreq / max(est, 1e-3)
+    reqs = serving.poisson_stream(nreq, rate=rate, seed=seed)
+    srv = serving.Server(budget_bytes=64 << 30, chunk_steps=50)
+    res, stats = srv.serve(reqs, keep_state=False, log=log_fn)
+    srv.close()
+    stats.update({"arrival_rate_per_s": rate, "budget_GiB": 64, "chunk_steps": 50,
+                  "grid_cells_min": min(r.problem.ncells() for r in reqs),
+                  "grid_cells_max": max(r.problem.ncells() for r in reqs),
+                  "steps_total": int(sum(x.iters for x in res))})
+    return stats
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -116,6 +179,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--form", choices=["both", "fp64", "mixed"], default="both",
                     help="profiling aid: run only one arithmetic form (no JSON line)")
+    ap.add_argument("--requests", type=int, default=24, help="serving: requests in the Poisson stream")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
+    ap.add_argument("--workload", choices=["shock", "axi", "batch", "serve"], default="shock",
+                    help="shock: the headline workload (default); axi, batch: the laminar axisymmetric case and "
+                         "the uneven 64-case batch, reported on their own JSON line")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only the HBM reference copy kernel (1 GiB, 10 launches) and print GB/s")
     args = ap.parse_args()
@@ -128,6 +196,16 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU only: the multi-GPU sharded step is not built",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+
+    if args.workload == "serve":
+        stats = serve_workload(args.requests, args.seed, log)
+        print(json.dumps({"metric": "p50/p99 solve latency (serving, continuous batching)", "value": stats["p50_s"],
+                          "unit": "s", "n_gpus": 1, "higher_is_better": False, "data": "synthetic (seeded)",
+                          "dtype": "f64", "serving": stats}))
+        return
+    if args.workload != "shock":
+        other_workload(args)
+        return
 
     from q2d_amd import cases
 
@@ -186,6 +264,10 @@ def main():
                            "mixed": mixed["cell_updates_per_s"] * bytes_update / 1e9},
         "roofline": roofline(mixed, "mixed"),
     }
+    if not args.no_serve:
+        log("bench: serving stream ...")
+        line["serving"] = serve_workload(args.requests, args.seed)
+        log(f"bench: serving p50 {line['serving']['p50_s']:.2f} s, p99 {line['serving']['p99_s']:.2f} s")
     if not args.no_cpu:
         log("bench: CPU baseline ...")
         line["cpu_baseline"] = cpu_baseline(problem_fn, args.cpu_steps)

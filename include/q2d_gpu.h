@@ -146,6 +146,8 @@ typedef struct q2d_case_desc {
   int32_t    reserved;
   double     res_threshold; /* res-threshold (absolute, on residual(1)) */
   double     time_threshold;
+  double     res_drop;      /* > 0: also stop when residual(1) <= res_drop * residual(1) of the case's
+                               first step (a relative drop; serving); 0: off, as in the reference */
 } q2d_case_desc;
 
 typedef struct q2d_problem {
@@ -182,6 +184,18 @@ int q2d_device_count(int *count);
  * scheme keys bound by Assign_Setup (Assign_Setup.f90:11-62). */
 int q2d_gpu_create(const q2d_problem *prob, q2d_handle **out);
 int q2d_gpu_destroy(q2d_handle *h);
+
+/* Continuous batching (no counterpart in the reference, whose domain is fixed
+ * at set-up): append the cases of `prob` to a live handle (their case indices
+ * start at *first_case; block indices continue after the existing blocks), or
+ * remove one case, releasing its device memory.  The other cases' state and
+ * results are untouched; a removed case keeps its index and reports todo 0.
+ * q2d_gpu_create accepts an empty problem (0 cases) for a handle filled later. */
+int q2d_gpu_add_cases(q2d_handle *h, const q2d_problem *prob, int32_t *first_case);
+int q2d_gpu_remove_case(q2d_handle *h, int32_t case_id);
+/* Case and block counts of the handle (removed ones included) and the bytes of
+ * device memory its blocks hold. */
+int q2d_gpu_case_count(q2d_handle *h, int32_t *ncases, int32_t *nblocks, int64_t *device_bytes);
 
 /* Exactly one Explicit_Step (Mod_Explicit.f90:10-174, called from Q2D_solve,
  * Wrap_Solve.f90:19-32) for every active case; out[] has ncases entries

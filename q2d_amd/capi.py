@@ -59,7 +59,7 @@ class BCEntry(C.Structure):
 
 class CaseDesc(C.Structure):
     _fields_ = [("scheme", Scheme), ("itermax", C.c_int32), ("reserved", C.c_int32),
-                ("res_threshold", C.c_double), ("time_threshold", C.c_double)]
+                ("res_threshold", C.c_double), ("time_threshold", C.c_double), ("res_drop", C.c_double)]
 
 
 class ProblemC(C.Structure):
@@ -111,6 +111,7 @@ def to_problem_c(problem):
         s.cfl, s.vnn, s.gas_R, s.gas_cp = c.cfl, c.vnn, c.gas_R, c.gas_cp
         s.mu_ref, s.T_ref, s.S_suth, s.Pr = c.mu_ref, c.T_ref, c.S_suth, c.Pr
         cases[k].itermax, cases[k].res_threshold, cases[k].time_threshold = c.itermax, c.res_threshold, c.time_threshold
+        cases[k].res_drop = c.res_drop
     pc = ProblemC()
     pc.nblocks, pc.nbc, pc.ncases = nb, nbc, nc
     pc.blocks = C.cast(blocks, C.POINTER(BlockDesc))
@@ -129,6 +130,9 @@ SIGNATURES = {
     "q2d_device_count": ([C.POINTER(C.c_int)], C.c_int),
     "q2d_gpu_create": ([C.POINTER(ProblemC), C.POINTER(C.c_void_p)], C.c_int),
     "q2d_gpu_destroy": ([C.c_void_p], C.c_int),
+    "q2d_gpu_add_cases": ([C.c_void_p, C.POINTER(ProblemC), C.POINTER(C.c_int32)], C.c_int),
+    "q2d_gpu_remove_case": ([C.c_void_p, C.c_int32], C.c_int),
+    "q2d_gpu_case_count": ([C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int64)], C.c_int),
     "q2d_gpu_step": ([C.c_void_p, C.POINTER(StepOut)], C.c_int),
     "q2d_gpu_run": ([C.c_void_p, C.c_int32, C.POINTER(StepOut)], C.c_int),
     "q2d_gpu_download_state": ([C.c_void_p, C.c_int32, C.c_void_p], C.c_int),
@@ -189,7 +193,27 @@ class GpuSolver:
         _check(self.lib, self.lib.q2d_gpu_create(C.byref(pc), C.byref(h)))
         self.h = h
         self.ncases = len(problem.cases)
-        self._out = (StepOut * self.ncases)()
+        self.blocks = list(problem.blocks)        # blocks by global index (grows with add_cases)
+        self._out = (StepOut * max(1, self.ncases))()
+
+    def add_cases(self, problem):
+        """Append the cases of `problem` to the live solver; returns (first case index, first block index)."""
+        pc, keep = to_problem_c(problem)
+        first = C.c_int32(0)
+        nb0 = len(self.blocks)
+        _check(self.lib, self.lib.q2d_gpu_add_cases(self.h, C.byref(pc), C.byref(first)))
+        self.blocks.extend(problem.blocks)
+        self.ncases += len(problem.cases)
+        self._out = (StepOut * max(1, self.ncases))()
+        return first.value, nb0
+
+    def remove_case(self, case):
+        _check(self.lib, self.lib.q2d_gpu_remove_case(self.h, int(case)))
+
+    def device_bytes(self):
+        nc, nb, by = C.c_int32(0), C.c_int32(0), C.c_int64(0)
+        _check(self.lib, self.lib.q2d_gpu_case_count(self.h, C.byref(nc), C.byref(nb), C.byref(by)))
+        return by.value
 
     def close(self):
         if getattr(self, "h", None):
@@ -206,16 +230,16 @@ class GpuSolver:
         rc = self.lib.q2d_gpu_step(self.h, self._out)
         if raise_on_error:
             _check(self.lib, rc)
-        return [o.as_dict() for o in self._out]
+        return [o.as_dict() for o in self._out[:self.ncases]]
 
     def run(self, nsteps, raise_on_error=True):
         rc = self.lib.q2d_gpu_run(self.h, int(nsteps), self._out)
         if raise_on_error:
             _check(self.lib, rc)
-        return [o.as_dict() for o in self._out]
+        return [o.as_dict() for o in self._out[:self.ncases]]
 
     def state(self, block):
-        b = self.problem.blocks[block]
+        b = self.blocks[block]
         P = np.empty((b.nj + 4, b.ni + 4, 4))
         _check(self.lib, self.lib.q2d_gpu_download_state(self.h, block, P.ctypes.data_as(C.c_void_p)))
         return P
@@ -225,7 +249,7 @@ class GpuSolver:
         _check(self.lib, self.lib.q2d_gpu_upload_state(self.h, block, P.ctypes.data_as(C.c_void_p)))
 
     def dtlocal(self, block):
-        b = self.problem.blocks[block]
+        b = self.blocks[block]
         d = np.empty((b.nj, b.ni))
         _check(self.lib, self.lib.q2d_gpu_download_dt(self.h, block, d.ctypes.data_as(C.c_void_p)))
         return d
@@ -241,6 +265,13 @@ class GpuSolver:
 
     def set_tiling(self, rows_per_item=0, waves_per_group=0):
         _check(self.lib, self.lib.q2d_gpu_set_tiling(self.h, int(rows_per_item), int(waves_per_group)))
+
+    def autotune(self, steps=4, use_cache=True, verbose=False):
+        """Pick the stage-kernel tiling for this problem (q2d_amd.autotune) and apply it."""
+        from .autotune import tune
+        r, w, _ = tune(self.problem, steps=steps, use_cache=use_cache, verbose=verbose)
+        self.set_tiling(r, w)
+        return r, w
 
     def tiling(self):
         r = C.c_int32(0)

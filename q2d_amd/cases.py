@@ -52,6 +52,7 @@ class CaseSpec:
     itermax: int = 1 << 30
     res_threshold: float = -1.0
     time_threshold: float = 1e300
+    res_drop: float = 0.0
 
 
 @dataclass
@@ -83,6 +84,17 @@ class Problem:
 
     def ncells(self):
         return sum(b.ni * b.nj for b in self.blocks)
+
+    def case(self, k):
+        """Case k alone, as a one-case Problem (its blocks, records and scheme)."""
+        blocks = [replace(b, case_id=0) for b in self.blocks if b.case_id == k]
+        bc = []
+        for r in self.bc:
+            if r["case_id"] == k:
+                r2 = dict(r)
+                r2["case_id"] = 0
+                bc.append(r2)
+        return Problem(blocks=blocks, bc=bc, cases=[self.cases[k]], meta={"from_case": k})
 
     def add(self, other: "Problem"):
         """Append the cases of another problem (batching)."""
@@ -376,7 +388,7 @@ def _tanh_wall_cluster(nj, first):
 
 
 def axisymmetric_pipe(seed=0, ni=2048, nj=512, first_cell=1e-5, reynolds=1e5, mach=None, radius=0.25,
-                      T_wall=1.0, precision="mixed", n_rk=2, limiter="vanleer", riemann="HLLC", cfl=0.5,
+                      T_wall=1.0, precision="mixed", n_rk=3, limiter="vanleer", riemann="HLLC", cfl=0.5,
                       vnn=0.2, steps=2000):
     """Axisymmetric laminar Navier-Stokes: supersonic inflow (BC 405, face 1) into a pipe of length 1 and
     radius `radius`; axis r = 0 on j = 0 (BC 300, face 3), isothermal no-slip wall on r = radius (BC 302,
@@ -384,10 +396,13 @@ def axisymmetric_pipe(seed=0, ni=2048, nj=512, first_cell=1e-5, reynolds=1e5, ma
     `first_cell` (lengths in units of the pipe length L = 1).  Non-dimensional gas R = 1, cp = 3.5 at
     rho = p = T = 1; Sutherland viscosity with S / T_ref = 110.4 / 273.15, Pr = 0.72, mu_inf from Re_L;
     wall at T_wall (default the free-stream temperature).  Mach ~ U[1.5, 3] from the seed unless given.
-    Local time step, CFL 0.5, SSP-RK2.  VNN 0.2 rather than the reference default 0.3
-    (Register_Numerics.f90:35): the VNN limit uses the kinematic viscosity, while heat conduction diffuses
-    at gamma/Pr = 1.94 times that rate, so 0.3 puts the wall cells past SSP-RK2's diffusive stability
-    limit (0.3 * 1.94 > 0.5); the explicit step then fails at the inflow/wall corner in half the seeds."""
+    Local time step, CFL 0.5, SSP-RK3, VNN 0.2.  Two departures from the reference defaults, both
+    for stability of this explicit scheme: (1) the CFL limit is taken per direction (Mod_dt.f90:76-84),
+    so on the near-square cells of the stretched mesh the 2D Courant number reaches 2 x 0.5 = 1, which
+    SSP-RK2 with MUSCL does not survive (it fails after ~350 steps at 2048x512); SSP-RK3 does.
+    (2) VNN 0.2 rather than 0.3 (Register_Numerics.f90:35): the VNN limit uses the kinematic
+    viscosity while heat conduction diffuses gamma/Pr = 1.94 times faster, so 0.3 puts the wall cells
+    past the diffusive stability limit and the step fails at the inflow/wall corner."""
     rng = np.random.default_rng(seed)
     if mach is None:
         mach = float(rng.uniform(1.5, 3.0))

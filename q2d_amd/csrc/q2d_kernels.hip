@@ -21,6 +21,7 @@
 
 #include <algorithm>
 #include <cfloat>
+#include <cstddef>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -37,6 +38,7 @@ namespace {
 constexpr int kWave = 64;
 constexpr int kOwn = 60;          // owned columns per strip
 constexpr int kMaxCases = 1 << 20;
+constexpr int kTimedSteps = 8;    // steps per timing graph (event pool size)
 // Fuse the next step's Compute_dt into the last RK stage (saves the separate dt pass over the
 // state).  The FP64 stage kernel keeps a separate dt pass: the fused version costs it registers.
 #ifndef Q2D_FUSE_DT_FP64
@@ -61,6 +63,7 @@ struct DevCase {
   double cfl, vnn, R, cp, cv, gam;
   double res_threshold, time_threshold;
   double mu_ref, T_ref, S_suth, Pr;     // Sutherland viscosity, Prandtl number (model 1)
+  double res_drop;                      // > 0: stop when residual(1) <= res_drop * residual(1) of step 1
 };
 
 struct DevCtrl {
@@ -71,6 +74,7 @@ struct DevCtrl {
   double residual[4];
   int iter, iter_from_call, todo, active;
   int cur, status, stage_count, dtsel;
+  double res_first;                     // residual(1) of the case's first step
 };
 
 struct DevBlock {
@@ -1231,8 +1235,10 @@ __global__ __launch_bounds__(256) void k_finish(DevCtrl* ctrl, const DevCase* ca
   }
   if (threadIdx.x == 0) {
     for (int v = 0; v < 4; ++v) K.residual[v] = sqrt(sh[v][0]);
+    if (K.iter == 1) K.res_first = K.residual[0];
     const bool endsim = (K.iter_from_call >= C.itermax) || (K.residual[0] <= C.res_threshold) ||
-                        (K.time >= C.time_threshold);
+                        (K.time >= C.time_threshold) ||
+                        (C.res_drop > 0.0 && K.iter > 1 && K.residual[0] <= C.res_drop * K.res_first);
     K.todo = endsim ? 3 : 1;
     K.cur = (K.cur + 1 + ((C.n_rk - 1) & 1)) % 3;          // buffer written by the last stage
     K.stage_count += 4;
@@ -1292,6 +1298,10 @@ struct q2d_handle {
   int* d_vrec = nullptr;
   std::vector<hipEvent_t> tev;   // per-launch event pairs of the timed stage launches of one step
   int tslot = 0;
+  bool capturing = false;
+  std::vector<char> case_live;   // removed cases keep their index but no blocks, records or items
+  bool prec_set = false;
+  size_t arena_bytes = 0;
   hipStream_t stream = nullptr;
   hipGraphExec_t graph = nullptr;
   int rows_per_item = 16, waves_per_group = 4, max_groups = 0, xcd_remap = 1;
@@ -1311,7 +1321,7 @@ int build_items(q2d_handle* h) {
   for (int pass = 0; pass < 4; ++pass) {
     h->range[pass] = (int)h->items.size();
     for (int c = 0; c < h->ncases; ++c) {
-      if (h->dc[c].riemann != (pass & 1) || (h->dc[c].model > 0) != (pass >= 2)) continue;
+      if (!h->case_live[c] || h->dc[c].riemann != (pass & 1) || (h->dc[c].model > 0) != (pass >= 2)) continue;
       h->dc[c].item_begin = (int)h->items.size();
       for (int b = 0; b < h->nblocks; ++b) {
         if (h->hb[b].case_id != c) continue;
@@ -1328,14 +1338,17 @@ int build_items(q2d_handle* h) {
   HIPCHK(hipMalloc(&h->d_items, sizeof(Item) * std::max<size_t>(1, h->items.size())));
   HIPCHK(hipMalloc(&h->d_partials, sizeof(double) * 4 * std::max<size_t>(1, h->items.size())));
   HIPCHK(hipMemcpy(h->d_items, h->items.data(), sizeof(Item) * h->items.size(), hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(h->d_cases, h->dc.data(), sizeof(DevCase) * h->dc.size(), hipMemcpyHostToDevice));
+  (void)hipFree(h->d_cases); h->d_cases = nullptr;
+  HIPCHK(hipMalloc(&h->d_cases, sizeof(DevCase) * std::max<size_t>(1, h->dc.size())));
+  if (!h->dc.empty())
+    HIPCHK(hipMemcpy(h->d_cases, h->dc.data(), sizeof(DevCase) * h->dc.size(), hipMemcpyHostToDevice));
   if (h->graph) { (void)hipGraphExecDestroy(h->graph); h->graph = nullptr; }
   return Q2D_OK;
 }
 
-// With timing on, each stage launch carries start/stop events that the runtime stamps from the kernel's own
-// dispatch packet (hipExtLaunchKernelGGL), so the measured time is the kernel's execution alone, the same
-// interval a kernel-trace profiler reports.
+// With timing on, each stage launch is bracketed by an event pair on the solver's stream.  The timed steps
+// replay a graph captured with these event nodes, so the stage kernels run back to back exactly as in
+// the untimed graph; only the event nodes are added.
 template <typename FT, bool FUSE>
 int launch_stage(q2d_handle* h, int irk) {
   const int tpb = kWave * h->waves_per_group;
@@ -1353,9 +1366,9 @@ int launch_stage(q2d_handle* h, int irk) {
     const int* bf = h->d_bcface;
     const BCFlux* bq = h->d_bcflux;
 #define Q2D_STAGE_LAUNCH(RIE, VISC)                                                                               \
-    hipExtLaunchKernelGGL((k_stage<FT, RIE, FUSE, VISC>), dim3(groups), dim3(tpb), 0, h->stream, e0, e1, 0,       \
-                          h->d_blocks, h->d_cases, h->d_ctrl, h->d_items, lo, hi, bf, bq, h->d_partials, irk,    \
-                          h->xcd_remap)
+    hipLaunchKernelGGL((k_stage<FT, RIE, FUSE, VISC>), dim3(groups), dim3(tpb), 0, h->stream, h->d_blocks,        \
+                       h->d_cases, h->d_ctrl, h->d_items, lo, hi, bf, bq, h->d_partials, irk, h->xcd_remap)
+    if (e0) HIPCHK(hipEventRecord(e0, h->stream));
     switch (pass) {
       case 0: Q2D_STAGE_LAUNCH(0, false); break;
       case 1: Q2D_STAGE_LAUNCH(1, false); break;
@@ -1363,11 +1376,28 @@ int launch_stage(q2d_handle* h, int irk) {
       default: Q2D_STAGE_LAUNCH(1, true); break;
     }
 #undef Q2D_STAGE_LAUNCH
+    if (e1) HIPCHK(hipEventRecord(e1, h->stream));
+  }
+  return Q2D_OK;
+}
+
+// Sum the event-pair intervals of the stage launches issued since the last collection.
+int collect_timing(q2d_handle* h) {
+  const int n = h->tslot;
+  h->tslot = 0;
+  if (n == 0) return Q2D_OK;
+  HIPCHK(hipEventSynchronize(h->tev[2 * n - 1]));
+  for (int k = 0; k < n; ++k) {
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, h->tev[2 * k], h->tev[2 * k + 1]));
+    h->timed_ms += ms;
+    h->timed_launches += 1;
   }
   return Q2D_OK;
 }
 
 int launch_step(q2d_handle* h) {
+  if (h->ncases == 0) return Q2D_OK;
   const int n_items = (int)h->items.size();
   const int tpb = kWave * h->waves_per_group;
   int groups = (n_items + h->waves_per_group - 1) / h->waves_per_group;
@@ -1377,7 +1407,6 @@ int launch_step(q2d_handle* h) {
   const int bgroups = (h->nbc + 255) / 256;
   const int use_next =
       (h->dt_fused_valid && !h->any_visc && (h->precision_mixed ? kFuseDtMixed : kFuseDtFp64)) ? 1 : 0;
-  h->tslot = 0;
   hipLaunchKernelGGL(k_begin, dim3(cgroups), dim3(cthreads), 0, h->stream, h->d_ctrl, h->ncases, use_next);
   if (!use_next)
     hipLaunchKernelGGL(k_dt, dim3(groups), dim3(tpb), 0, h->stream, h->d_blocks, h->d_cases, h->d_ctrl, h->d_items,
@@ -1406,30 +1435,28 @@ int launch_step(q2d_handle* h) {
   }
   hipLaunchKernelGGL(k_finish, dim3(h->ncases), dim3(256), 0, h->stream, h->d_ctrl, h->d_cases, h->d_partials, h->ncases);
   HIPCHK(hipGetLastError());
-  if (h->timing && h->tslot > 0) {                   // stage launches ran back to back; read their stamps now
-    HIPCHK(hipEventSynchronize(h->tev[2 * h->tslot - 1]));
-    for (int k = 0; k < h->tslot; ++k) {
-      float ms = 0.f;
-      HIPCHK(hipEventElapsedTime(&ms, h->tev[2 * k], h->tev[2 * k + 1]));
-      h->timed_ms += ms;
-      h->timed_launches += 1;
-    }
+  if (h->timing && !h->capturing) {
+    int rc = collect_timing(h);
+    if (rc != Q2D_OK) return rc;
   }
   h->dt_fused_valid = true;
   return Q2D_OK;
 }
 
 int read_out(q2d_handle* h, q2d_step_out* out, int* first_err) {
+  *first_err = Q2D_OK;
+  if (h->ncases == 0) return Q2D_OK;
   std::vector<DevCtrl> ctl(h->ncases);
   HIPCHK(hipMemcpyAsync(ctl.data(), h->d_ctrl, sizeof(DevCtrl) * h->ncases, hipMemcpyDeviceToHost, h->stream));
   HIPCHK(hipStreamSynchronize(h->stream));
   *first_err = Q2D_OK;
   for (int c = 0; c < h->ncases; ++c) {
     const DevCtrl& k = ctl[c];
-    if (k.status != 0 && *first_err == Q2D_OK) *first_err = k.status;
+    if (h->case_live[c] && k.status != 0 && *first_err == Q2D_OK) *first_err = k.status;
     if (!out) continue;
     q2d_step_out& o = out[c];
     memset(&o, 0, sizeof(o));
+    if (!h->case_live[c]) continue;               // removed: all zero (todo 0)
     o.dtglobal = k.dtglobal; o.time = k.time;
     for (int v = 0; v < 4; ++v) o.residual[v] = k.residual[v];
     o.iter = k.iter; o.todo = k.todo; o.status = k.status;
@@ -1483,16 +1510,77 @@ int q2d_gpu_destroy(q2d_handle* h) {
   return Q2D_OK;
 }
 
-int q2d_gpu_create(const q2d_problem* prob, q2d_handle** out) {
-  if (!prob || !out) return fail(Q2D_E_ARG, "NULL argument");
-  *out = nullptr;
-  if (prob->nblocks <= 0 || prob->ncases <= 0 || !prob->blocks || !prob->cases)
-    return fail(Q2D_E_ARG, "empty problem");
-  if (prob->ncases > kMaxCases) return fail(Q2D_E_ARG, "too many cases");
-  int ndev = 0;
-  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(Q2D_E_NODEVICE, "no HIP device");
+namespace {
 
-  // ---- validation ----
+// Rebuild the per-face record index (bcface, DevBlock::bcf), the viscous-record list and the work items
+// from the live blocks and records, and upload every descriptor array.  Called after cases are added or
+// removed; the captured step graph is invalidated (build_items).
+int rebuild_descriptors(q2d_handle* h) {
+  h->bcface.clear();
+  h->vrec.clear();
+  std::vector<std::vector<int>> face_list((size_t)h->nblocks * 4);
+  for (int b = 0; b < h->nblocks; ++b) {
+    for (int f = 0; f < 4; ++f) h->db[b].bcf[f] = -1;
+    if (!h->hb[b].arena) continue;
+    face_list[b * 4 + 0].assign(h->hb[b].nj, -1);
+    face_list[b * 4 + 1].assign(h->hb[b].nj, -1);
+    face_list[b * 4 + 2].assign(h->hb[b].ni, -1);
+    face_list[b * 4 + 3].assign(h->hb[b].ni, -1);
+  }
+  h->nbc = (int)h->dbc.size();
+  for (int k = 0; k < h->nbc; ++k) {
+    const DevBC& r = h->dbc[k];
+    std::vector<int>& fl = face_list[r.blk * 4 + (r.f - 1)];
+    const int pos = r.f <= 2 ? r.j - 1 : r.i - 1;
+    if (fl[pos] >= 0) return fail(Q2D_E_ARG, "two bc records for one boundary face");
+    fl[pos] = k;
+    if (h->dc[h->hb[r.blk].case_id].model > 0 && (r.type == 300 || r.type == 302)) h->vrec.push_back(k);
+  }
+  for (int b = 0; b < h->nblocks; ++b)
+    for (int f = 0; f < 4; ++f) {
+      std::vector<int>& fl = face_list[b * 4 + f];
+      bool any = false;
+      for (int v : fl) any |= (v >= 0);
+      if (!any) continue;
+      for (int v : fl) if (v < 0) return fail(Q2D_E_ARG, "boundary face partially covered by bc records");
+      h->db[b].bcf[f] = (int)h->bcface.size();
+      h->bcface.insert(h->bcface.end(), fl.begin(), fl.end());
+    }
+  h->max_rk = 1;
+  h->any_visc = false;
+  for (int c = 0; c < h->ncases; ++c) {
+    if (!h->case_live[c]) continue;
+    h->max_rk = std::max(h->max_rk, h->dc[c].n_rk);
+    if (h->dc[c].model > 0) h->any_visc = true;
+  }
+  // device arrays, re-allocated at the current sizes (descriptors are small)
+  auto realloc_upload = [&](void** dptr, const void* src, size_t bytes) -> int {
+    (void)hipFree(*dptr); *dptr = nullptr;
+    HIPCHK(hipMalloc(dptr, std::max<size_t>(bytes, 16)));
+    if (bytes) HIPCHK(hipMemcpy(*dptr, src, bytes, hipMemcpyHostToDevice));
+    return Q2D_OK;
+  };
+  int rc;
+  if ((rc = realloc_upload((void**)&h->d_blocks, h->db.data(), sizeof(DevBlock) * h->db.size())) != Q2D_OK) return rc;
+  if ((rc = realloc_upload((void**)&h->d_bc, h->dbc.data(), sizeof(DevBC) * h->dbc.size())) != Q2D_OK) return rc;
+  if ((rc = realloc_upload((void**)&h->d_bcface, h->bcface.data(), sizeof(int) * h->bcface.size())) != Q2D_OK) return rc;
+  if ((rc = realloc_upload((void**)&h->d_vrec, h->vrec.data(), sizeof(int) * h->vrec.size())) != Q2D_OK) return rc;
+  (void)hipFree(h->d_bcflux); h->d_bcflux = nullptr;
+  HIPCHK(hipMalloc(&h->d_bcflux, sizeof(BCFlux) * std::max<size_t>(1, h->dbc.size())));
+  return build_items(h);     // also uploads d_cases
+}
+
+// Validate `prob` and append its cases, blocks and records to the handle.
+int append_cases(q2d_handle* h, const q2d_problem* prob, int32_t* first_case) {
+  if (prob->nblocks < 0 || prob->ncases < 0 || prob->nbc < 0) return fail(Q2D_E_ARG, "negative count");
+  if ((prob->nblocks > 0 && !prob->blocks) || (prob->ncases > 0 && !prob->cases) || (prob->nbc > 0 && !prob->bc))
+    return fail(Q2D_E_ARG, "NULL array");
+  if ((prob->nblocks == 0) != (prob->ncases == 0)) return fail(Q2D_E_ARG, "cases without blocks or blocks without cases");
+  if ((long long)h->ncases + prob->ncases > kMaxCases) return fail(Q2D_E_ARG, "too many cases");
+  if (first_case) *first_case = h->ncases;
+  if (prob->ncases == 0) return Q2D_OK;
+
+  // ---- validation (nothing is changed before it passes) ----
   std::vector<int> first_block(prob->ncases, -1), nblk(prob->ncases, 0);
   for (int b = 0; b < prob->nblocks; ++b) {
     const q2d_block_desc& d = prob->blocks[b];
@@ -1503,6 +1591,7 @@ int q2d_gpu_create(const q2d_problem* prob, q2d_handle** out) {
     else if (b != first_block[d.case_id] + nblk[d.case_id]) return fail(Q2D_E_ARG, "blocks of a case must be contiguous");
     nblk[d.case_id]++;
   }
+  const int prec = h->prec_set ? (h->precision_mixed ? Q2D_PREC_MIXED : Q2D_PREC_FP64) : prob->cases[0].scheme.precision;
   for (int c = 0; c < prob->ncases; ++c) {
     const q2d_scheme& s = prob->cases[c].scheme;
     if (nblk[c] == 0) return fail(Q2D_E_ARG, "case without blocks");
@@ -1513,13 +1602,14 @@ int q2d_gpu_create(const q2d_problem* prob, q2d_handle** out) {
     if (s.model < 0 || s.model > 1) return fail(Q2D_E_UNSUPPORTED, "only the Euler (0) and laminar (1) models are available");
     if (s.model == 1 && s.source == Q2D_SRC_AREA)
       return fail(Q2D_E_UNSUPPORTED, "laminar model with the area-variation source not available on this path");
-    if (s.model == 1 && !(s.mu_ref > 0) ) return fail(Q2D_E_ARG, "laminar model needs mu_ref > 0");
+    if (s.model == 1 && !(s.mu_ref > 0)) return fail(Q2D_E_ARG, "laminar model needs mu_ref > 0");
     if (s.model == 1 && (!(s.T_ref > 0) || !(s.Pr > 0) || !(s.S_suth >= 0)))
       return fail(Q2D_E_ARG, "laminar model needs T_ref > 0, S >= 0, Pr > 0");
     if (s.source < 0 || s.source > 2) return fail(Q2D_E_ARG, "unknown source");
     if (s.precision < 0 || s.precision > 1) return fail(Q2D_E_ARG, "unknown precision");
     if (!(s.gas_R > 0) || !(s.gas_cp > s.gas_R)) return fail(Q2D_E_ARG, "gas constants invalid");
-    if (s.precision != prob->cases[0].scheme.precision) return fail(Q2D_E_ARG, "all cases must use one precision");
+    if (s.precision != prec) return fail(Q2D_E_ARG, "all cases of a handle must use one precision");
+    if (prob->cases[c].res_drop < 0 || prob->cases[c].res_drop >= 1) return fail(Q2D_E_ARG, "res_drop must be in [0, 1)");
     for (int k = 0; k < nblk[c]; ++k) {
       const q2d_block_desc& d = prob->blocks[first_block[c] + k];
       if (s.source == Q2D_SRC_AXI && !d.src_b) return fail(Q2D_E_ARG, "axisymmetric source needs src_b (axis_inv_r)");
@@ -1528,50 +1618,118 @@ int q2d_gpu_create(const q2d_problem* prob, q2d_handle** out) {
         return fail(Q2D_E_ARG, "axisymmetric laminar model needs src_a (axis_r_ave) and node");
     }
   }
-
-  q2d_handle* h = new q2d_handle();
-  h->nblocks = prob->nblocks; h->ncases = prob->ncases; h->nbc = prob->nbc;
-  h->precision_mixed = prob->cases[0].scheme.precision == Q2D_PREC_MIXED;
-  {
-    hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, 0) == hipSuccess) h->cu_count = prop.multiProcessorCount;
+  const int case0 = h->ncases, block0 = h->nblocks;
+  std::vector<DevBC> newbc(prob->nbc);
+  std::vector<std::vector<char>> seen(prob->nblocks * 4);
+  for (int b = 0; b < prob->nblocks; ++b)
+    for (int f = 0; f < 4; ++f) seen[b * 4 + f].assign(f < 2 ? prob->blocks[b].nj : prob->blocks[b].ni, 0);
+  int prev_case = -1, prev_blk = 0, prev_face = 0;
+  for (int k = 0; k < prob->nbc; ++k) {
+    const q2d_bc_entry& e = prob->bc[k];
+    if (e.case_id < 0 || e.case_id >= prob->ncases) return fail(Q2D_E_ARG, "bc case_id out of range");
+    const int c = e.case_id;
+    if (e.b < 1 || e.b > nblk[c] || e.f < 1 || e.f > 4) return fail(Q2D_E_ARG, "bc block/face out of range");
+    // records ordered by case, block, face (Mod_BC_Fluxes.f90:79-86)
+    if (c < prev_case || (c == prev_case && (e.b < prev_blk || (e.b == prev_blk && e.f < prev_face))))
+      return fail(Q2D_E_ARG, "bc records must be ordered by case, block and face");
+    prev_case = c; prev_blk = e.b; prev_face = e.f;
+    const int lb = first_block[c] + e.b - 1;
+    const q2d_block_desc& d = prob->blocks[lb];
+    const bool iface = e.f <= 2;
+    if (iface) {
+      if (e.j < 1 || e.j > d.nj || e.i != (e.f == 1 ? 1 : d.ni)) return fail(Q2D_E_ARG, "bc cell not on its face");
+    } else {
+      if (e.i < 1 || e.i > d.ni || e.j != (e.f == 3 ? 1 : d.nj)) return fail(Q2D_E_ARG, "bc cell not on its face");
+    }
+    char& sn = seen[lb * 4 + (e.f - 1)][iface ? e.j - 1 : e.i - 1];
+    if (sn) return fail(Q2D_E_ARG, "two bc records for one boundary face");
+    sn = 1;
+    switch (e.type) {
+      case 101: case 201: case 300: case 302: case 400: case 405: case 406:
+        break;
+      default: return fail(Q2D_E_UNSUPPORTED, "bc type not available on this path");
+    }
+    if (prob->cases[c].scheme.model > 0 && (e.type == 101 || e.type == 201))
+      return fail(Q2D_E_UNSUPPORTED, "block connections of a laminar case (BC_Connection_Visc) not available on this path");
+    DevBC& r = newbc[k];
+    memset(&r, 0, sizeof(r));
+    r.type = e.type; r.blk = block0 + lb; r.i = e.i; r.j = e.j; r.f = e.f;
+    if (e.type == 101 || e.type == 201) {
+      if (e.bs < 1 || e.bs > nblk[c] || e.fs < 1 || e.fs > 4) return fail(Q2D_E_ARG, "bc donor out of range");
+      const q2d_block_desc& sd = prob->blocks[first_block[c] + e.bs - 1];
+      // donor cells Is + guide(Fs)*(g-1), g = 1..gc, must be interior cells of the donor
+      static const int gd[5][2] = {{0, 0}, {1, 0}, {-1, 0}, {0, 1}, {0, -1}};
+      for (int g = 1; g <= 2; ++g) {
+        int ii = e.is + gd[e.fs][0] * (g - 1), jj = e.js + gd[e.fs][1] * (g - 1);
+        if (ii < 1 || ii > sd.ni || jj < 1 || jj > sd.nj) return fail(Q2D_E_ARG, "bc donor cell outside donor block");
+      }
+      r.dblk = block0 + first_block[c] + e.bs - 1; r.is = e.is; r.js = e.js; r.fs = e.fs;
+    }
+    r.mach = e.mach; r.T = e.T; r.p = e.p; r.rel_fac = e.rel_fac;
+    if (e.type == 405) {                                // Compute_Inflow_Direction (Lib_BC_Fluxes_Inflow.f90:548-560)
+      double n1, n2;
+      if (iface) { int fi = e.i - (e.f % 2); size_t so = ((size_t)(e.j - 1) * (d.ni + 1) + fi) * 3; n1 = d.face_i[so]; n2 = d.face_i[so + 1]; }
+      else { int fj = e.j - (e.f % 2); size_t so = ((size_t)fj * d.ni + (e.i - 1)) * 3; n1 = d.face_j[so]; n2 = d.face_j[so + 1]; }
+      double a = (e.alpha >= 0.5 * DBL_MAX) ? atan(n2 / (n1 + 1e-20)) : e.alpha;
+      r.cos_a = cos(a); r.sin_a = sin(a);
+    }
   }
-  auto bail = [&](int code) { q2d_gpu_destroy(h); return code; };
+  for (int b = 0; b < prob->nblocks; ++b)
+    for (int f = 0; f < 4; ++f) {
+      bool any = false, all = true;
+      for (char v : seen[b * 4 + f]) { any |= v != 0; all &= v != 0; }
+      if (any && !all) return fail(Q2D_E_ARG, "boundary face partially covered by bc records");
+    }
 
-  // ---- per-case constants ----
-  h->dc.resize(h->ncases);
-  for (int c = 0; c < h->ncases; ++c) {
+  // ---- per-case constants and control ----
+  if (!h->prec_set) { h->precision_mixed = prec == Q2D_PREC_MIXED; h->prec_set = true; }
+  for (int c = 0; c < prob->ncases; ++c) {
     const q2d_case_desc& cd = prob->cases[c];
     const q2d_scheme& s = cd.scheme;
-    DevCase& k = h->dc[c];
+    DevCase k;
     memset(&k, 0, sizeof(k));
     k.n_rk = s.n_rk; k.limiter = s.limiter; k.riemann = s.riemann; k.time_accurate = s.time_accurate;
     k.model = s.model; k.source = s.source; k.precision = s.precision; k.ramp_iter = s.ramp_iter;
     k.ndir = (prob->blocks[first_block[c]].nj == 1) ? 1 : 2;     // Check_Mesh_Type (Lib_Metrics.f90:24-37)
     k.itermax = cd.itermax; k.cfl = s.cfl; k.vnn = s.vnn;
     k.R = s.gas_R; k.cp = s.gas_cp; k.cv = s.gas_cp - s.gas_R; k.gam = s.gas_cp / k.cv;
-    k.res_threshold = cd.res_threshold; k.time_threshold = cd.time_threshold;
+    k.res_threshold = cd.res_threshold; k.time_threshold = cd.time_threshold; k.res_drop = cd.res_drop;
     k.mu_ref = s.mu_ref; k.T_ref = s.T_ref; k.S_suth = s.S_suth; k.Pr = s.Pr;
-    h->max_rk = std::max(h->max_rk, s.n_rk);
-    if (s.model > 0) h->any_visc = true;
+    h->dc.push_back(k);
+    h->case_live.push_back(1);
   }
+  {
+    DevCtrl* nc = nullptr;
+    HIPCHK(hipMalloc(&nc, sizeof(DevCtrl) * (h->ncases + prob->ncases)));
+    if (h->ncases > 0) {
+      HIPCHK(hipStreamSynchronize(h->stream));
+      HIPCHK(hipMemcpy(nc, h->d_ctrl, sizeof(DevCtrl) * h->ncases, hipMemcpyDeviceToDevice));
+    }
+    std::vector<DevCtrl> ctl(prob->ncases);
+    for (auto& k : ctl) { memset(&k, 0, sizeof(k)); k.active = 1; k.todo = 1; k.fail_key = ~0ull; }
+    HIPCHK(hipMemcpy(nc + h->ncases, ctl.data(), sizeof(DevCtrl) * prob->ncases, hipMemcpyHostToDevice));
+    (void)hipFree(h->d_ctrl);
+    h->d_ctrl = nc;
+  }
+  h->ncases += prob->ncases;
 
   // ---- blocks: SoA planes in one arena per block ----
-  h->hb.resize(h->nblocks);
-  h->db.resize(h->nblocks);
   // planes 0-11: 3 state buffers x 4; 12-24 geometry; 25, 28 local dt; 26, 27 source terms; 29 node radius;
   // viscous blocks add 30-38: diffusive fluxes of i- and j-faces and the viscous source, 3 planes each
-  for (int b = 0; b < h->nblocks; ++b) {
-    const q2d_block_desc& d = prob->blocks[b];
+  for (int lb = 0; lb < prob->nblocks; ++lb) {
+    const q2d_block_desc& d = prob->blocks[lb];
     const bool visc = prob->cases[d.case_id].scheme.model > 0;
     const int kPlanes = visc ? 39 : 30;
-    HostBlock& hb = h->hb[b];
-    hb.ni = d.ni; hb.nj = d.nj; hb.case_id = d.case_id; hb.blk_in_case = b - first_block[d.case_id] + 1;
+    h->hb.emplace_back();
+    h->db.emplace_back();
+    h->nblocks += 1;
+    HostBlock& hb = h->hb.back();
+    hb.ni = d.ni; hb.nj = d.nj; hb.case_id = case0 + d.case_id; hb.blk_in_case = lb - first_block[d.case_id] + 1;
     hb.pitch = ((d.ni + 4) + 15) / 16 * 16;
     hb.plane = (long long)hb.pitch * (d.nj + 4);
     hb.arena_bytes = sizeof(double) * (size_t)hb.plane * kPlanes;
-    if (hipMalloc(&hb.arena, hb.arena_bytes) != hipSuccess) return bail(fail(Q2D_E_NOMEM, "block allocation failed"));
-    if (hipMemset(hb.arena, 0, hb.arena_bytes) != hipSuccess) return bail(fail(Q2D_E_HIP, "memset failed"));
+    if (hipMalloc(&hb.arena, hb.arena_bytes) != hipSuccess) { hb.arena = nullptr; return fail(Q2D_E_NOMEM, "block allocation failed"); }
+    h->arena_bytes += hb.arena_bytes;
     const long long pl = hb.plane;
     const int ngi = d.ni + 4, ngj = d.nj + 4, pitch = hb.pitch;
     std::vector<double> host((size_t)pl * kPlanes, 0.0);
@@ -1581,7 +1739,7 @@ int q2d_gpu_create(const q2d_problem* prob, q2d_handle** out) {
       for (int i = 0; i < ngi; ++i)
         for (int v = 0; v < 4; ++v) {
           double val = d.P[((size_t)j * ngi + i) * 4 + v];
-          for (int s = 0; s < 3; ++s) plane(4 * s + v)[(size_t)j * pitch + i] = val;
+          for (int sb = 0; sb < 3; ++sb) plane(4 * sb + v)[(size_t)j * pitch + i] = val;
         }
     // geometry over the ghost extent
     for (int j = 0; j < ngj; ++j)
@@ -1616,13 +1774,12 @@ int q2d_gpu_create(const q2d_problem* prob, q2d_handle** out) {
       for (int j = 0; j <= d.nj; ++j)
         for (int i = 0; i <= d.ni; ++i)
           plane(29)[(size_t)(j + 1) * pitch + (i + 1)] = d.node[((size_t)j * (d.ni + 1) + i) * 2 + 1];
-    if (hipMemcpy(hb.arena, host.data(), hb.arena_bytes, hipMemcpyHostToDevice) != hipSuccess)
-      return bail(fail(Q2D_E_HIP, "upload failed"));
-    DevBlock& db = h->db[b];
+    HIPCHK(hipMemcpy(hb.arena, host.data(), hb.arena_bytes, hipMemcpyHostToDevice));
+    DevBlock& db = h->db.back();
     memset(&db, 0, sizeof(db));
-    db.ni = d.ni; db.nj = d.nj; db.pitch = pitch; db.case_id = d.case_id; db.blk_in_case = hb.blk_in_case;
+    db.ni = d.ni; db.nj = d.nj; db.pitch = pitch; db.case_id = hb.case_id; db.blk_in_case = hb.blk_in_case;
     db.plane = pl;
-    for (int s = 0; s < 3; ++s) db.P[s] = hb.arena + (size_t)4 * s * pl;
+    for (int sb = 0; sb < 3; ++sb) db.P[sb] = hb.arena + (size_t)4 * sb * pl;
     double* a = hb.arena;
     db.dlx = a + 12 * pl; db.dly = a + 13 * pl; db.sur = a + 14 * pl;
     db.m11 = a + 15 * pl; db.m21 = a + 16 * pl; db.m12 = a + 17 * pl; db.m22 = a + 18 * pl;
@@ -1633,112 +1790,87 @@ int q2d_gpu_create(const q2d_problem* prob, q2d_handle** out) {
     if (visc) { db.vdx = a + 30 * pl; db.vey = a + 33 * pl; db.vsv = a + 36 * pl; }
     for (int f = 0; f < 4; ++f) db.bcf[f] = -1;
   }
+  h->dbc.insert(h->dbc.end(), newbc.begin(), newbc.end());
+  return rebuild_descriptors(h);
+}
 
-  // ---- boundary records ----
-  std::vector<std::vector<int>> face_list(h->nblocks * 4);
-  for (int b = 0; b < h->nblocks; ++b) {
-    face_list[b * 4 + 0].assign(prob->blocks[b].nj, -1);
-    face_list[b * 4 + 1].assign(prob->blocks[b].nj, -1);
-    face_list[b * 4 + 2].assign(prob->blocks[b].ni, -1);
-    face_list[b * 4 + 3].assign(prob->blocks[b].ni, -1);
-  }
-  h->dbc.resize(h->nbc);
-  int prev_case = -1, prev_blk = 0, prev_face = 0;
-  for (int k = 0; k < h->nbc; ++k) {
-    const q2d_bc_entry& e = prob->bc[k];
-    if (e.case_id < 0 || e.case_id >= h->ncases) return bail(fail(Q2D_E_ARG, "bc case_id out of range"));
-    const int c = e.case_id;
-    if (e.b < 1 || e.b > nblk[c] || e.f < 1 || e.f > 4) return bail(fail(Q2D_E_ARG, "bc block/face out of range"));
-    // records ordered by case, block, face (Mod_BC_Fluxes.f90:79-86)
-    if (c < prev_case || (c == prev_case && (e.b < prev_blk || (e.b == prev_blk && e.f < prev_face))))
-      return bail(fail(Q2D_E_ARG, "bc records must be ordered by case, block and face"));
-    prev_case = c; prev_blk = e.b; prev_face = e.f;
-    const int gb = first_block[c] + e.b - 1;
-    const q2d_block_desc& d = prob->blocks[gb];
-    const bool iface = e.f <= 2;
-    if (iface) {
-      if (e.j < 1 || e.j > d.nj || e.i != (e.f == 1 ? 1 : d.ni)) return bail(fail(Q2D_E_ARG, "bc cell not on its face"));
-    } else {
-      if (e.i < 1 || e.i > d.ni || e.j != (e.f == 3 ? 1 : d.nj)) return bail(fail(Q2D_E_ARG, "bc cell not on its face"));
-    }
-    switch (e.type) {
-      case 101: case 201: case 300: case 302: case 400: case 405: case 406:
-      case 0: case 401: case 402: case 403: case 404: case 407: case 420:
-        break;
-      default: return bail(fail(Q2D_E_UNSUPPORTED, "bc type not available on this path"));
-    }
-    if (e.type != 101 && e.type != 201 && e.type != 300 && e.type != 302 && e.type != 400 && e.type != 405 &&
-        e.type != 406)
-      return bail(fail(Q2D_E_UNSUPPORTED, "bc flux type not available on this path"));
-    if (prob->cases[c].scheme.model > 0 && (e.type == 101 || e.type == 201))
-      return bail(fail(Q2D_E_UNSUPPORTED, "block connections of a laminar case (BC_Connection_Visc) not available on this path"));
-    if (prob->cases[c].scheme.model > 0 && (e.type == 300 || e.type == 302)) h->vrec.push_back(k);
-    DevBC& r = h->dbc[k];
-    memset(&r, 0, sizeof(r));
-    r.type = e.type; r.blk = gb; r.i = e.i; r.j = e.j; r.f = e.f;
-    if (e.type == 101 || e.type == 201) {
-      if (e.bs < 1 || e.bs > nblk[c] || e.fs < 1 || e.fs > 4) return bail(fail(Q2D_E_ARG, "bc donor out of range"));
-      const q2d_block_desc& s = prob->blocks[first_block[c] + e.bs - 1];
-      // donor cells Is + guide(Fs)*(g-1), g = 1..gc, must be interior cells of the donor
-      static const int gd[5][2] = {{0, 0}, {1, 0}, {-1, 0}, {0, 1}, {0, -1}};
-      for (int g = 1; g <= 2; ++g) {
-        int ii = e.is + gd[e.fs][0] * (g - 1), jj = e.js + gd[e.fs][1] * (g - 1);
-        if (ii < 1 || ii > s.ni || jj < 1 || jj > s.nj) return bail(fail(Q2D_E_ARG, "bc donor cell outside donor block"));
-      }
-      r.dblk = first_block[c] + e.bs - 1; r.is = e.is; r.js = e.js; r.fs = e.fs;
-    }
-    r.mach = e.mach; r.T = e.T; r.p = e.p; r.rel_fac = e.rel_fac;
-    if (e.type == 405) {                                // Compute_Inflow_Direction (Lib_BC_Fluxes_Inflow.f90:548-560)
-      double n1, n2;
-      if (iface) { int fi = e.i - (e.f % 2); size_t s = ((size_t)(e.j - 1) * (d.ni + 1) + fi) * 3; n1 = d.face_i[s]; n2 = d.face_i[s + 1]; }
-      else { int fj = e.j - (e.f % 2); size_t s = ((size_t)fj * d.ni + (e.i - 1)) * 3; n1 = d.face_j[s]; n2 = d.face_j[s + 1]; }
-      double a = (e.alpha >= 0.5 * DBL_MAX) ? atan(n2 / (n1 + 1e-20)) : e.alpha;
-      r.cos_a = cos(a); r.sin_a = sin(a);
-    }
-    std::vector<int>& fl = face_list[gb * 4 + (e.f - 1)];
-    const int pos = iface ? e.j - 1 : e.i - 1;
-    if (fl[pos] >= 0) return bail(fail(Q2D_E_ARG, "two bc records for one boundary face"));
-    fl[pos] = k;
-  }
-  // every boundary face must have a record
-  for (int b = 0; b < h->nblocks; ++b)
-    for (int f = 0; f < 4; ++f) {
-      std::vector<int>& fl = face_list[b * 4 + f];
-      bool any = false;
-      for (int v : fl) any |= (v >= 0);
-      if (!any) continue;
-      for (int v : fl) if (v < 0) return bail(fail(Q2D_E_ARG, "boundary face partially covered by bc records"));
-      h->db[b].bcf[f] = (int)h->bcface.size();
-      h->bcface.insert(h->bcface.end(), fl.begin(), fl.end());
-    }
+}  // namespace
 
-  // ---- device copies ----
+int q2d_gpu_create(const q2d_problem* prob, q2d_handle** out) {
+  if (!prob || !out) return fail(Q2D_E_ARG, "NULL argument");
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(Q2D_E_NODEVICE, "no HIP device");
+  q2d_handle* h = new q2d_handle();
+  auto bail = [&](int code) { q2d_gpu_destroy(h); return code; };
+  {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, 0) == hipSuccess) h->cu_count = prop.multiProcessorCount;
+  }
   if (hipStreamCreate(&h->stream) != hipSuccess) return bail(fail(Q2D_E_HIP, "stream creation failed"));
-  h->tev.assign(2 * 3 * 4, nullptr);                  // 3 stages x 4 kernel variants, start/stop each
+  h->tev.assign(2 * 3 * 4 * kTimedSteps, nullptr);    // start/stop x 3 stages x 4 kernel variants x steps
   for (auto& e : h->tev)
-    if (hipEventCreate(&e) != hipSuccess) return bail(fail(Q2D_E_HIP, "event creation failed"));
-  if (hipMalloc(&h->d_blocks, sizeof(DevBlock) * h->nblocks) != hipSuccess ||
-      hipMalloc(&h->d_cases, sizeof(DevCase) * h->ncases) != hipSuccess ||
-      hipMalloc(&h->d_ctrl, sizeof(DevCtrl) * h->ncases) != hipSuccess ||
-      hipMalloc(&h->d_bc, sizeof(DevBC) * std::max(1, h->nbc)) != hipSuccess ||
-      hipMalloc(&h->d_bcflux, sizeof(BCFlux) * std::max(1, h->nbc)) != hipSuccess ||
-      hipMalloc(&h->d_bcface, sizeof(int) * std::max<size_t>(1, h->bcface.size())) != hipSuccess ||
-      hipMalloc(&h->d_vrec, sizeof(int) * std::max<size_t>(1, h->vrec.size())) != hipSuccess)
-    return bail(fail(Q2D_E_NOMEM, "descriptor allocation failed"));
-  if (!h->vrec.empty() &&
-      hipMemcpy(h->d_vrec, h->vrec.data(), sizeof(int) * h->vrec.size(), hipMemcpyHostToDevice) != hipSuccess)
-    return bail(fail(Q2D_E_HIP, "descriptor upload failed"));
-  std::vector<DevCtrl> ctl(h->ncases);
-  for (auto& k : ctl) { memset(&k, 0, sizeof(k)); k.active = 1; k.todo = 1; k.fail_key = ~0ull; }
-  if (hipMemcpy(h->d_blocks, h->db.data(), sizeof(DevBlock) * h->nblocks, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(h->d_ctrl, ctl.data(), sizeof(DevCtrl) * h->ncases, hipMemcpyHostToDevice) != hipSuccess ||
-      (h->nbc > 0 && hipMemcpy(h->d_bc, h->dbc.data(), sizeof(DevBC) * h->nbc, hipMemcpyHostToDevice) != hipSuccess) ||
-      (!h->bcface.empty() &&
-       hipMemcpy(h->d_bcface, h->bcface.data(), sizeof(int) * h->bcface.size(), hipMemcpyHostToDevice) != hipSuccess))
-    return bail(fail(Q2D_E_HIP, "descriptor upload failed"));
-  int rc = build_items(h);
-  if (rc != Q2D_OK) return bail(rc);
+    if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess)
+      return bail(fail(Q2D_E_HIP, "event creation failed"));
+  int rc = append_cases(h, prob, nullptr);
+  if (rc == Q2D_OK && h->ncases == 0) rc = rebuild_descriptors(h);
+  if (rc != Q2D_OK) {
+    std::string msg = g_err;
+    q2d_gpu_destroy(h);
+    g_err = msg;
+    return rc;
+  }
   *out = h;
+  return Q2D_OK;
+}
+
+int q2d_gpu_add_cases(q2d_handle* h, const q2d_problem* prob, int32_t* first_case) {
+  if (!h || !prob) return fail(Q2D_E_ARG, "NULL argument");
+  HIPCHK(hipStreamSynchronize(h->stream));
+  const int nb0 = h->nblocks, nc0 = h->ncases;
+  const size_t nbc0 = h->dbc.size();
+  int rc = append_cases(h, prob, first_case);
+  if (rc != Q2D_OK && (h->nblocks != nb0 || h->ncases != nc0)) {
+    // roll back a partial append: free what was allocated and rebuild from the previous set
+    std::string msg = g_err;
+    for (int b = nb0; b < h->nblocks; ++b) if (h->hb[b].arena) { (void)hipFree(h->hb[b].arena); h->arena_bytes -= h->hb[b].arena_bytes; }
+    h->hb.resize(nb0); h->db.resize(nb0); h->nblocks = nb0;
+    h->dbc.resize(nbc0);
+    h->dc.resize(nc0); h->case_live.resize(nc0); h->ncases = nc0;
+    (void)rebuild_descriptors(h);
+    g_err = msg;
+  }
+  return rc;
+}
+
+int q2d_gpu_remove_case(q2d_handle* h, int32_t case_id) {
+  if (!h || case_id < 0 || case_id >= h->ncases) return fail(Q2D_E_ARG, "bad case index");
+  if (!h->case_live[case_id]) return fail(Q2D_E_ARG, "case already removed");
+  HIPCHK(hipStreamSynchronize(h->stream));
+  h->case_live[case_id] = 0;
+  const int zero = 0;
+  HIPCHK(hipMemcpy((char*)(h->d_ctrl + case_id) + offsetof(DevCtrl, active), &zero, sizeof(int), hipMemcpyHostToDevice));
+  for (int b = 0; b < h->nblocks; ++b) {
+    HostBlock& hb = h->hb[b];
+    if (hb.case_id != case_id || !hb.arena) continue;
+    (void)hipFree(hb.arena);
+    hb.arena = nullptr;
+    h->arena_bytes -= hb.arena_bytes;
+    DevBlock& db = h->db[b];
+    const int ni = db.ni, nj = db.nj;
+    memset(&db, 0, sizeof(db));
+    db.ni = ni; db.nj = nj; db.case_id = case_id;
+  }
+  std::vector<DevBC> keep;
+  keep.reserve(h->dbc.size());
+  for (const DevBC& r : h->dbc) if (h->hb[r.blk].case_id != case_id) keep.push_back(r);
+  h->dbc.swap(keep);
+  return rebuild_descriptors(h);
+}
+
+int q2d_gpu_case_count(q2d_handle* h, int32_t* ncases, int32_t* nblocks, int64_t* device_bytes) {
+  if (!h || !ncases || !nblocks || !device_bytes) return fail(Q2D_E_ARG, "NULL argument");
+  *ncases = h->ncases; *nblocks = h->nblocks; *device_bytes = (int64_t)h->arena_bytes;
   return Q2D_OK;
 }
 
@@ -1756,8 +1888,33 @@ int q2d_gpu_step(q2d_handle* h, q2d_step_out* out) {
 int q2d_gpu_run(q2d_handle* h, int32_t nsteps, q2d_step_out* out) {
   if (!h) return fail(Q2D_E_ARG, "NULL handle");
   if (nsteps < 0) return fail(Q2D_E_ARG, "nsteps < 0");
-  if (h->timing) {                                     // per-launch events cannot live inside a graph
-    for (int s = 0; s < nsteps; ++s) { int rc = launch_step(h); if (rc != Q2D_OK) return rc; }
+  if (h->ncases == 0) return Q2D_OK;
+  if (h->timing && nsteps > 0) {
+    // Timed steps: up to kTimedSteps consecutive steps captured into one graph with an event pair around
+    // every stage launch (events without the system-scope fence, so no cache flush is added), replayed
+    // once: the stage kernels run back to back as in the untimed graph, and every pair is read after.
+    h->tslot = 0;
+    if (!h->dt_fused_valid) { int rc = launch_step(h); if (rc == Q2D_OK) rc = collect_timing(h); if (rc != Q2D_OK) return rc; --nsteps; }
+    while (nsteps > 0) {
+      const int n = std::min(nsteps, kTimedSteps);
+      hipGraph_t g;
+      hipGraphExec_t ge = nullptr;
+      HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+      h->capturing = true;
+      int rc = Q2D_OK;
+      for (int k = 0; k < n && rc == Q2D_OK; ++k) rc = launch_step(h);
+      h->capturing = false;
+      hipError_t ce = hipStreamEndCapture(h->stream, &g);
+      if (rc != Q2D_OK) return rc;
+      if (ce != hipSuccess) return fail(Q2D_E_HIP, std::string("graph capture: ") + hipGetErrorString(ce));
+      HIPCHK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+      (void)hipGraphDestroy(g);
+      hipError_t le = hipGraphLaunch(ge, h->stream);
+      rc = le == hipSuccess ? collect_timing(h) : fail(Q2D_E_HIP, "graph launch failed");
+      (void)hipGraphExecDestroy(ge);
+      if (rc != Q2D_OK) return rc;
+      nsteps -= n;
+    }
   } else if (nsteps > 0) {
     if (!h->dt_fused_valid) {                          // first step after create/upload: eager, with k_dt
       int rc = launch_step(h);
@@ -1785,6 +1942,7 @@ int q2d_gpu_run(q2d_handle* h, int32_t nsteps, q2d_step_out* out) {
 
 int q2d_gpu_download_state(q2d_handle* h, int32_t block, double* P) {
   if (!h || !P || block < 0 || block >= h->nblocks) return fail(Q2D_E_ARG, "bad argument");
+  if (!h->hb[block].arena) return fail(Q2D_E_ARG, "block of a removed case");
   HostBlock& b = h->hb[block];
   DevCtrl k;
   HIPCHK(hipMemcpyAsync(&k, h->d_ctrl + b.case_id, sizeof(DevCtrl), hipMemcpyDeviceToHost, h->stream));
@@ -1800,6 +1958,7 @@ int q2d_gpu_download_state(q2d_handle* h, int32_t block, double* P) {
 
 int q2d_gpu_upload_state(q2d_handle* h, int32_t block, const double* P) {
   if (!h || !P || block < 0 || block >= h->nblocks) return fail(Q2D_E_ARG, "bad argument");
+  if (!h->hb[block].arena) return fail(Q2D_E_ARG, "block of a removed case");
   HostBlock& b = h->hb[block];
   DevCtrl k;
   HIPCHK(hipMemcpyAsync(&k, h->d_ctrl + b.case_id, sizeof(DevCtrl), hipMemcpyDeviceToHost, h->stream));
@@ -1813,6 +1972,7 @@ int q2d_gpu_upload_state(q2d_handle* h, int32_t block, const double* P) {
 
 int q2d_gpu_download_dt(q2d_handle* h, int32_t block, double* dtlocal) {
   if (!h || !dtlocal || block < 0 || block >= h->nblocks) return fail(Q2D_E_ARG, "bad argument");
+  if (!h->hb[block].arena) return fail(Q2D_E_ARG, "block of a removed case");
   HostBlock& b = h->hb[block];
   DevCtrl k;
   HIPCHK(hipMemcpy(&k, h->d_ctrl + b.case_id, sizeof(DevCtrl), hipMemcpyDeviceToHost));

@@ -44,6 +44,7 @@ module q2d_gpu
     type(q2d_scheme)   :: scheme
     integer(c_int32_t) :: itermax, reserved
     real(c_double)     :: res_threshold, time_threshold
+    real(c_double)     :: res_drop
   end type
 
   type, bind(C), public :: q2d_problem

@@ -1,0 +1,70 @@
+"""Continuous batching and the serving loop (needs the GPU).
+
+Cases join and leave a live solver (q2d_gpu_add_cases / q2d_gpu_remove_case) without disturbing the
+others: a case's state after N steps is bit-identical whether it ran alone or while others were added
+and removed around it, and a served request's result (state, iteration count) is the same alone and
+in the mix.  The relative-drop stop test fires at the step the oracle's residual history says."""
+import numpy as np
+import pytest
+
+from q2d_amd import GpuSolver, cases, serving
+
+pytestmark = pytest.mark.gpu
+
+
+def test_add_and_remove_leave_other_cases_untouched():
+    a = cases.riemann_planar(seed=11, ni=200, nj=8, steps=10 ** 6, precision="mixed")
+    b = cases.ramp_channel(ni=96, nj=48, precision="mixed", itermax=10 ** 6)
+    c = cases.uniform_inflow(128, 32, 1.7, area_a=0.4, precision="mixed", steps=10 ** 6)
+    alone = GpuSolver(a)
+    alone.run(70)
+    g = GpuSolver(cases.Problem())
+    ca, _ = g.add_cases(a)
+    g.run(20)
+    cb, bb = g.add_cases(b)
+    g.run(30)
+    cc, bc = g.add_cases(c)
+    mem = g.device_bytes()
+    g.remove_case(cb)
+    assert g.device_bytes() < mem
+    out = g.run(20)
+    assert out[ca]["iter"] == 70 and out[cb]["todo"] == 0 and out[cc]["iter"] == 20
+    assert np.array_equal(g.state(0)[2:-2, 2:-2], alone.state(0)[2:-2, 2:-2])
+    c_alone = GpuSolver(c)
+    c_alone.run(20)
+    assert np.array_equal(g.state(bc)[2:-2, 2:-2], c_alone.state(0)[2:-2, 2:-2])
+
+
+def test_relative_drop_stop_matches_oracle(oracle_mod):
+    pb = cases.ramp_channel(ni=64, nj=32, wedge_deg=8.0, mach=2.0, precision="fp64", itermax=5000)
+    pb.cases[0].res_drop = 1e-3
+    g = GpuSolver(pb)
+    out = g.run(5000)[0]
+    assert out["todo"] == 3 and out["iter"] < 5000
+    o = oracle_mod.Oracle(pb)
+    r1 = None
+    for k in range(1, out["iter"] + 1):
+        rc, oo = o.step()
+        assert rc == 0
+        r = oo[0]["residual"][0]
+        r1 = r if k == 1 else r1
+        if k > 1 and r <= 1e-3 * r1:
+            break
+    assert k == out["iter"]
+    assert np.array_equal(g.state(0)[2:-2, 2:-2], o.state(0)[2:-2, 2:-2])
+
+
+def test_served_results_equal_alone():
+    reqs = serving.poisson_stream(6, rate=40.0, seed=3, smax=0.3, res_drop=1e-3, itermax=4000, precision="mixed")
+    budget = 2 * max(serving.footprint_bytes(r.problem) for r in reqs)     # forces queueing
+    srv = serving.Server(budget_bytes=budget, chunk_steps=40)
+    res, stats = srv.serve(reqs)
+    srv.close()
+    assert stats["requests"] == 6 and stats["p99_s"] >= stats["p50_s"] > 0
+    for r, x in zip(reqs, res):
+        solo = serving.Server(budget_bytes=budget, chunk_steps=25)
+        rs, _ = solo.serve([serving.Request(r.rid, 0.0, r.problem)])
+        solo.close()
+        assert rs[0].iters == x.iters and rs[0].status == x.status == 0
+        for a, b in zip(rs[0].state, x.state):
+            assert np.array_equal(a, b)
