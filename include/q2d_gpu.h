@@ -219,10 +219,13 @@ int q2d_gpu_upload_state(q2d_handle *h, int32_t block, const double *P);
  * (Mod_dt.f90:46). */
 int q2d_gpu_download_dt(q2d_handle *h, int32_t block, double *dtlocal);
 
-/* Kernel-time accounting (no counterpart in the reference): when enabled,
- * every stage-kernel launch carries start/stop events stamped from its own
- * dispatch; stats returns the number of timed stage launches and their summed
- * duration in milliseconds.  Steps run eagerly (not from the graph) while on. */
+/* Kernel-time accounting (no counterpart in the reference): while enabled, every
+ * stage-kernel launch records its first workgroup start and last wavefront end
+ * on the device's constant-rate wall clock (two atomics per workgroup), in a
+ * ring of 64 steps that the host reads every 64 steps and at the end of each
+ * call; steps still replay one captured graph.  Enabling (again) clears the
+ * statistics.  stats returns the number of timed stage launches and their summed
+ * duration in milliseconds. */
 int q2d_gpu_set_timing(q2d_handle *h, int32_t enable);
 int q2d_gpu_timing_stats(q2d_handle *h, int64_t *launches, double *ms_total);
 

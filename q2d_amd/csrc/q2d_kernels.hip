@@ -38,7 +38,18 @@ namespace {
 constexpr int kWave = 64;
 constexpr int kOwn = 60;          // owned columns per strip
 constexpr int kMaxCases = 1 << 20;
-constexpr int kTimedSteps = 8;    // steps per timing graph (event pool size)
+constexpr int kTimedSteps = 64;                         // timed steps between two reads of the slots
+constexpr int kLaunchesPerStep = 3 * 4;                 // stage launches per step (3 stages x 4 variants)
+constexpr int kTimedSlots = kLaunchesPerStep * kTimedSteps;
+constexpr int kStampLanes = 64;                         // sub-slots per launch slot
+// Rows of state the stage kernel's loads run ahead of the row being computed (1 or 2), per form: the
+// FP64 form's longer rows leave room for a second row in flight, measured 3 % faster.
+#ifndef Q2D_PREFETCH_ROWS_FP64
+#define Q2D_PREFETCH_ROWS_FP64 2
+#endif
+#ifndef Q2D_PREFETCH_ROWS_MIXED
+#define Q2D_PREFETCH_ROWS_MIXED 1
+#endif
 // Fuse the next step's Compute_dt into the last RK stage (saves the separate dt pass over the
 // state).  The FP64 stage kernel keeps a separate dt pass: the fused version costs it registers.
 #ifndef Q2D_FUSE_DT_FP64
@@ -235,51 +246,61 @@ __device__ __forceinline__ double cell_dt(const DevCase& C, const DevCtrl& K, do
   return dtmin;
 }
 
+// Work unit of the dt pass: `count` (<= kDtCells) consecutive cells of one block, in row-major order from
+// the block's interior cell `start`.  One wavefront per unit: the block, case and control data are
+// wave-uniform (scalar loads), the lanes stream consecutive cells (coalesced row segments), four cells per
+// lane in flight, and the unit's min is published with one atomic.
+struct DtRange { int blk, start, count, ni; };
+constexpr int kDtCells = 1024;
+
 __global__ __launch_bounds__(256) void k_dt(const DevBlock* __restrict__ blocks, const DevCase* __restrict__ cases,
-                                            DevCtrl* __restrict__ ctrl, const Item* __restrict__ items, int n_items) {
+                                            DevCtrl* __restrict__ ctrl, const DtRange* __restrict__ units,
+                                            int nunits, int unused) {
+  (void)unused;
   const int lane = threadIdx.x & (kWave - 1);
   const int wpg = blockDim.x / kWave;
-  const int wave0 = blockIdx.x * wpg + threadIdx.x / kWave;
+  const int wave0 = __builtin_amdgcn_readfirstlane(blockIdx.x * wpg + threadIdx.x / kWave);
   const int nwaves = gridDim.x * wpg;
-  for (int it = wave0; it < n_items; it += nwaves) {
-    const Item item = items[it];
-    const DevBlock& B = blocks[item.blk];
+  for (int u = wave0; u < nunits; u += nwaves) {
+    const DtRange U = units[u];
+    const DevBlock& B = blocks[U.blk];
     DevCtrl& K = ctrl[B.case_id];
     if (!K.active) continue;
     const DevCase& C = cases[B.case_id];
-    const int x = item.x0 - 2 + lane;
-    const bool own = lane >= 2 && lane < 2 + kOwn && x <= B.ni;
+    const long long pl = B.plane;
+    const int pitch = B.pitch;
     GP P = gp(B.P[K.cur]);
     GP m11 = gp(B.m11), m21 = gp(B.m21), m12 = gp(B.m12), m22 = gp(B.m22), dlx = gp(B.dlx), dly = gp(B.dly);
     GPW dtl = gpw(B.dtl[K.dtsel]);
-    const long long pl = B.plane;
     const bool local = !C.time_accurate;
     double dmin = 1e300;
-    if (own) {
-      // four rows per trip: 40 independent loads in flight per lane
-      int r = item.y0;
-      for (; r + 3 <= item.y1; r += 4) {
-        double d[4];
+    for (int c0 = 0; c0 < U.count; c0 += 4 * kWave) {
+      long long o[4];
+      bool ok[4];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          const long long o = (long long)(r + k + 1) * B.pitch + (x + 1);
-          d[k] = cell_dt(C, K, P[o], P[pl + o], P[2 * pl + o], P[3 * pl + o], m11[o], m21[o], m12[o], m22[o], dlx[o], dly[o]);
-          if (local) dtl[o] = d[k];
-        }
-#pragma unroll
-        for (int k = 0; k < 4; ++k) if (d[k] < dmin) dmin = d[k];
+      for (int k = 0; k < 4; ++k) {
+        const int c = c0 + k * kWave + lane;
+        ok[k] = c < U.count;
+        const int cell = U.start + (ok[k] ? c : 0);
+        const int j = cell / U.ni, i = cell - j * U.ni;
+        o[k] = (long long)(j + 2) * pitch + (i + 2);
       }
-      for (; r <= item.y1; ++r) {
-        const long long o = (long long)(r + 1) * B.pitch + (x + 1);
-        const double d = cell_dt(C, K, P[o], P[pl + o], P[2 * pl + o], P[3 * pl + o], m11[o], m21[o], m12[o], m22[o], dlx[o], dly[o]);
-        if (local) dtl[o] = d;
-        if (d < dmin) dmin = d;
+      double d[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        d[k] = cell_dt(C, K, P[o[k]], P[pl + o[k]], P[2 * pl + o[k]], P[3 * pl + o[k]], m11[o[k]], m21[o[k]], m12[o[k]],
+                       m22[o[k]], dlx[o[k]], dly[o[k]]);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        if (!ok[k]) continue;
+        if (local) dtl[o[k]] = d[k];
+        if (d[k] < dmin) dmin = d[k];
       }
     }
-    // wavefront min (order-independent), then one atomic per wavefront
-    for (int s = 32; s >= 1; s >>= 1) {
-      double o = __shfl_xor(dmin, s, kWave);
-      if (o < dmin) dmin = o;
+    // wavefront min (order-independent), then one atomic per unit
+    for (int sft = 32; sft >= 1; sft >>= 1) {
+      const double q = __shfl_xor(dmin, sft, kWave);
+      if (q < dmin) dmin = q;
     }
     if (lane == 0 && dmin < 1e300) atomicMin(&K.dtmin_bits, (unsigned long long)__double_as_longlong(dmin));
   }
@@ -891,7 +912,18 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
                                                DevCtrl* __restrict__ ctrl, const Item* __restrict__ items, int item_lo,
                                                int item_hi, const int* __restrict__ bcface,
                                                const BCFlux* __restrict__ bcflux, double* __restrict__ partials, int irk,
-                                               int xcd_remap) {
+                                               int xcd_remap, unsigned long long* __restrict__ tstamp, int tslot) {
+  // launch timing (only when tstamp is set): earliest workgroup start and latest wavefront end on the
+  // device's constant-rate wall clock, in the slot of this launch within the current timed step
+  // (tstamp[0] counts steps; slots form a ring of kTimedSteps steps)
+  // (tstamp[0] counts steps; slots form a ring of kTimedSteps steps).  Each slot has kStampLanes
+  // sub-slots, picked by workgroup, so that the atomics of one launch spread over kStampLanes words.
+  unsigned long long* tslotp = nullptr;
+  if (tstamp) {
+    const int launch = (int)(tstamp[0] % kTimedSteps) * kLaunchesPerStep + tslot;
+    tslotp = tstamp + 2 + 2 * ((long long)launch * kStampLanes + (blockIdx.x % kStampLanes));
+    if (threadIdx.x == 0) atomicMin(&tslotp[0], (unsigned long long)wall_clock64());
+  }
   const int lane = threadIdx.x & (kWave - 1);
   const int wpg = blockDim.x / kWave;
   int grp = blockIdx.x;
@@ -938,10 +970,12 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
     auto at = [&](int r) { return (long long)(r + 1) * pitch + (xc + 1); };
 
     // ---- pipeline warm-up: rows y0-2, y0-1, y0 ----------------------------
-    double qA[4], qB[4], qC[4], qN[4];
-    double dlyA, dlyB, dlyC, dlyN;
+    constexpr int kPrefetchRows = sizeof(FT) == 8 ? Q2D_PREFETCH_ROWS_FP64 : Q2D_PREFETCH_ROWS_MIXED;
+    double qA[4], qB[4], qC[4], qN[4], qN2[4];
+    double dlyA, dlyB, dlyC, dlyN, dlyN2 = 0.0;
     FT sAB[4], phiA[4], Gprev[4];
-    RowGeo gcur, gnext;
+    RowGeo gcur;                                       // row r: i-face and update data, loaded at the top of its iteration
+    double fjxN, fjyN, fjlN;                           // j-face geometry of the next row (one row ahead)
     {
       double qm[4];
       const long long o0 = at(item.y0 - 2), o1 = at(item.y0 - 1), o2 = at(item.y0), o3 = at(item.y0 + 1);
@@ -949,9 +983,15 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
       for (int v = 0; v < 4; ++v) {
         qm[v] = Pin[v * pl + o0]; qA[v] = Pin[v * pl + o1]; qB[v] = Pin[v * pl + o2]; qN[v] = Pin[v * pl + o3];
       }
+      if (kPrefetchRows == 2) {
+        const long long o4 = at(min(item.y0 + 2, rmax));
+#pragma unroll
+        for (int v = 0; v < 4; ++v) qN2[v] = Pin[v * pl + o4];
+        dlyN2 = dlyp[o4];
+      }
       const double dlm = dlyp[o0];
       dlyA = dlyp[o1]; dlyB = dlyp[o2]; dlyN = dlyp[o3];
-      load_geo(gnext, at(item.y0 - 1), dlxp, fixp, fiyp, filp, fjxp, fjyp, fjlp, surp, dtlp, PO, pl, false, m11p, false);
+      { const long long og = at(item.y0 - 1); fjxN = fjxp[og]; fjyN = fjyp[og]; fjlN = fjlp[og]; }
       const FT l0 = (FT)(0.5 * (dlm + dlyA)), l1 = (FT)(0.5 * (dlyA + dlyB));
       FT s0[4];
 #pragma unroll
@@ -967,19 +1007,38 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
     double Eprev[3] = {0.0, 0.0, 0.0};                 // diffusive flux of j-face r-1 (VISC)
 
     for (int r = item.y0 - 1; r <= item.y1; ++r) {
-      // software pipeline: this iteration's row r+2 was loaded one iteration ago; issue row r+3
-      // and the per-row geometry of row r+1 now, to be consumed by the next iteration.
+      // software pipeline: this iteration's row r+2 was loaded one iteration ago; issue row r+3 and
+      // the next row's j-face geometry now, to be consumed by the next iteration.  The rest of row r's
+      // data (i-face geometry, sur, dt, RK base state) is issued here and first used after the j-face
+      // flux, which hides its latency without holding a second row of it in registers.
 #pragma unroll
       for (int v = 0; v < 4; ++v) qC[v] = qN[v];
       dlyC = dlyN;
-      gcur = gnext;
+      gcur.fjx = fjxN; gcur.fjy = fjyN; gcur.fjl = fjlN;
       {
-        const long long oN = at(min(r + 3, rmax));
+        if (kPrefetchRows == 2) {                      // rows r+3 (in flight since the last iteration) and r+4
 #pragma unroll
-        for (int v = 0; v < 4; ++v) qN[v] = Pin[v * pl + oN];
-        dlyN = dlyp[oN];
-        load_geo(gnext, at(min(r + 1, rmax)), dlxp, fixp, fiyp, filp, fjxp, fjyp, fjlp, surp, dtlp, PO, pl,
-                 own && !tacc, m11p, false);
+          for (int v = 0; v < 4; ++v) qN[v] = qN2[v];
+          dlyN = dlyN2;
+          const long long oN = at(min(r + 4, rmax));
+#pragma unroll
+          for (int v = 0; v < 4; ++v) qN2[v] = Pin[v * pl + oN];
+          dlyN2 = dlyp[oN];
+        } else {
+          const long long oN = at(min(r + 3, rmax));
+#pragma unroll
+          for (int v = 0; v < 4; ++v) qN[v] = Pin[v * pl + oN];
+          dlyN = dlyp[oN];
+        }
+        const long long og = at(min(r + 1, rmax));
+        fjxN = fjxp[og]; fjyN = fjyp[og]; fjlN = fjlp[og];
+        if (r >= item.y0) {
+          const long long oc = at(r);
+          gcur.dlx = dlxp[oc]; gcur.fix = fixp[oc]; gcur.fiy = fiyp[oc]; gcur.fil = filp[oc]; gcur.sur = surp[oc];
+          gcur.dtl = (own && !tacc) ? dtlp[oc] : 0.0;
+#pragma unroll
+          for (int v = 0; v < 4; ++v) gcur.po[v] = PO[v * pl + oc];
+        }
       }
       FT sBC[4], phiB[4], Gcur[4];
       {
@@ -1213,11 +1272,21 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
       if (lane == 0 && dtnext < 1e300) atomicMin(&K.dtmin_next_bits, (unsigned long long)__double_as_longlong(dtnext));
     }
   }
+  if (tslotp && lane == 0) atomicMax(&tslotp[1], (unsigned long long)wall_clock64());
+}
+
+// Timing slots: clear every sub-slot and the step counter.
+__global__ void k_tstamp_init(unsigned long long* tstamp, int nsub) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t == 0) tstamp[0] = 0ull;
+  if (t < nsub) { tstamp[2 + 2 * t] = ~0ull; tstamp[2 + 2 * t + 1] = 0ull; }
 }
 
 // Per-case residual reduction and end-of-step control — Mod_Explicit.f90:120-168.
-__global__ __launch_bounds__(256) void k_finish(DevCtrl* ctrl, const DevCase* cases, const double* partials, int ncases) {
+__global__ __launch_bounds__(256) void k_finish(DevCtrl* ctrl, const DevCase* cases, const double* partials, int ncases,
+                                                unsigned long long* tstamp) {
   const int c = blockIdx.x;
+  if (tstamp && c == 0 && threadIdx.x == 0) tstamp[0] = tstamp[0] + 1ull;   // end of a timed step
   if (c >= ncases) return;
   DevCtrl& K = ctrl[c];
   const DevCase& C = cases[c];
@@ -1293,12 +1362,16 @@ struct q2d_handle {
   double* d_partials = nullptr;
   BCFlux* d_bcflux = nullptr;
   int range[5] = {0, 0, 0, 0, 0}; // items sorted into [range[k], range[k+1]): k = 2*viscous + riemann
+  std::vector<DtRange> dtr;      // dt launch: units of up to kDtCells cells of one live block
+  DtRange* d_dtr = nullptr;
+  int dt_total = 0;
   bool any_visc = false;
   std::vector<int> vrec;         // boundary records with a viscous flux (BC 300/302 of model-1 cases)
   int* d_vrec = nullptr;
-  std::vector<hipEvent_t> tev;   // per-launch event pairs of the timed stage launches of one step
+  unsigned long long* d_tstamp = nullptr;   // [0] step counter, then (first start, last end) per stage launch
+  int timed_steps_pending = 0;               // timed steps issued since the slots were last read
+  double wallclock_khz = 100000.0;
   int tslot = 0;
-  bool capturing = false;
   std::vector<char> case_live;   // removed cases keep their index but no blocks, records or items
   bool prec_set = false;
   size_t arena_bytes = 0;
@@ -1333,6 +1406,17 @@ int build_items(q2d_handle* h) {
     }
   }
   h->range[4] = (int)h->items.size();
+  h->dtr.clear();
+  for (int b = 0; b < h->nblocks; ++b) {
+    if (!h->hb[b].arena || !h->case_live[h->hb[b].case_id]) continue;
+    const int n = h->hb[b].ni * h->hb[b].nj;
+    for (int st = 0; st < n; st += kDtCells) h->dtr.push_back(DtRange{b, st, std::min(kDtCells, n - st), h->hb[b].ni});
+  }
+  h->dt_total = (int)h->dtr.size();
+  (void)hipFree(h->d_dtr); h->d_dtr = nullptr;
+  HIPCHK(hipMalloc(&h->d_dtr, sizeof(DtRange) * std::max<size_t>(1, h->dtr.size())));
+  if (!h->dtr.empty())
+    HIPCHK(hipMemcpy(h->d_dtr, h->dtr.data(), sizeof(DtRange) * h->dtr.size(), hipMemcpyHostToDevice));
   if (h->d_items) { (void)hipFree(h->d_items); h->d_items = nullptr; }
   if (h->d_partials) { (void)hipFree(h->d_partials); h->d_partials = nullptr; }
   HIPCHK(hipMalloc(&h->d_items, sizeof(Item) * std::max<size_t>(1, h->items.size())));
@@ -1358,17 +1442,18 @@ int launch_stage(q2d_handle* h, int irk) {
     int groups = (hi - lo + h->waves_per_group - 1) / h->waves_per_group;
     if (h->max_groups > 0) groups = std::min(groups, h->max_groups);
     groups = std::max(groups, 1);
-    hipEvent_t e0 = nullptr, e1 = nullptr;
+    unsigned long long* ts = nullptr;
+    int slot = 0;
     if (h->timing) {
-      if ((size_t)(2 * h->tslot + 2) > h->tev.size()) return fail(Q2D_E_HIP, "timing event pool exhausted");
-      e0 = h->tev[2 * h->tslot]; e1 = h->tev[2 * h->tslot + 1]; h->tslot++;
+      if (h->tslot >= kLaunchesPerStep) return fail(Q2D_E_HIP, "timing slots exhausted");
+      ts = h->d_tstamp; slot = h->tslot++;
     }
     const int* bf = h->d_bcface;
     const BCFlux* bq = h->d_bcflux;
 #define Q2D_STAGE_LAUNCH(RIE, VISC)                                                                               \
     hipLaunchKernelGGL((k_stage<FT, RIE, FUSE, VISC>), dim3(groups), dim3(tpb), 0, h->stream, h->d_blocks,        \
-                       h->d_cases, h->d_ctrl, h->d_items, lo, hi, bf, bq, h->d_partials, irk, h->xcd_remap)
-    if (e0) HIPCHK(hipEventRecord(e0, h->stream));
+                       h->d_cases, h->d_ctrl, h->d_items, lo, hi, bf, bq, h->d_partials, irk, h->xcd_remap, ts,  \
+                       slot)
     switch (pass) {
       case 0: Q2D_STAGE_LAUNCH(0, false); break;
       case 1: Q2D_STAGE_LAUNCH(1, false); break;
@@ -1376,49 +1461,66 @@ int launch_stage(q2d_handle* h, int irk) {
       default: Q2D_STAGE_LAUNCH(1, true); break;
     }
 #undef Q2D_STAGE_LAUNCH
-    if (e1) HIPCHK(hipEventRecord(e1, h->stream));
   }
   return Q2D_OK;
 }
 
-// Sum the event-pair intervals of the stage launches issued since the last collection.
+// Clear the launch-timing slots and step counter (a kernel, so it is ordered on the stream).
+int reset_timing(q2d_handle* h) {
+  hipLaunchKernelGGL(k_tstamp_init, dim3((kTimedSlots * kStampLanes + 255) / 256), dim3(256), 0, h->stream,
+                     h->d_tstamp, kTimedSlots * kStampLanes);
+  HIPCHK(hipGetLastError());
+  h->timed_steps_pending = 0;
+  return Q2D_OK;
+}
+
+// Add the durations of the stage launches of the steps timed since the last reset, then reset.
 int collect_timing(q2d_handle* h) {
-  const int n = h->tslot;
-  h->tslot = 0;
-  if (n == 0) return Q2D_OK;
-  HIPCHK(hipEventSynchronize(h->tev[2 * n - 1]));
-  for (int k = 0; k < n; ++k) {
-    float ms = 0.f;
-    HIPCHK(hipEventElapsedTime(&ms, h->tev[2 * k], h->tev[2 * k + 1]));
-    h->timed_ms += ms;
+  const int nsteps = h->timed_steps_pending;
+  if (nsteps == 0) return Q2D_OK;
+  std::vector<unsigned long long> ts(2 + 2 * (size_t)kTimedSlots * kStampLanes);
+  HIPCHK(hipMemcpyAsync(ts.data(), h->d_tstamp, sizeof(unsigned long long) * ts.size(), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  for (int k = 0; k < std::min(nsteps, kTimedSteps) * kLaunchesPerStep; ++k) {
+    unsigned long long t0 = ~0ull, t1 = 0ull;
+    for (int l = 0; l < kStampLanes; ++l) {
+      t0 = std::min(t0, ts[2 + 2 * ((size_t)k * kStampLanes + l)]);
+      t1 = std::max(t1, ts[2 + 2 * ((size_t)k * kStampLanes + l) + 1]);
+    }
+    if (t1 < t0) continue;                            // unused slot, or a launch with no work
+    h->timed_ms += (double)(t1 - t0) / h->wallclock_khz;
     h->timed_launches += 1;
   }
-  return Q2D_OK;
+  return reset_timing(h);
 }
 
 int launch_step(q2d_handle* h) {
   if (h->ncases == 0) return Q2D_OK;
+  h->tslot = 0;
   const int n_items = (int)h->items.size();
   const int tpb = kWave * h->waves_per_group;
   int groups = (n_items + h->waves_per_group - 1) / h->waves_per_group;
   if (h->max_groups > 0) groups = std::min(groups, h->max_groups);
   groups = std::max(groups, 1);
   const int cthreads = 64, cgroups = (h->ncases + cthreads - 1) / cthreads;
-  const int bgroups = (h->nbc + 255) / 256;
+  const int bgroups = (h->nbc + 63) / 64;            // one wavefront per group: spread over every CU
   const int use_next =
       (h->dt_fused_valid && !h->any_visc && (h->precision_mixed ? kFuseDtMixed : kFuseDtFp64)) ? 1 : 0;
   hipLaunchKernelGGL(k_begin, dim3(cgroups), dim3(cthreads), 0, h->stream, h->d_ctrl, h->ncases, use_next);
   if (!use_next)
-    hipLaunchKernelGGL(k_dt, dim3(groups), dim3(tpb), 0, h->stream, h->d_blocks, h->d_cases, h->d_ctrl, h->d_items,
-                       n_items);
+    if (h->dt_total > 0) {                             // one wavefront per unit of kDtCells cells
+      const int dg = (h->dt_total + 3) / 4;
+      hipLaunchKernelGGL(k_dt, dim3(dg), dim3(256), 0, h->stream, h->d_blocks, h->d_cases, h->d_ctrl, h->d_dtr,
+                         h->dt_total, 0);
+    }
   hipLaunchKernelGGL(k_dt_finish, dim3(cgroups), dim3(cthreads), 0, h->stream, h->d_ctrl, h->d_cases, h->ncases);
   for (int irk = 1; irk <= h->max_rk; ++irk) {
     if (h->nbc > 0) {
       if (h->precision_mixed)
-        hipLaunchKernelGGL(k_bcflux<float>, dim3(bgroups), dim3(256), 0, h->stream, h->d_blocks, h->d_cases, h->d_ctrl,
+        hipLaunchKernelGGL(k_bcflux<float>, dim3(bgroups), dim3(64), 0, h->stream, h->d_blocks, h->d_cases, h->d_ctrl,
                            h->d_bc, h->nbc, irk, h->d_bcflux);
       else
-        hipLaunchKernelGGL(k_bcflux<double>, dim3(bgroups), dim3(256), 0, h->stream, h->d_blocks, h->d_cases, h->d_ctrl,
+        hipLaunchKernelGGL(k_bcflux<double>, dim3(bgroups), dim3(64), 0, h->stream, h->d_blocks, h->d_cases, h->d_ctrl,
                            h->d_bc, h->nbc, irk, h->d_bcflux);
     }
     if (!h->vrec.empty())
@@ -1433,12 +1535,9 @@ int launch_step(q2d_handle* h) {
     int rc = h->precision_mixed ? launch_stage<float, kFuseDtMixed>(h, irk) : launch_stage<double, kFuseDtFp64>(h, irk);
     if (rc != Q2D_OK) return rc;
   }
-  hipLaunchKernelGGL(k_finish, dim3(h->ncases), dim3(256), 0, h->stream, h->d_ctrl, h->d_cases, h->d_partials, h->ncases);
+  hipLaunchKernelGGL(k_finish, dim3(h->ncases), dim3(256), 0, h->stream, h->d_ctrl, h->d_cases, h->d_partials, h->ncases,
+                     h->timing ? h->d_tstamp : nullptr);
   HIPCHK(hipGetLastError());
-  if (h->timing && !h->capturing) {
-    int rc = collect_timing(h);
-    if (rc != Q2D_OK) return rc;
-  }
   h->dt_fused_valid = true;
   return Q2D_OK;
 }
@@ -1503,8 +1602,8 @@ int q2d_gpu_destroy(q2d_handle* h) {
   for (auto& b : h->hb) if (b.arena) (void)hipFree(b.arena);
   (void)hipFree(h->d_blocks); (void)hipFree(h->d_cases); (void)hipFree(h->d_ctrl); (void)hipFree(h->d_bc);
   (void)hipFree(h->d_bcface); (void)hipFree(h->d_items); (void)hipFree(h->d_partials); (void)hipFree(h->d_bcflux);
-  for (hipEvent_t e : h->tev) if (e) (void)hipEventDestroy(e);
-  (void)hipFree(h->d_vrec);
+  (void)hipFree(h->d_tstamp);
+  (void)hipFree(h->d_vrec); (void)hipFree(h->d_dtr);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
   return Q2D_OK;
@@ -1808,10 +1907,13 @@ int q2d_gpu_create(const q2d_problem* prob, q2d_handle** out) {
     if (hipGetDeviceProperties(&prop, 0) == hipSuccess) h->cu_count = prop.multiProcessorCount;
   }
   if (hipStreamCreate(&h->stream) != hipSuccess) return bail(fail(Q2D_E_HIP, "stream creation failed"));
-  h->tev.assign(2 * 3 * 4 * kTimedSteps, nullptr);    // start/stop x 3 stages x 4 kernel variants x steps
-  for (auto& e : h->tev)
-    if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess)
-      return bail(fail(Q2D_E_HIP, "event creation failed"));
+  {
+    int dev = 0, khz = 0;
+    if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) == hipSuccess && khz > 0)
+      h->wallclock_khz = khz;
+  }
+  if (hipMalloc(&h->d_tstamp, sizeof(unsigned long long) * (2 + 2 * (size_t)kTimedSlots * kStampLanes)) != hipSuccess)
+    return bail(fail(Q2D_E_NOMEM, "timing buffer allocation failed"));
   int rc = append_cases(h, prob, nullptr);
   if (rc == Q2D_OK && h->ncases == 0) rc = rebuild_descriptors(h);
   if (rc != Q2D_OK) {
@@ -1874,9 +1976,18 @@ int q2d_gpu_case_count(q2d_handle* h, int32_t* ncases, int32_t* nblocks, int64_t
   return Q2D_OK;
 }
 
+// After one more timed step has been issued: read the slots when the ring is full.
+int count_timed_step(q2d_handle* h) {
+  if (!h->timing) return Q2D_OK;
+  if (++h->timed_steps_pending >= kTimedSteps) return collect_timing(h);
+  return Q2D_OK;
+}
+
 int q2d_gpu_step(q2d_handle* h, q2d_step_out* out) {
   if (!h) return fail(Q2D_E_ARG, "NULL handle");
   int rc = launch_step(h);
+  if (rc == Q2D_OK) rc = count_timed_step(h);
+  if (rc == Q2D_OK && h->timing) rc = collect_timing(h);
   if (rc != Q2D_OK) return rc;
   int err = Q2D_OK;
   rc = read_out(h, out, &err);
@@ -1889,35 +2000,10 @@ int q2d_gpu_run(q2d_handle* h, int32_t nsteps, q2d_step_out* out) {
   if (!h) return fail(Q2D_E_ARG, "NULL handle");
   if (nsteps < 0) return fail(Q2D_E_ARG, "nsteps < 0");
   if (h->ncases == 0) return Q2D_OK;
-  if (h->timing && nsteps > 0) {
-    // Timed steps: up to kTimedSteps consecutive steps captured into one graph with an event pair around
-    // every stage launch (events without the system-scope fence, so no cache flush is added), replayed
-    // once: the stage kernels run back to back as in the untimed graph, and every pair is read after.
-    h->tslot = 0;
-    if (!h->dt_fused_valid) { int rc = launch_step(h); if (rc == Q2D_OK) rc = collect_timing(h); if (rc != Q2D_OK) return rc; --nsteps; }
-    while (nsteps > 0) {
-      const int n = std::min(nsteps, kTimedSteps);
-      hipGraph_t g;
-      hipGraphExec_t ge = nullptr;
-      HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-      h->capturing = true;
-      int rc = Q2D_OK;
-      for (int k = 0; k < n && rc == Q2D_OK; ++k) rc = launch_step(h);
-      h->capturing = false;
-      hipError_t ce = hipStreamEndCapture(h->stream, &g);
-      if (rc != Q2D_OK) return rc;
-      if (ce != hipSuccess) return fail(Q2D_E_HIP, std::string("graph capture: ") + hipGetErrorString(ce));
-      HIPCHK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
-      (void)hipGraphDestroy(g);
-      hipError_t le = hipGraphLaunch(ge, h->stream);
-      rc = le == hipSuccess ? collect_timing(h) : fail(Q2D_E_HIP, "graph launch failed");
-      (void)hipGraphExecDestroy(ge);
-      if (rc != Q2D_OK) return rc;
-      nsteps -= n;
-    }
-  } else if (nsteps > 0) {
+  if (nsteps > 0) {
     if (!h->dt_fused_valid) {                          // first step after create/upload: eager, with k_dt
       int rc = launch_step(h);
+      if (rc == Q2D_OK) rc = count_timed_step(h);
       if (rc != Q2D_OK) return rc;
       --nsteps;
     }
@@ -1931,7 +2017,12 @@ int q2d_gpu_run(q2d_handle* h, int32_t nsteps, q2d_step_out* out) {
       HIPCHK(hipGraphInstantiate(&h->graph, g, nullptr, nullptr, 0));
       (void)hipGraphDestroy(g);
     }
-    for (int s = 0; s < nsteps; ++s) HIPCHK(hipGraphLaunch(h->graph, h->stream));
+    for (int s = 0; s < nsteps; ++s) {
+      HIPCHK(hipGraphLaunch(h->graph, h->stream));
+      int rc = count_timed_step(h);
+      if (rc != Q2D_OK) return rc;
+    }
+    if (h->timing) { int rc = collect_timing(h); if (rc != Q2D_OK) return rc; }
   }
   int err = Q2D_OK;
   int rc = read_out(h, out, &err);
@@ -1990,9 +2081,11 @@ int q2d_gpu_download_dt(q2d_handle* h, int32_t block, double* dtlocal) {
 
 int q2d_gpu_set_timing(q2d_handle* h, int32_t enable) {
   if (!h) return fail(Q2D_E_ARG, "NULL handle");
-  h->timing = enable != 0;
+  const bool on = enable != 0;
+  if (on != h->timing && h->graph) { (void)hipGraphExecDestroy(h->graph); h->graph = nullptr; }   // launch args change
+  h->timing = on;
   h->timed_launches = 0; h->timed_ms = 0.0;
-  return Q2D_OK;
+  return h->timing ? reset_timing(h) : Q2D_OK;
 }
 
 int q2d_gpu_timing_stats(q2d_handle* h, int64_t* launches, double* ms_total) {
