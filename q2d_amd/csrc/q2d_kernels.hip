@@ -622,31 +622,32 @@ __device__ __forceinline__ void grad_m(double g0, double g1, const double M[4], 
   o1 = g0 * M[2] + g1 * M[3];
 }
 
+// One cell's primitive state and temperature T = p / (rho R) (co_rotot_Rtot + the EOS, one species).
+struct VC { double r, u, v, p, T; };
+__device__ __forceinline__ VC vc_shdn(const VC& c) { return VC{shdn(c.r), shdn(c.u), shdn(c.v), shdn(c.p), shdn(c.T)}; }
+__device__ __forceinline__ VC vc_shup(const VC& c) { return VC{shup(c.r), shup(c.u), shup(c.v), shup(c.p), shup(c.T)}; }
+__device__ __forceinline__ void vc_temperature(VC& c, double R) { c.T = c.p / (c.r * R); }
+
 // Diffusive_Flux + Tangential_Gradient + Compute_Diffusive_Flux — Lib_Diffusive.f90:10-199 for one interior
-// face.  q1, q2: the face's two cells; q3..q6: the tangential stencil (q4 - q3 + q6 - q5) / 4; a: 0 for an
+// face.  c1, c2: the face's two cells; c3..c6: the tangential stencil (c4 - c3 + c6 - c5) / 4; a: 0 for an
 // i-face, 1 for a j-face; M: the two cells' metric averaged.  One species: the species flux and the
 // enthalpy-diffusion term vanish (grad Y = 0), so D = (Flux(nu), Flux(nv), Flux(np)).
 __device__ __forceinline__ void visc_face(const DevCase& C, bool axi, int a, double n1, double n2, double il,
-                                          const double q1[4], const double q2[4], const double q3[4],
-                                          const double q4[4], const double q5[4], const double q6[4],
-                                          const double M[4], double r_ave, double D[3]) {
-  const double R = C.R;
-  const double T1 = q1[3] / (q1[0] * R), T2 = q2[3] / (q2[0] * R);
-  const double T3 = q3[3] / (q3[0] * R), T4 = q4[3] / (q4[0] * R);
-  const double T5 = q5[3] / (q5[0] * R), T6 = q6[3] / (q6[0] * R);
-  const double nU = q2[1] - q1[1], nV = q2[2] - q1[2], nT = T2 - T1;
-  const double tU = (q4[1] - q3[1] + q6[1] - q5[1]) * 0.25;
-  const double tV = (q4[2] - q3[2] + q6[2] - q5[2]) * 0.25;
-  const double tT = (T4 - T3 + T6 - T5) * 0.25;
+                                          const VC& c1, const VC& c2, const VC& c3, const VC& c4, const VC& c5,
+                                          const VC& c6, const double M[4], double r_ave, double D[3]) {
+  const double nU = c2.u - c1.u, nV = c2.v - c1.v, nT = c2.T - c1.T;
+  const double tU = (c4.u - c3.u + c6.u - c5.u) * 0.25;
+  const double tV = (c4.v - c3.v + c6.v - c5.v) * 0.25;
+  const double tT = (c4.T - c3.T + c6.T - c5.T) * 0.25;
   double gUx, gUy, gVx, gVy, gTx, gTy;
   if (a == 0) {
     grad_m(nU, tU, M, gUx, gUy); grad_m(nV, tV, M, gVx, gVy); grad_m(nT, tT, M, gTx, gTy);
   } else {
     grad_m(tU, nU, M, gUx, gUy); grad_m(tV, nV, M, gVx, gVy); grad_m(tT, nT, M, gTx, gTy);
   }
-  const double pm0 = 0.5 * (q1[0] + q2[0]), pm1 = 0.5 * (q1[1] + q2[1]);
-  const double pm2 = 0.5 * (q1[2] + q2[2]), pm3 = 0.5 * (q1[3] + q2[3]);
-  const double T = pm3 / (pm0 * R);
+  const double pm0 = 0.5 * (c1.r + c2.r), pm1 = 0.5 * (c1.u + c2.u);
+  const double pm2 = 0.5 * (c1.v + c2.v), pm3 = 0.5 * (c1.p + c2.p);
+  const double T = pm3 / (pm0 * C.R);
   const double mil = mu_lam(C, T);
   const double kappa = k_lam(C, mil) + 0.0;
   double S1, S2;
@@ -657,12 +658,14 @@ __device__ __forceinline__ void visc_face(const DevCase& C, bool axi, int a, dou
 }
 
 // Interior diffusive face fluxes and the axisymmetric viscous source of every cell of the viscous work
-// items (one wavefront per item, one lane per column).  Cell (x,r) stores the flux of its i-face (x|x+1)
-// (Mod_Fluxes.f90:160-183), of its j-face (r|r+1) (:207-230) and its Source_Axi_Visc_Blk term
-// (Lib_Source_Axisymmetry.f90:81-247); the stage kernel adds them to the residual in the reference's order.
-// The faces read the ghost cells as Fill_Ghost_Cell left them; the source reads a wall face's ghost as
-// Ghost_Wall_Extrapolation (Lib_Ghost.f90:366-399) rewrites it just before the sources: the boundary
-// cell's own state with the velocity reversed.
+// items.  Cell (x,r) stores the flux of its i-face (x|x+1) (Mod_Fluxes.f90:160-183), of its j-face
+// (r|r+1) (:207-230) and its Source_Axi_Visc_Blk term (Lib_Source_Axisymmetry.f90:81-247); the stage
+// kernel adds them to the residual in the reference's order.  Like the stage kernel, one wavefront
+// marches down a 60-column strip: rows r-1, r, r+1 (state and temperature, metric, node radius) roll
+// through registers and the x-1 / x+1 neighbours come from the adjacent lanes (DPP), so every cell's
+// state is loaded and its temperature computed once.  The faces read the ghost cells as Fill_Ghost_Cell
+// left them; the source reads a wall face's ghost as Ghost_Wall_Extrapolation (Lib_Ghost.f90:366-399)
+// rewrites it just before the sources: the boundary cell's own state with the velocity reversed.
 __global__ __launch_bounds__(256) void k_visc(const DevBlock* __restrict__ blocks, const DevCase* __restrict__ cases,
                                               const DevCtrl* __restrict__ ctrl, const Item* __restrict__ items,
                                               int item_lo, int item_hi, const DevBC* __restrict__ bcs,
@@ -677,92 +680,99 @@ __global__ __launch_bounds__(256) void k_visc(const DevBlock* __restrict__ block
     const DevCtrl& K = ctrl[B.case_id];
     const DevCase& C = cases[B.case_id];
     if (!K.active || irk > C.n_rk) continue;
-    const int x = item.x0 - 2 + lane;
-    if (!(lane >= 2 && lane < 2 + kOwn && x <= B.ni)) continue;
     const int ni = B.ni, nj = B.nj, pitch = B.pitch;
     const long long pl = B.plane;
     const bool axi = C.source == Q2D_SRC_AXI;
+    const double R = C.R;
+    const int x = item.x0 - 2 + lane;                  // Fortran column of this lane
+    const int xc = x <= ni + 2 ? x : ni + 2;           // clamped column for loads
+    const bool own = lane >= 2 && lane < 2 + kOwn && x <= ni;
+    const int rmax = nj + 2;
     GP P = gp(B.P[stage_in(K.cur, irk)]);
-    GP nodr = gp(B.nodr);
+    GP m11 = gp(B.m11), m21 = gp(B.m21), m12 = gp(B.m12), m22 = gp(B.m22), nodr = gp(B.nodr);
     GPW vdx = gpw(B.vdx), vey = gpw(B.vey), vsv = gpw(B.vsv);
-    auto at = [pitch, x](int dx, int r) { return (long long)(r + 1) * pitch + (x + dx + 1); };
-    auto ld = [&](long long o, double q[4]) {
-#pragma unroll
-      for (int v = 0; v < 4; ++v) q[v] = P[v * pl + o];
-    };
-    auto ldM = [&](long long o, double M[4]) {
-      M[0] = gp(B.m11)[o]; M[1] = gp(B.m21)[o]; M[2] = gp(B.m12)[o]; M[3] = gp(B.m22)[o];
-    };
+    auto at = [&](int r) { return (long long)(min(r, rmax) + 1) * pitch + (xc + 1); };
+    auto ldc = [&](int r) { const long long o = at(r); return VC{P[o], P[pl + o], P[2 * pl + o], P[3 * pl + o], 0.0}; };
+    auto ldm = [&](int r, double M[4]) { const long long o = at(r); M[0] = m11[o]; M[1] = m21[o]; M[2] = m12[o]; M[3] = m22[o]; };
     // wall faces of the boundary cells of this column (BC 301..304)
     auto is_wall = [&](int f, int pos) {
       if (B.bcf[f] < 0) return false;
       const int t = bcs[bcface[B.bcf[f] + pos - 1]].type;
       return t >= 301 && t <= 304;
     };
-    const bool wall3 = axi && is_wall(2, x), wall4 = axi && is_wall(3, x);
+    const bool wall3 = axi && own && is_wall(2, x), wall4 = axi && own && is_wall(3, x);
+    // rows y0-1, y0, y0+1 resident; row y0+2 in flight
+    VC cS = ldc(item.y0 - 1), cC = ldc(item.y0), cN = ldc(item.y0 + 1), cF = ldc(item.y0 + 2);
+    vc_temperature(cS, R); vc_temperature(cC, R); vc_temperature(cN, R);
+    double MC[4], MN[4], MF[4];
+    ldm(item.y0, MC); ldm(item.y0 + 1, MN); ldm(item.y0 + 2, MF);
+    double nodS = nodr[at(item.y0 - 1)], nodC = nodr[at(item.y0)], nodN = nodr[at(item.y0 + 1)];
     for (int r = item.y0; r <= item.y1; ++r) {
-      const long long o = at(0, r);
-      double q0[4], qE[4], qW[4], qS[4], qN[4], qNE[4], qSE[4], qNW[4], M0[4];
-      ld(o, q0); ld(at(1, r), qE); ld(at(-1, r), qW); ld(at(0, r - 1), qS); ld(at(0, r + 1), qN);
-      ld(at(1, r + 1), qNE); ld(at(1, r - 1), qSE); ld(at(-1, r + 1), qNW);
-      ldM(o, M0);
-      if (x <= ni - 1) {                               // i-face (x | x+1)
-        double M1[4], Ma[4], D[3];
-        ldM(at(1, r), M1);
+      const long long o = at(r);
+      // this row's face geometry and source terms (first used after the neighbour shifts)
+      const double fix = gp(B.fix)[o], fiy = gp(B.fiy)[o], fil = gp(B.fil)[o];
+      const double fjx = gp(B.fjx)[o], fjy = gp(B.fjy)[o], fjl = gp(B.fjl)[o];
+      const double sa = axi ? gp(B.sa)[o] : 0.0, sb = axi ? gp(B.sb)[o] : 0.0;
+      const VC cE = vc_shdn(cC), cW = vc_shup(cC);      // (x+1, r), (x-1, r)
+      const VC cSE = vc_shdn(cS);                      // (x+1, r-1)
+      const VC cNE = vc_shdn(cN), cNW = vc_shup(cN);    // (x+1, r+1), (x-1, r+1)
+      double ME[4];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) Ma[k] = 0.5 * (M0[k] + M1[k]);
-        const double r_ave = 0.5 * (nodr[o] + nodr[at(0, r - 1)]);
-        visc_face(C, axi, 0, gp(B.fix)[o], gp(B.fiy)[o], gp(B.fil)[o], q0, qE, qS, qN, qSE, qNE, Ma, r_ave, D);
+      for (int k = 0; k < 4; ++k) ME[k] = shdn(MC[k]);
+      const double nodW = shup(nodC);                  // node(x-1, r)
+      if (own && x <= ni - 1) {                        // i-face (x | x+1)
+        double Ma[4], D[3];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) Ma[k] = 0.5 * (MC[k] + ME[k]);
+        visc_face(C, axi, 0, fix, fiy, fil, cC, cE, cS, cN, cSE, cNE, Ma, 0.5 * (nodC + nodS), D);
 #pragma unroll
         for (int k = 0; k < 3; ++k) vdx[k * pl + o] = D[k];
       }
-      if (r <= nj - 1) {                               // j-face (r | r+1)
-        double M1[4], Ma[4], D[3];
-        ldM(at(0, r + 1), M1);
+      if (own && r <= nj - 1) {                        // j-face (r | r+1)
+        double Ma[4], D[3];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) Ma[k] = 0.5 * (M0[k] + M1[k]);
-        const double r_ave = 0.5 * (nodr[o] + nodr[at(-1, r)]);
-        visc_face(C, axi, 1, gp(B.fjx)[o], gp(B.fjy)[o], gp(B.fjl)[o], q0, qN, qW, qE, qNW, qNE, Ma, r_ave, D);
+        for (int k = 0; k < 4; ++k) Ma[k] = 0.5 * (MC[k] + MN[k]);
+        visc_face(C, axi, 1, fjx, fjy, fjl, cC, cN, cW, cE, cNW, cNE, Ma, 0.5 * (nodC + nodW), D);
 #pragma unroll
         for (int k = 0; k < 3; ++k) vey[k * pl + o] = D[k];
       }
-      if (axi) {
+      if (axi && own) {
         // Source_Axi_Visc_Blk :125-236 with the wall ghosts reflected
-        const bool w1 = (x == 1) && is_wall(0, r), w2 = (x == ni) && is_wall(1, r);
-        const bool w3 = (r == 1) && wall3, w4 = (r == nj) && wall4;
-        auto refl = [&](double q[4]) { q[0] = q0[0]; q[1] = -q0[1]; q[2] = -q0[2]; q[3] = q0[3]; };
-        if (w1) refl(qW);
-        if (w2) refl(qE);
-        if (w3) refl(qS);
-        if (w4) refl(qN);
-        const double R = C.R;
-        const double rho = q0[0];
-        const double T = q0[3] / (rho * R);
-        const double T1 = qE[3] / (qE[0] * R), T2 = qW[3] / (qW[0] * R);
-        const double T3 = qN[3] / (qN[0] * R), T4 = qS[3] / (qS[0] * R);
-        const double mil = mu_lam(C, T);
+        VC qE = cE, qW = cW, qS = cS, qN = cN;
+        const VC refl{cC.r, -cC.u, -cC.v, cC.p, cC.T};
+        if (x == 1 && is_wall(0, r)) qW = refl;
+        if (x == ni && is_wall(1, r)) qE = refl;
+        if (r == 1 && wall3) qS = refl;
+        if (r == nj && wall4) qN = refl;
+        const double mil = mu_lam(C, cC.T);
         const double kl = k_lam(C, mil);
         double gUx, gUy, gVx, gVy, gTx, gTy;
-        grad_m((qE[1] - qW[1]) * 0.5, (qN[1] - qS[1]) * 0.5, M0, gUx, gUy);
-        grad_m((qE[2] - qW[2]) * 0.5, (qN[2] - qS[2]) * 0.5, M0, gVx, gVy);
-        grad_m((T1 - T2) * 0.5, (T3 - T4) * 0.5, M0, gTx, gTy);
+        grad_m((qE.u - qW.u) * 0.5, (qN.u - qS.u) * 0.5, MC, gUx, gUy);
+        grad_m((qE.v - qW.v) * 0.5, (qN.v - qS.v) * 0.5, MC, gVx, gVy);
+        grad_m((qE.T - qW.T) * 0.5, (qN.T - qS.T) * 0.5, MC, gTx, gTy);
         (void)gTx;
         const double kappa = kl + 0.0;
-        const double r_ave = gp(B.sa)[o], invr = gp(B.sb)[o];
         // Stress_Tensor_Axi — Lib_Fluid.f90:166-198
-        const double vr = q0[2];
+        const double vr = cC.v;
         const double mi = mil + 0.0;
         const double lam = kNegTwoThird * mi;
-        const double div = gUx + gVy + vr / r_ave;
+        const double div = gUx + gVy + vr / sa;
         const double ld2 = lam * div;
         const double tyy = ld2 + 2.0 * mi * gVy;
         const double txy = mi * (gUy + gVx);
-        const double tzz = 2.0 * mi * vr / r_ave + ld2;
+        const double tzz = 2.0 * mi * vr / sa + ld2;
         const double qr = -kappa * gTy;
-        vsv[o] = txy * invr;
-        vsv[pl + o] = (tyy - tzz) * invr;
-        vsv[2 * pl + o] = (-qr + q0[1] * txy + q0[2] * tyy + 0.0) * invr;
+        vsv[o] = txy * sb;
+        vsv[pl + o] = (tyy - tzz) * sb;
+        vsv[2 * pl + o] = (-qr + cC.u * txy + cC.v * tyy + 0.0) * sb;
       }
+      // roll the rows; issue row r+3
+      cS = cC; cC = cN; cN = cF; vc_temperature(cN, R);
+      cF = ldc(r + 3);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) { MC[k] = MN[k]; MN[k] = MF[k]; }
+      ldm(r + 3, MF);
+      nodS = nodC; nodC = nodN; nodN = nodr[at(r + 2)];
     }
   }
 }
