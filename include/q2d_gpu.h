@@ -230,8 +230,10 @@ int q2d_gpu_set_timing(q2d_handle *h, int32_t enable);
 int q2d_gpu_timing_stats(q2d_handle *h, int64_t *launches, double *ms_total);
 
 /* Launch-shape control of the stage kernel (rows per work item, waves per
- * workgroup); no counterpart in the reference.  0 keeps the current value.
- * Used by the tile autotuner. */
+ * workgroup); no counterpart in the reference.  0 keeps the current value;
+ * rows_per_item -1 picks, for every stage launch, the row count whose work
+ * items finish soonest on the wavefront slots the device holds at the
+ * kernel's occupancy (default: 16 rows, 4 waves).  Used by the tile autotuner. */
 int q2d_gpu_set_tiling(q2d_handle *h, int32_t rows_per_item, int32_t waves_per_group);
 int q2d_gpu_get_tiling(q2d_handle *h, int32_t *rows_per_item, int32_t *waves_per_group);
 

@@ -17,7 +17,7 @@ import time
 from dataclasses import replace
 from pathlib import Path
 
-ROWS = (8, 16, 32, 64)
+ROWS = (-1, 8, 16, 32, 64)          # -1: the library's occupancy-balanced row count per launch
 WAVES = (1, 2, 4)
 
 

@@ -37,7 +37,7 @@ def test_batch_matches_oracle_fp64(oracle_mod):
 def test_tiling_does_not_change_results():
     pb = cases.batch_uneven(seed=3, ncases=6, steps=30, precision="mixed", smax=1.0)
     ref = None
-    for rows, waves in ((8, 1), (16, 4), (64, 2)):
+    for rows, waves in ((8, 1), (16, 4), (64, 2), (-1, 4)):     # -1: occupancy-balanced rows per launch
         g = GpuSolver(pb)
         g.set_tiling(rows, waves)
         assert g.tiling() == (rows, waves)
