@@ -197,4 +197,96 @@ __device__ __forceinline__ void cons2prim(const Gas<double>& g, const double U[4
   q[0] = rho; q[1] = vx; q[2] = vy; q[3] = rho * g.R * T;
 }
 
+// ---- mixed form: FP32 face arithmetic --------------------------------------
+// The mixed-precision form evaluates reconstruction, limiter and Riemann flux in FP32 from FP32
+// geometry.  It is not held to the reference's operation order (it is checked against the FP64
+// form within a stated tolerance), so these functions use the closed forms the single-species
+// constant-cp closure reduces to, fused multiply-adds, and the hardware reciprocal, reciprocal
+// square root and square root.
+struct GasF { float gam, gm1, gam_gm1; };          // gamma, gamma - 1, gamma / (gamma - 1) = cp / R
+
+__device__ __forceinline__ float frcp(float a) { return __builtin_amdgcn_rcpf(a); }
+__device__ __forceinline__ float frsq(float a) { return __builtin_amdgcn_rsqf(a); }
+__device__ __forceinline__ float fsqrt(float a) { return __builtin_amdgcn_sqrtf(a); }
+
+// rlimiter(x, y) — Lib_Limiters.f90:26-112.  van Leer in its product form: for r = y/x > 0,
+// x (r + |r|) / (1 + |r|) = 2 x y / (x + y); 0 otherwise.  The others as in limiter<float>.
+__device__ __forceinline__ float limiter_mx(int id, float x, float y) {
+#pragma clang fp contract(fast)
+  if (id == 1) {                                          // minmod
+    if (x * y <= 0.f) return 0.f;
+    return __builtin_fabsf(x) > __builtin_fabsf(y) ? y : x;
+  }
+  if (id == 2) {                                          // van Leer
+    const float xy = x * y;
+    return xy > 0.f ? 2.f * xy * frcp(x + y) : 0.f;
+  }
+  return limiter<float>(id, x, y);
+}
+
+// HLLC (RIE 0, Batten et al. 1997 as in riemann_HLLC, Lib_Riemann_HLL.f90:34-123) or HLLE (RIE 1,
+// Einfeldt, riemann_HLLE :126-169) flux through a face of unit normal (nx, ny), times its length il,
+// from primitive states L, R = (rho, u, v, p).  Wave speeds S1 = min(0, unL - aL, un~ - a~),
+// S4 = max(0, unR + aR, un~ + a~) with Roe averages (roe_averages :229-317, which for one species
+// of constant cp reduces to a~^2 = (gamma - 1) (H~ - |u~|^2 / 2)).
+template <int RIE>
+__device__ __forceinline__ void flux_mx(const GasF& g, const float L[4], const float R[4], float nx, float ny,
+                                        float il, float F[4]) {
+#pragma clang fp contract(fast)
+  const float rsL = frsq(L[0]), rsR = frsq(R[0]);         // 1 / sqrt(rho)
+  const float sL = L[0] * rsL, sR = R[0] * rsR;           // sqrt(rho)
+  const float irL = rsL * rsL, irR = rsR * rsR;           // 1 / rho
+  const float aL = fsqrt(g.gam * L[3] * irL), aR = fsqrt(g.gam * R[3] * irR);
+  const float unL = L[1] * nx + L[2] * ny, unR = R[1] * nx + R[2] * ny;
+  const float hL = g.gam_gm1 * L[3] * irL + 0.5f * (L[1] * L[1] + L[2] * L[2]);   // h0 = cp T + |u|^2 / 2
+  const float hR = g.gam_gm1 * R[3] * irR + 0.5f * (R[1] * R[1] + R[2] * R[2]);
+  const float w = frcp(sL + sR);
+  const float ut = (sL * L[1] + sR * R[1]) * w, vt = (sL * L[2] + sR * R[2]) * w;
+  const float ht = (sL * hL + sR * hR) * w;
+  const float at = fsqrt(g.gm1 * (ht - 0.5f * (ut * ut + vt * vt)));
+  const float unt = ut * nx + vt * ny;
+  const float S1 = __builtin_fminf(__builtin_fminf(0.f, unL - aL), unt - at);
+  const float S4 = __builtin_fmaxf(__builtin_fmaxf(0.f, unR + aR), unt + at);
+  const float FrL = L[0] * unL, FrR = R[0] * unR;
+  if (RIE == 1) {
+    const float inv = frcp(S4 - S1), S14 = S1 * S4;
+    F[0] = (S4 * FrL - S1 * FrR + S14 * (R[0] - L[0])) * inv;
+    F[1] = (S4 * (FrL * L[1] + L[3] * nx) - S1 * (FrR * R[1] + R[3] * nx) + S14 * (R[0] * R[1] - L[0] * L[1])) * inv;
+    F[2] = (S4 * (FrL * L[2] + L[3] * ny) - S1 * (FrR * R[2] + R[3] * ny) + S14 * (R[0] * R[2] - L[0] * L[2])) * inv;
+    F[3] = (S4 * FrL * hL - S1 * FrR * hR + S14 * ((R[0] * hR - R[3]) - (L[0] * hL - L[3]))) * inv;
+  } else {
+    const float mL = L[0] * (S1 - unL), mR = R[0] * (S4 - unR);
+    const float Sst = (R[3] - L[3] + mL * unL - mR * unR) * frcp(mL - mR);
+    const float pst = L[0] * (unL - S1) * (unL - Sst) + L[3];
+    const bool left = (S1 >= 0.f) || (Sst >= 0.f);
+    const bool star = (S1 < 0.f) && ((Sst >= 0.f) || (S4 > 0.f));
+    const float Sk = left ? S1 : S4, dk = left ? L[0] : R[0], uk = left ? L[1] : R[1], vk = left ? L[2] : R[2];
+    const float pk = left ? L[3] : R[3], unk = left ? unL : unR, hk = left ? hL : hR, irk = left ? irL : irR;
+    float Fr = dk * unk;
+    float Fu = Fr * uk + pk * nx, Fv = Fr * vk + pk * ny, FE = Fr * hk;
+    if (star) {
+      const float id = frcp(Sk - Sst);
+      const float U1 = dk * (Sk - unk) * id;              // star-state density
+      const float dp = (pst - pk) * id;
+      const float e0k = hk - pk * irk;                    // total energy per unit mass
+      Fr = Fr + Sk * (U1 - dk);
+      Fu = Fu + Sk * (U1 * uk + dp * nx - dk * uk);
+      Fv = Fv + Sk * (U1 * vk + dp * ny - dk * vk);
+      FE = FE + Sk * (U1 * e0k + (pst * Sst - pk * unk) * id - dk * e0k);
+    }
+    F[0] = Fr; F[1] = Fu; F[2] = Fv; F[3] = FE;
+  }
+#pragma unroll
+  for (int v = 0; v < 4; ++v) F[v] = F[v] * il;
+}
+
+// FP64 reciprocal: hardware estimate refined by two Newton steps (within an ulp of 1/a).
+__device__ __forceinline__ double rcp_nr(double a) {
+  double r = __builtin_amdgcn_rcp(a);
+  double e = __builtin_fma(-a, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  e = __builtin_fma(-a, r, 1.0);
+  return __builtin_fma(r, e, r);
+}
+
 }  // namespace q2d

@@ -52,16 +52,18 @@ constexpr int kStampLanes = 64;                         // sub-slots per launch 
 #ifndef Q2D_PREFETCH_ROWS_MIXED
 #define Q2D_PREFETCH_ROWS_MIXED 1
 #endif
-// Fuse the next step's Compute_dt into the last RK stage (saves the separate dt pass over the
-// state).  The FP64 stage kernel keeps a separate dt pass: the fused version costs it registers.
+// Fuse the next step's Compute_dt into the last RK stage of Euler steps (saves the separate dt pass
+// over the state; the first step after create, upload or add_cases runs the dt pass).  Measured on
+// 4 x 1024^2, SSP-RK3: FP64 1.115 -> 1.040 ms per step.
 #ifndef Q2D_FUSE_DT_FP64
-#define Q2D_FUSE_DT_FP64 0
+#define Q2D_FUSE_DT_FP64 1
 #endif
 constexpr bool kFuseDtFp64 = Q2D_FUSE_DT_FP64 != 0;
-#ifndef Q2D_FUSE_DT_MIXED
-#define Q2D_FUSE_DT_MIXED 0
+// Fused next-step dt in the mixed-form Euler kernel (k_stage_mx): 0.625 -> 0.554 ms per step.
+#ifndef Q2D_FUSE_DT_MX
+#define Q2D_FUSE_DT_MX 1
 #endif
-constexpr bool kFuseDtMixed = Q2D_FUSE_DT_MIXED != 0;
+constexpr bool kFuseDtMx = Q2D_FUSE_DT_MX != 0;
 #ifndef Q2D_STAGE_MIN_WAVES
 #define Q2D_STAGE_MIN_WAVES 1
 #endif
@@ -105,6 +107,8 @@ struct DevBlock {
   const double* nodr;            // node(i,j)%c(2) at cell (i,j), i = 0..ni, j = 0..nj (viscous cases)
   double *vdx, *vey, *vsv;       // 3 planes each: diffusive flux of i-face (x|x+1), of j-face (r|r+1), and
                                  // the axisymmetric viscous source, components rho u, rho v, rho E
+  const float4 *mgi, *mgj, *mgd; // mixed-form Euler blocks: FP32 geometry (see k_stage_mx)
+  const float* mhy;
 };
 
 struct DevBC {
@@ -115,6 +119,8 @@ struct DevBC {
 };
 
 struct Item { int blk, x0, y0, y1; };
+// Per-item results of a step's last stage: residual-norm partials (4) and the fused next-step dt (1).
+constexpr int kPartials = 5;
 
 // Laminar transport of one species: Sutherland viscosity and Fourier conductivity k = mu cp / Pr, the
 // closed form behind the reference's f_laminarViscosity / co_k_mi_lam_Wilke calls (FP64 in both forms).
@@ -164,7 +170,7 @@ __device__ __forceinline__ float shup(float v) { return __shfl_up(v, 1, kWave); 
 #endif
 
 // Per-row data the stage kernel needs for the row it finalises.
-struct RowGeo { double dlx, fix, fiy, fil, fjx, fjy, fjl, sur, dtl, po[4]; };
+struct RowGeo { double dlx, fix, fiy, fil, fjx, fjy, fjl, sur, dtl, po[4], mm[4]; };
 __device__ __forceinline__ void load_geo(RowGeo& g, long long o, GP dlx, GP fix, GP fiy, GP fil, GP fjx, GP fjy,
                                          GP fjl, GP sur, GP dtl, GP PO, long long pl, bool want_dtl, GP m11,
                                          bool want_m) {
@@ -248,6 +254,27 @@ __device__ __forceinline__ double cell_dt(const DevCase& C, const DevCtrl& K, do
   return dtmin;
 }
 
+// Compute_dt of a cell of a mixed-form Euler block, from its FP32 geometry: half lengths hx, hy and the
+// normalised rows of M (gd); FP64 arithmetic.  The dt pass and the fused dt of k_stage_mx's last stage
+// both evaluate exactly this, so a step may take its dt from either.  iter: the iteration being started.
+__device__ __forceinline__ double cell_dt_mx(const DevCase& C, int iter, double rho, double u, double v, double p,
+                                             float hx, float hy, float g1x, float g1y, float g2x, float g2y) {
+  const double snd = sqrt(C.gam * p * rcp_nr(rho));
+  double dtmin = 1e8;
+  {
+    const double lambda = fabs(u * (double)g1x + v * (double)g1y) + snd;
+    const double dtd = 2.0 * (double)hx * rcp_nr(lambda) * C.cfl;
+    if (dtd < dtmin) dtmin = dtd;
+  }
+  if (C.ndir == 2) {
+    const double lambda = fabs(u * (double)g2x + v * (double)g2y) + snd;
+    const double dtd = 2.0 * (double)hy * rcp_nr(lambda) * C.cfl;
+    if (dtd < dtmin) dtmin = dtd;
+  }
+  if (iter < C.ramp_iter) dtmin = dtmin * (double)iter / (double)C.ramp_iter;   // Mod_dt.f90:43
+  return dtmin;
+}
+
 // Work unit of the dt pass: `count` (<= kDtCells) consecutive cells of one block, in row-major order from
 // the block's interior cell `start`.  One wavefront per unit: the block, case and control data are
 // wave-uniform (scalar loads), the lanes stream consecutive cells (coalesced row segments), four cells per
@@ -288,10 +315,19 @@ __global__ __launch_bounds__(256) void k_dt(const DevBlock* __restrict__ blocks,
         o[k] = (long long)(j + 2) * pitch + (i + 2);
       }
       double d[4];
+      if (B.mgd) {                                   // mixed-form Euler block: the fused dt's formula
+        const __attribute__((address_space(1))) float* gd = (const __attribute__((address_space(1))) float*)B.mgd;
 #pragma unroll
-      for (int k = 0; k < 4; ++k)
-        d[k] = cell_dt(C, K, P[o[k]], P[pl + o[k]], P[2 * pl + o[k]], P[3 * pl + o[k]], m11[o[k]], m21[o[k]], m12[o[k]],
-                       m22[o[k]], dlx[o[k]], dly[o[k]]);
+        for (int k = 0; k < 4; ++k)
+          d[k] = cell_dt_mx(C, K.iter, P[o[k]], P[pl + o[k]], P[2 * pl + o[k]], P[3 * pl + o[k]],
+                            (float)(0.5 * dlx[o[k]]), (float)(0.5 * dly[o[k]]), gd[4 * o[k]], gd[4 * o[k] + 1],
+                            gd[4 * o[k] + 2], gd[4 * o[k] + 3]);
+      } else {
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+          d[k] = cell_dt(C, K, P[o[k]], P[pl + o[k]], P[2 * pl + o[k]], P[3 * pl + o[k]], m11[o[k]], m21[o[k]],
+                         m12[o[k]], m22[o[k]], dlx[o[k]], dly[o[k]]);
+      }
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         if (!ok[k]) continue;
@@ -956,11 +992,15 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
     const bool last = (irk == C.n_rk);
     const int ni = B.ni, nj = B.nj, pitch = B.pitch;
     const long long pl = B.plane;
-    const int bin = stage_in(K.cur, irk), bout = stage_out(K.cur, irk);
+    // the case's control words, read once per item (wave-uniform; a read inside the row loop would be a
+    // vector load whose wait also drains the row prefetch)
+    const int kcur = __builtin_amdgcn_readfirstlane(K.cur), kdtsel = __builtin_amdgcn_readfirstlane(K.dtsel);
+    const int kiter = __builtin_amdgcn_readfirstlane(K.iter);
+    const int bin = stage_in(kcur, irk), bout = stage_out(kcur, irk);
     GP Pin = gp(B.P[bin]);
-    GP PO = gp(B.P[K.cur]);
+    GP PO = gp(B.P[kcur]);
     GPW Pout = gpw(B.P[bout]);
-    GP dlxp = gp(B.dlx), dlyp = gp(B.dly), surp = gp(B.sur), dtlp = gp(B.dtl[K.dtsel]);
+    GP dlxp = gp(B.dlx), dlyp = gp(B.dly), surp = gp(B.sur), dtlp = gp(B.dtl[kdtsel]);
     GP fixp = gp(B.fix), fiyp = gp(B.fiy), filp = gp(B.fil), fjxp = gp(B.fjx), fjyp = gp(B.fjy), fjlp = gp(B.fjl);
     GP m11p = gp(B.m11);
     const int lim = C.limiter;
@@ -1028,6 +1068,21 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
       dlyC = dlyN;
       gcur.fjx = fjxN; gcur.fjy = fjyN; gcur.fjl = fjlN;
       {
+        // row r's own data first: vector-memory waits are in issue order, so data consumed in this
+        // iteration is issued before the prefetch consumed in the next one
+        if (r >= item.y0) {
+          const long long oc = at(r);
+          gcur.dlx = dlxp[oc]; gcur.fix = fixp[oc]; gcur.fiy = fiyp[oc]; gcur.fil = filp[oc]; gcur.sur = surp[oc];
+          gcur.dtl = (own && !tacc) ? dtlp[oc] : 0.0;
+          if (irk != 1) {                              // stage 1 reads its RK base state as its input (qA)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) gcur.po[v] = PO[v * pl + oc];
+          }
+          if (FUSE && last) {                          // M of row r for the fused dt (c11, c21, c12, c22)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) gcur.mm[v] = m11p[v * pl + oc];
+          }
+        }
         if (kPrefetchRows == 2) {                      // rows r+3 (in flight since the last iteration) and r+4
 #pragma unroll
           for (int v = 0; v < 4; ++v) qN[v] = qN2[v];
@@ -1044,15 +1099,6 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
         }
         const long long og = at(min(r + 1, rmax));
         fjxN = fjxp[og]; fjyN = fjyp[og]; fjlN = fjlp[og];
-        if (r >= item.y0) {
-          const long long oc = at(r);
-          gcur.dlx = dlxp[oc]; gcur.fix = fixp[oc]; gcur.fiy = fiyp[oc]; gcur.fil = filp[oc]; gcur.sur = surp[oc];
-          gcur.dtl = (own && !tacc) ? dtlp[oc] : 0.0;
-          if (irk != 1) {                              // stage 1 reads its RK base state as its input (qA)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) gcur.po[v] = PO[v * pl + oc];
-          }
-        }
       }
       FT sBC[4], phiB[4], Gcur[4];
       {
@@ -1234,9 +1280,7 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
             // next step's Compute_dt would read exactly these values.
             const double snd = sqrt(C.gam * qn2[3] / qn2[0]);
             double dtmin = 1e8;
-            double mm[4];
-#pragma unroll
-            for (int v = 0; v < 4; ++v) mm[v] = m11p[v * pl + oA];   // c11, c21, c12, c22
+            const double* mm = gcur.mm;                 // c11, c21, c12, c22
             {
               const double m1 = mm[0], m2 = mm[2];
               const double nrm = sqrt(m1 * m1 + m2 * m2);
@@ -1253,9 +1297,9 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
               const double dtd = dlyA / lambda * C.cfl;
               if (dtd < dtmin) dtmin = dtd;
             }
-            const int iter_next = K.iter + 1;
+            const int iter_next = kiter + 1;
             if (iter_next < C.ramp_iter) dtmin = dtmin * (double)iter_next / (double)C.ramp_iter;
-            if (!tacc) gpw(B.dtl[K.dtsel ^ 1])[oA] = dtmin;
+            if (!tacc) gpw(B.dtl[kdtsel ^ 1])[oA] = dtmin;
             if (dtmin < dtnext) dtnext = dtmin;
           }
         }
@@ -1278,16 +1322,324 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
       for (int v = 0; v < 4; ++v) {
         double s = acc[v];
         for (int k = 32; k >= 1; k >>= 1) s = s + __shfl_xor(s, k, kWave);
-        if (lane == 0) partials[(long long)it * 4 + v] = s;
+        if (lane == 0) partials[(long long)it * kPartials + v] = s;
       }
     }
     if (last && FUSE) {
-      // next step's global dt: exact min, order independent
+      // next step's dt: the item's exact min, reduced over the case's items by k_finish (no atomics)
       for (int k = 32; k >= 1; k >>= 1) {
         const double o = __shfl_xor(dtnext, k, kWave);
         if (o < dtnext) dtnext = o;
       }
-      if (lane == 0 && dtnext < 1e300) atomicMin(&K.dtmin_next_bits, (unsigned long long)__double_as_longlong(dtnext));
+      if (lane == 0) partials[(long long)it * kPartials + 4] = dtnext;
+    }
+  }
+  if (tslotp && lane == 0) atomicMax(&tslotp[1], (unsigned long long)wall_clock64());
+}
+
+// ---------------------------------------------------------------------------
+// Mixed-form stage kernel for Euler cases.  The same work items, row march, lane layout, residual
+// order and update as k_stage, with the face work in FP32: the j-stencil rolls FP32 copies of the
+// FP64 rows, the geometry comes from FP32 planes (one float4 load per face family per row), and the
+// face flux is flux_mx.  The update stays FP64 (division-free prim2cons, one refined reciprocal for
+// cons2prim, FMA), and so do the residual norm, the dt and their reductions.
+//   GI(x, r) = (N(1), N(2), int_l of i-face x|x+1, dl_x / 2 of cell x)
+//   GJ(x, r) = (N(1), N(2), int_l of j-face r|r+1, 1 / sur of cell (x, r))
+//   HY(x, r) = dl_y / 2,  GD(x, r) = rows 1 and 2 of M, each normalised (Mod_dt.f90:76-84)
+// With FUSE the last RK stage also evaluates the next step's Compute_dt (Mod_dt.f90:59-104) on the
+// state it writes, so the step needs no separate dt pass.
+// ---------------------------------------------------------------------------
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef const __attribute__((address_space(1))) float* GPF;
+typedef const __attribute__((address_space(1))) f4v* GP4;
+
+#ifndef Q2D_STAGE_MX_MIN_WAVES
+#define Q2D_STAGE_MX_MIN_WAVES 1
+#endif
+template <int RIE, bool FUSE>
+__global__ __launch_bounds__(256, Q2D_STAGE_MX_MIN_WAVES) void k_stage_mx(const DevBlock* __restrict__ blocks, const DevCase* __restrict__ cases,
+                                                  DevCtrl* __restrict__ ctrl, const Item* __restrict__ items, int item_lo,
+                                                  int item_hi, const int* __restrict__ bcface,
+                                                  const BCFlux* __restrict__ bcflux, double* __restrict__ partials,
+                                                  int irk, int xcd_remap, unsigned long long* __restrict__ tstamp,
+                                                  int tslot) {
+  unsigned long long* tslotp = nullptr;
+  if (tstamp) {
+    const int launch = (int)(tstamp[0] % kTimedSteps) * kLaunchesPerStep + tslot;
+    tslotp = tstamp + 2 + 2 * ((long long)launch * kStampLanes + (blockIdx.x % kStampLanes));
+    if (threadIdx.x == 0) atomicMin(&tslotp[0], (unsigned long long)wall_clock64());
+  }
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wpg = blockDim.x / kWave;
+  int grp = blockIdx.x;
+  if (xcd_remap) {
+    const int ng = gridDim.x;
+    const int per = ng / 8;
+    if (per > 0 && grp < per * 8) grp = (grp % 8) * per + grp / 8;
+  }
+  const int wave0 = __builtin_amdgcn_readfirstlane(grp * wpg + threadIdx.x / kWave);
+  const int nwaves = gridDim.x * wpg;
+
+  for (int it = item_lo + wave0; it < item_hi; it += nwaves) {
+    const Item item = items[it];
+    const DevBlock& B = blocks[item.blk];
+    DevCtrl& K = ctrl[B.case_id];
+    const DevCase& C = cases[B.case_id];
+    if (!K.active || irk > C.n_rk) continue;
+    const bool last = (irk == C.n_rk);
+    const int ni = B.ni, nj = B.nj, pitch = B.pitch;
+    const long long pl = B.plane;
+    const int kcur = __builtin_amdgcn_readfirstlane(K.cur), kdtsel = __builtin_amdgcn_readfirstlane(K.dtsel);
+    const int kiter = __builtin_amdgcn_readfirstlane(K.iter);   // control words once per item (see k_stage)
+    const int bin = stage_in(kcur, irk), bout = stage_out(kcur, irk);
+    GP Pin = gp(B.P[bin]);
+    GP PO = gp(B.P[kcur]);
+    GPW Pout = gpw(B.P[bout]);
+    GP dtlp = gp(B.dtl[kdtsel]);
+    GPW dtl_next = gpw(B.dtl[kdtsel ^ 1]);
+    GP4 gip = (GP4)B.mgi, gjp = (GP4)B.mgj, gdp = (GP4)B.mgd;
+    GPF hyp = (GPF)B.mhy;
+    const int lim = C.limiter;
+    const bool tacc = C.time_accurate != 0;
+    const GasF gf{(float)C.gam, (float)(C.gam - 1.0), (float)(C.gam / (C.gam - 1.0))};
+    const double gm1 = C.gam - 1.0, rgm1 = 1.0 / (C.gam - 1.0);
+    const double c_rk = (C.n_rk == 1) ? 1.0 : (C.n_rk == 2 ? (irk == 1 ? 1.0 : 0.5)
+                                                            : (irk == 1 ? 1.0 : (irk == 2 ? 0.25 : 2.0 / 3.0)));
+    const double dtg = K.dtglobal;
+    const int stage_id = K.stage_count + irk;
+
+    const int x = item.x0 - 2 + lane;
+    const int xc = x <= ni + 2 ? x : ni + 2;
+    const bool own = lane >= 2 && lane < 2 + kOwn && x <= ni;
+    const bool xdo = lane >= 1 && lane <= 61 && x >= 1 && x <= ni - 1;
+    const int rec3 = (own && B.bcf[2] >= 0) ? bcface[B.bcf[2] + x - 1] : -1;
+    const int rec4 = (own && B.bcf[3] >= 0) ? bcface[B.bcf[3] + x - 1] : -1;
+    const int rmax = nj + 2;
+    auto at = [&](int r) { return (long long)(r + 1) * pitch + (xc + 1); };
+
+    // ---- pipeline warm-up: rows y0-2 .. y0+1 ------------------------------
+    double qA[4], qB[4], qN[4];                        // FP64 rows r, r+1 and r+2 (in flight)
+    float fA[4], fB[4];                                // their FP32 copies (rows r, r+1)
+    float hyA, hyB, hyN;
+    float sAB[4], phiA[4], Gprev[4];
+    f4v gjN = gjp[at(item.y0 - 1)];                    // j-face geometry, one row ahead
+    {
+      double qm[4];
+      const long long o0 = at(item.y0 - 2), o1 = at(item.y0 - 1), o2 = at(item.y0), o3 = at(item.y0 + 1);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        qm[v] = Pin[v * pl + o0]; qA[v] = Pin[v * pl + o1]; qB[v] = Pin[v * pl + o2]; qN[v] = Pin[v * pl + o3];
+      }
+      const float hym = hyp[o0];
+      hyA = hyp[o1]; hyB = hyp[o2]; hyN = hyp[o3];
+      const float i0 = frcp(hym + hyA), i1 = frcp(hyA + hyB);
+      float s0[4];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        fA[v] = (float)qA[v]; fB[v] = (float)qB[v];
+        s0[v] = (fA[v] - (float)qm[v]) * i0;
+        sAB[v] = (fB[v] - fA[v]) * i1;
+        Gprev[v] = 0.f;
+      }
+#pragma unroll
+      for (int v = 0; v < 4; ++v) phiA[v] = limiter_mx(lim, sAB[v], s0[v]);
+    }
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    double dtnext = 1e300;
+
+    for (int r = item.y0 - 1; r <= item.y1; ++r) {
+      // Loads of this iteration, in the order they are consumed (vector-memory waits are in issue
+      // order): first row r's own data (i-face geometry, RK base state, local dt, the fused dt's
+      // directions; first used after the j-face flux), then row r+3 and row r+1's j-face geometry for the
+      // next iteration.  Row r+2 and row r's j-face geometry and 1/sur arrived (issued one iteration ago).
+      double qC[4];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) qC[v] = qN[v];
+      const float hyC = hyN;
+      const f4v gj = gjN;
+      const long long oc = at(r);
+      f4v gi = {0.f, 0.f, 0.f, 0.f}, gd = {0.f, 0.f, 0.f, 0.f};
+      double po[4] = {0.0, 0.0, 0.0, 0.0};
+      double dtl = 0.0;
+      if (r >= item.y0) {
+        gi = gip[oc];
+        if (irk != 1) {
+#pragma unroll
+          for (int v = 0; v < 4; ++v) po[v] = PO[v * pl + oc];
+        }
+        if (own && !tacc) dtl = dtlp[oc];
+        if (FUSE && last) gd = gdp[oc];
+      }
+      {
+        const long long oN = at(min(r + 3, rmax));
+#pragma unroll
+        for (int v = 0; v < 4; ++v) qN[v] = Pin[v * pl + oN];
+        hyN = hyp[oN];
+        gjN = gjp[at(min(r + 1, rmax))];
+      }
+      float fC[4], sBC[4], phiB[4], Gcur[4];
+      {
+        const float iBC = frcp(hyB + hyC);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) { fC[v] = (float)qC[v]; sBC[v] = (fC[v] - fB[v]) * iBC; }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) phiB[v] = limiter_mx(lim, sBC[v], sAB[v]);
+      }
+      // interior j-face (r | r+1) — Mod_Fluxes.f90:187-232
+      if (own && r >= 1 && r <= nj - 1) {
+        float L[4], R[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) { L[v] = fA[v] + phiA[v] * hyA; R[v] = fB[v] - phiB[v] * hyB; }
+        if (!(L[0] > 0.f && L[3] > 0.f && R[0] > 0.f && R[3] > 0.f))
+          record_fail(&K, Q2D_E_INTERFACE, fail_key(stage_id, 0, B.blk_in_case, r, x));
+        flux_mx<RIE>(gf, L, R, gj.x, gj.y, gj.z, Gcur);
+      } else {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) Gcur[v] = 0.f;
+      }
+
+      if (r >= item.y0) {
+        // ---- i-direction on row r (DPP shifts between lanes) ------------------
+        const float hx = gi.w, hxn = shdn(hx);
+        float fn[4], sx[4], phx[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) fn[v] = shdn(fA[v]);
+        const float ix = frcp(hx + hxn);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) sx[v] = (fn[v] - fA[v]) * ix;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) phx[v] = limiter_mx(lim, sx[v], shup(sx[v]));
+        float phxn[4];                                 // lane shifts with every lane active
+#pragma unroll
+        for (int v = 0; v < 4; ++v) phxn[v] = shdn(phx[v]);
+        float Fx[4];
+        if (xdo) {                                     // interior i-face (x|x+1) — Mod_Fluxes.f90:140-185
+          float L[4], R[4];
+#pragma unroll
+          for (int v = 0; v < 4; ++v) { L[v] = fA[v] + phx[v] * hx; R[v] = fn[v] - phxn[v] * hxn; }
+          if (!(L[0] > 0.f && L[3] > 0.f && R[0] > 0.f && R[3] > 0.f))
+            record_fail(&K, Q2D_E_INTERFACE, fail_key(stage_id, 0, B.blk_in_case, r, x));
+          flux_mx<RIE>(gf, L, R, gi.x, gi.y, gi.z, Fx);
+        } else {
+#pragma unroll
+          for (int v = 0; v < 4; ++v) Fx[v] = 0.f;
+        }
+        float Fl[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) Fl[v] = shup(Fx[v]);
+
+        if (own) {
+          // ---- residual of the cell, faces in the reference's order -----------
+          double res[4];
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            double rv = 0.0;
+            if (x >= 2) rv = rv - (double)Fl[v];
+            if (x <= ni - 1) rv = rv + (double)Fx[v];
+            if (r >= 2) rv = rv - (double)Gprev[v];
+            if (r <= nj - 1) rv = rv + (double)Gcur[v];
+            res[v] = rv;
+          }
+          if (x == 1 || x == ni || r == 1 || r == nj) {  // BC faces 1..4 in record order (Mod_BC_Fluxes.f90:79-86)
+#pragma unroll 1
+            for (int f = 0; f < 4; ++f) {
+              int rec = -1;
+              if (f == 0 && x == 1 && B.bcf[0] >= 0) rec = bcface[B.bcf[0] + r - 1];
+              if (f == 1 && x == ni && B.bcf[1] >= 0) rec = bcface[B.bcf[1] + r - 1];
+              if (f == 2 && r == 1) rec = rec3;
+              if (f == 3 && r == nj) rec = rec4;
+              if (rec < 0) continue;
+              const BCFlux& q = bcflux[rec];
+#pragma unroll
+              for (int v = 0; v < 4; ++v)
+                if ((q.mask >> v) & 1) res[v] = res[v] + q.F[v];
+            }
+          }
+          // geometric sources (Lib_Source_Axisymmetry.f90:40-78, Lib_Source_Areavariation.f90:104-144),
+          // per unit area: Res/sur - S enters the update
+          double S[4] = {0.0, 0.0, 0.0, 0.0};
+          if (C.source != Q2D_SRC_NONE) {
+            const double rho = qA[0], u = qA[1], v = qA[2], p = qA[3];
+            const double rhoE0 = p * rgm1 + 0.5 * rho * (u * u + v * v);
+            if (C.source == Q2D_SRC_AXI) {
+              const double invr = gp(B.sb)[oc];
+              S[0] = -(rho * v * invr);
+              S[1] = -(rho * u * v * invr);
+              S[2] = -(rho * v * v * invr);
+              S[3] = -((rhoE0 + p) * v * invr);
+            } else {
+              const double VdG = u * gp(B.sa)[oc] + v * gp(B.sb)[oc];
+              S[0] = rho * VdG;
+              S[1] = rho * u * VdG;
+              S[2] = rho * v * VdG;
+              S[3] = (rhoE0 + p) * VdG;
+            }
+          }
+          // ---- FP64 update: RK_stage (Lib_RK.f90:15-27) on the conservative state ----
+          if (irk == 1) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) po[v] = qA[v];
+          }
+          const double dt = tacc ? dtg : dtl;
+          const double dts = dt * (double)gj.w;
+          double Uo[4], U[4], qn2[4];
+          Uo[0] = po[0]; Uo[1] = po[0] * po[1]; Uo[2] = po[0] * po[2];
+          Uo[3] = __builtin_fma(po[3], rgm1, 0.5 * po[0] * (po[1] * po[1] + po[2] * po[2]));
+          U[0] = qA[0]; U[1] = qA[0] * qA[1]; U[2] = qA[0] * qA[2];
+          U[3] = __builtin_fma(qA[3], rgm1, 0.5 * qA[0] * (qA[1] * qA[1] + qA[2] * qA[2]));
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const double inc = __builtin_fma(-res[v], dts, S[v] * dt);
+            U[v] = __builtin_fma(U[v] - Uo[v] + inc, c_rk, Uo[v]);
+          }
+          {
+            const double ir = rcp_nr(U[0]);
+            const double vx = U[1] * ir, vy = U[2] * ir;
+            qn2[0] = U[0]; qn2[1] = vx; qn2[2] = vy;
+            qn2[3] = gm1 * __builtin_fma(-0.5, U[1] * vx + U[2] * vy, U[3]);
+          }
+          // Check_And_Fix_State — Lib_Newstate.f90:316-350
+          if (__builtin_isnan(qn2[0]) || __builtin_isnan(qn2[1]) || __builtin_isnan(qn2[2]) ||
+              __builtin_isnan(qn2[3]) || qn2[3] < 0.0)
+            record_fail(&K, Q2D_E_STATE, fail_key(stage_id, 1, B.blk_in_case, r, x));
+          if (qn2[0] < 1e-20) qn2[0] = 1e-20;
+#pragma unroll
+          for (int v = 0; v < 4; ++v) Pout[v * pl + oc] = qn2[v];
+          if (last) {                                  // Compute_Residual — Mod_Diagnostic.f90:452-458
+#pragma unroll
+            for (int v = 0; v < 4; ++v) { const double d = qn2[v] - po[v]; acc[v] = __builtin_fma(d, d, acc[v]); }
+          }
+          if (last && FUSE) {
+            // next step's Compute_dt on the state just written (Mod_dt.f90:59-104), FP64
+            const double dtmin = cell_dt_mx(C, kiter + 1, qn2[0], qn2[1], qn2[2], qn2[3], hx, hyA, gd.x, gd.y,
+                                            gd.z, gd.w);
+            if (!tacc) dtl_next[oc] = dtmin;
+            if (dtmin < dtnext) dtnext = dtmin;
+          }
+        }
+      }
+      // ---- roll the j-stencil -------------------------------------------------
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        qA[v] = qB[v]; qB[v] = qC[v]; fA[v] = fB[v]; fB[v] = fC[v];
+        sAB[v] = sBC[v]; phiA[v] = phiB[v]; Gprev[v] = Gcur[v];
+      }
+      hyA = hyB; hyB = hyC;
+    }
+    if (last) {                                        // fixed-pattern wavefront sum (deterministic)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        double s = acc[v];
+        for (int k = 32; k >= 1; k >>= 1) s = s + __shfl_xor(s, k, kWave);
+        if (lane == 0) partials[(long long)it * kPartials + v] = s;
+      }
+    }
+    if (last && FUSE) {                                // next step's dt: the item's exact min (see k_stage)
+      for (int k = 32; k >= 1; k >>= 1) {
+        const double o = __shfl_xor(dtnext, k, kWave);
+        if (o < dtnext) dtnext = o;
+      }
+      if (lane == 0) partials[(long long)it * kPartials + 4] = dtnext;
     }
   }
   if (tslotp && lane == 0) atomicMax(&tslotp[1], (unsigned long long)wall_clock64());
@@ -1302,25 +1654,33 @@ __global__ void k_tstamp_init(unsigned long long* tstamp, int nsub) {
 
 // Per-case residual reduction and end-of-step control — Mod_Explicit.f90:120-168.
 __global__ __launch_bounds__(256) void k_finish(DevCtrl* ctrl, const DevCase* cases, const double* partials, int ncases,
-                                                unsigned long long* tstamp) {
+                                                unsigned long long* tstamp, int fused_dt) {
   const int c = blockIdx.x;
   if (tstamp && c == 0 && threadIdx.x == 0) tstamp[0] = tstamp[0] + 1ull;   // end of a timed step
   if (c >= ncases) return;
   DevCtrl& K = ctrl[c];
   const DevCase& C = cases[c];
   if (!K.active) return;
-  __shared__ double sh[4][256];
+  __shared__ double sh[5][256];
   double s[4] = {0, 0, 0, 0};
-  for (int it = C.item_begin + threadIdx.x; it < C.item_end; it += blockDim.x)
-    for (int v = 0; v < 4; ++v) s[v] = s[v] + partials[(long long)it * 4 + v];
+  double dmin = 1e300;
+  for (int it = C.item_begin + threadIdx.x; it < C.item_end; it += blockDim.x) {
+    for (int v = 0; v < 4; ++v) s[v] = s[v] + partials[(long long)it * kPartials + v];
+    if (fused_dt) dmin = fmin(dmin, partials[(long long)it * kPartials + 4]);
+  }
   for (int v = 0; v < 4; ++v) sh[v][threadIdx.x] = s[v];
+  sh[4][threadIdx.x] = dmin;
   __syncthreads();
   for (int w = blockDim.x / 2; w >= 1; w >>= 1) {
-    if ((int)threadIdx.x < w)
+    if ((int)threadIdx.x < w) {
       for (int v = 0; v < 4; ++v) sh[v][threadIdx.x] = sh[v][threadIdx.x] + sh[v][threadIdx.x + w];
+      sh[4][threadIdx.x] = fmin(sh[4][threadIdx.x], sh[4][threadIdx.x + w]);
+    }
     __syncthreads();
   }
   if (threadIdx.x == 0) {
+    // next step's dt from the fused last stage (Set_Global_dt's min, Mod_dt.f90:109-128)
+    if (fused_dt && sh[4][0] < 1e300) K.dtmin_next_bits = (unsigned long long)__double_as_longlong(sh[4][0]);
     for (int v = 0; v < 4; ++v) K.residual[v] = sqrt(sh[v][0]);
     if (K.iter == 1) K.res_first = K.residual[0];
     const bool endsim = (K.iter_from_call >= C.itermax) || (K.residual[0] <= C.res_threshold) ||
@@ -1411,10 +1771,10 @@ namespace {
 const void* stage_kernel(const q2d_handle* h, int pass) {
   if (h->precision_mixed) {
     switch (pass) {
-      case 0: return reinterpret_cast<const void*>(&k_stage<float, 0, kFuseDtMixed, false>);
-      case 1: return reinterpret_cast<const void*>(&k_stage<float, 1, kFuseDtMixed, false>);
-      case 2: return reinterpret_cast<const void*>(&k_stage<float, 0, kFuseDtMixed, true>);
-      default: return reinterpret_cast<const void*>(&k_stage<float, 1, kFuseDtMixed, true>);
+      case 0: return reinterpret_cast<const void*>(&k_stage_mx<0, kFuseDtMx>);
+      case 1: return reinterpret_cast<const void*>(&k_stage_mx<1, kFuseDtMx>);
+      case 2: return reinterpret_cast<const void*>(&k_stage<float, 0, false, true>);
+      default: return reinterpret_cast<const void*>(&k_stage<float, 1, false, true>);
     }
   }
   switch (pass) {
@@ -1533,7 +1893,7 @@ int build_items(q2d_handle* h) {
   if (h->d_items) { (void)hipFree(h->d_items); h->d_items = nullptr; }
   if (h->d_partials) { (void)hipFree(h->d_partials); h->d_partials = nullptr; }
   HIPCHK(hipMalloc(&h->d_items, sizeof(Item) * std::max<size_t>(1, h->items.size())));
-  HIPCHK(hipMalloc(&h->d_partials, sizeof(double) * 4 * std::max<size_t>(1, h->items.size())));
+  HIPCHK(hipMalloc(&h->d_partials, sizeof(double) * kPartials * std::max<size_t>(1, h->items.size())));
   HIPCHK(hipMemcpy(h->d_items, h->items.data(), sizeof(Item) * h->items.size(), hipMemcpyHostToDevice));
   (void)hipFree(h->d_cases); h->d_cases = nullptr;
   HIPCHK(hipMalloc(&h->d_cases, sizeof(DevCase) * std::max<size_t>(1, h->dc.size())));
@@ -1567,13 +1927,26 @@ int launch_stage(q2d_handle* h, int irk) {
     hipLaunchKernelGGL((k_stage<FT, RIE, FUSE, VISC>), dim3(groups), dim3(tpb), 0, h->stream, h->d_blocks,        \
                        h->d_cases, h->d_ctrl, h->d_items, lo, hi, bf, bq, h->d_partials, irk, h->xcd_remap, ts,  \
                        slot)
-    switch (pass) {
-      case 0: Q2D_STAGE_LAUNCH(0, false); break;
-      case 1: Q2D_STAGE_LAUNCH(1, false); break;
-      case 2: Q2D_STAGE_LAUNCH(0, true); break;
-      default: Q2D_STAGE_LAUNCH(1, true); break;
+#define Q2D_STAGE_MX_LAUNCH(RIE)                                                                                  \
+    hipLaunchKernelGGL((k_stage_mx<RIE, kFuseDtMx>), dim3(groups), dim3(tpb), 0, h->stream, h->d_blocks, h->d_cases, \
+                       h->d_ctrl, h->d_items, lo, hi, bf, bq, h->d_partials, irk, h->xcd_remap, ts, slot)
+    if constexpr (sizeof(FT) == 4) {                   // mixed form: Euler items run k_stage_mx
+      switch (pass) {
+        case 0: Q2D_STAGE_MX_LAUNCH(0); break;
+        case 1: Q2D_STAGE_MX_LAUNCH(1); break;
+        case 2: Q2D_STAGE_LAUNCH(0, true); break;
+        default: Q2D_STAGE_LAUNCH(1, true); break;
+      }
+    } else {
+      switch (pass) {
+        case 0: Q2D_STAGE_LAUNCH(0, false); break;
+        case 1: Q2D_STAGE_LAUNCH(1, false); break;
+        case 2: Q2D_STAGE_LAUNCH(0, true); break;
+        default: Q2D_STAGE_LAUNCH(1, true); break;
+      }
     }
 #undef Q2D_STAGE_LAUNCH
+#undef Q2D_STAGE_MX_LAUNCH
   }
   return Q2D_OK;
 }
@@ -1618,7 +1991,7 @@ int launch_step(q2d_handle* h) {
   const int cthreads = 64, cgroups = (h->ncases + cthreads - 1) / cthreads;
   const int bgroups = (h->nbc + 63) / 64;            // one wavefront per group: spread over every CU
   const int use_next =
-      (h->dt_fused_valid && !h->any_visc && (h->precision_mixed ? kFuseDtMixed : kFuseDtFp64)) ? 1 : 0;
+      (h->dt_fused_valid && !h->any_visc && (h->precision_mixed ? kFuseDtMx : kFuseDtFp64)) ? 1 : 0;
   hipLaunchKernelGGL(k_begin, dim3(cgroups), dim3(cthreads), 0, h->stream, h->d_ctrl, h->ncases, use_next);
   if (!use_next)
     if (h->dt_total > 0) {                             // one wavefront per unit of kDtCells cells
@@ -1645,11 +2018,12 @@ int launch_step(q2d_handle* h) {
       hipLaunchKernelGGL(k_visc, dim3(std::max(vg, 1)), dim3(tpb), 0, h->stream, h->d_blocks, h->d_cases, h->d_ctrl,
                          h->d_items, h->range[2], h->range[4], h->d_bc, h->d_bcface, irk);
     }
-    int rc = h->precision_mixed ? launch_stage<float, kFuseDtMixed>(h, irk) : launch_stage<double, kFuseDtFp64>(h, irk);
+    int rc = h->precision_mixed ? launch_stage<float, false>(h, irk) : launch_stage<double, kFuseDtFp64>(h, irk);
     if (rc != Q2D_OK) return rc;
   }
+  const int fused_dt = (!h->any_visc && (h->precision_mixed ? kFuseDtMx : kFuseDtFp64)) ? 1 : 0;
   hipLaunchKernelGGL(k_finish, dim3(h->ncases), dim3(256), 0, h->stream, h->d_ctrl, h->d_cases, h->d_partials, h->ncases,
-                     h->timing ? h->d_tstamp : nullptr);
+                     h->timing ? h->d_tstamp : nullptr, fused_dt);
   HIPCHK(hipGetLastError());
   h->dt_fused_valid = true;
   return Q2D_OK;
@@ -1931,7 +2305,8 @@ int append_cases(q2d_handle* h, const q2d_problem* prob, int32_t* first_case) {
   for (int lb = 0; lb < prob->nblocks; ++lb) {
     const q2d_block_desc& d = prob->blocks[lb];
     const bool visc = prob->cases[d.case_id].scheme.model > 0;
-    const int kPlanes = visc ? 39 : 30;
+    const bool mxgeo = prec == Q2D_PREC_MIXED && !visc;   // FP32 geometry planes of k_stage_mx
+    const int kPlanes = visc ? 39 : (mxgeo ? 37 : 30);
     h->hb.emplace_back();
     h->db.emplace_back();
     h->nblocks += 1;
@@ -1986,6 +2361,27 @@ int append_cases(q2d_handle* h, const q2d_problem* prob, int32_t* first_case) {
       for (int j = 0; j <= d.nj; ++j)
         for (int i = 0; i <= d.ni; ++i)
           plane(29)[(size_t)(j + 1) * pitch + (i + 1)] = d.node[((size_t)j * (d.ni + 1) + i) * 2 + 1];
+    if (mxgeo) {
+      // planes 30-31: GI, 32-33: GJ, 34-35: GD (float4 each), 36: HY (float), over the ghost extent
+      float* gi = reinterpret_cast<float*>(plane(30));
+      float* gj = reinterpret_cast<float*>(plane(32));
+      float* gd = reinterpret_cast<float*>(plane(34));
+      float* hy = reinterpret_cast<float*>(plane(36));
+      for (int j = 0; j < ngj; ++j)
+        for (int i = 0; i < ngi; ++i) {
+          const size_t c = (size_t)j * pitch + i;
+          gi[4 * c + 0] = (float)plane(19)[c]; gi[4 * c + 1] = (float)plane(20)[c];
+          gi[4 * c + 2] = (float)plane(21)[c]; gi[4 * c + 3] = (float)(0.5 * plane(12)[c]);
+          const double sr = plane(14)[c];
+          gj[4 * c + 0] = (float)plane(22)[c]; gj[4 * c + 1] = (float)plane(23)[c];
+          gj[4 * c + 2] = (float)plane(24)[c]; gj[4 * c + 3] = sr != 0.0 ? (float)(1.0 / sr) : 0.f;
+          const double c11 = plane(15)[c], c21 = plane(16)[c], c12 = plane(17)[c], c22 = plane(18)[c];
+          const double n1 = sqrt(c11 * c11 + c12 * c12), n2 = sqrt(c21 * c21 + c22 * c22);
+          gd[4 * c + 0] = n1 > 0.0 ? (float)(c11 / n1) : 0.f; gd[4 * c + 1] = n1 > 0.0 ? (float)(c12 / n1) : 0.f;
+          gd[4 * c + 2] = n2 > 0.0 ? (float)(c21 / n2) : 0.f; gd[4 * c + 3] = n2 > 0.0 ? (float)(c22 / n2) : 0.f;
+          hy[c] = (float)(0.5 * plane(13)[c]);
+        }
+    }
     HIPCHK(hipMemcpy(hb.arena, host.data(), hb.arena_bytes, hipMemcpyHostToDevice));
     DevBlock& db = h->db.back();
     memset(&db, 0, sizeof(db));
@@ -2000,6 +2396,10 @@ int append_cases(q2d_handle* h, const q2d_problem* prob, int32_t* first_case) {
     db.dtl[0] = a + 25 * pl; db.dtl[1] = a + 28 * pl; db.sa = a + 26 * pl; db.sb = a + 27 * pl;
     db.nodr = a + 29 * pl;
     if (visc) { db.vdx = a + 30 * pl; db.vey = a + 33 * pl; db.vsv = a + 36 * pl; }
+    if (mxgeo) {
+      db.mgi = reinterpret_cast<const float4*>(a + 30 * pl); db.mgj = reinterpret_cast<const float4*>(a + 32 * pl);
+      db.mgd = reinterpret_cast<const float4*>(a + 34 * pl); db.mhy = reinterpret_cast<const float*>(a + 36 * pl);
+    }
     for (int f = 0; f < 4; ++f) db.bcf[f] = -1;
   }
   h->dbc.insert(h->dbc.end(), newbc.begin(), newbc.end());
@@ -2045,6 +2445,9 @@ int q2d_gpu_add_cases(q2d_handle* h, const q2d_problem* prob, int32_t* first_cas
   const int nb0 = h->nblocks, nc0 = h->ncases;
   const size_t nbc0 = h->dbc.size();
   int rc = append_cases(h, prob, first_case);
+  // the next step runs the dt pass: the added cases have no dt from a previous last stage (the dt pass
+  // and the fused dt evaluate the same formula, so the cases already running are not affected)
+  h->dt_fused_valid = false;
   if (rc != Q2D_OK && (h->nblocks != nb0 || h->ncases != nc0)) {
     // roll back a partial append: free what was allocated and rebuild from the previous set
     std::string msg = g_err;
