@@ -92,3 +92,26 @@ def test_deterministic_residual(oracle_mod):
     r1 = GpuSolver(pb).run(10)[0]["residual"]
     r2 = GpuSolver(pb).run(10)[0]["residual"]
     assert r1 == r2
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_one_cell_wide_block(oracle_mod, precision):
+    """The reference's 1D Shu-Osher case (400 x 1 cells, BC 400 ends, BC 300 on the y faces): a block one
+    cell wide, where the y-face records read each other's ghost cells (two-phase ghost fill and BC flux).
+    FP64: bit-identical to the oracle; mixed: relative L1(rho) <= 1e-5."""
+    pb = cases.shu_osher(400, limiter="minmod")
+    for c in pb.cases:
+        c.precision = precision
+    g = GpuSolver(pb)
+    go = g.run(300)
+    pb64 = cases.shu_osher(400, limiter="minmod")
+    o = oracle_mod.Oracle(pb64)
+    rc, oo = o.run(300, stop_on_todo=False)
+    assert rc == 0 and go[0]["iter"] == oo[0]["iter"] == 300
+    Pg, Po = g.state(0)[2:-2, 2:-2], o.state(0)[2:-2, 2:-2]
+    if precision == "fp64":
+        assert np.array_equal(Pg, Po), np.abs(Pg - Po).max()
+        assert go[0]["time"] == oo[0]["time"]
+    else:
+        err = np.abs(Pg[..., 0] - Po[..., 0]).sum() / np.abs(Po[..., 0]).sum()
+        assert 0.0 < err <= 1e-5, err
