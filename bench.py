@@ -177,7 +177,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=1024)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--form", choices=["both", "fp64", "mixed"], default="both",
                     help="profiling aid: run only one arithmetic form (no JSON line)")
@@ -238,7 +238,7 @@ def main():
         ach = alg / (r["stage_kernel_ms"] * 1e-3) / 1e9
         return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic.get(form),
-                "kernel": "k_stage", "kernel_ms": round(r["stage_kernel_ms"], 4),
+                "kernel": "k_stage_mx" if form == "mixed" else "k_stage", "kernel_ms": round(r["stage_kernel_ms"], 4),
                 "alg_bytes_per_launch": alg}
 
     line = {
