@@ -234,7 +234,8 @@ int q2d_gpu_timing_stats(q2d_handle *h, int64_t *launches, double *ms_total);
  * rows_per_item -1 picks, for every stage launch, the row count whose work
  * items finish soonest on the wavefront slots the device holds at the
  * kernel's occupancy.  A new handle uses 4 waves and its form's default rows
- * (get_tiling reports 0): balanced (-1) for FP64, 16 for the mixed form.
+ * (get_tiling reports 0): balanced for FP64, the smaller of balanced and 16 for
+ * the mixed form.
  * Used by the tile autotuner. */
 int q2d_gpu_set_tiling(q2d_handle *h, int32_t rows_per_item, int32_t waves_per_group);
 int q2d_gpu_get_tiling(q2d_handle *h, int32_t *rows_per_item, int32_t *waves_per_group);

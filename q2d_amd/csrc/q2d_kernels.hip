@@ -1882,8 +1882,10 @@ int build_items(q2d_handle* h) {
     h->range[pass] = (int)h->items.size();
     // Form defaults, measured on 4 x 1024^2: the FP64 stage kernel (2 waves per SIMD, issue-bound)
     // gains 3 % from balanced items; the mixed one (3 waves per SIMD, bandwidth-bound) is 4 % faster
-    // with 16 rows than balanced.
-    int H = h->rows_per_item != 0 ? h->rows_per_item : (h->precision_mixed ? 16 : -1);
+    // with 16 rows than balanced (more, shorter items in flight), unless balanced items are shorter
+    // still (a problem too small to fill the device with 16-row items).
+    const bool form_default = h->rows_per_item == 0;
+    int H = form_default ? -1 : h->rows_per_item;
     if (H <= 0) {                                  // balanced for this launch on this device
       std::vector<std::pair<int, int>> strips;
       for (int c = 0; c < h->ncases; ++c) {
@@ -1900,6 +1902,7 @@ int build_items(q2d_handle* h) {
           per_cu = 2 * 4 / h->waves_per_group;    // 2 waves per SIMD
         H = balanced_rows(strips, (long long)per_cu * h->waves_per_group * h->cu_count);
       }
+      if (form_default && h->precision_mixed) H = std::min(H, 16);
     }
     h->rows_used[pass] = H;
     for (int c = 0; c < h->ncases; ++c) {
