@@ -574,9 +574,10 @@ static int convective_flux(int lim, int rid, const gas_t *g, const double *dl, d
 /* ------------------------------------------------------------------------ */
 static const double negativetwothird = -2.0 / 3.0;   /* Lib_Fluid.f90:21 */
 
-/* Stress_Vector_Std (:73-99, V_dot_G = 0) / Stress_Vector_Axi (:102-125). gv[a][b] = Gradvel(a+1,b+1). */
+/* Stress_Vector_Std (:26-48; VG = V_dot_G, 0 without area variation) / Stress_Vector_Axi (:51-76, VG not
+ * used). gv[a][b] = Gradvel(a+1,b+1). */
 static void stress_vector(int axi, const double gv[2][2], double n1, double n2, double mil, double r_ave, double vr,
-                          double *S1, double *S2) {
+                          double VG, double *S1, double *S2) {
   double mi = mil + 0.0;
   double lam = negativetwothird * mi;
   double Divervel;
@@ -584,7 +585,7 @@ static void stress_vector(int axi, const double gv[2][2], double n1, double n2, 
     if (r_ave <= 1.0e-6) Divervel = gv[0][0] + gv[1][1] + gv[1][1];
     else Divervel = gv[0][0] + gv[1][1] + vr / r_ave;
   } else {
-    Divervel = gv[0][0] + gv[1][1] - 0.0;
+    Divervel = gv[0][0] + gv[1][1] - VG;
   }
   double lamDivervel = lam * Divervel;
   double Tau_xx = lamDivervel + 2.0 * mi * gv[0][0];
@@ -592,6 +593,19 @@ static void stress_vector(int axi, const double gv[2][2], double n1, double n2, 
   double Tau_xy = mi * (gv[0][1] + gv[1][0]);
   *S1 = Tau_xx * n1 + Tau_xy * n2;
   *S2 = Tau_xy * n1 + Tau_yy * n2;
+}
+
+/* V_dot_G_face_avg — Lib_Fluid.f90:550-564: <V>_face . <G>_face */
+static double v_dot_g_face(double u1, double v1, double u2, double v2, double gx1, double gy1, double gx2, double gy2) {
+  double Va0 = 0.5 * (u1 + u2), Va1 = 0.5 * (v1 + v2);
+  double Ga0 = 0.5 * (gx1 + gx2), Ga1 = 0.5 * (gy1 + gy2);
+  return Va0 * Ga0 + Va1 * Ga1;
+}
+
+/* Area-variation geometric term G of cell (i,j) (area_geom_term_x/y); 0 without area variation. */
+static void area_g(const oblk *b, const ocase *cs, int i, int j, double *gx, double *gy) {
+  if (cs->d.scheme.source == Q2D_SRC_AREA && b->sa) { *gx = b->sa[IN(b, i, j)]; *gy = b->sb[IN(b, i, j)]; }
+  else { *gx = 0.0; *gy = 0.0; }
 }
 
 /* Gradient(k,:) <- Gradient(k,:) . M   (matmul(Gradient, M), M(l,c) = MM(l,c)) */
@@ -605,7 +619,8 @@ static void grad_times_M(double g[2], const double M[4]) {
  * P1,P2: the two cells of the face; P3..P6 the tangential stencil; a = direction of the face normal (0/1). */
 static void diffusive_flux(const ocase *cs, double n1, double n2, double int_l, const double *P1, const double *P2,
                            const double *P3, const double *P4, const double *P5, const double *P6,
-                           const double *M1, const double *M2, double r2, double r1, int a, double *Flux) {
+                           const double *M1, const double *M2, double r2, double r1, int a, const double G1[2],
+                           const double G2[2], double *Flux) {
   const gas_t *g = &cs->gas;
   int axi = cs->d.scheme.source == Q2D_SRC_AXI;
   int b = 1 - a;
@@ -636,8 +651,10 @@ static void diffusive_flux(const ocase *cs, double n1, double n2, double int_l, 
   double mil = mu_lam(g, T), kl = k_lam(g, mil);
   double kappa = kl + 0.0;
   double gv[2][2] = {{Gr[1][0], Gr[1][1]}, {Gr[2][0], Gr[2][1]}};
+  /* V.G at the face (Definition A; :61-68) */
+  double VG = v_dot_g_face(P1[1], P1[2], P2[1], P2[2], G1[0], G1[1], G2[0], G2[1]);
   double S1, S2;
-  stress_vector(axi, gv, n1, n2, mil, r_ave, Pm[2], &S1, &S2);
+  stress_vector(axi, gv, n1, n2, mil, r_ave, Pm[2], VG, &S1, &S2);
   /* one species: Dm*grad(Y) = 0, so the species flux and the enthalpy diffusion terms vanish */
   Flux[0] = 0.0;
   Flux[1] = int_l * S1;
@@ -668,9 +685,12 @@ static int fluxes_block(oblk *b, const ocase *cs) {
       for (int v = 0; v < NP; v++) { RV(b, v, i, j) = RV(b, v, i, j) + F[v]; RV(b, v, i + 1, j) = RV(b, v, i + 1, j) - F[v]; }
       if (s->model > 0) {
         double D[NP];
+        double G1[2], G2[2];
+        area_g(b, cs, i, j, &G1[0], &G1[1]);
+        area_g(b, cs, i + 1, j, &G2[0], &G2[1]);
         diffusive_flux(cs, FI(b, 0, i, j), FI(b, 1, i, j), FI(b, 2, i, j), &PV(b, 0, i, j), &PV(b, 0, i + 1, j),
                        &PV(b, 0, i, j - 1), &PV(b, 0, i, j + 1), &PV(b, 0, i + 1, j - 1), &PV(b, 0, i + 1, j + 1),
-                       &MM(b, 1, 1, i, j), &MM(b, 1, 1, i + 1, j), NODE(b, 1, i, j), NODE(b, 1, i, j - 1), 0, D);
+                       &MM(b, 1, 1, i, j), &MM(b, 1, 1, i + 1, j), NODE(b, 1, i, j), NODE(b, 1, i, j - 1), 0, G1, G2, D);
         for (int v = 0; v < NP; v++) { RV(b, v, i, j) = RV(b, v, i, j) - D[v]; RV(b, v, i + 1, j) = RV(b, v, i + 1, j) + D[v]; }
       }
     }
@@ -691,9 +711,12 @@ static int fluxes_block(oblk *b, const ocase *cs) {
       for (int v = 0; v < NP; v++) { RV(b, v, i, j) = RV(b, v, i, j) + F[v]; RV(b, v, i, j + 1) = RV(b, v, i, j + 1) - F[v]; }
       if (s->model > 0) {
         double D[NP];
+        double G1[2], G2[2];
+        area_g(b, cs, i, j, &G1[0], &G1[1]);
+        area_g(b, cs, i, j + 1, &G2[0], &G2[1]);
         diffusive_flux(cs, FJ(b, 0, i, j), FJ(b, 1, i, j), FJ(b, 2, i, j), &PV(b, 0, i, j), &PV(b, 0, i, j + 1),
                        &PV(b, 0, i - 1, j), &PV(b, 0, i + 1, j), &PV(b, 0, i - 1, j + 1), &PV(b, 0, i + 1, j + 1),
-                       &MM(b, 1, 1, i, j), &MM(b, 1, 1, i, j + 1), NODE(b, 1, i, j), NODE(b, 1, i - 1, j), 1, D);
+                       &MM(b, 1, 1, i, j), &MM(b, 1, 1, i, j + 1), NODE(b, 1, i, j), NODE(b, 1, i - 1, j), 1, G1, G2, D);
         for (int v = 0; v < NP; v++) { RV(b, v, i, j) = RV(b, v, i, j) - D[v]; RV(b, v, i, j + 1) = RV(b, v, i, j + 1) + D[v]; }
       }
     }
@@ -967,9 +990,12 @@ static void bc_symmetry_visc(oblk *b, const ocase *cs, int Im, int Jm, int Fm) {
   grad_times_M(Gr[1], M);
   double r_ave = face_r_ave(b, Im, Jm, Fm);
   double Vm[2] = {0.5 * (V1[0] + V2[0]), 0.5 * (V1[1] + V2[1])};
+  /* V.G at the symmetry face: <V>.G_loc (:196-200) */
+  double Gl[2];
+  area_g(b, cs, Im, Jm, &Gl[0], &Gl[1]);
+  double VG = v_dot_g_face(V1[0], V1[1], V2[0], V2[1], Gl[0], Gl[1], Gl[0], Gl[1]);
   double S1, S2;
-  (void)axi;
-  stress_vector(axi, (const double(*)[2])Gr, n1, n2, mil, r_ave, 0.5 * (PV(b, 2, Im, Jm) + V2[1]), &S1, &S2);
+  stress_vector(axi, (const double(*)[2])Gr, n1, n2, mil, r_ave, 0.5 * (PV(b, 2, Im, Jm) + V2[1]), VG, &S1, &S2);
   double F1 = int_l * S1, F2 = int_l * S2;
   double F3 = int_l * (Vm[0] * S1 + Vm[1] * S2);
   RV(b, 1, Im, Jm) = RV(b, 1, Im, Jm) - (double)modfm2 * F1;
@@ -997,7 +1023,7 @@ static void bc_wall_temperature(oblk *b, const ocase *cs, int Im, int Jm, int Fm
   double r_ave = face_r_ave(b, Im, Jm, Fm);
   double gv[2][2] = {{Gr[1][0], Gr[1][1]}, {Gr[2][0], Gr[2][1]}};
   double S1, S2;
-  stress_vector(axi, gv, n1, n2, mil, r_ave, 0.0, &S1, &S2);
+  stress_vector(axi, gv, n1, n2, mil, r_ave, 0.0, 0.0, &S1, &S2);   /* the wall flux passes no V_dot_G */
   double Flux[4];
   Flux[0] = 0.0;
   Flux[1] = S1 * int_l;
@@ -1098,6 +1124,60 @@ static void source_axi_visc(oblk *b, const ocase *cs) {
   }
 }
 
+/* Source_Area_Visc_Blk — Lib_Source_Areavariation.f90:147-349 (laminar, one species: the species
+ * gradient (rho/rho) is 0 exactly, so the species, enthalpy-diffusion and Hirschfelder terms vanish) */
+static void source_area_visc(oblk *b, const ocase *cs) {
+  const gas_t *g = &cs->gas;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+  for (int j = 1; j <= b->nj; j++) {
+    for (int i = 1; i <= b->ni; i++) {
+      double Gx = b->sa[IN(b, i, j)], Gy = b->sb[IN(b, i, j)];
+      double rho = PV(b, 0, i, j);
+      double rho1 = PV(b, 0, i + 1, j), rho2 = PV(b, 0, i - 1, j), rho3 = PV(b, 0, i, j + 1), rho4 = PV(b, 0, i, j - 1);
+      double u = PV(b, 1, i, j), v = PV(b, 2, i, j);
+      double V_dot_G = u * Gx + v * Gy;
+      double T = PV(b, 3, i, j) / (rho * g->R);
+      double T1 = PV(b, 3, i + 1, j) / (rho1 * g->R);
+      double T2 = PV(b, 3, i - 1, j) / (rho2 * g->R);
+      double T3 = PV(b, 3, i, j + 1) / (rho3 * g->R);
+      double T4 = PV(b, 3, i, j - 1) / (rho4 * g->R);
+      double mil = mu_lam(g, T), kl = k_lam(g, mil);
+      double Gr[4][2];
+      Gr[0][0] = (PV(b, 0, i + 1, j) / rho1 - PV(b, 0, i - 1, j) / rho2) * 0.5;
+      Gr[0][1] = (PV(b, 0, i, j + 1) / rho3 - PV(b, 0, i, j - 1) / rho4) * 0.5;
+      Gr[1][0] = (PV(b, 1, i + 1, j) - PV(b, 1, i - 1, j)) * 0.5;
+      Gr[2][0] = (PV(b, 2, i + 1, j) - PV(b, 2, i - 1, j)) * 0.5;
+      Gr[1][1] = (PV(b, 1, i, j + 1) - PV(b, 1, i, j - 1)) * 0.5;
+      Gr[2][1] = (PV(b, 2, i, j + 1) - PV(b, 2, i, j - 1)) * 0.5;
+      Gr[3][0] = (T1 - T2) * 0.5;
+      Gr[3][1] = (T3 - T4) * 0.5;
+      const double *M = &MM(b, 1, 1, i, j);
+      for (int k = 0; k < 4; k++) grad_times_M(Gr[k], M);
+      double kappa = kl + 0.0;
+      /* Stress_Tensor_Std with V_dot_G — Lib_Fluid.f90:128-163 */
+      double mi = mil + 0.0;
+      double lam = negativetwothird * mi;
+      double Divervel = Gr[1][0] + Gr[2][1] - V_dot_G;
+      double lamDivervel = lam * Divervel;
+      double T11 = lamDivervel + 2.0 * mi * Gr[1][0];
+      double T22 = lamDivervel + 2.0 * mi * Gr[2][1];
+      double T12 = mi * (Gr[1][1] + Gr[2][0]);
+      double T33 = lamDivervel + 2.0 * mi * (-V_dot_G);
+      double qx = -kappa * Gr[3][0];
+      double qy = -kappa * Gr[3][1];
+      double S[4];
+      S[0] = 0.0;
+      S[1] = Gx * (T33 - T11) - Gy * T12;
+      S[2] = -Gx * T12 + Gy * (T33 - T22);
+      S[3] = -u * (T11 * Gx + T12 * Gy) - v * (T12 * Gx + T22 * Gy) + Gx * qx + Gy * qy;
+      double sr = SUR(b, i, j);
+      for (int k = 0; k < NP; k++) RV(b, k, i, j) = RV(b, k, i, j) - S[k] * sr;
+    }
+  }
+}
+
 static void sources(q2do *o, ocase *cs) {
   const q2d_scheme *s = &cs->d.scheme;
   if (s->source == Q2D_SRC_NONE) return;
@@ -1131,6 +1211,7 @@ static void sources(q2do *o, ocase *cs) {
       }
     }
     if (s->source == Q2D_SRC_AXI && s->model > 0) source_axi_visc(b, cs);
+    if (s->source == Q2D_SRC_AREA && s->model > 0 && b->sa) source_area_visc(b, cs);   /* Lib_Sources.f90:98-115 */
   }
 }
 

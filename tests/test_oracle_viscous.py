@@ -19,6 +19,15 @@ Lib_BC_Fluxes_Wall_Temperature.f90:20-106) and the VNN time-step limit
   scheme's update invariant; any transposed metric, swapped stencil index or
   misplaced normal breaks it.
 * Halving vnn halves the step wherever the viscous limit is the smaller one.
+* Area variation with viscosity (Definition A: Stress_Vector/Stress_Tensor with V.G,
+  Source_Area_Visc_Blk): a uniform stream (U, V) at uniform T through a Cartesian mesh
+  with a quadratic geometric term G = (c x^2 + g0, e y^2 + h0) has no velocity or
+  temperature gradient, so the in-plane stress is (2/3) mu V.G I and tau_zz = -(4/3) mu V.G
+  exactly.  The viscous part of one Euler step (laminar minus inviscid run, same dt) is then,
+  per cell, (2/3) mu grad(V.G_face) from the face fluxes (the central difference of a
+  quadratic is exact, and only the two-sided face average of G gives it) plus the source
+  -2 mu V.G G (momentum) and -(2/3) mu (V.G)^2 (energy).  With symmetry planes on the
+  y-faces the wall rows must take the same x-momentum and no y-momentum.
 """
 import math
 
@@ -159,3 +168,89 @@ def test_vnn_limit(oracle_mod):
     mixed = ~(visc | conv)                # viscous only at the smaller vnn
     assert visc.any() and conv.any()
     assert np.all((d2[mixed] > 0.5 * d1[mixed]) & (d2[mixed] < d1[mixed]))
+
+
+def _area_uniform_step(oracle_mod, model, U, V, c, g0, e, h0, bc_y):
+    ni, nj, mu = 10, 9, 0.03
+    node = cases.cartesian_nodes(ni, nj, 0.0, 1.0, 0.0, 0.9)
+    pb, geo = _case(node, {1: 400, 2: 400, 3: bc_y, 4: bc_y}, None,
+                    n_rk=1, limiter="iord", riemann="HLLC", time_accurate=True, model=model,
+                    source="area_variation", cfl=0.3, vnn=1e6, mu_ref=mu, T_ref=1.0, S_suth=0.38, Pr=0.72,
+                    itermax=1)
+    b = pb.blocks[0]
+    xc = _centres(node)
+    b.src_a = np.ascontiguousarray(c * xc[..., 0] ** 2 + g0)
+    b.src_b = np.ascontiguousarray(e * xc[..., 1] ** 2 + h0)
+    b.P[..., 1] = U
+    b.P[..., 2] = V
+    o = oracle_mod.Oracle(pb)
+    rc, out = o.step()
+    assert rc == 0
+    return o.state(0)[2:-2, 2:-2], out[0]["dtglobal"], xc, mu
+
+
+@pytest.mark.parametrize("U,V,c,g0,e,h0,bc_y", [(0.7, 0.0, 0.8, 0.1, 0.0, 0.0, 400),
+                                               (0.0, 0.6, 0.0, 0.0, -0.5, 0.2, 400),
+                                               (0.5, -0.3, 0.4, -0.2, 0.7, 0.1, 400),
+                                               (0.6, 0.0, -0.9, 0.3, 0.0, 0.4, 300)])
+def test_area_variation_viscous_uniform_stream(oracle_mod, U, V, c, g0, e, h0, bc_y):
+    P1, dt1, xc, mu = _area_uniform_step(oracle_mod, 1, U, V, c, g0, e, h0, bc_y)
+    P0, dt0, _, _ = _area_uniform_step(oracle_mod, 0, U, V, c, g0, e, h0, bc_y)
+    assert dt1 == dt0                                   # the CFL limit binds in both runs
+    cons = lambda P: np.stack([P[..., 0], P[..., 0] * P[..., 1], P[..., 0] * P[..., 2],
+                               P[..., 3] / (GAM - 1.0) + 0.5 * P[..., 0] * (P[..., 1] ** 2 + P[..., 2] ** 2)], -1)
+    d = cons(P1) - cons(P0)                             # the viscous part of the step
+    x, y = xc[..., 0], xc[..., 1]
+    Gx, Gy = c * x * x + g0, e * y * y + h0
+    VG = U * Gx + V * Gy
+    dVGx, dVGy = U * 2.0 * c * x, V * 2.0 * e * y       # d(V.G)/dx, d(V.G)/dy of the face values
+    want_u = dt1 * mu * ((2.0 / 3.0) * dVGx - 2.0 * VG * Gx)
+    want_v = dt1 * mu * ((2.0 / 3.0) * dVGy - 2.0 * VG * Gy)
+    want_E = dt1 * mu * ((2.0 / 3.0) * (U * dVGx + V * dVGy) - (2.0 / 3.0) * VG * VG)
+    # cells with all four faces interior; with symmetry planes on the y-faces, every row
+    inner = (slice(0, None) if bc_y == 300 else slice(1, -1), slice(1, -1))
+    scale = dt1 * mu
+    assert np.abs(d[inner][..., 0]).max() <= 1e-12
+    assert np.abs(d[inner][..., 1] - want_u[inner]).max() <= 1e-10 * scale
+    assert np.abs(d[inner][..., 2] - want_v[inner]).max() <= 1e-10 * scale
+    assert np.abs(d[inner][..., 3] - want_E[inner]).max() <= 1e-10 * scale
+
+
+def test_area_variation_viscous_heat_flux(oracle_mod):
+    """Fluid at rest with a linear temperature field T = (p0 + a x + b y) / R (rho = 1): V.G = 0, so the
+    geometric term enters only through the source's heat-flux term G.q = -kappa (Gx dT/dx + Gy dT/dy),
+    kappa = mu(T) cp / Pr (Sutherland).  The energy change of one step with G minus without G is
+    dt G.q in every interior cell (the central differences of a linear field are exact)."""
+    ni, nj, mu_ref, Tr, S, Pr, R, cp = 10, 9, 0.03, 1.0, 0.38, 0.72, 1.0, 3.5
+    p0, a, b = 1.0, 0.3, -0.2
+
+    def run(with_g):
+        node = cases.cartesian_nodes(ni, nj, 0.0, 1.0, 0.0, 0.9)
+        pb, geo = _case(node, {1: 400, 2: 400, 3: 400, 4: 400}, None,
+                        n_rk=1, limiter="iord", riemann="HLLC", time_accurate=True, model=1,
+                        source="area_variation", cfl=0.3, vnn=1e6, mu_ref=mu_ref, T_ref=Tr, S_suth=S, Pr=Pr,
+                        gas_R=R, gas_cp=cp, itermax=1)
+        blk = pb.blocks[0]
+        xc = _centres(node)
+        xg = np.arange(-1.5, ni + 2.0) / ni                      # centres of every column, ghosts included
+        yg = np.arange(-1.5, nj + 2.0) * 0.9 / nj
+        blk.P[..., 3] = p0 + a * xg[None, :] + b * yg[:, None]
+        k = 1.0 if with_g else 0.0
+        blk.src_a = np.ascontiguousarray(k * (0.6 * xc[..., 0] - 0.2))
+        blk.src_b = np.ascontiguousarray(k * (0.4 * xc[..., 1] ** 2 + 0.3))
+        o = oracle_mod.Oracle(pb)
+        rc, out = o.step()
+        assert rc == 0
+        P = o.state(0)[2:-2, 2:-2]
+        return P[..., 3] / 0.4 + 0.5 * P[..., 0] * (P[..., 1] ** 2 + P[..., 2] ** 2), out[0]["dtglobal"], xc
+    E1, dt1, xc = run(True)
+    E0, dt0, _ = run(False)
+    assert dt1 == dt0
+    x, y = xc[..., 0], xc[..., 1]
+    T = (p0 + a * x + b * y) / R
+    mu = mu_ref * (T / Tr) ** 1.5 * (Tr + S) / (T + S)
+    kappa = mu * cp / Pr
+    Gx, Gy = 0.6 * x - 0.2, 0.4 * y * y + 0.3
+    want = dt1 * (-kappa * (Gx * a / R + Gy * b / R))
+    inner = (slice(1, -1), slice(1, -1))
+    assert np.abs((E1 - E0)[inner] - want[inner]).max() <= 1e-9 * np.abs(want[inner]).max()
