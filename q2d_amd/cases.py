@@ -12,6 +12,7 @@ Generators (all seeded, numpy Generator):
   batch_uneven       many independent planar / quasi-2D cases of uneven size
   ramp_channel       supersonic compression ramp (steady, local dt)
   axisymmetric_pipe  laminar axisymmetric Navier-Stokes, supersonic pipe inflow, isothermal wall
+  laminar_area_channel  quasi-2D laminar Navier-Stokes with area variation (Definition A)
 """
 from __future__ import annotations
 
@@ -430,4 +431,39 @@ def axisymmetric_pipe(seed=0, ni=2048, nj=512, first_cell=1e-5, reynolds=1e5, ma
                  meta={"name": "axisymmetric_pipe", "mach": mach, "reynolds": reynolds, "T_wall": T_wall})
     finish(pb, [geo])
     blk.src_a, blk.src_b = axisymmetric_geometry(geo)
+    return pb
+
+
+def laminar_area_channel(seed=0, ni=128, nj=40, area_a=0.5, first_cell=2e-4, reynolds=2e4, mach=None, height=0.25,
+                         T_wall=1.0, precision="fp64", n_rk=3, limiter="vanleer", riemann="HLLC", cfl=0.5, vnn=0.2,
+                         steps=200):
+    """Quasi-2D laminar Navier-Stokes with area variation (Definition A): the planar half-channel of
+    axisymmetric_pipe (supersonic inflow BC 405, outflow BC 400, symmetry plane BC 300 on y = 0, isothermal
+    no-slip wall BC 302 on y = height, tanh clustering at the wall) with the nodal cross-section
+    A(x) = 1 + a sin^2(pi x) of the reference's quasi-2D cases (INPUT/block<N>_area.dat layout,
+    Lib_Source_Areavariation.f90:53-101).  Local dt, SSP-RK3, VNN 0.2 as in axisymmetric_pipe."""
+    rng = np.random.default_rng(seed)
+    if mach is None:
+        mach = float(rng.uniform(1.5, 3.0))
+    U = mach * math.sqrt(1.4)
+    xs = np.arange(ni + 1) / ni
+    ys = height * _tanh_wall_cluster(nj, first_cell / height)
+    node = np.empty((nj + 1, ni + 1, 2))
+    node[..., 0] = xs[None, :]
+    node[..., 1] = ys[:, None]
+    geo = BlockGeometry(node)
+    P = np.zeros((nj + 4, ni + 4, 4))
+    P[2:-2, 2:-2] = [1.0, U, 0.0, 1.0]
+    fill_initial_ghosts(P)
+    props = {1: {"mach": mach, "T": 1.0, "p": 1.0, "rel_fac": 1.0, "alpha": 0.0}, 4: {"T": T_wall}}
+    recs = face_records(1, ni, nj, {1: 405, 2: 400, 3: 300, 4: 302}, props=props)
+    spec = CaseSpec(n_rk=n_rk, limiter=limiter, riemann=riemann, time_accurate=False, model=1,
+                    source="area_variation", precision=precision, cfl=cfl, vnn=vnn, mu_ref=U / reynolds, T_ref=1.0,
+                    S_suth=110.4 / 273.15, Pr=0.72, itermax=steps)
+    blk = Block(geo, P)
+    pb = Problem(blocks=[blk], bc=recs, cases=[spec],
+                 meta={"name": "laminar_area_channel", "mach": mach, "reynolds": reynolds, "area_a": area_a})
+    finish(pb, [geo])
+    A = 1.0 + area_a * np.sin(math.pi * node[..., 0]) ** 2
+    blk.src_a, blk.src_b = area_geometry(geo, A)
     return pb

@@ -668,21 +668,31 @@ __global__ __launch_bounds__(256) void k_bcflux(const DevBlock* __restrict__ blo
 // ---------------------------------------------------------------------------
 constexpr double kNegTwoThird = -2.0 / 3.0;
 
-// Stress_Vector_Std / Stress_Vector_Axi — Lib_Fluid.f90:73-125, laminar (mi = mi_l + 0).  gUX = Gradient(nu,X),
-// gVX = Gradient(nv,X) after the metric transform; the Std form gets V.G = 0 (no area variation).
+// Stress_Vector_Std / Stress_Vector_Axi — Lib_Fluid.f90:26-76, laminar (mi = mi_l + 0).  gUX = Gradient(nu,X),
+// gVX = Gradient(nv,X) after the metric transform; VG: V.G of the face (Definition A; 0 without area
+// variation), which the Axi form does not use.
 __device__ __forceinline__ void stress_vector(bool axi, double gUx, double gUy, double gVx, double gVy, double n1,
-                                              double n2, double mil, double r_ave, double vr, double& S1, double& S2) {
+                                              double n2, double mil, double r_ave, double vr, double VG, double& S1,
+                                              double& S2) {
   const double mi = mil + 0.0;
   const double lam = kNegTwoThird * mi;
   double div;
   if (axi) div = (r_ave <= 1.0e-6) ? (gUx + gVy + gVy) : (gUx + gVy + vr / r_ave);
-  else div = gUx + gVy - 0.0;
+  else div = gUx + gVy - VG;
   const double ld = lam * div;
   const double txx = ld + 2.0 * mi * gUx;
   const double tyy = ld + 2.0 * mi * gVy;
   const double txy = mi * (gUy + gVx);
   S1 = txx * n1 + txy * n2;
   S2 = txy * n1 + tyy * n2;
+}
+
+// V_dot_G_face_avg — Lib_Fluid.f90:550-564: <V>_face . <G>_face
+__device__ __forceinline__ double v_dot_g_face(double u1, double v1, double u2, double v2, double gx1, double gy1,
+                                               double gx2, double gy2) {
+  const double Va0 = 0.5 * (u1 + u2), Va1 = 0.5 * (v1 + v2);
+  const double Ga0 = 0.5 * (gx1 + gx2), Ga1 = 0.5 * (gy1 + gy2);
+  return Va0 * Ga0 + Va1 * Ga1;
 }
 
 // one row of matmul(Gradient, M): (g0, g1) . M, M(l,c) column-major = (m11, m21, m12, m22)
@@ -703,7 +713,7 @@ __device__ __forceinline__ void vc_temperature(VC& c, double R) { c.T = c.p / (c
 // enthalpy-diffusion term vanish (grad Y = 0), so D = (Flux(nu), Flux(nv), Flux(np)).
 __device__ __forceinline__ void visc_face(const DevCase& C, bool axi, int a, double n1, double n2, double il,
                                           const VC& c1, const VC& c2, const VC& c3, const VC& c4, const VC& c5,
-                                          const VC& c6, const double M[4], double r_ave, double D[3]) {
+                                          const VC& c6, const double M[4], double r_ave, double VG, double D[3]) {
   const double nU = c2.u - c1.u, nV = c2.v - c1.v, nT = c2.T - c1.T;
   const double tU = (c4.u - c3.u + c6.u - c5.u) * 0.25;
   const double tV = (c4.v - c3.v + c6.v - c5.v) * 0.25;
@@ -720,7 +730,7 @@ __device__ __forceinline__ void visc_face(const DevCase& C, bool axi, int a, dou
   const double mil = mu_lam(C, T);
   const double kappa = k_lam(C, mil) + 0.0;
   double S1, S2;
-  stress_vector(axi, gUx, gUy, gVx, gVy, n1, n2, mil, r_ave, pm2, S1, S2);
+  stress_vector(axi, gUx, gUy, gVx, gVy, n1, n2, mil, r_ave, pm2, VG, S1, S2);
   D[0] = il * S1;
   D[1] = il * S2;
   D[2] = il * ((S1 * pm1 + S2 * pm2) + kappa * (gTx * n1 + gTy * n2) + 0.0);
@@ -751,7 +761,7 @@ __global__ __launch_bounds__(256) void k_visc(const DevBlock* __restrict__ block
     if (!K.active || irk > C.n_rk) continue;
     const int ni = B.ni, nj = B.nj, pitch = B.pitch;
     const long long pl = B.plane;
-    const bool axi = C.source == Q2D_SRC_AXI;
+    const bool axi = C.source == Q2D_SRC_AXI, area = C.source == Q2D_SRC_AREA;
     const double R = C.R;
     const int x = item.x0 - 2 + lane;                  // Fortran column of this lane
     const int xc = x <= ni + 2 ? x : ni + 2;           // clamped column for loads
@@ -769,7 +779,11 @@ __global__ __launch_bounds__(256) void k_visc(const DevBlock* __restrict__ block
       const int t = bcs[bcface[B.bcf[f] + pos - 1]].type;
       return t >= 301 && t <= 304;
     };
-    const bool wall3 = axi && own && is_wall(2, x), wall4 = axi && own && is_wall(3, x);
+    const bool src = axi || area;                      // a viscous source term (Lib_Sources.f90)
+    const bool wall3 = src && own && is_wall(2, x), wall4 = src && own && is_wall(3, x);
+    // area variation: G of rows r (gxC, gyC) and r+1 (gxN, gyN) for the V.G of the j-faces
+    GP gsa = gp(B.sa), gsb = gp(B.sb);
+    double gxN = area ? gsa[at(item.y0)] : 0.0, gyN = area ? gsb[at(item.y0)] : 0.0;
     // rows y0-1, y0, y0+1 resident; row y0+2 in flight
     VC cS = ldc(item.y0 - 1), cC = ldc(item.y0), cN = ldc(item.y0 + 1), cF = ldc(item.y0 + 2);
     vc_temperature(cS, R); vc_temperature(cC, R); vc_temperature(cN, R);
@@ -781,7 +795,9 @@ __global__ __launch_bounds__(256) void k_visc(const DevBlock* __restrict__ block
       // this row's face geometry and source terms (first used after the neighbour shifts)
       const double fix = gp(B.fix)[o], fiy = gp(B.fiy)[o], fil = gp(B.fil)[o];
       const double fjx = gp(B.fjx)[o], fjy = gp(B.fjy)[o], fjl = gp(B.fjl)[o];
-      const double sa = axi ? gp(B.sa)[o] : 0.0, sb = axi ? gp(B.sb)[o] : 0.0;
+      const double sa = src ? gsa[o] : 0.0, sb = src ? gsb[o] : 0.0;   // axi: r_ave, 1/r; area: G_x, G_y
+      const double gxC = gxN, gyC = gyN;
+      if (area) { gxN = gsa[at(r + 1)]; gyN = gsb[at(r + 1)]; }
       const VC cE = vc_shdn(cC), cW = vc_shup(cC);      // (x+1, r), (x-1, r)
       const VC cSE = vc_shdn(cS);                      // (x+1, r-1)
       const VC cNE = vc_shdn(cN), cNW = vc_shup(cN);    // (x+1, r+1), (x-1, r+1)
@@ -789,11 +805,13 @@ __global__ __launch_bounds__(256) void k_visc(const DevBlock* __restrict__ block
 #pragma unroll
       for (int k = 0; k < 4; ++k) ME[k] = shdn(MC[k]);
       const double nodW = shup(nodC);                  // node(x-1, r)
+      const double gxE = shdn(gxC), gyE = shdn(gyC);   // G(x+1, r)
       if (own && x <= ni - 1) {                        // i-face (x | x+1)
         double Ma[4], D[3];
 #pragma unroll
         for (int k = 0; k < 4; ++k) Ma[k] = 0.5 * (MC[k] + ME[k]);
-        visc_face(C, axi, 0, fix, fiy, fil, cC, cE, cS, cN, cSE, cNE, Ma, 0.5 * (nodC + nodS), D);
+        const double VG = v_dot_g_face(cC.u, cC.v, cE.u, cE.v, gxC, gyC, gxE, gyE);   // Lib_Diffusive.f90:61-68
+        visc_face(C, axi, 0, fix, fiy, fil, cC, cE, cS, cN, cSE, cNE, Ma, 0.5 * (nodC + nodS), VG, D);
 #pragma unroll
         for (int k = 0; k < 3; ++k) vdx[k * pl + o] = D[k];
       }
@@ -801,12 +819,13 @@ __global__ __launch_bounds__(256) void k_visc(const DevBlock* __restrict__ block
         double Ma[4], D[3];
 #pragma unroll
         for (int k = 0; k < 4; ++k) Ma[k] = 0.5 * (MC[k] + MN[k]);
-        visc_face(C, axi, 1, fjx, fjy, fjl, cC, cN, cW, cE, cNW, cNE, Ma, 0.5 * (nodC + nodW), D);
+        const double VG = v_dot_g_face(cC.u, cC.v, cN.u, cN.v, gxC, gyC, gxN, gyN);
+        visc_face(C, axi, 1, fjx, fjy, fjl, cC, cN, cW, cE, cNW, cNE, Ma, 0.5 * (nodC + nodW), VG, D);
 #pragma unroll
         for (int k = 0; k < 3; ++k) vey[k * pl + o] = D[k];
       }
-      if (axi && own) {
-        // Source_Axi_Visc_Blk :125-236 with the wall ghosts reflected
+      if (src && own) {
+        // Source_Axi_Visc_Blk :125-236 / Source_Area_Visc_Blk, with the wall ghosts reflected
         VC qE = cE, qW = cW, qS = cS, qN = cN;
         const VC refl{cC.r, -cC.u, -cC.v, cC.p, cC.T};
         if (x == 1 && is_wall(0, r)) qW = refl;
@@ -821,19 +840,37 @@ __global__ __launch_bounds__(256) void k_visc(const DevBlock* __restrict__ block
         grad_m((qE.T - qW.T) * 0.5, (qN.T - qS.T) * 0.5, MC, gTx, gTy);
         (void)gTx;
         const double kappa = kl + 0.0;
-        // Stress_Tensor_Axi — Lib_Fluid.f90:166-198
-        const double vr = cC.v;
         const double mi = mil + 0.0;
         const double lam = kNegTwoThird * mi;
-        const double div = gUx + gVy + vr / sa;
-        const double ld2 = lam * div;
-        const double tyy = ld2 + 2.0 * mi * gVy;
-        const double txy = mi * (gUy + gVx);
-        const double tzz = 2.0 * mi * vr / sa + ld2;
-        const double qr = -kappa * gTy;
-        vsv[o] = txy * sb;
-        vsv[pl + o] = (tyy - tzz) * sb;
-        vsv[2 * pl + o] = (-qr + cC.u * txy + cC.v * tyy + 0.0) * sb;
+        if (axi) {
+          // Stress_Tensor_Axi — Lib_Fluid.f90:166-198
+          (void)gTx;
+          const double vr = cC.v;
+          const double div = gUx + gVy + vr / sa;
+          const double ld2 = lam * div;
+          const double tyy = ld2 + 2.0 * mi * gVy;
+          const double txy = mi * (gUy + gVx);
+          const double tzz = 2.0 * mi * vr / sa + ld2;
+          const double qr = -kappa * gTy;
+          vsv[o] = txy * sb;
+          vsv[pl + o] = (tyy - tzz) * sb;
+          vsv[2 * pl + o] = (-qr + cC.u * txy + cC.v * tyy + 0.0) * sb;
+        } else {
+          // Source_Area_Visc_Blk — Lib_Source_Areavariation.f90:147-349 (one species: the species,
+          // enthalpy-diffusion and Hirschfelder terms are 0) with Stress_Tensor_Std and V.G (:128-163)
+          const double Gx = sa, Gy = sb;
+          const double VdG = cC.u * Gx + cC.v * Gy;
+          const double div = gUx + gVy - VdG;
+          const double ld2 = lam * div;
+          const double t11 = ld2 + 2.0 * mi * gUx;
+          const double t22 = ld2 + 2.0 * mi * gVy;
+          const double t12 = mi * (gUy + gVx);
+          const double t33 = ld2 + 2.0 * mi * (-VdG);
+          const double qx = -kappa * gTx, qy = -kappa * gTy;
+          vsv[o] = Gx * (t33 - t11) - Gy * t12;
+          vsv[pl + o] = -Gx * t12 + Gy * (t33 - t22);
+          vsv[2 * pl + o] = -cC.u * (t11 * Gx + t12 * Gy) - cC.v * (t12 * Gx + t22 * Gy) + Gx * qx + Gy * qy;
+        }
       }
       // roll the rows; issue row r+3
       cS = cC; cC = cN; cN = cF; vc_temperature(cN, R);
@@ -902,8 +939,11 @@ __global__ __launch_bounds__(256) void k_bcvisc(const DevBlock* __restrict__ blo
     if (idir) { grad_m(aU, bU, M, gUx, gUy); grad_m(aV, bV, M, gVx, gVy); }
     else { grad_m(bU, aU, M, gUx, gUy); grad_m(bV, aV, M, gVx, gVy); }
     const double Vmx = 0.5 * (u + V2x), Vmy = 0.5 * (v + V2y);
+    // V.G at the symmetry face: <V> . G_loc (:196-200); 0 without area variation
+    const double Gxl = C.source == Q2D_SRC_AREA ? B.sa[oB] : 0.0, Gyl = C.source == Q2D_SRC_AREA ? B.sb[oB] : 0.0;
+    const double VG = v_dot_g_face(u, v, V2x, V2y, Gxl, Gyl, Gxl, Gyl);
     double S1, S2;
-    stress_vector(axi, gUx, gUy, gVx, gVy, n1, n2, mil, r_ave, 0.5 * (v + V2y), S1, S2);
+    stress_vector(axi, gUx, gUy, gVx, gVy, n1, n2, mil, r_ave, 0.5 * (v + V2y), VG, S1, S2);
     F1 = il * S1; F2 = il * S2;
     F3 = il * (Vmx * S1 + Vmy * S2);
   } else {                                           // 302
@@ -916,7 +956,7 @@ __global__ __launch_bounds__(256) void k_bcvisc(const DevBlock* __restrict__ blo
     double gUx, gUy, gVx, gVy, gTx, gTy;
     grad_m(gU0, gU1, M, gUx, gUy); grad_m(gV0, gV1, M, gVx, gVy); grad_m(gT0, gT1, M, gTx, gTy);
     double S1, S2;
-    stress_vector(axi, gUx, gUy, gVx, gVy, n1, n2, mil, r_ave, 0.0, S1, S2);
+    stress_vector(axi, gUx, gUy, gVx, gVy, n1, n2, mil, r_ave, 0.0, 0.0, S1, S2);   // the wall passes no V.G
     F1 = S1 * il; F2 = S2 * il;
     F3 = il * kl * (gTx * n1 + gTy * n2);
   }
@@ -1264,7 +1304,7 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
             }
 #pragma unroll
             for (int k = 0; k < 4; ++k) res[k] = res[k] - S[k] * sr;
-            if (VISC && C.source == Q2D_SRC_AXI) {      // Source_Axi_Visc_Blk, after the Euler source
+            if (VISC) {                                 // Source_Axi_Visc_Blk / Source_Area_Visc_Blk, after the Euler source
 #pragma unroll
               for (int k = 1; k < 4; ++k) res[k] = res[k] - gp(B.vsv)[(k - 1) * pl + oA] * sr;
             }
@@ -2231,8 +2271,6 @@ int append_cases(q2d_handle* h, const q2d_problem* prob, int32_t* first_case) {
     if (s.limiter < 0 || s.limiter > 6) return fail(Q2D_E_ARG, "unknown limiter");
     if (s.riemann < 0 || s.riemann > 1) return fail(Q2D_E_ARG, "unknown Riemann solver");
     if (s.model < 0 || s.model > 1) return fail(Q2D_E_UNSUPPORTED, "only the Euler (0) and laminar (1) models are available");
-    if (s.model == 1 && s.source == Q2D_SRC_AREA)
-      return fail(Q2D_E_UNSUPPORTED, "laminar model with the area-variation source not available on this path");
     if (s.model == 1 && !(s.mu_ref > 0)) return fail(Q2D_E_ARG, "laminar model needs mu_ref > 0");
     if (s.model == 1 && (!(s.T_ref > 0) || !(s.Pr > 0) || !(s.S_suth >= 0)))
       return fail(Q2D_E_ARG, "laminar model needs T_ref > 0, S >= 0, Pr > 0");

@@ -89,6 +89,29 @@ def test_axisymmetric_pipe_mixed_within_tolerance(oracle_mod):
     assert 0 < err <= TOL_L1, err
 
 
+def test_laminar_area_variation_fp64_bit_exact(oracle_mod):
+    """Quasi-2D laminar channel with area variation (V.G in the face and symmetry-plane stresses,
+    Source_Area_Visc_Blk), local dt: bit-identical to the oracle."""
+    pb = cases.laminar_area_channel(seed=5, ni=124, nj=40, area_a=0.6, precision="fp64", steps=150)
+    g, go, o, oo = _run_both(oracle_mod, pb, 150)
+    assert go[0]["iter"] == oo[0]["iter"] == 150
+    assert np.array_equal(g.state(0)[2:-2, 2:-2], o.state(0)[2:-2, 2:-2]), \
+        np.abs(g.state(0)[2:-2, 2:-2] - o.state(0)[2:-2, 2:-2]).max()
+    assert np.array_equal(g.dtlocal(0), o.dtlocal(0))
+
+
+def test_laminar_area_variation_mixed_within_tolerance(oracle_mod):
+    make = lambda prec: cases.laminar_area_channel(seed=6, ni=124, nj=40, area_a=0.4, precision=prec, steps=200)
+    g = GpuSolver(make("mixed"))
+    go = g.run(200)
+    o = oracle_mod.Oracle(make("fp64"))
+    rc, oo = o.run(200, stop_on_todo=False)
+    assert rc == 0 and go[0]["iter"] == oo[0]["iter"] == 200
+    a, r = g.state(0)[2:-2, 2:-2, 0], o.state(0)[2:-2, 2:-2, 0]
+    err = np.abs(a - r).sum() / np.abs(r).sum()
+    assert 0 < err <= TOL_L1, err
+
+
 def test_viscous_and_inviscid_cases_in_one_batch(oracle_mod):
     """A laminar and an Euler case share one launch; each equals its own oracle run bit for bit."""
     pv = cases.axisymmetric_pipe(seed=3, ni=64, nj=24, first_cell=5e-4, reynolds=1e4, precision="fp64", steps=60)
