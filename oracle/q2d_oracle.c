@@ -615,6 +615,25 @@ static void grad_times_M(double g[2], const double M[4]) {
   g[0] = a; g[1] = c;
 }
 
+/* Compute_Diffusive_Flux — Lib_Diffusive.f90:113-199, laminar, one species: Pm the face state, Gr the
+ * gradient after the metric transform (rows rho-fraction, u, v, T).  Dm*grad(Y) = 0, so the species flux
+ * and the enthalpy-diffusion terms vanish. */
+static void compute_diffusive_flux(const ocase *cs, int axi, const double Pm[4], const double Gr[4][2], double n1,
+                                   double n2, double int_l, double r_ave, double VG, double *Flux) {
+  const gas_t *g = &cs->gas;
+  double rho = Pm[0];
+  double T = Pm[3] / (rho * g->R);
+  double mil = mu_lam(g, T), kl = k_lam(g, mil);
+  double kappa = kl + 0.0;
+  double gv[2][2] = {{Gr[1][0], Gr[1][1]}, {Gr[2][0], Gr[2][1]}};
+  double S1, S2;
+  stress_vector(axi, gv, n1, n2, mil, r_ave, Pm[2], VG, &S1, &S2);
+  Flux[0] = 0.0;
+  Flux[1] = int_l * S1;
+  Flux[2] = int_l * S2;
+  Flux[3] = int_l * ((S1 * Pm[1] + S2 * Pm[2]) + kappa * (Gr[3][0] * n1 + Gr[3][1] * n2) + 0.0);
+}
+
 /* Diffusive_Flux + Tangential_Gradient + Compute_Diffusive_Flux — Lib_Diffusive.f90:10-199.
  * P1,P2: the two cells of the face; P3..P6 the tangential stencil; a = direction of the face normal (0/1). */
 static void diffusive_flux(const ocase *cs, double n1, double n2, double int_l, const double *P1, const double *P2,
@@ -645,21 +664,9 @@ static void diffusive_flux(const ocase *cs, double n1, double n2, double int_l, 
   for (int k = 0; k < 4; k++) grad_times_M(Gr[k], M);
   double Pm[4];
   for (int k = 0; k < 4; k++) Pm[k] = 0.5 * (P1[k] + P2[k]);
-  /* Compute_Diffusive_Flux :113-199 */
-  double rho = Pm[0];
-  double T = Pm[3] / (rho * g->R);
-  double mil = mu_lam(g, T), kl = k_lam(g, mil);
-  double kappa = kl + 0.0;
-  double gv[2][2] = {{Gr[1][0], Gr[1][1]}, {Gr[2][0], Gr[2][1]}};
   /* V.G at the face (Definition A; :61-68) */
   double VG = v_dot_g_face(P1[1], P1[2], P2[1], P2[2], G1[0], G1[1], G2[0], G2[1]);
-  double S1, S2;
-  stress_vector(axi, gv, n1, n2, mil, r_ave, Pm[2], VG, &S1, &S2);
-  /* one species: Dm*grad(Y) = 0, so the species flux and the enthalpy diffusion terms vanish */
-  Flux[0] = 0.0;
-  Flux[1] = int_l * S1;
-  Flux[2] = int_l * S2;
-  Flux[3] = int_l * ((S1 * Pm[1] + S2 * Pm[2]) + kappa * (Gr[3][0] * n1 + Gr[3][1] * n2) + 0.0);
+  compute_diffusive_flux(cs, axi, Pm, (const double(*)[2])Gr, n1, n2, int_l, r_ave, VG, Flux);
 }
 
 /* Fluxes_blk_omp_do — Mod_Fluxes.f90:97-234 (convective + laminar diffusive) */
@@ -954,6 +961,62 @@ static double face_r_ave(const oblk *b, int Im, int Jm, int Fm) {
   }
 }
 
+/* BC_Connection_Visc — Lib_BC_Fluxes_Connection.f90:135-265 (laminar, one species).  The ghost stencil:
+ * Pg(:,1), the donor cell the first ghost layer copies (Ghost_Connection, Lib_Ghost.f90:143-160), and
+ * Pg(:,3:4), the donor's neighbours along the face, d11 giving the orientation (Fill_BC_Ghost_Connection,
+ * :302-326), read after every ghost cell is filled. */
+static void bc_connection_visc(q2do *o, oblk *b, const ocase *cs, const q2d_bc_entry *e) {
+  const gas_t *g = &cs->gas;
+  int axi = cs->d.scheme.source == Q2D_SRC_AXI;
+  int Im = e->i, Jm = e->j, Fm = e->f;
+  int modfm2 = 1 - 2 * (Fm % 2);
+  int dir; double n1, n2, int_l;
+  face_geom(b, Fm, Im, Jm, &dir, &n1, &n2, &int_l);
+  int Ig = Im - guide[Fm][0], Jg = Jm - guide[Fm][1];
+  double M[4];
+  for (int k = 0; k < 4; k++) M[k] = 0.5 * ((&MM(b, 1, 1, Im, Jm))[k] + (&MM(b, 1, 1, Ig, Jg))[k]);
+  const oblk *bs = &o->blk[cs->first_block + e->bs - 1];
+  int Is = e->is, Js = e->js, i1, j1, i2, j2;
+  if (e->fs <= 2) { i1 = Is; j1 = Js - e->d11; i2 = Is; j2 = Js + e->d11; }
+  else { i1 = Is - e->d11; j1 = Js; i2 = Is + e->d11; j2 = Js; }
+  const double *Pl = &PV(b, 0, Im, Jm), *Pgh = &PV(bs, 0, Is, Js);
+  /* Visc_Variables: (rho_s / rho, u, v, T) */
+  double Vl[4], Vg[4], Vip[4], Vim[4], Vjp[4], Vjm[4], Vg3[4], Vg4[4];
+  const double *src[8] = {Pl, Pgh, &PV(b, 0, Im + 1, Jm), &PV(b, 0, Im - 1, Jm), &PV(b, 0, Im, Jm + 1),
+                          &PV(b, 0, Im, Jm - 1), &PV(bs, 0, i1, j1), &PV(bs, 0, i2, j2)};
+  double *dst[8] = {Vl, Vg, Vip, Vim, Vjp, Vjm, Vg3, Vg4};
+  for (int k = 0; k < 8; k++) {
+    const double *P = src[k];
+    dst[k][0] = P[0] / P[0]; dst[k][1] = P[1]; dst[k][2] = P[2]; dst[k][3] = P[3] / (P[0] * g->R);
+  }
+  double Gl[4][2], Gg[4][2];
+  double m2 = (double)modfm2, mm2 = (double)(-modfm2);
+  for (int v = 0; v < 4; v++) {
+    if (Fm <= 2) {
+      Gl[v][0] = (Vg[v] - Vl[v]) * m2;
+      Gl[v][1] = (Vjp[v] - Vjm[v]) * 0.5;
+      Gg[v][0] = (Vl[v] - Vg[v]) * mm2;
+      Gg[v][1] = (Vg4[v] - Vg3[v]) * 0.5;
+    } else {
+      Gl[v][0] = (Vip[v] - Vim[v]) * 0.5;
+      Gl[v][1] = (Vg[v] - Vl[v]) * m2;
+      Gg[v][0] = (Vg4[v] - Vg3[v]) * 0.5;
+      Gg[v][1] = (Vl[v] - Vg[v]) * mm2;
+    }
+  }
+  for (int v = 0; v < 4; v++) { grad_times_M(Gl[v], M); grad_times_M(Gg[v], M); }
+  double Pm[4];
+  for (int v = 0; v < 4; v++) Pm[v] = (Pl[v] + Pgh[v]) * 0.5;
+  double r_ave = face_r_ave(b, Im, Jm, Fm);
+  double Gloc[2];
+  area_g(b, cs, Im, Jm, &Gloc[0], &Gloc[1]);
+  double VG = v_dot_g_face(Pl[1], Pl[2], Pgh[1], Pgh[2], Gloc[0], Gloc[1], Gloc[0], Gloc[1]);
+  double Fl[NP], Fg[NP];
+  compute_diffusive_flux(cs, axi, Pm, (const double(*)[2])Gl, n1, n2, int_l, r_ave, VG, Fl);
+  compute_diffusive_flux(cs, axi, Pm, (const double(*)[2])Gg, n1, n2, int_l, r_ave, VG, Fg);
+  for (int v = 1; v < NP; v++) RV(b, v, Im, Jm) = RV(b, v, Im, Jm) - 0.5 * m2 * (Fl[v] + Fg[v]);
+}
+
 /* BC_Symmetry_Visc — Lib_BC_Fluxes_Symmetry.f90:127-236 (laminar) */
 static void bc_symmetry_visc(oblk *b, const ocase *cs, int Im, int Jm, int Fm) {
   const gas_t *g = &cs->gas;
@@ -1039,7 +1102,12 @@ static int bc_fluxes(q2do *o, ocase *cs) {
     const q2d_bc_entry *e = &rec->e;
     oblk *b = &o->blk[cs->first_block + e->b - 1];
     switch (e->type) {
-      case 101: case 201: { int r = bc_connection_eul(b, cs, e->i, e->j, e->f); if (r) err = r; break; }
+      case 101: case 201: {
+        int r = bc_connection_eul(b, cs, e->i, e->j, e->f);
+        if (r) err = r;
+        if (cs->d.scheme.model > 0) bc_connection_visc(o, b, cs, e);   /* Mod_BC_Fluxes.f90:109-113 */
+        break;
+      }
       case 300:
         bc_symmetry_eul(b, cs, e->i, e->j, e->f);
         if (cs->d.scheme.model > 0) bc_symmetry_visc(b, cs, e->i, e->j, e->f);

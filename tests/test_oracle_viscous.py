@@ -254,3 +254,31 @@ def test_area_variation_viscous_heat_flux(oracle_mod):
     want = dt1 * (-kappa * (Gx * a / R + Gy * b / R))
     inner = (slice(1, -1), slice(1, -1))
     assert np.abs((E1 - E0)[inner] - want[inner]).max() <= 1e-9 * np.abs(want[inner]).max()
+
+
+def _laminar(pb):
+    for c in pb.cases:
+        c.model, c.mu_ref, c.T_ref, c.S_suth, c.Pr, c.vnn = 1, 2e-3, 1.0, 0.38, 0.72, 0.2
+    return pb
+
+
+def test_viscous_block_connection_matches_single_block(oracle_mod):
+    """BC_Connection_Visc: the laminar planar shock on 2x2 blocks of n^2 (BC 101 seams, BC 201 periodic)
+    equals the same grid as one block of (2n)^2, whose x-seam is interior faces, to rounding.  The
+    connection's half-and-half gradient (cell gradient of each side, tangential central differences of
+    the donor's neighbours Pg(:,3:4)) averages to the interior face's stencil, so any wrong side, sign or
+    orientation shows as an O(1) seam."""
+    n = 16
+    o4 = oracle_mod.Oracle(_laminar(cases.planar_shock(seed=3, n=n, steps=10, layout=(2, 2))))
+    o1 = oracle_mod.Oracle(_laminar(cases.planar_shock(seed=3, n=2 * n, steps=10, layout=(1, 1))))
+    rc4, out4 = o4.run(10, stop_on_todo=False)
+    rc1, out1 = o1.run(10, stop_on_todo=False)
+    assert rc4 == 0 and rc1 == 0
+    assert out4[0]["dtglobal"] == out1[0]["dtglobal"]
+    S1 = o1.state(0)[2:-2, 2:-2]
+    S4 = np.zeros_like(S1)
+    for by in range(2):
+        for bx in range(2):
+            S4[by * n:(by + 1) * n, bx * n:(bx + 1) * n] = o4.state(by * 2 + bx)[2:-2, 2:-2]
+    scale = np.abs(S1).max(axis=(0, 1))
+    assert (np.abs(S4 - S1).max(axis=(0, 1)) <= 1e-13 * scale).all(), np.abs(S4 - S1).max(axis=(0, 1))
