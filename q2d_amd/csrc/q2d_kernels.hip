@@ -114,7 +114,7 @@ struct DevBlock {
 struct DevBC {
   int type, blk, i, j;           // blk: global block index
   int f, dblk, is, js;           // donor (101/201)
-  int fs, pad0, pad1, pad2;
+  int fs, pad0, pad1, pad2;      // pad0: d11 of a connection (the donor's orientation along the face)
   double mach, T, p, rel_fac, cos_a, sin_a;
 };
 
@@ -707,6 +707,21 @@ __device__ __forceinline__ VC vc_shdn(const VC& c) { return VC{shdn(c.r), shdn(c
 __device__ __forceinline__ VC vc_shup(const VC& c) { return VC{shup(c.r), shup(c.u), shup(c.v), shup(c.p), shup(c.T)}; }
 __device__ __forceinline__ void vc_temperature(VC& c, double R) { c.T = c.p / (c.r * R); }
 
+// Compute_Diffusive_Flux — Lib_Diffusive.f90:113-199 from the face state (pm0..pm3) and the transformed
+// gradients of u, v and T; D = (Flux(nu), Flux(nv), Flux(np)).
+__device__ __forceinline__ void diff_flux(const DevCase& C, bool axi, double pm0, double pm1, double pm2, double pm3,
+                                          double gUx, double gUy, double gVx, double gVy, double gTx, double gTy,
+                                          double n1, double n2, double il, double r_ave, double VG, double D[3]) {
+  const double T = pm3 / (pm0 * C.R);
+  const double mil = mu_lam(C, T);
+  const double kappa = k_lam(C, mil) + 0.0;
+  double S1, S2;
+  stress_vector(axi, gUx, gUy, gVx, gVy, n1, n2, mil, r_ave, pm2, VG, S1, S2);
+  D[0] = il * S1;
+  D[1] = il * S2;
+  D[2] = il * ((S1 * pm1 + S2 * pm2) + kappa * (gTx * n1 + gTy * n2) + 0.0);
+}
+
 // Diffusive_Flux + Tangential_Gradient + Compute_Diffusive_Flux — Lib_Diffusive.f90:10-199 for one interior
 // face.  c1, c2: the face's two cells; c3..c6: the tangential stencil (c4 - c3 + c6 - c5) / 4; a: 0 for an
 // i-face, 1 for a j-face; M: the two cells' metric averaged.  One species: the species flux and the
@@ -726,14 +741,7 @@ __device__ __forceinline__ void visc_face(const DevCase& C, bool axi, int a, dou
   }
   const double pm0 = 0.5 * (c1.r + c2.r), pm1 = 0.5 * (c1.u + c2.u);
   const double pm2 = 0.5 * (c1.v + c2.v), pm3 = 0.5 * (c1.p + c2.p);
-  const double T = pm3 / (pm0 * C.R);
-  const double mil = mu_lam(C, T);
-  const double kappa = k_lam(C, mil) + 0.0;
-  double S1, S2;
-  stress_vector(axi, gUx, gUy, gVx, gVy, n1, n2, mil, r_ave, pm2, VG, S1, S2);
-  D[0] = il * S1;
-  D[1] = il * S2;
-  D[2] = il * ((S1 * pm1 + S2 * pm2) + kappa * (gTx * n1 + gTy * n2) + 0.0);
+  diff_flux(C, axi, pm0, pm1, pm2, pm3, gUx, gUy, gVx, gVy, gTx, gTy, n1, n2, il, r_ave, VG, D);
 }
 
 // Interior diffusive face fluxes and the axisymmetric viscous source of every cell of the viscous work
@@ -946,6 +954,62 @@ __global__ __launch_bounds__(256) void k_bcvisc(const DevBlock* __restrict__ blo
     stress_vector(axi, gUx, gUy, gVx, gVy, n1, n2, mil, r_ave, 0.5 * (v + V2y), VG, S1, S2);
     F1 = il * S1; F2 = il * S2;
     F3 = il * (Vmx * S1 + Vmy * S2);
+  } else if (E.type == 101 || E.type == 201) {
+    // BC_Connection_Visc — Lib_BC_Fluxes_Connection.f90:135-265: the mean of the face flux from this side's
+    // cell gradient and from the donor side's (Pg(:,1) the donor cell, Pg(:,3:4) its neighbours along the
+    // face, oriented by d11: Fill_BC_Ghost_Connection, Lib_Ghost.f90:302-326)
+    const DevBlock& S = blocks[E.dblk];
+    const double* Ps = S.P[stage_in(K.cur, irk)];
+    const long long pls = S.plane;
+    auto offs = [&S](int i, int j) { return (long long)(j + 1) * S.pitch + (i + 1); };
+    const int d11 = E.pad0;
+    int i1, j1, i2, j2;
+    if (E.fs <= 2) { i1 = E.is; j1 = E.js - d11; i2 = E.is; j2 = E.js + d11; }
+    else { i1 = E.is - d11; j1 = E.js; i2 = E.is + d11; j2 = E.js; }
+    const int Ig = Im - c_guide[Fm][0], Jg = Jm - c_guide[Fm][1];
+    const long long oG = off(Ig, Jg);
+    double Mc[4] = {B.m11[oG], B.m21[oG], B.m12[oG], B.m22[oG]};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) Mc[k] = 0.5 * (M[k] + Mc[k]);
+    // Visc_Variables (u, v, T; the species fraction rho/rho has zero gradient) of the eight stencil cells
+    const long long oS = offs(E.is, E.js), oS3 = offs(i1, j1), oS4 = offs(i2, j2);
+    const double gr = Ps[oS], gu = Ps[pls + oS], gv = Ps[2 * pls + oS], gpr = Ps[3 * pls + oS];
+    auto vvar = [&](const double* Q, long long pq, long long o, double out[3]) {
+      const double r = Q[o];
+      out[0] = Q[pq + o]; out[1] = Q[2 * pq + o]; out[2] = Q[3 * pq + o] / (r * C.R);
+    };
+    double Vl[3], Vg[3], Vip[3], Vim[3], Vjp[3], Vjm[3], Vg3[3], Vg4[3];
+    vvar(P, pl, oB, Vl); vvar(Ps, pls, oS, Vg);
+    vvar(P, pl, off(Im + 1, Jm), Vip); vvar(P, pl, off(Im - 1, Jm), Vim);
+    vvar(P, pl, off(Im, Jm + 1), Vjp); vvar(P, pl, off(Im, Jm - 1), Vjm);
+    vvar(Ps, pls, oS3, Vg3); vvar(Ps, pls, oS4, Vg4);
+    const double m2 = (double)modfm2, mm2 = (double)(-modfm2);
+    double gl[3][2], gg[3][2];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      if (idir) {
+        gl[k][0] = (Vg[k] - Vl[k]) * m2;   gl[k][1] = (Vjp[k] - Vjm[k]) * 0.5;
+        gg[k][0] = (Vl[k] - Vg[k]) * mm2;  gg[k][1] = (Vg4[k] - Vg3[k]) * 0.5;
+      } else {
+        gl[k][0] = (Vip[k] - Vim[k]) * 0.5; gl[k][1] = (Vg[k] - Vl[k]) * m2;
+        gg[k][0] = (Vg4[k] - Vg3[k]) * 0.5; gg[k][1] = (Vl[k] - Vg[k]) * mm2;
+      }
+    }
+    double a[3][2], b[3][2];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) { grad_m(gl[k][0], gl[k][1], Mc, a[k][0], a[k][1]); grad_m(gg[k][0], gg[k][1], Mc, b[k][0], b[k][1]); }
+    const double pm0 = (rho + gr) * 0.5, pm1 = (u + gu) * 0.5, pm2 = (v + gv) * 0.5, pm3 = (p + gpr) * 0.5;
+    const double Gxl = C.source == Q2D_SRC_AREA ? B.sa[oB] : 0.0, Gyl = C.source == Q2D_SRC_AREA ? B.sb[oB] : 0.0;
+    const double VG = v_dot_g_face(u, v, gu, gv, Gxl, Gyl, Gxl, Gyl);
+    double Dl[3], Dg[3];
+    diff_flux(C, axi, pm0, pm1, pm2, pm3, a[0][0], a[0][1], a[1][0], a[1][1], a[2][0], a[2][1], n1, n2, il, r_ave, VG, Dl);
+    diff_flux(C, axi, pm0, pm1, pm2, pm3, b[0][0], b[0][1], b[1][0], b[1][1], b[2][0], b[2][1], n1, n2, il, r_ave, VG, Dg);
+    BCFlux& q = out[k];
+    q.V[0] = 0.0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) q.V[c + 1] = -(0.5 * m2 * (Dl[c] + Dg[c]));
+    q.vmask = 0xE;
+    return;
   } else {                                           // 302
     const double T = p / (rho * C.R);
     const double mil = mu_lam(C, E.T), kl = k_lam(C, mil);
@@ -2202,7 +2266,8 @@ int rebuild_descriptors(q2d_handle* h) {
     const int pos = r.f <= 2 ? r.j - 1 : r.i - 1;
     if (fl[pos] >= 0) return fail(Q2D_E_ARG, "two bc records for one boundary face");
     fl[pos] = k;
-    if (h->dc[h->hb[r.blk].case_id].model > 0 && (r.type == 300 || r.type == 302)) h->vrec.push_back(k);
+    if (h->dc[h->hb[r.blk].case_id].model > 0 && (r.type == 300 || r.type == 302 || r.type == 101 || r.type == 201))
+      h->vrec.push_back(k);
   }
   for (int b = 0; b < h->nblocks; ++b)
     for (int f = 0; f < 4; ++f) {
@@ -2318,8 +2383,8 @@ int append_cases(q2d_handle* h, const q2d_problem* prob, int32_t* first_case) {
         break;
       default: return fail(Q2D_E_UNSUPPORTED, "bc type not available on this path");
     }
-    if (prob->cases[c].scheme.model > 0 && (e.type == 101 || e.type == 201))
-      return fail(Q2D_E_UNSUPPORTED, "block connections of a laminar case (BC_Connection_Visc) not available on this path");
+    if (prob->cases[c].scheme.model > 0 && (e.type == 101 || e.type == 201) && e.d11 != 1 && e.d11 != -1)
+      return fail(Q2D_E_ARG, "laminar block connection needs d11 = +1 or -1 (the donor's orientation along the face)");
     DevBC& r = newbc[k];
     memset(&r, 0, sizeof(r));
     r.type = e.type; r.blk = block0 + lb; r.i = e.i; r.j = e.j; r.f = e.f;
@@ -2333,6 +2398,7 @@ int append_cases(q2d_handle* h, const q2d_problem* prob, int32_t* first_case) {
         if (ii < 1 || ii > sd.ni || jj < 1 || jj > sd.nj) return fail(Q2D_E_ARG, "bc donor cell outside donor block");
       }
       r.dblk = block0 + first_block[c] + e.bs - 1; r.is = e.is; r.js = e.js; r.fs = e.fs;
+      r.pad0 = e.d11;                                 // orientation of the donor along the face (d11)
     }
     r.mach = e.mach; r.T = e.T; r.p = e.p; r.rel_fac = e.rel_fac;
     if (e.type == 405) {                                // Compute_Inflow_Direction (Lib_BC_Fluxes_Inflow.f90:548-560)

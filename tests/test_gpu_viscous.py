@@ -127,10 +127,38 @@ def test_viscous_and_inviscid_cases_in_one_batch(oracle_mod):
         assert np.array_equal(g.state(k)[2:-2, 2:-2], o.state(0)[2:-2, 2:-2]), k
 
 
-def test_laminar_connection_rejected():
-    pb = cases.planar_shock(seed=0, n=32, steps=5)
+def _laminar(pb):
     for c in pb.cases:
-        c.model, c.mu_ref = 1, 1e-3
+        c.model, c.mu_ref, c.T_ref, c.S_suth, c.Pr, c.vnn = 1, 2e-3, 1.0, 0.38, 0.72, 0.2
+    return pb
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_laminar_block_connections(oracle_mod, precision):
+    """Laminar planar shock on 2x2 blocks: BC 101 seams and BC 201 periodicity carry the viscous connection
+    flux (BC_Connection_Visc).  FP64: bit-identical to the oracle; mixed: relative L1(rho) <= 1e-5."""
+    pb = _laminar(cases.planar_shock(seed=0, n=32, steps=40, precision=precision))
+    g = GpuSolver(pb)
+    go = g.run(40)
+    o = oracle_mod.Oracle(_laminar(cases.planar_shock(seed=0, n=32, steps=40, precision="fp64")))
+    rc, oo = o.run(40, stop_on_todo=False)
+    assert rc == 0 and go[0]["iter"] == oo[0]["iter"] == 40
+    if precision == "fp64":
+        assert go[0]["dtglobal"] == oo[0]["dtglobal"]
+        for b in range(4):
+            assert np.array_equal(g.state(b)[2:-2, 2:-2], o.state(b)[2:-2, 2:-2]), \
+                (b, np.abs(g.state(b)[2:-2, 2:-2] - o.state(b)[2:-2, 2:-2]).max())
+    else:
+        num = sum(np.abs(g.state(b)[2:-2, 2:-2, 0] - o.state(b)[2:-2, 2:-2, 0]).sum() for b in range(4))
+        den = sum(np.abs(o.state(b)[2:-2, 2:-2, 0]).sum() for b in range(4))
+        assert 0 < num / den <= TOL_L1, num / den
+
+
+def test_laminar_connection_needs_orientation():
+    pb = _laminar(cases.planar_shock(seed=0, n=16, steps=5))
+    for r in pb.bc:
+        if r["type"] in (101, 201):
+            r["d11"] = 0
     with pytest.raises(Q2DError) as ei:
         GpuSolver(pb)
-    assert ei.value.code == -6
+    assert ei.value.code == -1
