@@ -126,7 +126,9 @@ typedef struct q2d_block_desc {
 /* One boundary-cell record of domain%bc (Advanced_Types_m.f90:78-122, read by
  * IO_BC.f90:163-...).  b/bs are 1-based block numbers inside the record's case;
  * i, j, is, js are 1-based cell indices; f, fs in 1..4.  Records of one case
- * are ordered by block then face, as BC_Fluxes assumes (Mod_BC_Fluxes.f90:79-86). */
+ * are ordered by block then face, as BC_Fluxes assumes (Mod_BC_Fluxes.f90:79-86).
+ * For a connection (101/201) of a laminar case, d11 (+1 or -1) orients the donor
+ * along the face (Fill_BC_Ghost_Connection, Lib_Ghost.f90:302-326). */
 typedef struct q2d_bc_entry {
   int32_t b, i, j, f, type;
   int32_t bs, is, js, fs, d11, d12, d21, d22;
