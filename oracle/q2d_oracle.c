@@ -1372,7 +1372,11 @@ static int step_case(q2do *o, ocase *cs, q2d_step_out *out) {
     { int r = bc_fluxes(o, cs); if (r && !status) status = r; }
     if (s->source != Q2D_SRC_NONE) ghost_wall_extrapolation(o, cs);   /* Mod_Explicit.f90:83 */
     sources(o, cs);
-    { int r = newstate(o, cs, irk, &fb, &fi, &fj); if (r && !status) status = r; }
+    {                                                   /* the first failure stops the reference */
+      int tb = 0, ti = 0, tj = 0;
+      int r = newstate(o, cs, irk, &tb, &ti, &tj);
+      if (r && !status) { status = r; fb = tb; fi = ti; fj = tj; }
+    }
   }
   double res[4];
   residual(o, cs, res);

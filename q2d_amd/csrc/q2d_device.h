@@ -224,6 +224,21 @@ __device__ __forceinline__ float limiter_mx(int id, float x, float y) {
   return limiter<float>(id, x, y);
 }
 
+// The same with the limiter fixed at compile time (no branch on the limiter id).
+template <int LIM>
+__device__ __forceinline__ float limiter_mxt(float x, float y) {
+#pragma clang fp contract(fast)
+  if constexpr (LIM == 1) {                               // minmod
+    if (x * y <= 0.f) return 0.f;
+    return __builtin_fabsf(x) > __builtin_fabsf(y) ? y : x;
+  } else if constexpr (LIM == 2) {                        // van Leer
+    const float xy = x * y;
+    return xy > 0.f ? 2.f * xy * frcp(x + y) : 0.f;
+  } else {
+    return limiter<float>(LIM, x, y);
+  }
+}
+
 // HLLC (RIE 0, Batten et al. 1997 as in riemann_HLLC, Lib_Riemann_HLL.f90:34-123) or HLLE (RIE 1,
 // Einfeldt, riemann_HLLE :126-169) flux through a face of unit normal (nx, ny), times its length il,
 // from primitive states L, R = (rho, u, v, p).  Wave speeds S1 = min(0, unL - aL, un~ - a~),
@@ -262,19 +277,18 @@ __device__ __forceinline__ void flux_mx(const GasF& g, const float L[4], const f
     const bool star = (S1 < 0.f) && ((Sst >= 0.f) || (S4 > 0.f));
     const float Sk = left ? S1 : S4, dk = left ? L[0] : R[0], uk = left ? L[1] : R[1], vk = left ? L[2] : R[2];
     const float pk = left ? L[3] : R[3], unk = left ? unL : unR, hk = left ? hL : hR, irk = left ? irL : irR;
-    float Fr = dk * unk;
-    float Fu = Fr * uk + pk * nx, Fv = Fr * vk + pk * ny, FE = Fr * hk;
-    if (star) {
-      const float id = frcp(Sk - Sst);
-      const float U1 = dk * (Sk - unk) * id;              // star-state density
-      const float dp = (pst - pk) * id;
-      const float e0k = hk - pk * irk;                    // total energy per unit mass
-      Fr = Fr + Sk * (U1 - dk);
-      Fu = Fu + Sk * (U1 * uk + dp * nx - dk * uk);
-      Fv = Fv + Sk * (U1 * vk + dp * ny - dk * vk);
-      FE = FE + Sk * (U1 * e0k + (pst * Sst - pk * unk) * id - dk * e0k);
-    }
-    F[0] = Fr; F[1] = Fu; F[2] = Fv; F[3] = FE;
+    const float Fr = dk * unk;
+    const float Fu = Fr * uk + pk * nx, Fv = Fr * vk + pk * ny, FE = Fr * hk;
+    // star-state correction, evaluated on every lane and selected (no divergent branch; where it is
+    // not selected its operands may be non-finite)
+    const float id = frcp(Sk - Sst);
+    const float U1 = dk * (Sk - unk) * id;                // star-state density
+    const float dp = (pst - pk) * id;
+    const float e0k = hk - pk * irk;                      // total energy per unit mass
+    F[0] = star ? Fr + Sk * (U1 - dk) : Fr;
+    F[1] = star ? Fu + Sk * (U1 * uk + dp * nx - dk * uk) : Fu;
+    F[2] = star ? Fv + Sk * (U1 * vk + dp * ny - dk * vk) : Fv;
+    F[3] = star ? FE + Sk * (U1 * e0k + (pst * Sst - pk * unk) * id - dk * e0k) : FE;
   }
 #pragma unroll
   for (int v = 0; v < 4; ++v) F[v] = F[v] * il;

@@ -27,6 +27,7 @@
 #include <cstring>
 #include <functional>
 #include <string>
+#include <type_traits>
 #include <utility>
 #include <vector>
 
@@ -43,6 +44,7 @@ constexpr int kMaxCases = 1 << 20;
 constexpr int kTimedSteps = 64;                         // timed steps between two reads of the slots
 constexpr int kLaunchesPerStep = 3 * 4;                 // stage launches per step (3 stages x 4 variants)
 constexpr int kTimedSlots = kLaunchesPerStep * kTimedSteps;
+constexpr int kGraphSteps = 8;                          // steps per replay of the multi-step graph
 constexpr int kStampLanes = 64;                         // sub-slots per launch slot
 // Rows of state the stage kernel's loads run ahead of the row being computed (1 or 2), per form: the
 // FP64 form's longer rows leave room for a second row in flight, measured 3 % faster.
@@ -114,7 +116,8 @@ struct DevBlock {
 struct DevBC {
   int type, blk, i, j;           // blk: global block index
   int f, dblk, is, js;           // donor (101/201)
-  int fs, pad0, pad1, pad2;      // pad0: d11 of a connection (the donor's orientation along the face)
+  int fs, pad0, cid, pad2;       // pad0: d11 of a connection (the donor's orientation along the face);
+                                 // cid: global case index of the record's block
   double mach, T, p, rel_fac, cos_a, sin_a;
 };
 
@@ -198,9 +201,7 @@ __device__ __forceinline__ void record_fail(DevCtrl* c, int code, unsigned long 
 // use_next = 1: the previous step's last stage already reduced this step's dt (from the same
 // state Compute_dt reads); use_next = 0: k_dt computes it (first step, or after an upload).
 // With use_next the step's dt is already known, so this kernel also finishes it (k_dt_finish's work).
-__global__ void k_begin(DevCtrl* ctrl, const DevCase* cases, int ncases, int use_next) {
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncases) return;
+__device__ __forceinline__ void begin_case(DevCtrl* ctrl, const DevCase* cases, int c, int use_next) {
   DevCtrl& k = ctrl[c];
   if (!k.active) return;
   k.iter += 1;                                // Mod_Explicit.f90:38-40
@@ -213,6 +214,11 @@ __global__ void k_begin(DevCtrl* ctrl, const DevCase* cases, int ncases, int use
     k.dtglobal = __longlong_as_double((long long)k.dtmin_bits);   // Mod_dt.f90:54-55
     if (cases[c].time_accurate) k.time = k.time + k.dtglobal;     // Mod_Explicit.f90:51-54
   }
+}
+
+__global__ void k_begin(DevCtrl* ctrl, const DevCase* cases, int ncases, int use_next) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < ncases) begin_case(ctrl, cases, c, use_next);
 }
 
 __global__ void k_dt_finish(DevCtrl* ctrl, const DevCase* cases, int ncases) {
@@ -442,15 +448,19 @@ template <typename FT>
 // phase 0: ghost fill and flux of each record in one pass.  Blocks one cell wide in a direction need
 // two passes (phase 1: every record's ghost fill, phase 2: the fluxes), because there a record's
 // interior neighbour is a ghost cell of the opposite face's record.
+// begin_cases > 0 (the step's first launch, when its dt came from the previous step's last stage): the
+// launch also does k_begin's per-case work, which touches no control word that a record reads.
 __global__ __launch_bounds__(256) void k_bcflux(const DevBlock* __restrict__ blocks, const DevCase* __restrict__ cases,
-                                                const DevCtrl* __restrict__ ctrl, const DevBC* __restrict__ bcs, int nbc,
-                                                int irk, BCFlux* __restrict__ out, int phase) {
+                                                DevCtrl* __restrict__ ctrl, const DevBC* __restrict__ bcs, int nbc,
+                                                int irk, BCFlux* __restrict__ out, int phase, int begin_cases) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  for (int c = t; c < begin_cases; c += gridDim.x * blockDim.x) begin_case(ctrl, cases, c, 1);
   if (t >= nbc) return;
   const DevBC E = bcs[t];
+  // the case's descriptors are indexed from the record itself, so their loads do not wait for the block's
   const DevBlock& B = blocks[E.blk];
-  const DevCtrl& K = ctrl[B.case_id];
-  const DevCase& C = cases[B.case_id];
+  const DevCtrl& K = ctrl[E.cid];
+  const DevCase& C = cases[E.cid];
   if (!K.active || irk > C.n_rk) return;
   const int buf = stage_in(K.cur, irk);
   const double* P = B.P[buf];
@@ -524,7 +534,7 @@ __global__ __launch_bounds__(256) void k_bcflux(const DevBlock* __restrict__ blo
       R[v] = (FT)p3 - sr * (FT)Dl3;
     }
     const bool bad = !(L[0] > FT(0) && L[3] > FT(0) && R[0] > FT(0) && R[3] > FT(0));
-    if (bad) record_fail(const_cast<DevCtrl*>(&K), Q2D_E_INTERFACE, fail_key(K.stage_count + irk, 0, B.blk_in_case, Jm, Im));
+    if (bad) record_fail(&ctrl[E.cid], Q2D_E_INTERFACE, fail_key(K.stage_count + irk, 0, B.blk_in_case, Jm, Im));
     FT F[4];
     riemann_rt<FT>(rid, gf, L, R, (FT)n1, (FT)n2, F);
     // one species: Sel_L*rho_L/rho_L - Sel_R*rho_R/rho_R == 1, so Flux(1) = F_r*int_l
@@ -1098,7 +1108,6 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
                                                int xcd_remap, unsigned long long* __restrict__ tstamp, int tslot) {
   // launch timing (only when tstamp is set): earliest workgroup start and latest wavefront end on the
   // device's constant-rate wall clock, in the slot of this launch within the current timed step
-  // (tstamp[0] counts steps; slots form a ring of kTimedSteps steps)
   // (tstamp[0] counts steps; slots form a ring of kTimedSteps steps).  Each slot has kStampLanes
   // sub-slots, picked by workgroup, so that the atomics of one launch spread over kStampLanes words.
   unsigned long long* tslotp = nullptr;
@@ -1491,6 +1500,15 @@ typedef const __attribute__((address_space(1))) f4v* GP4;
 #ifndef Q2D_STAGE_MX_MIN_WAVES
 #define Q2D_STAGE_MX_MIN_WAVES 1
 #endif
+// Streaming (non-temporal) access for the data each mixed stage touches once: the new state it writes
+// and the RK base state it reads; the L2 keeps the state rows that neighbouring items re-read (their
+// halos).  Measured on 4 x 1024^2, SSP-RK3: 0.516 -> 0.508 ms per step.
+#ifndef Q2D_MX_NT_STORE
+#define Q2D_MX_NT_STORE 1
+#endif
+#ifndef Q2D_MX_NT_LOAD
+#define Q2D_MX_NT_LOAD 1
+#endif
 template <int RIE, bool FUSE>
 __global__ __launch_bounds__(256, Q2D_STAGE_MX_MIN_WAVES) void k_stage_mx(const DevBlock* __restrict__ blocks, const DevCase* __restrict__ cases,
                                                   DevCtrl* __restrict__ ctrl, const Item* __restrict__ items, int item_lo,
@@ -1521,119 +1539,154 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MX_MIN_WAVES) void k_stage_mx(const 
     DevCtrl& K = ctrl[B.case_id];
     const DevCase& C = cases[B.case_id];
     if (!K.active || irk > C.n_rk) continue;
-    const bool last = (irk == C.n_rk);
-    const int ni = B.ni, nj = B.nj, pitch = B.pitch;
-    const long long pl = B.plane;
-    const int kcur = __builtin_amdgcn_readfirstlane(K.cur), kdtsel = __builtin_amdgcn_readfirstlane(K.dtsel);
-    const int kiter = __builtin_amdgcn_readfirstlane(K.iter);   // control words once per item (see k_stage)
-    const int bin = stage_in(kcur, irk), bout = stage_out(kcur, irk);
-    GP Pin = gp(B.P[bin]);
-    GP PO = gp(B.P[kcur]);
-    GPW Pout = gpw(B.P[bout]);
-    GP dtlp = gp(B.dtl[kdtsel]);
-    GPW dtl_next = gpw(B.dtl[kdtsel ^ 1]);
-    GP4 gip = (GP4)B.mgi, gjp = (GP4)B.mgj, gdp = (GP4)B.mgd;
-    GPF hyp = (GPF)B.mhy;
-    const int lim = C.limiter;
-    const bool tacc = C.time_accurate != 0;
-    const GasF gf{(float)C.gam, (float)(C.gam - 1.0), (float)(C.gam / (C.gam - 1.0))};
-    const double gm1 = C.gam - 1.0, rgm1 = 1.0 / (C.gam - 1.0);
-    const double c_rk = (C.n_rk == 1) ? 1.0 : (C.n_rk == 2 ? (irk == 1 ? 1.0 : 0.5)
-                                                            : (irk == 1 ? 1.0 : (irk == 2 ? 0.25 : 2.0 / 3.0)));
-    const double dtg = K.dtglobal;
-    const int stage_id = K.stage_count + irk;
+    // The item body is instantiated per limiter (a uniform switch per item), so the row loop has no
+    // limiter branches: straight-line code that the compiler can schedule across the whole row.
+    auto item_body = [&](auto limc) {
+      constexpr int LIM = decltype(limc)::value;
+      const bool last = (irk == C.n_rk);
+      const int ni = B.ni, nj = B.nj, pitch = B.pitch;
+      const long long pl = B.plane;
+      const int kcur = __builtin_amdgcn_readfirstlane(K.cur), kdtsel = __builtin_amdgcn_readfirstlane(K.dtsel);
+      const int kiter = __builtin_amdgcn_readfirstlane(K.iter);   // control words once per item (see k_stage)
+      const int bin = stage_in(kcur, irk), bout = stage_out(kcur, irk);
+      GP Pin = gp(B.P[bin]);
+      GP PO = gp(B.P[kcur]);
+      GPW Pout = gpw(B.P[bout]);
+      GP dtlp = gp(B.dtl[kdtsel]);
+      GPW dtl_next = gpw(B.dtl[kdtsel ^ 1]);
+      GP4 gip = (GP4)B.mgi, gjp = (GP4)B.mgj, gdp = (GP4)B.mgd;
+      GPF hyp = (GPF)B.mhy;
+      const bool tacc = C.time_accurate != 0;
+      const GasF gf{(float)C.gam, (float)(C.gam - 1.0), (float)(C.gam / (C.gam - 1.0))};
+      const double gm1 = C.gam - 1.0, rgm1 = 1.0 / (C.gam - 1.0);
+      const double c_rk = (C.n_rk == 1) ? 1.0 : (C.n_rk == 2 ? (irk == 1 ? 1.0 : 0.5)
+                                                              : (irk == 1 ? 1.0 : (irk == 2 ? 0.25 : 2.0 / 3.0)));
+      const double dtg = K.dtglobal;
+      const int stage_id = K.stage_count + irk;
 
-    const int x = item.x0 - 2 + lane;
-    const int xc = x <= ni + 2 ? x : ni + 2;
-    const bool own = lane >= 2 && lane < 2 + kOwn && x <= ni;
-    const bool xdo = lane >= 1 && lane <= 61 && x >= 1 && x <= ni - 1;
-    const int rec3 = (own && B.bcf[2] >= 0) ? bcface[B.bcf[2] + x - 1] : -1;
-    const int rec4 = (own && B.bcf[3] >= 0) ? bcface[B.bcf[3] + x - 1] : -1;
-    const int rmax = nj + 2;
-    auto at = [&](int r) { return (long long)(r + 1) * pitch + (xc + 1); };
+      const int x = item.x0 - 2 + lane;
+      const int xc = x <= ni + 2 ? x : ni + 2;
+      const bool own = lane >= 2 && lane < 2 + kOwn && x <= ni;
+      const bool xdo = lane >= 1 && lane <= 61 && x >= 1 && x <= ni - 1;
+      const int rec3 = (own && B.bcf[2] >= 0) ? bcface[B.bcf[2] + x - 1] : -1;
+      const int rec4 = (own && B.bcf[3] >= 0) ? bcface[B.bcf[3] + x - 1] : -1;
+      const int rmax = nj + 2;
+      auto at = [&](int r) { return (long long)(r + 1) * pitch + (xc + 1); };
 
-    // ---- pipeline warm-up: rows y0-2 .. y0+1 ------------------------------
-    double qA[4], qB[4], qN[4];                        // FP64 rows r, r+1 and r+2 (in flight)
-    float fA[4], fB[4];                                // their FP32 copies (rows r, r+1)
-    float hyA, hyB, hyN;
-    float sAB[4], phiA[4], Gprev[4];
-    f4v gjN = gjp[at(item.y0 - 1)];                    // j-face geometry, one row ahead
-    {
-      double qm[4];
-      const long long o0 = at(item.y0 - 2), o1 = at(item.y0 - 1), o2 = at(item.y0), o3 = at(item.y0 + 1);
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        qm[v] = Pin[v * pl + o0]; qA[v] = Pin[v * pl + o1]; qB[v] = Pin[v * pl + o2]; qN[v] = Pin[v * pl + o3];
-      }
-      const float hym = hyp[o0];
-      hyA = hyp[o1]; hyB = hyp[o2]; hyN = hyp[o3];
-      const float i0 = frcp(hym + hyA), i1 = frcp(hyA + hyB);
-      float s0[4];
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        fA[v] = (float)qA[v]; fB[v] = (float)qB[v];
-        s0[v] = (fA[v] - (float)qm[v]) * i0;
-        sAB[v] = (fB[v] - fA[v]) * i1;
-        Gprev[v] = 0.f;
-      }
-#pragma unroll
-      for (int v = 0; v < 4; ++v) phiA[v] = limiter_mx(lim, sAB[v], s0[v]);
-    }
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    double dtnext = 1e300;
-
-    for (int r = item.y0 - 1; r <= item.y1; ++r) {
-      // Loads of this iteration, in the order they are consumed (vector-memory waits are in issue
-      // order): first row r's own data (i-face geometry, RK base state, local dt, the fused dt's
-      // directions; first used after the j-face flux), then row r+3 and row r+1's j-face geometry for the
-      // next iteration.  Row r+2 and row r's j-face geometry and 1/sur arrived (issued one iteration ago).
-      double qC[4];
-#pragma unroll
-      for (int v = 0; v < 4; ++v) qC[v] = qN[v];
-      const float hyC = hyN;
-      const f4v gj = gjN;
-      const long long oc = at(r);
-      f4v gi = {0.f, 0.f, 0.f, 0.f}, gd = {0.f, 0.f, 0.f, 0.f};
-      double po[4] = {0.0, 0.0, 0.0, 0.0};
-      double dtl = 0.0;
-      if (r >= item.y0) {
-        gi = gip[oc];
-        if (irk != 1) {
-#pragma unroll
-          for (int v = 0; v < 4; ++v) po[v] = PO[v * pl + oc];
-        }
-        if (own && !tacc) dtl = dtlp[oc];
-        if (FUSE && last) gd = gdp[oc];
-      }
+      // ---- pipeline warm-up: rows y0-2 .. y0+1 ------------------------------
+      double qA[4], qB[4], qN[4];                        // FP64 rows r, r+1 and r+2 (in flight)
+      float fA[4], fB[4];                                // their FP32 copies (rows r, r+1)
+      float hyA, hyB, hyN;
+      float sAB[4], phiA[4], Gprev[4];
+      f4v gjN = gjp[at(item.y0 - 1)];                    // j-face geometry, one row ahead
       {
+        double qm[4];
+        const long long o0 = at(item.y0 - 2), o1 = at(item.y0 - 1), o2 = at(item.y0), o3 = at(item.y0 + 1);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          qm[v] = Pin[v * pl + o0]; qA[v] = Pin[v * pl + o1]; qB[v] = Pin[v * pl + o2]; qN[v] = Pin[v * pl + o3];
+        }
+        const float hym = hyp[o0];
+        hyA = hyp[o1]; hyB = hyp[o2]; hyN = hyp[o3];
+        const float i0 = frcp(hym + hyA), i1 = frcp(hyA + hyB);
+        float s0[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          fA[v] = (float)qA[v]; fB[v] = (float)qB[v];
+          s0[v] = (fA[v] - (float)qm[v]) * i0;
+          sAB[v] = (fB[v] - fA[v]) * i1;
+          Gprev[v] = 0.f;
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) phiA[v] = limiter_mxt<LIM>(sAB[v], s0[v]);
+      }
+      double acc[4] = {0.0, 0.0, 0.0, 0.0};
+      double dtnext = 1e300;
+      // Failures are kept per lane (the first row of each kind) and recorded once after the rows, so
+      // the row loop has no branch around them; the recorded key and status are the same minima.
+      constexpr int kNoRow = 0x7fffffff;
+      int fail_if_r = kNoRow, fail_st_r = kNoRow;
+
+      // One j-face (r | r+1) of the march: the FP32 copy and slopes of row r+2 (qC), the limited slopes
+      // of row r+1 and the face flux from rows r, r+1 (fA, fB) — Mod_Fluxes.f90:187-232.  The flux is
+      // evaluated on every lane; lanes that own no face of this row discard it.
+      float fC[4], sBC[4], phiB[4], Gcur[4];
+      auto jface = [&](int r, const double (&qC)[4], float hyC, const f4v& gj) {
+        const float iBC = frcp(hyB + hyC);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) { fC[v] = (float)qC[v]; sBC[v] = (fC[v] - fB[v]) * iBC; }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) phiB[v] = limiter_mxt<LIM>(sBC[v], sAB[v]);
+        const bool jdo = own && r >= 1 && r <= nj - 1;
+        float L[4], R[4], G[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) { L[v] = fA[v] + phiA[v] * hyA; R[v] = fB[v] - phiB[v] * hyB; }
+        const bool bad = jdo && !(L[0] > 0.f && L[3] > 0.f && R[0] > 0.f && R[3] > 0.f);
+        fail_if_r = bad ? min(fail_if_r, r) : fail_if_r;
+        flux_mx<RIE>(gf, L, R, gj.x, gj.y, gj.z, G);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) Gcur[v] = jdo ? G[v] : 0.f;
+      };
+      auto roll = [&](const double (&qC)[4], float hyC) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          qA[v] = qB[v]; qB[v] = qC[v]; fA[v] = fB[v]; fB[v] = fC[v];
+          sAB[v] = sBC[v]; phiA[v] = phiB[v]; Gprev[v] = Gcur[v];
+        }
+        hyA = hyB; hyB = hyC;
+      };
+
+      {                                                  // row y0-1: only its j-face (y0-1 | y0)
+        const int r = item.y0 - 1;
+        double qC[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) qC[v] = qN[v];
+        const float hyC = hyN;
+        const f4v gj = gjN;
         const long long oN = at(min(r + 3, rmax));
 #pragma unroll
         for (int v = 0; v < 4; ++v) qN[v] = Pin[v * pl + oN];
         hyN = hyp[oN];
         gjN = gjp[at(min(r + 1, rmax))];
-      }
-      float fC[4], sBC[4], phiB[4], Gcur[4];
-      {
-        const float iBC = frcp(hyB + hyC);
-#pragma unroll
-        for (int v = 0; v < 4; ++v) { fC[v] = (float)qC[v]; sBC[v] = (fC[v] - fB[v]) * iBC; }
-#pragma unroll
-        for (int v = 0; v < 4; ++v) phiB[v] = limiter_mx(lim, sBC[v], sAB[v]);
-      }
-      // interior j-face (r | r+1) — Mod_Fluxes.f90:187-232
-      if (own && r >= 1 && r <= nj - 1) {
-        float L[4], R[4];
-#pragma unroll
-        for (int v = 0; v < 4; ++v) { L[v] = fA[v] + phiA[v] * hyA; R[v] = fB[v] - phiB[v] * hyB; }
-        if (!(L[0] > 0.f && L[3] > 0.f && R[0] > 0.f && R[3] > 0.f))
-          record_fail(&K, Q2D_E_INTERFACE, fail_key(stage_id, 0, B.blk_in_case, r, x));
-        flux_mx<RIE>(gf, L, R, gj.x, gj.y, gj.z, Gcur);
-      } else {
-#pragma unroll
-        for (int v = 0; v < 4; ++v) Gcur[v] = 0.f;
+        jface(r, qC, hyC, gj);
+        roll(qC, hyC);
       }
 
-      if (r >= item.y0) {
+      for (int r = item.y0; r <= item.y1; ++r) {
+        // Loads of this iteration, in the order they are consumed (vector-memory waits are in issue
+        // order): first row r's own data (i-face geometry, RK base state, local dt, the fused dt's
+        // directions; first used after the j-face flux), then row r+3 and row r+1's j-face geometry for the
+        // next iteration.  Row r+2 and row r's j-face geometry and 1/sur arrived (issued one iteration ago).
+        double qC[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) qC[v] = qN[v];
+        const float hyC = hyN;
+        const f4v gj = gjN;
+        const long long oc = at(r);
+        f4v gd = {0.f, 0.f, 0.f, 0.f};
+        double po[4] = {0.0, 0.0, 0.0, 0.0};
+        double dtl = 0.0;
+        const f4v gi = gip[oc];
+        if (irk != 1) {
+#if Q2D_MX_NT_LOAD
+#pragma unroll
+          for (int v = 0; v < 4; ++v) po[v] = __builtin_nontemporal_load(&PO[v * pl + oc]);
+#else
+#pragma unroll
+          for (int v = 0; v < 4; ++v) po[v] = PO[v * pl + oc];
+#endif
+        }
+        if (!tacc) dtl = dtlp[oc];
+        if (FUSE && last) gd = gdp[oc];
+        {
+          const long long oN = at(min(r + 3, rmax));
+#pragma unroll
+          for (int v = 0; v < 4; ++v) qN[v] = Pin[v * pl + oN];
+          hyN = hyp[oN];
+          gjN = gjp[at(min(r + 1, rmax))];
+        }
+        jface(r, qC, hyC, gj);
+
         // ---- i-direction on row r (DPP shifts between lanes) ------------------
         const float hx = gi.w, hxn = shdn(hx);
         float fn[4], sx[4], phx[4];
@@ -1643,138 +1696,148 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MX_MIN_WAVES) void k_stage_mx(const 
 #pragma unroll
         for (int v = 0; v < 4; ++v) sx[v] = (fn[v] - fA[v]) * ix;
 #pragma unroll
-        for (int v = 0; v < 4; ++v) phx[v] = limiter_mx(lim, sx[v], shup(sx[v]));
-        float phxn[4];                                 // lane shifts with every lane active
+        for (int v = 0; v < 4; ++v) phx[v] = limiter_mxt<LIM>(sx[v], shup(sx[v]));
+        float phxn[4];                                   // lane shifts with every lane active
 #pragma unroll
         for (int v = 0; v < 4; ++v) phxn[v] = shdn(phx[v]);
         float Fx[4];
-        if (xdo) {                                     // interior i-face (x|x+1) — Mod_Fluxes.f90:140-185
-          float L[4], R[4];
+        {                                                // interior i-face (x|x+1) — Mod_Fluxes.f90:140-185
+          float L[4], R[4], F[4];
 #pragma unroll
           for (int v = 0; v < 4; ++v) { L[v] = fA[v] + phx[v] * hx; R[v] = fn[v] - phxn[v] * hxn; }
-          if (!(L[0] > 0.f && L[3] > 0.f && R[0] > 0.f && R[3] > 0.f))
-            record_fail(&K, Q2D_E_INTERFACE, fail_key(stage_id, 0, B.blk_in_case, r, x));
-          flux_mx<RIE>(gf, L, R, gi.x, gi.y, gi.z, Fx);
-        } else {
+          const bool bad = xdo && !(L[0] > 0.f && L[3] > 0.f && R[0] > 0.f && R[3] > 0.f);
+          fail_if_r = bad ? min(fail_if_r, r) : fail_if_r;
+          flux_mx<RIE>(gf, L, R, gi.x, gi.y, gi.z, F);
 #pragma unroll
-          for (int v = 0; v < 4; ++v) Fx[v] = 0.f;
+          for (int v = 0; v < 4; ++v) Fx[v] = xdo ? F[v] : 0.f;
         }
         float Fl[4];
 #pragma unroll
         for (int v = 0; v < 4; ++v) Fl[v] = shup(Fx[v]);
 
-        if (own) {
-          // ---- residual of the cell, faces in the reference's order -----------
-          double res[4];
+        // ---- residual of the cell, faces in the reference's order (every lane; owners store) ----------
+        double res[4];
 #pragma unroll
-          for (int v = 0; v < 4; ++v) {
-            double rv = 0.0;
-            if (x >= 2) rv = rv - (double)Fl[v];
-            if (x <= ni - 1) rv = rv + (double)Fx[v];
-            if (r >= 2) rv = rv - (double)Gprev[v];
-            if (r <= nj - 1) rv = rv + (double)Gcur[v];
-            res[v] = rv;
-          }
-          if (x == 1 || x == ni || r == 1 || r == nj) {  // BC faces 1..4 in record order (Mod_BC_Fluxes.f90:79-86)
+        for (int v = 0; v < 4; ++v) {
+          double rv = 0.0;
+          if (x >= 2) rv = rv - (double)Fl[v];
+          if (x <= ni - 1) rv = rv + (double)Fx[v];
+          if (r >= 2) rv = rv - (double)Gprev[v];
+          if (r <= nj - 1) rv = rv + (double)Gcur[v];
+          res[v] = rv;
+        }
+        if (own && (x == 1 || x == ni || r == 1 || r == nj)) {  // BC faces 1..4 in record order (Mod_BC_Fluxes.f90:79-86)
 #pragma unroll 1
-            for (int f = 0; f < 4; ++f) {
-              int rec = -1;
-              if (f == 0 && x == 1 && B.bcf[0] >= 0) rec = bcface[B.bcf[0] + r - 1];
-              if (f == 1 && x == ni && B.bcf[1] >= 0) rec = bcface[B.bcf[1] + r - 1];
-              if (f == 2 && r == 1) rec = rec3;
-              if (f == 3 && r == nj) rec = rec4;
-              if (rec < 0) continue;
-              const BCFlux& q = bcflux[rec];
+          for (int f = 0; f < 4; ++f) {
+            int rec = -1;
+            if (f == 0 && x == 1 && B.bcf[0] >= 0) rec = bcface[B.bcf[0] + r - 1];
+            if (f == 1 && x == ni && B.bcf[1] >= 0) rec = bcface[B.bcf[1] + r - 1];
+            if (f == 2 && r == 1) rec = rec3;
+            if (f == 3 && r == nj) rec = rec4;
+            if (rec < 0) continue;
+            const BCFlux& q = bcflux[rec];
 #pragma unroll
-              for (int v = 0; v < 4; ++v)
-                if ((q.mask >> v) & 1) res[v] = res[v] + q.F[v];
-            }
-          }
-          // geometric sources (Lib_Source_Axisymmetry.f90:40-78, Lib_Source_Areavariation.f90:104-144),
-          // per unit area: Res/sur - S enters the update
-          double S[4] = {0.0, 0.0, 0.0, 0.0};
-          if (C.source != Q2D_SRC_NONE) {
-            const double rho = qA[0], u = qA[1], v = qA[2], p = qA[3];
-            const double rhoE0 = p * rgm1 + 0.5 * rho * (u * u + v * v);
-            if (C.source == Q2D_SRC_AXI) {
-              const double invr = gp(B.sb)[oc];
-              S[0] = -(rho * v * invr);
-              S[1] = -(rho * u * v * invr);
-              S[2] = -(rho * v * v * invr);
-              S[3] = -((rhoE0 + p) * v * invr);
-            } else {
-              const double VdG = u * gp(B.sa)[oc] + v * gp(B.sb)[oc];
-              S[0] = rho * VdG;
-              S[1] = rho * u * VdG;
-              S[2] = rho * v * VdG;
-              S[3] = (rhoE0 + p) * VdG;
-            }
-          }
-          // ---- FP64 update: RK_stage (Lib_RK.f90:15-27) on the conservative state ----
-          if (irk == 1) {
-#pragma unroll
-            for (int v = 0; v < 4; ++v) po[v] = qA[v];
-          }
-          const double dt = tacc ? dtg : dtl;
-          const double dts = dt * (double)gj.w;
-          double Uo[4], U[4], qn2[4];
-          Uo[0] = po[0]; Uo[1] = po[0] * po[1]; Uo[2] = po[0] * po[2];
-          Uo[3] = __builtin_fma(po[3], rgm1, 0.5 * po[0] * (po[1] * po[1] + po[2] * po[2]));
-          U[0] = qA[0]; U[1] = qA[0] * qA[1]; U[2] = qA[0] * qA[2];
-          U[3] = __builtin_fma(qA[3], rgm1, 0.5 * qA[0] * (qA[1] * qA[1] + qA[2] * qA[2]));
-#pragma unroll
-          for (int v = 0; v < 4; ++v) {
-            const double inc = __builtin_fma(-res[v], dts, S[v] * dt);
-            U[v] = __builtin_fma(U[v] - Uo[v] + inc, c_rk, Uo[v]);
-          }
-          {
-            const double ir = rcp_nr(U[0]);
-            const double vx = U[1] * ir, vy = U[2] * ir;
-            qn2[0] = U[0]; qn2[1] = vx; qn2[2] = vy;
-            qn2[3] = gm1 * __builtin_fma(-0.5, U[1] * vx + U[2] * vy, U[3]);
-          }
-          // Check_And_Fix_State — Lib_Newstate.f90:316-350
-          if (__builtin_isnan(qn2[0]) || __builtin_isnan(qn2[1]) || __builtin_isnan(qn2[2]) ||
-              __builtin_isnan(qn2[3]) || qn2[3] < 0.0)
-            record_fail(&K, Q2D_E_STATE, fail_key(stage_id, 1, B.blk_in_case, r, x));
-          if (qn2[0] < 1e-20) qn2[0] = 1e-20;
-#pragma unroll
-          for (int v = 0; v < 4; ++v) Pout[v * pl + oc] = qn2[v];
-          if (last) {                                  // Compute_Residual — Mod_Diagnostic.f90:452-458
-#pragma unroll
-            for (int v = 0; v < 4; ++v) { const double d = qn2[v] - po[v]; acc[v] = __builtin_fma(d, d, acc[v]); }
-          }
-          if (last && FUSE) {
-            // next step's Compute_dt on the state just written (Mod_dt.f90:59-104), FP64
-            const double dtmin = cell_dt_mx(C, kiter + 1, qn2[0], qn2[1], qn2[2], qn2[3], hx, hyA, gd.x, gd.y,
-                                            gd.z, gd.w);
-            if (!tacc) dtl_next[oc] = dtmin;
-            if (dtmin < dtnext) dtnext = dtmin;
+            for (int v = 0; v < 4; ++v)
+              if ((q.mask >> v) & 1) res[v] = res[v] + q.F[v];
           }
         }
-      }
-      // ---- roll the j-stencil -------------------------------------------------
+        // geometric sources (Lib_Source_Axisymmetry.f90:40-78, Lib_Source_Areavariation.f90:104-144),
+        // per unit area: Res/sur - S enters the update
+        double S[4] = {0.0, 0.0, 0.0, 0.0};
+        if (C.source != Q2D_SRC_NONE) {
+          const double rho = qA[0], u = qA[1], v = qA[2], p = qA[3];
+          const double rhoE0 = p * rgm1 + 0.5 * rho * (u * u + v * v);
+          if (C.source == Q2D_SRC_AXI) {
+            const double invr = gp(B.sb)[oc];
+            S[0] = -(rho * v * invr);
+            S[1] = -(rho * u * v * invr);
+            S[2] = -(rho * v * v * invr);
+            S[3] = -((rhoE0 + p) * v * invr);
+          } else {
+            const double VdG = u * gp(B.sa)[oc] + v * gp(B.sb)[oc];
+            S[0] = rho * VdG;
+            S[1] = rho * u * VdG;
+            S[2] = rho * v * VdG;
+            S[3] = (rhoE0 + p) * VdG;
+          }
+        }
+        // ---- FP64 update: RK_stage (Lib_RK.f90:15-27) on the conservative state ----
+        if (irk == 1) {
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        qA[v] = qB[v]; qB[v] = qC[v]; fA[v] = fB[v]; fB[v] = fC[v];
-        sAB[v] = sBC[v]; phiA[v] = phiB[v]; Gprev[v] = Gcur[v];
-      }
-      hyA = hyB; hyB = hyC;
-    }
-    if (last) {                                        // fixed-pattern wavefront sum (deterministic)
+          for (int v = 0; v < 4; ++v) po[v] = qA[v];
+        }
+        const double dt = tacc ? dtg : dtl;
+        const double dts = dt * (double)gj.w;
+        double Uo[4], U[4], qn2[4];
+        Uo[0] = po[0]; Uo[1] = po[0] * po[1]; Uo[2] = po[0] * po[2];
+        Uo[3] = __builtin_fma(po[3], rgm1, 0.5 * po[0] * (po[1] * po[1] + po[2] * po[2]));
+        U[0] = qA[0]; U[1] = qA[0] * qA[1]; U[2] = qA[0] * qA[2];
+        U[3] = __builtin_fma(qA[3], rgm1, 0.5 * qA[0] * (qA[1] * qA[1] + qA[2] * qA[2]));
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        double s = acc[v];
-        for (int k = 32; k >= 1; k >>= 1) s = s + __shfl_xor(s, k, kWave);
-        if (lane == 0) partials[(long long)it * kPartials + v] = s;
+        for (int v = 0; v < 4; ++v) {
+          const double inc = __builtin_fma(-res[v], dts, S[v] * dt);
+          U[v] = __builtin_fma(U[v] - Uo[v] + inc, c_rk, Uo[v]);
+        }
+        {
+          const double ir = rcp_nr(U[0]);
+          const double vx = U[1] * ir, vy = U[2] * ir;
+          qn2[0] = U[0]; qn2[1] = vx; qn2[2] = vy;
+          qn2[3] = gm1 * __builtin_fma(-0.5, U[1] * vx + U[2] * vy, U[3]);
+        }
+        // Check_And_Fix_State — Lib_Newstate.f90:316-350
+        const bool bad_state = own && (__builtin_isnan(qn2[0]) || __builtin_isnan(qn2[1]) || __builtin_isnan(qn2[2]) ||
+                                       __builtin_isnan(qn2[3]) || qn2[3] < 0.0);
+        fail_st_r = bad_state ? min(fail_st_r, r) : fail_st_r;
+        if (qn2[0] < 1e-20) qn2[0] = 1e-20;
+        if (last) {                                    // Compute_Residual — Mod_Diagnostic.f90:452-458
+#pragma unroll
+          for (int v = 0; v < 4; ++v) { const double d = own ? qn2[v] - po[v] : 0.0; acc[v] = __builtin_fma(d, d, acc[v]); }
+        }
+        double dtmin = 0.0;
+        if (last && FUSE) {
+          // next step's Compute_dt on the state just written (Mod_dt.f90:59-104), FP64
+          dtmin = cell_dt_mx(C, kiter + 1, qn2[0], qn2[1], qn2[2], qn2[3], hx, hyA, gd.x, gd.y, gd.z, gd.w);
+          dtnext = (own && dtmin < dtnext) ? dtmin : dtnext;
+        }
+        if (own) {
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+#if Q2D_MX_NT_STORE
+            __builtin_nontemporal_store(qn2[v], &Pout[v * pl + oc]);
+#else
+            Pout[v * pl + oc] = qn2[v];
+#endif
+          }
+          if (last && FUSE && !tacc) dtl_next[oc] = dtmin;
+        }
+        roll(qC, hyC);
       }
-    }
-    if (last && FUSE) {                                // next step's dt: the item's exact min (see k_stage)
-      for (int k = 32; k >= 1; k >>= 1) {
-        const double o = __shfl_xor(dtnext, k, kWave);
-        if (o < dtnext) dtnext = o;
+      if (fail_if_r != kNoRow) record_fail(&K, Q2D_E_INTERFACE, fail_key(stage_id, 0, B.blk_in_case, fail_if_r, x));
+      if (fail_st_r != kNoRow) record_fail(&K, Q2D_E_STATE, fail_key(stage_id, 1, B.blk_in_case, fail_st_r, x));
+      if (last) {                                        // fixed-pattern wavefront sum (deterministic)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          double s = acc[v];
+          for (int k = 32; k >= 1; k >>= 1) s = s + __shfl_xor(s, k, kWave);
+          if (lane == 0) partials[(long long)it * kPartials + v] = s;
+        }
       }
-      if (lane == 0) partials[(long long)it * kPartials + 4] = dtnext;
+      if (last && FUSE) {                                // next step's dt: the item's exact min (see k_stage)
+        for (int k = 32; k >= 1; k >>= 1) {
+          const double o = __shfl_xor(dtnext, k, kWave);
+          if (o < dtnext) dtnext = o;
+        }
+        if (lane == 0) partials[(long long)it * kPartials + 4] = dtnext;
+      }
+    };
+    switch (C.limiter) {
+      case 1: item_body(std::integral_constant<int, 1>{}); break;
+      case 2: item_body(std::integral_constant<int, 2>{}); break;
+      case 3: item_body(std::integral_constant<int, 3>{}); break;
+      case 4: item_body(std::integral_constant<int, 4>{}); break;
+      case 5: item_body(std::integral_constant<int, 5>{}); break;
+      case 6: item_body(std::integral_constant<int, 6>{}); break;
+      default: item_body(std::integral_constant<int, 0>{}); break;
     }
   }
   if (tslotp && lane == 0) atomicMax(&tslotp[1], (unsigned long long)wall_clock64());
@@ -1788,30 +1851,55 @@ __global__ void k_tstamp_init(unsigned long long* tstamp, int nsub) {
 }
 
 // Per-case residual reduction and end-of-step control — Mod_Explicit.f90:120-168.
-__global__ __launch_bounds__(256) void k_finish(DevCtrl* ctrl, const DevCase* cases, const double* partials, int ncases,
-                                                unsigned long long* tstamp, int fused_dt) {
+// One workgroup of kFinishThreads per case.  Thread t sums items t, t + kFinishThreads, ... in that order
+// (loads of kFinishUnroll items issued together), then a fixed xor-shuffle tree per wavefront and a
+// fixed tree over the wavefronts: the same order on every run (bit-stable).
+constexpr int kFinishThreads = 1024, kFinishUnroll = 4;
+__global__ __launch_bounds__(kFinishThreads) void k_finish(DevCtrl* ctrl, const DevCase* cases, const double* partials,
+                                                           int ncases, unsigned long long* tstamp, int fused_dt) {
   const int c = blockIdx.x;
   if (tstamp && c == 0 && threadIdx.x == 0) tstamp[0] = tstamp[0] + 1ull;   // end of a timed step
   if (c >= ncases) return;
   DevCtrl& K = ctrl[c];
   const DevCase& C = cases[c];
   if (!K.active) return;
-  __shared__ double sh[5][256];
+  __shared__ double sh[5][kFinishThreads / kWave];
   double s[4] = {0, 0, 0, 0};
   double dmin = 1e300;
-  for (int it = C.item_begin + threadIdx.x; it < C.item_end; it += blockDim.x) {
-    for (int v = 0; v < 4; ++v) s[v] = s[v] + partials[(long long)it * kPartials + v];
-    if (fused_dt) dmin = fmin(dmin, partials[(long long)it * kPartials + 4]);
-  }
-  for (int v = 0; v < 4; ++v) sh[v][threadIdx.x] = s[v];
-  sh[4][threadIdx.x] = dmin;
-  __syncthreads();
-  for (int w = blockDim.x / 2; w >= 1; w >>= 1) {
-    if ((int)threadIdx.x < w) {
-      for (int v = 0; v < 4; ++v) sh[v][threadIdx.x] = sh[v][threadIdx.x] + sh[v][threadIdx.x + w];
-      sh[4][threadIdx.x] = fmin(sh[4][threadIdx.x], sh[4][threadIdx.x + w]);
+  for (int it0 = C.item_begin + threadIdx.x; it0 < C.item_end; it0 += kFinishUnroll * kFinishThreads) {
+    double a[kFinishUnroll][5];
+#pragma unroll
+    for (int u = 0; u < kFinishUnroll; ++u) {
+      const int it = it0 + u * kFinishThreads;
+      const bool in = it < C.item_end;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) a[u][v] = in ? partials[(long long)it * kPartials + v] : 0.0;
+      a[u][4] = (in && fused_dt) ? partials[(long long)it * kPartials + 4] : 1e300;
     }
-    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < kFinishUnroll; ++u) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) s[v] = s[v] + a[u][v];
+      dmin = fmin(dmin, a[u][4]);
+    }
+  }
+  const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+#pragma unroll
+  for (int v = 0; v < 4; ++v)
+    for (int k = 32; k >= 1; k >>= 1) s[v] = s[v] + __shfl_xor(s[v], k, kWave);
+  for (int k = 32; k >= 1; k >>= 1) dmin = fmin(dmin, __shfl_xor(dmin, k, kWave));
+  if (lane == 0) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) sh[v][wv] = s[v];
+    sh[4][wv] = dmin;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kFinishThreads / kWave; ++w) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) sh[v][0] = sh[v][0] + sh[v][w];
+      sh[4][0] = fmin(sh[4][0], sh[4][w]);
+    }
   }
   if (threadIdx.x == 0) {
     // next step's dt from the fused last stage (Set_Global_dt's min, Mod_dt.f90:109-128)
@@ -1890,7 +1978,8 @@ struct q2d_handle {
   bool prec_set = false;
   size_t arena_bytes = 0;
   hipStream_t stream = nullptr;
-  hipGraphExec_t graph = nullptr;
+  hipGraphExec_t graph = nullptr;        // one step
+  hipGraphExec_t graph_multi = nullptr;  // kGraphSteps steps back to back
   // rows per item: > 0 fixed, -1 balanced per launch, 0 the form's default (FP64: balanced; mixed: 16)
   int rows_per_item = 0, waves_per_group = 4, max_groups = 0, xcd_remap = 1;
   int rows_used[4] = {0, 0, 0, 0};   // rows per item chosen for each stage-kernel variant
@@ -1920,6 +2009,12 @@ const void* stage_kernel(const q2d_handle* h, int pass) {
     case 2: return reinterpret_cast<const void*>(&k_stage<double, 0, kFuseDtFp64, true>);
     default: return reinterpret_cast<const void*>(&k_stage<double, 1, kFuseDtFp64, true>);
   }
+}
+
+// The captured step graphs hold the launch arguments; they are dropped whenever those change.
+void drop_graphs(q2d_handle* h) {
+  if (h->graph) { (void)hipGraphExecDestroy(h->graph); h->graph = nullptr; }
+  if (h->graph_multi) { (void)hipGraphExecDestroy(h->graph_multi); h->graph_multi = nullptr; }
 }
 
 // Rows per work item for one stage launch, chosen so that the launch finishes soonest on the waves the
@@ -2042,7 +2137,7 @@ int build_items(q2d_handle* h) {
   HIPCHK(hipMalloc(&h->d_cases, sizeof(DevCase) * std::max<size_t>(1, h->dc.size())));
   if (!h->dc.empty())
     HIPCHK(hipMemcpy(h->d_cases, h->dc.data(), sizeof(DevCase) * h->dc.size(), hipMemcpyHostToDevice));
-  if (h->graph) { (void)hipGraphExecDestroy(h->graph); h->graph = nullptr; }
+  drop_graphs(h);
   return Q2D_OK;
 }
 
@@ -2135,7 +2230,10 @@ int launch_step(q2d_handle* h) {
   const int bgroups = (h->nbc + 63) / 64;            // one wavefront per group: spread over every CU
   const int use_next =
       (h->dt_fused_valid && !h->any_visc && (h->precision_mixed ? kFuseDtMx : kFuseDtFp64)) ? 1 : 0;
-  hipLaunchKernelGGL(k_begin, dim3(cgroups), dim3(cthreads), 0, h->stream, h->d_ctrl, h->d_cases, h->ncases, use_next);
+  // with the dt already reduced, the first boundary launch also does k_begin's work (one launch fewer)
+  const bool fold_begin = use_next && h->nbc > 0;
+  if (!fold_begin)
+    hipLaunchKernelGGL(k_begin, dim3(cgroups), dim3(cthreads), 0, h->stream, h->d_ctrl, h->d_cases, h->ncases, use_next);
   if (!use_next)
     if (h->dt_total > 0) {                             // one wavefront per unit of kDtCells cells
       const int dg = (h->dt_total + 3) / 4;
@@ -2147,12 +2245,13 @@ int launch_step(q2d_handle* h) {
   for (int irk = 1; irk <= h->max_rk; ++irk) {
     if (h->nbc > 0) {
       for (int ph = h->bc_two_phase ? 1 : 0; ph <= (h->bc_two_phase ? 2 : 0); ++ph) {
+        const int begin_cases = (fold_begin && irk == 1 && ph <= 1) ? h->ncases : 0;
         if (h->precision_mixed)
           hipLaunchKernelGGL(k_bcflux<float>, dim3(bgroups), dim3(64), 0, h->stream, h->d_blocks, h->d_cases,
-                             h->d_ctrl, h->d_bc, h->nbc, irk, h->d_bcflux, ph);
+                             h->d_ctrl, h->d_bc, h->nbc, irk, h->d_bcflux, ph, begin_cases);
         else
           hipLaunchKernelGGL(k_bcflux<double>, dim3(bgroups), dim3(64), 0, h->stream, h->d_blocks, h->d_cases,
-                             h->d_ctrl, h->d_bc, h->nbc, irk, h->d_bcflux, ph);
+                             h->d_ctrl, h->d_bc, h->nbc, irk, h->d_bcflux, ph, begin_cases);
       }
     }
     if (!h->vrec.empty())
@@ -2168,7 +2267,7 @@ int launch_step(q2d_handle* h) {
     if (rc != Q2D_OK) return rc;
   }
   const int fused_dt = (!h->any_visc && (h->precision_mixed ? kFuseDtMx : kFuseDtFp64)) ? 1 : 0;
-  hipLaunchKernelGGL(k_finish, dim3(h->ncases), dim3(256), 0, h->stream, h->d_ctrl, h->d_cases, h->d_partials, h->ncases,
+  hipLaunchKernelGGL(k_finish, dim3(h->ncases), dim3(kFinishThreads), 0, h->stream, h->d_ctrl, h->d_cases, h->d_partials, h->ncases,
                      h->timing ? h->d_tstamp : nullptr, fused_dt);
   HIPCHK(hipGetLastError());
   h->dt_fused_valid = true;
@@ -2183,7 +2282,10 @@ int read_out(q2d_handle* h, q2d_step_out* out, int* first_err) {
   HIPCHK(hipStreamSynchronize(h->stream));
   *first_err = Q2D_OK;
   for (int c = 0; c < h->ncases; ++c) {
-    const DevCtrl& k = ctl[c];
+    DevCtrl& k = ctl[c];
+    // The reported failure is the first in the reference's order (stage, then fluxes before the new
+    // state, then block, row, column): the minimum key; its kind bit names the stop it maps to.
+    if (k.status != 0) k.status = ((k.fail_key >> 41) & 1ull) ? Q2D_E_STATE : Q2D_E_INTERFACE;
     if (h->case_live[c] && k.status != 0 && *first_err == Q2D_OK) *first_err = k.status;
     if (!out) continue;
     q2d_step_out& o = out[c];
@@ -2231,7 +2333,7 @@ int q2d_device_count(int* count) {
 
 int q2d_gpu_destroy(q2d_handle* h) {
   if (!h) return Q2D_OK;
-  if (h->graph) (void)hipGraphExecDestroy(h->graph);
+  drop_graphs(h);
   for (auto& b : h->hb) if (b.arena) (void)hipFree(b.arena);
   (void)hipFree(h->d_blocks); (void)hipFree(h->d_cases); (void)hipFree(h->d_ctrl); (void)hipFree(h->d_bc);
   (void)hipFree(h->d_bcface); (void)hipFree(h->d_items); (void)hipFree(h->d_partials); (void)hipFree(h->d_bcflux);
@@ -2387,7 +2489,7 @@ int append_cases(q2d_handle* h, const q2d_problem* prob, int32_t* first_case) {
       return fail(Q2D_E_ARG, "laminar block connection needs d11 = +1 or -1 (the donor's orientation along the face)");
     DevBC& r = newbc[k];
     memset(&r, 0, sizeof(r));
-    r.type = e.type; r.blk = block0 + lb; r.i = e.i; r.j = e.j; r.f = e.f;
+    r.type = e.type; r.blk = block0 + lb; r.i = e.i; r.j = e.j; r.f = e.f; r.cid = case0 + c;
     if (e.type == 101 || e.type == 201) {
       if (e.bs < 1 || e.bs > nblk[c] || e.fs < 1 || e.fs > 4) return fail(Q2D_E_ARG, "bc donor out of range");
       const q2d_block_desc& sd = prob->blocks[first_block[c] + e.bs - 1];
@@ -2672,16 +2774,33 @@ int q2d_gpu_run(q2d_handle* h, int32_t nsteps, q2d_step_out* out) {
       if (rc != Q2D_OK) return rc;
       --nsteps;
     }
-    if (!h->graph && nsteps > 0) {
+    // Steps replay captured graphs: kGraphSteps steps per launch while that many remain (the replays of
+    // one graph run back to back; each graph launch costs ~8 us of idle device time), then single steps.
+    // Every kernel checks its case's active flag, so steps after a case stops are no-ops for it.
+    auto capture = [h](int nst, hipGraphExec_t* out) -> int {
       hipGraph_t g;
       HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-      int rc = launch_step(h);
+      int rc = Q2D_OK;
+      for (int k = 0; k < nst && rc == Q2D_OK; ++k) rc = launch_step(h);
       hipError_t ce = hipStreamEndCapture(h->stream, &g);
-      if (rc != Q2D_OK) return rc;
+      if (rc != Q2D_OK) { if (ce == hipSuccess) (void)hipGraphDestroy(g); return rc; }
       if (ce != hipSuccess) return fail(Q2D_E_HIP, std::string("graph capture: ") + hipGetErrorString(ce));
-      HIPCHK(hipGraphInstantiate(&h->graph, g, nullptr, nullptr, 0));
+      hipError_t ie = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
       (void)hipGraphDestroy(g);
+      if (ie != hipSuccess) return fail(Q2D_E_HIP, std::string("graph instantiate: ") + hipGetErrorString(ie));
+      return Q2D_OK;
+    };
+    while (nsteps >= kGraphSteps) {
+      if (!h->graph_multi) { int rc = capture(kGraphSteps, &h->graph_multi); if (rc != Q2D_OK) return rc; }
+      if (h->timing && h->timed_steps_pending + kGraphSteps > kTimedSteps) {
+        int rc = collect_timing(h);                   // the device's timing ring holds kTimedSteps steps
+        if (rc != Q2D_OK) return rc;
+      }
+      HIPCHK(hipGraphLaunch(h->graph_multi, h->stream));
+      if (h->timing) h->timed_steps_pending += kGraphSteps;
+      nsteps -= kGraphSteps;
     }
+    if (!h->graph && nsteps > 0) { int rc = capture(1, &h->graph); if (rc != Q2D_OK) return rc; }
     for (int s = 0; s < nsteps; ++s) {
       HIPCHK(hipGraphLaunch(h->graph, h->stream));
       int rc = count_timed_step(h);
@@ -2747,7 +2866,7 @@ int q2d_gpu_download_dt(q2d_handle* h, int32_t block, double* dtlocal) {
 int q2d_gpu_set_timing(q2d_handle* h, int32_t enable) {
   if (!h) return fail(Q2D_E_ARG, "NULL handle");
   const bool on = enable != 0;
-  if (on != h->timing && h->graph) { (void)hipGraphExecDestroy(h->graph); h->graph = nullptr; }   // launch args change
+  if (on != h->timing) drop_graphs(h);             // launch args change
   h->timing = on;
   h->timed_launches = 0; h->timed_ms = 0.0;
   return h->timing ? reset_timing(h) : Q2D_OK;

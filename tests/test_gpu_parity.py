@@ -87,6 +87,50 @@ def test_step_equals_graph_run(oracle_mod):
     assert np.array_equal(g1.state(0), g2.state(0))
 
 
+def test_multi_step_graph_stops_at_itermax(oracle_mod):
+    """A case whose itermax falls inside a multi-step graph replay stops there, like the oracle."""
+    pb = cases.riemann_planar(seed=5, ni=150, nj=10, steps=21)
+    g = GpuSolver(pb)
+    out = g.run(30)
+    assert out[0]["iter"] == 21 and out[0]["todo"] == 3
+    o = oracle_mod.Oracle(pb)
+    rc, oo = o.run(21)
+    assert rc == 0 and oo[0]["iter"] == 21
+    assert np.array_equal(g.state(0)[2:-2, 2:-2], o.state(0)[2:-2, 2:-2])
+    assert out[0]["dtglobal"] == oo[0]["dtglobal"] and out[0]["time"] == oo[0]["time"]
+
+
+def two_rarefaction(U, p, cfl, limiter, precision):
+    """64x8 tube, rho = 1, p small, u = -U left of the diaphragm and +U right of it: a vacuum opens."""
+    pb = cases.riemann_planar(seed=3, ni=64, nj=8, steps=50, limiter=limiter, cfl=cfl, precision=precision)
+    P = pb.blocks[0].P
+    x = np.arange(P.shape[1])
+    P[:, :, 1] = np.where(x[None, :] < P.shape[1] // 2, -U, U)
+    P[:, :, 0] = 1.0
+    P[:, :, 3] = p
+    return pb
+
+
+@pytest.mark.parametrize("U,p,cfl,limiter", [(5.0, 1e-3, 0.9, "superbee"), (2.0, 0.05, 1.5, "minmod")])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_failure_reported_like_oracle(oracle_mod, U, p, cfl, limiter, precision):
+    """Negative pressure (Check_And_Fix_State's stop, Lib_Newstate.f90:327-331): the device reports it in
+    the step and cell where the oracle stops (FP64: same status and cell; mixed: same step, a failure),
+    and the case stops advancing."""
+    o = oracle_mod.Oracle(two_rarefaction(U, p, cfl, limiter, "fp64"))
+    rc, oo = o.run(50)
+    assert rc == -4 and oo[0]["status"] == -4
+    g = GpuSolver(two_rarefaction(U, p, cfl, limiter, precision))
+    out = g.run(50, raise_on_error=False)
+    assert out[0]["iter"] == oo[0]["iter"]
+    if precision == "fp64":
+        assert out[0]["status"] == oo[0]["status"] and out[0]["fail"] == oo[0]["fail"]
+    else:
+        assert out[0]["status"] in (-4, -5)
+    again = g.run(5, raise_on_error=False)           # a failed case is frozen
+    assert again[0]["iter"] == out[0]["iter"]
+
+
 def test_deterministic_residual(oracle_mod):
     pb = cases.planar_shock(seed=4, n=64, steps=10)
     r1 = GpuSolver(pb).run(10)[0]["residual"]
