@@ -1500,15 +1500,23 @@ typedef const __attribute__((address_space(1))) f4v* GP4;
 #ifndef Q2D_STAGE_MX_MIN_WAVES
 #define Q2D_STAGE_MX_MIN_WAVES 1
 #endif
-// Streaming (non-temporal) access for the data each mixed stage touches once: the new state it writes
-// and the RK base state it reads; the L2 keeps the state rows that neighbouring items re-read (their
-// halos).  Measured on 4 x 1024^2, SSP-RK3: 0.516 -> 0.508 ms per step.
+// Streaming (non-temporal) access for the data each mixed stage touches once: the new state it writes,
+// the RK base state and the i-face, j-face and dt geometry it reads; the L2 keeps the state rows that
+// neighbouring items re-read (their halos).  Measured on 4 x 1024^2, SSP-RK3: 0.516 -> 0.508 ms per
+// step (state), -> 0.495 (geometry).
 #ifndef Q2D_MX_NT_STORE
 #define Q2D_MX_NT_STORE 1
 #endif
 #ifndef Q2D_MX_NT_LOAD
 #define Q2D_MX_NT_LOAD 1
 #endif
+__device__ __forceinline__ f4v mx_stream_load(GP4 p) {
+#if Q2D_MX_NT_LOAD
+  return __builtin_nontemporal_load(p);
+#else
+  return *p;
+#endif
+}
 template <int RIE, bool FUSE>
 __global__ __launch_bounds__(256, Q2D_STAGE_MX_MIN_WAVES) void k_stage_mx(const DevBlock* __restrict__ blocks, const DevCase* __restrict__ cases,
                                                   DevCtrl* __restrict__ ctrl, const Item* __restrict__ items, int item_lo,
@@ -1647,7 +1655,7 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MX_MIN_WAVES) void k_stage_mx(const 
 #pragma unroll
         for (int v = 0; v < 4; ++v) qN[v] = Pin[v * pl + oN];
         hyN = hyp[oN];
-        gjN = gjp[at(min(r + 1, rmax))];
+        gjN = mx_stream_load(&gjp[at(min(r + 1, rmax))]);
         jface(r, qC, hyC, gj);
         roll(qC, hyC);
       }
@@ -1666,7 +1674,7 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MX_MIN_WAVES) void k_stage_mx(const 
         f4v gd = {0.f, 0.f, 0.f, 0.f};
         double po[4] = {0.0, 0.0, 0.0, 0.0};
         double dtl = 0.0;
-        const f4v gi = gip[oc];
+        const f4v gi = mx_stream_load(&gip[oc]);
         if (irk != 1) {
 #if Q2D_MX_NT_LOAD
 #pragma unroll
@@ -1677,13 +1685,13 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MX_MIN_WAVES) void k_stage_mx(const 
 #endif
         }
         if (!tacc) dtl = dtlp[oc];
-        if (FUSE && last) gd = gdp[oc];
+        if (FUSE && last) gd = mx_stream_load(&gdp[oc]);
         {
           const long long oN = at(min(r + 3, rmax));
 #pragma unroll
           for (int v = 0; v < 4; ++v) qN[v] = Pin[v * pl + oN];
           hyN = hyp[oN];
-          gjN = gjp[at(min(r + 1, rmax))];
+          gjN = mx_stream_load(&gjp[at(min(r + 1, rmax))]);
         }
         jface(r, qC, hyC, gj);
 
