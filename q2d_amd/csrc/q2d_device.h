@@ -239,6 +239,58 @@ __device__ __forceinline__ float limiter_mxt(float x, float y) {
   }
 }
 
+// The same fluxes from the face's length-scaled normal (nx, ny) = int_l * n: the flux through the face
+// times its length is homogeneous of degree one in the normal once the sound speeds are scaled by
+// int_l = |(nx, ny)| and the HLLC star pressure is formed from unit-normal speeds (the 1/int_l^2
+// factor).  Used by the mixed form, whose geometry stores only the scaled normal (8 B per face).
+template <int RIE>
+__device__ __forceinline__ void flux_mxs(const GasF& g, const float L[4], const float R[4], float nx, float ny,
+                                         float F[4]) {
+#pragma clang fp contract(fast)
+  const float l2 = nx * nx + ny * ny;
+  const float len = fsqrt(l2);                            // int_l
+  const float rsL = frsq(L[0]), rsR = frsq(R[0]);         // 1 / sqrt(rho)
+  const float sL = L[0] * rsL, sR = R[0] * rsR;           // sqrt(rho)
+  const float irL = rsL * rsL, irR = rsR * rsR;           // 1 / rho
+  const float aL = fsqrt(g.gam * L[3] * irL) * len, aR = fsqrt(g.gam * R[3] * irR) * len;
+  const float unL = L[1] * nx + L[2] * ny, unR = R[1] * nx + R[2] * ny;
+  const float hL = g.gam_gm1 * L[3] * irL + 0.5f * (L[1] * L[1] + L[2] * L[2]);   // h0 = cp T + |u|^2 / 2
+  const float hR = g.gam_gm1 * R[3] * irR + 0.5f * (R[1] * R[1] + R[2] * R[2]);
+  const float w = frcp(sL + sR);
+  const float ut = (sL * L[1] + sR * R[1]) * w, vt = (sL * L[2] + sR * R[2]) * w;
+  const float ht = (sL * hL + sR * hR) * w;
+  const float at = fsqrt(g.gm1 * (ht - 0.5f * (ut * ut + vt * vt))) * len;
+  const float unt = ut * nx + vt * ny;
+  const float S1 = __builtin_fminf(__builtin_fminf(0.f, unL - aL), unt - at);
+  const float S4 = __builtin_fmaxf(__builtin_fmaxf(0.f, unR + aR), unt + at);
+  const float FrL = L[0] * unL, FrR = R[0] * unR;
+  if (RIE == 1) {
+    const float inv = frcp(S4 - S1), S14 = S1 * S4;
+    F[0] = (S4 * FrL - S1 * FrR + S14 * (R[0] - L[0])) * inv;
+    F[1] = (S4 * (FrL * L[1] + L[3] * nx) - S1 * (FrR * R[1] + R[3] * nx) + S14 * (R[0] * R[1] - L[0] * L[1])) * inv;
+    F[2] = (S4 * (FrL * L[2] + L[3] * ny) - S1 * (FrR * R[2] + R[3] * ny) + S14 * (R[0] * R[2] - L[0] * L[2])) * inv;
+    F[3] = (S4 * FrL * hL - S1 * FrR * hR + S14 * ((R[0] * hR - R[3]) - (L[0] * hL - L[3]))) * inv;
+  } else {
+    const float mL = L[0] * (S1 - unL), mR = R[0] * (S4 - unR);
+    const float Sst = (l2 * (R[3] - L[3]) + mL * unL - mR * unR) * frcp(mL - mR);
+    const float pst = L[0] * (unL - S1) * (unL - Sst) * frcp(l2) + L[3];
+    const bool left = (S1 >= 0.f) || (Sst >= 0.f);
+    const bool star = (S1 < 0.f) && ((Sst >= 0.f) || (S4 > 0.f));
+    const float Sk = left ? S1 : S4, dk = left ? L[0] : R[0], uk = left ? L[1] : R[1], vk = left ? L[2] : R[2];
+    const float pk = left ? L[3] : R[3], unk = left ? unL : unR, hk = left ? hL : hR, irk = left ? irL : irR;
+    const float Fr = dk * unk;
+    const float Fu = Fr * uk + pk * nx, Fv = Fr * vk + pk * ny, FE = Fr * hk;
+    const float id = frcp(Sk - Sst);                      // star-state correction, selected (see flux_mx)
+    const float U1 = dk * (Sk - unk) * id;
+    const float dp = (pst - pk) * id;
+    const float e0k = hk - pk * irk;
+    F[0] = star ? Fr + Sk * (U1 - dk) : Fr;
+    F[1] = star ? Fu + Sk * (U1 * uk + dp * nx - dk * uk) : Fu;
+    F[2] = star ? Fv + Sk * (U1 * vk + dp * ny - dk * vk) : Fv;
+    F[3] = star ? FE + Sk * (U1 * e0k + (pst * Sst - pk * unk) * id - dk * e0k) : FE;
+  }
+}
+
 // HLLC (RIE 0, Batten et al. 1997 as in riemann_HLLC, Lib_Riemann_HLL.f90:34-123) or HLLE (RIE 1,
 // Einfeldt, riemann_HLLE :126-169) flux through a face of unit normal (nx, ny), times its length il,
 // from primitive states L, R = (rho, u, v, p).  Wave speeds S1 = min(0, unL - aL, un~ - a~),

@@ -109,8 +109,11 @@ struct DevBlock {
   const double* nodr;            // node(i,j)%c(2) at cell (i,j), i = 0..ni, j = 0..nj (viscous cases)
   double *vdx, *vey, *vsv;       // 3 planes each: diffusive flux of i-face (x|x+1), of j-face (r|r+1), and
                                  // the axisymmetric viscous source, components rho u, rho v, rho E
-  const float4 *mgi, *mgj, *mgd; // mixed-form Euler blocks: FP32 geometry (see k_stage_mx)
+  const float4* mgn;             // mixed-form Euler blocks: FP32 geometry (see k_stage_mx)
+  const float2* mgs;
+  const float4* mgd;
   const float* mhy;
+  const float2* msrc;
 };
 
 struct DevBC {
@@ -1517,6 +1520,18 @@ __device__ __forceinline__ f4v mx_stream_load(GP4 p) {
   return *p;
 #endif
 }
+typedef const __attribute__((address_space(1))) unsigned long long* GPF2;   // float2 planes, one 8-B load per cell
+__device__ __forceinline__ float2 f2_bits(unsigned long long b) {
+  return make_float2(__uint_as_float((unsigned)b), __uint_as_float((unsigned)(b >> 32)));
+}
+__device__ __forceinline__ float2 mx_load2(GPF2 p) { return f2_bits(*p); }
+__device__ __forceinline__ float2 mx_stream_load2(GPF2 p) {
+#if Q2D_MX_NT_LOAD
+  return f2_bits(__builtin_nontemporal_load(p));
+#else
+  return f2_bits(*p);
+#endif
+}
 template <int RIE, bool FUSE>
 __global__ __launch_bounds__(256, Q2D_STAGE_MX_MIN_WAVES) void k_stage_mx(const DevBlock* __restrict__ blocks, const DevCase* __restrict__ cases,
                                                   DevCtrl* __restrict__ ctrl, const Item* __restrict__ items, int item_lo,
@@ -1562,7 +1577,8 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MX_MIN_WAVES) void k_stage_mx(const 
       GPW Pout = gpw(B.P[bout]);
       GP dtlp = gp(B.dtl[kdtsel]);
       GPW dtl_next = gpw(B.dtl[kdtsel ^ 1]);
-      GP4 gip = (GP4)B.mgi, gjp = (GP4)B.mgj, gdp = (GP4)B.mgd;
+      GP4 gnp = (GP4)B.mgn, gdp = (GP4)B.mgd;
+      GPF2 gsp = (GPF2)B.mgs, srcp = (GPF2)B.msrc;
       GPF hyp = (GPF)B.mhy;
       const bool tacc = C.time_accurate != 0;
       const GasF gf{(float)C.gam, (float)(C.gam - 1.0), (float)(C.gam / (C.gam - 1.0))};
@@ -1586,7 +1602,7 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MX_MIN_WAVES) void k_stage_mx(const 
       float fA[4], fB[4];                                // their FP32 copies (rows r, r+1)
       float hyA, hyB, hyN;
       float sAB[4], phiA[4], Gprev[4];
-      f4v gjN = gjp[at(item.y0 - 1)];                    // j-face geometry, one row ahead
+      f4v gnN = gnp[at(item.y0 - 1)];                    // face normals, one row ahead
       {
         double qm[4];
         const long long o0 = at(item.y0 - 2), o1 = at(item.y0 - 1), o2 = at(item.y0), o3 = at(item.y0 + 1);
@@ -1619,7 +1635,7 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MX_MIN_WAVES) void k_stage_mx(const 
       // of row r+1 and the face flux from rows r, r+1 (fA, fB) — Mod_Fluxes.f90:187-232.  The flux is
       // evaluated on every lane; lanes that own no face of this row discard it.
       float fC[4], sBC[4], phiB[4], Gcur[4];
-      auto jface = [&](int r, const double (&qC)[4], float hyC, const f4v& gj) {
+      auto jface = [&](int r, const double (&qC)[4], float hyC, const f4v& gn) {
         const float iBC = frcp(hyB + hyC);
 #pragma unroll
         for (int v = 0; v < 4; ++v) { fC[v] = (float)qC[v]; sBC[v] = (fC[v] - fB[v]) * iBC; }
@@ -1631,7 +1647,7 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MX_MIN_WAVES) void k_stage_mx(const 
         for (int v = 0; v < 4; ++v) { L[v] = fA[v] + phiA[v] * hyA; R[v] = fB[v] - phiB[v] * hyB; }
         const bool bad = jdo && !(L[0] > 0.f && L[3] > 0.f && R[0] > 0.f && R[3] > 0.f);
         fail_if_r = bad ? min(fail_if_r, r) : fail_if_r;
-        flux_mx<RIE>(gf, L, R, gj.x, gj.y, gj.z, G);
+        flux_mxs<RIE>(gf, L, R, gn.z, gn.w, G);
 #pragma unroll
         for (int v = 0; v < 4; ++v) Gcur[v] = jdo ? G[v] : 0.f;
       };
@@ -1650,13 +1666,13 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MX_MIN_WAVES) void k_stage_mx(const 
 #pragma unroll
         for (int v = 0; v < 4; ++v) qC[v] = qN[v];
         const float hyC = hyN;
-        const f4v gj = gjN;
+        const f4v gn = gnN;
         const long long oN = at(min(r + 3, rmax));
 #pragma unroll
         for (int v = 0; v < 4; ++v) qN[v] = Pin[v * pl + oN];
         hyN = hyp[oN];
-        gjN = mx_stream_load(&gjp[at(min(r + 1, rmax))]);
-        jface(r, qC, hyC, gj);
+        gnN = mx_stream_load(&gnp[at(min(r + 1, rmax))]);
+        jface(r, qC, hyC, gn);
         roll(qC, hyC);
       }
 
@@ -1669,12 +1685,12 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MX_MIN_WAVES) void k_stage_mx(const 
 #pragma unroll
         for (int v = 0; v < 4; ++v) qC[v] = qN[v];
         const float hyC = hyN;
-        const f4v gj = gjN;
+        const f4v gn = gnN;
         const long long oc = at(r);
         f4v gd = {0.f, 0.f, 0.f, 0.f};
         double po[4] = {0.0, 0.0, 0.0, 0.0};
         double dtl = 0.0;
-        const f4v gi = mx_stream_load(&gip[oc]);
+        const float2 gs = mx_stream_load2(&gsp[oc]);
         if (irk != 1) {
 #if Q2D_MX_NT_LOAD
 #pragma unroll
@@ -1691,12 +1707,12 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MX_MIN_WAVES) void k_stage_mx(const 
 #pragma unroll
           for (int v = 0; v < 4; ++v) qN[v] = Pin[v * pl + oN];
           hyN = hyp[oN];
-          gjN = mx_stream_load(&gjp[at(min(r + 1, rmax))]);
+          gnN = mx_stream_load(&gnp[at(min(r + 1, rmax))]);
         }
-        jface(r, qC, hyC, gj);
+        jface(r, qC, hyC, gn);
 
         // ---- i-direction on row r (DPP shifts between lanes) ------------------
-        const float hx = gi.w, hxn = shdn(hx);
+        const float hx = gs.x, hxn = shdn(hx);
         float fn[4], sx[4], phx[4];
 #pragma unroll
         for (int v = 0; v < 4; ++v) fn[v] = shdn(fA[v]);
@@ -1715,7 +1731,7 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MX_MIN_WAVES) void k_stage_mx(const 
           for (int v = 0; v < 4; ++v) { L[v] = fA[v] + phx[v] * hx; R[v] = fn[v] - phxn[v] * hxn; }
           const bool bad = xdo && !(L[0] > 0.f && L[3] > 0.f && R[0] > 0.f && R[3] > 0.f);
           fail_if_r = bad ? min(fail_if_r, r) : fail_if_r;
-          flux_mx<RIE>(gf, L, R, gi.x, gi.y, gi.z, F);
+          flux_mxs<RIE>(gf, L, R, gn.x, gn.y, F);
 #pragma unroll
           for (int v = 0; v < 4; ++v) Fx[v] = xdo ? F[v] : 0.f;
         }
@@ -1755,14 +1771,15 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MX_MIN_WAVES) void k_stage_mx(const 
         if (C.source != Q2D_SRC_NONE) {
           const double rho = qA[0], u = qA[1], v = qA[2], p = qA[3];
           const double rhoE0 = p * rgm1 + 0.5 * rho * (u * u + v * v);
+          const float2 sg = mx_load2(&srcp[oc]);                  // FP32 source geometry: (r_ave, 1/r) or (G_x, G_y)
           if (C.source == Q2D_SRC_AXI) {
-            const double invr = gp(B.sb)[oc];
+            const double invr = (double)sg.y;
             S[0] = -(rho * v * invr);
             S[1] = -(rho * u * v * invr);
             S[2] = -(rho * v * v * invr);
             S[3] = -((rhoE0 + p) * v * invr);
           } else {
-            const double VdG = u * gp(B.sa)[oc] + v * gp(B.sb)[oc];
+            const double VdG = u * (double)sg.x + v * (double)sg.y;
             S[0] = rho * VdG;
             S[1] = rho * u * VdG;
             S[2] = rho * v * VdG;
@@ -1775,7 +1792,7 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MX_MIN_WAVES) void k_stage_mx(const 
           for (int v = 0; v < 4; ++v) po[v] = qA[v];
         }
         const double dt = tacc ? dtg : dtl;
-        const double dts = dt * (double)gj.w;
+        const double dts = dt * (double)gs.y;
         double Uo[4], U[4], qn2[4];
         Uo[0] = po[0]; Uo[1] = po[0] * po[1]; Uo[2] = po[0] * po[2];
         Uo[3] = __builtin_fma(po[3], rgm1, 0.5 * po[0] * (po[1] * po[1] + po[2] * po[2]));
@@ -2621,24 +2638,27 @@ int append_cases(q2d_handle* h, const q2d_problem* prob, int32_t* first_case) {
         for (int i = 0; i <= d.ni; ++i)
           plane(29)[(size_t)(j + 1) * pitch + (i + 1)] = d.node[((size_t)j * (d.ni + 1) + i) * 2 + 1];
     if (mxgeo) {
-      // planes 30-31: GI, 32-33: GJ, 34-35: GD (float4 each), 36: HY (float), over the ghost extent
-      float* gi = reinterpret_cast<float*>(plane(30));
-      float* gj = reinterpret_cast<float*>(plane(32));
-      float* gd = reinterpret_cast<float*>(plane(34));
-      float* hy = reinterpret_cast<float*>(plane(36));
+      // planes 30-31: GN (float4: int_l * N of the i-face x|x+1 and of the j-face r|r+1), 32: GS (float2:
+      // dl_x / 2, 1 / sur), 33-34: GD (float4), 35: HY (float), 36: SRC (float2: source geometry), over the
+      // ghost extent
+      float* gn = reinterpret_cast<float*>(plane(30));
+      float* gs = reinterpret_cast<float*>(plane(32));
+      float* gd = reinterpret_cast<float*>(plane(33));
+      float* hy = reinterpret_cast<float*>(plane(35));
+      float* sg = reinterpret_cast<float*>(plane(36));
       for (int j = 0; j < ngj; ++j)
         for (int i = 0; i < ngi; ++i) {
           const size_t c = (size_t)j * pitch + i;
-          gi[4 * c + 0] = (float)plane(19)[c]; gi[4 * c + 1] = (float)plane(20)[c];
-          gi[4 * c + 2] = (float)plane(21)[c]; gi[4 * c + 3] = (float)(0.5 * plane(12)[c]);
+          gn[4 * c + 0] = (float)(plane(21)[c] * plane(19)[c]); gn[4 * c + 1] = (float)(plane(21)[c] * plane(20)[c]);
+          gn[4 * c + 2] = (float)(plane(24)[c] * plane(22)[c]); gn[4 * c + 3] = (float)(plane(24)[c] * plane(23)[c]);
           const double sr = plane(14)[c];
-          gj[4 * c + 0] = (float)plane(22)[c]; gj[4 * c + 1] = (float)plane(23)[c];
-          gj[4 * c + 2] = (float)plane(24)[c]; gj[4 * c + 3] = sr != 0.0 ? (float)(1.0 / sr) : 0.f;
+          gs[2 * c + 0] = (float)(0.5 * plane(12)[c]); gs[2 * c + 1] = sr != 0.0 ? (float)(1.0 / sr) : 0.f;
           const double c11 = plane(15)[c], c21 = plane(16)[c], c12 = plane(17)[c], c22 = plane(18)[c];
           const double n1 = sqrt(c11 * c11 + c12 * c12), n2 = sqrt(c21 * c21 + c22 * c22);
           gd[4 * c + 0] = n1 > 0.0 ? (float)(c11 / n1) : 0.f; gd[4 * c + 1] = n1 > 0.0 ? (float)(c12 / n1) : 0.f;
           gd[4 * c + 2] = n2 > 0.0 ? (float)(c21 / n2) : 0.f; gd[4 * c + 3] = n2 > 0.0 ? (float)(c22 / n2) : 0.f;
           hy[c] = (float)(0.5 * plane(13)[c]);
+          sg[2 * c + 0] = (float)plane(26)[c]; sg[2 * c + 1] = (float)plane(27)[c];
         }
     }
     HIPCHK(hipMemcpy(hb.arena, host.data(), hb.arena_bytes, hipMemcpyHostToDevice));
@@ -2656,8 +2676,9 @@ int append_cases(q2d_handle* h, const q2d_problem* prob, int32_t* first_case) {
     db.nodr = a + 29 * pl;
     if (visc) { db.vdx = a + 30 * pl; db.vey = a + 33 * pl; db.vsv = a + 36 * pl; }
     if (mxgeo) {
-      db.mgi = reinterpret_cast<const float4*>(a + 30 * pl); db.mgj = reinterpret_cast<const float4*>(a + 32 * pl);
-      db.mgd = reinterpret_cast<const float4*>(a + 34 * pl); db.mhy = reinterpret_cast<const float*>(a + 36 * pl);
+      db.mgn = reinterpret_cast<const float4*>(a + 30 * pl); db.mgs = reinterpret_cast<const float2*>(a + 32 * pl);
+      db.mgd = reinterpret_cast<const float4*>(a + 33 * pl); db.mhy = reinterpret_cast<const float*>(a + 35 * pl);
+      db.msrc = reinterpret_cast<const float2*>(a + 36 * pl);
     }
     for (int f = 0; f < 4; ++f) db.bcf[f] = -1;
   }
