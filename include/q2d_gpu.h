@@ -166,8 +166,10 @@ typedef struct q2d_step_out {
   double  residual[4];      /* residuotot(1:nres): sqrt of the summed squares */
   int32_t iter;             /* domain%iter after the step */
   int32_t todo;             /* 1 continue, 3 end of simulation (iter/res/time thresholds) */
-  int32_t status;           /* Q2D_OK or Q2D_E_STATE / Q2D_E_INTERFACE */
-  int32_t fail_b, fail_i, fail_j;   /* first failing cell (block 1-based within case) */
+  int32_t status;           /* Q2D_OK or Q2D_E_STATE / Q2D_E_INTERFACE: the stop of the first
+                               failure in the reference's order (RK stage, then interface states
+                               before the new state, then block, j, i) */
+  int32_t fail_b, fail_i, fail_j;   /* that failure's cell or face (block 1-based within case) */
   int32_t reserved[2];
 } q2d_step_out;
 
@@ -204,8 +206,8 @@ int q2d_gpu_case_count(q2d_handle *h, int32_t *ncases, int32_t *nblocks, int64_t
  * (may be NULL).  Synchronises with the device. */
 int q2d_gpu_step(q2d_handle *h, q2d_step_out *out);
 
-/* nsteps explicit steps without host synchronisation in between (each step a
- * replay of one captured HIP graph): the host's time loop around Explicit_Step
+/* nsteps explicit steps without host synchronisation in between (replays of
+ * captured HIP graphs of eight steps and of one step): the host's time loop around Explicit_Step
  * (main.f90:48 -> Wrap_Solve.f90:24) with its TODO test (Mod_Explicit.f90:
  * 120-168) evaluated on the device: a case whose todo became 3 is frozen.
  * out[] as for step. */
@@ -225,7 +227,7 @@ int q2d_gpu_download_dt(q2d_handle *h, int32_t block, double *dtlocal);
  * stage-kernel launch records its first workgroup start and last wavefront end
  * on the device's constant-rate wall clock (two atomics per workgroup), in a
  * ring of 64 steps that the host reads every 64 steps and at the end of each
- * call; steps still replay one captured graph.  Enabling (again) clears the
+ * call; steps still replay the captured graphs.  Enabling (again) clears the
  * statistics.  stats returns the number of timed stage launches and their summed
  * duration in milliseconds. */
 int q2d_gpu_set_timing(q2d_handle *h, int32_t enable);
