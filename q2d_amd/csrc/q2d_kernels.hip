@@ -119,8 +119,9 @@ struct DevBlock {
 struct DevBC {
   int type, blk, i, j;           // blk: global block index
   int f, dblk, is, js;           // donor (101/201)
-  int fs, pad0, cid, pad2;       // pad0: d11 of a connection (the donor's orientation along the face);
-                                 // cid: global case index of the record's block
+  int fs, pad0, cid, dpos;       // pad0: d11 of a connection (the donor's orientation along the face);
+                                 // cid: global case index of the record's block; dpos: index of the
+                                 // record's boundary cell in the per-face arrays (DevBlock::bcf + position)
   double mach, T, p, rel_fac, cos_a, sin_a;
 };
 
@@ -439,6 +440,21 @@ __device__ __forceinline__ void ghost_fill(const DevBlock* blocks, const DevBC& 
 // V: viscous boundary flux as -(modfm2 * Flux), added after F where vmask has the bit (BC 300/302, model 1).
 struct BCFlux { double F[4]; double V[4]; int mask; int vmask; };
 
+// A record's result: the BCFlux struct (read by the FP64 and laminar stage kernels in record order) and,
+// for the mixed Euler stage kernel, its convective flux with the mask applied, in FP32 (the precision of
+// that form's face fluxes), in the dense per-face array (a float4 per boundary cell, indexed like the
+// per-face record index), so that kernel reads a boundary cell's flux with one load and no indirection.
+__device__ __forceinline__ void emit_bc(BCFlux* __restrict__ out, float4* __restrict__ dense, int t, int dpos,
+                                        const BCFlux& res) {
+  out[t] = res;
+  if (dense) {
+    float f[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) f[v] = ((res.mask >> v) & 1) ? (float)res.F[v] : 0.f;
+    dense[dpos] = make_float4(f[0], f[1], f[2], f[3]);
+  }
+}
+
 template <typename FT>
 __device__ __forceinline__ void riemann_rt(int rid, const Gas<FT>& g, const FT L[4], const FT R[4], FT nx, FT ny, FT F[4]) {
   const FT aL = sound<FT>(g, L[3], L[0]);
@@ -455,7 +471,8 @@ template <typename FT>
 // launch also does k_begin's per-case work, which touches no control word that a record reads.
 __global__ __launch_bounds__(256) void k_bcflux(const DevBlock* __restrict__ blocks, const DevCase* __restrict__ cases,
                                                 DevCtrl* __restrict__ ctrl, const DevBC* __restrict__ bcs, int nbc,
-                                                int irk, BCFlux* __restrict__ out, int phase, int begin_cases) {
+                                                int irk, BCFlux* __restrict__ out, int phase, int begin_cases,
+                                                float4* __restrict__ dense) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   for (int c = t; c < begin_cases; c += gridDim.x * blockDim.x) begin_case(ctrl, cases, c, 1);
   if (t >= nbc) return;
@@ -543,7 +560,7 @@ __global__ __launch_bounds__(256) void k_bcflux(const DevBlock* __restrict__ blo
     // one species: Sel_L*rho_L/rho_L - Sel_R*rho_R/rho_R == 1, so Flux(1) = F_r*int_l
 #pragma unroll
     for (int v = 0; v < 4; ++v) res.F[v] = ((double)F[v] * il) * sgn;
-    out[t] = res;
+    emit_bc(out, dense, t, E.dpos, res);
     return;
   }
 
@@ -561,7 +578,7 @@ __global__ __launch_bounds__(256) void k_bcflux(const DevBlock* __restrict__ blo
     Fb[3] = Fmass * (gd.cp * Tt + 0.5 * vmag * vmag);
 #pragma unroll
     for (int k = 0; k < 4; ++k) res.F[k] = sgn * Fb[k];
-    out[t] = res;
+    emit_bc(out, dense, t, E.dpos, res);
     return;
   }
   if (ty == 406) {
@@ -605,7 +622,7 @@ __global__ __launch_bounds__(256) void k_bcflux(const DevBlock* __restrict__ blo
       Fb[3] = mdot * (0.5 * (Fp[1] * Fp[1] + Fp[2] * Fp[2]) + Hf);
 #pragma unroll
       for (int k = 0; k < 4; ++k) res.F[k] = sgn * Fb[k];
-      out[t] = res;
+      emit_bc(out, dense, t, E.dpos, res);
       return;
     }
   }
@@ -673,7 +690,7 @@ __global__ __launch_bounds__(256) void k_bcflux(const DevBlock* __restrict__ blo
 #pragma unroll
   for (int v = 0; v < 4; ++v) res.F[v] = sgn * ((double)F[v] * il);
   if (res.mask == 0x6) { res.F[0] = 0.0; res.F[3] = 0.0; }
-  out[t] = res;
+  emit_bc(out, dense, t, E.dpos, res);
 }
 
 // ---------------------------------------------------------------------------
@@ -1500,8 +1517,11 @@ typedef float f4v __attribute__((ext_vector_type(4)));
 typedef const __attribute__((address_space(1))) float* GPF;
 typedef const __attribute__((address_space(1))) f4v* GP4;
 
+// 3 waves per SIMD for the mixed stage kernel: the register budget (168 VGPRs) is enforced rather than
+// left to the scheduler, which otherwise trades a wave per SIMD for ILP (0.458 -> 0.415 ms per step on
+// 4 x 1024^2; a few spilled registers).
 #ifndef Q2D_STAGE_MX_MIN_WAVES
-#define Q2D_STAGE_MX_MIN_WAVES 1
+#define Q2D_STAGE_MX_MIN_WAVES 3
 #endif
 // Streaming (non-temporal) access for the data each mixed stage touches once: the new state it writes,
 // the RK base state and the i-face, j-face and dt geometry it reads; the L2 keeps the state rows that
@@ -1520,6 +1540,7 @@ __device__ __forceinline__ f4v mx_stream_load(GP4 p) {
   return *p;
 #endif
 }
+__device__ __forceinline__ f4v mx_load4(const float4* p) { return *(GP4)p; }
 typedef const __attribute__((address_space(1))) unsigned long long* GPF2;   // float2 planes, one 8-B load per cell
 __device__ __forceinline__ float2 f2_bits(unsigned long long b) {
   return make_float2(__uint_as_float((unsigned)b), __uint_as_float((unsigned)(b >> 32)));
@@ -1535,8 +1556,7 @@ __device__ __forceinline__ float2 mx_stream_load2(GPF2 p) {
 template <int RIE, bool FUSE>
 __global__ __launch_bounds__(256, Q2D_STAGE_MX_MIN_WAVES) void k_stage_mx(const DevBlock* __restrict__ blocks, const DevCase* __restrict__ cases,
                                                   DevCtrl* __restrict__ ctrl, const Item* __restrict__ items, int item_lo,
-                                                  int item_hi, const int* __restrict__ bcface,
-                                                  const BCFlux* __restrict__ bcflux, double* __restrict__ partials,
+                                                  int item_hi, const float4* __restrict__ bcdense, double* __restrict__ partials,
                                                   int irk, int xcd_remap, unsigned long long* __restrict__ tstamp,
                                                   int tslot) {
   unsigned long long* tslotp = nullptr;
@@ -1547,6 +1567,8 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MX_MIN_WAVES) void k_stage_mx(const 
   }
   const int lane = threadIdx.x & (kWave - 1);
   const int wpg = blockDim.x / kWave;
+  __shared__ f4v sbc[4][2][16];                        // per wavefront: face 1 and 2 boundary fluxes, 16 rows
+  const int wslot = threadIdx.x / kWave;
   int grp = blockIdx.x;
   if (xcd_remap) {
     const int ng = gridDim.x;
@@ -1592,8 +1614,11 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MX_MIN_WAVES) void k_stage_mx(const 
       const int xc = x <= ni + 2 ? x : ni + 2;
       const bool own = lane >= 2 && lane < 2 + kOwn && x <= ni;
       const bool xdo = lane >= 1 && lane <= 61 && x >= 1 && x <= ni - 1;
-      const int rec3 = (own && B.bcf[2] >= 0) ? bcface[B.bcf[2] + x - 1] : -1;
-      const int rec4 = (own && B.bcf[3] >= 0) ? bcface[B.bcf[3] + x - 1] : -1;
+      // boundary-face fluxes (from k_bcflux's dense per-face array): faces 1 and 2 of the strip's edge
+      // lanes are staged 16 rows at a time in LDS (one load per lane per 16 rows instead of a load chain
+      // per row); faces 3 and 4 are read by every owning lane in the block's first and last rows
+      const bool lface = B.bcf[0] >= 0 && item.x0 == 1;
+      const bool rface = B.bcf[1] >= 0 && ni < item.x0 + kOwn;
       const int rmax = nj + 2;
       auto at = [&](int r) { return (long long)(r + 1) * pitch + (xc + 1); };
 
@@ -1676,7 +1701,15 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MX_MIN_WAVES) void k_stage_mx(const 
         roll(qC, hyC);
       }
 
-      for (int r = item.y0; r <= item.y1; ++r) {
+      for (int r0 = item.y0; r0 <= item.y1; r0 += 16) {
+      if ((lface || rface) && lane < 32) {
+        // lanes 0-15: face 1, lanes 16-31: face 2, rows r0 .. r0+15 (zero past the item)
+        const int k = lane & 15, fsel = lane >> 4, row = r0 + k;
+        const bool use = (fsel == 0 ? lface : rface) && row <= item.y1;
+        const f4v z = {0.f, 0.f, 0.f, 0.f};
+        sbc[wslot][fsel][k] = use ? mx_load4(&bcdense[B.bcf[fsel] + row - 1]) : z;
+      }
+      for (int r = r0; r <= min(r0 + 15, item.y1); ++r) {
         // Loads of this iteration, in the order they are consumed (vector-memory waits are in issue
         // order): first row r's own data (i-face geometry, RK base state, local dt, the fused dt's
         // directions; first used after the j-face flux), then row r+3 and row r+1's j-face geometry for the
@@ -1739,31 +1772,34 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MX_MIN_WAVES) void k_stage_mx(const 
 #pragma unroll
         for (int v = 0; v < 4; ++v) Fl[v] = shup(Fx[v]);
 
-        // ---- residual of the cell, faces in the reference's order (every lane; owners store) ----------
+        // ---- boundary faces (BC_Fluxes, Mod_BC_Fluxes.f90:49-271, from k_bcflux): each enters the cell's
+        // residual through the face-flux slot of its face (Fl, Fx, Gprev, Gcur), carrying the face sign
+        const bool bl = lface && x == 1, br = rface && x == ni;
+        const bool bb = r == 1 && B.bcf[2] >= 0, bt = r == nj && B.bcf[3] >= 0;
+        if (lface || rface) {                            // edge strip: faces 1 / 2 from the staged rows
+          const f4v q1 = sbc[wslot][0][r - r0], q2 = sbc[wslot][1][r - r0];
+          Fl[0] = bl ? -q1.x : Fl[0]; Fl[1] = bl ? -q1.y : Fl[1]; Fl[2] = bl ? -q1.z : Fl[2]; Fl[3] = bl ? -q1.w : Fl[3];
+          Fx[0] = br ? q2.x : Fx[0]; Fx[1] = br ? q2.y : Fx[1]; Fx[2] = br ? q2.z : Fx[2]; Fx[3] = br ? q2.w : Fx[3];
+        }
+        if (bb) {                                        // first row: face 3
+          const f4v q = mx_load4(&bcdense[B.bcf[2] + min(max(x, 1), ni) - 1]);
+          Gprev[0] = -q.x; Gprev[1] = -q.y; Gprev[2] = -q.z; Gprev[3] = -q.w;
+        }
+        if (bt) {                                        // last row: face 4 (also the first in a one-row block)
+          const f4v q = mx_load4(&bcdense[B.bcf[3] + min(max(x, 1), ni) - 1]);
+          Gcur[0] = q.x; Gcur[1] = q.y; Gcur[2] = q.z; Gcur[3] = q.w;
+        }
+
+        // ---- residual of the cell (every lane; owners store) ----------
         double res[4];
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           double rv = 0.0;
-          if (x >= 2) rv = rv - (double)Fl[v];
-          if (x <= ni - 1) rv = rv + (double)Fx[v];
-          if (r >= 2) rv = rv - (double)Gprev[v];
-          if (r <= nj - 1) rv = rv + (double)Gcur[v];
+          if (x >= 2 || bl) rv = rv - (double)Fl[v];
+          if (x <= ni - 1 || br) rv = rv + (double)Fx[v];
+          if (r >= 2 || bb) rv = rv - (double)Gprev[v];
+          if (r <= nj - 1 || bt) rv = rv + (double)Gcur[v];
           res[v] = rv;
-        }
-        if (own && (x == 1 || x == ni || r == 1 || r == nj)) {  // BC faces 1..4 in record order (Mod_BC_Fluxes.f90:79-86)
-#pragma unroll 1
-          for (int f = 0; f < 4; ++f) {
-            int rec = -1;
-            if (f == 0 && x == 1 && B.bcf[0] >= 0) rec = bcface[B.bcf[0] + r - 1];
-            if (f == 1 && x == ni && B.bcf[1] >= 0) rec = bcface[B.bcf[1] + r - 1];
-            if (f == 2 && r == 1) rec = rec3;
-            if (f == 3 && r == nj) rec = rec4;
-            if (rec < 0) continue;
-            const BCFlux& q = bcflux[rec];
-#pragma unroll
-            for (int v = 0; v < 4; ++v)
-              if ((q.mask >> v) & 1) res[v] = res[v] + q.F[v];
-          }
         }
         // geometric sources (Lib_Source_Axisymmetry.f90:40-78, Lib_Source_Areavariation.f90:104-144),
         // per unit area: Res/sur - S enters the update
@@ -1836,6 +1872,7 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MX_MIN_WAVES) void k_stage_mx(const 
           if (last && FUSE && !tacc) dtl_next[oc] = dtmin;
         }
         roll(qC, hyC);
+      }
       }
       if (fail_if_r != kNoRow) record_fail(&K, Q2D_E_INTERFACE, fail_key(stage_id, 0, B.blk_in_case, fail_if_r, x));
       if (fail_st_r != kNoRow) record_fail(&K, Q2D_E_STATE, fail_key(stage_id, 1, B.blk_in_case, fail_st_r, x));
@@ -1987,6 +2024,7 @@ struct q2d_handle {
   Item* d_items = nullptr;
   double* d_partials = nullptr;
   BCFlux* d_bcflux = nullptr;
+  float4* d_bcdense = nullptr;   // per-face convective boundary fluxes (mixed form), FP32, one float4 per cell
   int range[5] = {0, 0, 0, 0, 0}; // items sorted into [range[k], range[k+1]): k = 2*viscous + riemann
   std::vector<DtRange> dtr;      // dt launch: units of up to kDtCells cells of one live block
   DtRange* d_dtr = nullptr;
@@ -2192,7 +2230,7 @@ int launch_stage(q2d_handle* h, int irk) {
                        slot)
 #define Q2D_STAGE_MX_LAUNCH(RIE)                                                                                  \
     hipLaunchKernelGGL((k_stage_mx<RIE, kFuseDtMx>), dim3(groups), dim3(tpb), 0, h->stream, h->d_blocks, h->d_cases, \
-                       h->d_ctrl, h->d_items, lo, hi, bf, bq, h->d_partials, irk, h->xcd_remap, ts, slot)
+                       h->d_ctrl, h->d_items, lo, hi, h->d_bcdense, h->d_partials, irk, h->xcd_remap, ts, slot)
     if constexpr (sizeof(FT) == 4) {                   // mixed form: Euler items run k_stage_mx
       switch (pass) {
         case 0: Q2D_STAGE_MX_LAUNCH(0); break;
@@ -2273,10 +2311,10 @@ int launch_step(q2d_handle* h) {
         const int begin_cases = (fold_begin && irk == 1 && ph <= 1) ? h->ncases : 0;
         if (h->precision_mixed)
           hipLaunchKernelGGL(k_bcflux<float>, dim3(bgroups), dim3(64), 0, h->stream, h->d_blocks, h->d_cases,
-                             h->d_ctrl, h->d_bc, h->nbc, irk, h->d_bcflux, ph, begin_cases);
+                             h->d_ctrl, h->d_bc, h->nbc, irk, h->d_bcflux, ph, begin_cases, h->d_bcdense);
         else
           hipLaunchKernelGGL(k_bcflux<double>, dim3(bgroups), dim3(64), 0, h->stream, h->d_blocks, h->d_cases,
-                             h->d_ctrl, h->d_bc, h->nbc, irk, h->d_bcflux, ph, begin_cases);
+                             h->d_ctrl, h->d_bc, h->nbc, irk, h->d_bcflux, ph, begin_cases, h->d_bcdense);
       }
     }
     if (!h->vrec.empty())
@@ -2362,6 +2400,7 @@ int q2d_gpu_destroy(q2d_handle* h) {
   for (auto& b : h->hb) if (b.arena) (void)hipFree(b.arena);
   (void)hipFree(h->d_blocks); (void)hipFree(h->d_cases); (void)hipFree(h->d_ctrl); (void)hipFree(h->d_bc);
   (void)hipFree(h->d_bcface); (void)hipFree(h->d_items); (void)hipFree(h->d_partials); (void)hipFree(h->d_bcflux);
+  (void)hipFree(h->d_bcdense);
   (void)hipFree(h->d_tstamp);
   (void)hipFree(h->d_vrec); (void)hipFree(h->d_dtr);
   if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -2406,6 +2445,7 @@ int rebuild_descriptors(q2d_handle* h) {
       h->db[b].bcf[f] = (int)h->bcface.size();
       h->bcface.insert(h->bcface.end(), fl.begin(), fl.end());
     }
+  for (DevBC& r : h->dbc) r.dpos = h->db[r.blk].bcf[r.f - 1] + (r.f <= 2 ? r.j - 1 : r.i - 1);
   h->max_rk = 1;
   h->any_visc = false;
   h->bc_two_phase = false;
@@ -2430,6 +2470,8 @@ int rebuild_descriptors(q2d_handle* h) {
   if ((rc = realloc_upload((void**)&h->d_vrec, h->vrec.data(), sizeof(int) * h->vrec.size())) != Q2D_OK) return rc;
   (void)hipFree(h->d_bcflux); h->d_bcflux = nullptr;
   HIPCHK(hipMalloc(&h->d_bcflux, sizeof(BCFlux) * std::max<size_t>(1, h->dbc.size())));
+  (void)hipFree(h->d_bcdense); h->d_bcdense = nullptr;
+  if (h->precision_mixed) HIPCHK(hipMalloc(&h->d_bcdense, sizeof(float4) * std::max<size_t>(1, h->bcface.size())));
   return build_items(h);     // also uploads d_cases
 }
 
