@@ -440,13 +440,14 @@ __device__ __forceinline__ void ghost_fill(const DevBlock* blocks, const DevBC& 
 // V: viscous boundary flux as -(modfm2 * Flux), added after F where vmask has the bit (BC 300/302, model 1).
 struct BCFlux { double F[4]; double V[4]; int mask; int vmask; };
 
-// A record's result: the BCFlux struct (read by the FP64 and laminar stage kernels in record order) and,
+// A record's result, stored at the record's boundary cell (DevBC::dpos): the BCFlux struct (read by the
+// FP64 and laminar stage kernels, which add the faces in record order) and,
 // for the mixed Euler stage kernel, its convective flux with the mask applied, in FP32 (the precision of
 // that form's face fluxes), in the dense per-face array (a float4 per boundary cell, indexed like the
 // per-face record index), so that kernel reads a boundary cell's flux with one load and no indirection.
-__device__ __forceinline__ void emit_bc(BCFlux* __restrict__ out, float4* __restrict__ dense, int t, int dpos,
+__device__ __forceinline__ void emit_bc(BCFlux* __restrict__ out, float4* __restrict__ dense, int dpos,
                                         const BCFlux& res) {
-  out[t] = res;
+  out[dpos] = res;
   if (dense) {
     float f[4];
 #pragma unroll
@@ -560,7 +561,7 @@ __global__ __launch_bounds__(256) void k_bcflux(const DevBlock* __restrict__ blo
     // one species: Sel_L*rho_L/rho_L - Sel_R*rho_R/rho_R == 1, so Flux(1) = F_r*int_l
 #pragma unroll
     for (int v = 0; v < 4; ++v) res.F[v] = ((double)F[v] * il) * sgn;
-    emit_bc(out, dense, t, E.dpos, res);
+    emit_bc(out, dense, E.dpos, res);
     return;
   }
 
@@ -578,7 +579,7 @@ __global__ __launch_bounds__(256) void k_bcflux(const DevBlock* __restrict__ blo
     Fb[3] = Fmass * (gd.cp * Tt + 0.5 * vmag * vmag);
 #pragma unroll
     for (int k = 0; k < 4; ++k) res.F[k] = sgn * Fb[k];
-    emit_bc(out, dense, t, E.dpos, res);
+    emit_bc(out, dense, E.dpos, res);
     return;
   }
   if (ty == 406) {
@@ -622,7 +623,7 @@ __global__ __launch_bounds__(256) void k_bcflux(const DevBlock* __restrict__ blo
       Fb[3] = mdot * (0.5 * (Fp[1] * Fp[1] + Fp[2] * Fp[2]) + Hf);
 #pragma unroll
       for (int k = 0; k < 4; ++k) res.F[k] = sgn * Fb[k];
-      emit_bc(out, dense, t, E.dpos, res);
+      emit_bc(out, dense, E.dpos, res);
       return;
     }
   }
@@ -690,7 +691,7 @@ __global__ __launch_bounds__(256) void k_bcflux(const DevBlock* __restrict__ blo
 #pragma unroll
   for (int v = 0; v < 4; ++v) res.F[v] = sgn * ((double)F[v] * il);
   if (res.mask == 0x6) { res.F[0] = 0.0; res.F[3] = 0.0; }
-  emit_bc(out, dense, t, E.dpos, res);
+  emit_bc(out, dense, E.dpos, res);
 }
 
 // ---------------------------------------------------------------------------
@@ -1034,7 +1035,7 @@ __global__ __launch_bounds__(256) void k_bcvisc(const DevBlock* __restrict__ blo
     double Dl[3], Dg[3];
     diff_flux(C, axi, pm0, pm1, pm2, pm3, a[0][0], a[0][1], a[1][0], a[1][1], a[2][0], a[2][1], n1, n2, il, r_ave, VG, Dl);
     diff_flux(C, axi, pm0, pm1, pm2, pm3, b[0][0], b[0][1], b[1][0], b[1][1], b[2][0], b[2][1], n1, n2, il, r_ave, VG, Dg);
-    BCFlux& q = out[k];
+    BCFlux& q = out[E.dpos];
     q.V[0] = 0.0;
 #pragma unroll
     for (int c = 0; c < 3; ++c) q.V[c + 1] = -(0.5 * m2 * (Dl[c] + Dg[c]));
@@ -1054,7 +1055,7 @@ __global__ __launch_bounds__(256) void k_bcvisc(const DevBlock* __restrict__ blo
     F1 = S1 * il; F2 = S2 * il;
     F3 = il * kl * (gTx * n1 + gTy * n2);
   }
-  BCFlux& q = out[k];
+  BCFlux& q = out[E.dpos];
   q.V[0] = -(sgn * 0.0);                             // Flux(1) = 0; BC 302 updates R(:), BC 300 R(nu:np)
   q.V[1] = -(sgn * F1); q.V[2] = -(sgn * F2); q.V[3] = -(sgn * F3);
   q.vmask = E.type == 302 ? 0xF : 0xE;
@@ -1138,6 +1139,8 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
   }
   const int lane = threadIdx.x & (kWave - 1);
   const int wpg = blockDim.x / kWave;
+  __shared__ BCFlux sbq[4][2][16];                     // per wavefront: face 1 and 2 boundary results, 16 rows
+  const int wslot = threadIdx.x / kWave;
   int grp = blockIdx.x;
   if (xcd_remap) {                                     // consecutive logical groups on one XCD (8 XCDs, round-robin dispatch)
     const int ng = gridDim.x;
@@ -1179,8 +1182,12 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
     const int xc = x <= ni + 2 ? x : ni + 2;           // clamped column for loads
     const bool own = lane >= 2 && lane < 2 + kOwn && x <= ni;
     const bool xdo = lane >= 1 && lane <= 61 && x >= 1 && x <= ni - 1;   // interior face (x|x+1)
-    const int rec3 = (own && B.bcf[2] >= 0) ? bcface[B.bcf[2] + x - 1] : -1;
-    const int rec4 = (own && B.bcf[3] >= 0) ? bcface[B.bcf[3] + x - 1] : -1;
+    // boundary-face results (BCFlux at each boundary cell): faces 1 and 2 of the strip's edge lanes are
+    // staged 16 rows at a time in LDS (one load per lane per 16 rows instead of a load per row on the
+    // edge lane's critical path); faces 3 and 4 are read by every owning lane in the first and last rows
+    const bool lface = B.bcf[0] >= 0 && item.x0 == 1;
+    const bool rface = B.bcf[1] >= 0 && ni < item.x0 + kOwn;
+    (void)bcface;
     const int rmax = nj + 2;
 
     auto at = [&](int r) { return (long long)(r + 1) * pitch + (xc + 1); };
@@ -1223,6 +1230,20 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
     double Eprev[3] = {0.0, 0.0, 0.0};                 // diffusive flux of j-face r-1 (VISC)
 
     for (int r = item.y0 - 1; r <= item.y1; ++r) {
+      if ((lface || rface) && r >= item.y0 && ((r - item.y0) & 15) == 0 && lane < 32) {
+        // lanes 0-15: face 1, lanes 16-31: face 2, rows r .. r+15 (empty past the item)
+        const int kq = lane & 15, fsel = lane >> 4, row = r + kq;
+        const bool use = (fsel == 0 ? lface : rface) && row <= item.y1;
+        BCFlux& dst = sbq[wslot][fsel][kq];
+        if (use) {
+          const BCFlux& src = bcflux[B.bcf[fsel] + row - 1];
+#pragma unroll
+          for (int v = 0; v < 4; ++v) { dst.F[v] = src.F[v]; dst.V[v] = src.V[v]; }
+          dst.mask = src.mask; dst.vmask = src.vmask;
+        } else {
+          dst.mask = 0; dst.vmask = 0;
+        }
+      }
       // software pipeline: this iteration's row r+2 was loaded one iteration ago; issue row r+3 and
       // the next row's j-face geometry now, to be consumed by the next iteration.  The rest of row r's
       // data (i-face geometry, sur, dt, RK base state) is issued here and first used after the j-face
@@ -1353,27 +1374,20 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
             }
             res[v] = rv;
           }
-          // ---- BC faces 1..4 in record order (Mod_BC_Fluxes.f90:79-86) ----
-          if (x == 1 || x == ni || r == 1 || r == nj) {
-#pragma unroll 1
-            for (int f = 0; f < 4; ++f) {
-              int rec = -1;
-              if (f == 0 && x == 1 && B.bcf[0] >= 0) rec = bcface[B.bcf[0] + r - 1];
-              if (f == 1 && x == ni && B.bcf[1] >= 0) rec = bcface[B.bcf[1] + r - 1];
-              if (f == 2 && r == 1) rec = rec3;
-              if (f == 3 && r == nj) rec = rec4;
-              if (rec < 0) continue;
-              const BCFlux& q = bcflux[rec];
+          // ---- BC faces 1..4 in record order (Mod_BC_Fluxes.f90:79-86): convective, then viscous ----
+          auto add_bc = [&](const BCFlux& q) {
 #pragma unroll
-              for (int v = 0; v < 4; ++v)
-                if ((q.mask >> v) & 1) res[v] = res[v] + q.F[v];
-              if (VISC) {
+            for (int v = 0; v < 4; ++v) res[v] = ((q.mask >> v) & 1) ? res[v] + q.F[v] : res[v];
+            if (VISC) {
 #pragma unroll
-                for (int v = 0; v < 4; ++v)
-                  if ((q.vmask >> v) & 1) res[v] = res[v] + q.V[v];
-              }
+              for (int v = 0; v < 4; ++v) res[v] = ((q.vmask >> v) & 1) ? res[v] + q.V[v] : res[v];
             }
-          }
+          };
+          const int kq = (r - item.y0) & 15;
+          if (lface && x == 1) add_bc(sbq[wslot][0][kq]);
+          if (rface && x == ni) add_bc(sbq[wslot][1][kq]);
+          if (r == 1 && B.bcf[2] >= 0) add_bc(bcflux[B.bcf[2] + x - 1]);
+          if (r == nj && B.bcf[3] >= 0) add_bc(bcflux[B.bcf[3] + x - 1]);
           // ---- geometric sources — Lib_Source_Axisymmetry.f90:40-78, Lib_Source_Areavariation.f90:104-144
           const double sr = gcur.sur;
           const Gas<double> gd{C.R, C.cp, C.cv, C.gam};
@@ -1706,8 +1720,9 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MX_MIN_WAVES) void k_stage_mx(const 
         // lanes 0-15: face 1, lanes 16-31: face 2, rows r0 .. r0+15 (zero past the item)
         const int k = lane & 15, fsel = lane >> 4, row = r0 + k;
         const bool use = (fsel == 0 ? lface : rface) && row <= item.y1;
-        const f4v z = {0.f, 0.f, 0.f, 0.f};
-        sbc[wslot][fsel][k] = use ? mx_load4(&bcdense[B.bcf[fsel] + row - 1]) : z;
+        f4v q = {0.f, 0.f, 0.f, 0.f};
+        if (use) q = mx_load4(&bcdense[B.bcf[fsel] + row - 1]);
+        sbc[wslot][fsel][k] = q;
       }
       for (int r = r0; r <= min(r0 + 15, item.y1); ++r) {
         // Loads of this iteration, in the order they are consumed (vector-memory waits are in issue
