@@ -446,8 +446,8 @@ struct BCFlux { double F[4]; double V[4]; int mask; int vmask; };
 // that form's face fluxes), in the dense per-face array (a float4 per boundary cell, indexed like the
 // per-face record index), so that kernel reads a boundary cell's flux with one load and no indirection.
 __device__ __forceinline__ void emit_bc(BCFlux* __restrict__ out, float4* __restrict__ dense, int dpos,
-                                        const BCFlux& res) {
-  out[dpos] = res;
+                                        const BCFlux& res, bool euler) {
+  if (!(dense && euler)) out[dpos] = res;             // the mixed Euler stage kernel reads only the dense array
   if (dense) {
     float f[4];
 #pragma unroll
@@ -521,10 +521,25 @@ __global__ __launch_bounds__(256) void k_bcflux(const DevBlock* __restrict__ blo
   // Face_Index — Lib_BC_Fluxes.f90:34-50
   const bool idir = Fm <= 2;
   const long long ofc = idir ? off(Im - modfm, Jm) : off(Im, Jm - modfm);
-  const double n1 = idir ? B.fix[ofc] : B.fjx[ofc];
-  const double n2 = idir ? B.fiy[ofc] : B.fjy[ofc];
-  const double il = idir ? B.fil[ofc] : B.fjl[ofc];
+  // mixed-form Euler blocks read the compact FP32 geometry (face length-scaled normal, half cell sizes):
+  // fewer cache lines per record, which matters for i-face records (one line per lane per plane)
+  const bool mxg = sizeof(FT) == 4 && B.mgn != nullptr;
+  double n1, n2, il;
+  if (mxg) {
+    const float4 g = B.mgn[ofc];
+    const float sx = idir ? g.x : g.z, sy = idir ? g.y : g.w;
+    const float l = __builtin_sqrtf(sx * sx + sy * sy);
+    n1 = (double)(sx / l); n2 = (double)(sy / l); il = (double)l;
+  } else {
+    n1 = idir ? B.fix[ofc] : B.fjx[ofc];
+    n2 = idir ? B.fiy[ofc] : B.fjy[ofc];
+    il = idir ? B.fil[ofc] : B.fjl[ofc];
+  }
   const double* dlp = idir ? B.dlx : B.dly;
+  auto half_dl = [&](long long o) -> double {        // dl / 2 of the cell at offset o in the face's direction
+    if (mxg) return idir ? (double)__uint_as_float((unsigned)((const unsigned long long*)B.mgs)[o]) : (double)B.mhy[o];
+    return dlp[o] * 0.5;
+  };
   BCFlux res;
   res.mask = 0xF; res.vmask = 0;
   res.V[0] = res.V[1] = res.V[2] = res.V[3] = 0.0;
@@ -537,7 +552,7 @@ __global__ __launch_bounds__(256) void k_bcflux(const DevBlock* __restrict__ blo
     const int I3 = modfm1 * (Im - g1) + modfm * Im, J3 = modfm1 * (Jm - g2) + modfm * Jm;
     const int I4 = modfm1 * (Im - g1 * 2) + modfm * (Im + g1), J4 = modfm1 * (Jm - g2 * 2) + modfm * (Jm + g2);
     const long long o1 = off(I1, J1), o2 = off(I2, J2), o3 = off(I3, J3), o4 = off(I4, J4);
-    const double Dl1 = dlp[o1] * 0.5, Dl2 = dlp[o2] * 0.5, Dl3 = dlp[o3] * 0.5, Dl4 = dlp[o4] * 0.5;
+    const double Dl1 = half_dl(o1), Dl2 = half_dl(o2), Dl3 = half_dl(o3), Dl4 = half_dl(o4);
     const FT Dist23 = (FT)(Dl2 + Dl3), Dist12 = (FT)(Dl1 + Dl2), Dist34 = (FT)(Dl3 + Dl4);
     FT L[4], R[4];
 #pragma unroll
@@ -561,7 +576,7 @@ __global__ __launch_bounds__(256) void k_bcflux(const DevBlock* __restrict__ blo
     // one species: Sel_L*rho_L/rho_L - Sel_R*rho_R/rho_R == 1, so Flux(1) = F_r*int_l
 #pragma unroll
     for (int v = 0; v < 4; ++v) res.F[v] = ((double)F[v] * il) * sgn;
-    emit_bc(out, dense, E.dpos, res);
+    emit_bc(out, dense, E.dpos, res, C.model == 0);
     return;
   }
 
@@ -579,7 +594,7 @@ __global__ __launch_bounds__(256) void k_bcflux(const DevBlock* __restrict__ blo
     Fb[3] = Fmass * (gd.cp * Tt + 0.5 * vmag * vmag);
 #pragma unroll
     for (int k = 0; k < 4; ++k) res.F[k] = sgn * Fb[k];
-    emit_bc(out, dense, E.dpos, res);
+    emit_bc(out, dense, E.dpos, res, C.model == 0);
     return;
   }
   if (ty == 406) {
@@ -623,7 +638,7 @@ __global__ __launch_bounds__(256) void k_bcflux(const DevBlock* __restrict__ blo
       Fb[3] = mdot * (0.5 * (Fp[1] * Fp[1] + Fp[2] * Fp[2]) + Hf);
 #pragma unroll
       for (int k = 0; k < 4; ++k) res.F[k] = sgn * Fb[k];
-      emit_bc(out, dense, E.dpos, res);
+      emit_bc(out, dense, E.dpos, res, C.model == 0);
       return;
     }
   }
@@ -660,7 +675,7 @@ __global__ __launch_bounds__(256) void k_bcflux(const DevBlock* __restrict__ blo
   if (ty == 300 || ty == 302) {
     // BC_Symmetry_Eul — Lib_BC_Fluxes_Symmetry.f90:28-124 (Un_wall = 0); momentum only
     res.mask = 0x6;
-    const double Bound_Dl = 0.5 * dlp[oB], Int_Dl = 0.5 * dlp[oI];
+    const double Bound_Dl = half_dl(oB), Int_Dl = half_dl(oI);
     double Gp[4] = {Bp[0], Bp[1], Bp[2], Bp[3]};
     double Un = Bp[1] * n1 + Bp[2] * n2;
     Gp[1] = Gp[1] - 2.0 * Un * n1;
@@ -691,7 +706,7 @@ __global__ __launch_bounds__(256) void k_bcflux(const DevBlock* __restrict__ blo
 #pragma unroll
   for (int v = 0; v < 4; ++v) res.F[v] = sgn * ((double)F[v] * il);
   if (res.mask == 0x6) { res.F[0] = 0.0; res.F[3] = 0.0; }
-  emit_bc(out, dense, E.dpos, res);
+  emit_bc(out, dense, E.dpos, res, C.model == 0);
 }
 
 // ---------------------------------------------------------------------------
