@@ -1540,7 +1540,8 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
 //   GJ(x, r) = (N(1), N(2), int_l of j-face r|r+1, 1 / sur of cell (x, r))
 //   HY(x, r) = dl_y / 2,  GD(x, r) = rows 1 and 2 of M, each normalised (Mod_dt.f90:76-84)
 // With FUSE the last RK stage also evaluates the next step's Compute_dt (Mod_dt.f90:59-104) on the
-// state it writes, so the step needs no separate dt pass.
+// state it writes, so the step needs no separate dt pass.  RIE < 0 (the instantiation launched): the
+// Riemann solver is picked per item from its case, so HLLC and HLLE cases share one launch per stage.
 // ---------------------------------------------------------------------------
 typedef float f4v __attribute__((ext_vector_type(4)));
 typedef const __attribute__((address_space(1))) float* GPF;
@@ -1615,8 +1616,9 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MX_MIN_WAVES) void k_stage_mx(const 
     if (!K.active || irk > C.n_rk) continue;
     // The item body is instantiated per limiter (a uniform switch per item), so the row loop has no
     // limiter branches: straight-line code that the compiler can schedule across the whole row.
-    auto item_body = [&](auto limc) {
+    auto item_body = [&](auto limc, auto riec) {
       constexpr int LIM = decltype(limc)::value;
+      constexpr int RV = decltype(riec)::value;         // Riemann solver of this item's case
       const bool last = (irk == C.n_rk);
       const int ni = B.ni, nj = B.nj, pitch = B.pitch;
       const long long pl = B.plane;
@@ -1701,7 +1703,7 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MX_MIN_WAVES) void k_stage_mx(const 
         for (int v = 0; v < 4; ++v) { L[v] = fA[v] + phiA[v] * hyA; R[v] = fB[v] - phiB[v] * hyB; }
         const bool bad = jdo && !(L[0] > 0.f && L[3] > 0.f && R[0] > 0.f && R[3] > 0.f);
         fail_if_r = bad ? min(fail_if_r, r) : fail_if_r;
-        flux_mxs<RIE>(gf, L, R, gn.z, gn.w, G);
+        flux_mxs<RV>(gf, L, R, gn.z, gn.w, G);
 #pragma unroll
         for (int v = 0; v < 4; ++v) Gcur[v] = jdo ? G[v] : 0.f;
       };
@@ -1794,7 +1796,7 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MX_MIN_WAVES) void k_stage_mx(const 
           for (int v = 0; v < 4; ++v) { L[v] = fA[v] + phx[v] * hx; R[v] = fn[v] - phxn[v] * hxn; }
           const bool bad = xdo && !(L[0] > 0.f && L[3] > 0.f && R[0] > 0.f && R[3] > 0.f);
           fail_if_r = bad ? min(fail_if_r, r) : fail_if_r;
-          flux_mxs<RIE>(gf, L, R, gn.x, gn.y, F);
+          flux_mxs<RV>(gf, L, R, gn.x, gn.y, F);
 #pragma unroll
           for (int v = 0; v < 4; ++v) Fx[v] = xdo ? F[v] : 0.f;
         }
@@ -1922,14 +1924,23 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MX_MIN_WAVES) void k_stage_mx(const 
         if (lane == 0) partials[(long long)it * kPartials + 4] = dtnext;
       }
     };
-    switch (C.limiter) {
-      case 1: item_body(std::integral_constant<int, 1>{}); break;
-      case 2: item_body(std::integral_constant<int, 2>{}); break;
-      case 3: item_body(std::integral_constant<int, 3>{}); break;
-      case 4: item_body(std::integral_constant<int, 4>{}); break;
-      case 5: item_body(std::integral_constant<int, 5>{}); break;
-      case 6: item_body(std::integral_constant<int, 6>{}); break;
-      default: item_body(std::integral_constant<int, 0>{}); break;
+    // RIE < 0: HLLC and HLLE cases share the launch, the solver picked per item (uniform switch)
+    auto by_limiter = [&](auto riec) {
+      switch (C.limiter) {
+        case 1: item_body(std::integral_constant<int, 1>{}, riec); break;
+        case 2: item_body(std::integral_constant<int, 2>{}, riec); break;
+        case 3: item_body(std::integral_constant<int, 3>{}, riec); break;
+        case 4: item_body(std::integral_constant<int, 4>{}, riec); break;
+        case 5: item_body(std::integral_constant<int, 5>{}, riec); break;
+        case 6: item_body(std::integral_constant<int, 6>{}, riec); break;
+        default: item_body(std::integral_constant<int, 0>{}, riec); break;
+      }
+    };
+    if constexpr (RIE < 0) {
+      if (C.riemann == Q2D_RIEMANN_HLLE) by_limiter(std::integral_constant<int, 1>{});
+      else by_limiter(std::integral_constant<int, 0>{});
+    } else {
+      by_limiter(std::integral_constant<int, RIE>{});
     }
   }
   if (tslotp && lane == 0) atomicMax(&tslotp[1], (unsigned long long)wall_clock64());
@@ -2090,8 +2101,8 @@ namespace {
 const void* stage_kernel(const q2d_handle* h, int pass) {
   if (h->precision_mixed) {
     switch (pass) {
-      case 0: return reinterpret_cast<const void*>(&k_stage_mx<0, kFuseDtMx>);
-      case 1: return reinterpret_cast<const void*>(&k_stage_mx<1, kFuseDtMx>);
+      case 0:
+      case 1: return reinterpret_cast<const void*>(&k_stage_mx<-1, kFuseDtMx>);
       case 2: return reinterpret_cast<const void*>(&k_stage<float, 0, false, true>);
       default: return reinterpret_cast<const void*>(&k_stage<float, 1, false, true>);
     }
@@ -2241,7 +2252,11 @@ template <typename FT, bool FUSE>
 int launch_stage(q2d_handle* h, int irk) {
   const int tpb = kWave * h->waves_per_group;
   for (int pass = 0; pass < 4; ++pass) {
-    const int lo = h->range[pass], hi = h->range[pass + 1];
+    int lo = h->range[pass], hi = h->range[pass + 1];
+    if constexpr (sizeof(FT) == 4) {                   // mixed form: the Euler passes 0 and 1 share one launch
+      if (pass == 1) continue;
+      if (pass == 0) hi = h->range[2];
+    }
     if (hi <= lo) continue;
     int groups = (hi - lo + h->waves_per_group - 1) / h->waves_per_group;
     if (h->max_groups > 0) groups = std::min(groups, h->max_groups);
@@ -2263,8 +2278,7 @@ int launch_stage(q2d_handle* h, int irk) {
                        h->d_ctrl, h->d_items, lo, hi, h->d_bcdense, h->d_partials, irk, h->xcd_remap, ts, slot)
     if constexpr (sizeof(FT) == 4) {                   // mixed form: Euler items run k_stage_mx
       switch (pass) {
-        case 0: Q2D_STAGE_MX_LAUNCH(0); break;
-        case 1: Q2D_STAGE_MX_LAUNCH(1); break;
+        case 0: Q2D_STAGE_MX_LAUNCH(-1); break;         // items of passes 0 and 1 (HLLC and HLLE)
         case 2: Q2D_STAGE_LAUNCH(0, true); break;
         default: Q2D_STAGE_LAUNCH(1, true); break;
       }
