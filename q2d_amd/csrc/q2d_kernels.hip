@@ -1136,7 +1136,7 @@ __device__ __forceinline__ bool face_flux(const Gas<FT>& g, const double qa[4], 
   return bad;
 }
 
-template <typename FT, int RIE, bool FUSE, bool VISC>
+template <typename FT, int RIE_T, bool FUSE, bool VISC>
 __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlock* __restrict__ blocks, const DevCase* __restrict__ cases,
                                                DevCtrl* __restrict__ ctrl, const Item* __restrict__ items, int item_lo,
                                                int item_hi, const int* __restrict__ bcface,
@@ -1171,360 +1171,370 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
     DevCtrl& K = ctrl[B.case_id];
     const DevCase& C = cases[B.case_id];
     if (!K.active || irk > C.n_rk) continue;
-    const bool last = (irk == C.n_rk);
-    const int ni = B.ni, nj = B.nj, pitch = B.pitch;
-    const long long pl = B.plane;
-    // the case's control words, read once per item (wave-uniform; a read inside the row loop would be a
-    // vector load whose wait also drains the row prefetch)
-    const int kcur = __builtin_amdgcn_readfirstlane(K.cur), kdtsel = __builtin_amdgcn_readfirstlane(K.dtsel);
-    const int kiter = __builtin_amdgcn_readfirstlane(K.iter);
-    const int bin = stage_in(kcur, irk), bout = stage_out(kcur, irk);
-    GP Pin = gp(B.P[bin]);
-    GP PO = gp(B.P[kcur]);
-    GPW Pout = gpw(B.P[bout]);
-    GP dlxp = gp(B.dlx), dlyp = gp(B.dly), surp = gp(B.sur), dtlp = gp(B.dtl[kdtsel]);
-    GP fixp = gp(B.fix), fiyp = gp(B.fiy), filp = gp(B.fil), fjxp = gp(B.fjx), fjyp = gp(B.fjy), fjlp = gp(B.fjl);
-    GP m11p = gp(B.m11);
-    const int lim = C.limiter;
-    const bool tacc = C.time_accurate != 0;
-    const Gas<FT> gf{(FT)C.R, (FT)C.cp, (FT)C.cv, (FT)C.gam};
-    const double c_rk = (C.n_rk == 1) ? 1.0 : (C.n_rk == 2 ? (irk == 1 ? 1.0 : 0.5)
-                                                            : (irk == 1 ? 1.0 : (irk == 2 ? 0.25 : 2.0 / 3.0)));
-    const double dtg = K.dtglobal;
-    const int stage_id = K.stage_count + irk;
+    // RIE_T < 0: HLLC and HLLE cases share the launch; the item's case picks the solver (uniform switch)
+    auto item_body = [&](auto riec) {
+      constexpr int RIE = decltype(riec)::value;
+      const bool last = (irk == C.n_rk);
+      const int ni = B.ni, nj = B.nj, pitch = B.pitch;
+      const long long pl = B.plane;
+      // the case's control words, read once per item (wave-uniform; a read inside the row loop would be a
+      // vector load whose wait also drains the row prefetch)
+      const int kcur = __builtin_amdgcn_readfirstlane(K.cur), kdtsel = __builtin_amdgcn_readfirstlane(K.dtsel);
+      const int kiter = __builtin_amdgcn_readfirstlane(K.iter);
+      const int bin = stage_in(kcur, irk), bout = stage_out(kcur, irk);
+      GP Pin = gp(B.P[bin]);
+      GP PO = gp(B.P[kcur]);
+      GPW Pout = gpw(B.P[bout]);
+      GP dlxp = gp(B.dlx), dlyp = gp(B.dly), surp = gp(B.sur), dtlp = gp(B.dtl[kdtsel]);
+      GP fixp = gp(B.fix), fiyp = gp(B.fiy), filp = gp(B.fil), fjxp = gp(B.fjx), fjyp = gp(B.fjy), fjlp = gp(B.fjl);
+      GP m11p = gp(B.m11);
+      const int lim = C.limiter;
+      const bool tacc = C.time_accurate != 0;
+      const Gas<FT> gf{(FT)C.R, (FT)C.cp, (FT)C.cv, (FT)C.gam};
+      const double c_rk = (C.n_rk == 1) ? 1.0 : (C.n_rk == 2 ? (irk == 1 ? 1.0 : 0.5)
+                                                              : (irk == 1 ? 1.0 : (irk == 2 ? 0.25 : 2.0 / 3.0)));
+      const double dtg = K.dtglobal;
+      const int stage_id = K.stage_count + irk;
 
-    const int x = item.x0 - 2 + lane;                  // Fortran column of this lane
-    const int xc = x <= ni + 2 ? x : ni + 2;           // clamped column for loads
-    const bool own = lane >= 2 && lane < 2 + kOwn && x <= ni;
-    const bool xdo = lane >= 1 && lane <= 61 && x >= 1 && x <= ni - 1;   // interior face (x|x+1)
-    // boundary-face results (BCFlux at each boundary cell): faces 1 and 2 of the strip's edge lanes are
-    // staged 16 rows at a time in LDS (one load per lane per 16 rows instead of a load per row on the
-    // edge lane's critical path); faces 3 and 4 are read by every owning lane in the first and last rows
-    const bool lface = B.bcf[0] >= 0 && item.x0 == 1;
-    const bool rface = B.bcf[1] >= 0 && ni < item.x0 + kOwn;
-    (void)bcface;
-    const int rmax = nj + 2;
+      const int x = item.x0 - 2 + lane;                  // Fortran column of this lane
+      const int xc = x <= ni + 2 ? x : ni + 2;           // clamped column for loads
+      const bool own = lane >= 2 && lane < 2 + kOwn && x <= ni;
+      const bool xdo = lane >= 1 && lane <= 61 && x >= 1 && x <= ni - 1;   // interior face (x|x+1)
+      // boundary-face results (BCFlux at each boundary cell): faces 1 and 2 of the strip's edge lanes are
+      // staged 16 rows at a time in LDS (one load per lane per 16 rows instead of a load per row on the
+      // edge lane's critical path); faces 3 and 4 are read by every owning lane in the first and last rows
+      const bool lface = B.bcf[0] >= 0 && item.x0 == 1;
+      const bool rface = B.bcf[1] >= 0 && ni < item.x0 + kOwn;
+      (void)bcface;
+      const int rmax = nj + 2;
 
-    auto at = [&](int r) { return (long long)(r + 1) * pitch + (xc + 1); };
+      auto at = [&](int r) { return (long long)(r + 1) * pitch + (xc + 1); };
 
-    // ---- pipeline warm-up: rows y0-2, y0-1, y0 ----------------------------
-    constexpr int kPrefetchRows = sizeof(FT) == 8 ? Q2D_PREFETCH_ROWS_FP64 : Q2D_PREFETCH_ROWS_MIXED;
-    double qA[4], qB[4], qC[4], qN[4], qN2[4];
-    double dlyA, dlyB, dlyC, dlyN, dlyN2 = 0.0;
-    FT sAB[4], phiA[4], Gprev[4];
-    RowGeo gcur;                                       // row r: i-face and update data, loaded at the top of its iteration
-    double fjxN, fjyN, fjlN;                           // j-face geometry of the next row (one row ahead)
-    {
-      double qm[4];
-      const long long o0 = at(item.y0 - 2), o1 = at(item.y0 - 1), o2 = at(item.y0), o3 = at(item.y0 + 1);
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        qm[v] = Pin[v * pl + o0]; qA[v] = Pin[v * pl + o1]; qB[v] = Pin[v * pl + o2]; qN[v] = Pin[v * pl + o3];
-      }
-      if (kPrefetchRows == 2) {
-        const long long o4 = at(min(item.y0 + 2, rmax));
-#pragma unroll
-        for (int v = 0; v < 4; ++v) qN2[v] = Pin[v * pl + o4];
-        dlyN2 = dlyp[o4];
-      }
-      const double dlm = dlyp[o0];
-      dlyA = dlyp[o1]; dlyB = dlyp[o2]; dlyN = dlyp[o3];
-      { const long long og = at(item.y0 - 1); fjxN = fjxp[og]; fjyN = fjyp[og]; fjlN = fjlp[og]; }
-      const FT l0 = (FT)(0.5 * (dlm + dlyA)), l1 = (FT)(0.5 * (dlyA + dlyB));
-      FT s0[4];
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        s0[v] = qdiv<FT>((FT)(qA[v] - qm[v]), l0);
-        sAB[v] = qdiv<FT>((FT)(qB[v] - qA[v]), l1);
-        Gprev[v] = FT(0);
-      }
-      limiter4<FT>(lim, sAB, s0, phiA);
-    }
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    double dtnext = 1e300;
-    double Eprev[3] = {0.0, 0.0, 0.0};                 // diffusive flux of j-face r-1 (VISC)
-
-    for (int r = item.y0 - 1; r <= item.y1; ++r) {
-      if ((lface || rface) && r >= item.y0 && ((r - item.y0) & 15) == 0 && lane < 32) {
-        // lanes 0-15: face 1, lanes 16-31: face 2, rows r .. r+15 (empty past the item)
-        const int kq = lane & 15, fsel = lane >> 4, row = r + kq;
-        const bool use = (fsel == 0 ? lface : rface) && row <= item.y1;
-        BCFlux& dst = sbq[wslot][fsel][kq];
-        if (use) {
-          const BCFlux& src = bcflux[B.bcf[fsel] + row - 1];
-#pragma unroll
-          for (int v = 0; v < 4; ++v) { dst.F[v] = src.F[v]; dst.V[v] = src.V[v]; }
-          dst.mask = src.mask; dst.vmask = src.vmask;
-        } else {
-          dst.mask = 0; dst.vmask = 0;
-        }
-      }
-      // software pipeline: this iteration's row r+2 was loaded one iteration ago; issue row r+3 and
-      // the next row's j-face geometry now, to be consumed by the next iteration.  The rest of row r's
-      // data (i-face geometry, sur, dt, RK base state) is issued here and first used after the j-face
-      // flux, which hides its latency without holding a second row of it in registers.
-#pragma unroll
-      for (int v = 0; v < 4; ++v) qC[v] = qN[v];
-      dlyC = dlyN;
-      gcur.fjx = fjxN; gcur.fjy = fjyN; gcur.fjl = fjlN;
+      // ---- pipeline warm-up: rows y0-2, y0-1, y0 ----------------------------
+      constexpr int kPrefetchRows = sizeof(FT) == 8 ? Q2D_PREFETCH_ROWS_FP64 : Q2D_PREFETCH_ROWS_MIXED;
+      double qA[4], qB[4], qC[4], qN[4], qN2[4];
+      double dlyA, dlyB, dlyC, dlyN, dlyN2 = 0.0;
+      FT sAB[4], phiA[4], Gprev[4];
+      RowGeo gcur;                                       // row r: i-face and update data, loaded at the top of its iteration
+      double fjxN, fjyN, fjlN;                           // j-face geometry of the next row (one row ahead)
       {
-        // row r's own data first: vector-memory waits are in issue order, so data consumed in this
-        // iteration is issued before the prefetch consumed in the next one
-        if (r >= item.y0) {
-          const long long oc = at(r);
-          gcur.dlx = dlxp[oc]; gcur.fix = fixp[oc]; gcur.fiy = fiyp[oc]; gcur.fil = filp[oc]; gcur.sur = surp[oc];
-          gcur.dtl = (own && !tacc) ? dtlp[oc] : 0.0;
-          if (irk != 1) {                              // stage 1 reads its RK base state as its input (qA)
+        double qm[4];
+        const long long o0 = at(item.y0 - 2), o1 = at(item.y0 - 1), o2 = at(item.y0), o3 = at(item.y0 + 1);
 #pragma unroll
-            for (int v = 0; v < 4; ++v) gcur.po[v] = PO[v * pl + oc];
-          }
-          if (FUSE && last) {                          // M of row r for the fused dt (c11, c21, c12, c22)
+        for (int v = 0; v < 4; ++v) {
+          qm[v] = Pin[v * pl + o0]; qA[v] = Pin[v * pl + o1]; qB[v] = Pin[v * pl + o2]; qN[v] = Pin[v * pl + o3];
+        }
+        if (kPrefetchRows == 2) {
+          const long long o4 = at(min(item.y0 + 2, rmax));
 #pragma unroll
-            for (int v = 0; v < 4; ++v) gcur.mm[v] = m11p[v * pl + oc];
+          for (int v = 0; v < 4; ++v) qN2[v] = Pin[v * pl + o4];
+          dlyN2 = dlyp[o4];
+        }
+        const double dlm = dlyp[o0];
+        dlyA = dlyp[o1]; dlyB = dlyp[o2]; dlyN = dlyp[o3];
+        { const long long og = at(item.y0 - 1); fjxN = fjxp[og]; fjyN = fjyp[og]; fjlN = fjlp[og]; }
+        const FT l0 = (FT)(0.5 * (dlm + dlyA)), l1 = (FT)(0.5 * (dlyA + dlyB));
+        FT s0[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          s0[v] = qdiv<FT>((FT)(qA[v] - qm[v]), l0);
+          sAB[v] = qdiv<FT>((FT)(qB[v] - qA[v]), l1);
+          Gprev[v] = FT(0);
+        }
+        limiter4<FT>(lim, sAB, s0, phiA);
+      }
+      double acc[4] = {0.0, 0.0, 0.0, 0.0};
+      double dtnext = 1e300;
+      double Eprev[3] = {0.0, 0.0, 0.0};                 // diffusive flux of j-face r-1 (VISC)
+
+      for (int r = item.y0 - 1; r <= item.y1; ++r) {
+        if ((lface || rface) && r >= item.y0 && ((r - item.y0) & 15) == 0 && lane < 32) {
+          // lanes 0-15: face 1, lanes 16-31: face 2, rows r .. r+15 (empty past the item)
+          const int kq = lane & 15, fsel = lane >> 4, row = r + kq;
+          const bool use = (fsel == 0 ? lface : rface) && row <= item.y1;
+          BCFlux& dst = sbq[wslot][fsel][kq];
+          if (use) {
+            const BCFlux& src = bcflux[B.bcf[fsel] + row - 1];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) { dst.F[v] = src.F[v]; dst.V[v] = src.V[v]; }
+            dst.mask = src.mask; dst.vmask = src.vmask;
+          } else {
+            dst.mask = 0; dst.vmask = 0;
           }
         }
-        if (kPrefetchRows == 2) {                      // rows r+3 (in flight since the last iteration) and r+4
+        // software pipeline: this iteration's row r+2 was loaded one iteration ago; issue row r+3 and
+        // the next row's j-face geometry now, to be consumed by the next iteration.  The rest of row r's
+        // data (i-face geometry, sur, dt, RK base state) is issued here and first used after the j-face
+        // flux, which hides its latency without holding a second row of it in registers.
 #pragma unroll
-          for (int v = 0; v < 4; ++v) qN[v] = qN2[v];
-          dlyN = dlyN2;
-          const long long oN = at(min(r + 4, rmax));
-#pragma unroll
-          for (int v = 0; v < 4; ++v) qN2[v] = Pin[v * pl + oN];
-          dlyN2 = dlyp[oN];
-        } else {
-          const long long oN = at(min(r + 3, rmax));
-#pragma unroll
-          for (int v = 0; v < 4; ++v) qN[v] = Pin[v * pl + oN];
-          dlyN = dlyp[oN];
-        }
-        const long long og = at(min(r + 1, rmax));
-        fjxN = fjxp[og]; fjyN = fjyp[og]; fjlN = fjlp[og];
-      }
-      FT sBC[4], phiB[4], Gcur[4];
-      {
-        const FT lBC = (FT)(0.5 * (dlyB + dlyC));
-#pragma unroll
-        for (int v = 0; v < 4; ++v) sBC[v] = qdiv<FT>((FT)(qC[v] - qB[v]), lBC);
-        limiter4<FT>(lim, sBC, sAB, phiB);            // phi_{r+1} = rlimiter(s(r+1|r+2), s(r|r+1))
-      }
-      double Ecur[3] = {0.0, 0.0, 0.0};
-      if (VISC && own && r >= 1 && r <= nj - 1) {
-        const long long oE = at(r);
-#pragma unroll
-        for (int k = 0; k < 3; ++k) Ecur[k] = gp(B.vey)[k * pl + oE];
-      }
-      // interior j-face (r | r+1) — Mod_Fluxes.f90:187-232
-      if (own && r >= 1 && r <= nj - 1) {
-        if (face_flux<FT, RIE>(gf, qA, phiA, (FT)(0.5 * dlyA), qB, phiB, (FT)(0.5 * dlyB), (FT)gcur.fjx,
-                               (FT)gcur.fjy, (FT)gcur.fjl, Gcur))
-          record_fail(&K, Q2D_E_INTERFACE, fail_key(stage_id, 0, B.blk_in_case, r, x));
-      } else {
-#pragma unroll
-        for (int v = 0; v < 4; ++v) Gcur[v] = FT(0);
-      }
-
-      if (r >= item.y0) {
-        // ---- i-direction on row r (lanes share the stencil through DPP shifts) ----
-        const double dlx = gcur.dlx;
-        double qn[4];
-#pragma unroll
-        for (int v = 0; v < 4; ++v) qn[v] = shdn(qA[v]);
-        const double dlxn = shdn(dlx);
-        FT sx[4], sxp[4], phx[4], phxn[4];
+        for (int v = 0; v < 4; ++v) qC[v] = qN[v];
+        dlyC = dlyN;
+        gcur.fjx = fjxN; gcur.fjy = fjyN; gcur.fjl = fjlN;
         {
-          const FT l1 = (FT)(0.5 * (dlx + dlxn));
+          // row r's own data first: vector-memory waits are in issue order, so data consumed in this
+          // iteration is issued before the prefetch consumed in the next one
+          if (r >= item.y0) {
+            const long long oc = at(r);
+            gcur.dlx = dlxp[oc]; gcur.fix = fixp[oc]; gcur.fiy = fiyp[oc]; gcur.fil = filp[oc]; gcur.sur = surp[oc];
+            gcur.dtl = (own && !tacc) ? dtlp[oc] : 0.0;
+            if (irk != 1) {                              // stage 1 reads its RK base state as its input (qA)
 #pragma unroll
-          for (int v = 0; v < 4; ++v) sx[v] = qdiv<FT>((FT)(qn[v] - qA[v]), l1);
+              for (int v = 0; v < 4; ++v) gcur.po[v] = PO[v * pl + oc];
+            }
+            if (FUSE && last) {                          // M of row r for the fused dt (c11, c21, c12, c22)
+#pragma unroll
+              for (int v = 0; v < 4; ++v) gcur.mm[v] = m11p[v * pl + oc];
+            }
+          }
+          if (kPrefetchRows == 2) {                      // rows r+3 (in flight since the last iteration) and r+4
+#pragma unroll
+            for (int v = 0; v < 4; ++v) qN[v] = qN2[v];
+            dlyN = dlyN2;
+            const long long oN = at(min(r + 4, rmax));
+#pragma unroll
+            for (int v = 0; v < 4; ++v) qN2[v] = Pin[v * pl + oN];
+            dlyN2 = dlyp[oN];
+          } else {
+            const long long oN = at(min(r + 3, rmax));
+#pragma unroll
+            for (int v = 0; v < 4; ++v) qN[v] = Pin[v * pl + oN];
+            dlyN = dlyp[oN];
+          }
+          const long long og = at(min(r + 1, rmax));
+          fjxN = fjxp[og]; fjyN = fjyp[og]; fjlN = fjlp[og];
         }
+        FT sBC[4], phiB[4], Gcur[4];
+        {
+          const FT lBC = (FT)(0.5 * (dlyB + dlyC));
 #pragma unroll
-        for (int v = 0; v < 4; ++v) sxp[v] = shup(sx[v]);
-        limiter4<FT>(lim, sx, sxp, phx);
+          for (int v = 0; v < 4; ++v) sBC[v] = qdiv<FT>((FT)(qC[v] - qB[v]), lBC);
+          limiter4<FT>(lim, sBC, sAB, phiB);            // phi_{r+1} = rlimiter(s(r+1|r+2), s(r|r+1))
+        }
+        double Ecur[3] = {0.0, 0.0, 0.0};
+        if (VISC && own && r >= 1 && r <= nj - 1) {
+          const long long oE = at(r);
 #pragma unroll
-        for (int v = 0; v < 4; ++v) phxn[v] = shdn(phx[v]);
-        FT Fx[4];
-        if (xdo) {                                     // interior i-face (x|x+1) — Mod_Fluxes.f90:140-185
-          if (face_flux<FT, RIE>(gf, qA, phx, (FT)(0.5 * dlx), qn, phxn, (FT)(0.5 * dlxn), (FT)gcur.fix,
-                                 (FT)gcur.fiy, (FT)gcur.fil, Fx))
+          for (int k = 0; k < 3; ++k) Ecur[k] = gp(B.vey)[k * pl + oE];
+        }
+        // interior j-face (r | r+1) — Mod_Fluxes.f90:187-232
+        if (own && r >= 1 && r <= nj - 1) {
+          if (face_flux<FT, RIE>(gf, qA, phiA, (FT)(0.5 * dlyA), qB, phiB, (FT)(0.5 * dlyB), (FT)gcur.fjx,
+                                 (FT)gcur.fjy, (FT)gcur.fjl, Gcur))
             record_fail(&K, Q2D_E_INTERFACE, fail_key(stage_id, 0, B.blk_in_case, r, x));
         } else {
 #pragma unroll
-          for (int v = 0; v < 4; ++v) Fx[v] = FT(0);
+          for (int v = 0; v < 4; ++v) Gcur[v] = FT(0);
         }
-        FT Fl[4];
-#pragma unroll
-        for (int v = 0; v < 4; ++v) Fl[v] = shup(Fx[v]);
 
-        double Dx[3] = {0.0, 0.0, 0.0}, Dl[3] = {0.0, 0.0, 0.0};
-        if (VISC) {                                    // diffusive flux of i-face x, and of x-1 from the left lane
-          const long long oD = at(r);
+        if (r >= item.y0) {
+          // ---- i-direction on row r (lanes share the stencil through DPP shifts) ----
+          const double dlx = gcur.dlx;
+          double qn[4];
 #pragma unroll
-          for (int k = 0; k < 3; ++k) Dx[k] = gp(B.vdx)[k * pl + oD];
+          for (int v = 0; v < 4; ++v) qn[v] = shdn(qA[v]);
+          const double dlxn = shdn(dlx);
+          FT sx[4], sxp[4], phx[4], phxn[4];
+          {
+            const FT l1 = (FT)(0.5 * (dlx + dlxn));
 #pragma unroll
-          for (int k = 0; k < 3; ++k) Dl[k] = shup(Dx[k]);
-        }
-        if (own) {
-          const long long oA = at(r);
-          // ---- residual in the reference's accumulation order ------------
-          // per face: convective flux (Convective_Flux), then diffusive flux (Diffusive_Flux: Res1 -= F,
-          // Res2 += F; its density component is 0)
-          double res[4];
-#pragma unroll
-          for (int v = 0; v < 4; ++v) {
-            double rv = 0.0;
-            if (x >= 2) {                                   // face x-1 (Mod_Fluxes.f90:140-185)
-              rv = 0.0 - (double)Fl[v];
-              if (VISC) rv = rv + (v == 0 ? 0.0 : Dl[v - 1]);
-            }
-            if (x <= ni - 1) {                              // face x
-              rv = rv + (double)Fx[v];
-              if (VISC) rv = rv - (v == 0 ? 0.0 : Dx[v - 1]);
-            }
-            if (r >= 2) {                                   // face r-1 (:187-232)
-              rv = rv - (double)Gprev[v];
-              if (VISC) rv = rv + (v == 0 ? 0.0 : Eprev[v - 1]);
-            }
-            if (r <= nj - 1) {                              // face r
-              rv = rv + (double)Gcur[v];
-              if (VISC) rv = rv - (v == 0 ? 0.0 : Ecur[v - 1]);
-            }
-            res[v] = rv;
+            for (int v = 0; v < 4; ++v) sx[v] = qdiv<FT>((FT)(qn[v] - qA[v]), l1);
           }
-          // ---- BC faces 1..4 in record order (Mod_BC_Fluxes.f90:79-86): convective, then viscous ----
-          auto add_bc = [&](const BCFlux& q) {
 #pragma unroll
-            for (int v = 0; v < 4; ++v) res[v] = ((q.mask >> v) & 1) ? res[v] + q.F[v] : res[v];
-            if (VISC) {
+          for (int v = 0; v < 4; ++v) sxp[v] = shup(sx[v]);
+          limiter4<FT>(lim, sx, sxp, phx);
 #pragma unroll
-              for (int v = 0; v < 4; ++v) res[v] = ((q.vmask >> v) & 1) ? res[v] + q.V[v] : res[v];
-            }
-          };
-          const int kq = (r - item.y0) & 15;
-          if (lface && x == 1) add_bc(sbq[wslot][0][kq]);
-          if (rface && x == ni) add_bc(sbq[wslot][1][kq]);
-          if (r == 1 && B.bcf[2] >= 0) add_bc(bcflux[B.bcf[2] + x - 1]);
-          if (r == nj && B.bcf[3] >= 0) add_bc(bcflux[B.bcf[3] + x - 1]);
-          // ---- geometric sources — Lib_Source_Axisymmetry.f90:40-78, Lib_Source_Areavariation.f90:104-144
-          const double sr = gcur.sur;
-          const Gas<double> gd{C.R, C.cp, C.cv, C.gam};
-          if (C.source != Q2D_SRC_NONE) {
-            const double rho = qA[0], u = qA[1], v = qA[2], p = qA[3];
-            const double mod_v = sqrt(u * u + v * v);
-            const double E0 = gd.cv * (p / (rho * gd.R)) + 0.5 * mod_v * mod_v;
-            double S[4];
-            if (C.source == Q2D_SRC_AXI) {
-              const double invr = gp(B.sb)[oA];
-              S[0] = -(rho * v * invr);
-              S[1] = -(rho * u * v * invr);
-              S[2] = -(rho * v * v * invr);
-              S[3] = -((rho * E0 + p) * v * invr);
-            } else {
-              const double VdG = u * gp(B.sa)[oA] + v * gp(B.sb)[oA];
-              S[0] = rho * VdG;
-              S[1] = rho * u * VdG;
-              S[2] = rho * v * VdG;
-              S[3] = (rho * E0 + p) * VdG;
-            }
+          for (int v = 0; v < 4; ++v) phxn[v] = shdn(phx[v]);
+          FT Fx[4];
+          if (xdo) {                                     // interior i-face (x|x+1) — Mod_Fluxes.f90:140-185
+            if (face_flux<FT, RIE>(gf, qA, phx, (FT)(0.5 * dlx), qn, phxn, (FT)(0.5 * dlxn), (FT)gcur.fix,
+                                   (FT)gcur.fiy, (FT)gcur.fil, Fx))
+              record_fail(&K, Q2D_E_INTERFACE, fail_key(stage_id, 0, B.blk_in_case, r, x));
+          } else {
 #pragma unroll
-            for (int k = 0; k < 4; ++k) res[k] = res[k] - S[k] * sr;
-            if (VISC) {                                 // Source_Axi_Visc_Blk / Source_Area_Visc_Blk, after the Euler source
-#pragma unroll
-              for (int k = 1; k < 4; ++k) res[k] = res[k] - gp(B.vsv)[(k - 1) * pl + oA] * sr;
-            }
+            for (int v = 0; v < 4; ++v) Fx[v] = FT(0);
           }
-          // ---- Compute_Cons_NoIRS — Lib_Newstate.f90:131-159 -------------
-          const double dt = tacc ? dtg : gcur.dtl;
-          if (irk == 1) {                              // RK base state == stage input on stage 1
+          FT Fl[4];
 #pragma unroll
-            for (int v = 0; v < 4; ++v) gcur.po[v] = qA[v];
+          for (int v = 0; v < 4; ++v) Fl[v] = shup(Fx[v]);
+
+          double Dx[3] = {0.0, 0.0, 0.0}, Dl[3] = {0.0, 0.0, 0.0};
+          if (VISC) {                                    // diffusive flux of i-face x, and of x-1 from the left lane
+            const long long oD = at(r);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) Dx[k] = gp(B.vdx)[k * pl + oD];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) Dl[k] = shup(Dx[k]);
           }
-          double Uo[4], U[4], qn2[4];
-          prim2cons(gd, gcur.po[0], gcur.po[1], gcur.po[2], gcur.po[3], Uo);
-          prim2cons(gd, qA[0], qA[1], qA[2], qA[3], U);
-          if (sizeof(FT) == 8) {
+          if (own) {
+            const long long oA = at(r);
+            // ---- residual in the reference's accumulation order ------------
+            // per face: convective flux (Convective_Flux), then diffusive flux (Diffusive_Flux: Res1 -= F,
+            // Res2 += F; its density component is 0)
+            double res[4];
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-              const double rr = -res[v] / sr * dt;
-              U[v] = Uo[v] + (U[v] - Uo[v] + rr) * c_rk;   // RK_stage — Lib_RK.f90:15-27
+              double rv = 0.0;
+              if (x >= 2) {                                   // face x-1 (Mod_Fluxes.f90:140-185)
+                rv = 0.0 - (double)Fl[v];
+                if (VISC) rv = rv + (v == 0 ? 0.0 : Dl[v - 1]);
+              }
+              if (x <= ni - 1) {                              // face x
+                rv = rv + (double)Fx[v];
+                if (VISC) rv = rv - (v == 0 ? 0.0 : Dx[v - 1]);
+              }
+              if (r >= 2) {                                   // face r-1 (:187-232)
+                rv = rv - (double)Gprev[v];
+                if (VISC) rv = rv + (v == 0 ? 0.0 : Eprev[v - 1]);
+              }
+              if (r <= nj - 1) {                              // face r
+                rv = rv + (double)Gcur[v];
+                if (VISC) rv = rv - (v == 0 ? 0.0 : Ecur[v - 1]);
+              }
+              res[v] = rv;
             }
-            cons2prim(gd, U, qn2);
-          } else {
-            // mixed form: FP64 update with one reciprocal of sur and of rho
-            const double dts = dt * rcp64(sr);
+            // ---- BC faces 1..4 in record order (Mod_BC_Fluxes.f90:79-86): convective, then viscous ----
+            auto add_bc = [&](const BCFlux& q) {
 #pragma unroll
-            for (int v = 0; v < 4; ++v) U[v] = Uo[v] + (U[v] - Uo[v] - res[v] * dts) * c_rk;
-            const double ir = rcp64(U[0]);
-            const double vx = U[1] * ir, vy = U[2] * ir;
-            const double e = U[3] * ir - 0.5 * (vx * vx + vy * vy);
-            qn2[0] = U[0]; qn2[1] = vx; qn2[2] = vy; qn2[3] = U[0] * (gd.gam - 1.0) * e;
-          }
-          // Check_And_Fix_State — Lib_Newstate.f90:316-350
-          if (__builtin_isnan(qn2[0]) || __builtin_isnan(qn2[1]) || __builtin_isnan(qn2[2]) ||
-              __builtin_isnan(qn2[3]) || qn2[3] < 0.0)
-            record_fail(&K, Q2D_E_STATE, fail_key(stage_id, 1, B.blk_in_case, r, x));
-          if (qn2[0] < 1e-20) qn2[0] = 1e-20;
+              for (int v = 0; v < 4; ++v) res[v] = ((q.mask >> v) & 1) ? res[v] + q.F[v] : res[v];
+              if (VISC) {
 #pragma unroll
-          for (int v = 0; v < 4; ++v) Pout[v * pl + oA] = qn2[v];
-          if (last) {                                        // Compute_Residual — Mod_Diagnostic.f90:452-458
+                for (int v = 0; v < 4; ++v) res[v] = ((q.vmask >> v) & 1) ? res[v] + q.V[v] : res[v];
+              }
+            };
+            const int kq = (r - item.y0) & 15;
+            if (lface && x == 1) add_bc(sbq[wslot][0][kq]);
+            if (rface && x == ni) add_bc(sbq[wslot][1][kq]);
+            if (r == 1 && B.bcf[2] >= 0) add_bc(bcflux[B.bcf[2] + x - 1]);
+            if (r == nj && B.bcf[3] >= 0) add_bc(bcflux[B.bcf[3] + x - 1]);
+            // ---- geometric sources — Lib_Source_Axisymmetry.f90:40-78, Lib_Source_Areavariation.f90:104-144
+            const double sr = gcur.sur;
+            const Gas<double> gd{C.R, C.cp, C.cv, C.gam};
+            if (C.source != Q2D_SRC_NONE) {
+              const double rho = qA[0], u = qA[1], v = qA[2], p = qA[3];
+              const double mod_v = sqrt(u * u + v * v);
+              const double E0 = gd.cv * (p / (rho * gd.R)) + 0.5 * mod_v * mod_v;
+              double S[4];
+              if (C.source == Q2D_SRC_AXI) {
+                const double invr = gp(B.sb)[oA];
+                S[0] = -(rho * v * invr);
+                S[1] = -(rho * u * v * invr);
+                S[2] = -(rho * v * v * invr);
+                S[3] = -((rho * E0 + p) * v * invr);
+              } else {
+                const double VdG = u * gp(B.sa)[oA] + v * gp(B.sb)[oA];
+                S[0] = rho * VdG;
+                S[1] = rho * u * VdG;
+                S[2] = rho * v * VdG;
+                S[3] = (rho * E0 + p) * VdG;
+              }
 #pragma unroll
-            for (int v = 0; v < 4; ++v) { const double d = fabs(qn2[v] - gcur.po[v]); acc[v] = acc[v] + d * d; }
-          }
-          if (last && FUSE) {
-            // Compute_dt of the next step (Mod_dt.f90:59-104) on the state just written; the
-            // next step's Compute_dt would read exactly these values.
-            const double snd = sqrt(C.gam * qn2[3] / qn2[0]);
-            double dtmin = 1e8;
-            const double* mm = gcur.mm;                 // c11, c21, c12, c22
-            {
-              const double m1 = mm[0], m2 = mm[2];
-              const double nrm = sqrt(m1 * m1 + m2 * m2);
-              const double v1 = m1 / nrm, v2 = m2 / nrm;
-              const double lambda = fabs(qn2[1] * v1 + qn2[2] * v2) + snd;
-              const double dtd = dlx / lambda * C.cfl;
-              if (dtd < dtmin) dtmin = dtd;
+              for (int k = 0; k < 4; ++k) res[k] = res[k] - S[k] * sr;
+              if (VISC) {                                 // Source_Axi_Visc_Blk / Source_Area_Visc_Blk, after the Euler source
+#pragma unroll
+                for (int k = 1; k < 4; ++k) res[k] = res[k] - gp(B.vsv)[(k - 1) * pl + oA] * sr;
+              }
             }
-            if (C.ndir == 2) {
-              const double m1 = mm[1], m2 = mm[3];
-              const double nrm = sqrt(m1 * m1 + m2 * m2);
-              const double v1 = m1 / nrm, v2 = m2 / nrm;
-              const double lambda = fabs(qn2[1] * v1 + qn2[2] * v2) + snd;
-              const double dtd = dlyA / lambda * C.cfl;
-              if (dtd < dtmin) dtmin = dtd;
+            // ---- Compute_Cons_NoIRS — Lib_Newstate.f90:131-159 -------------
+            const double dt = tacc ? dtg : gcur.dtl;
+            if (irk == 1) {                              // RK base state == stage input on stage 1
+#pragma unroll
+              for (int v = 0; v < 4; ++v) gcur.po[v] = qA[v];
             }
-            const int iter_next = kiter + 1;
-            if (iter_next < C.ramp_iter) dtmin = dtmin * (double)iter_next / (double)C.ramp_iter;
-            if (!tacc) gpw(B.dtl[kdtsel ^ 1])[oA] = dtmin;
-            if (dtmin < dtnext) dtnext = dtmin;
+            double Uo[4], U[4], qn2[4];
+            prim2cons(gd, gcur.po[0], gcur.po[1], gcur.po[2], gcur.po[3], Uo);
+            prim2cons(gd, qA[0], qA[1], qA[2], qA[3], U);
+            if (sizeof(FT) == 8) {
+#pragma unroll
+              for (int v = 0; v < 4; ++v) {
+                const double rr = -res[v] / sr * dt;
+                U[v] = Uo[v] + (U[v] - Uo[v] + rr) * c_rk;   // RK_stage — Lib_RK.f90:15-27
+              }
+              cons2prim(gd, U, qn2);
+            } else {
+              // mixed form: FP64 update with one reciprocal of sur and of rho
+              const double dts = dt * rcp64(sr);
+#pragma unroll
+              for (int v = 0; v < 4; ++v) U[v] = Uo[v] + (U[v] - Uo[v] - res[v] * dts) * c_rk;
+              const double ir = rcp64(U[0]);
+              const double vx = U[1] * ir, vy = U[2] * ir;
+              const double e = U[3] * ir - 0.5 * (vx * vx + vy * vy);
+              qn2[0] = U[0]; qn2[1] = vx; qn2[2] = vy; qn2[3] = U[0] * (gd.gam - 1.0) * e;
+            }
+            // Check_And_Fix_State — Lib_Newstate.f90:316-350
+            if (__builtin_isnan(qn2[0]) || __builtin_isnan(qn2[1]) || __builtin_isnan(qn2[2]) ||
+                __builtin_isnan(qn2[3]) || qn2[3] < 0.0)
+              record_fail(&K, Q2D_E_STATE, fail_key(stage_id, 1, B.blk_in_case, r, x));
+            if (qn2[0] < 1e-20) qn2[0] = 1e-20;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) Pout[v * pl + oA] = qn2[v];
+            if (last) {                                        // Compute_Residual — Mod_Diagnostic.f90:452-458
+#pragma unroll
+              for (int v = 0; v < 4; ++v) { const double d = fabs(qn2[v] - gcur.po[v]); acc[v] = acc[v] + d * d; }
+            }
+            if (last && FUSE) {
+              // Compute_dt of the next step (Mod_dt.f90:59-104) on the state just written; the
+              // next step's Compute_dt would read exactly these values.
+              const double snd = sqrt(C.gam * qn2[3] / qn2[0]);
+              double dtmin = 1e8;
+              const double* mm = gcur.mm;                 // c11, c21, c12, c22
+              {
+                const double m1 = mm[0], m2 = mm[2];
+                const double nrm = sqrt(m1 * m1 + m2 * m2);
+                const double v1 = m1 / nrm, v2 = m2 / nrm;
+                const double lambda = fabs(qn2[1] * v1 + qn2[2] * v2) + snd;
+                const double dtd = dlx / lambda * C.cfl;
+                if (dtd < dtmin) dtmin = dtd;
+              }
+              if (C.ndir == 2) {
+                const double m1 = mm[1], m2 = mm[3];
+                const double nrm = sqrt(m1 * m1 + m2 * m2);
+                const double v1 = m1 / nrm, v2 = m2 / nrm;
+                const double lambda = fabs(qn2[1] * v1 + qn2[2] * v2) + snd;
+                const double dtd = dlyA / lambda * C.cfl;
+                if (dtd < dtmin) dtmin = dtd;
+              }
+              const int iter_next = kiter + 1;
+              if (iter_next < C.ramp_iter) dtmin = dtmin * (double)iter_next / (double)C.ramp_iter;
+              if (!tacc) gpw(B.dtl[kdtsel ^ 1])[oA] = dtmin;
+              if (dtmin < dtnext) dtnext = dtmin;
+            }
           }
         }
-      }
-      // ---- roll the j-stencil -------------------------------------------
+        // ---- roll the j-stencil -------------------------------------------
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        qA[v] = qB[v]; qB[v] = qC[v];
-        sAB[v] = sBC[v]; phiA[v] = phiB[v]; Gprev[v] = Gcur[v];
-      }
-      if (VISC) {
+        for (int v = 0; v < 4; ++v) {
+          qA[v] = qB[v]; qB[v] = qC[v];
+          sAB[v] = sBC[v]; phiA[v] = phiB[v]; Gprev[v] = Gcur[v];
+        }
+        if (VISC) {
 #pragma unroll
-        for (int k = 0; k < 3; ++k) Eprev[k] = Ecur[k];
+          for (int k = 0; k < 3; ++k) Eprev[k] = Ecur[k];
+        }
+        dlyA = dlyB; dlyB = dlyC;
       }
-      dlyA = dlyB; dlyB = dlyC;
-    }
-    if (last) {
-      // fixed-pattern wavefront sum -> one partial per work item (deterministic)
+      if (last) {
+        // fixed-pattern wavefront sum -> one partial per work item (deterministic)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        double s = acc[v];
-        for (int k = 32; k >= 1; k >>= 1) s = s + __shfl_xor(s, k, kWave);
-        if (lane == 0) partials[(long long)it * kPartials + v] = s;
+        for (int v = 0; v < 4; ++v) {
+          double s = acc[v];
+          for (int k = 32; k >= 1; k >>= 1) s = s + __shfl_xor(s, k, kWave);
+          if (lane == 0) partials[(long long)it * kPartials + v] = s;
+        }
       }
-    }
-    if (last && FUSE) {
-      // next step's dt: the item's exact min, reduced over the case's items by k_finish (no atomics)
-      for (int k = 32; k >= 1; k >>= 1) {
-        const double o = __shfl_xor(dtnext, k, kWave);
-        if (o < dtnext) dtnext = o;
+      if (last && FUSE) {
+        // next step's dt: the item's exact min, reduced over the case's items by k_finish (no atomics)
+        for (int k = 32; k >= 1; k >>= 1) {
+          const double o = __shfl_xor(dtnext, k, kWave);
+          if (o < dtnext) dtnext = o;
+        }
+        if (lane == 0) partials[(long long)it * kPartials + 4] = dtnext;
       }
-      if (lane == 0) partials[(long long)it * kPartials + 4] = dtnext;
+    };
+    if constexpr (RIE_T < 0) {
+      if (C.riemann == Q2D_RIEMANN_HLLE) item_body(std::integral_constant<int, 1>{});
+      else item_body(std::integral_constant<int, 0>{});
+    } else {
+      item_body(std::integral_constant<int, RIE_T>{});
     }
   }
   if (tslotp && lane == 0) atomicMax(&tslotp[1], (unsigned long long)wall_clock64());
@@ -2099,20 +2109,11 @@ namespace {
 
 // The stage-kernel instantiation that runs the items of `pass` (2 * viscous + riemann).
 const void* stage_kernel(const q2d_handle* h, int pass) {
-  if (h->precision_mixed) {
-    switch (pass) {
-      case 0:
-      case 1: return reinterpret_cast<const void*>(&k_stage_mx<-1, kFuseDtMx>);
-      case 2: return reinterpret_cast<const void*>(&k_stage<float, 0, false, true>);
-      default: return reinterpret_cast<const void*>(&k_stage<float, 1, false, true>);
-    }
-  }
-  switch (pass) {
-    case 0: return reinterpret_cast<const void*>(&k_stage<double, 0, kFuseDtFp64, false>);
-    case 1: return reinterpret_cast<const void*>(&k_stage<double, 1, kFuseDtFp64, false>);
-    case 2: return reinterpret_cast<const void*>(&k_stage<double, 0, kFuseDtFp64, true>);
-    default: return reinterpret_cast<const void*>(&k_stage<double, 1, kFuseDtFp64, true>);
-  }
+  if (h->precision_mixed)
+    return pass < 2 ? reinterpret_cast<const void*>(&k_stage_mx<-1, kFuseDtMx>)
+                    : reinterpret_cast<const void*>(&k_stage<float, -1, false, true>);
+  return pass < 2 ? reinterpret_cast<const void*>(&k_stage<double, -1, kFuseDtFp64, false>)
+                  : reinterpret_cast<const void*>(&k_stage<double, -1, kFuseDtFp64, true>);
 }
 
 // The captured step graphs hold the launch arguments; they are dropped whenever those change.
@@ -2252,11 +2253,9 @@ template <typename FT, bool FUSE>
 int launch_stage(q2d_handle* h, int irk) {
   const int tpb = kWave * h->waves_per_group;
   for (int pass = 0; pass < 4; ++pass) {
-    int lo = h->range[pass], hi = h->range[pass + 1];
-    if constexpr (sizeof(FT) == 4) {                   // mixed form: the Euler passes 0 and 1 share one launch
-      if (pass == 1) continue;
-      if (pass == 0) hi = h->range[2];
-    }
+    // the two Riemann solvers' passes of each kernel family (Euler 0-1, laminar 2-3) share one launch
+    if (pass == 1 || pass == 3) continue;
+    const int lo = h->range[pass], hi = h->range[pass + 2];
     if (hi <= lo) continue;
     int groups = (hi - lo + h->waves_per_group - 1) / h->waves_per_group;
     if (h->max_groups > 0) groups = std::min(groups, h->max_groups);
@@ -2277,18 +2276,11 @@ int launch_stage(q2d_handle* h, int irk) {
     hipLaunchKernelGGL((k_stage_mx<RIE, kFuseDtMx>), dim3(groups), dim3(tpb), 0, h->stream, h->d_blocks, h->d_cases, \
                        h->d_ctrl, h->d_items, lo, hi, h->d_bcdense, h->d_partials, irk, h->xcd_remap, ts, slot)
     if constexpr (sizeof(FT) == 4) {                   // mixed form: Euler items run k_stage_mx
-      switch (pass) {
-        case 0: Q2D_STAGE_MX_LAUNCH(-1); break;         // items of passes 0 and 1 (HLLC and HLLE)
-        case 2: Q2D_STAGE_LAUNCH(0, true); break;
-        default: Q2D_STAGE_LAUNCH(1, true); break;
-      }
+      if (pass == 0) Q2D_STAGE_MX_LAUNCH(-1);
+      else Q2D_STAGE_LAUNCH(-1, true);
     } else {
-      switch (pass) {
-        case 0: Q2D_STAGE_LAUNCH(0, false); break;
-        case 1: Q2D_STAGE_LAUNCH(1, false); break;
-        case 2: Q2D_STAGE_LAUNCH(0, true); break;
-        default: Q2D_STAGE_LAUNCH(1, true); break;
-      }
+      if (pass == 0) Q2D_STAGE_LAUNCH(-1, false);
+      else Q2D_STAGE_LAUNCH(-1, true);
     }
 #undef Q2D_STAGE_LAUNCH
 #undef Q2D_STAGE_MX_LAUNCH
