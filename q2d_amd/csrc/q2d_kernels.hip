@@ -2622,7 +2622,11 @@ int append_cases(q2d_handle* h, const q2d_problem* prob, int32_t* first_case) {
     }
 
   // ---- per-case constants and control ----
-  if (!h->prec_set) { h->precision_mixed = prec == Q2D_PREC_MIXED; h->prec_set = true; }
+  if (!h->prec_set) {
+    h->precision_mixed = prec == Q2D_PREC_MIXED; h->prec_set = true;
+    // default workgroup: FP64 4 wavefronts; mixed 2 (measured ~1 % faster on 4 x 1024^2; tunable)
+    if (h->precision_mixed) h->waves_per_group = 2;
+  }
   for (int c = 0; c < prob->ncases; ++c) {
     const q2d_case_desc& cd = prob->cases[c];
     const q2d_scheme& s = cd.scheme;
