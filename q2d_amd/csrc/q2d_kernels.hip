@@ -98,6 +98,7 @@ struct DevBlock {
   int ni, nj, pitch, case_id;
   int blk_in_case, pad0;
   int bcf[4];                    // offset of the per-cell BC-record index list of faces 1..4 (-1: none)
+  int wallf[4];                  // faces 1..4: 0 no wall record (301-304), 1 every record a wall, 2 mixed
   long long plane;               // doubles per plane
   double* P[3];                  // three state buffers, 4 planes each
   const double *dlx, *dly, *sur;
@@ -828,8 +829,9 @@ __global__ __launch_bounds__(256) void k_visc(const DevBlock* __restrict__ block
     auto ldc = [&](int r) { const long long o = at(r); return VC{P[o], P[pl + o], P[2 * pl + o], P[3 * pl + o], 0.0}; };
     auto ldm = [&](int r, double M[4]) { const long long o = at(r); M[0] = m11[o]; M[1] = m21[o]; M[2] = m12[o]; M[3] = m22[o]; };
     // wall faces of the boundary cells of this column (BC 301..304)
-    auto is_wall = [&](int f, int pos) {
-      if (B.bcf[f] < 0) return false;
+    auto is_wall = [&](int f, int pos) {               // uniform faces need no record lookup
+      if (B.bcf[f] < 0 || B.wallf[f] == 0) return false;
+      if (B.wallf[f] == 1) return true;
       const int t = bcs[bcface[B.bcf[f] + pos - 1]].type;
       return t >= 301 && t <= 304;
     };
@@ -2480,6 +2482,9 @@ int rebuild_descriptors(q2d_handle* h) {
       for (int v : fl) if (v < 0) return fail(Q2D_E_ARG, "boundary face partially covered by bc records");
       h->db[b].bcf[f] = (int)h->bcface.size();
       h->bcface.insert(h->bcface.end(), fl.begin(), fl.end());
+      int nwall = 0;
+      for (int v : fl) nwall += (h->dbc[v].type >= 301 && h->dbc[v].type <= 304) ? 1 : 0;
+      h->db[b].wallf[f] = nwall == 0 ? 0 : (nwall == (int)fl.size() ? 1 : 2);
     }
   for (DevBC& r : h->dbc) r.dpos = h->db[r.blk].bcf[r.f - 1] + (r.f <= 2 ? r.j - 1 : r.i - 1);
   h->max_rk = 1;
@@ -2622,11 +2627,7 @@ int append_cases(q2d_handle* h, const q2d_problem* prob, int32_t* first_case) {
     }
 
   // ---- per-case constants and control ----
-  if (!h->prec_set) {
-    h->precision_mixed = prec == Q2D_PREC_MIXED; h->prec_set = true;
-    // default workgroup: FP64 4 wavefronts; mixed 2 (measured ~1 % faster on 4 x 1024^2; tunable)
-    if (h->precision_mixed) h->waves_per_group = 2;
-  }
+  if (!h->prec_set) { h->precision_mixed = prec == Q2D_PREC_MIXED; h->prec_set = true; }
   for (int c = 0; c < prob->ncases; ++c) {
     const q2d_case_desc& cd = prob->cases[c];
     const q2d_scheme& s = cd.scheme;
