@@ -105,6 +105,9 @@ typedef struct q2d_scheme {
   double  cfl, vnn;
   double  gas_R, gas_cp;   /* gas constant and constant cp */
   double  mu_ref, T_ref, S_suth, Pr;   /* Sutherland law mu = mu_ref (T/T_ref)^1.5 (T_ref+S)/(T+S); k = mu cp / Pr */
+  double  irs_beta;        /* irs-beta (Register_Numerics.f90:53): > 0 enables implicit residual smoothing
+                              (Assign_Setup.f90:61; Lib_IRS.f90:17-177 from Newstate_Conservative,
+                              Lib_Newstate.f90:35-64); 0 leaves it off */
 } q2d_scheme;
 
 /* One structured block, in the reference's allocation layout (see top). */

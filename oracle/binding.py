@@ -35,9 +35,11 @@ def load():
     lib.q2do_destroy.argtypes = [C.c_void_p]
     lib.q2do_step.argtypes = [C.c_void_p, C.POINTER(StepOut)]
     lib.q2do_step.restype = C.c_int
-    for f in ("q2do_get_state", "q2do_set_state", "q2do_get_dt"):
+    for f in ("q2do_get_state", "q2do_set_state", "q2do_get_dt", "q2do_get_residual", "q2do_set_residual"):
         getattr(lib, f).argtypes = [C.c_void_p, C.c_int, C.c_void_p]
         getattr(lib, f).restype = C.c_int
+    lib.q2do_irs_smooth.argtypes = [C.c_void_p, C.c_int, C.c_double]
+    lib.q2do_irs_smooth.restype = C.c_int
     lib.q2do_set_threads.argtypes = [C.c_int]
     lib.q2do_set_threads.restype = C.c_int
     lib.q2do_get_threads.restype = C.c_int
@@ -103,6 +105,25 @@ class Oracle:
         P = np.empty((b.nj + 4, b.ni + 4, 4))
         self.lib.q2do_get_state(self.h, block, P.ctypes.data_as(C.c_void_p))
         return P
+
+    def residual(self, block):
+        """R(nprim, 1-gc:ni+gc, 1-gc:nj+gc) of a block as (nj+4, ni+4, 4)."""
+        b = self.problem.blocks[block]
+        R = np.empty((b.nj + 4, b.ni + 4, 4))
+        self.lib.q2do_get_residual(self.h, block, R.ctypes.data_as(C.c_void_p))
+        return R
+
+    def set_residual(self, block, R):
+        R = np.ascontiguousarray(R, dtype=np.float64)
+        b = self.problem.blocks[block]
+        assert R.shape == (b.nj + 4, b.ni + 4, 4)
+        self.lib.q2do_set_residual(self.h, block, R.ctypes.data_as(C.c_void_p))
+
+    def irs_smooth(self, case_id, beta):
+        """Residual_Smoothing (Lib_IRS.f90:17-44) of one case on the residual arrays as set."""
+        rc = self.lib.q2do_irs_smooth(self.h, int(case_id), float(beta))
+        if rc != 0:
+            raise RuntimeError(f"q2do_irs_smooth: {rc}")
 
     def dtlocal(self, block):
         b = self.problem.blocks[block]

@@ -1284,15 +1284,99 @@ static void sources(q2do *o, ocase *cs) {
 }
 
 /* ------------------------------------------------------------------------ */
-/* Newstate_Conservative / Compute_Cons_NoIRS — Lib_Newstate.f90:14-161      */
+/* Implicit residual smoothing — Lib_IRS.f90:17-177                          */
+/* ------------------------------------------------------------------------ */
+/* Ghost_Residual_Extrapolation (:130-177): one ghost layer per record.  A connection 101 takes the
+ * donor cell's residual (:163-169); every other type, periodic 201 included, takes the boundary cell's
+ * own (`case default`, :171-172).  Ghosts of a face without records keep what they hold (zero here:
+ * the reference leaves that array uninitialised, Mod_Allocate_Data.f90:123). */
+static void irs_ghosts(q2do *o, ocase *cs) {
+  for (int k = 0; k < cs->nbc; k++) {
+    const q2d_bc_entry *e = &o->bc[cs->first_bc + k].e;
+    oblk *bm = &o->blk[cs->first_block + e->b - 1];
+    int ig = e->i - guide[e->f][0], jg = e->j - guide[e->f][1];
+    if (e->type == Q2D_BC_CONNECT) {
+      oblk *bs = &o->blk[cs->first_block + e->bs - 1];
+      for (int v = 0; v < NP; v++) RV(bm, v, ig, jg) = RV(bs, v, e->is, e->js);
+    } else {
+      for (int v = 0; v < NP; v++) RV(bm, v, ig, jg) = RV(bm, v, e->i, e->j);
+    }
+  }
+}
+
+/* Residual_Smoothing_Blk (:47-127) along direction d: Res_Star = Res over 0..n+1 (:60-65), then
+ * njacobi = 2 sweeps Res_Star_New = (Res + beta*(Res_Star(k-1) + Res_Star(k+1))) / (1 + 2 beta) on the
+ * interior (:68-93, stencil :106-125), then Res = Res_Star on the interior (:97-102).  rs and rsn are
+ * scratch arrays of the block's full extent. */
+static void irs_block(oblk *b, int d, double beta, double *rs, double *rsn) {
+  for (int j = 0; j <= b->nj + 1; j++)
+    for (int i = 0; i <= b->ni + 1; i++)
+      for (int v = 0; v < NP; v++) rs[cidx(b, i, j) * NP + v] = RV(b, v, i, j);
+  for (int jacobi = 1; jacobi <= 2; jacobi++) {
+    for (int j = 1; j <= b->nj; j++)
+      for (int i = 1; i <= b->ni; i++) {
+        int i1 = d == 1 ? i - 1 : i, j1 = d == 1 ? j : j - 1;
+        int i2 = d == 1 ? i + 1 : i, j2 = d == 1 ? j : j + 1;
+        for (int v = 0; v < NP; v++) {
+          double r = RV(b, v, i, j), rm = rs[cidx(b, i1, j1) * NP + v], rp = rs[cidx(b, i2, j2) * NP + v];
+          rsn[cidx(b, i, j) * NP + v] = (r + beta * (rm + rp)) / (1.0 + 2.0 * beta);
+        }
+      }
+    for (int j = 1; j <= b->nj; j++)
+      for (int i = 1; i <= b->ni; i++)
+        for (int v = 0; v < NP; v++) rs[cidx(b, i, j) * NP + v] = rsn[cidx(b, i, j) * NP + v];
+  }
+  for (int j = 1; j <= b->nj; j++)
+    for (int i = 1; i <= b->ni; i++)
+      for (int v = 0; v < NP; v++) RV(b, v, i, j) = rs[cidx(b, i, j) * NP + v];
+}
+
+/* Residual_Smoothing (:17-44): for d = 1..ndir, the ghost extrapolation of every record, then every
+ * block's sweeps along d. */
+static int irs_smooth(q2do *o, ocase *cs, double beta) {
+  size_t mx = 1;
+  for (int bb = 0; bb < cs->nblocks; bb++) {
+    const oblk *b = &o->blk[cs->first_block + bb];
+    size_t n = (size_t)(b->ni + 2 * GC) * (size_t)(b->nj + 2 * GC) * NP;
+    if (n > mx) mx = n;
+  }
+  double *rs = (double *)calloc(mx, sizeof(double)), *rsn = (double *)calloc(mx, sizeof(double));
+  if (!rs || !rsn) { free(rs); free(rsn); return Q2D_E_NOMEM; }
+  for (int d = 1; d <= cs->ndir; d++) {
+    irs_ghosts(o, cs);
+    for (int bb = 0; bb < cs->nblocks; bb++) irs_block(&o->blk[cs->first_block + bb], d, beta, rs, rsn);
+  }
+  free(rs); free(rsn);
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Newstate_Conservative — Lib_Newstate.f90:14-161                           */
 /* ------------------------------------------------------------------------ */
 static const double RKcoeff[3][3] = { {1.0, 0.0, 0.0}, {1.0, 0.5, 0.0}, {1.0, 0.25, 2.0 / 3.0} };
 
+/* Without IRS (beta = 0) each cell scales its residual and updates at once (Compute_Cons_NoIRS,
+ * :66-82 and :130-159).  With IRS (beta > 0, Assign_Setup.f90:61) every residual is scaled first
+ * (Scale_Residual_Cons, :37-49 and :93-99), then smoothed (Residual_Smoothing, :52), then every cell
+ * updates with its smoothed residual (Update_State_Cons_IRS, :55-64 and :101-127). */
 static int newstate(q2do *o, ocase *cs, int irk, int *fb, int *fi, int *fj) {
   const q2d_scheme *s = &cs->d.scheme;
   double c = RKcoeff[s->n_rk - 1][irk - 1];
   double strang = 1.0;
+  const int irs = s->irs_beta > 0.0;
   int err = 0;
+  if (irs) {
+    for (int bb = 0; bb < cs->nblocks; bb++) {
+      oblk *b = &o->blk[cs->first_block + bb];
+      for (int j = 1; j <= b->nj; j++)
+        for (int i = 1; i <= b->ni; i++) {
+          double dt = b->dtlocal[IN(b, i, j)], sr = SUR(b, i, j);
+          for (int v = 0; v < NP; v++) RV(b, v, i, j) = -RV(b, v, i, j) / sr * dt * strang;
+        }
+    }
+    int r = irs_smooth(o, cs, s->irs_beta);
+    if (r) return r;
+  }
   for (int bb = 0; bb < cs->nblocks; bb++) {
     oblk *b = &o->blk[cs->first_block + bb];
     for (int j = 1; j <= b->nj; j++) {
@@ -1300,7 +1384,7 @@ static int newstate(q2do *o, ocase *cs, int irk, int *fb, int *fi, int *fj) {
         double dt = b->dtlocal[IN(b, i, j)], sr = SUR(b, i, j);
         double res[NP], prim[NP], primold[NP], cons[NP], consold[NP];
         for (int v = 0; v < NP; v++) {
-          res[v] = -RV(b, v, i, j) / sr * dt * strang;
+          res[v] = irs ? RV(b, v, i, j) : -RV(b, v, i, j) / sr * dt * strang;
           prim[v] = PV(b, v, i, j); primold[v] = POV(b, v, i, j);
         }
         prim2cons(&cs->gas, primold, consold);
@@ -1461,4 +1545,21 @@ int q2do_convective_flux(int lim, int rid, double gas_R, double gas_cp, const do
                          double nx, double ny, double int_l, const double *prim, double *flux) {
   gas_t g = mkgas(gas_R, gas_cp);
   return convective_flux(lim, rid, &g, dl, nx, ny, int_l, (const double(*)[NP])prim, flux);
+}
+
+int q2do_get_residual(q2do *o, int block, double *R) {
+  if (block < 0 || block >= o->nblocks) return Q2D_E_ARG;
+  oblk *b = &o->blk[block];
+  memcpy(R, b->R, sizeof(double) * NP * (size_t)(b->ni + 2 * GC) * (size_t)(b->nj + 2 * GC));
+  return 0;
+}
+int q2do_set_residual(q2do *o, int block, const double *R) {
+  if (block < 0 || block >= o->nblocks) return Q2D_E_ARG;
+  oblk *b = &o->blk[block];
+  memcpy(b->R, R, sizeof(double) * NP * (size_t)(b->ni + 2 * GC) * (size_t)(b->nj + 2 * GC));
+  return 0;
+}
+int q2do_irs_smooth(q2do *o, int case_id, double beta) {
+  if (case_id < 0 || case_id >= o->ncases) return Q2D_E_ARG;
+  return irs_smooth(o, &o->cs[case_id], beta);
 }

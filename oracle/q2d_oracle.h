@@ -54,6 +54,11 @@ void q2do_roe_averages(double gas_R, double gas_cp, const double *primL, const d
 int q2do_convective_flux(int limiter, int riemann, double gas_R, double gas_cp,
                          const double *dl, double nx, double ny, double int_l,
                          const double *prim, double *flux);
+/* Residual array R(nprim, 1-gc:ni+gc, 1-gc:nj+gc) of block `block` (0-based global), and
+ * Residual_Smoothing (Lib_IRS.f90:17-44) of case `case_id` applied to those arrays as they stand. */
+int q2do_get_residual(q2do *o, int block, double *R);
+int q2do_set_residual(q2do *o, int block, const double *R);
+int q2do_irs_smooth(q2do *o, int case_id, double beta);
 
 #ifdef __cplusplus
 }

@@ -54,6 +54,7 @@ class CaseSpec:
     res_threshold: float = -1.0
     time_threshold: float = 1e300
     res_drop: float = 0.0
+    irs_beta: float = 0.0         # irs-beta: > 0 enables implicit residual smoothing
 
 
 @dataclass

@@ -81,6 +81,7 @@ struct DevCase {
   double res_threshold, time_threshold;
   double mu_ref, T_ref, S_suth, Pr;     // Sutherland viscosity, Prandtl number (model 1)
   double res_drop;                      // > 0: stop when residual(1) <= res_drop * residual(1) of step 1
+  double irs_beta;                      // > 0: implicit residual smoothing (k_irs_i, k_irs_j)
 };
 
 struct DevCtrl {
@@ -115,6 +116,7 @@ struct DevBlock {
   const float4* mgd;
   const float* mhy;
   const float2* msrc;
+  double* irs[2];                // IRS cases: scaled residual (4 planes) and its smoothing along i (4 planes)
 };
 
 struct DevBC {
@@ -1177,6 +1179,7 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
     auto item_body = [&](auto riec) {
       constexpr int RIE = decltype(riec)::value;
       const bool last = (irk == C.n_rk);
+      const bool irs = C.irs_beta > 0.0;                 // residual to B.irs[0]; k_irs_i / k_irs_j update
       const int ni = B.ni, nj = B.nj, pitch = B.pitch;
       const long long pl = B.plane;
       // the case's control words, read once per item (wave-uniform; a read inside the row loop would be a
@@ -1279,11 +1282,11 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
             const long long oc = at(r);
             gcur.dlx = dlxp[oc]; gcur.fix = fixp[oc]; gcur.fiy = fiyp[oc]; gcur.fil = filp[oc]; gcur.sur = surp[oc];
             gcur.dtl = (own && !tacc) ? dtlp[oc] : 0.0;
-            if (irk != 1) {                              // stage 1 reads its RK base state as its input (qA)
+            if (irk != 1 && !irs) {                      // stage 1 reads its RK base state as its input (qA)
 #pragma unroll
               for (int v = 0; v < 4; ++v) gcur.po[v] = PO[v * pl + oc];
             }
-            if (FUSE && last) {                          // M of row r for the fused dt (c11, c21, c12, c22)
+            if (FUSE && last && !irs) {                  // M of row r for the fused dt (c11, c21, c12, c22)
 #pragma unroll
               for (int v = 0; v < 4; ++v) gcur.mm[v] = m11p[v * pl + oc];
             }
@@ -1436,8 +1439,20 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
                 for (int k = 1; k < 4; ++k) res[k] = res[k] - gp(B.vsv)[(k - 1) * pl + oA] * sr;
               }
             }
-            // ---- Compute_Cons_NoIRS — Lib_Newstate.f90:131-159 -------------
             const double dt = tacc ? dtg : gcur.dtl;
+            if (irs) {
+              // Scale_Residual_Cons (Lib_Newstate.f90:93-99); smoothing and update follow in k_irs_i, k_irs_j
+              GPW rs = gpw(B.irs[0]);
+              if (sizeof(FT) == 8) {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) rs[v * pl + oA] = -res[v] / sr * dt;
+              } else {
+                const double dts = dt * rcp64(sr);
+#pragma unroll
+                for (int v = 0; v < 4; ++v) rs[v * pl + oA] = -res[v] * dts;
+              }
+            } else {
+            // ---- Compute_Cons_NoIRS — Lib_Newstate.f90:131-159 -------------
             if (irk == 1) {                              // RK base state == stage input on stage 1
 #pragma unroll
               for (int v = 0; v < 4; ++v) gcur.po[v] = qA[v];
@@ -1500,6 +1515,7 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
               if (!tacc) gpw(B.dtl[kdtsel ^ 1])[oA] = dtmin;
               if (dtmin < dtnext) dtnext = dtmin;
             }
+            }   // !irs
           }
         }
         // ---- roll the j-stencil -------------------------------------------
@@ -1514,7 +1530,7 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
         }
         dlyA = dlyB; dlyB = dlyC;
       }
-      if (last) {
+      if (last && !irs) {
         // fixed-pattern wavefront sum -> one partial per work item (deterministic)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
@@ -1523,7 +1539,7 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
           if (lane == 0) partials[(long long)it * kPartials + v] = s;
         }
       }
-      if (last && FUSE) {
+      if (last && FUSE && !irs) {
         // next step's dt: the item's exact min, reduced over the case's items by k_finish (no atomics)
         for (int k = 32; k >= 1; k >>= 1) {
           const double o = __shfl_xor(dtnext, k, kWave);
@@ -1595,7 +1611,9 @@ __device__ __forceinline__ float2 mx_stream_load2(GPF2 p) {
   return f2_bits(*p);
 #endif
 }
-template <int RIE, bool FUSE>
+// IRS: the launch holds cases with implicit residual smoothing (a separate instantiation, so that the
+// register allocation of the plain kernel is not affected by the residual-store branch).
+template <int RIE, bool FUSE, bool IRS>
 __global__ __launch_bounds__(256, Q2D_STAGE_MX_MIN_WAVES) void k_stage_mx(const DevBlock* __restrict__ blocks, const DevCase* __restrict__ cases,
                                                   DevCtrl* __restrict__ ctrl, const Item* __restrict__ items, int item_lo,
                                                   int item_hi, const float4* __restrict__ bcdense, double* __restrict__ partials,
@@ -1632,6 +1650,7 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MX_MIN_WAVES) void k_stage_mx(const 
       constexpr int LIM = decltype(limc)::value;
       constexpr int RV = decltype(riec)::value;         // Riemann solver of this item's case
       const bool last = (irk == C.n_rk);
+      const bool irs = IRS && C.irs_beta > 0.0;          // residual to B.irs[0]; k_irs_i / k_irs_j update
       const int ni = B.ni, nj = B.nj, pitch = B.pitch;
       const long long pl = B.plane;
       const int kcur = __builtin_amdgcn_readfirstlane(K.cur), kdtsel = __builtin_amdgcn_readfirstlane(K.dtsel);
@@ -1768,7 +1787,7 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MX_MIN_WAVES) void k_stage_mx(const 
         double po[4] = {0.0, 0.0, 0.0, 0.0};
         double dtl = 0.0;
         const float2 gs = mx_stream_load2(&gsp[oc]);
-        if (irk != 1) {
+        if (irk != 1 && !irs) {
 #if Q2D_MX_NT_LOAD
 #pragma unroll
           for (int v = 0; v < 4; ++v) po[v] = __builtin_nontemporal_load(&PO[v * pl + oc]);
@@ -1778,7 +1797,7 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MX_MIN_WAVES) void k_stage_mx(const 
 #endif
         }
         if (!tacc) dtl = dtlp[oc];
-        if (FUSE && last) gd = mx_stream_load(&gdp[oc]);
+        if (FUSE && last && !irs) gd = mx_stream_load(&gdp[oc]);
         {
           const long long oN = at(min(r + 3, rmax));
 #pragma unroll
@@ -1866,13 +1885,21 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MX_MIN_WAVES) void k_stage_mx(const 
             S[3] = (rhoE0 + p) * VdG;
           }
         }
+        const double dt = tacc ? dtg : dtl;
+        const double dts = dt * (double)gs.y;
+        if (irs) {
+          // Scale_Residual_Cons (Lib_Newstate.f90:93-99); smoothing and update follow in k_irs_i, k_irs_j
+          if (own) {
+            GPW rs = gpw(B.irs[0]);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) rs[v * pl + oc] = __builtin_fma(-res[v], dts, S[v] * dt);
+          }
+        } else {
         // ---- FP64 update: RK_stage (Lib_RK.f90:15-27) on the conservative state ----
         if (irk == 1) {
 #pragma unroll
           for (int v = 0; v < 4; ++v) po[v] = qA[v];
         }
-        const double dt = tacc ? dtg : dtl;
-        const double dts = dt * (double)gs.y;
         double Uo[4], U[4], qn2[4];
         Uo[0] = po[0]; Uo[1] = po[0] * po[1]; Uo[2] = po[0] * po[2];
         Uo[3] = __builtin_fma(po[3], rgm1, 0.5 * po[0] * (po[1] * po[1] + po[2] * po[2]));
@@ -1915,12 +1942,13 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MX_MIN_WAVES) void k_stage_mx(const 
           }
           if (last && FUSE && !tacc) dtl_next[oc] = dtmin;
         }
+        }   // !irs
         roll(qC, hyC);
       }
       }
       if (fail_if_r != kNoRow) record_fail(&K, Q2D_E_INTERFACE, fail_key(stage_id, 0, B.blk_in_case, fail_if_r, x));
       if (fail_st_r != kNoRow) record_fail(&K, Q2D_E_STATE, fail_key(stage_id, 1, B.blk_in_case, fail_st_r, x));
-      if (last) {                                        // fixed-pattern wavefront sum (deterministic)
+      if (last && !irs) {                                // fixed-pattern wavefront sum (deterministic)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           double s = acc[v];
@@ -1928,7 +1956,7 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MX_MIN_WAVES) void k_stage_mx(const 
           if (lane == 0) partials[(long long)it * kPartials + v] = s;
         }
       }
-      if (last && FUSE) {                                // next step's dt: the item's exact min (see k_stage)
+      if (last && FUSE && !irs) {                        // next step's dt: the item's exact min (see k_stage)
         for (int k = 32; k >= 1; k >>= 1) {
           const double o = __shfl_xor(dtnext, k, kWave);
           if (o < dtnext) dtnext = o;
@@ -1956,6 +1984,246 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MX_MIN_WAVES) void k_stage_mx(const 
     }
   }
   if (tslotp && lane == 0) atomicMax(&tslotp[1], (unsigned long long)wall_clock64());
+}
+
+// ---------------------------------------------------------------------------
+// Implicit residual smoothing — Lib_IRS.f90:17-177, for cases with irs_beta > 0 (Newstate_Conservative,
+// Lib_Newstate.f90:35-64).  The stage kernel stores the scaled residual -R / sur * dt
+// (Scale_Residual_Cons) in B.irs[0] instead of updating.  k_irs_i smooths it along i into B.irs[1];
+// k_irs_j smooths that along j and applies the RK stage (Update_State_Cons_IRS, :101-127) with the
+// stage kernel's tail: state check, residual-norm partials, next step's dt.  A direction is two Jacobi
+// sweeps (Residual_Smoothing_Blk, :47-127):
+//     s1(k) = (R(k) + beta (R(k-1) + R(k+1))) / (1 + 2 beta)     interior k  (s1 = R on a ghost)
+//     s2(k) = (R(k) + beta (s1(k-1) + s1(k+1))) / (1 + 2 beta)
+// evaluated in one pass from R(k-2 .. k+2): along i the neighbours come from the adjacent lanes (DPP),
+// along j from the rows the wavefront marches through.  Work items are the stage kernel's, one wavefront
+// each.  Ghost residuals (Ghost_Residual_Extrapolation, :130-177) are formed where they are read, once
+// the previous launch has finished every block: the boundary cell's record gives the donor cell's
+// residual for a connection 101 and the boundary cell's own for every other type (201 included); a face
+// without records reads its ghost slot, which is never written and holds zero.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void irs_ghost(const DevBlock* __restrict__ blocks, const DevBlock& B,
+                                          const DevBC* __restrict__ bc, const int* __restrict__ bcface, int buf,
+                                          int f, int pos, double out[4]) {
+  int gi = f == 1 ? 0 : (f == 2 ? B.ni + 1 : pos), gj = f == 3 ? 0 : (f == 4 ? B.nj + 1 : pos);
+  const double* base = B.irs[buf];
+  int pitch = B.pitch;
+  long long pl = B.plane;
+  if (B.bcf[f - 1] >= 0) {
+    const DevBC& e = bc[bcface[B.bcf[f - 1] + pos - 1]];
+    if (e.type == Q2D_BC_CONNECT) {
+      const DevBlock& D = blocks[e.dblk];
+      base = D.irs[buf]; pitch = D.pitch; pl = D.plane; gi = e.is; gj = e.js;
+    } else {
+      gi = e.i; gj = e.j;
+    }
+  }
+  GP src = gp(base) + (long long)(gj + 1) * pitch + (gi + 1);
+#pragma unroll
+  for (int v = 0; v < 4; ++v) out[v] = src[v * pl];
+}
+
+__global__ __launch_bounds__(256) void k_irs_i(const DevBlock* __restrict__ blocks, const DevCase* __restrict__ cases,
+                                               const DevCtrl* __restrict__ ctrl, const Item* __restrict__ items,
+                                               int item_lo, int item_hi, const DevBC* __restrict__ bc,
+                                               const int* __restrict__ bcface, int irk) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wpg = blockDim.x / kWave;
+  const int wave0 = __builtin_amdgcn_readfirstlane(blockIdx.x * wpg + threadIdx.x / kWave);
+  const int nwaves = gridDim.x * wpg;
+  for (int it = item_lo + wave0; it < item_hi; it += nwaves) {
+    const Item item = items[it];
+    const DevBlock& B = blocks[item.blk];
+    const DevCase& C = cases[B.case_id];
+    if (!ctrl[B.case_id].active || irk > C.n_rk || !(C.irs_beta > 0.0)) continue;
+    const double beta = C.irs_beta, den = 1.0 + 2.0 * beta;
+    const int ni = B.ni, pitch = B.pitch;
+    const long long pl = B.plane;
+    const int x = item.x0 - 2 + lane;                  // lanes 0..63: columns x0-2 .. x0+61
+    const int xc = min(max(x, 0), ni + 1);
+    const bool own = lane >= 2 && lane < 2 + kOwn && x <= ni;
+    const bool inner = x >= 1 && x <= ni;
+    const bool ghost = x == 0 || x == ni + 1;
+    GP R0 = gp(B.irs[0]);
+    GPW R1 = gpw(B.irs[1]);
+    for (int r = item.y0; r <= item.y1; ++r) {
+      const long long o = (long long)(r + 1) * pitch + (xc + 1);
+      double q[4];
+      if (ghost) {
+        irs_ghost(blocks, B, bc, bcface, 0, x == 0 ? 1 : 2, r, q);
+      } else {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) q[v] = R0[v * pl + o];
+      }
+      double s1[4];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const double qm = shup(q[v]), qp = shdn(q[v]);
+        s1[v] = inner ? (q[v] + beta * (qm + qp)) / den : q[v];
+      }
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const double sm = shup(s1[v]), sp = shdn(s1[v]);
+        const double s2 = (q[v] + beta * (sm + sp)) / den;
+        if (own) R1[v * pl + o] = s2;
+      }
+    }
+  }
+}
+
+// Compute_dt (Mod_dt.f90:59-86) of an Euler cell: the form the stage kernel fuses into its last stage.
+__device__ __forceinline__ double cell_dt_euler(const DevCase& C, int iter, const double q[4], const double mm[4],
+                                                double dlx, double dly) {
+  const double snd = sqrt(C.gam * q[3] / q[0]);
+  double dtmin = 1e8;
+  {
+    const double m1 = mm[0], m2 = mm[2];
+    const double nrm = sqrt(m1 * m1 + m2 * m2);
+    const double v1 = m1 / nrm, v2 = m2 / nrm;
+    const double lambda = fabs(q[1] * v1 + q[2] * v2) + snd;
+    const double dtd = dlx / lambda * C.cfl;
+    if (dtd < dtmin) dtmin = dtd;
+  }
+  if (C.ndir == 2) {
+    const double m1 = mm[1], m2 = mm[3];
+    const double nrm = sqrt(m1 * m1 + m2 * m2);
+    const double v1 = m1 / nrm, v2 = m2 / nrm;
+    const double lambda = fabs(q[1] * v1 + q[2] * v2) + snd;
+    const double dtd = dly / lambda * C.cfl;
+    if (dtd < dtmin) dtmin = dtd;
+  }
+  if (iter < C.ramp_iter) dtmin = dtmin * (double)iter / (double)C.ramp_iter;
+  return dtmin;
+}
+
+__global__ __launch_bounds__(256) void k_irs_j(const DevBlock* __restrict__ blocks, const DevCase* __restrict__ cases,
+                                               DevCtrl* __restrict__ ctrl, const Item* __restrict__ items, int item_lo,
+                                               int item_hi, const DevBC* __restrict__ bc, const int* __restrict__ bcface,
+                                               double* __restrict__ partials, int irk) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wpg = blockDim.x / kWave;
+  const int wave0 = __builtin_amdgcn_readfirstlane(blockIdx.x * wpg + threadIdx.x / kWave);
+  const int nwaves = gridDim.x * wpg;
+  for (int it = item_lo + wave0; it < item_hi; it += nwaves) {
+    const Item item = items[it];
+    const DevBlock& B = blocks[item.blk];
+    DevCtrl& K = ctrl[B.case_id];
+    const DevCase& C = cases[B.case_id];
+    if (!K.active || irk > C.n_rk || !(C.irs_beta > 0.0)) continue;
+    const bool last = irk == C.n_rk;
+    const double beta = C.irs_beta, den = 1.0 + 2.0 * beta;
+    const bool smooth = C.ndir == 2;                   // Residual_Smoothing's d = 1..ndir
+    const int ni = B.ni, nj = B.nj, pitch = B.pitch;
+    const long long pl = B.plane;
+    const int kcur = __builtin_amdgcn_readfirstlane(K.cur), kdtsel = __builtin_amdgcn_readfirstlane(K.dtsel);
+    const int kiter = __builtin_amdgcn_readfirstlane(K.iter);
+    const int bin = stage_in(kcur, irk), bout = stage_out(kcur, irk);
+    GP Pin = gp(B.P[bin]), PO = gp(B.P[kcur]);
+    GPW Pout = gpw(B.P[bout]);
+    GP R1 = gp(B.irs[1]);
+    const bool tacc = C.time_accurate != 0;
+    const bool mixed = C.precision == Q2D_PREC_MIXED;
+    const Gas<double> gd{C.R, C.cp, C.cv, C.gam};
+    const double c_rk = (C.n_rk == 1) ? 1.0 : (C.n_rk == 2 ? (irk == 1 ? 1.0 : 0.5)
+                                                            : (irk == 1 ? 1.0 : (irk == 2 ? 0.25 : 2.0 / 3.0)));
+    const int stage_id = K.stage_count + irk;
+    const int x = item.x0 - 2 + lane;
+    const bool own = lane >= 2 && lane < 2 + kOwn && x <= ni;
+    const int xc = min(max(x, 1), ni);
+    // R at row k of this lane's column: rows 0 and nj+1 are the ghosts of faces 3 and 4
+    auto row = [&](int k, double q[4]) {
+      if (own && (k == 0 || k == nj + 1)) {
+        irs_ghost(blocks, B, bc, bcface, 1, k == 0 ? 3 : 4, x, q);
+      } else if (own && k >= 1 && k <= nj) {
+        const long long o = (long long)(k + 1) * pitch + (xc + 1);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) q[v] = R1[v * pl + o];
+      } else {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) q[v] = 0.0;
+      }
+    };
+    auto sweep1 = [&](int k, const double a[4], const double b[4], const double c[4], double o[4]) {
+      const bool in = k >= 1 && k <= nj;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) o[v] = in ? (b[v] + beta * (a[v] + c[v])) / den : b[v];
+    };
+    double Rm[4], R0[4], Rp[4], sA[4], sB[4];
+    {
+      double Rm2[4];
+      row(item.y0 - 2, Rm2); row(item.y0 - 1, Rm); row(item.y0, R0); row(item.y0 + 1, Rp);
+      sweep1(item.y0 - 1, Rm2, Rm, R0, sA);
+      sweep1(item.y0, Rm, R0, Rp, sB);
+    }
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    double dtnext = 1e300;
+    for (int r = item.y0; r <= item.y1; ++r) {
+      double Rp2[4], sC[4], s2[4];
+      row(r + 2, Rp2);
+      sweep1(r + 1, R0, Rp, Rp2, sC);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) s2[v] = smooth ? (R0[v] + beta * (sA[v] + sC[v])) / den : R0[v];
+      if (own) {
+        const long long o = (long long)(r + 1) * pitch + (x + 1);
+        double q[4], po[4], Uo[4], U[4], qn2[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) { q[v] = Pin[v * pl + o]; po[v] = PO[v * pl + o]; }
+        prim2cons(gd, po[0], po[1], po[2], po[3], Uo);
+        prim2cons(gd, q[0], q[1], q[2], q[3], U);
+        if (!mixed) {
+#pragma unroll
+          for (int v = 0; v < 4; ++v) U[v] = Uo[v] + (U[v] - Uo[v] + s2[v]) * c_rk;   // RK_stage — Lib_RK.f90:15-27
+          cons2prim(gd, U, qn2);
+        } else {
+#pragma unroll
+          for (int v = 0; v < 4; ++v) U[v] = Uo[v] + (U[v] - Uo[v] + s2[v]) * c_rk;
+          const double ir = rcp64(U[0]);
+          const double vx = U[1] * ir, vy = U[2] * ir;
+          const double e = U[3] * ir - 0.5 * (vx * vx + vy * vy);
+          qn2[0] = U[0]; qn2[1] = vx; qn2[2] = vy; qn2[3] = U[0] * (gd.gam - 1.0) * e;
+        }
+        // Check_And_Fix_State — Lib_Newstate.f90:316-350
+        if (__builtin_isnan(qn2[0]) || __builtin_isnan(qn2[1]) || __builtin_isnan(qn2[2]) ||
+            __builtin_isnan(qn2[3]) || qn2[3] < 0.0)
+          record_fail(&K, Q2D_E_STATE, fail_key(stage_id, 1, B.blk_in_case, r, x));
+        if (qn2[0] < 1e-20) qn2[0] = 1e-20;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) Pout[v * pl + o] = qn2[v];
+        if (last) {                                    // Compute_Residual — Mod_Diagnostic.f90:452-458
+#pragma unroll
+          for (int v = 0; v < 4; ++v) { const double d = fabs(qn2[v] - po[v]); acc[v] = acc[v] + d * d; }
+          // the next step's Compute_dt on the state just written (as the stage kernel's fused form)
+          double dtmin;
+          if (B.mgd) {                                 // mixed-form Euler block: its FP32 dt geometry
+            const float2 gs = mx_load2((GPF2)&B.mgs[o]);
+            const f4v g4 = mx_load4(&B.mgd[o]);
+            dtmin = cell_dt_mx(C, kiter + 1, qn2[0], qn2[1], qn2[2], qn2[3], gs.x, ((GPF)B.mhy)[o], g4.x, g4.y,
+                               g4.z, g4.w);
+          } else {
+            const double mm[4] = {gp(B.m11)[o], gp(B.m11)[pl + o], gp(B.m11)[2 * pl + o], gp(B.m11)[3 * pl + o]};
+            dtmin = cell_dt_euler(C, kiter + 1, qn2, mm, gp(B.dlx)[o], gp(B.dly)[o]);
+          }
+          if (!tacc) gpw(B.dtl[kdtsel ^ 1])[o] = dtmin;
+          if (dtmin < dtnext) dtnext = dtmin;
+        }
+      }
+#pragma unroll
+      for (int v = 0; v < 4; ++v) { sA[v] = sB[v]; sB[v] = sC[v]; R0[v] = Rp[v]; Rp[v] = Rp2[v]; }
+    }
+    if (last) {                                        // the stage kernel's per-item partials (deterministic)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        double s = acc[v];
+        for (int k = 32; k >= 1; k >>= 1) s = s + __shfl_xor(s, k, kWave);
+        if (lane == 0) partials[(long long)it * kPartials + v] = s;
+      }
+      for (int k = 32; k >= 1; k >>= 1) {
+        const double o = __shfl_xor(dtnext, k, kWave);
+        if (o < dtnext) dtnext = o;
+      }
+      if (lane == 0) partials[(long long)it * kPartials + 4] = dtnext;
+    }
+  }
 }
 
 // Timing slots: clear every sub-slot and the step counter.
@@ -2083,6 +2351,7 @@ struct q2d_handle {
   DtRange* d_dtr = nullptr;
   int dt_total = 0;
   bool any_visc = false;
+  bool any_irs = false;          // a live case smooths its residual (k_irs_i, k_irs_j after each stage)
   bool bc_two_phase = false;     // a live block is one cell wide: ghost fill and BC fluxes in two launches
   std::vector<int> vrec;         // boundary records with a viscous flux (BC 300/302 of model-1 cases)
   int* d_vrec = nullptr;
@@ -2112,7 +2381,7 @@ namespace {
 // The stage-kernel instantiation that runs the items of `pass` (2 * viscous + riemann).
 const void* stage_kernel(const q2d_handle* h, int pass) {
   if (h->precision_mixed)
-    return pass < 2 ? reinterpret_cast<const void*>(&k_stage_mx<-1, kFuseDtMx>)
+    return pass < 2 ? reinterpret_cast<const void*>(&k_stage_mx<-1, kFuseDtMx, false>)
                     : reinterpret_cast<const void*>(&k_stage<float, -1, false, true>);
   return pass < 2 ? reinterpret_cast<const void*>(&k_stage<double, -1, kFuseDtFp64, false>)
                   : reinterpret_cast<const void*>(&k_stage<double, -1, kFuseDtFp64, true>);
@@ -2274,11 +2543,13 @@ int launch_stage(q2d_handle* h, int irk) {
     hipLaunchKernelGGL((k_stage<FT, RIE, FUSE, VISC>), dim3(groups), dim3(tpb), 0, h->stream, h->d_blocks,        \
                        h->d_cases, h->d_ctrl, h->d_items, lo, hi, bf, bq, h->d_partials, irk, h->xcd_remap, ts,  \
                        slot)
-#define Q2D_STAGE_MX_LAUNCH(RIE)                                                                                  \
-    hipLaunchKernelGGL((k_stage_mx<RIE, kFuseDtMx>), dim3(groups), dim3(tpb), 0, h->stream, h->d_blocks, h->d_cases, \
-                       h->d_ctrl, h->d_items, lo, hi, h->d_bcdense, h->d_partials, irk, h->xcd_remap, ts, slot)
+#define Q2D_STAGE_MX_LAUNCH(RIE, IRS)                                                                             \
+    hipLaunchKernelGGL((k_stage_mx<RIE, kFuseDtMx, IRS>), dim3(groups), dim3(tpb), 0, h->stream, h->d_blocks,      \
+                       h->d_cases, h->d_ctrl, h->d_items, lo, hi, h->d_bcdense, h->d_partials, irk, h->xcd_remap, ts, \
+                       slot)
     if constexpr (sizeof(FT) == 4) {                   // mixed form: Euler items run k_stage_mx
-      if (pass == 0) Q2D_STAGE_MX_LAUNCH(-1);
+      if (pass == 0 && h->any_irs) Q2D_STAGE_MX_LAUNCH(-1, true);
+      else if (pass == 0) Q2D_STAGE_MX_LAUNCH(-1, false);
       else Q2D_STAGE_LAUNCH(-1, true);
     } else {
       if (pass == 0) Q2D_STAGE_LAUNCH(-1, false);
@@ -2366,6 +2637,12 @@ int launch_step(q2d_handle* h) {
     }
     int rc = h->precision_mixed ? launch_stage<float, false>(h, irk) : launch_stage<double, kFuseDtFp64>(h, irk);
     if (rc != Q2D_OK) return rc;
+    if (h->any_irs) {                                  // residual smoothing along i, then along j + the update
+      hipLaunchKernelGGL(k_irs_i, dim3(groups), dim3(tpb), 0, h->stream, h->d_blocks, h->d_cases, h->d_ctrl,
+                         h->d_items, 0, n_items, h->d_bc, h->d_bcface, irk);
+      hipLaunchKernelGGL(k_irs_j, dim3(groups), dim3(tpb), 0, h->stream, h->d_blocks, h->d_cases, h->d_ctrl,
+                         h->d_items, 0, n_items, h->d_bc, h->d_bcface, h->d_partials, irk);
+    }
   }
   const int fused_dt = (!h->any_visc && (h->precision_mixed ? kFuseDtMx : kFuseDtFp64)) ? 1 : 0;
   hipLaunchKernelGGL(k_finish, dim3(h->ncases), dim3(kFinishThreads), 0, h->stream, h->d_ctrl, h->d_cases, h->d_partials, h->ncases,
@@ -2489,6 +2766,7 @@ int rebuild_descriptors(q2d_handle* h) {
   for (DevBC& r : h->dbc) r.dpos = h->db[r.blk].bcf[r.f - 1] + (r.f <= 2 ? r.j - 1 : r.i - 1);
   h->max_rk = 1;
   h->any_visc = false;
+  h->any_irs = false;
   h->bc_two_phase = false;
   for (int b = 0; b < h->nblocks; ++b)
     if (h->hb[b].arena && (h->hb[b].ni < 2 || h->hb[b].nj < 2)) h->bc_two_phase = true;
@@ -2496,6 +2774,7 @@ int rebuild_descriptors(q2d_handle* h) {
     if (!h->case_live[c]) continue;
     h->max_rk = std::max(h->max_rk, h->dc[c].n_rk);
     if (h->dc[c].model > 0) h->any_visc = true;
+    if (h->dc[c].irs_beta > 0.0) h->any_irs = true;
   }
   // device arrays, re-allocated at the current sizes (descriptors are small)
   auto realloc_upload = [&](void** dptr, const void* src, size_t bytes) -> int {
@@ -2554,6 +2833,7 @@ int append_cases(q2d_handle* h, const q2d_problem* prob, int32_t* first_case) {
     if (!(s.gas_R > 0) || !(s.gas_cp > s.gas_R)) return fail(Q2D_E_ARG, "gas constants invalid");
     if (s.precision != prec) return fail(Q2D_E_ARG, "all cases of a handle must use one precision");
     if (prob->cases[c].res_drop < 0 || prob->cases[c].res_drop >= 1) return fail(Q2D_E_ARG, "res_drop must be in [0, 1)");
+    if (!(s.irs_beta >= 0) || !std::isfinite(s.irs_beta)) return fail(Q2D_E_ARG, "irs_beta must be finite and >= 0");
     for (int k = 0; k < nblk[c]; ++k) {
       const q2d_block_desc& d = prob->blocks[first_block[c] + k];
       if (s.source == Q2D_SRC_AXI && !d.src_b) return fail(Q2D_E_ARG, "axisymmetric source needs src_b (axis_inv_r)");
@@ -2640,6 +2920,7 @@ int append_cases(q2d_handle* h, const q2d_problem* prob, int32_t* first_case) {
     k.R = s.gas_R; k.cp = s.gas_cp; k.cv = s.gas_cp - s.gas_R; k.gam = s.gas_cp / k.cv;
     k.res_threshold = cd.res_threshold; k.time_threshold = cd.time_threshold; k.res_drop = cd.res_drop;
     k.mu_ref = s.mu_ref; k.T_ref = s.T_ref; k.S_suth = s.S_suth; k.Pr = s.Pr;
+    k.irs_beta = s.irs_beta;
     h->dc.push_back(k);
     h->case_live.push_back(1);
   }
@@ -2660,12 +2941,15 @@ int append_cases(q2d_handle* h, const q2d_problem* prob, int32_t* first_case) {
 
   // ---- blocks: SoA planes in one arena per block ----
   // planes 0-11: 3 state buffers x 4; 12-24 geometry; 25, 28 local dt; 26, 27 source terms; 29 node radius;
-  // viscous blocks add 30-38: diffusive fluxes of i- and j-faces and the viscous source, 3 planes each
+  // viscous blocks add 30-38: diffusive fluxes of i- and j-faces and the viscous source, 3 planes each;
+  // blocks of IRS cases add 8 after those: the scaled residual and its smoothing along i, 4 planes each
   for (int lb = 0; lb < prob->nblocks; ++lb) {
     const q2d_block_desc& d = prob->blocks[lb];
     const bool visc = prob->cases[d.case_id].scheme.model > 0;
     const bool mxgeo = prec == Q2D_PREC_MIXED && !visc;   // FP32 geometry planes of k_stage_mx
-    const int kPlanes = visc ? 39 : (mxgeo ? 37 : 30);
+    const bool irs = prob->cases[d.case_id].scheme.irs_beta > 0;
+    const int irs_plane = visc ? 39 : (mxgeo ? 37 : 30);
+    const int kPlanes = irs_plane + (irs ? 8 : 0);
     h->hb.emplace_back();
     h->db.emplace_back();
     h->nblocks += 1;
@@ -2763,6 +3047,7 @@ int append_cases(q2d_handle* h, const q2d_problem* prob, int32_t* first_case) {
       db.mgd = reinterpret_cast<const float4*>(a + 33 * pl); db.mhy = reinterpret_cast<const float*>(a + 35 * pl);
       db.msrc = reinterpret_cast<const float2*>(a + 36 * pl);
     }
+    if (irs) { db.irs[0] = a + (size_t)irs_plane * pl; db.irs[1] = a + (size_t)(irs_plane + 4) * pl; }
     for (int f = 0; f < 4; ++f) db.bcf[f] = -1;
   }
   h->dbc.insert(h->dbc.end(), newbc.begin(), newbc.end());

@@ -25,6 +25,7 @@ module q2d_gpu
     real(c_double)     :: cfl, vnn
     real(c_double)     :: gas_R, gas_cp
     real(c_double)     :: mu_ref, T_ref, S_suth, Pr
+    real(c_double)     :: irs_beta
   end type
 
   type, bind(C), public :: q2d_block_desc
