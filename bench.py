@@ -149,6 +149,61 @@ def other_workload(args):
     }))
 
 
+def irs_workload(args):
+    """--workload irs: implicit residual smoothing (irs-beta 0.5) in the reference's steady setting
+    (compression ramp, MUSCL minmod, HLLE, SSP-RK3, local dt, CFL 0.9: test/2D/euler/ramp-channel/
+    input.ini).  Step throughput with and without smoothing on a 4096x1024 ramp, both forms; then, unless
+    --no-conv, steps and wall time to a 1e-6 drop of residual(1) on a 256x128 ramp without smoothing
+    and with it, at CFL 0.9 and at CFL 1.8."""
+    from q2d_amd import GpuSolver, cases
+
+    def ramp(precision, beta, ni=4096, nj=1024, cfl=0.9):
+        pb = cases.ramp_channel(ni=ni, nj=nj, wedge_deg=10.0, mach=2.5, precision=precision, n_rk=3,
+                                limiter="minmod", riemann="HLLE", cfl=cfl, itermax=1 << 30)
+        pb.cases[0].irs_beta = beta
+        return pb
+
+    thr = {}
+    for prec in ("fp64", "mixed"):
+        for beta in (0.0, 0.5):
+            r, _ = time_form(lambda p, b=beta: ramp(p, b), prec, args.steps, args.warmup)
+            thr[f"{prec}_beta{beta}"] = {"cell_updates_per_s": r["cell_updates_per_s"], "ms_per_step": r["ms_per_step"],
+                                         "stage_kernel_ms": round(r["stage_kernel_ms"], 4)}
+            log(f"bench: irs {prec} beta {beta}: {r['cell_updates_per_s'] / 1e9:.3f} G cell-updates/s, "
+                f"{r['ms_per_step']:.3f} ms/step, stage kernel {r['stage_kernel_ms']:.4f} ms")
+    conv = []
+    if not args.no_conv:
+        for beta, cfl in ((0.0, 0.5), (0.0, 0.9), (0.5, 0.9), (0.5, 1.8)):
+            pb = ramp("fp64", beta, ni=256, nj=128, cfl=cfl)
+            pb.cases[0].res_drop = 1e-6
+            g = GpuSolver(pb)
+            t0 = time.perf_counter()
+            while True:
+                out = g.run(500, raise_on_error=False)[0]
+                if out["todo"] == 3 or out["status"] != 0 or out["iter"] >= 60000:
+                    break                          # converged, failed (NaN or p < 0: the reference stops), or capped
+            status = out["status"]
+            sec = time.perf_counter() - t0
+            g.close()
+            conv.append({"irs_beta": beta, "cfl": cfl, "steps": out["iter"], "seconds": round(sec, 3),
+                         "converged": status == 0 and out["todo"] == 3, "status": status})
+            log(f"bench: irs convergence beta {beta} cfl {cfl}: {out['iter']} steps, {sec:.2f} s, status {status}")
+    m, p = thr["mixed_beta0.5"], thr["fp64_beta0.5"]
+    print(json.dumps({
+        "metric": "cell-updates/s with implicit residual smoothing (irs-beta 0.5)",
+        "value": m["cell_updates_per_s"], "unit": "cell-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": m["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32 flux / f64 state",
+        "data": "synthetic (compression ramp)",
+        "config": {"workload": "compression ramp 4096x1024 (10 deg, Mach 2.5), MUSCL minmod, HLLE, SSP-RK3, local dt, "
+                               "CFL 0.9, irs-beta 0.5", "cells": 4096 * 1024, "parallelism": "single GPU"},
+        "throughput": thr,
+        "irs_cost": {"fp64": round(thr["fp64_beta0.0"]["cell_updates_per_s"] / p["cell_updates_per_s"], 3),
+                     "mixed": round(thr["mixed_beta0.0"]["cell_updates_per_s"] / m["cell_updates_per_s"], 3)},
+        "convergence_256x128": conv,
+    }))
+
+
 def serve_workload(nreq, seed, log_fn=None):
     """Poisson stream of steady compression-ramp solves (wedge ~ U[5,15] deg, Mach ~ U[1.5,4], grids
     log-uniform 64x64 .. 4096x1024), each run in FP64 to a 1e-6 drop of residual(1); continuous batching
@@ -183,7 +238,8 @@ def main():
                     help="profiling aid: run only one arithmetic form (no JSON line)")
     ap.add_argument("--requests", type=int, default=24, help="serving: requests in the Poisson stream")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
-    ap.add_argument("--workload", choices=["shock", "axi", "batch", "serve"], default="shock",
+    ap.add_argument("--no-conv", action="store_true", help="irs: skip the convergence runs")
+    ap.add_argument("--workload", choices=["shock", "axi", "batch", "serve", "irs"], default="shock",
                     help="shock: the headline workload (default); axi, batch: the laminar axisymmetric case and "
                          "the uneven 64-case batch, reported on their own JSON line")
     ap.add_argument("--copy-probe", action="store_true",
@@ -204,6 +260,9 @@ def main():
         print(json.dumps({"metric": "p50/p99 solve latency (serving, continuous batching)", "value": stats["p50_s"],
                           "unit": "s", "n_gpus": 1, "higher_is_better": False, "data": "synthetic (seeded)",
                           "dtype": "f64", "serving": stats}))
+        return
+    if args.workload == "irs":
+        irs_workload(args)
         return
     if args.workload != "shock":
         other_workload(args)

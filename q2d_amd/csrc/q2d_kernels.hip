@@ -81,7 +81,7 @@ struct DevCase {
   double res_threshold, time_threshold;
   double mu_ref, T_ref, S_suth, Pr;     // Sutherland viscosity, Prandtl number (model 1)
   double res_drop;                      // > 0: stop when residual(1) <= res_drop * residual(1) of step 1
-  double irs_beta;                      // > 0: implicit residual smoothing (k_irs_i, k_irs_j)
+  double irs_beta;                      // > 0: implicit residual smoothing (k_irs)
 };
 
 struct DevCtrl {
@@ -116,7 +116,7 @@ struct DevBlock {
   const float4* mgd;
   const float* mhy;
   const float2* msrc;
-  double* irs[2];                // IRS cases: scaled residual (4 planes) and its smoothing along i (4 planes)
+  double* irs[1];                // IRS cases: the stage's scaled residual (4 planes)
 };
 
 struct DevBC {
@@ -1179,7 +1179,7 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
     auto item_body = [&](auto riec) {
       constexpr int RIE = decltype(riec)::value;
       const bool last = (irk == C.n_rk);
-      const bool irs = C.irs_beta > 0.0;                 // residual to B.irs[0]; k_irs_i / k_irs_j update
+      const bool irs = C.irs_beta > 0.0;                 // residual to B.irs[0]; k_irs smooths and updates
       const int ni = B.ni, nj = B.nj, pitch = B.pitch;
       const long long pl = B.plane;
       // the case's control words, read once per item (wave-uniform; a read inside the row loop would be a
@@ -1441,7 +1441,7 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
             }
             const double dt = tacc ? dtg : gcur.dtl;
             if (irs) {
-              // Scale_Residual_Cons (Lib_Newstate.f90:93-99); smoothing and update follow in k_irs_i, k_irs_j
+              // Scale_Residual_Cons (Lib_Newstate.f90:93-99); smoothing and update follow in k_irs
               GPW rs = gpw(B.irs[0]);
               if (sizeof(FT) == 8) {
 #pragma unroll
@@ -1650,7 +1650,7 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MX_MIN_WAVES) void k_stage_mx(const 
       constexpr int LIM = decltype(limc)::value;
       constexpr int RV = decltype(riec)::value;         // Riemann solver of this item's case
       const bool last = (irk == C.n_rk);
-      const bool irs = IRS && C.irs_beta > 0.0;          // residual to B.irs[0]; k_irs_i / k_irs_j update
+      const bool irs = IRS && C.irs_beta > 0.0;          // residual to B.irs[0]; k_irs smooths and updates
       const int ni = B.ni, nj = B.nj, pitch = B.pitch;
       const long long pl = B.plane;
       const int kcur = __builtin_amdgcn_readfirstlane(K.cur), kdtsel = __builtin_amdgcn_readfirstlane(K.dtsel);
@@ -1888,7 +1888,7 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MX_MIN_WAVES) void k_stage_mx(const 
         const double dt = tacc ? dtg : dtl;
         const double dts = dt * (double)gs.y;
         if (irs) {
-          // Scale_Residual_Cons (Lib_Newstate.f90:93-99); smoothing and update follow in k_irs_i, k_irs_j
+          // Scale_Residual_Cons (Lib_Newstate.f90:93-99); smoothing and update follow in k_irs
           if (own) {
             GPW rs = gpw(B.irs[0]);
 #pragma unroll
@@ -1989,85 +1989,64 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MX_MIN_WAVES) void k_stage_mx(const 
 // ---------------------------------------------------------------------------
 // Implicit residual smoothing — Lib_IRS.f90:17-177, for cases with irs_beta > 0 (Newstate_Conservative,
 // Lib_Newstate.f90:35-64).  The stage kernel stores the scaled residual -R / sur * dt
-// (Scale_Residual_Cons) in B.irs[0] instead of updating.  k_irs_i smooths it along i into B.irs[1];
-// k_irs_j smooths that along j and applies the RK stage (Update_State_Cons_IRS, :101-127) with the
-// stage kernel's tail: state check, residual-norm partials, next step's dt.  A direction is two Jacobi
-// sweeps (Residual_Smoothing_Blk, :47-127):
+// (Scale_Residual_Cons) in B.irs[0] instead of updating; k_irs smooths it along i, then along j, and
+// applies the RK stage (Update_State_Cons_IRS, :101-127) with the stage kernel's tail: state check,
+// residual-norm partials, next step's dt.  A direction is two Jacobi sweeps (Residual_Smoothing_Blk,
+// :47-127):
 //     s1(k) = (R(k) + beta (R(k-1) + R(k+1))) / (1 + 2 beta)     interior k  (s1 = R on a ghost)
 //     s2(k) = (R(k) + beta (s1(k-1) + s1(k+1))) / (1 + 2 beta)
-// evaluated in one pass from R(k-2 .. k+2): along i the neighbours come from the adjacent lanes (DPP),
-// along j from the rows the wavefront marches through.  Work items are the stage kernel's, one wavefront
-// each.  Ghost residuals (Ghost_Residual_Extrapolation, :130-177) are formed where they are read, once
-// the previous launch has finished every block: the boundary cell's record gives the donor cell's
-// residual for a connection 101 and the boundary cell's own for every other type (201 included); a face
-// without records reads its ghost slot, which is never written and holds zero.
+// evaluated in one pass from R(k-2 .. k+2).  Work items are the stage kernel's, one wavefront each: the
+// wavefront marches its rows, smooths each row along i as it reads it (neighbours from the adjacent
+// lanes, DPP), and keeps the i-smoothed rows r-2 .. r+2 in registers for the j-sweep of row r, so the
+// i-smoothed field never goes to memory.  Ghost residuals (Ghost_Residual_Extrapolation, :130-177) are
+// formed where they are read: the boundary cell's record gives the donor cell's residual for a
+// connection 101 (for the j-direction, the donor's i-smoothed residual, computed there from its own
+// row) and the boundary cell's own for every other type (201 included); a face without records gives
+// zero (the reference never writes that ghost).
 // ---------------------------------------------------------------------------
+// Ghost of face f (1..4) beyond boundary cell `pos` (1-based along the face) of block B: scaled residual.
 __device__ __forceinline__ void irs_ghost(const DevBlock* __restrict__ blocks, const DevBlock& B,
-                                          const DevBC* __restrict__ bc, const int* __restrict__ bcface, int buf,
-                                          int f, int pos, double out[4]) {
-  int gi = f == 1 ? 0 : (f == 2 ? B.ni + 1 : pos), gj = f == 3 ? 0 : (f == 4 ? B.nj + 1 : pos);
-  const double* base = B.irs[buf];
-  int pitch = B.pitch;
-  long long pl = B.plane;
-  if (B.bcf[f - 1] >= 0) {
-    const DevBC& e = bc[bcface[B.bcf[f - 1] + pos - 1]];
-    if (e.type == Q2D_BC_CONNECT) {
-      const DevBlock& D = blocks[e.dblk];
-      base = D.irs[buf]; pitch = D.pitch; pl = D.plane; gi = e.is; gj = e.js;
-    } else {
-      gi = e.i; gj = e.j;
-    }
-  }
-  GP src = gp(base) + (long long)(gj + 1) * pitch + (gi + 1);
+                                          const DevBC* __restrict__ bc, const int* __restrict__ bcface, int f,
+                                          int pos, double out[4]) {
+  if (B.bcf[f - 1] < 0) {
 #pragma unroll
-  for (int v = 0; v < 4; ++v) out[v] = src[v * pl];
+    for (int v = 0; v < 4; ++v) out[v] = 0.0;
+    return;
+  }
+  const DevBC& e = bc[bcface[B.bcf[f - 1] + pos - 1]];
+  const DevBlock& D = e.type == Q2D_BC_CONNECT ? blocks[e.dblk] : B;
+  const int gi = e.type == Q2D_BC_CONNECT ? e.is : e.i, gj = e.type == Q2D_BC_CONNECT ? e.js : e.j;
+  GP src = gp(D.irs[0]) + (long long)(gj + 1) * D.pitch + (gi + 1);
+#pragma unroll
+  for (int v = 0; v < 4; ++v) out[v] = src[v * D.plane];
 }
 
-__global__ __launch_bounds__(256) void k_irs_i(const DevBlock* __restrict__ blocks, const DevCase* __restrict__ cases,
-                                               const DevCtrl* __restrict__ ctrl, const Item* __restrict__ items,
-                                               int item_lo, int item_hi, const DevBC* __restrict__ bc,
-                                               const int* __restrict__ bcface, int irk) {
-  const int lane = threadIdx.x & (kWave - 1);
-  const int wpg = blockDim.x / kWave;
-  const int wave0 = __builtin_amdgcn_readfirstlane(blockIdx.x * wpg + threadIdx.x / kWave);
-  const int nwaves = gridDim.x * wpg;
-  for (int it = item_lo + wave0; it < item_hi; it += nwaves) {
-    const Item item = items[it];
-    const DevBlock& B = blocks[item.blk];
-    const DevCase& C = cases[B.case_id];
-    if (!ctrl[B.case_id].active || irk > C.n_rk || !(C.irs_beta > 0.0)) continue;
-    const double beta = C.irs_beta, den = 1.0 + 2.0 * beta;
-    const int ni = B.ni, pitch = B.pitch;
-    const long long pl = B.plane;
-    const int x = item.x0 - 2 + lane;                  // lanes 0..63: columns x0-2 .. x0+61
-    const int xc = min(max(x, 0), ni + 1);
-    const bool own = lane >= 2 && lane < 2 + kOwn && x <= ni;
-    const bool inner = x >= 1 && x <= ni;
-    const bool ghost = x == 0 || x == ni + 1;
-    GP R0 = gp(B.irs[0]);
-    GPW R1 = gpw(B.irs[1]);
-    for (int r = item.y0; r <= item.y1; ++r) {
-      const long long o = (long long)(r + 1) * pitch + (xc + 1);
-      double q[4];
-      if (ghost) {
-        irs_ghost(blocks, B, bc, bcface, 0, x == 0 ? 1 : 2, r, q);
-      } else {
+// The i-smoothed residual of one cell (i, j) of block B, evaluated by one lane: the same expressions, in
+// the same order, as the lane-parallel form in k_irs.
+__device__ __forceinline__ void irs_i_at(const DevBlock* __restrict__ blocks, const DevBlock& B,
+                                         const DevBC* __restrict__ bc, const int* __restrict__ bcface, double beta,
+                                         double den, int i, int j, double out[4]) {
+  double q[5][4];
 #pragma unroll
-        for (int v = 0; v < 4; ++v) q[v] = R0[v * pl + o];
-      }
-      double s1[4];
+  for (int d = 0; d < 5; ++d) {
+    const int k = i - 2 + d;
+    if (k == 0 || k == B.ni + 1) {
+      irs_ghost(blocks, B, bc, bcface, k == 0 ? 1 : 2, j, q[d]);
+    } else if (k >= 1 && k <= B.ni) {
+      GP src = gp(B.irs[0]) + (long long)(j + 1) * B.pitch + (k + 1);
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const double qm = shup(q[v]), qp = shdn(q[v]);
-        s1[v] = inner ? (q[v] + beta * (qm + qp)) / den : q[v];
-      }
+      for (int v = 0; v < 4; ++v) q[d][v] = src[v * B.plane];
+    } else {
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const double sm = shup(s1[v]), sp = shdn(s1[v]);
-        const double s2 = (q[v] + beta * (sm + sp)) / den;
-        if (own) R1[v * pl + o] = s2;
-      }
+      for (int v = 0; v < 4; ++v) q[d][v] = 0.0;
     }
+  }
+  const bool inm = i - 1 >= 1, inp = i + 1 <= B.ni;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const double sm = inm ? (q[1][v] + beta * (q[0][v] + q[2][v])) / den : q[1][v];
+    const double sp = inp ? (q[3][v] + beta * (q[2][v] + q[4][v])) / den : q[3][v];
+    out[v] = (q[2][v] + beta * (sm + sp)) / den;
   }
 }
 
@@ -2096,10 +2075,10 @@ __device__ __forceinline__ double cell_dt_euler(const DevCase& C, int iter, cons
   return dtmin;
 }
 
-__global__ __launch_bounds__(256) void k_irs_j(const DevBlock* __restrict__ blocks, const DevCase* __restrict__ cases,
-                                               DevCtrl* __restrict__ ctrl, const Item* __restrict__ items, int item_lo,
-                                               int item_hi, const DevBC* __restrict__ bc, const int* __restrict__ bcface,
-                                               double* __restrict__ partials, int irk) {
+__global__ __launch_bounds__(256) void k_irs(const DevBlock* __restrict__ blocks, const DevCase* __restrict__ cases,
+                                             DevCtrl* __restrict__ ctrl, const Item* __restrict__ items, int item_lo,
+                                             int item_hi, const DevBC* __restrict__ bc, const int* __restrict__ bcface,
+                                             double* __restrict__ partials, int irk) {
   const int lane = threadIdx.x & (kWave - 1);
   const int wpg = blockDim.x / kWave;
   const int wave0 = __builtin_amdgcn_readfirstlane(blockIdx.x * wpg + threadIdx.x / kWave);
@@ -2112,7 +2091,7 @@ __global__ __launch_bounds__(256) void k_irs_j(const DevBlock* __restrict__ bloc
     if (!K.active || irk > C.n_rk || !(C.irs_beta > 0.0)) continue;
     const bool last = irk == C.n_rk;
     const double beta = C.irs_beta, den = 1.0 + 2.0 * beta;
-    const bool smooth = C.ndir == 2;                   // Residual_Smoothing's d = 1..ndir
+    const bool smooth_j = C.ndir == 2;                 // Residual_Smoothing's d = 1..ndir
     const int ni = B.ni, nj = B.nj, pitch = B.pitch;
     const long long pl = B.plane;
     const int kcur = __builtin_amdgcn_readfirstlane(K.cur), kdtsel = __builtin_amdgcn_readfirstlane(K.dtsel);
@@ -2120,27 +2099,53 @@ __global__ __launch_bounds__(256) void k_irs_j(const DevBlock* __restrict__ bloc
     const int bin = stage_in(kcur, irk), bout = stage_out(kcur, irk);
     GP Pin = gp(B.P[bin]), PO = gp(B.P[kcur]);
     GPW Pout = gpw(B.P[bout]);
-    GP R1 = gp(B.irs[1]);
+    GP Rs = gp(B.irs[0]);
     const bool tacc = C.time_accurate != 0;
     const bool mixed = C.precision == Q2D_PREC_MIXED;
     const Gas<double> gd{C.R, C.cp, C.cv, C.gam};
     const double c_rk = (C.n_rk == 1) ? 1.0 : (C.n_rk == 2 ? (irk == 1 ? 1.0 : 0.5)
                                                             : (irk == 1 ? 1.0 : (irk == 2 ? 0.25 : 2.0 / 3.0)));
     const int stage_id = K.stage_count + irk;
-    const int x = item.x0 - 2 + lane;
+    const int x = item.x0 - 2 + lane;                  // lanes 0..63: columns x0-2 .. x0+61
+    const int xc = min(max(x, 0), ni + 1);
     const bool own = lane >= 2 && lane < 2 + kOwn && x <= ni;
-    const int xc = min(max(x, 1), ni);
-    // R at row k of this lane's column: rows 0 and nj+1 are the ghosts of faces 3 and 4
-    auto row = [&](int k, double q[4]) {
-      if (own && (k == 0 || k == nj + 1)) {
-        irs_ghost(blocks, B, bc, bcface, 1, k == 0 ? 3 : 4, x, q);
-      } else if (own && k >= 1 && k <= nj) {
+    const bool inner = x >= 1 && x <= ni;
+    const bool ghostx = x == 0 || x == ni + 1;
+    // i-smoothed residual of interior row k at this lane (meaningful where 1 <= x <= ni); every lane
+    // takes part (DPP)
+    auto r1_interior = [&](int k, double out[4]) {
+      double q[4];
+      if (ghostx) {
+        irs_ghost(blocks, B, bc, bcface, x == 0 ? 1 : 2, k, q);
+      } else {
         const long long o = (long long)(k + 1) * pitch + (xc + 1);
 #pragma unroll
-        for (int v = 0; v < 4; ++v) q[v] = R1[v * pl + o];
-      } else {
+        for (int v = 0; v < 4; ++v) q[v] = Rs[v * pl + o];
+      }
+      double s1[4];
 #pragma unroll
-        for (int v = 0; v < 4; ++v) q[v] = 0.0;
+      for (int v = 0; v < 4; ++v) {
+        const double qm = shup(q[v]), qp = shdn(q[v]);
+        s1[v] = inner ? (q[v] + beta * (qm + qp)) / den : q[v];
+      }
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const double sm = shup(s1[v]), sp = shdn(s1[v]);
+        out[v] = (q[v] + beta * (sm + sp)) / den;
+      }
+    };
+    // row k of the i-smoothed field, ghost rows 0 and nj+1 included (k is wave-uniform)
+    auto r1row = [&](int k, double out[4]) {
+      if (k >= 1 && k <= nj) { r1_interior(k, out); return; }
+#pragma unroll
+      for (int v = 0; v < 4; ++v) out[v] = 0.0;
+      if (!smooth_j || (k != 0 && k != nj + 1)) return;
+      const int f = k == 0 ? 3 : 4;
+      if (B.bcf[f - 1] < 0) return;                   // no records: zero
+      r1_interior(k == 0 ? 1 : nj, out);              // zero-gradient: the boundary cell's own
+      if (own) {
+        const DevBC& e = bc[bcface[B.bcf[f - 1] + x - 1]];
+        if (e.type == Q2D_BC_CONNECT) irs_i_at(blocks, blocks[e.dblk], bc, bcface, beta, den, e.is, e.js, out);
       }
     };
     auto sweep1 = [&](int k, const double a[4], const double b[4], const double c[4], double o[4]) {
@@ -2148,35 +2153,49 @@ __global__ __launch_bounds__(256) void k_irs_j(const DevBlock* __restrict__ bloc
 #pragma unroll
       for (int v = 0; v < 4; ++v) o[v] = in ? (b[v] + beta * (a[v] + c[v])) / den : b[v];
     };
-    double Rm[4], R0[4], Rp[4], sA[4], sB[4];
+    double Rm[4], Rc[4], Rp[4], sA[4], sB[4];
     {
       double Rm2[4];
-      row(item.y0 - 2, Rm2); row(item.y0 - 1, Rm); row(item.y0, R0); row(item.y0 + 1, Rp);
-      sweep1(item.y0 - 1, Rm2, Rm, R0, sA);
-      sweep1(item.y0, Rm, R0, Rp, sB);
+      if (smooth_j) { r1row(item.y0 - 2, Rm2); r1row(item.y0 - 1, Rm); }
+      r1row(item.y0, Rc);
+      if (smooth_j) {
+        r1row(item.y0 + 1, Rp);
+        sweep1(item.y0 - 1, Rm2, Rm, Rc, sA);
+        sweep1(item.y0, Rm, Rc, Rp, sB);
+      }
     }
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     double dtnext = 1e300;
     for (int r = item.y0; r <= item.y1; ++r) {
-      double Rp2[4], sC[4], s2[4];
-      row(r + 2, Rp2);
-      sweep1(r + 1, R0, Rp, Rp2, sC);
-#pragma unroll
-      for (int v = 0; v < 4; ++v) s2[v] = smooth ? (R0[v] + beta * (sA[v] + sC[v])) / den : R0[v];
-      if (own) {
-        const long long o = (long long)(r + 1) * pitch + (x + 1);
-        double q[4], po[4], Uo[4], U[4], qn2[4];
+      const long long o = (long long)(r + 1) * pitch + (xc + 1);
+      double q[4], po[4];
+      if (own) {                                       // the row's state first: in flight during the sweeps
 #pragma unroll
         for (int v = 0; v < 4; ++v) { q[v] = Pin[v * pl + o]; po[v] = PO[v * pl + o]; }
+      }
+      double s2[4];
+      if (smooth_j) {
+        double Rp2[4], sC[4];
+        r1row(r + 2, Rp2);
+        sweep1(r + 1, Rc, Rp, Rp2, sC);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) s2[v] = (Rc[v] + beta * (sA[v] + sC[v])) / den;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) { sA[v] = sB[v]; sB[v] = sC[v]; Rm[v] = Rc[v]; Rc[v] = Rp[v]; Rp[v] = Rp2[v]; }
+      } else {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) s2[v] = Rc[v];
+        if (r < item.y1) r1row(r + 1, Rc);
+      }
+      if (own) {
+        double Uo[4], U[4], qn2[4];
         prim2cons(gd, po[0], po[1], po[2], po[3], Uo);
         prim2cons(gd, q[0], q[1], q[2], q[3], U);
-        if (!mixed) {
 #pragma unroll
-          for (int v = 0; v < 4; ++v) U[v] = Uo[v] + (U[v] - Uo[v] + s2[v]) * c_rk;   // RK_stage — Lib_RK.f90:15-27
+        for (int v = 0; v < 4; ++v) U[v] = Uo[v] + (U[v] - Uo[v] + s2[v]) * c_rk;   // RK_stage — Lib_RK.f90:15-27
+        if (!mixed) {
           cons2prim(gd, U, qn2);
         } else {
-#pragma unroll
-          for (int v = 0; v < 4; ++v) U[v] = Uo[v] + (U[v] - Uo[v] + s2[v]) * c_rk;
           const double ir = rcp64(U[0]);
           const double vx = U[1] * ir, vy = U[2] * ir;
           const double e = U[3] * ir - 0.5 * (vx * vx + vy * vy);
@@ -2207,8 +2226,6 @@ __global__ __launch_bounds__(256) void k_irs_j(const DevBlock* __restrict__ bloc
           if (dtmin < dtnext) dtnext = dtmin;
         }
       }
-#pragma unroll
-      for (int v = 0; v < 4; ++v) { sA[v] = sB[v]; sB[v] = sC[v]; R0[v] = Rp[v]; Rp[v] = Rp2[v]; }
     }
     if (last) {                                        // the stage kernel's per-item partials (deterministic)
 #pragma unroll
@@ -2218,8 +2235,8 @@ __global__ __launch_bounds__(256) void k_irs_j(const DevBlock* __restrict__ bloc
         if (lane == 0) partials[(long long)it * kPartials + v] = s;
       }
       for (int k = 32; k >= 1; k >>= 1) {
-        const double o = __shfl_xor(dtnext, k, kWave);
-        if (o < dtnext) dtnext = o;
+        const double od = __shfl_xor(dtnext, k, kWave);
+        if (od < dtnext) dtnext = od;
       }
       if (lane == 0) partials[(long long)it * kPartials + 4] = dtnext;
     }
@@ -2351,7 +2368,7 @@ struct q2d_handle {
   DtRange* d_dtr = nullptr;
   int dt_total = 0;
   bool any_visc = false;
-  bool any_irs = false;          // a live case smooths its residual (k_irs_i, k_irs_j after each stage)
+  bool any_irs = false;          // a live case smooths its residual (k_irs after each stage)
   bool bc_two_phase = false;     // a live block is one cell wide: ghost fill and BC fluxes in two launches
   std::vector<int> vrec;         // boundary records with a viscous flux (BC 300/302 of model-1 cases)
   int* d_vrec = nullptr;
@@ -2637,12 +2654,9 @@ int launch_step(q2d_handle* h) {
     }
     int rc = h->precision_mixed ? launch_stage<float, false>(h, irk) : launch_stage<double, kFuseDtFp64>(h, irk);
     if (rc != Q2D_OK) return rc;
-    if (h->any_irs) {                                  // residual smoothing along i, then along j + the update
-      hipLaunchKernelGGL(k_irs_i, dim3(groups), dim3(tpb), 0, h->stream, h->d_blocks, h->d_cases, h->d_ctrl,
-                         h->d_items, 0, n_items, h->d_bc, h->d_bcface, irk);
-      hipLaunchKernelGGL(k_irs_j, dim3(groups), dim3(tpb), 0, h->stream, h->d_blocks, h->d_cases, h->d_ctrl,
+    if (h->any_irs)                                    // residual smoothing along i and j, then the update
+      hipLaunchKernelGGL(k_irs, dim3(groups), dim3(tpb), 0, h->stream, h->d_blocks, h->d_cases, h->d_ctrl,
                          h->d_items, 0, n_items, h->d_bc, h->d_bcface, h->d_partials, irk);
-    }
   }
   const int fused_dt = (!h->any_visc && (h->precision_mixed ? kFuseDtMx : kFuseDtFp64)) ? 1 : 0;
   hipLaunchKernelGGL(k_finish, dim3(h->ncases), dim3(kFinishThreads), 0, h->stream, h->d_ctrl, h->d_cases, h->d_partials, h->ncases,
@@ -2942,14 +2956,14 @@ int append_cases(q2d_handle* h, const q2d_problem* prob, int32_t* first_case) {
   // ---- blocks: SoA planes in one arena per block ----
   // planes 0-11: 3 state buffers x 4; 12-24 geometry; 25, 28 local dt; 26, 27 source terms; 29 node radius;
   // viscous blocks add 30-38: diffusive fluxes of i- and j-faces and the viscous source, 3 planes each;
-  // blocks of IRS cases add 8 after those: the scaled residual and its smoothing along i, 4 planes each
+  // blocks of IRS cases add 4 after those: the stage's scaled residual
   for (int lb = 0; lb < prob->nblocks; ++lb) {
     const q2d_block_desc& d = prob->blocks[lb];
     const bool visc = prob->cases[d.case_id].scheme.model > 0;
     const bool mxgeo = prec == Q2D_PREC_MIXED && !visc;   // FP32 geometry planes of k_stage_mx
     const bool irs = prob->cases[d.case_id].scheme.irs_beta > 0;
     const int irs_plane = visc ? 39 : (mxgeo ? 37 : 30);
-    const int kPlanes = irs_plane + (irs ? 8 : 0);
+    const int kPlanes = irs_plane + (irs ? 4 : 0);
     h->hb.emplace_back();
     h->db.emplace_back();
     h->nblocks += 1;
@@ -3047,7 +3061,7 @@ int append_cases(q2d_handle* h, const q2d_problem* prob, int32_t* first_case) {
       db.mgd = reinterpret_cast<const float4*>(a + 33 * pl); db.mhy = reinterpret_cast<const float*>(a + 35 * pl);
       db.msrc = reinterpret_cast<const float2*>(a + 36 * pl);
     }
-    if (irs) { db.irs[0] = a + (size_t)irs_plane * pl; db.irs[1] = a + (size_t)(irs_plane + 4) * pl; }
+    if (irs) db.irs[0] = a + (size_t)irs_plane * pl;
     for (int f = 0; f < 4; ++f) db.bcf[f] = -1;
   }
   h->dbc.insert(h->dbc.end(), newbc.begin(), newbc.end());

@@ -17,6 +17,7 @@ pytestmark = pytest.mark.gpu
 @pytest.mark.skipif(not DRIVER.exists(), reason="Fortran driver not built (make -C q2d_amd/fortran)")
 def test_fortran_host_matches_python_host_and_oracle(oracle_mod, tmp_path):
     pb = cases.planar_shock(seed=6, n=48, steps=30)
+    pb.cases[0].irs_beta = 0.5                       # every field of the scheme crosses the boundary
     pb.add(cases.axisymmetric_pipe(seed=2, ni=64, nj=24, first_cell=5e-4, reynolds=1e4, precision="fp64", steps=30))
     write_case_file(pb, tmp_path / "case.bin", 30)
     r = subprocess.run([str(DRIVER), str(tmp_path / "case.bin"), str(tmp_path / "out.bin")],
