@@ -82,6 +82,8 @@ struct DevCase {
   double mu_ref, T_ref, S_suth, Pr;     // Sutherland viscosity, Prandtl number (model 1)
   double res_drop;                      // > 0: stop when residual(1) <= res_drop * residual(1) of step 1
   double irs_beta;                      // > 0: implicit residual smoothing (k_irs)
+  int irs_begin, irs_end;               // the case's k_irs work items (their partials replace the stage's)
+  int pad_irs[2];
 };
 
 struct DevCtrl {
@@ -2111,10 +2113,8 @@ __global__ __launch_bounds__(256) void k_irs(const DevBlock* __restrict__ blocks
     const bool own = lane >= 2 && lane < 2 + kOwn && x <= ni;
     const bool inner = x >= 1 && x <= ni;
     const bool ghostx = x == 0 || x == ni + 1;
-    // i-smoothed residual of interior row k at this lane (meaningful where 1 <= x <= ni); every lane
-    // takes part (DPP)
-    auto r1_interior = [&](int k, double out[4]) {
-      double q[4];
+    // scaled residual of interior row k at this lane, the i-ghost at x = 0 / ni + 1
+    auto raw_load = [&](int k, double q[4]) {
       if (ghostx) {
         irs_ghost(blocks, B, bc, bcface, x == 0 ? 1 : 2, k, q);
       } else {
@@ -2122,6 +2122,9 @@ __global__ __launch_bounds__(256) void k_irs(const DevBlock* __restrict__ blocks
 #pragma unroll
         for (int v = 0; v < 4; ++v) q[v] = Rs[v * pl + o];
       }
+    };
+    // i-smoothing of one row (meaningful where 1 <= x <= ni); every lane takes part (DPP)
+    auto isweep = [&](const double q[4], double out[4]) {
       double s1[4];
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
@@ -2133,6 +2136,11 @@ __global__ __launch_bounds__(256) void k_irs(const DevBlock* __restrict__ blocks
         const double sm = shup(s1[v]), sp = shdn(s1[v]);
         out[v] = (q[v] + beta * (sm + sp)) / den;
       }
+    };
+    auto r1_interior = [&](int k, double out[4]) {
+      double q[4];
+      raw_load(k, q);
+      isweep(q, out);
     };
     // row k of the i-smoothed field, ghost rows 0 and nj+1 included (k is wave-uniform)
     auto r1row = [&](int k, double out[4]) {
@@ -2166,17 +2174,36 @@ __global__ __launch_bounds__(256) void k_irs(const DevBlock* __restrict__ blocks
     }
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     double dtnext = 1e300;
-    for (int r = item.y0; r <= item.y1; ++r) {
-      const long long o = (long long)(r + 1) * pitch + (xc + 1);
-      double q[4], po[4];
-      if (own) {                                       // the row's state first: in flight during the sweeps
+    // software pipeline: the state of row r + 1 and the residual row r + 3 are loaded one iteration
+    // ahead, so each iteration's loads are in flight during the previous one's arithmetic
+    double qN[4] = {0.0, 0.0, 0.0, 0.0}, poN[4] = {0.0, 0.0, 0.0, 0.0}, rawN[4] = {0.0, 0.0, 0.0, 0.0};
+    auto state_load = [&](int k, double q[4], double po[4]) {
+      if (own) {
+        const long long o = (long long)(k + 1) * pitch + (xc + 1);
 #pragma unroll
         for (int v = 0; v < 4; ++v) { q[v] = Pin[v * pl + o]; po[v] = PO[v * pl + o]; }
       }
+    };
+    state_load(item.y0, qN, poN);
+    if (smooth_j && item.y0 + 2 <= nj) raw_load(item.y0 + 2, rawN);
+    for (int r = item.y0; r <= item.y1; ++r) {
+      const long long o = (long long)(r + 1) * pitch + (xc + 1);
+      double q[4], po[4];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) { q[v] = qN[v]; po[v] = poN[v]; }
+      if (r < item.y1) state_load(r + 1, qN, poN);
       double s2[4];
       if (smooth_j) {
         double Rp2[4], sC[4];
-        r1row(r + 2, Rp2);
+        if (r + 2 <= nj) {                             // interior row r + 2 from the prefetched residual
+          double raw[4];
+#pragma unroll
+          for (int v = 0; v < 4; ++v) raw[v] = rawN[v];
+          if (r < item.y1 && r + 3 <= nj) raw_load(r + 3, rawN);
+          isweep(raw, Rp2);
+        } else {
+          r1row(r + 2, Rp2);                           // ghost row nj + 1, or beyond
+        }
         sweep1(r + 1, Rc, Rp, Rp2, sC);
 #pragma unroll
         for (int v = 0; v < 4; ++v) s2[v] = (Rc[v] + beta * (sA[v] + sC[v])) / den;
@@ -2266,12 +2293,14 @@ __global__ __launch_bounds__(kFinishThreads) void k_finish(DevCtrl* ctrl, const 
   __shared__ double sh[5][kFinishThreads / kWave];
   double s[4] = {0, 0, 0, 0};
   double dmin = 1e300;
-  for (int it0 = C.item_begin + threadIdx.x; it0 < C.item_end; it0 += kFinishUnroll * kFinishThreads) {
+  // a smoothed case's last stage ends in k_irs, whose items hold the partials
+  const int ib = C.irs_beta > 0.0 ? C.irs_begin : C.item_begin, ie = C.irs_beta > 0.0 ? C.irs_end : C.item_end;
+  for (int it0 = ib + threadIdx.x; it0 < ie; it0 += kFinishUnroll * kFinishThreads) {
     double a[kFinishUnroll][5];
 #pragma unroll
     for (int u = 0; u < kFinishUnroll; ++u) {
       const int it = it0 + u * kFinishThreads;
-      const bool in = it < C.item_end;
+      const bool in = it < ie;
 #pragma unroll
       for (int v = 0; v < 4; ++v) a[u][v] = in ? partials[(long long)it * kPartials + v] : 0.0;
       a[u][4] = (in && fused_dt) ? partials[(long long)it * kPartials + 4] : 1e300;
@@ -2354,6 +2383,8 @@ struct q2d_handle {
   std::vector<DevBC> dbc;
   std::vector<int> bcface;
   std::vector<Item> items;
+  std::vector<Item> irs_items;   // k_irs work items, stored after `items` in d_items (own row count)
+  int irs_rows = 0;
   DevBlock* d_blocks = nullptr;
   DevCase* d_cases = nullptr;
   DevCtrl* d_ctrl = nullptr;
@@ -2510,6 +2541,38 @@ int build_items(q2d_handle* h) {
     }
   }
   h->range[4] = (int)h->items.size();
+  // k_irs items: the same 60-column strips, rows per item balanced for k_irs's own occupancy (the stage
+  // kernel's row count leaves k_irs with a part-full last round of wavefronts)
+  h->irs_items.clear();
+  {
+    std::vector<std::pair<int, int>> strips;
+    for (int b = 0; b < h->nblocks; ++b)
+      if (h->hb[b].arena && h->case_live[h->hb[b].case_id] && h->dc[h->hb[b].case_id].irs_beta > 0.0)
+        strips.emplace_back((h->hb[b].ni + kOwn - 1) / kOwn, h->hb[b].nj);
+    int H = 16;
+    if (!strips.empty()) {
+      int per_cu = 0;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&k_irs),
+                                                       kWave * h->waves_per_group, 0) != hipSuccess || per_cu <= 0)
+        per_cu = 3 * 4 / h->waves_per_group;
+      H = h->rows_per_item > 0 ? h->rows_per_item
+                               : balanced_rows(strips, (long long)per_cu * h->waves_per_group * h->cu_count);
+    }
+    h->irs_rows = H;
+    const int base = (int)h->items.size();
+    for (int c = 0; c < h->ncases; ++c) {
+      h->dc[c].irs_begin = h->dc[c].irs_end = 0;
+      if (!h->case_live[c] || !(h->dc[c].irs_beta > 0.0)) continue;
+      h->dc[c].irs_begin = base + (int)h->irs_items.size();
+      for (int b = 0; b < h->nblocks; ++b) {
+        if (h->hb[b].case_id != c) continue;
+        const int ni = h->hb[b].ni, nj = h->hb[b].nj;
+        for (int y0 = 1; y0 <= nj; y0 += H)
+          for (int x0 = 1; x0 <= ni; x0 += kOwn) h->irs_items.push_back(Item{b, x0, y0, std::min(nj, y0 + H - 1)});
+      }
+      h->dc[c].irs_end = base + (int)h->irs_items.size();
+    }
+  }
   h->dtr.clear();
   for (int b = 0; b < h->nblocks; ++b) {
     if (!h->hb[b].arena || !h->case_live[h->hb[b].case_id]) continue;
@@ -2523,9 +2586,13 @@ int build_items(q2d_handle* h) {
     HIPCHK(hipMemcpy(h->d_dtr, h->dtr.data(), sizeof(DtRange) * h->dtr.size(), hipMemcpyHostToDevice));
   if (h->d_items) { (void)hipFree(h->d_items); h->d_items = nullptr; }
   if (h->d_partials) { (void)hipFree(h->d_partials); h->d_partials = nullptr; }
-  HIPCHK(hipMalloc(&h->d_items, sizeof(Item) * std::max<size_t>(1, h->items.size())));
-  HIPCHK(hipMalloc(&h->d_partials, sizeof(double) * kPartials * std::max<size_t>(1, h->items.size())));
+  const size_t nall = h->items.size() + h->irs_items.size();
+  HIPCHK(hipMalloc(&h->d_items, sizeof(Item) * std::max<size_t>(1, nall)));
+  HIPCHK(hipMalloc(&h->d_partials, sizeof(double) * kPartials * std::max<size_t>(1, nall)));
   HIPCHK(hipMemcpy(h->d_items, h->items.data(), sizeof(Item) * h->items.size(), hipMemcpyHostToDevice));
+  if (!h->irs_items.empty())
+    HIPCHK(hipMemcpy(h->d_items + h->items.size(), h->irs_items.data(), sizeof(Item) * h->irs_items.size(),
+                     hipMemcpyHostToDevice));
   (void)hipFree(h->d_cases); h->d_cases = nullptr;
   HIPCHK(hipMalloc(&h->d_cases, sizeof(DevCase) * std::max<size_t>(1, h->dc.size())));
   if (!h->dc.empty())
@@ -2654,9 +2721,13 @@ int launch_step(q2d_handle* h) {
     }
     int rc = h->precision_mixed ? launch_stage<float, false>(h, irk) : launch_stage<double, kFuseDtFp64>(h, irk);
     if (rc != Q2D_OK) return rc;
-    if (h->any_irs)                                    // residual smoothing along i and j, then the update
-      hipLaunchKernelGGL(k_irs, dim3(groups), dim3(tpb), 0, h->stream, h->d_blocks, h->d_cases, h->d_ctrl,
-                         h->d_items, 0, n_items, h->d_bc, h->d_bcface, h->d_partials, irk);
+    if (h->any_irs && !h->irs_items.empty()) {        // residual smoothing along i and j, then the update
+      const int n_irs = (int)h->irs_items.size();
+      int ig = (n_irs + h->waves_per_group - 1) / h->waves_per_group;
+      if (h->max_groups > 0) ig = std::min(ig, h->max_groups);
+      hipLaunchKernelGGL(k_irs, dim3(std::max(ig, 1)), dim3(tpb), 0, h->stream, h->d_blocks, h->d_cases, h->d_ctrl,
+                         h->d_items, n_items, n_items + n_irs, h->d_bc, h->d_bcface, h->d_partials, irk);
+    }
   }
   const int fused_dt = (!h->any_visc && (h->precision_mixed ? kFuseDtMx : kFuseDtFp64)) ? 1 : 0;
   hipLaunchKernelGGL(k_finish, dim3(h->ncases), dim3(kFinishThreads), 0, h->stream, h->d_ctrl, h->d_cases, h->d_partials, h->ncases,
