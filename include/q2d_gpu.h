@@ -12,7 +12,8 @@
  * (Lib_Ghost.f90:16-140), Fluxes (Mod_Fluxes.f90:97-234), BC_Fluxes
  * (Mod_BC_Fluxes.f90:49-271), Ghost_Wall_Extrapolation (Lib_Ghost.f90:366),
  * Compute_Sources (Mod_Sources.f90:79-91) and RK_Newstate
- * (Lib_Newstate.f90:14-161), then Compute_Residual
+ * (Lib_Newstate.f90:14-161, with Residual_Smoothing, Lib_IRS.f90:17-177, for
+ * cases with irs-beta > 0), then Compute_Residual
  * (Mod_Diagnostic.f90:438-469) — runs on the GPU behind these entry points.
  *
  * Memory layouts accepted at the boundary are the reference's own Fortran
