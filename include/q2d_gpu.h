@@ -109,6 +109,11 @@ typedef struct q2d_scheme {
   double  irs_beta;        /* irs-beta (Register_Numerics.f90:53): > 0 enables implicit residual smoothing
                               (Assign_Setup.f90:61; Lib_IRS.f90:17-177 from Newstate_Conservative,
                               Lib_Newstate.f90:35-64); 0 leaves it off */
+  int32_t space_sd;        /* 1: space-reconstruction = 'MUSCL-SD' (obj_space_scheme%SD, Assign_Setup.f90:
+                              49-53): Tramel's pressure detector (Lib_Shock_Detector.f90:14-38) scales the
+                              MUSCL slopes of every interior and block-connection face; 0: 'MUSCL'.
+                              Plain FP64 form only. */
+  int32_t reserved_sd;
 } q2d_scheme;
 
 /* One structured block, in the reference's allocation layout (see top). */

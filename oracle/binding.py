@@ -38,6 +38,8 @@ def load():
     for f in ("q2do_get_state", "q2do_set_state", "q2do_get_dt", "q2do_get_residual", "q2do_set_residual"):
         getattr(lib, f).argtypes = [C.c_void_p, C.c_int, C.c_void_p]
         getattr(lib, f).restype = C.c_int
+    lib.q2do_sd_tramel.argtypes = [C.c_double] * 5
+    lib.q2do_sd_tramel.restype = C.c_double
     lib.q2do_irs_smooth.argtypes = [C.c_void_p, C.c_int, C.c_double]
     lib.q2do_irs_smooth.restype = C.c_int
     lib.q2do_set_threads.argtypes = [C.c_int]

@@ -302,6 +302,7 @@ typedef struct {
   double *node;                       /* node(0:ni,0:nj)%c(2) (optional) */
   double *sa, *sb;                    /* (1:ni,1:nj) source geometry (optional) */
   double *dtlocal;                    /* (1:ni,1:nj) */
+  double *beta;                       /* (1:ni,1:nj) shock detector of the stage (MUSCL-SD cases) */
 } oblk;
 
 static inline int cidx(const oblk *b, int i, int j) { return (j + 1) * b->ngi + (i + 1); }
@@ -372,6 +373,7 @@ q2do *q2do_create(const q2d_problem *prob) {
     k->sa = d->src_a ? dupd(d->src_a, (size_t)d->ni * d->nj) : NULL;
     k->sb = d->src_b ? dupd(d->src_b, (size_t)d->ni * d->nj) : NULL;
     k->dtlocal = dupd(NULL, (size_t)d->ni * d->nj);
+    k->beta = dupd(NULL, (size_t)d->ni * d->nj);
   }
   for (int c = 0; c < o->ncases; c++) {
     ocase *cs = &o->cs[c];
@@ -417,7 +419,7 @@ void q2do_destroy(q2do *o) {
   for (int b = 0; b < o->nblocks; b++) {
     oblk *k = &o->blk[b];
     free(k->P); free(k->PO); free(k->R); free(k->dl); free(k->M); free(k->sur);
-    free(k->fi); free(k->fj); free(k->node); free(k->sa); free(k->sb); free(k->dtlocal);
+    free(k->fi); free(k->fj); free(k->node); free(k->sa); free(k->sb); free(k->dtlocal); free(k->beta);
   }
   free(o->blk); free(o->bc); free(o->cs); free(o);
 }
@@ -539,17 +541,38 @@ static void fill_ghost(q2do *o, ocase *cs) {
 }
 
 /* ------------------------------------------------------------------------ */
+/* SD_Tramel — Lib_Shock_Detector.f90:14-38 (DELTA = 20): p at the cell, p1 / p2 its i-1 / i+1 and      */
+/* p3 / p4 its j-1 / j+1 neighbours (pmatrix(0,1), (2,1), (1,0), (1,2)).                               */
+/* tanh(x) is evaluated as x - x*x*x/3: here x = 10 phi^3 <= 10 (1/400)^3 = 1.6e-7, where the next     */
+/* term of the series, 2x^5/15, is below 1e-28 x, so this is tanh rounded to nearest (the device       */
+/* evaluates the same expression; the reference's libm tanh agrees to within one rounding of t).       */
+/* ------------------------------------------------------------------------ */
+static double sd_tramel(double p, double p1, double p2, double p3, double p4) {
+  const double DELTA = 20.0;
+  double kp1 = fabs((p2 - 2.0 * p + p1) / (p2 + 2.0 * p + p1));
+  double kp2 = fabs((p4 - 2.0 * p + p3) / (p4 + 2.0 * p + p3));
+  double sw = kp1 > kp2 ? kp1 : kp2;
+  if (sw < 1.0 / DELTA) {
+    double phi = sw / DELTA;
+    if (phi < 0.0) phi = 0.0;
+    double x = 10.0 * phi * phi * phi;
+    double t = x - x * x * x / 3.0;
+    return 1.0 - t;
+  }
+  return 0.0;
+}
+
+/* ------------------------------------------------------------------------ */
 /* Convective_Flux — Lib_Convective.f90:15-82                                */
 /* ------------------------------------------------------------------------ */
 /* dl[-1..2] -> dl[0..3], prim[c][v] for cells -1..2 -> c = 0..3. Returns 0 or Q2D_E_INTERFACE. */
 static int convective_flux(int lim, int rid, const gas_t *g, const double *dl, double nx, double ny,
-                           double int_l, const double (*prim)[NP], double *Flux) {
+                           double int_l, const double (*prim)[NP], double beta_, double *Flux) {
   double l0 = 0.5 * (dl[0] + dl[1]);
   double l1 = 0.5 * (dl[1] + dl[2]);
   double l2 = 0.5 * (dl[2] + dl[3]);
   double lm = 0.5 * dl[1];
   double lp = 0.5 * dl[2];
-  double beta_ = 1.0;
   double P1[NP], P4[NP];
   for (int v = 0; v < NP; v++)
     reconstruct(lim, prim[0][v], prim[1][v], prim[2][v], prim[3][v], l0, l1, l2, lm, lp, beta_, &P1[v], &P4[v]);
@@ -676,6 +699,14 @@ static int fluxes_block(oblk *b, const ocase *cs) {
   for (int j = 1; j <= b->nj; j++)
     for (int i = 1; i <= b->ni; i++)
       for (int v = 0; v < NP; v++) RV(b, v, i, j) = 0.0;
+  /* shock detector on Prim(np, i-1:i+1, j-1:j+1) (Mod_Fluxes.f90:129-138); the faces take the value of
+   * their cell i (i-faces, :144-150) or j (j-faces, :191-197), with SD_limiter (Lib_Convective.f90:42-46) */
+  const int sd = s->space_sd != 0;
+  if (sd)
+    for (int j = 1; j <= b->nj; j++)
+      for (int i = 1; i <= b->ni; i++)
+        b->beta[IN(b, i, j)] = sd_tramel(PV(b, 3, i, j), PV(b, 3, i - 1, j), PV(b, 3, i + 1, j), PV(b, 3, i, j - 1),
+                                         PV(b, 3, i, j + 1));
   /* i-faces: rows in parallel, each face once (:140-185) */
 #ifdef _OPENMP
 #pragma omp parallel for schedule(static) reduction(| : err)
@@ -688,7 +719,7 @@ static int fluxes_block(oblk *b, const ocase *cs) {
         for (int v = 0; v < NP; v++) prim[c][v] = PV(b, v, i - 1 + c, j);
       }
       err |= -convective_flux(s->limiter, s->riemann, &cs->gas, dl, FI(b, 0, i, j), FI(b, 1, i, j),
-                              FI(b, 2, i, j), (const double(*)[NP])prim, F);
+                              FI(b, 2, i, j), (const double(*)[NP])prim, sd ? b->beta[IN(b, i, j)] : 1.0, F);
       for (int v = 0; v < NP; v++) { RV(b, v, i, j) = RV(b, v, i, j) + F[v]; RV(b, v, i + 1, j) = RV(b, v, i + 1, j) - F[v]; }
       if (s->model > 0) {
         double D[NP];
@@ -714,7 +745,7 @@ static int fluxes_block(oblk *b, const ocase *cs) {
         for (int v = 0; v < NP; v++) prim[c][v] = PV(b, v, i, j - 1 + c);
       }
       err |= -convective_flux(s->limiter, s->riemann, &cs->gas, dl, FJ(b, 0, i, j), FJ(b, 1, i, j),
-                              FJ(b, 2, i, j), (const double(*)[NP])prim, F);
+                              FJ(b, 2, i, j), (const double(*)[NP])prim, sd ? b->beta[IN(b, i, j)] : 1.0, F);
       for (int v = 0; v < NP; v++) { RV(b, v, i, j) = RV(b, v, i, j) + F[v]; RV(b, v, i, j + 1) = RV(b, v, i, j + 1) - F[v]; }
       if (s->model > 0) {
         double D[NP];
@@ -768,9 +799,11 @@ static int bc_connection_eul(oblk *b, const ocase *cs, int Im, int Jm, int Fm) {
   double Dist12 = Dl1 + Dl2;
   double Dist34 = Dl3 + Dl4;
   double PL[NP], PR[NP];
+  /* SD_limiter: the boundary cell's detector value of this stage (Lib_BC_Fluxes_Connection.f90:64-69) */
+  double beta_ = s->space_sd ? b->beta[IN(b, Im, Jm)] : 1.0;
   for (int v = 0; v < NP; v++)
     reconstruct(s->limiter, PV(b, v, I1, J1), PV(b, v, I2, J2), PV(b, v, I3, J3), PV(b, v, I4, J4),
-                Dist12, Dist23, Dist34, Dl2, Dl3, 1.0, &PL[v], &PR[v]);
+                Dist12, Dist23, Dist34, Dl2, Dl3, beta_, &PL[v], &PR[v]);
   if (check_gas_state(PL[0], PL[3]) || check_gas_state(PR[0], PR[3])) return Q2D_E_INTERFACE;
   double rho_L = PL[0], rho_R = PR[0];
   double a_L = f_ss(&cs->gas, PL[3], rho_L);
@@ -1544,7 +1577,7 @@ void q2do_roe_averages(double gas_R, double gas_cp, const double *L, const doubl
 int q2do_convective_flux(int lim, int rid, double gas_R, double gas_cp, const double *dl,
                          double nx, double ny, double int_l, const double *prim, double *flux) {
   gas_t g = mkgas(gas_R, gas_cp);
-  return convective_flux(lim, rid, &g, dl, nx, ny, int_l, (const double(*)[NP])prim, flux);
+  return convective_flux(lim, rid, &g, dl, nx, ny, int_l, (const double(*)[NP])prim, 1.0, flux);
 }
 
 int q2do_get_residual(q2do *o, int block, double *R) {
@@ -1562,4 +1595,8 @@ int q2do_set_residual(q2do *o, int block, const double *R) {
 int q2do_irs_smooth(q2do *o, int case_id, double beta) {
   if (case_id < 0 || case_id >= o->ncases) return Q2D_E_ARG;
   return irs_smooth(o, &o->cs[case_id], beta);
+}
+
+double q2do_sd_tramel(double p, double p_im, double p_ip, double p_jm, double p_jp) {
+  return sd_tramel(p, p_im, p_ip, p_jm, p_jp);
 }

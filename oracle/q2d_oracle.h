@@ -54,6 +54,8 @@ void q2do_roe_averages(double gas_R, double gas_cp, const double *primL, const d
 int q2do_convective_flux(int limiter, int riemann, double gas_R, double gas_cp,
                          const double *dl, double nx, double ny, double int_l,
                          const double *prim, double *flux);
+/* SD_Tramel (Lib_Shock_Detector.f90:14-38) of a cell with pressure p and neighbours i-1, i+1, j-1, j+1. */
+double q2do_sd_tramel(double p, double p_im, double p_ip, double p_jm, double p_jp);
 /* Residual array R(nprim, 1-gc:ni+gc, 1-gc:nj+gc) of block `block` (0-based global), and
  * Residual_Smoothing (Lib_IRS.f90:17-44) of case `case_id` applied to those arrays as they stand. */
 int q2do_get_residual(q2do *o, int block, double *R);

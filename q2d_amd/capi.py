@@ -39,7 +39,7 @@ class Scheme(C.Structure):
                 ("model", C.c_int32), ("source", C.c_int32), ("precision", C.c_int32), ("ramp_iter", C.c_int32),
                 ("cfl", C.c_double), ("vnn", C.c_double), ("gas_R", C.c_double), ("gas_cp", C.c_double),
                 ("mu_ref", C.c_double), ("T_ref", C.c_double), ("S_suth", C.c_double), ("Pr", C.c_double),
-                ("irs_beta", C.c_double)]
+                ("irs_beta", C.c_double), ("space_sd", C.c_int32), ("reserved_sd", C.c_int32)]
 
 
 class BlockDesc(C.Structure):
@@ -112,6 +112,7 @@ def to_problem_c(problem):
         s.cfl, s.vnn, s.gas_R, s.gas_cp = c.cfl, c.vnn, c.gas_R, c.gas_cp
         s.mu_ref, s.T_ref, s.S_suth, s.Pr = c.mu_ref, c.T_ref, c.S_suth, c.Pr
         s.irs_beta = c.irs_beta
+        s.space_sd = int(c.space_sd)
         cases[k].itermax, cases[k].res_threshold, cases[k].time_threshold = c.itermax, c.res_threshold, c.time_threshold
         cases[k].res_drop = c.res_drop
     pc = ProblemC()

@@ -55,6 +55,7 @@ class CaseSpec:
     time_threshold: float = 1e300
     res_drop: float = 0.0
     irs_beta: float = 0.0         # irs-beta: > 0 enables implicit residual smoothing
+    space_sd: bool = False        # space-reconstruction = MUSCL-SD (Tramel's shock detector on the slopes)
 
 
 @dataclass

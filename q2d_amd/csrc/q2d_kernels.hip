@@ -83,7 +83,8 @@ struct DevCase {
   double res_drop;                      // > 0: stop when residual(1) <= res_drop * residual(1) of step 1
   double irs_beta;                      // > 0: implicit residual smoothing (k_irs)
   int irs_begin, irs_end;               // the case's k_irs work items (their partials replace the stage's)
-  int pad_irs[2];
+  int space_sd;                         // MUSCL-SD: Tramel's detector scales the slopes (sd_tramel)
+  int pad_irs;
 };
 
 struct DevCtrl {
@@ -443,6 +444,19 @@ __device__ __forceinline__ void ghost_fill(const DevBlock* blocks, const DevBC& 
 // ---------------------------------------------------------------------------
 // F: convective boundary flux, already carrying the face sign modfm2, added where mask has the bit.
 // V: viscous boundary flux as -(modfm2 * Flux), added after F where vmask has the bit (BC 300/302, model 1).
+// SD_Tramel — Lib_Shock_Detector.f90:14-38 (DELTA = 20): p at the cell, p1 / p2 at i-1 / i+1, p3 / p4 at
+// j-1 / j+1.  tanh(x) for x = 10 phi^3 <= 1.6e-7 is x - x^3/3 (the next term of its series is below
+// 1e-28 x): tanh rounded to nearest, the same expression as the oracle's.
+__device__ __forceinline__ double sd_tramel(double p, double p1, double p2, double p3, double p4) {
+  const double kp1 = fabs((p2 - 2.0 * p + p1) / (p2 + 2.0 * p + p1));
+  const double kp2 = fabs((p4 - 2.0 * p + p3) / (p4 + 2.0 * p + p3));
+  const double sw = kp1 > kp2 ? kp1 : kp2;
+  if (!(sw < 1.0 / 20.0)) return 0.0;
+  const double phi = sw / 20.0;
+  const double x = 10.0 * phi * phi * phi;
+  return 1.0 - (x - x * x * x / 3.0);
+}
+
 struct BCFlux { double F[4]; double V[4]; int mask; int vmask; };
 
 // A record's result, stored at the record's boundary cell (DevBC::dpos): the BCFlux struct (read by the
@@ -559,6 +573,12 @@ __global__ __launch_bounds__(256) void k_bcflux(const DevBlock* __restrict__ blo
     const long long o1 = off(I1, J1), o2 = off(I2, J2), o3 = off(I3, J3), o4 = off(I4, J4);
     const double Dl1 = half_dl(o1), Dl2 = half_dl(o2), Dl3 = half_dl(o3), Dl4 = half_dl(o4);
     const FT Dist23 = (FT)(Dl2 + Dl3), Dist12 = (FT)(Dl1 + Dl2), Dist34 = (FT)(Dl3 + Dl4);
+    // MUSCL-SD: the boundary cell's detector value (Lib_BC_Fluxes_Connection.f90:64-69), from the state
+    // after every record's ghost fill (such handles run the two-phase launch)
+    FT bt = FT(1);
+    if (C.space_sd)
+      bt = (FT)sd_tramel(Bp[3], P[3 * pl + off(Im - 1, Jm)], P[3 * pl + off(Im + 1, Jm)], P[3 * pl + off(Im, Jm - 1)],
+                         P[3 * pl + off(Im, Jm + 1)]);
     FT L[4], R[4];
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
@@ -571,8 +591,8 @@ __global__ __launch_bounds__(256) void k_bcflux(const DevBlock* __restrict__ blo
       const FT s2 = qdiv<FT>((FT)(p4 - p3), Dist34);
       const FT sl = limiter<FT>(lim, s1, s0);
       const FT sr = limiter<FT>(lim, s2, s1);
-      L[v] = (FT)p2 + sl * (FT)Dl2;
-      R[v] = (FT)p3 - sr * (FT)Dl3;
+      L[v] = (FT)p2 + (bt * sl) * (FT)Dl2;
+      R[v] = (FT)p3 - (bt * sr) * (FT)Dl3;
     }
     const bool bad = !(L[0] > FT(0) && L[3] > FT(0) && R[0] > FT(0) && R[3] > FT(0));
     if (bad) record_fail(&ctrl[E.cid], Q2D_E_INTERFACE, fail_key(K.stage_count + irk, 0, B.blk_in_case, Jm, Im));
@@ -1124,12 +1144,12 @@ __device__ __forceinline__ void limiter4(int lim, const FT x[4], const FT y[4], 
 // interface state (check_gas_state).
 template <typename FT, int RIE>
 __device__ __forceinline__ bool face_flux(const Gas<FT>& g, const double qa[4], const FT pa[4], FT lm, const double qb[4],
-                                          const FT pb[4], FT lp, FT nx, FT ny, FT il, FT out[4]) {
+                                          const FT pb[4], FT lp, FT bt, FT nx, FT ny, FT il, FT out[4]) {
   FT qL[4], qR[4];
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
-    qL[v] = (FT)qa[v] + pa[v] * lm;                   // priml = local + beta*slopel*dll
-    qR[v] = (FT)qb[v] - pb[v] * lp;                   // primr = next  - beta*sloper*dlr
+    qL[v] = (FT)qa[v] + (bt * pa[v]) * lm;            // priml = local + beta*slopel*dll
+    qR[v] = (FT)qb[v] - (bt * pb[v]) * lp;            // primr = next  - beta*sloper*dlr
   }
   const bool bad = !(qL[0] > FT(0) && qL[3] > FT(0) && qR[0] > FT(0) && qR[3] > FT(0));
   const FT aL = sound<FT>(g, qL[3], qL[0]);
@@ -1182,6 +1202,7 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
       constexpr int RIE = decltype(riec)::value;
       const bool last = (irk == C.n_rk);
       const bool irs = C.irs_beta > 0.0;                 // residual to B.irs[0]; k_irs smooths and updates
+      const bool sd = C.space_sd != 0;                   // MUSCL-SD: faces take their cell's detector value
       const int ni = B.ni, nj = B.nj, pitch = B.pitch;
       const long long pl = B.plane;
       // the case's control words, read once per item (wave-uniform; a read inside the row loop would be a
@@ -1220,6 +1241,7 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
       // ---- pipeline warm-up: rows y0-2, y0-1, y0 ----------------------------
       constexpr int kPrefetchRows = sizeof(FT) == 8 ? Q2D_PREFETCH_ROWS_FP64 : Q2D_PREFETCH_ROWS_MIXED;
       double qA[4], qB[4], qC[4], qN[4], qN2[4];
+      double pPrev = 0.0;                                // p of row r-1 (the detector's j-1 neighbour)
       double dlyA, dlyB, dlyC, dlyN, dlyN2 = 0.0;
       FT sAB[4], phiA[4], Gprev[4];
       RowGeo gcur;                                       // row r: i-face and update data, loaded at the top of its iteration
@@ -1238,6 +1260,7 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
           dlyN2 = dlyp[o4];
         }
         const double dlm = dlyp[o0];
+        pPrev = qm[3];
         dlyA = dlyp[o1]; dlyB = dlyp[o2]; dlyN = dlyp[o3];
         { const long long og = at(item.y0 - 1); fjxN = fjxp[og]; fjyN = fjyp[og]; fjlN = fjlp[og]; }
         const FT l0 = (FT)(0.5 * (dlm + dlyA)), l1 = (FT)(0.5 * (dlyA + dlyB));
@@ -1317,6 +1340,12 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
           for (int v = 0; v < 4; ++v) sBC[v] = qdiv<FT>((FT)(qC[v] - qB[v]), lBC);
           limiter4<FT>(lim, sBC, sAB, phiB);            // phi_{r+1} = rlimiter(s(r+1|r+2), s(r|r+1))
         }
+        // Tramel's detector of cell (x, r) (Mod_Fluxes.f90:129-138): its i-face x|x+1 and j-face r|r+1
+        FT bt = FT(1);
+        if (sd) {
+          const double pc = qA[3], pim = shup(qA[3]), pip = shdn(qA[3]);
+          bt = (FT)sd_tramel(pc, pim, pip, pPrev, qB[3]);
+        }
         double Ecur[3] = {0.0, 0.0, 0.0};
         if (VISC && own && r >= 1 && r <= nj - 1) {
           const long long oE = at(r);
@@ -1325,7 +1354,7 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
         }
         // interior j-face (r | r+1) — Mod_Fluxes.f90:187-232
         if (own && r >= 1 && r <= nj - 1) {
-          if (face_flux<FT, RIE>(gf, qA, phiA, (FT)(0.5 * dlyA), qB, phiB, (FT)(0.5 * dlyB), (FT)gcur.fjx,
+          if (face_flux<FT, RIE>(gf, qA, phiA, (FT)(0.5 * dlyA), qB, phiB, (FT)(0.5 * dlyB), bt, (FT)gcur.fjx,
                                  (FT)gcur.fjy, (FT)gcur.fjl, Gcur))
             record_fail(&K, Q2D_E_INTERFACE, fail_key(stage_id, 0, B.blk_in_case, r, x));
         } else {
@@ -1353,7 +1382,7 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
           for (int v = 0; v < 4; ++v) phxn[v] = shdn(phx[v]);
           FT Fx[4];
           if (xdo) {                                     // interior i-face (x|x+1) — Mod_Fluxes.f90:140-185
-            if (face_flux<FT, RIE>(gf, qA, phx, (FT)(0.5 * dlx), qn, phxn, (FT)(0.5 * dlxn), (FT)gcur.fix,
+            if (face_flux<FT, RIE>(gf, qA, phx, (FT)(0.5 * dlx), qn, phxn, (FT)(0.5 * dlxn), bt, (FT)gcur.fix,
                                    (FT)gcur.fiy, (FT)gcur.fil, Fx))
               record_fail(&K, Q2D_E_INTERFACE, fail_key(stage_id, 0, B.blk_in_case, r, x));
           } else {
@@ -1521,6 +1550,7 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
           }
         }
         // ---- roll the j-stencil -------------------------------------------
+        pPrev = qA[3];
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           qA[v] = qB[v]; qB[v] = qC[v];
@@ -2860,6 +2890,7 @@ int rebuild_descriptors(q2d_handle* h) {
     h->max_rk = std::max(h->max_rk, h->dc[c].n_rk);
     if (h->dc[c].model > 0) h->any_visc = true;
     if (h->dc[c].irs_beta > 0.0) h->any_irs = true;
+    if (h->dc[c].space_sd) h->bc_two_phase = true;   // the connection faces' detector reads other records' ghosts
   }
   // device arrays, re-allocated at the current sizes (descriptors are small)
   auto realloc_upload = [&](void** dptr, const void* src, size_t bytes) -> int {
@@ -2919,6 +2950,9 @@ int append_cases(q2d_handle* h, const q2d_problem* prob, int32_t* first_case) {
     if (s.precision != prec) return fail(Q2D_E_ARG, "all cases of a handle must use one precision");
     if (prob->cases[c].res_drop < 0 || prob->cases[c].res_drop >= 1) return fail(Q2D_E_ARG, "res_drop must be in [0, 1)");
     if (!(s.irs_beta >= 0) || !std::isfinite(s.irs_beta)) return fail(Q2D_E_ARG, "irs_beta must be finite and >= 0");
+    if (s.space_sd != 0 && s.space_sd != 1) return fail(Q2D_E_ARG, "space_sd must be 0 or 1");
+    if (s.space_sd && s.precision != Q2D_PREC_FP64)
+      return fail(Q2D_E_UNSUPPORTED, "MUSCL-SD is available in the plain FP64 form only");
     for (int k = 0; k < nblk[c]; ++k) {
       const q2d_block_desc& d = prob->blocks[first_block[c] + k];
       if (s.source == Q2D_SRC_AXI && !d.src_b) return fail(Q2D_E_ARG, "axisymmetric source needs src_b (axis_inv_r)");
@@ -3006,6 +3040,7 @@ int append_cases(q2d_handle* h, const q2d_problem* prob, int32_t* first_case) {
     k.res_threshold = cd.res_threshold; k.time_threshold = cd.time_threshold; k.res_drop = cd.res_drop;
     k.mu_ref = s.mu_ref; k.T_ref = s.T_ref; k.S_suth = s.S_suth; k.Pr = s.Pr;
     k.irs_beta = s.irs_beta;
+    k.space_sd = s.space_sd;
     h->dc.push_back(k);
     h->case_live.push_back(1);
   }

@@ -26,6 +26,7 @@ module q2d_gpu
     real(c_double)     :: gas_R, gas_cp
     real(c_double)     :: mu_ref, T_ref, S_suth, Pr
     real(c_double)     :: irs_beta
+    integer(c_int32_t) :: space_sd, reserved_sd
   end type
 
   type, bind(C), public :: q2d_block_desc

@@ -1,0 +1,70 @@
+"""MUSCL-SD on the device (Tramel's detector in the stage kernel's row march and in the block-connection
+boundary fluxes) against the oracle, through the C-ABI: FP64 bit-identical."""
+import numpy as np
+import pytest
+
+from q2d_amd import GpuSolver, cases
+from q2d_amd.capi import Q2DError
+
+pytestmark = pytest.mark.gpu
+
+
+def run_both(oracle_mod, pb, steps):
+    print(f"sd case: {pb.meta.get('name')} steps={steps}", flush=True)
+    g = GpuSolver(pb)
+    go = g.run(steps)
+    o = oracle_mod.Oracle(pb)
+    rc, oo = o.run(steps, stop_on_todo=False)
+    assert rc == 0
+    for b in range(len(pb.blocks)):
+        Pg, Po = g.state(b)[2:-2, 2:-2], o.state(b)[2:-2, 2:-2]
+        assert np.array_equal(Pg, Po), (b, np.abs(Pg - Po).max())
+    assert go[0]["iter"] == oo[0]["iter"] == steps
+    assert go[0]["dtglobal"] == oo[0]["dtglobal"]
+    return g, o
+
+
+def test_sd_ramp_reference_setting(oracle_mod):
+    """test/2D/euler/ramp-channel/input.ini: MUSCL-SD, minmod, HLLE, SSP-RK3, local dt, CFL 0.9, irs-beta 0.5."""
+    pb = cases.ramp_channel(ni=96, nj=48, wedge_deg=12.0, mach=2.5, cfl=0.9, itermax=80)
+    pb.cases[0].space_sd = True
+    pb.cases[0].irs_beta = 0.5
+    g, o = run_both(oracle_mod, pb, 80)
+    assert np.array_equal(g.dtlocal(0), o.dtlocal(0))
+
+
+@pytest.mark.parametrize("rie,lim", [("HLLC", "vanleer"), ("HLLE", "superbee")])
+def test_sd_multiblock_shock(oracle_mod, rie, lim):
+    """The shock crosses the 101 connections between the 2x2 blocks: connection faces take the boundary
+    cell's detector value."""
+    pb = cases.planar_shock(seed=2, n=64, steps=30, riemann=rie, limiter=lim)
+    pb.cases[0].space_sd = True
+    run_both(oracle_mod, pb, 30)
+
+
+def test_sd_laminar(oracle_mod):
+    pb = cases.laminar_area_channel(seed=5, ni=124, nj=40, area_a=0.6, precision="fp64", steps=60)
+    pb.cases[0].space_sd = True
+    run_both(oracle_mod, pb, 60)
+
+
+def test_sd_batch_with_plain_cases(oracle_mod):
+    pb = cases.planar_shock(seed=4, n=48, steps=20)
+    pb.cases[0].space_sd = True
+    pb.add(cases.riemann_planar(seed=4, ni=150, nj=10, steps=20))
+    g = GpuSolver(pb)
+    g.run(20)
+    for c in range(2):
+        o = oracle_mod.Oracle(pb.case(c))
+        rc, _ = o.run(20, stop_on_todo=False)
+        assert rc == 0
+        blocks = [k for k, b in enumerate(pb.blocks) if b.case_id == c]
+        for kb, b in enumerate(blocks):
+            assert np.array_equal(g.state(b)[2:-2, 2:-2], o.state(kb)[2:-2, 2:-2]), (c, kb)
+
+
+def test_sd_mixed_form_rejected():
+    pb = cases.planar_shock(seed=1, n=32, steps=5, precision="mixed")
+    pb.cases[0].space_sd = True
+    with pytest.raises(Q2DError, match="UNSUPPORTED"):
+        GpuSolver(pb)

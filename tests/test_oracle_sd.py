@@ -1,0 +1,89 @@
+"""MUSCL-SD: Tramel's pressure shock detector (Lib_Shock_Detector.f90:14-38) scaling the MUSCL slopes
+(Lib_Reconstruction.f90:10-29, Lib_Convective.f90:42-46), oracle checks on the CPU.
+
+Pinned by what the formulas fix: beta = 1 on uniform and on linear pressure (a sign slip in the
+second difference or its normalisation fails), beta = 0 past the threshold 1/DELTA, and on smooth
+curvature the closed form 1 - tanh(10 (sw/20)^3) with the library tanh; the reconstruction is linear
+in beta (beta = 0 gives the cell values); a step on a pressure-uniform field equals the MUSCL step bit
+for bit, and on a shock the detector acts at the shock and (almost) nowhere else."""
+import math
+
+import numpy as np
+import pytest
+
+from q2d_amd import cases
+
+
+def textbook_beta(p, pim, pip, pjm, pjp):
+    k1 = abs((pip - 2 * p + pim) / (pip + 2 * p + pim))
+    k2 = abs((pjp - 2 * p + pjm) / (pjp + 2 * p + pjm))
+    sw = max(k1, k2)
+    return 1.0 - math.tanh(10.0 * (sw / 20.0) ** 3) if sw < 0.05 else 0.0
+
+
+def test_detector_closed_forms(oracle_mod):
+    f = oracle_mod.load().q2do_sd_tramel
+    assert f(2.0, 2.0, 2.0, 2.0, 2.0) == 1.0
+    assert f(2.0, 1.5, 2.5, 1.0, 3.0) == 1.0                    # linear in both directions
+    assert f(1.0, 1.0, 10.0, 1.0, 1.0) == 0.0                   # a shock: second difference 9/13 > 1/20
+    assert f(1.0, 1.0, 1.0, 1.0, 10.0) == 0.0
+    rng = np.random.default_rng(1)
+    for _ in range(2000):
+        p = rng.uniform(0.5, 2.0)
+        nb = p * (1.0 + rng.uniform(-0.08, 0.08, 4))
+        got = f(p, *nb)
+        want = textbook_beta(p, *nb)
+        assert abs(got - want) <= 2e-16, (p, nb, got, want)
+    # on smooth data the detector is 1 - O(sw^3): close to 1, not equal
+    assert 1.0 - 1e-6 < f(1.0, 0.99, 1.02, 1.0, 1.0) < 1.0
+
+
+def test_reconstruction_linear_in_beta(oracle_mod):
+    o = oracle_mod.load()
+    out = lambda beta: _recon(o, beta)
+    l1, r1 = out(1.0)
+    l0, r0 = out(0.0)
+    lh, rh = out(0.5)
+    assert (l0, r0) == (0.7, 1.3)                               # first order: the cell values
+    assert abs((lh - 0.7) - 0.5 * (l1 - 0.7)) <= 4e-16 and abs((1.3 - rh) - 0.5 * (1.3 - r1)) <= 4e-16
+
+
+def _recon(o, beta):
+    import ctypes as C
+    pl, pr = C.c_double(), C.c_double()
+    o.q2do_reconstruct(1, 0.5, 0.7, 1.3, 1.6, 0.1, 0.1, 0.1, 0.05, 0.05, beta, C.byref(pl), C.byref(pr))
+    return pl.value, pr.value
+
+
+def _run(oracle_mod, pb, steps):
+    o = oracle_mod.Oracle(pb)
+    rc, _ = o.run(steps, stop_on_todo=False)
+    assert rc == 0
+    st = [o.state(b)[2:-2, 2:-2].copy() for b in range(len(pb.blocks))]
+    o.close()
+    return st
+
+
+def test_uniform_pressure_step_equals_muscl(oracle_mod):
+    def make(sd):
+        pb = cases.riemann_planar(seed=1, ni=64, nj=8, steps=1, limiter="vanleer")
+        for b in pb.blocks:                                     # a contact: density jump, uniform pressure
+            b.P[..., 3] = 0.7
+            b.P[..., 1] = 0.3
+        pb.cases[0].space_sd = sd
+        return pb
+    a = _run(oracle_mod, make(False), 1)
+    b = _run(oracle_mod, make(True), 1)
+    assert np.array_equal(a[0], b[0])
+
+
+def test_detector_acts_at_the_shock(oracle_mod):
+    def make(sd):
+        pb = cases.planar_shock(seed=3, n=32, steps=3)
+        pb.cases[0].space_sd = sd
+        return pb
+    a = _run(oracle_mod, make(False), 3)
+    b = _run(oracle_mod, make(True), 3)
+    d = np.concatenate([np.abs(x - y)[..., 0].ravel() for x, y in zip(a, b)])
+    assert d.max() > 1e-3                                       # first order at the shock
+    assert np.median(d) < 1e-9                                  # 1 - O(sw^3) in smooth flow
