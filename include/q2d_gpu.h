@@ -72,6 +72,7 @@ extern "C" {
 /* Riemann solver ids — Mod_Riemann.f90:25-109 */
 #define Q2D_RIEMANN_HLLC  0
 #define Q2D_RIEMANN_HLLE  1
+#define Q2D_RIEMANN_HLLEM 2   /* Lib_Riemann_HLL.f90:355-432; plain FP64 form only */
 
 /* source ids — Mod_Sources.f90:16-91 ('axisymmetric' | 'area_variation') */
 #define Q2D_SRC_NONE      0
@@ -97,7 +98,7 @@ extern "C" {
 typedef struct q2d_scheme {
   int32_t n_rk;            /* 1 euler, 2 RK2, 3 RK3  (Lib_RK.f90:8-10) */
   int32_t limiter;         /* Q2D_LIM_* ; 'first-order' reconstruction == IORD */
-  int32_t riemann;         /* Q2D_RIEMANN_* */
+  int32_t riemann;         /* Q2D_RIEMANN_* ('HLLC', 'HLLE', 'HLLEM') */
   int32_t time_accurate;   /* 1: global dt (Set_Global_dt), 0: local dt */
   int32_t model;           /* 0 Euler, 1 laminar Navier-Stokes */
   int32_t source;          /* Q2D_SRC_* */

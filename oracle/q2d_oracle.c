@@ -280,6 +280,48 @@ static void riemann_hlle(const gas_t *g, double dl, double ul, double vl, double
           + S1 * S4 * (drtot * (h0r - pr / drtot) - dltot * (h0l - pl / dltot))) * inv_denom;
 }
 
+/* fluxes_helper — Lib_Riemann_HLL.f90:334-350: inviscid flux from primitives (one species, H0 closure) */
+static void fluxes_helper(const gas_t *g, double p, double r, double u, double v, double nx, double ny, double *F) {
+  F[0] = r * (u * nx + v * ny);
+  F[1] = F[0] * u + p * nx;
+  F[2] = F[0] * v + p * ny;
+  F[3] = F[0] * f_H0(g, p, r, sqrt(u * u + v * v));
+}
+
+/* riemann_HLLEM — Lib_Riemann_HLL.f90:355-432 (Einfeldt et al. 1991): the HLLE flux (energy through
+ * e0 = h0 - p/rho) with the contact and the one 2D shear wave restored in the subsonic range; the
+ * detector's beta is not used. */
+static void riemann_hllem(const gas_t *g, double dl, double ul, double vl, double pl, double al, double dltot,
+                          double dr, double ur, double vr, double pr, double ar, double drtot,
+                          double nx, double ny, double *F) {
+  roe_t ra = roe_averages(g, dl, dr, dltot, drtot, ul, vl, ur, vr, pl, pr);
+  double un_roe = ra.u * nx + ra.v * ny;
+  double S1 = min3(0.0, (ul * nx + vl * ny) - al, un_roe - ra.a);
+  double S4 = max3(0.0, (ur * nx + vr * ny) + ar, un_roe + ra.a);
+  double FL[4], FR[4];
+  fluxes_helper(g, pl, dl, ul, vl, nx, ny, FL);
+  fluxes_helper(g, pr, dr, ur, vr, nx, ny, FR);
+  double e0l = ra.h0l - pl / dltot;
+  double e0r = ra.h0r - pr / drtot;
+  double inv_denom = 1.0 / (S4 - S1);
+  F[0] = (S4 * FL[0] - S1 * FR[0] + S1 * S4 * (drtot - dltot)) * inv_denom;
+  F[1] = (S4 * FL[1] - S1 * FR[1] + S1 * S4 * (drtot * ur - dltot * ul)) * inv_denom;
+  F[2] = (S4 * FL[2] - S1 * FR[2] + S1 * S4 * (drtot * vr - dltot * vl)) * inv_denom;
+  F[3] = (S4 * FL[3] - S1 * FR[3] + S1 * S4 * (drtot * e0r - dltot * e0l)) * inv_denom;
+  if (S1 < 0.0 && S4 > 0.0) {
+    double drho = drtot - dltot, dp = pr - pl, du = ur - ul, dv = vr - vl;
+    double tx = -ny, ty = nx;
+    double alpha_c = drho - dp / (ra.a * ra.a);
+    double alpha_s = ra.rho * (du * tx + dv * ty);
+    double delta = ra.a / (ra.a + fabs(0.5 * (S1 + S4)));
+    double C = (S1 * S4) * inv_denom;
+    F[0] = F[0] - C * delta * alpha_c;
+    F[1] = F[1] - C * delta * (alpha_c * ra.u + alpha_s * tx);
+    F[2] = F[2] - C * delta * (alpha_c * ra.v + alpha_s * ty);
+    F[3] = F[3] - C * delta * (alpha_c * 0.5 * (ra.u * ra.u + ra.v * ra.v) + alpha_s * (ra.u * tx + ra.v * ty));
+  }
+}
+
 /* Riemann pointer bound by Assign_Riemann_Solver (Mod_Riemann.f90:25-109).
  * Argument list of riemann_if: (dl,ul,vl,pl,al,dltot, dr,ur,vr,pr,ar,drtot, beta, nx,ny). */
 static void riemann(int id, const gas_t *g, double dl, double ul, double vl, double pl, double al, double dltot,
@@ -287,6 +329,8 @@ static void riemann(int id, const gas_t *g, double dl, double ul, double vl, dou
                     double nx, double ny, double *F) {
   if (id == Q2D_RIEMANN_HLLE)
     riemann_hlle(g, dl, ul, vl, pl, al, dltot, dr, ur, vr, pr, ar, drtot, nx, ny, F);
+  else if (id == Q2D_RIEMANN_HLLEM)
+    riemann_hllem(g, dl, ul, vl, pl, al, dltot, dr, ur, vr, pr, ar, drtot, nx, ny, F);
   else
     riemann_hllc(g, dl, ul, vl, pl, al, dltot, dr, ur, vr, pr, ar, drtot, nx, ny, F);
 }

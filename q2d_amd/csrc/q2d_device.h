@@ -119,7 +119,7 @@ __device__ __forceinline__ void roe_averages(const Gas<T>& g, T dltot, T drtot, 
   h0l_o = h0l; h0r_o = h0r;
 }
 
-// riemann_HLLC :34-123 and riemann_HLLE :126-169.  F = (F_r, F_u, F_v, F_E).
+// riemann_HLLC :34-123, riemann_HLLE :126-169 and riemann_HLLEM :355-432.  F = (F_r, F_u, F_v, F_E).
 template <typename T, int RIE>
 __device__ __forceinline__ void riemann(const Gas<T>& g, T ul, T vl, T pl, T al, T dltot,
                                         T ur, T vr, T pr, T ar, T drtot, T nx, T ny, T F[4]) {
@@ -128,6 +128,42 @@ __device__ __forceinline__ void riemann(const Gas<T>& g, T ul, T vl, T pl, T al,
   T h0l, h0r, u_roe, v_roe, a_roe;
   roe_averages<T>(g, dltot, drtot, ul, vl, ur, vr, pl, pr, h0l, h0r, u_roe, v_roe, a_roe);
   T un_roe = u_roe * nx + v_roe * ny;
+  if (RIE == 2) {                               // HLLEM :355-432 (the detector's beta is not used)
+    T S1 = min3<T>(T(0), unl - al, un_roe - a_roe);
+    T S4 = max3<T>(T(0), unr + ar, un_roe + a_roe);
+    // fluxes_helper :334-350, H0 = cp T + |V|^2 / 2 with |V| = sqrt(u^2 + v^2)
+    auto phys = [&](T p, T r, T u, T v, T Fo[4]) {
+      Fo[0] = r * (u * nx + v * ny);
+      Fo[1] = Fo[0] * u + p * nx;
+      Fo[2] = Fo[0] * v + p * ny;
+      const T vmag = qsqrt<T>(u * u + v * v);
+      Fo[3] = Fo[0] * (g.cp * qdiv<T>(p, r * g.R) + T(0.5) * vmag * vmag);
+    };
+    T FL[4], FR[4];
+    phys(pl, dltot, ul, vl, FL);
+    phys(pr, drtot, ur, vr, FR);
+    T e0l = h0l - qdiv<T>(pl, dltot);
+    T e0r = h0r - qdiv<T>(pr, drtot);
+    T inv_denom = qdiv<T>(T(1), S4 - S1);
+    F[0] = (S4 * FL[0] - S1 * FR[0] + S1 * S4 * (drtot - dltot)) * inv_denom;
+    F[1] = (S4 * FL[1] - S1 * FR[1] + S1 * S4 * (drtot * ur - dltot * ul)) * inv_denom;
+    F[2] = (S4 * FL[2] - S1 * FR[2] + S1 * S4 * (drtot * vr - dltot * vl)) * inv_denom;
+    F[3] = (S4 * FL[3] - S1 * FR[3] + S1 * S4 * (drtot * e0r - dltot * e0l)) * inv_denom;
+    if (S1 < T(0) && S4 > T(0)) {                 // contact and shear restored in the subsonic range
+      const T rho_roe = qsqrt<T>(dltot) * qsqrt<T>(drtot);
+      const T drho = drtot - dltot, dp = pr - pl, du = ur - ul, dv = vr - vl;
+      const T tx = -ny, ty = nx;
+      const T alpha_c = drho - qdiv<T>(dp, a_roe * a_roe);
+      const T alpha_s = rho_roe * (du * tx + dv * ty);
+      const T delta = qdiv<T>(a_roe, a_roe + fabs(T(0.5) * (S1 + S4)));
+      const T C = (S1 * S4) * inv_denom;
+      F[0] = F[0] - C * delta * alpha_c;
+      F[1] = F[1] - C * delta * (alpha_c * u_roe + alpha_s * tx);
+      F[2] = F[2] - C * delta * (alpha_c * v_roe + alpha_s * ty);
+      F[3] = F[3] - C * delta * (alpha_c * T(0.5) * (u_roe * u_roe + v_roe * v_roe) + alpha_s * (u_roe * tx + v_roe * ty));
+    }
+    return;
+  }
   if (RIE == 1) {                               // HLLE
     T S1 = min3<T>(T(0), unl - al, un_roe - a_roe);
     T S4 = max3<T>(T(0), unr + ar, un_roe + a_roe);

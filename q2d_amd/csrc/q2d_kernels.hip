@@ -480,6 +480,7 @@ __device__ __forceinline__ void riemann_rt(int rid, const Gas<FT>& g, const FT L
   const FT aL = sound<FT>(g, L[3], L[0]);
   const FT aR = sound<FT>(g, R[3], R[0]);
   if (rid == Q2D_RIEMANN_HLLE) riemann<FT, 1>(g, L[1], L[2], L[3], aL, L[0], R[1], R[2], R[3], aR, R[0], nx, ny, F);
+  else if (rid == Q2D_RIEMANN_HLLEM) riemann<FT, 2>(g, L[1], L[2], L[3], aL, L[0], R[1], R[2], R[3], aR, R[0], nx, ny, F);
   else riemann<FT, 0>(g, L[1], L[2], L[3], aL, L[0], R[1], R[2], R[3], aR, R[0], nx, ny, F);
 }
 
@@ -1582,6 +1583,7 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
     };
     if constexpr (RIE_T < 0) {
       if (C.riemann == Q2D_RIEMANN_HLLE) item_body(std::integral_constant<int, 1>{});
+      else if (C.riemann == Q2D_RIEMANN_HLLEM) item_body(std::integral_constant<int, 2>{});
       else item_body(std::integral_constant<int, 0>{});
     } else {
       item_body(std::integral_constant<int, RIE_T>{});
@@ -2542,7 +2544,7 @@ int build_items(q2d_handle* h) {
     if (H <= 0) {                                  // balanced for this launch on this device
       std::vector<std::pair<int, int>> strips;
       for (int c = 0; c < h->ncases; ++c) {
-        if (!h->case_live[c] || h->dc[c].riemann != (pass & 1) || (h->dc[c].model > 0) != (pass >= 2)) continue;
+        if (!h->case_live[c] || (h->dc[c].riemann != 0) != ((pass & 1) != 0) || (h->dc[c].model > 0) != (pass >= 2)) continue;
         for (int b = 0; b < h->nblocks; ++b)
           if (h->hb[b].case_id == c) strips.emplace_back((h->hb[b].ni + kOwn - 1) / kOwn, h->hb[b].nj);
       }
@@ -2559,7 +2561,7 @@ int build_items(q2d_handle* h) {
     }
     h->rows_used[pass] = H;
     for (int c = 0; c < h->ncases; ++c) {
-      if (!h->case_live[c] || h->dc[c].riemann != (pass & 1) || (h->dc[c].model > 0) != (pass >= 2)) continue;
+      if (!h->case_live[c] || (h->dc[c].riemann != 0) != ((pass & 1) != 0) || (h->dc[c].model > 0) != (pass >= 2)) continue;
       h->dc[c].item_begin = (int)h->items.size();
       for (int b = 0; b < h->nblocks; ++b) {
         if (h->hb[b].case_id != c) continue;
@@ -2939,7 +2941,9 @@ int append_cases(q2d_handle* h, const q2d_problem* prob, int32_t* first_case) {
     if (nblk[c] > 255) return fail(Q2D_E_ARG, "more than 255 blocks in a case");
     if (s.n_rk < 1 || s.n_rk > 3) return fail(Q2D_E_ARG, "n_rk must be 1, 2 or 3");
     if (s.limiter < 0 || s.limiter > 6) return fail(Q2D_E_ARG, "unknown limiter");
-    if (s.riemann < 0 || s.riemann > 1) return fail(Q2D_E_ARG, "unknown Riemann solver");
+    if (s.riemann < 0 || s.riemann > 2) return fail(Q2D_E_ARG, "unknown Riemann solver");
+    if (s.riemann == Q2D_RIEMANN_HLLEM && s.precision != Q2D_PREC_FP64)
+      return fail(Q2D_E_UNSUPPORTED, "HLLEM is available in the plain FP64 form only");
     if (s.model < 0 || s.model > 1) return fail(Q2D_E_UNSUPPORTED, "only the Euler (0) and laminar (1) models are available");
     if (s.model == 1 && !(s.mu_ref > 0)) return fail(Q2D_E_ARG, "laminar model needs mu_ref > 0");
     if (s.model == 1 && (!(s.T_ref > 0) || !(s.Pr > 0) || !(s.S_suth >= 0)))

@@ -1,5 +1,6 @@
-"""MUSCL-SD on the device (Tramel's detector in the stage kernel's row march and in the block-connection
-boundary fluxes) against the oracle, through the C-ABI: FP64 bit-identical."""
+"""MUSCL-SD (Tramel's detector in the stage kernel's row march and in the block-connection boundary
+fluxes) and the HLLEM Riemann solver on the device against the oracle, through the C-ABI: FP64
+bit-identical."""
 import numpy as np
 import pytest
 
@@ -33,13 +34,22 @@ def test_sd_ramp_reference_setting(oracle_mod):
     assert np.array_equal(g.dtlocal(0), o.dtlocal(0))
 
 
-@pytest.mark.parametrize("rie,lim", [("HLLC", "vanleer"), ("HLLE", "superbee")])
+@pytest.mark.parametrize("rie,lim", [("HLLC", "vanleer"), ("HLLE", "superbee"), ("HLLEM", "minmod")])
 def test_sd_multiblock_shock(oracle_mod, rie, lim):
     """The shock crosses the 101 connections between the 2x2 blocks: connection faces take the boundary
     cell's detector value."""
     pb = cases.planar_shock(seed=2, n=64, steps=30, riemann=rie, limiter=lim)
     pb.cases[0].space_sd = True
     run_both(oracle_mod, pb, 30)
+
+
+@pytest.mark.parametrize("sd", [False, True])
+def test_hllem_ramp(oracle_mod, sd):
+    """HLLEM (Lib_Riemann_HLL.f90:355-432), with and without the detector, local dt."""
+    pb = cases.ramp_channel(ni=96, nj=48, wedge_deg=12.0, mach=2.5, riemann="HLLEM", itermax=60)
+    pb.cases[0].space_sd = sd
+    g, o = run_both(oracle_mod, pb, 60)
+    assert np.array_equal(g.dtlocal(0), o.dtlocal(0))
 
 
 def test_sd_laminar(oracle_mod):
@@ -63,8 +73,9 @@ def test_sd_batch_with_plain_cases(oracle_mod):
             assert np.array_equal(g.state(b)[2:-2, 2:-2], o.state(kb)[2:-2, 2:-2]), (c, kb)
 
 
-def test_sd_mixed_form_rejected():
-    pb = cases.planar_shock(seed=1, n=32, steps=5, precision="mixed")
-    pb.cases[0].space_sd = True
+@pytest.mark.parametrize("what", ["sd", "hllem"])
+def test_mixed_form_rejects(what):
+    pb = cases.planar_shock(seed=1, n=32, steps=5, precision="mixed", riemann="HLLEM" if what == "hllem" else "HLLC")
+    pb.cases[0].space_sd = what == "sd"
     with pytest.raises(Q2DError, match="UNSUPPORTED"):
         GpuSolver(pb)

@@ -87,3 +87,49 @@ def test_detector_acts_at_the_shock(oracle_mod):
     d = np.concatenate([np.abs(x - y)[..., 0].ravel() for x, y in zip(a, b)])
     assert d.max() > 1e-3                                       # first order at the shock
     assert np.median(d) < 1e-9                                  # 1 - O(sw^3) in smooth flow
+
+
+# ---- HLLEM (Lib_Riemann_HLL.f90:355-432) ------------------------------------------------------------
+def _phys(L, nx, ny, R=1.0, cp=3.5):
+    rho, u, v, p = L
+    un = u * nx + v * ny
+    T = p / (rho * R)
+    return np.array([rho * un, rho * un * u + p * nx, rho * un * v + p * ny, rho * un * (cp * T + 0.5 * (u * u + v * v))])
+
+
+def test_hllem_consistent(oracle_mod):
+    o = oracle_mod.Oracle.__new__(oracle_mod.Oracle)
+    o.lib = oracle_mod.load()
+    rng = np.random.default_rng(4)
+    for _ in range(200):
+        L = np.array([rng.uniform(0.2, 3), rng.normal(), rng.normal(), rng.uniform(0.2, 3)])
+        th = rng.uniform(0, 2 * math.pi)
+        F = o.riemann(2, 1.0, 3.5, L, L, math.cos(th), math.sin(th))
+        np.testing.assert_allclose(F, _phys(L, math.cos(th), math.sin(th)), rtol=1e-13, atol=1e-13)
+
+
+def test_hllem_keeps_a_stationary_shear_layer(oracle_mod):
+    """Equal density and pressure, zero normal velocity, different tangential velocity: HLLEM restores the
+    shear wave, so no mass and no tangential momentum cross the face (HLLE diffuses them)."""
+    o = oracle_mod.Oracle.__new__(oracle_mod.Oracle)
+    o.lib = oracle_mod.load()
+    th = 0.37
+    nx, ny = math.cos(th), math.sin(th)
+    tx, ty = -ny, nx
+    L = np.array([1.3, 0.8 * tx, 0.8 * ty, 0.9])
+    R = np.array([1.3, -0.5 * tx, -0.5 * ty, 0.9])
+    F = o.riemann(2, 1.0, 3.5, L, R, nx, ny)
+    assert abs(F[0]) <= 1e-15
+    assert abs(F[1] * tx + F[2] * ty) <= 1e-15                     # tangential momentum
+    assert abs((F[1] * nx + F[2] * ny) - 0.9) <= 1e-15             # normal momentum: the pressure
+    Fe = o.riemann(1, 1.0, 3.5, L, R, nx, ny)
+    assert abs(Fe[1] * tx + Fe[2] * ty) > 1e-2
+
+
+def test_hllem_upwinds_supersonic_flow(oracle_mod):
+    o = oracle_mod.Oracle.__new__(oracle_mod.Oracle)
+    o.lib = oracle_mod.load()
+    L = np.array([1.0, 3.0, 0.2, 1.0])
+    R = np.array([0.8, 2.7, -0.1, 0.9])
+    F = o.riemann(2, 1.0, 3.5, L, R, 1.0, 0.0)
+    np.testing.assert_allclose(F, _phys(L, 1.0, 0.0), rtol=1e-14)
