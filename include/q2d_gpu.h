@@ -73,6 +73,7 @@ extern "C" {
 #define Q2D_RIEMANN_HLLC  0
 #define Q2D_RIEMANN_HLLE  1
 #define Q2D_RIEMANN_HLLEM 2   /* Lib_Riemann_HLL.f90:355-432; plain FP64 form only */
+#define Q2D_RIEMANN_HLLEPP 3  /* 'HLLE++', Lib_Riemann_HLL.f90:435-530 (uses the shock detector); plain FP64 only */
 
 /* source ids — Mod_Sources.f90:16-91 ('axisymmetric' | 'area_variation') */
 #define Q2D_SRC_NONE      0

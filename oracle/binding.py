@@ -49,7 +49,7 @@ def load():
     lib.q2do_limiter.restype = C.c_double
     dp = C.POINTER(C.c_double)
     lib.q2do_reconstruct.argtypes = [C.c_int] + [C.c_double] * 10 + [dp, dp]
-    lib.q2do_riemann.argtypes = [C.c_int, C.c_double, C.c_double, dp, dp, C.c_double, C.c_double, dp]
+    lib.q2do_riemann.argtypes = [C.c_int, C.c_double, C.c_double, dp, dp, C.c_double, C.c_double, C.c_double, dp]
     lib.q2do_roe_averages.argtypes = [C.c_double, C.c_double, dp, dp, dp]
     lib.q2do_convective_flux.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, dp, C.c_double, C.c_double,
                                          C.c_double, dp, dp]
@@ -137,9 +137,9 @@ class Oracle:
     def limiter(self, lid, x, y):
         return self.lib.q2do_limiter(lid, x, y)
 
-    def riemann(self, rid, R, cp, L, Rs, nx, ny):
+    def riemann(self, rid, R, cp, L, Rs, nx, ny, beta=1.0):
         L = np.ascontiguousarray(L, dtype=np.float64)
         Rs = np.ascontiguousarray(Rs, dtype=np.float64)
         F = np.zeros(4)
-        self.lib.q2do_riemann(rid, R, cp, _dp(L), _dp(Rs), nx, ny, _dp(F))
+        self.lib.q2do_riemann(rid, R, cp, _dp(L), _dp(Rs), beta, nx, ny, _dp(F))
         return F

@@ -322,11 +322,66 @@ static void riemann_hllem(const gas_t *g, double dl, double ul, double vl, doubl
   }
 }
 
+/* hartenHyman — Lib_Riemann_HLL.f90:320-331 (entropy fix of the acoustic eigenvalues) */
+static double harten_hyman(double v, double e) {
+  double al = fabs(v);
+  double l = al;
+  if (al <= e) l = (v * v + e * e) / (2.0 * e);
+  return l;
+}
+
+/* riemann_HLLEpp — Lib_Riemann_HLL.f90:435-530 (Tramel et al. 2009): the centred flux minus the Roe
+ * wave decomposition (contact, one 2D shear wave, two acoustic waves), each wave's speed blended by
+ * the detector between the Roe value (beta = 1; Harten-Hyman on the acoustic waves, |un| on the
+ * contact and shear) and the HLLE value (beta = 0). */
+static void riemann_hllepp(const gas_t *g, double dl, double ul, double vl, double pl, double al, double dltot,
+                           double dr, double ur, double vr, double pr, double ar, double drtot, double beta,
+                           double nx, double ny, double *F) {
+  double unl = ul * nx + vl * ny;
+  double unr = ur * nx + vr * ny;
+  roe_t ra = roe_averages(g, dl, dr, dltot, drtot, ul, vl, ur, vr, pl, pr);
+  double un_roe = ra.u * nx + ra.v * ny;
+  double Sm = min3(0.0, unl - al, un_roe - ra.a);
+  double Sp = max3(0.0, unr + ar, un_roe + ra.a);
+  double inv_denom = 1.0 / (Sp - Sm);
+  double l1h = fabs((Sp + Sm) * un_roe * inv_denom - 2.0 * Sp * Sm * inv_denom);
+  double l2h = fabs((Sp + Sm) * (un_roe - ra.a) * inv_denom - 2.0 * Sp * Sm * inv_denom);
+  double l3h = fabs((Sp + Sm) * (un_roe + ra.a) * inv_denom - 2.0 * Sp * Sm * inv_denom);
+  double eps = 0.3 * (fabs(un_roe) + ra.a);
+  double l1r = fabs(un_roe);
+  double l2r = harten_hyman(un_roe - ra.a, eps);
+  double l3r = harten_hyman(un_roe + ra.a, eps);
+  double l1 = beta * l1r + (1.0 - beta) * l1h;
+  double l2 = beta * l2r + (1.0 - beta) * l2h;
+  double l3 = beta * l3r + (1.0 - beta) * l3h;
+  double FL[4], FR[4];
+  fluxes_helper(g, pl, dl, ul, vl, nx, ny, FL);
+  fluxes_helper(g, pr, dr, ur, vr, nx, ny, FR);
+  for (int k = 0; k < 4; k++) F[k] = 0.5 * (FL[k] + FR[k]);
+  double drho = drtot - dltot, dp = pr - pl, du = ur - ul, dv = vr - vl, dun = unr - unl;
+  double tx = -ny, ty = nx;
+  double alpha_m = (dp - ra.rho * ra.a * dun) / (2.0 * ra.a * ra.a);
+  double alpha_c = drho - dp / (ra.a * ra.a);
+  double alpha_s = ra.rho * (du * tx + dv * ty);
+  double alpha_p = (dp + ra.rho * ra.a * dun) / (2.0 * ra.a * ra.a);
+  F[0] = F[0] - 0.5 * (alpha_c * l1 + alpha_m * l2 + alpha_p * l3);
+  F[1] = F[1] - 0.5 * (alpha_c * l1 * ra.u + alpha_s * l1 * tx + alpha_m * l2 * (ra.u - ra.a * nx) +
+                       alpha_p * l3 * (ra.u + ra.a * nx));
+  F[2] = F[2] - 0.5 * (alpha_c * l1 * ra.v + alpha_s * l1 * ty + alpha_m * l2 * (ra.v - ra.a * ny) +
+                       alpha_p * l3 * (ra.v + ra.a * ny));
+  F[3] = F[3] - 0.5 * (alpha_c * l1 * 0.5 * (ra.u * ra.u + ra.v * ra.v) + alpha_s * l1 * (ra.u * tx + ra.v * ty) +
+                       alpha_m * l2 * (ra.h0 - ra.a * un_roe) + alpha_p * l3 * (ra.h0 + ra.a * un_roe));
+}
+
 /* Riemann pointer bound by Assign_Riemann_Solver (Mod_Riemann.f90:25-109).
  * Argument list of riemann_if: (dl,ul,vl,pl,al,dltot, dr,ur,vr,pr,ar,drtot, beta, nx,ny). */
 static void riemann(int id, const gas_t *g, double dl, double ul, double vl, double pl, double al, double dltot,
-                    double dr, double ur, double vr, double pr, double ar, double drtot,
+                    double dr, double ur, double vr, double pr, double ar, double drtot, double beta,
                     double nx, double ny, double *F) {
+  if (id == Q2D_RIEMANN_HLLEPP) {
+    riemann_hllepp(g, dl, ul, vl, pl, al, dltot, dr, ur, vr, pr, ar, drtot, beta, nx, ny, F);
+    return;
+  }
   if (id == Q2D_RIEMANN_HLLE)
     riemann_hlle(g, dl, ul, vl, pl, al, dltot, dr, ur, vr, pr, ar, drtot, nx, ny, F);
   else if (id == Q2D_RIEMANN_HLLEM)
@@ -610,8 +665,10 @@ static double sd_tramel(double p, double p1, double p2, double p3, double p4) {
 /* Convective_Flux — Lib_Convective.f90:15-82                                */
 /* ------------------------------------------------------------------------ */
 /* dl[-1..2] -> dl[0..3], prim[c][v] for cells -1..2 -> c = 0..3. Returns 0 or Q2D_E_INTERFACE. */
+/* beta_: the reconstruction's (the detector with SD_limiter, else 1); beta_rie: the Riemann solver's (the
+ * detector whenever it is computed, else 1) — Lib_Convective.f90:42-46, :70-72 */
 static int convective_flux(int lim, int rid, const gas_t *g, const double *dl, double nx, double ny,
-                           double int_l, const double (*prim)[NP], double beta_, double *Flux) {
+                           double int_l, const double (*prim)[NP], double beta_, double beta_rie, double *Flux) {
   double l0 = 0.5 * (dl[0] + dl[1]);
   double l1 = 0.5 * (dl[1] + dl[2]);
   double l2 = 0.5 * (dl[2] + dl[3]);
@@ -625,7 +682,7 @@ static int convective_flux(int lim, int rid, const gas_t *g, const double *dl, d
   double a1 = f_ss(g, P1[3], rotot1);
   double a4 = f_ss(g, P4[3], rotot4);
   double F[4];
-  riemann(rid, g, P1[0], P1[1], P1[2], P1[3], a1, rotot1, P4[0], P4[1], P4[2], P4[3], a4, rotot4, nx, ny, F);
+  riemann(rid, g, P1[0], P1[1], P1[2], P1[3], a1, rotot1, P4[0], P4[1], P4[2], P4[3], a4, rotot4, beta_rie, nx, ny, F);
   double su = copysign(0.5, F[0]);
   double sp = 0.5 + su;
   double sm = su - 0.5;
@@ -746,7 +803,8 @@ static int fluxes_block(oblk *b, const ocase *cs) {
   /* shock detector on Prim(np, i-1:i+1, j-1:j+1) (Mod_Fluxes.f90:129-138); the faces take the value of
    * their cell i (i-faces, :144-150) or j (j-faces, :191-197), with SD_limiter (Lib_Convective.f90:42-46) */
   const int sd = s->space_sd != 0;
-  if (sd)
+  const int det = sd || s->riemann == Q2D_RIEMANN_HLLEPP;   /* obj_shock_detector%SD, Assign_Setup.f90:49-53 */
+  if (det)
     for (int j = 1; j <= b->nj; j++)
       for (int i = 1; i <= b->ni; i++)
         b->beta[IN(b, i, j)] = sd_tramel(PV(b, 3, i, j), PV(b, 3, i - 1, j), PV(b, 3, i + 1, j), PV(b, 3, i, j - 1),
@@ -763,7 +821,8 @@ static int fluxes_block(oblk *b, const ocase *cs) {
         for (int v = 0; v < NP; v++) prim[c][v] = PV(b, v, i - 1 + c, j);
       }
       err |= -convective_flux(s->limiter, s->riemann, &cs->gas, dl, FI(b, 0, i, j), FI(b, 1, i, j),
-                              FI(b, 2, i, j), (const double(*)[NP])prim, sd ? b->beta[IN(b, i, j)] : 1.0, F);
+                              FI(b, 2, i, j), (const double(*)[NP])prim, sd ? b->beta[IN(b, i, j)] : 1.0,
+                              det ? b->beta[IN(b, i, j)] : 1.0, F);
       for (int v = 0; v < NP; v++) { RV(b, v, i, j) = RV(b, v, i, j) + F[v]; RV(b, v, i + 1, j) = RV(b, v, i + 1, j) - F[v]; }
       if (s->model > 0) {
         double D[NP];
@@ -789,7 +848,8 @@ static int fluxes_block(oblk *b, const ocase *cs) {
         for (int v = 0; v < NP; v++) prim[c][v] = PV(b, v, i, j - 1 + c);
       }
       err |= -convective_flux(s->limiter, s->riemann, &cs->gas, dl, FJ(b, 0, i, j), FJ(b, 1, i, j),
-                              FJ(b, 2, i, j), (const double(*)[NP])prim, sd ? b->beta[IN(b, i, j)] : 1.0, F);
+                              FJ(b, 2, i, j), (const double(*)[NP])prim, sd ? b->beta[IN(b, i, j)] : 1.0,
+                              det ? b->beta[IN(b, i, j)] : 1.0, F);
       for (int v = 0; v < NP; v++) { RV(b, v, i, j) = RV(b, v, i, j) + F[v]; RV(b, v, i, j + 1) = RV(b, v, i, j + 1) - F[v]; }
       if (s->model > 0) {
         double D[NP];
@@ -853,7 +913,8 @@ static int bc_connection_eul(oblk *b, const ocase *cs, int Im, int Jm, int Fm) {
   double a_L = f_ss(&cs->gas, PL[3], rho_L);
   double a_R = f_ss(&cs->gas, PR[3], rho_R);
   double F[4];
-  riemann(s->riemann, &cs->gas, PL[0], PL[1], PL[2], PL[3], a_L, rho_L, PR[0], PR[1], PR[2], PR[3], a_R, rho_R, n1, n2, F);
+  double beta_rie = s->riemann == Q2D_RIEMANN_HLLEPP ? b->beta[IN(b, Im, Jm)] : 1.0;   /* SD_riemann :103-108 */
+  riemann(s->riemann, &cs->gas, PL[0], PL[1], PL[2], PL[3], a_L, rho_L, PR[0], PR[1], PR[2], PR[3], a_R, rho_R, beta_rie, n1, n2, F);
   double su = copysign(0.5, F[0]);
   double sel_L = 0.5 + su, sel_R = su - 0.5;
   double Fmass = F[0] * int_l;
@@ -901,7 +962,7 @@ static void bc_symmetry_eul(oblk *b, const ocase *cs, int Im, int Jm, int Fm) {
   double U1x = Un1 * n1 + vt1, U1y = Un1 * n2 + vt2;
   double U4x = -Un1 * n1 + vt1, U4y = -Un1 * n2 + vt2;
   double F[4];
-  riemann(s->riemann, &cs->gas, Fp[0], U1x, U1y, Fp[3], sound, rotot, Fp[0], U4x, U4y, Fp[3], sound, rotot, n1, n2, F);
+  riemann(s->riemann, &cs->gas, Fp[0], U1x, U1y, Fp[3], sound, rotot, Fp[0], U4x, U4y, Fp[3], sound, rotot, 1.0, n1, n2, F);
   double fu = F[1] * int_l, fv = F[2] * int_l;
   RV(b, 1, Im, Jm) = RV(b, 1, Im, Jm) + dm2 * fu;
   RV(b, 2, Im, Jm) = RV(b, 2, Im, Jm) + dm2 * fv;
@@ -965,7 +1026,7 @@ static void bc_supersonic_inflow(oblk *b, const ocase *cs, const obc *rec, int I
   double rho_L = PL[0], rho_R = PR[0];
   double a_L = f_ss(g, PL[3], rho_L), a_R = f_ss(g, PR[3], rho_R);
   double F[4];
-  riemann(s->riemann, g, PL[0], PL[1], PL[2], PL[3], a_L, rho_L, PR[0], PR[1], PR[2], PR[3], a_R, rho_R, n1, n2, F);
+  riemann(s->riemann, g, PL[0], PL[1], PL[2], PL[3], a_L, rho_L, PR[0], PR[1], PR[2], PR[3], a_R, rho_R, 1.0, n1, n2, F);
   double su = copysign(1.0, F[0]);
   double Sel_L = (1.0 + su) / 2.0, Sel_R = (su - 1.0) / 2.0;
   double Fmass = F[0] * int_l;
@@ -1606,10 +1667,10 @@ void q2do_reconstruct(int lim, double prev, double local, double next, double ne
 }
 
 void q2do_riemann(int id, double gas_R, double gas_cp, const double *L, const double *Rr,
-                  double nx, double ny, double *F) {
+                  double beta, double nx, double ny, double *F) {
   gas_t g = mkgas(gas_R, gas_cp);
   double aL = f_ss(&g, L[3], L[0]), aR = f_ss(&g, Rr[3], Rr[0]);
-  riemann(id, &g, L[0], L[1], L[2], L[3], aL, L[0], Rr[0], Rr[1], Rr[2], Rr[3], aR, Rr[0], nx, ny, F);
+  riemann(id, &g, L[0], L[1], L[2], L[3], aL, L[0], Rr[0], Rr[1], Rr[2], Rr[3], aR, Rr[0], beta, nx, ny, F);
 }
 
 void q2do_roe_averages(double gas_R, double gas_cp, const double *L, const double *Rr, double *out) {
@@ -1621,7 +1682,7 @@ void q2do_roe_averages(double gas_R, double gas_cp, const double *L, const doubl
 int q2do_convective_flux(int lim, int rid, double gas_R, double gas_cp, const double *dl,
                          double nx, double ny, double int_l, const double *prim, double *flux) {
   gas_t g = mkgas(gas_R, gas_cp);
-  return convective_flux(lim, rid, &g, dl, nx, ny, int_l, (const double(*)[NP])prim, 1.0, flux);
+  return convective_flux(lim, rid, &g, dl, nx, ny, int_l, (const double(*)[NP])prim, 1.0, 1.0, flux);
 }
 
 int q2do_get_residual(q2do *o, int block, double *R) {

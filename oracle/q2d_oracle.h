@@ -46,7 +46,7 @@ void q2do_reconstruct(int limiter, double prev, double local, double next, doubl
                       double *priml, double *primr);
 /* Riemann flux (Lib_Riemann_HLL.f90) for primitive L/R states; F[4] = F_r,F_u,F_v,F_E. */
 void q2do_riemann(int id, double gas_R, double gas_cp,
-                  const double *primL, const double *primR, double nx, double ny, double *F);
+                  const double *primL, const double *primR, double beta, double nx, double ny, double *F);
 /* Roe averages (Lib_Riemann_HLL.f90:229-317): out = h0l,h0r,rho,u,v,a,h0. */
 void q2do_roe_averages(double gas_R, double gas_cp, const double *primL, const double *primR, double *out);
 /* Convective_Flux (Lib_Convective.f90:15-82) for one face: stencil prim[4][4]

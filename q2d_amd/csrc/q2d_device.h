@@ -122,12 +122,58 @@ __device__ __forceinline__ void roe_averages(const Gas<T>& g, T dltot, T drtot, 
 // riemann_HLLC :34-123, riemann_HLLE :126-169 and riemann_HLLEM :355-432.  F = (F_r, F_u, F_v, F_E).
 template <typename T, int RIE>
 __device__ __forceinline__ void riemann(const Gas<T>& g, T ul, T vl, T pl, T al, T dltot,
-                                        T ur, T vr, T pr, T ar, T drtot, T nx, T ny, T F[4]) {
+                                        T ur, T vr, T pr, T ar, T drtot, T nx, T ny, T F[4], T beta = T(1)) {
   T unl = ul * nx + vl * ny;
   T unr = ur * nx + vr * ny;
   T h0l, h0r, u_roe, v_roe, a_roe;
   roe_averages<T>(g, dltot, drtot, ul, vl, ur, vr, pl, pr, h0l, h0r, u_roe, v_roe, a_roe);
   T un_roe = u_roe * nx + v_roe * ny;
+  if (RIE == 3) {                               // HLLE++ :435-530: Roe waves, speeds blended HLLE <-> Roe by beta
+    const T srL = qsqrt<T>(dltot), srR = qsqrt<T>(drtot);
+    const T rho_roe = srL * srR;
+    const T h0_roe = (srR * h0r + srL * h0l) * qdiv<T>(T(1), srL + srR);
+    const T Sm = min3<T>(T(0), unl - al, un_roe - a_roe);
+    const T Sp = max3<T>(T(0), unr + ar, un_roe + a_roe);
+    const T inv_denom = qdiv<T>(T(1), Sp - Sm);
+    const T l1h = fabs((Sp + Sm) * un_roe * inv_denom - T(2) * Sp * Sm * inv_denom);
+    const T l2h = fabs((Sp + Sm) * (un_roe - a_roe) * inv_denom - T(2) * Sp * Sm * inv_denom);
+    const T l3h = fabs((Sp + Sm) * (un_roe + a_roe) * inv_denom - T(2) * Sp * Sm * inv_denom);
+    const T eps = T(0.3) * (fabs(un_roe) + a_roe);
+    auto hh = [&](T v) {                        // hartenHyman :320-331
+      const T av = fabs(v);
+      return av <= eps ? qdiv<T>(v * v + eps * eps, T(2) * eps) : av;
+    };
+    const T l1r = fabs(un_roe), l2r = hh(un_roe - a_roe), l3r = hh(un_roe + a_roe);
+    const T l1 = beta * l1r + (T(1) - beta) * l1h;
+    const T l2 = beta * l2r + (T(1) - beta) * l2h;
+    const T l3 = beta * l3r + (T(1) - beta) * l3h;
+    auto phys = [&](T p, T r, T u, T v, T Fo[4]) {   // fluxes_helper :334-350
+      Fo[0] = r * (u * nx + v * ny);
+      Fo[1] = Fo[0] * u + p * nx;
+      Fo[2] = Fo[0] * v + p * ny;
+      const T vmag = qsqrt<T>(u * u + v * v);
+      Fo[3] = Fo[0] * (g.cp * qdiv<T>(p, r * g.R) + T(0.5) * vmag * vmag);
+    };
+    T FL[4], FR[4];
+    phys(pl, dltot, ul, vl, FL);
+    phys(pr, drtot, ur, vr, FR);
+    const T drho = drtot - dltot, dp = pr - pl, du = ur - ul, dv = vr - vl, dun = unr - unl;
+    const T tx = -ny, ty = nx;
+    const T alpha_m = qdiv<T>(dp - rho_roe * a_roe * dun, T(2) * a_roe * a_roe);
+    const T alpha_c = drho - qdiv<T>(dp, a_roe * a_roe);
+    const T alpha_s = rho_roe * (du * tx + dv * ty);
+    const T alpha_p = qdiv<T>(dp + rho_roe * a_roe * dun, T(2) * a_roe * a_roe);
+    F[0] = T(0.5) * (FL[0] + FR[0]) - T(0.5) * (alpha_c * l1 + alpha_m * l2 + alpha_p * l3);
+    F[1] = T(0.5) * (FL[1] + FR[1]) - T(0.5) * (alpha_c * l1 * u_roe + alpha_s * l1 * tx + alpha_m * l2 * (u_roe - a_roe * nx) +
+                                                alpha_p * l3 * (u_roe + a_roe * nx));
+    F[2] = T(0.5) * (FL[2] + FR[2]) - T(0.5) * (alpha_c * l1 * v_roe + alpha_s * l1 * ty + alpha_m * l2 * (v_roe - a_roe * ny) +
+                                                alpha_p * l3 * (v_roe + a_roe * ny));
+    F[3] = T(0.5) * (FL[3] + FR[3]) - T(0.5) * (alpha_c * l1 * T(0.5) * (u_roe * u_roe + v_roe * v_roe) +
+                                                alpha_s * l1 * (u_roe * tx + v_roe * ty) +
+                                                alpha_m * l2 * (h0_roe - a_roe * un_roe) +
+                                                alpha_p * l3 * (h0_roe + a_roe * un_roe));
+    return;
+  }
   if (RIE == 2) {                               // HLLEM :355-432 (the detector's beta is not used)
     T S1 = min3<T>(T(0), unl - al, un_roe - a_roe);
     T S4 = max3<T>(T(0), unr + ar, un_roe + a_roe);

@@ -476,11 +476,14 @@ __device__ __forceinline__ void emit_bc(BCFlux* __restrict__ out, float4* __rest
 }
 
 template <typename FT>
-__device__ __forceinline__ void riemann_rt(int rid, const Gas<FT>& g, const FT L[4], const FT R[4], FT nx, FT ny, FT F[4]) {
+__device__ __forceinline__ void riemann_rt(int rid, const Gas<FT>& g, const FT L[4], const FT R[4], FT nx, FT ny, FT F[4],
+                                           FT beta = FT(1)) {
   const FT aL = sound<FT>(g, L[3], L[0]);
   const FT aR = sound<FT>(g, R[3], R[0]);
   if (rid == Q2D_RIEMANN_HLLE) riemann<FT, 1>(g, L[1], L[2], L[3], aL, L[0], R[1], R[2], R[3], aR, R[0], nx, ny, F);
   else if (rid == Q2D_RIEMANN_HLLEM) riemann<FT, 2>(g, L[1], L[2], L[3], aL, L[0], R[1], R[2], R[3], aR, R[0], nx, ny, F);
+  else if (rid == Q2D_RIEMANN_HLLEPP)
+    riemann<FT, 3>(g, L[1], L[2], L[3], aL, L[0], R[1], R[2], R[3], aR, R[0], nx, ny, F, beta);
   else riemann<FT, 0>(g, L[1], L[2], L[3], aL, L[0], R[1], R[2], R[3], aR, R[0], nx, ny, F);
 }
 
@@ -576,10 +579,13 @@ __global__ __launch_bounds__(256) void k_bcflux(const DevBlock* __restrict__ blo
     const FT Dist23 = (FT)(Dl2 + Dl3), Dist12 = (FT)(Dl1 + Dl2), Dist34 = (FT)(Dl3 + Dl4);
     // MUSCL-SD: the boundary cell's detector value (Lib_BC_Fluxes_Connection.f90:64-69), from the state
     // after every record's ghost fill (such handles run the two-phase launch)
-    FT bt = FT(1);
-    if (C.space_sd)
-      bt = (FT)sd_tramel(Bp[3], P[3 * pl + off(Im - 1, Jm)], P[3 * pl + off(Im + 1, Jm)], P[3 * pl + off(Im, Jm - 1)],
-                         P[3 * pl + off(Im, Jm + 1)]);
+    FT bt = FT(1), brie = FT(1);                       // SD_limiter :64-69, SD_riemann :103-108
+    if (C.space_sd || rid == Q2D_RIEMANN_HLLEPP) {
+      const FT d = (FT)sd_tramel(Bp[3], P[3 * pl + off(Im - 1, Jm)], P[3 * pl + off(Im + 1, Jm)],
+                                 P[3 * pl + off(Im, Jm - 1)], P[3 * pl + off(Im, Jm + 1)]);
+      bt = C.space_sd ? d : FT(1);
+      brie = rid == Q2D_RIEMANN_HLLEPP ? d : FT(1);
+    }
     FT L[4], R[4];
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
@@ -598,7 +604,7 @@ __global__ __launch_bounds__(256) void k_bcflux(const DevBlock* __restrict__ blo
     const bool bad = !(L[0] > FT(0) && L[3] > FT(0) && R[0] > FT(0) && R[3] > FT(0));
     if (bad) record_fail(&ctrl[E.cid], Q2D_E_INTERFACE, fail_key(K.stage_count + irk, 0, B.blk_in_case, Jm, Im));
     FT F[4];
-    riemann_rt<FT>(rid, gf, L, R, (FT)n1, (FT)n2, F);
+    riemann_rt<FT>(rid, gf, L, R, (FT)n1, (FT)n2, F, brie);
     // one species: Sel_L*rho_L/rho_L - Sel_R*rho_R/rho_R == 1, so Flux(1) = F_r*int_l
 #pragma unroll
     for (int v = 0; v < 4; ++v) res.F[v] = ((double)F[v] * il) * sgn;
@@ -1145,7 +1151,7 @@ __device__ __forceinline__ void limiter4(int lim, const FT x[4], const FT y[4], 
 // interface state (check_gas_state).
 template <typename FT, int RIE>
 __device__ __forceinline__ bool face_flux(const Gas<FT>& g, const double qa[4], const FT pa[4], FT lm, const double qb[4],
-                                          const FT pb[4], FT lp, FT bt, FT nx, FT ny, FT il, FT out[4]) {
+                                          const FT pb[4], FT lp, FT bt, FT brie, FT nx, FT ny, FT il, FT out[4]) {
   FT qL[4], qR[4];
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
@@ -1156,7 +1162,7 @@ __device__ __forceinline__ bool face_flux(const Gas<FT>& g, const double qa[4], 
   const FT aL = sound<FT>(g, qL[3], qL[0]);
   const FT aR = sound<FT>(g, qR[3], qR[0]);
   FT F[4];
-  riemann<FT, RIE>(g, qL[1], qL[2], qL[3], aL, qL[0], qR[1], qR[2], qR[3], aR, qR[0], nx, ny, F);
+  riemann<FT, RIE>(g, qL[1], qL[2], qL[3], aL, qL[0], qR[1], qR[2], qR[3], aR, qR[0], nx, ny, F, brie);
   // one species: sp*(rho1/rotot1) - sm*(rho4/rotot4) == 1 exactly, so Flux(1) = F_r*int_l
 #pragma unroll
   for (int v = 0; v < 4; ++v) out[v] = F[v] * il;
@@ -1198,12 +1204,15 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
     DevCtrl& K = ctrl[B.case_id];
     const DevCase& C = cases[B.case_id];
     if (!K.active || irk > C.n_rk) continue;
-    // RIE_T < 0: HLLC and HLLE cases share the launch; the item's case picks the solver (uniform switch)
+    // RIE_T < 0: the cases' solvers share the launch; the item's case picks one (uniform switch).  HLLE++ is
+    // instantiated only with RIE_T == -2, launched for handles that hold an HLLE++ case, so that the other
+    // launches keep their register allocation (2 waves per SIMD)
     auto item_body = [&](auto riec) {
       constexpr int RIE = decltype(riec)::value;
       const bool last = (irk == C.n_rk);
       const bool irs = C.irs_beta > 0.0;                 // residual to B.irs[0]; k_irs smooths and updates
       const bool sd = C.space_sd != 0;                   // MUSCL-SD: faces take their cell's detector value
+      const bool det = sd || C.riemann == Q2D_RIEMANN_HLLEPP;   // the detector is computed (Assign_Setup.f90:49-53)
       const int ni = B.ni, nj = B.nj, pitch = B.pitch;
       const long long pl = B.plane;
       // the case's control words, read once per item (wave-uniform; a read inside the row loop would be a
@@ -1342,10 +1351,11 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
           limiter4<FT>(lim, sBC, sAB, phiB);            // phi_{r+1} = rlimiter(s(r+1|r+2), s(r|r+1))
         }
         // Tramel's detector of cell (x, r) (Mod_Fluxes.f90:129-138): its i-face x|x+1 and j-face r|r+1
-        FT bt = FT(1);
-        if (sd) {
+        FT bt = FT(1), brie = FT(1);                     // the reconstruction's and the Riemann solver's
+        if (det) {
           const double pc = qA[3], pim = shup(qA[3]), pip = shdn(qA[3]);
-          bt = (FT)sd_tramel(pc, pim, pip, pPrev, qB[3]);
+          brie = (FT)sd_tramel(pc, pim, pip, pPrev, qB[3]);
+          bt = sd ? brie : FT(1);
         }
         double Ecur[3] = {0.0, 0.0, 0.0};
         if (VISC && own && r >= 1 && r <= nj - 1) {
@@ -1355,7 +1365,7 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
         }
         // interior j-face (r | r+1) — Mod_Fluxes.f90:187-232
         if (own && r >= 1 && r <= nj - 1) {
-          if (face_flux<FT, RIE>(gf, qA, phiA, (FT)(0.5 * dlyA), qB, phiB, (FT)(0.5 * dlyB), bt, (FT)gcur.fjx,
+          if (face_flux<FT, RIE>(gf, qA, phiA, (FT)(0.5 * dlyA), qB, phiB, (FT)(0.5 * dlyB), bt, brie, (FT)gcur.fjx,
                                  (FT)gcur.fjy, (FT)gcur.fjl, Gcur))
             record_fail(&K, Q2D_E_INTERFACE, fail_key(stage_id, 0, B.blk_in_case, r, x));
         } else {
@@ -1383,7 +1393,7 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
           for (int v = 0; v < 4; ++v) phxn[v] = shdn(phx[v]);
           FT Fx[4];
           if (xdo) {                                     // interior i-face (x|x+1) — Mod_Fluxes.f90:140-185
-            if (face_flux<FT, RIE>(gf, qA, phx, (FT)(0.5 * dlx), qn, phxn, (FT)(0.5 * dlxn), bt, (FT)gcur.fix,
+            if (face_flux<FT, RIE>(gf, qA, phx, (FT)(0.5 * dlx), qn, phxn, (FT)(0.5 * dlxn), bt, brie, (FT)gcur.fix,
                                    (FT)gcur.fiy, (FT)gcur.fil, Fx))
               record_fail(&K, Q2D_E_INTERFACE, fail_key(stage_id, 0, B.blk_in_case, r, x));
           } else {
@@ -1584,6 +1594,7 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
     if constexpr (RIE_T < 0) {
       if (C.riemann == Q2D_RIEMANN_HLLE) item_body(std::integral_constant<int, 1>{});
       else if (C.riemann == Q2D_RIEMANN_HLLEM) item_body(std::integral_constant<int, 2>{});
+      else if (RIE_T == -2 && C.riemann == Q2D_RIEMANN_HLLEPP) item_body(std::integral_constant<int, 3>{});
       else item_body(std::integral_constant<int, 0>{});
     } else {
       item_body(std::integral_constant<int, RIE_T>{});
@@ -2432,6 +2443,7 @@ struct q2d_handle {
   int dt_total = 0;
   bool any_visc = false;
   bool any_irs = false;          // a live case smooths its residual (k_irs after each stage)
+  bool any_hllepp = false;       // a live case uses HLLE++: the stage kernel's RIE_T = -2 instantiation
   bool bc_two_phase = false;     // a live block is one cell wide: ghost fill and BC fluxes in two launches
   std::vector<int> vrec;         // boundary records with a viscous flux (BC 300/302 of model-1 cases)
   int* d_vrec = nullptr;
@@ -2667,6 +2679,9 @@ int launch_stage(q2d_handle* h, int irk) {
       if (pass == 0 && h->any_irs) Q2D_STAGE_MX_LAUNCH(-1, true);
       else if (pass == 0) Q2D_STAGE_MX_LAUNCH(-1, false);
       else Q2D_STAGE_LAUNCH(-1, true);
+    } else if (h->any_hllepp) {                        // HLLE++ (plain FP64 form only)
+      if (pass == 0) Q2D_STAGE_LAUNCH(-2, false);
+      else Q2D_STAGE_LAUNCH(-2, true);
     } else {
       if (pass == 0) Q2D_STAGE_LAUNCH(-1, false);
       else Q2D_STAGE_LAUNCH(-1, true);
@@ -2884,6 +2899,7 @@ int rebuild_descriptors(q2d_handle* h) {
   h->max_rk = 1;
   h->any_visc = false;
   h->any_irs = false;
+  h->any_hllepp = false;
   h->bc_two_phase = false;
   for (int b = 0; b < h->nblocks; ++b)
     if (h->hb[b].arena && (h->hb[b].ni < 2 || h->hb[b].nj < 2)) h->bc_two_phase = true;
@@ -2892,7 +2908,9 @@ int rebuild_descriptors(q2d_handle* h) {
     h->max_rk = std::max(h->max_rk, h->dc[c].n_rk);
     if (h->dc[c].model > 0) h->any_visc = true;
     if (h->dc[c].irs_beta > 0.0) h->any_irs = true;
-    if (h->dc[c].space_sd) h->bc_two_phase = true;   // the connection faces' detector reads other records' ghosts
+    if (h->dc[c].riemann == Q2D_RIEMANN_HLLEPP) h->any_hllepp = true;
+    if (h->dc[c].space_sd || h->dc[c].riemann == Q2D_RIEMANN_HLLEPP)
+      h->bc_two_phase = true;                          // the connection faces' detector reads other records' ghosts
   }
   // device arrays, re-allocated at the current sizes (descriptors are small)
   auto realloc_upload = [&](void** dptr, const void* src, size_t bytes) -> int {
@@ -2941,9 +2959,9 @@ int append_cases(q2d_handle* h, const q2d_problem* prob, int32_t* first_case) {
     if (nblk[c] > 255) return fail(Q2D_E_ARG, "more than 255 blocks in a case");
     if (s.n_rk < 1 || s.n_rk > 3) return fail(Q2D_E_ARG, "n_rk must be 1, 2 or 3");
     if (s.limiter < 0 || s.limiter > 6) return fail(Q2D_E_ARG, "unknown limiter");
-    if (s.riemann < 0 || s.riemann > 2) return fail(Q2D_E_ARG, "unknown Riemann solver");
-    if (s.riemann == Q2D_RIEMANN_HLLEM && s.precision != Q2D_PREC_FP64)
-      return fail(Q2D_E_UNSUPPORTED, "HLLEM is available in the plain FP64 form only");
+    if (s.riemann < 0 || s.riemann > 3) return fail(Q2D_E_ARG, "unknown Riemann solver");
+    if (s.riemann >= Q2D_RIEMANN_HLLEM && s.precision != Q2D_PREC_FP64)
+      return fail(Q2D_E_UNSUPPORTED, "HLLEM and HLLE++ are available in the plain FP64 form only");
     if (s.model < 0 || s.model > 1) return fail(Q2D_E_UNSUPPORTED, "only the Euler (0) and laminar (1) models are available");
     if (s.model == 1 && !(s.mu_ref > 0)) return fail(Q2D_E_ARG, "laminar model needs mu_ref > 0");
     if (s.model == 1 && (!(s.T_ref > 0) || !(s.Pr > 0) || !(s.S_suth >= 0)))

@@ -34,7 +34,8 @@ def test_sd_ramp_reference_setting(oracle_mod):
     assert np.array_equal(g.dtlocal(0), o.dtlocal(0))
 
 
-@pytest.mark.parametrize("rie,lim", [("HLLC", "vanleer"), ("HLLE", "superbee"), ("HLLEM", "minmod")])
+@pytest.mark.parametrize("rie,lim", [("HLLC", "vanleer"), ("HLLE", "superbee"), ("HLLEM", "minmod"),
+                                     ("HLLE++", "vanleer")])
 def test_sd_multiblock_shock(oracle_mod, rie, lim):
     """The shock crosses the 101 connections between the 2x2 blocks: connection faces take the boundary
     cell's detector value."""
@@ -43,17 +44,18 @@ def test_sd_multiblock_shock(oracle_mod, rie, lim):
     run_both(oracle_mod, pb, 30)
 
 
-@pytest.mark.parametrize("sd", [False, True])
-def test_hllem_ramp(oracle_mod, sd):
-    """HLLEM (Lib_Riemann_HLL.f90:355-432), with and without the detector, local dt."""
-    pb = cases.ramp_channel(ni=96, nj=48, wedge_deg=12.0, mach=2.5, riemann="HLLEM", itermax=60)
+@pytest.mark.parametrize("rie,sd", [("HLLEM", False), ("HLLEM", True), ("HLLE++", False), ("HLLE++", True)])
+def test_hllem_ramp(oracle_mod, rie, sd):
+    """HLLEM (Lib_Riemann_HLL.f90:355-432) and HLLE++ (:435-530), with and without MUSCL-SD, local dt."""
+    pb = cases.ramp_channel(ni=96, nj=48, wedge_deg=12.0, mach=2.5, riemann=rie, itermax=60)
     pb.cases[0].space_sd = sd
     g, o = run_both(oracle_mod, pb, 60)
     assert np.array_equal(g.dtlocal(0), o.dtlocal(0))
 
 
-def test_sd_laminar(oracle_mod):
-    pb = cases.laminar_area_channel(seed=5, ni=124, nj=40, area_a=0.6, precision="fp64", steps=60)
+@pytest.mark.parametrize("rie", ["HLLC", "HLLE++"])
+def test_sd_laminar(oracle_mod, rie):
+    pb = cases.laminar_area_channel(seed=5, ni=124, nj=40, area_a=0.6, precision="fp64", steps=60, riemann=rie)
     pb.cases[0].space_sd = True
     run_both(oracle_mod, pb, 60)
 
@@ -73,9 +75,9 @@ def test_sd_batch_with_plain_cases(oracle_mod):
             assert np.array_equal(g.state(b)[2:-2, 2:-2], o.state(kb)[2:-2, 2:-2]), (c, kb)
 
 
-@pytest.mark.parametrize("what", ["sd", "hllem"])
+@pytest.mark.parametrize("what", ["sd", "HLLEM", "HLLE++"])
 def test_mixed_form_rejects(what):
-    pb = cases.planar_shock(seed=1, n=32, steps=5, precision="mixed", riemann="HLLEM" if what == "hllem" else "HLLC")
+    pb = cases.planar_shock(seed=1, n=32, steps=5, precision="mixed", riemann="HLLC" if what == "sd" else what)
     pb.cases[0].space_sd = what == "sd"
     with pytest.raises(Q2DError, match="UNSUPPORTED"):
         GpuSolver(pb)

@@ -133,3 +133,43 @@ def test_hllem_upwinds_supersonic_flow(oracle_mod):
     R = np.array([0.8, 2.7, -0.1, 0.9])
     F = o.riemann(2, 1.0, 3.5, L, R, 1.0, 0.0)
     np.testing.assert_allclose(F, _phys(L, 1.0, 0.0), rtol=1e-14)
+
+
+# ---- HLLE++ (Lib_Riemann_HLL.f90:435-530) -----------------------------------------------------------
+def _rand_states(rng):
+    L = np.array([rng.uniform(0.3, 3), rng.normal(), rng.normal(), rng.uniform(0.3, 3)])
+    R = np.array([rng.uniform(0.3, 3), rng.normal(), rng.normal(), rng.uniform(0.3, 3)])
+    th = rng.uniform(0, 2 * math.pi)
+    return L, R, math.cos(th), math.sin(th)
+
+
+def test_hllepp_at_beta_zero_is_hlle(oracle_mod):
+    """beta = 0: the HLLE wave speeds in the Roe wave decomposition.  With the Roe property dF = A dU of the
+    perfect-gas Roe average, that is the HLLE flux (S+ F_L - S- F_R + S+ S- dU) / (S+ - S-) rewritten, so it
+    must equal riemann_HLLE; a wrong wave strength, eigenvector or eigenvalue breaks the identity."""
+    o = oracle_mod.Oracle.__new__(oracle_mod.Oracle)
+    o.lib = oracle_mod.load()
+    rng = np.random.default_rng(8)
+    for _ in range(300):
+        L, R, nx, ny = _rand_states(rng)
+        Fpp = o.riemann(3, 1.0, 3.5, L, R, nx, ny, beta=0.0)
+        Fe = o.riemann(1, 1.0, 3.5, L, R, nx, ny)
+        scale = 1.0 + np.abs(Fe).max()
+        assert np.abs(Fpp - Fe).max() <= 1e-12 * scale, (L, R, Fpp, Fe)
+
+
+def test_hllepp_affine_in_beta_and_consistent(oracle_mod):
+    o = oracle_mod.Oracle.__new__(oracle_mod.Oracle)
+    o.lib = oracle_mod.load()
+    rng = np.random.default_rng(9)
+    for _ in range(100):
+        L, R, nx, ny = _rand_states(rng)
+        F0, F1, Fb = (o.riemann(3, 1.0, 3.5, L, R, nx, ny, beta=b) for b in (0.0, 1.0, 0.3))
+        assert np.abs(Fb - (F0 + 0.3 * (F1 - F0))).max() <= 1e-12 * (1.0 + np.abs(F0).max())
+        for b in (0.0, 0.6, 1.0):
+            np.testing.assert_allclose(o.riemann(3, 1.0, 3.5, L, L, nx, ny, beta=b), _phys(L, nx, ny), rtol=1e-13,
+                                       atol=1e-13)
+    # beta = 1 keeps a stationary contact (Roe): no mass crosses; HLLE (beta = 0) smears it
+    L, R = np.array([1.0, 0.0, 0.0, 1.0]), np.array([0.3, 0.0, 0.0, 1.0])
+    assert abs(o.riemann(3, 1.0, 3.5, L, R, 1.0, 0.0, beta=1.0)[0]) <= 1e-15
+    assert abs(o.riemann(3, 1.0, 3.5, L, R, 1.0, 0.0, beta=0.0)[0]) > 1e-2
