@@ -74,6 +74,7 @@ extern "C" {
 #define Q2D_RIEMANN_HLLE  1
 #define Q2D_RIEMANN_HLLEM 2   /* Lib_Riemann_HLL.f90:355-432; plain FP64 form only */
 #define Q2D_RIEMANN_HLLEPP 3  /* 'HLLE++', Lib_Riemann_HLL.f90:435-530 (uses the shock detector); plain FP64 only */
+#define Q2D_RIEMANN_HLLCP  4  /* 'HLLC+', Lib_Riemann_HLL.f90:571-603 (uses the shock detector); plain FP64 only */
 
 /* source ids — Mod_Sources.f90:16-91 ('axisymmetric' | 'area_variation') */
 #define Q2D_SRC_NONE      0
@@ -99,7 +100,7 @@ extern "C" {
 typedef struct q2d_scheme {
   int32_t n_rk;            /* 1 euler, 2 RK2, 3 RK3  (Lib_RK.f90:8-10) */
   int32_t limiter;         /* Q2D_LIM_* ; 'first-order' reconstruction == IORD */
-  int32_t riemann;         /* Q2D_RIEMANN_* ('HLLC', 'HLLE', 'HLLEM') */
+  int32_t riemann;         /* Q2D_RIEMANN_* ('HLLC', 'HLLE', 'HLLEM', 'HLLE++', 'HLLC+') */
   int32_t time_accurate;   /* 1: global dt (Set_Global_dt), 0: local dt */
   int32_t model;           /* 0 Euler, 1 laminar Navier-Stokes */
   int32_t source;          /* Q2D_SRC_* */

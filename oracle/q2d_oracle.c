@@ -382,6 +382,19 @@ static void riemann(int id, const gas_t *g, double dl, double ul, double vl, dou
     riemann_hllepp(g, dl, ul, vl, pl, al, dltot, dr, ur, vr, pr, ar, drtot, beta, nx, ny, F);
     return;
   }
+  if (id == Q2D_RIEMANN_HLLCP) {                 /* riemann_HLLCSD — Lib_Riemann_HLL.f90:571-603 */
+    if (beta == 0.0) {
+      riemann_hlle(g, dl, ul, vl, pl, al, dltot, dr, ur, vr, pr, ar, drtot, nx, ny, F);
+    } else if (beta >= 1.0) {
+      riemann_hllc(g, dl, ul, vl, pl, al, dltot, dr, ur, vr, pr, ar, drtot, nx, ny, F);
+    } else {
+      double FE[4], FC[4];
+      riemann_hlle(g, dl, ul, vl, pl, al, dltot, dr, ur, vr, pr, ar, drtot, nx, ny, FE);
+      riemann_hllc(g, dl, ul, vl, pl, al, dltot, dr, ur, vr, pr, ar, drtot, nx, ny, FC);
+      for (int k = 0; k < 4; k++) F[k] = beta * FC[k] + (1.0 - beta) * FE[k];
+    }
+    return;
+  }
   if (id == Q2D_RIEMANN_HLLE)
     riemann_hlle(g, dl, ul, vl, pl, al, dltot, dr, ur, vr, pr, ar, drtot, nx, ny, F);
   else if (id == Q2D_RIEMANN_HLLEM)
@@ -803,7 +816,7 @@ static int fluxes_block(oblk *b, const ocase *cs) {
   /* shock detector on Prim(np, i-1:i+1, j-1:j+1) (Mod_Fluxes.f90:129-138); the faces take the value of
    * their cell i (i-faces, :144-150) or j (j-faces, :191-197), with SD_limiter (Lib_Convective.f90:42-46) */
   const int sd = s->space_sd != 0;
-  const int det = sd || s->riemann == Q2D_RIEMANN_HLLEPP;   /* obj_shock_detector%SD, Assign_Setup.f90:49-53 */
+  const int det = sd || s->riemann >= Q2D_RIEMANN_HLLEPP;   /* obj_shock_detector%SD, Assign_Setup.f90:49-53 */
   if (det)
     for (int j = 1; j <= b->nj; j++)
       for (int i = 1; i <= b->ni; i++)
@@ -913,7 +926,7 @@ static int bc_connection_eul(oblk *b, const ocase *cs, int Im, int Jm, int Fm) {
   double a_L = f_ss(&cs->gas, PL[3], rho_L);
   double a_R = f_ss(&cs->gas, PR[3], rho_R);
   double F[4];
-  double beta_rie = s->riemann == Q2D_RIEMANN_HLLEPP ? b->beta[IN(b, Im, Jm)] : 1.0;   /* SD_riemann :103-108 */
+  double beta_rie = s->riemann >= Q2D_RIEMANN_HLLEPP ? b->beta[IN(b, Im, Jm)] : 1.0;   /* SD_riemann :103-108 */
   riemann(s->riemann, &cs->gas, PL[0], PL[1], PL[2], PL[3], a_L, rho_L, PR[0], PR[1], PR[2], PR[3], a_R, rho_R, beta_rie, n1, n2, F);
   double su = copysign(0.5, F[0]);
   double sel_L = 0.5 + su, sel_R = su - 0.5;

@@ -23,7 +23,7 @@ Q2D_E_STATE = -4
 Q2D_E_INTERFACE = -5
 
 LIMITERS = {"first-order": 0, "iord": 0, "minmod": 1, "vanleer": 2, "vanalbada": 3, "mc": 4, "superbee": 5, "LIMO3": 6}
-RIEMANN = {"HLLC": 0, "HLLE": 1, "HLLEM": 2, "HLLE++": 3}
+RIEMANN = {"HLLC": 0, "HLLE": 1, "HLLEM": 2, "HLLE++": 3, "HLLC+": 4}
 SOURCES = {"none": 0, "axisymmetric": 1, "area_variation": 2}
 PRECISION = {"fp64": 0, "mixed": 1}
 

@@ -128,6 +128,20 @@ __device__ __forceinline__ void riemann(const Gas<T>& g, T ul, T vl, T pl, T al,
   T h0l, h0r, u_roe, v_roe, a_roe;
   roe_averages<T>(g, dltot, drtot, ul, vl, ur, vr, pl, pr, h0l, h0r, u_roe, v_roe, a_roe);
   T un_roe = u_roe * nx + v_roe * ny;
+  if (RIE == 4) {                               // HLLC+ (riemann_HLLCSD :571-603): HLLC and HLLE blended by beta
+    if (beta == T(0)) {
+      riemann<T, 1>(g, ul, vl, pl, al, dltot, ur, vr, pr, ar, drtot, nx, ny, F);
+    } else if (beta >= T(1)) {
+      riemann<T, 0>(g, ul, vl, pl, al, dltot, ur, vr, pr, ar, drtot, nx, ny, F);
+    } else {
+      T FE[4], FC[4];
+      riemann<T, 1>(g, ul, vl, pl, al, dltot, ur, vr, pr, ar, drtot, nx, ny, FE);
+      riemann<T, 0>(g, ul, vl, pl, al, dltot, ur, vr, pr, ar, drtot, nx, ny, FC);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) F[k] = beta * FC[k] + (T(1) - beta) * FE[k];
+    }
+    return;
+  }
   if (RIE == 3) {                               // HLLE++ :435-530: Roe waves, speeds blended HLLE <-> Roe by beta
     const T srL = qsqrt<T>(dltot), srR = qsqrt<T>(drtot);
     const T rho_roe = srL * srR;

@@ -484,6 +484,8 @@ __device__ __forceinline__ void riemann_rt(int rid, const Gas<FT>& g, const FT L
   else if (rid == Q2D_RIEMANN_HLLEM) riemann<FT, 2>(g, L[1], L[2], L[3], aL, L[0], R[1], R[2], R[3], aR, R[0], nx, ny, F);
   else if (rid == Q2D_RIEMANN_HLLEPP)
     riemann<FT, 3>(g, L[1], L[2], L[3], aL, L[0], R[1], R[2], R[3], aR, R[0], nx, ny, F, beta);
+  else if (rid == Q2D_RIEMANN_HLLCP)
+    riemann<FT, 4>(g, L[1], L[2], L[3], aL, L[0], R[1], R[2], R[3], aR, R[0], nx, ny, F, beta);
   else riemann<FT, 0>(g, L[1], L[2], L[3], aL, L[0], R[1], R[2], R[3], aR, R[0], nx, ny, F);
 }
 
@@ -580,11 +582,11 @@ __global__ __launch_bounds__(256) void k_bcflux(const DevBlock* __restrict__ blo
     // MUSCL-SD: the boundary cell's detector value (Lib_BC_Fluxes_Connection.f90:64-69), from the state
     // after every record's ghost fill (such handles run the two-phase launch)
     FT bt = FT(1), brie = FT(1);                       // SD_limiter :64-69, SD_riemann :103-108
-    if (C.space_sd || rid == Q2D_RIEMANN_HLLEPP) {
+    if (C.space_sd || rid >= Q2D_RIEMANN_HLLEPP) {
       const FT d = (FT)sd_tramel(Bp[3], P[3 * pl + off(Im - 1, Jm)], P[3 * pl + off(Im + 1, Jm)],
                                  P[3 * pl + off(Im, Jm - 1)], P[3 * pl + off(Im, Jm + 1)]);
       bt = C.space_sd ? d : FT(1);
-      brie = rid == Q2D_RIEMANN_HLLEPP ? d : FT(1);
+      brie = rid >= Q2D_RIEMANN_HLLEPP ? d : FT(1);
     }
     FT L[4], R[4];
 #pragma unroll
@@ -1212,7 +1214,7 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
       const bool last = (irk == C.n_rk);
       const bool irs = C.irs_beta > 0.0;                 // residual to B.irs[0]; k_irs smooths and updates
       const bool sd = C.space_sd != 0;                   // MUSCL-SD: faces take their cell's detector value
-      const bool det = sd || C.riemann == Q2D_RIEMANN_HLLEPP;   // the detector is computed (Assign_Setup.f90:49-53)
+      const bool det = sd || C.riemann >= Q2D_RIEMANN_HLLEPP;   // the detector is computed (Assign_Setup.f90:49-53)
       const int ni = B.ni, nj = B.nj, pitch = B.pitch;
       const long long pl = B.plane;
       // the case's control words, read once per item (wave-uniform; a read inside the row loop would be a
@@ -1595,6 +1597,7 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
       if (C.riemann == Q2D_RIEMANN_HLLE) item_body(std::integral_constant<int, 1>{});
       else if (C.riemann == Q2D_RIEMANN_HLLEM) item_body(std::integral_constant<int, 2>{});
       else if (RIE_T == -2 && C.riemann == Q2D_RIEMANN_HLLEPP) item_body(std::integral_constant<int, 3>{});
+      else if (RIE_T == -2 && C.riemann == Q2D_RIEMANN_HLLCP) item_body(std::integral_constant<int, 4>{});
       else item_body(std::integral_constant<int, 0>{});
     } else {
       item_body(std::integral_constant<int, RIE_T>{});
@@ -2443,7 +2446,7 @@ struct q2d_handle {
   int dt_total = 0;
   bool any_visc = false;
   bool any_irs = false;          // a live case smooths its residual (k_irs after each stage)
-  bool any_hllepp = false;       // a live case uses HLLE++: the stage kernel's RIE_T = -2 instantiation
+  bool any_hllepp = false;       // a live case uses HLLE++ or HLLC+: the stage kernel's RIE_T = -2 instantiation
   bool bc_two_phase = false;     // a live block is one cell wide: ghost fill and BC fluxes in two launches
   std::vector<int> vrec;         // boundary records with a viscous flux (BC 300/302 of model-1 cases)
   int* d_vrec = nullptr;
@@ -2679,7 +2682,7 @@ int launch_stage(q2d_handle* h, int irk) {
       if (pass == 0 && h->any_irs) Q2D_STAGE_MX_LAUNCH(-1, true);
       else if (pass == 0) Q2D_STAGE_MX_LAUNCH(-1, false);
       else Q2D_STAGE_LAUNCH(-1, true);
-    } else if (h->any_hllepp) {                        // HLLE++ (plain FP64 form only)
+    } else if (h->any_hllepp) {                        // HLLE++ / HLLC+ (plain FP64 form only)
       if (pass == 0) Q2D_STAGE_LAUNCH(-2, false);
       else Q2D_STAGE_LAUNCH(-2, true);
     } else {
@@ -2908,8 +2911,8 @@ int rebuild_descriptors(q2d_handle* h) {
     h->max_rk = std::max(h->max_rk, h->dc[c].n_rk);
     if (h->dc[c].model > 0) h->any_visc = true;
     if (h->dc[c].irs_beta > 0.0) h->any_irs = true;
-    if (h->dc[c].riemann == Q2D_RIEMANN_HLLEPP) h->any_hllepp = true;
-    if (h->dc[c].space_sd || h->dc[c].riemann == Q2D_RIEMANN_HLLEPP)
+    if (h->dc[c].riemann >= Q2D_RIEMANN_HLLEPP) h->any_hllepp = true;
+    if (h->dc[c].space_sd || h->dc[c].riemann >= Q2D_RIEMANN_HLLEPP)
       h->bc_two_phase = true;                          // the connection faces' detector reads other records' ghosts
   }
   // device arrays, re-allocated at the current sizes (descriptors are small)
@@ -2959,9 +2962,9 @@ int append_cases(q2d_handle* h, const q2d_problem* prob, int32_t* first_case) {
     if (nblk[c] > 255) return fail(Q2D_E_ARG, "more than 255 blocks in a case");
     if (s.n_rk < 1 || s.n_rk > 3) return fail(Q2D_E_ARG, "n_rk must be 1, 2 or 3");
     if (s.limiter < 0 || s.limiter > 6) return fail(Q2D_E_ARG, "unknown limiter");
-    if (s.riemann < 0 || s.riemann > 3) return fail(Q2D_E_ARG, "unknown Riemann solver");
+    if (s.riemann < 0 || s.riemann > 4) return fail(Q2D_E_ARG, "unknown Riemann solver");
     if (s.riemann >= Q2D_RIEMANN_HLLEM && s.precision != Q2D_PREC_FP64)
-      return fail(Q2D_E_UNSUPPORTED, "HLLEM and HLLE++ are available in the plain FP64 form only");
+      return fail(Q2D_E_UNSUPPORTED, "HLLEM, HLLE++ and HLLC+ are available in the plain FP64 form only");
     if (s.model < 0 || s.model > 1) return fail(Q2D_E_UNSUPPORTED, "only the Euler (0) and laminar (1) models are available");
     if (s.model == 1 && !(s.mu_ref > 0)) return fail(Q2D_E_ARG, "laminar model needs mu_ref > 0");
     if (s.model == 1 && (!(s.T_ref > 0) || !(s.Pr > 0) || !(s.S_suth >= 0)))

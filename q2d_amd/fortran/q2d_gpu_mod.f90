@@ -15,7 +15,8 @@ module q2d_gpu
   integer(c_int), parameter, public :: Q2D_LIM_IORD = 0, Q2D_LIM_MINMOD = 1, Q2D_LIM_VANLEER = 2, &
                                        Q2D_LIM_VANALBADA = 3, Q2D_LIM_MC = 4, Q2D_LIM_SUPERBEE = 5, &
                                        Q2D_LIM_LIMO3 = 6
-  integer(c_int), parameter, public :: Q2D_RIEMANN_HLLC = 0, Q2D_RIEMANN_HLLE = 1, Q2D_RIEMANN_HLLEM = 2, Q2D_RIEMANN_HLLEPP = 3
+  integer(c_int), parameter, public :: Q2D_RIEMANN_HLLC = 0, Q2D_RIEMANN_HLLE = 1, Q2D_RIEMANN_HLLEM = 2, Q2D_RIEMANN_HLLEPP = 3, &
+                                       Q2D_RIEMANN_HLLCP = 4
   integer(c_int), parameter, public :: Q2D_SRC_NONE = 0, Q2D_SRC_AXI = 1, Q2D_SRC_AREA = 2
   integer(c_int), parameter, public :: Q2D_PREC_FP64 = 0, Q2D_PREC_MIXED = 1
 

@@ -35,7 +35,7 @@ def test_sd_ramp_reference_setting(oracle_mod):
 
 
 @pytest.mark.parametrize("rie,lim", [("HLLC", "vanleer"), ("HLLE", "superbee"), ("HLLEM", "minmod"),
-                                     ("HLLE++", "vanleer")])
+                                     ("HLLE++", "vanleer"), ("HLLC+", "minmod")])
 def test_sd_multiblock_shock(oracle_mod, rie, lim):
     """The shock crosses the 101 connections between the 2x2 blocks: connection faces take the boundary
     cell's detector value."""
@@ -44,9 +44,11 @@ def test_sd_multiblock_shock(oracle_mod, rie, lim):
     run_both(oracle_mod, pb, 30)
 
 
-@pytest.mark.parametrize("rie,sd", [("HLLEM", False), ("HLLEM", True), ("HLLE++", False), ("HLLE++", True)])
+@pytest.mark.parametrize("rie,sd", [("HLLEM", False), ("HLLEM", True), ("HLLE++", False), ("HLLE++", True),
+                                    ("HLLC+", False), ("HLLC+", True)])
 def test_hllem_ramp(oracle_mod, rie, sd):
-    """HLLEM (Lib_Riemann_HLL.f90:355-432) and HLLE++ (:435-530), with and without MUSCL-SD, local dt."""
+    """HLLEM (Lib_Riemann_HLL.f90:355-432), HLLE++ (:435-530) and HLLC+ (:571-603), with and without
+    MUSCL-SD, local dt."""
     pb = cases.ramp_channel(ni=96, nj=48, wedge_deg=12.0, mach=2.5, riemann=rie, itermax=60)
     pb.cases[0].space_sd = sd
     g, o = run_both(oracle_mod, pb, 60)
@@ -75,7 +77,7 @@ def test_sd_batch_with_plain_cases(oracle_mod):
             assert np.array_equal(g.state(b)[2:-2, 2:-2], o.state(kb)[2:-2, 2:-2]), (c, kb)
 
 
-@pytest.mark.parametrize("what", ["sd", "HLLEM", "HLLE++"])
+@pytest.mark.parametrize("what", ["sd", "HLLEM", "HLLE++", "HLLC+"])
 def test_mixed_form_rejects(what):
     pb = cases.planar_shock(seed=1, n=32, steps=5, precision="mixed", riemann="HLLC" if what == "sd" else what)
     pb.cases[0].space_sd = what == "sd"

@@ -173,3 +173,18 @@ def test_hllepp_affine_in_beta_and_consistent(oracle_mod):
     L, R = np.array([1.0, 0.0, 0.0, 1.0]), np.array([0.3, 0.0, 0.0, 1.0])
     assert abs(o.riemann(3, 1.0, 3.5, L, R, 1.0, 0.0, beta=1.0)[0]) <= 1e-15
     assert abs(o.riemann(3, 1.0, 3.5, L, R, 1.0, 0.0, beta=0.0)[0]) > 1e-2
+
+
+# ---- HLLC+ (riemann_HLLCSD, Lib_Riemann_HLL.f90:571-603) -------------------------------------------
+def test_hllc_plus_limits_and_blend(oracle_mod):
+    """beta = 0 is HLLE and beta >= 1 is HLLC, bit for bit; between them the convex blend of the two."""
+    o = oracle_mod.Oracle.__new__(oracle_mod.Oracle)
+    o.lib = oracle_mod.load()
+    rng = np.random.default_rng(12)
+    for _ in range(100):
+        L, R, nx, ny = _rand_states(rng)
+        Fc, Fe = o.riemann(0, 1.0, 3.5, L, R, nx, ny), o.riemann(1, 1.0, 3.5, L, R, nx, ny)
+        assert np.array_equal(o.riemann(4, 1.0, 3.5, L, R, nx, ny, beta=0.0), Fe)
+        assert np.array_equal(o.riemann(4, 1.0, 3.5, L, R, nx, ny, beta=1.0), Fc)
+        Fb = o.riemann(4, 1.0, 3.5, L, R, nx, ny, beta=0.3)
+        assert np.abs(Fb - (0.3 * Fc + 0.7 * Fe)).max() <= 1e-14 * (1.0 + np.abs(Fc).max())
