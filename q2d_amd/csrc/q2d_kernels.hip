@@ -2447,6 +2447,8 @@ struct q2d_handle {
   bool any_visc = false;
   bool any_irs = false;          // a live case smooths its residual (k_irs after each stage)
   bool any_hllepp = false;       // a live case uses HLLE++ or HLLC+: the stage kernel's RIE_T = -2 instantiation
+  bool mixed_generic = false;    // mixed form with a MUSCL-SD / HLLEM / HLLE++ / HLLC+ case: Euler items run the
+                                 // generic FP32-face kernel k_stage<float> (k_stage_mx has no room for them)
   bool bc_two_phase = false;     // a live block is one cell wide: ghost fill and BC fluxes in two launches
   std::vector<int> vrec;         // boundary records with a viscous flux (BC 300/302 of model-1 cases)
   int* d_vrec = nullptr;
@@ -2475,6 +2477,9 @@ namespace {
 
 // The stage-kernel instantiation that runs the items of `pass` (2 * viscous + riemann).
 const void* stage_kernel(const q2d_handle* h, int pass) {
+  if (h->precision_mixed && h->mixed_generic)
+    return pass < 2 ? reinterpret_cast<const void*>(&k_stage<float, -1, false, false>)
+                    : reinterpret_cast<const void*>(&k_stage<float, -1, false, true>);
   if (h->precision_mixed)
     return pass < 2 ? reinterpret_cast<const void*>(&k_stage_mx<-1, kFuseDtMx, false>)
                     : reinterpret_cast<const void*>(&k_stage<float, -1, false, true>);
@@ -2679,8 +2684,11 @@ int launch_stage(q2d_handle* h, int irk) {
                        h->d_cases, h->d_ctrl, h->d_items, lo, hi, h->d_bcdense, h->d_partials, irk, h->xcd_remap, ts, \
                        slot)
     if constexpr (sizeof(FT) == 4) {                   // mixed form: Euler items run k_stage_mx
-      if (pass == 0 && h->any_irs) Q2D_STAGE_MX_LAUNCH(-1, true);
-      else if (pass == 0) Q2D_STAGE_MX_LAUNCH(-1, false);
+      if (pass == 0 && !h->mixed_generic && h->any_irs) Q2D_STAGE_MX_LAUNCH(-1, true);
+      else if (pass == 0 && !h->mixed_generic) Q2D_STAGE_MX_LAUNCH(-1, false);
+      else if (h->any_hllepp && pass == 0) Q2D_STAGE_LAUNCH(-2, false);
+      else if (h->any_hllepp) Q2D_STAGE_LAUNCH(-2, true);
+      else if (pass == 0) Q2D_STAGE_LAUNCH(-1, false);
       else Q2D_STAGE_LAUNCH(-1, true);
     } else if (h->any_hllepp) {                        // HLLE++ / HLLC+ (plain FP64 form only)
       if (pass == 0) Q2D_STAGE_LAUNCH(-2, false);
@@ -2735,7 +2743,7 @@ int launch_step(q2d_handle* h) {
   const int cthreads = 64, cgroups = (h->ncases + cthreads - 1) / cthreads;
   const int bgroups = (h->nbc + 63) / 64;            // one wavefront per group: spread over every CU
   const int use_next =
-      (h->dt_fused_valid && !h->any_visc && (h->precision_mixed ? kFuseDtMx : kFuseDtFp64)) ? 1 : 0;
+      (h->dt_fused_valid && !h->any_visc && !h->mixed_generic && (h->precision_mixed ? kFuseDtMx : kFuseDtFp64)) ? 1 : 0;
   // with the dt already reduced, the first boundary launch also does k_begin's work (one launch fewer)
   const bool fold_begin = use_next && h->nbc > 0;
   if (!fold_begin)
@@ -2779,7 +2787,7 @@ int launch_step(q2d_handle* h) {
                          h->d_items, n_items, n_items + n_irs, h->d_bc, h->d_bcface, h->d_partials, irk);
     }
   }
-  const int fused_dt = (!h->any_visc && (h->precision_mixed ? kFuseDtMx : kFuseDtFp64)) ? 1 : 0;
+  const int fused_dt = (!h->any_visc && !h->mixed_generic && (h->precision_mixed ? kFuseDtMx : kFuseDtFp64)) ? 1 : 0;
   hipLaunchKernelGGL(k_finish, dim3(h->ncases), dim3(kFinishThreads), 0, h->stream, h->d_ctrl, h->d_cases, h->d_partials, h->ncases,
                      h->timing ? h->d_tstamp : nullptr, fused_dt);
   HIPCHK(hipGetLastError());
@@ -2903,6 +2911,7 @@ int rebuild_descriptors(q2d_handle* h) {
   h->any_visc = false;
   h->any_irs = false;
   h->any_hllepp = false;
+  h->mixed_generic = false;
   h->bc_two_phase = false;
   for (int b = 0; b < h->nblocks; ++b)
     if (h->hb[b].arena && (h->hb[b].ni < 2 || h->hb[b].nj < 2)) h->bc_two_phase = true;
@@ -2912,6 +2921,7 @@ int rebuild_descriptors(q2d_handle* h) {
     if (h->dc[c].model > 0) h->any_visc = true;
     if (h->dc[c].irs_beta > 0.0) h->any_irs = true;
     if (h->dc[c].riemann >= Q2D_RIEMANN_HLLEPP) h->any_hllepp = true;
+    if (h->precision_mixed && (h->dc[c].space_sd || h->dc[c].riemann >= Q2D_RIEMANN_HLLEM)) h->mixed_generic = true;
     if (h->dc[c].space_sd || h->dc[c].riemann >= Q2D_RIEMANN_HLLEPP)
       h->bc_two_phase = true;                          // the connection faces' detector reads other records' ghosts
   }
@@ -2963,8 +2973,7 @@ int append_cases(q2d_handle* h, const q2d_problem* prob, int32_t* first_case) {
     if (s.n_rk < 1 || s.n_rk > 3) return fail(Q2D_E_ARG, "n_rk must be 1, 2 or 3");
     if (s.limiter < 0 || s.limiter > 6) return fail(Q2D_E_ARG, "unknown limiter");
     if (s.riemann < 0 || s.riemann > 4) return fail(Q2D_E_ARG, "unknown Riemann solver");
-    if (s.riemann >= Q2D_RIEMANN_HLLEM && s.precision != Q2D_PREC_FP64)
-      return fail(Q2D_E_UNSUPPORTED, "HLLEM, HLLE++ and HLLC+ are available in the plain FP64 form only");
+
     if (s.model < 0 || s.model > 1) return fail(Q2D_E_UNSUPPORTED, "only the Euler (0) and laminar (1) models are available");
     if (s.model == 1 && !(s.mu_ref > 0)) return fail(Q2D_E_ARG, "laminar model needs mu_ref > 0");
     if (s.model == 1 && (!(s.T_ref > 0) || !(s.Pr > 0) || !(s.S_suth >= 0)))
@@ -2976,8 +2985,7 @@ int append_cases(q2d_handle* h, const q2d_problem* prob, int32_t* first_case) {
     if (prob->cases[c].res_drop < 0 || prob->cases[c].res_drop >= 1) return fail(Q2D_E_ARG, "res_drop must be in [0, 1)");
     if (!(s.irs_beta >= 0) || !std::isfinite(s.irs_beta)) return fail(Q2D_E_ARG, "irs_beta must be finite and >= 0");
     if (s.space_sd != 0 && s.space_sd != 1) return fail(Q2D_E_ARG, "space_sd must be 0 or 1");
-    if (s.space_sd && s.precision != Q2D_PREC_FP64)
-      return fail(Q2D_E_UNSUPPORTED, "MUSCL-SD is available in the plain FP64 form only");
+
     for (int k = 0; k < nblk[c]; ++k) {
       const q2d_block_desc& d = prob->blocks[first_block[c] + k];
       if (s.source == Q2D_SRC_AXI && !d.src_b) return fail(Q2D_E_ARG, "axisymmetric source needs src_b (axis_inv_r)");
