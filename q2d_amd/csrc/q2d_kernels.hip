@@ -2760,9 +2760,10 @@ int launch_step(q2d_handle* h) {
     if (h->nbc > 0) {
       for (int ph = h->bc_two_phase ? 1 : 0; ph <= (h->bc_two_phase ? 2 : 0); ++ph) {
         const int begin_cases = (fold_begin && irk == 1 && ph <= 1) ? h->ncases : 0;
-        if (h->precision_mixed)
+        if (h->precision_mixed)                        // the dense FP32 face array feeds k_stage_mx only
           hipLaunchKernelGGL(k_bcflux<float>, dim3(bgroups), dim3(64), 0, h->stream, h->d_blocks, h->d_cases,
-                             h->d_ctrl, h->d_bc, h->nbc, irk, h->d_bcflux, ph, begin_cases, h->d_bcdense);
+                             h->d_ctrl, h->d_bc, h->nbc, irk, h->d_bcflux, ph, begin_cases,
+                             h->mixed_generic ? nullptr : h->d_bcdense);
         else
           hipLaunchKernelGGL(k_bcflux<double>, dim3(bgroups), dim3(64), 0, h->stream, h->d_blocks, h->d_cases,
                              h->d_ctrl, h->d_bc, h->nbc, irk, h->d_bcflux, ph, begin_cases, h->d_bcdense);
