@@ -72,9 +72,9 @@ extern "C" {
 /* Riemann solver ids — Mod_Riemann.f90:25-109 */
 #define Q2D_RIEMANN_HLLC  0
 #define Q2D_RIEMANN_HLLE  1
-#define Q2D_RIEMANN_HLLEM 2   /* Lib_Riemann_HLL.f90:355-432; plain FP64 form only */
-#define Q2D_RIEMANN_HLLEPP 3  /* 'HLLE++', Lib_Riemann_HLL.f90:435-530 (uses the shock detector); plain FP64 only */
-#define Q2D_RIEMANN_HLLCP  4  /* 'HLLC+', Lib_Riemann_HLL.f90:571-603 (uses the shock detector); plain FP64 only */
+#define Q2D_RIEMANN_HLLEM 2   /* Lib_Riemann_HLL.f90:355-432 */
+#define Q2D_RIEMANN_HLLEPP 3  /* 'HLLE++', Lib_Riemann_HLL.f90:435-530 (uses the shock detector) */
+#define Q2D_RIEMANN_HLLCP  4  /* 'HLLC+', Lib_Riemann_HLL.f90:571-603 (uses the shock detector) */
 
 /* source ids — Mod_Sources.f90:16-91 ('axisymmetric' | 'area_variation') */
 #define Q2D_SRC_NONE      0
@@ -114,8 +114,7 @@ typedef struct q2d_scheme {
                               Lib_Newstate.f90:35-64); 0 leaves it off */
   int32_t space_sd;        /* 1: space-reconstruction = 'MUSCL-SD' (obj_space_scheme%SD, Assign_Setup.f90:
                               49-53): Tramel's pressure detector (Lib_Shock_Detector.f90:14-38) scales the
-                              MUSCL slopes of every interior and block-connection face; 0: 'MUSCL'.
-                              Plain FP64 form only. */
+                              MUSCL slopes of every interior and block-connection face; 0: 'MUSCL'. */
   int32_t reserved_sd;
 } q2d_scheme;
 
