@@ -3,6 +3,7 @@ from __future__ import annotations
 
 import ctypes as C
 import subprocess
+from contextlib import contextmanager
 from pathlib import Path
 
 import numpy as np
@@ -54,12 +55,60 @@ def load():
     lib.q2do_convective_flux.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, dp, C.c_double, C.c_double,
                                          C.c_double, dp, dp]
     lib.q2do_convective_flux.restype = C.c_int
+    lib.q2do_pow.argtypes = [C.c_double, C.c_double]
+    lib.q2do_pow.restype = C.c_double
+    lib.q2do_plant.argtypes = [C.c_int]
+    lib.q2do_plant.restype = C.c_int
+    lib.q2do_bc_inflow_flux.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, dp, dp] + \
+        [C.c_double] * 8 + [dp]
+    lib.q2do_bc_inflow_flux.restype = C.c_int
+    lib.q2do_bc_outflow_flux.argtypes = [C.c_double, C.c_double, C.c_int, dp] + [C.c_double] * 5 + [dp]
+    lib.q2do_bc_outflow_flux.restype = C.c_int
     _LIB = lib
     return lib
 
 
 def _dp(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+# planted mistakes (q2d_oracle.h Q2DO_PLANT_*): a test shows it detects each one
+PLANTS = {"axi_sign": 1, "axi_energy": 2, "inflow_angle": 3, "inflow_rho": 4, "outflow_riem": 5,
+          "outflow_isen": 6, "ramp": 7}
+
+
+@contextmanager
+def planted(name):
+    """Run the oracle with one planted mistake (tests only)."""
+    lib = load()
+    old = lib.q2do_plant(PLANTS[name])
+    try:
+        yield
+    finally:
+        lib.q2do_plant(old)
+
+
+def pow_(x, y):
+    """exact_pow of the oracle (the `**` of the characteristic outflow)."""
+    return load().q2do_pow(float(x), float(y))
+
+
+def inflow_flux(limiter, riemann, R, cp, face, Bp, Ip, n1, n2, int_l, mach, T, p, rel_fac, alpha):
+    """Supersonic_Inflow face flux (before the face sign); None where the reference falls back to symmetry."""
+    Bp = np.ascontiguousarray(Bp, dtype=np.float64)
+    Ip = np.ascontiguousarray(Ip, dtype=np.float64)
+    F = np.zeros(4)
+    rc = load().q2do_bc_inflow_flux(limiter, riemann, R, cp, face, _dp(Bp), _dp(Ip), n1, n2, int_l, mach, T, p,
+                                    rel_fac, alpha, _dp(F))
+    return None if rc else F
+
+
+def outflow_flux(R, cp, face, Bp, n1, n2, int_l, p_exit, rel_fac):
+    """Outflow face flux (before the face sign); None where the reference falls back to symmetry."""
+    Bp = np.ascontiguousarray(Bp, dtype=np.float64)
+    F = np.zeros(4)
+    rc = load().q2do_bc_outflow_flux(R, cp, face, _dp(Bp), n1, n2, int_l, p_exit, rel_fac, _dp(F))
+    return None if rc else F
 
 
 class Oracle:

@@ -49,6 +49,62 @@ static gas_t mkgas(double R, double cp) {
   gas_t g; g.R = R; g.cp = cp; g.cv = cp - R; g.gam = cp / g.cv;
   g.mu_ref = 0.0; g.T_ref = 1.0; g.S = 0.0; g.Pr = 0.72; return g;
 }
+/* Planted mistakes (tests only, q2do_plant): each makes one restated formula wrong in a way a test of
+ * that formula must detect.  0 = none. */
+static int g_plant = 0;
+
+/* x^y for x > 0, from IEEE-754 binary64 +, -, *, /, fma, rint, frexp and ldexp only (each exactly
+ * specified), so that any conforming evaluation of this sequence gives the same bits; the device
+ * evaluates the same sequence.  Method: x = m 2^e, m in [sqrt(1/2), sqrt(2)); ln m = 2 atanh(s),
+ * s = (m - 1) / (m + 1), with s and the leading term carried in double-double and the odd series
+ * 2 s^3 (1/3 + s^2/5 + ...) in double; ln x = e ln2 + ln m and t = y ln x in double-double (Cody-Waite
+ * split of ln 2); exp(t) = 2^k exp(r), k = rint(t / ln 2), r = t - k ln 2, exp(r) by its Taylor series
+ * to r^13/13!.  The reference's `**` is the C library's pow; tests hold this within 1 ulp of it. */
+static void two_sum(double a, double b, double *s, double *e) {
+  double t = a + b, bb = t - a;
+  *s = t; *e = (a - (t - bb)) + (b - bb);
+}
+static double exact_pow(double x, double y) {
+  if (!(x > 0.0) || isinf(x) || isinf(y) || isnan(y)) return NAN;
+  if (y == 0.0) return 1.0;
+  const double ln2_hi = 6.93147180369123816490e-01, ln2_lo = 1.90821492927058770002e-10;
+  const double inv_ln2 = 1.44269504088896338700e+00;
+  int e;
+  double m = frexp(x, &e);                              /* m in [0.5, 1) */
+  if (m < 0.70710678118654752440) { m = m * 2.0; e = e - 1; }
+  const double num = m - 1.0;                           /* exact (Sterbenz) */
+  double den, den_lo;
+  two_sum(m, 1.0, &den, &den_lo);
+  const double s = num / den;
+  const double s_lo = (fma(-s, den, num) - s * den_lo) / den;
+  const double s2 = s * s;
+  double q = 1.0 / 23.0;
+  q = q * s2 + 1.0 / 21.0; q = q * s2 + 1.0 / 19.0; q = q * s2 + 1.0 / 17.0; q = q * s2 + 1.0 / 15.0;
+  q = q * s2 + 1.0 / 13.0; q = q * s2 + 1.0 / 11.0; q = q * s2 + 1.0 / 9.0; q = q * s2 + 1.0 / 7.0;
+  q = q * s2 + 1.0 / 5.0; q = q * s2 + 1.0 / 3.0;
+  const double tail = 2.0 * s * s2 * q;
+  double lm, lm_lo;
+  two_sum(2.0 * s, 2.0 * s_lo + tail, &lm, &lm_lo);      /* ln m */
+  const double de = (double)e;
+  double lh, ll;
+  two_sum(de * ln2_hi, lm, &lh, &ll);                    /* ln x = lh + ll */
+  ll = ll + (lm_lo + de * ln2_lo);
+  double th = y * lh;
+  double tl = fma(y, lh, -th) + y * ll;                  /* t = y ln x */
+  two_sum(th, tl, &th, &tl);
+  const double k = rint(th * inv_ln2);
+  const double r_hi = th - k * ln2_hi;                   /* exact for |k| < 2^10 (Sterbenz) */
+  const double r_lo = tl - k * ln2_lo;
+  double r, r_err;
+  two_sum(r_hi, r_lo, &r, &r_err);                       /* r + r_err = t - k ln 2 */
+  double p = 1.0 / 6227020800.0;                         /* 1/13! */
+  p = p * r + 1.0 / 479001600.0; p = p * r + 1.0 / 39916800.0; p = p * r + 1.0 / 3628800.0;
+  p = p * r + 1.0 / 362880.0; p = p * r + 1.0 / 40320.0; p = p * r + 1.0 / 5040.0; p = p * r + 1.0 / 720.0;
+  p = p * r + 1.0 / 120.0; p = p * r + 1.0 / 24.0; p = p * r + 1.0 / 6.0; p = p * r + 0.5;
+  const double em1 = r + (r_err + r * r * p);            /* exp(r) - 1 */
+  return ldexp(1.0 + em1, (int)k);
+}
+
 /* Laminar transport (FLINT f_laminarViscosity / co_k_mi_lam_Wilke, closed form):
  * Sutherland mu = mu_ref * (T/T_ref)^1.5 * (T_ref + S) / (T + S), with (T/T_ref)^1.5 = x*sqrt(x);
  * k = mu * cp / Pr. */
@@ -572,7 +628,8 @@ static void compute_dt(q2do *o, ocase *cs) {
           }
         }
         double dtcell = dtmin;
-        if (cs->iter < s->ramp_iter) dtcell = dtcell * cs->iter / s->ramp_iter;
+        if (cs->iter < s->ramp_iter)
+          dtcell = g_plant == Q2DO_PLANT_RAMP ? dtcell * (cs->iter - 1) / s->ramp_iter : dtcell * cs->iter / s->ramp_iter;
         b->dtlocal[IN(b, i, j)] = dtcell;
         if (dtcell < dtmin_blk) dtmin_blk = dtcell;
       }
@@ -997,90 +1054,101 @@ static void bc_extrapolation(oblk *b, const ocase *cs, int Im, int Jm, int Fm) {
   for (int k = 0; k < NP; k++) RV(b, k, Im, Jm) = RV(b, k, Im, Jm) + sgn * Flux[k];
 }
 
-/* BC_Inlet_Supersonic_Static + Supersonic_Inflow — Lib_BC_Fluxes_Inflow.f90:272-403 */
-static void bc_supersonic_inflow(oblk *b, const ocase *cs, const obc *rec, int Im, int Jm, int Fm) {
-  const q2d_scheme *s = &cs->d.scheme;
-  const gas_t *g = &cs->gas;
-  const q2d_bc_entry *e = &rec->e;
+/* Supersonic_Inflow — Lib_BC_Fluxes_Inflow.f90:318-403: the flux of the boundary face from the boundary
+ * cell P2, its interior neighbour Ip and the record's inflow values (Mach, T, p, relaxation, direction).
+ * Returns 1 when the flow leaves through the face (BC_Sign * Un >= 0, :296-299): the caller then runs
+ * BC_Symmetry_Eul (Mod_BC_Fluxes.f90:221). */
+static int inflow_flux(const gas_t *g, int lim, int rid, int Fm, const double *P2, const double *Ip, double n1,
+                       double n2, double int_l, double mach, double T_in, double p_in, double rel_fac,
+                       double cos_a, double sin_a, double *Flux) {
   int modfm = Fm % 2, modfm1 = 1 - modfm, modfm2 = 1 - 2 * modfm;
-  int Int_i = Im + guide[Fm][0], Int_j = Jm + guide[Fm][1];
-  int dir; double n1, n2, int_l;
-  face_geom(b, Fm, Im, Jm, &dir, &n1, &n2, &int_l);
-  double P2[NP], Ip[NP];
-  for (int v = 0; v < NP; v++) { P2[v] = PV(b, v, Im, Jm); Ip[v] = PV(b, v, Int_i, Int_j); }
   double BC_Sign = (double)modfm2;
   double Un = P2[1] * n1 + P2[2] * n2;
-  if (BC_Sign * Un >= 0.0) {                           /* :296-299 -> BC_Symmetry_Eul (Mod_BC_Fluxes.f90:221) */
-    bc_symmetry_eul(b, cs, Im, Jm, Fm);
-    return;
-  }
+  if (BC_Sign * Un >= 0.0) return 1;
   double flow_Sound = f_ss(g, P2[3], P2[0]);
   double flow_Mach = sqrt(P2[1] * P2[1] + P2[2] * P2[2]) / flow_Sound;
-  double Mach_l = e->rel_fac * e->mach + (1.0 - e->rel_fac) * flow_Mach;
-  double p0_l = e->rel_fac * e->p + (1.0 - e->rel_fac) * P2[3];
+  double Mach_l = rel_fac * mach + (1.0 - rel_fac) * flow_Mach;
+  double p0_l = rel_fac * p_in + (1.0 - rel_fac) * P2[3];
   double Sup[NP];
   Sup[3] = p0_l;
-  double Sup_T = e->T;
+  double Sup_T = T_in;
   double Sup_Rgas = 1.0 * g->R;
-  double Sup_rho = Sup[3] / (Sup_Rgas * Sup_T);
+  double Sup_rho = g_plant == Q2DO_PLANT_INFLOW_RHO ? Sup[3] / Sup_T : Sup[3] / (Sup_Rgas * Sup_T);
   Sup[0] = Sup_rho * 1.0;
   double Sup_Sound = f_ss(g, Sup[3], Sup_rho);
-  Sup[1] = Mach_l * Sup_Sound * rec->cos_a;
-  Sup[2] = Mach_l * Sup_Sound * rec->sin_a;
+  if (g_plant == Q2DO_PLANT_INFLOW_ANGLE) { double t = cos_a; cos_a = sin_a; sin_a = t; }
+  Sup[1] = Mach_l * Sup_Sound * cos_a;
+  Sup[2] = Mach_l * Sup_Sound * sin_a;
   double dm = (double)modfm, dm1 = (double)modfm1, dm2 = (double)modfm2;
   double P1[NP], P3[NP], Face[NP], PL[NP], PR[NP];
   for (int v = 0; v < NP; v++) { P1[v] = dm * Sup[v] + dm1 * Ip[v]; P3[v] = dm * Ip[v] + dm1 * Sup[v]; }
   for (int v = 0; v < NP; v++) {
     double d32 = P3[v] - P2[v], d21 = P2[v] - P1[v];
-    double sl = limiter(s->limiter, d32, d21);
+    double sl = limiter(lim, d32, d21);
     Face[v] = P2[v] + 0.5 * sl * dm2;
   }
   for (int v = 0; v < NP; v++) { PL[v] = dm * Sup[v] + dm1 * Face[v]; PR[v] = dm * Face[v] + dm1 * Sup[v]; }
   double rho_L = PL[0], rho_R = PR[0];
   double a_L = f_ss(g, PL[3], rho_L), a_R = f_ss(g, PR[3], rho_R);
   double F[4];
-  riemann(s->riemann, g, PL[0], PL[1], PL[2], PL[3], a_L, rho_L, PR[0], PR[1], PR[2], PR[3], a_R, rho_R, 1.0, n1, n2, F);
+  riemann(rid, g, PL[0], PL[1], PL[2], PL[3], a_L, rho_L, PR[0], PR[1], PR[2], PR[3], a_R, rho_R, 1.0, n1, n2, F);
   double su = copysign(1.0, F[0]);
   double Sel_L = (1.0 + su) / 2.0, Sel_R = (su - 1.0) / 2.0;
   double Fmass = F[0] * int_l;
-  double Flux[NP];
   Flux[0] = Fmass * (Sel_L * PL[0] / rho_L - Sel_R * PR[0] / rho_R);
   Flux[1] = F[1] * int_l;
   Flux[2] = F[2] * int_l;
   Flux[3] = F[3] * int_l;
-  for (int v = 0; v < NP; v++) RV(b, v, Im, Jm) = RV(b, v, Im, Jm) + dm2 * Flux[v];
+  return 0;
 }
 
-/* BC_Outflow + Outflow — Lib_BC_Fluxes_Outflow.f90:36-165 */
-static void bc_outflow(oblk *b, const ocase *cs, const obc *rec, int Im, int Jm, int Fm) {
-  const gas_t *g = &cs->gas;
+/* BC_Inlet_Supersonic_Static — Lib_BC_Fluxes_Inflow.f90:272-316 */
+static void bc_supersonic_inflow(oblk *b, const ocase *cs, const obc *rec, int Im, int Jm, int Fm) {
+  const q2d_scheme *s = &cs->d.scheme;
   const q2d_bc_entry *e = &rec->e;
-  int modfm2 = 1 - 2 * (Fm % 2);
+  int Int_i = Im + guide[Fm][0], Int_j = Jm + guide[Fm][1];
   int dir; double n1, n2, int_l;
   face_geom(b, Fm, Im, Jm, &dir, &n1, &n2, &int_l);
-  double Bp[NP];
-  for (int v = 0; v < NP; v++) Bp[v] = PV(b, v, Im, Jm);
+  double P2[NP], Ip[NP], Flux[NP];
+  for (int v = 0; v < NP; v++) { P2[v] = PV(b, v, Im, Jm); Ip[v] = PV(b, v, Int_i, Int_j); }
+  if (inflow_flux(&cs->gas, s->limiter, s->riemann, Fm, P2, Ip, n1, n2, int_l, e->mach, e->T, e->p, e->rel_fac,
+                  rec->cos_a, rec->sin_a, Flux)) {
+    bc_symmetry_eul(b, cs, Im, Jm, Fm);
+    return;
+  }
+  int modfm2 = 1 - 2 * (Fm % 2);
+  for (int v = 0; v < NP; v++) RV(b, v, Im, Jm) = RV(b, v, Im, Jm) + (double)modfm2 * Flux[v];
+}
+
+/* Outflow — Lib_BC_Fluxes_Outflow.f90:89-165: the flux of the boundary face from the boundary cell Bp.
+ * Returns 1 when the flow enters through the face (BC_Sign * Un < 0, :68-72): the caller then runs
+ * BC_Symmetry_Eul (Mod_BC_Fluxes.f90:227).  `**` is exact_pow. */
+static int outflow_flux(const gas_t *g, int Fm, const double *Bp, double n1, double n2, double int_l,
+                        double BC_pexit, double BC_rel_fac, double *Flux) {
+  int modfm2 = 1 - 2 * (Fm % 2);
   double BC_Sign = (double)modfm2;
   double Un = Bp[1] * n1 + Bp[2] * n2;
-  if (BC_Sign * Un < 0.0) { bc_symmetry_eul(b, cs, Im, Jm, Fm); return; }
+  if (BC_Sign * Un < 0.0) return 1;
   double Bound_rho = Bp[0], Bound_Rgas = g->R;
   double Bound_Sound = f_ss(g, Bp[3], Bound_rho);
   double Bound_Gamma = g->gam;
-  double Riem = Bound_Sound + BC_Sign * 0.5 * (Bound_Gamma - 1.0) * Un;
+  double Riem = g_plant == Q2DO_PLANT_OUTFLOW_RIEM ? Bound_Sound - BC_Sign * 0.5 * (Bound_Gamma - 1.0) * Un
+                                                   : Bound_Sound + BC_Sign * 0.5 * (Bound_Gamma - 1.0) * Un;
   double VN1 = Un * n1, VN2 = Un * n2;
   double VT1 = Bp[1] - VN1, VT2 = Bp[2] - VN2;
   double Mach_N = fabs(Un / Bound_Sound);
   double Fp[NP], Face_Un, Face_rho, Face_Rgas;
-  if (Mach_N < 1.0 && e->p > 0.0) {
-    double p_exit = e->p;
+  if (Mach_N < 1.0 && BC_pexit > 0.0) {
+    double p_exit = BC_pexit;
     double delta = 0.5 * (Bound_Gamma - 1.0);
     double ex = Bound_Gamma / (Bound_Gamma - 1.0);
-    double den = 1.0 / pow(1.0 + delta, ex);
-    double pstar = Bp[3] * den * pow(1.0 + delta * (Mach_N * Mach_N), ex);
+    double den = 1.0 / exact_pow(1.0 + delta, ex);
+    double pstar = Bp[3] * den * exact_pow(1.0 + delta * (Mach_N * Mach_N), ex);
     if (pstar >= p_exit) p_exit = pstar;
-    double Isentropic = (Bound_Gamma - 1.0) / (2.0 * Bound_Gamma);
-    Fp[3] = p_exit * e->rel_fac + Bp[3] * (1.0 - e->rel_fac);
-    double Face_Sound = Bound_Sound * pow(Fp[3] / Bp[3], Isentropic);
+    double Isentropic = g_plant == Q2DO_PLANT_OUTFLOW_ISEN ? (Bound_Gamma - 1.0) / Bound_Gamma
+                                                           : (Bound_Gamma - 1.0) / (2.0 * Bound_Gamma);
+    Fp[3] = p_exit * BC_rel_fac + Bp[3] * (1.0 - BC_rel_fac);
+    double Face_Sound = Bound_Sound * exact_pow(Fp[3] / Bp[3], Isentropic);
     Face_Un = (Riem - Face_Sound) / (BC_Sign * 0.5 * (Bound_Gamma - 1.0));
     Face_rho = Bound_Gamma * Fp[3] / (Face_Sound * Face_Sound);
     Fp[0] = Bp[0] / Bound_rho * Face_rho;
@@ -1093,12 +1161,23 @@ static void bc_outflow(oblk *b, const ocase *cs, const obc *rec, int Im, int Jm,
   }
   double Face_T = Fp[3] / (Face_Rgas * Face_rho);
   double Face_Enthalpy = 0.0 + Fp[0] / Face_rho * (g->cp * Face_T);
-  double Flux[NP];
   Flux[0] = Fp[0] * int_l * Face_Un;
   double mdot = Face_rho * int_l * Face_Un;
   Flux[1] = Fp[1] * mdot + Fp[3] * int_l * n1;
   Flux[2] = Fp[2] * mdot + Fp[3] * int_l * n2;
   Flux[3] = mdot * (0.5 * (Fp[1] * Fp[1] + Fp[2] * Fp[2]) + Face_Enthalpy);
+  return 0;
+}
+
+/* BC_Outflow — Lib_BC_Fluxes_Outflow.f90:36-86 */
+static void bc_outflow(oblk *b, const ocase *cs, const obc *rec, int Im, int Jm, int Fm) {
+  const q2d_bc_entry *e = &rec->e;
+  int dir; double n1, n2, int_l;
+  face_geom(b, Fm, Im, Jm, &dir, &n1, &n2, &int_l);
+  double Bp[NP], Flux[NP];
+  for (int v = 0; v < NP; v++) Bp[v] = PV(b, v, Im, Jm);
+  if (outflow_flux(&cs->gas, Fm, Bp, n1, n2, int_l, e->p, e->rel_fac, Flux)) { bc_symmetry_eul(b, cs, Im, Jm, Fm); return; }
+  int modfm2 = 1 - 2 * (Fm % 2);
   for (int v = 0; v < NP; v++) RV(b, v, Im, Jm) = RV(b, v, Im, Jm) + (double)modfm2 * Flux[v];
 }
 
@@ -1415,7 +1494,10 @@ static void sources(q2do *o, ocase *cs) {
           S[0] = -(rho * v * invr);
           S[1] = -(rho * u * v * invr);
           S[2] = -(rho * v * v * invr);
-          S[3] = -((rho * f_E0(&cs->gas, p, rho, mod_v) + p) * v * invr);
+          S[3] = g_plant == Q2DO_PLANT_AXI_ENERGY ? -((rho * f_E0(&cs->gas, p, rho, mod_v)) * v * invr)
+                                                  : -((rho * f_E0(&cs->gas, p, rho, mod_v) + p) * v * invr);
+          if (g_plant == Q2DO_PLANT_AXI_SIGN)
+            for (int k = 0; k < NP; k++) S[k] = -S[k];
         } else {
           if (!b->sa) continue;
           double Gx = b->sa[IN(b, i, j)], Gy = b->sb[IN(b, i, j)];
@@ -1717,4 +1799,20 @@ int q2do_irs_smooth(q2do *o, int case_id, double beta) {
 
 double q2do_sd_tramel(double p, double p_im, double p_ip, double p_jm, double p_jp) {
   return sd_tramel(p, p_im, p_ip, p_jm, p_jp);
+}
+
+double q2do_pow(double x, double y) { return exact_pow(x, y); }
+int q2do_plant(int which) { int old = g_plant; g_plant = which; return old; }
+
+int q2do_bc_inflow_flux(int limiter_id, int riemann_id, double gas_R, double gas_cp, int face, const double *Bp,
+                        const double *Ip, double n1, double n2, double int_l, double mach, double T, double p,
+                        double rel_fac, double alpha, double *F) {
+  gas_t g = mkgas(gas_R, gas_cp);
+  return inflow_flux(&g, limiter_id, riemann_id, face, Bp, Ip, n1, n2, int_l, mach, T, p, rel_fac, cos(alpha),
+                     sin(alpha), F);
+}
+int q2do_bc_outflow_flux(double gas_R, double gas_cp, int face, const double *Bp, double n1, double n2,
+                         double int_l, double p_exit, double rel_fac, double *F) {
+  gas_t g = mkgas(gas_R, gas_cp);
+  return outflow_flux(&g, face, Bp, n1, n2, int_l, p_exit, rel_fac, F);
 }

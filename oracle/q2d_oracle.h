@@ -62,6 +62,29 @@ int q2do_get_residual(q2do *o, int block, double *R);
 int q2do_set_residual(q2do *o, int block, const double *R);
 int q2do_irs_smooth(q2do *o, int case_id, double beta);
 
+/* x^y of the characteristic outflow (exact_pow: the same exactly specified operations as the device's). */
+double q2do_pow(double x, double y);
+/* Boundary-face fluxes of one record from its cells (no ghost fill), the flux before the face sign:
+ * Supersonic_Inflow (Lib_BC_Fluxes_Inflow.f90:318-403) with the inflow direction alpha [rad], and
+ * Outflow (Lib_BC_Fluxes_Outflow.f90:89-165).  Return 1 where the reference falls back to BC_Symmetry_Eul. */
+int q2do_bc_inflow_flux(int limiter, int riemann, double gas_R, double gas_cp, int face, const double *Bp,
+                        const double *Ip, double n1, double n2, double int_l, double mach, double T, double p,
+                        double rel_fac, double alpha, double *F);
+int q2do_bc_outflow_flux(double gas_R, double gas_cp, int face, const double *Bp, double n1, double n2,
+                         double int_l, double p_exit, double rel_fac, double *F);
+
+/* Planted mistakes, for tests that must show they detect them (0 restores the restatement).
+ * Returns the previous setting. */
+#define Q2DO_PLANT_NONE          0
+#define Q2DO_PLANT_AXI_SIGN      1   /* Source_Axi_Eul_Blk with the sign of the source reversed */
+#define Q2DO_PLANT_AXI_ENERGY    2   /* Source_Axi_Eul_Blk energy term without + p */
+#define Q2DO_PLANT_INFLOW_ANGLE  3   /* Supersonic_Inflow with cos / sin of the flow angle swapped */
+#define Q2DO_PLANT_INFLOW_RHO    4   /* Supersonic_Inflow density p / T (gas constant dropped) */
+#define Q2DO_PLANT_OUTFLOW_RIEM  5   /* Outflow Riemann invariant with the sign of the Un term reversed */
+#define Q2DO_PLANT_OUTFLOW_ISEN  6   /* Outflow isentropic exponent (g-1)/g instead of (g-1)/(2g) */
+#define Q2DO_PLANT_RAMP          7   /* CFL ramp with iter - 1 in place of iter */
+int q2do_plant(int which);
+
 #ifdef __cplusplus
 }
 #endif

@@ -469,3 +469,81 @@ def laminar_area_channel(seed=0, ni=128, nj=40, area_a=0.5, first_cell=2e-4, rey
     A = 1.0 + area_a * np.sin(math.pi * node[..., 0]) ** 2
     blk.src_a, blk.src_b = area_geometry(geo, A)
     return pb
+
+
+def noh_cylindrical(nj=1000, ni=4, r_max=2.5, rho0=1.0, u0=1.0, p0=1e-9, t_final=1.0, precision="fp64"):
+    """test/2D/euler/noh-cylindrical (gen_case.py, input.ini): cylindrical Noh implosion on an (x, r) strip,
+    r in [0, r_max] resolved by nj cells, ni cells in x with dx = dr; rho = rho0, u = 0, v = -u0, p = p0.
+    BC 300 on faces 1, 2 (x-invariance) and 3 (the axis), 400 on face 4.  First-order reconstruction,
+    HLLE, SSP-RK3, CFL 0.5, time-accurate to time-threshold t_final, axisymmetric source, Air table gas."""
+    dr = r_max / nj
+    node = cartesian_nodes(ni, nj, 0.0, ni * dr, 0.0, r_max)
+    geo = BlockGeometry(node)
+    P = np.zeros((nj + 4, ni + 4, 4))
+    P[2:-2, 2:-2] = [rho0, 0.0, -u0, p0]
+    fill_initial_ghosts(P)
+    spec = CaseSpec(n_rk=3, limiter="first-order", riemann="HLLE", cfl=0.5, gas_R=AIR_R, gas_cp=AIR_CP,
+                    source="axisymmetric", precision=precision, time_threshold=t_final, itermax=1 << 30)
+    blk = Block(geo, P)
+    pb = Problem(blocks=[blk], bc=face_records(1, ni, nj, {1: 300, 2: 300, 3: 300, 4: 400}), cases=[spec],
+                 meta={"name": "noh_cylindrical", "rho0": rho0, "u0": u0, "p0": p0, "t_final": t_final})
+    finish(pb, [geo])
+    blk.src_a, blk.src_b = axisymmetric_geometry(geo)
+    return pb
+
+
+# The reference's 1D refinement ladder (test/1D/<case>/input.ini and INPUT/ic_x<L>.tec, MESH/mesh_x<L>.tec):
+# x in [0, 1] with 100 L cells, one row of height 0.01, left / right states split at x0, Air table gas,
+# SSP-RK3, time-accurate to time-threshold.  States as the ic files hold them (%.15E).
+LADDER_1D = {
+    "Ein91": dict(x0=0.5, t=0.15, left=(0.999999980682838, -2.0, 0.0, 0.4), right=(0.999999980682838, 2.0, 0.0, 0.4),
+                  limiter="vanleer", riemann="HLLE", cfl=0.9, bc=(400, 400, 400, 400)),
+    "Tor99": dict(x0=0.3, t=0.20, left=(0.999998975822492, 0.75, 0.0, 1.0), right=(0.124999889921731, 0.0, 0.0, 0.1),
+                  limiter="vanleer", riemann="HLLE++", cfl=0.9, bc=(400, 400, 300, 300)),
+    "Noh87": dict(x0=0.5, t=1.0, left=(1.0, 1.0, 0.0, 1e-15), right=(1.0, -1.0, 0.0, 1e-15),
+                  limiter="first-order", riemann="HLLE", cfl=0.5, bc=(400, 400, 400, 400)),
+    "Sod79": dict(x0=0.5, t=0.20, left=(0.999998975822492, 0.0, 0.0, 1.0), right=(0.124999889921731, 0.0, 0.0, 0.1),
+                  limiter="mc", riemann="HLLC", cfl=0.9, bc=(300, 300, 300, 300)),
+}
+
+
+def ladder_1d(name, level=1, precision="fp64"):
+    """One level of the reference's 1D ladder (LADDER_1D).  Sod79's reference solver is SLAU, which this
+    path does not build: it runs here with HLLC."""
+    c = LADDER_1D[name]
+    n = 100 * level
+    xn = np.array([float(f"{i / n: .15E}") for i in range(n + 1)])
+    node = np.empty((2, n + 1, 2))
+    node[..., 0] = xn[None, :]
+    node[0, :, 1] = 0.0
+    node[1, :, 1] = 0.01
+    geo = BlockGeometry(node)
+    xc = 0.5 * (xn[:-1] + xn[1:])
+    P = np.zeros((5, n + 4, 4))
+    P[2, 2:-2] = np.where((xc < c["x0"])[:, None], np.array(c["left"])[None, :], np.array(c["right"])[None, :])
+    fill_initial_ghosts(P)
+    spec = CaseSpec(n_rk=3, limiter=c["limiter"], riemann=c["riemann"], cfl=c["cfl"], gas_R=AIR_R, gas_cp=AIR_CP,
+                    precision=precision, time_threshold=c["t"], itermax=1 << 30)
+    b1, b2, b3, b4 = c["bc"]
+    pb = Problem(blocks=[Block(geo, P)], bc=face_records(1, n, 1, {1: b1, 2: b2, 3: b3, 4: b4}), cases=[spec],
+                 meta={"name": name, "level": level, "xc": xc})
+    return finish(pb, [geo])
+
+
+def outflow_channel(ni=96, nj=32, mach=0.5, p_exit=1.2, rel_fac=0.5, wedge_deg=6.0, backflow_cols=0,
+                    precision="fp64", n_rk=3, limiter="vanleer", riemann="HLLC", time_accurate=False, steps=100):
+    """A channel with the characteristic outflow BC 406 (exit pressure p_exit, relaxation rel_fac) on face 2:
+    the sheared mesh of ramp_channel (so exit faces are not aligned with x), BC 405 imposing the
+    stream at Mach `mach` on face 1, BC 300 on face 3, BC 400 on face 4.  Uniform stream, except that the
+    last `backflow_cols` columns flow back into the domain (BC 406 then falls back to BC_Symmetry_Eul)."""
+    pb = ramp_channel(ni=ni, nj=nj, wedge_deg=wedge_deg, mach=mach, precision=precision, n_rk=n_rk,
+                      limiter=limiter, riemann=riemann, itermax=steps)
+    pb.cases[0].time_accurate = time_accurate
+    for r in pb.bc:
+        if r["f"] == 2:
+            r.update(type=406, p=p_exit, rel_fac=rel_fac)
+    if backflow_cols:
+        P = pb.blocks[0].P
+        P[2:-2, -2 - backflow_cols:-2, 1] *= -0.5
+    pb.meta.update(name="outflow_channel", p_exit=p_exit, rel_fac=rel_fac)
+    return pb

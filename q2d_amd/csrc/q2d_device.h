@@ -36,6 +36,59 @@ template <typename T> __device__ __forceinline__ T max3(T a, T b, T c) { T m = a
 // a = sqrt(gamma*p/rho)  (f_ss)
 template <typename T> __device__ __forceinline__ T sound(const Gas<T>& g, T p, T rho) { return qsqrt<T>(qdiv<T>(g.gam * p, rho)); }
 
+// ---- x^y from exactly specified operations --------------------------------
+// The `**` of the characteristic outflow (Lib_BC_Fluxes_Outflow.f90:125-134) for x > 0, built from
+// IEEE-754 binary64 +, -, *, /, fma, rint, frexp and ldexp only, so the result does not depend on a
+// math library's rounding: x = m 2^e, m in [sqrt(1/2), sqrt(2)); ln m = 2 atanh(s), s = (m-1)/(m+1),
+// with s and the leading term in double-double and the odd series 2 s^3 (1/3 + s^2/5 + ... + s^20/23)
+// in double; t = y (e ln2 + ln m) in double-double (ln 2 split hi/lo as in Cody-Waite); exp(t) = 2^k
+// exp(r), k = rint(t / ln 2), r = t - k ln 2, exp(r) - 1 by its Taylor series to r^13/13!.  Within one
+// ulp of the correctly rounded power (measured against the C library's pow).
+__device__ __forceinline__ void dd_two_sum(double a, double b, double& s, double& e) {
+  const double t = a + b, bb = t - a;
+  s = t; e = (a - (t - bb)) + (b - bb);
+}
+__device__ __forceinline__ double exact_pow(double x, double y) {
+  if (!(x > 0.0) || __builtin_isinf(x) || __builtin_isinf(y) || __builtin_isnan(y)) return __builtin_nan("");
+  if (y == 0.0) return 1.0;
+  const double ln2_hi = 6.93147180369123816490e-01, ln2_lo = 1.90821492927058770002e-10;
+  const double inv_ln2 = 1.44269504088896338700e+00;
+  int e;
+  double m = __builtin_frexp(x, &e);
+  if (m < 0.70710678118654752440) { m = m * 2.0; e = e - 1; }
+  const double num = m - 1.0;
+  double den, den_lo;
+  dd_two_sum(m, 1.0, den, den_lo);
+  const double s = num / den;
+  const double s_lo = (__builtin_fma(-s, den, num) - s * den_lo) / den;
+  const double s2 = s * s;
+  double q = 1.0 / 23.0;
+  q = q * s2 + 1.0 / 21.0; q = q * s2 + 1.0 / 19.0; q = q * s2 + 1.0 / 17.0; q = q * s2 + 1.0 / 15.0;
+  q = q * s2 + 1.0 / 13.0; q = q * s2 + 1.0 / 11.0; q = q * s2 + 1.0 / 9.0; q = q * s2 + 1.0 / 7.0;
+  q = q * s2 + 1.0 / 5.0; q = q * s2 + 1.0 / 3.0;
+  const double tail = 2.0 * s * s2 * q;
+  double lm, lm_lo;
+  dd_two_sum(2.0 * s, 2.0 * s_lo + tail, lm, lm_lo);
+  const double de = (double)e;
+  double lh, ll;
+  dd_two_sum(de * ln2_hi, lm, lh, ll);
+  ll = ll + (lm_lo + de * ln2_lo);
+  double th = y * lh;
+  double tl = __builtin_fma(y, lh, -th) + y * ll;
+  dd_two_sum(th, tl, th, tl);
+  const double k = __builtin_rint(th * inv_ln2);
+  const double r_hi = th - k * ln2_hi;
+  const double r_lo = tl - k * ln2_lo;
+  double r, r_err;
+  dd_two_sum(r_hi, r_lo, r, r_err);
+  double p = 1.0 / 6227020800.0;
+  p = p * r + 1.0 / 479001600.0; p = p * r + 1.0 / 39916800.0; p = p * r + 1.0 / 3628800.0;
+  p = p * r + 1.0 / 362880.0; p = p * r + 1.0 / 40320.0; p = p * r + 1.0 / 5040.0; p = p * r + 1.0 / 720.0;
+  p = p * r + 1.0 / 120.0; p = p * r + 1.0 / 24.0; p = p * r + 1.0 / 6.0; p = p * r + 0.5;
+  const double em1 = r + (r_err + r * r * p);
+  return __builtin_ldexp(1.0 + em1, (int)k);
+}
+
 // ---- limiters — Lib_Limiters.f90:26-112 ----------------------------------
 template <typename T> __device__ __forceinline__ T limiter(int id, T x, T y) {
   const T tiny = T(1e-30);

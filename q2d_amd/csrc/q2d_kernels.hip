@@ -94,7 +94,7 @@ struct DevCtrl {
   double dtglobal, time;
   double residual[4];
   int iter, iter_from_call, todo, active;
-  int cur, status, stage_count, dtsel;
+  int cur, status, pad_ctrl, dtsel;
   double res_first;                     // residual(1) of the case's first step
 };
 
@@ -195,9 +195,13 @@ __device__ __forceinline__ void load_geo(RowGeo& g, long long o, GP dlx, GP fix,
   (void)m11; (void)want_m;
 }
 
+// Order key of a failure: RK stage (1..3), then kind (0 interface state, 1 new state), block (1..255),
+// row, column (1..60000), compared as one integer.  A failing case is frozen at the end of its step
+// (k_finish), so only failures within one step are ever compared and the stage within the step orders
+// them; create / add_cases reject blocks and cases beyond the field widths.
 __device__ __forceinline__ unsigned long long fail_key(int stage, int kind, int b, int j, int i) {
-  return ((unsigned long long)stage << 42) | ((unsigned long long)kind << 41) | ((unsigned long long)(b & 0xff) << 32) |
-         ((unsigned long long)(j & 0xffff) << 16) | (unsigned long long)(i & 0xffff);
+  return ((unsigned long long)(stage & 0x3f) << 42) | ((unsigned long long)kind << 41) |
+         ((unsigned long long)(b & 0xff) << 32) | ((unsigned long long)(j & 0xffff) << 16) | (unsigned long long)(i & 0xffff);
 }
 
 __device__ __forceinline__ void record_fail(DevCtrl* c, int code, unsigned long long key) {
@@ -604,7 +608,7 @@ __global__ __launch_bounds__(256) void k_bcflux(const DevBlock* __restrict__ blo
       R[v] = (FT)p3 - (bt * sr) * (FT)Dl3;
     }
     const bool bad = !(L[0] > FT(0) && L[3] > FT(0) && R[0] > FT(0) && R[3] > FT(0));
-    if (bad) record_fail(&ctrl[E.cid], Q2D_E_INTERFACE, fail_key(K.stage_count + irk, 0, B.blk_in_case, Jm, Im));
+    if (bad) record_fail(&ctrl[E.cid], Q2D_E_INTERFACE, fail_key(irk, 0, B.blk_in_case, Jm, Im));
     FT F[4];
     riemann_rt<FT>(rid, gf, L, R, (FT)n1, (FT)n2, F, brie);
     // one species: Sel_L*rho_L/rho_L - Sel_R*rho_R/rho_R == 1, so Flux(1) = F_r*int_l
@@ -647,12 +651,12 @@ __global__ __launch_bounds__(256) void k_bcflux(const DevBlock* __restrict__ blo
         double p_exit = E.p;
         const double delta = 0.5 * (G - 1.0);
         const double ex = G / (G - 1.0);
-        const double den = 1.0 / pow(1.0 + delta, ex);
-        const double pstar = Bp[3] * den * pow(1.0 + delta * (Mach_N * Mach_N), ex);
+        const double den = 1.0 / exact_pow(1.0 + delta, ex);
+        const double pstar = Bp[3] * den * exact_pow(1.0 + delta * (Mach_N * Mach_N), ex);
         if (pstar >= p_exit) p_exit = pstar;
         const double Isen = (G - 1.0) / (2.0 * G);
         Fp[3] = p_exit * E.rel_fac + Bp[3] * (1.0 - E.rel_fac);
-        const double Face_Sound = Bound_Sound * pow(Fp[3] / Bp[3], Isen);
+        const double Face_Sound = Bound_Sound * exact_pow(Fp[3] / Bp[3], Isen);
         Face_Un = (Riem - Face_Sound) / (sgn * 0.5 * (G - 1.0));
         Face_rho = G * Fp[3] / (Face_Sound * Face_Sound);
         Fp[0] = Bp[0] / Bp[0] * Face_rho;
@@ -1234,7 +1238,7 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
       const double c_rk = (C.n_rk == 1) ? 1.0 : (C.n_rk == 2 ? (irk == 1 ? 1.0 : 0.5)
                                                               : (irk == 1 ? 1.0 : (irk == 2 ? 0.25 : 2.0 / 3.0)));
       const double dtg = K.dtglobal;
-      const int stage_id = K.stage_count + irk;
+      const int stage_id = irk;
 
       const int x = item.x0 - 2 + lane;                  // Fortran column of this lane
       const int xc = x <= ni + 2 ? x : ni + 2;           // clamped column for loads
@@ -1718,7 +1722,7 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MX_MIN_WAVES) void k_stage_mx(const 
       const double c_rk = (C.n_rk == 1) ? 1.0 : (C.n_rk == 2 ? (irk == 1 ? 1.0 : 0.5)
                                                               : (irk == 1 ? 1.0 : (irk == 2 ? 0.25 : 2.0 / 3.0)));
       const double dtg = K.dtglobal;
-      const int stage_id = K.stage_count + irk;
+      const int stage_id = irk;
 
       const int x = item.x0 - 2 + lane;
       const int xc = x <= ni + 2 ? x : ni + 2;
@@ -2153,7 +2157,7 @@ __global__ __launch_bounds__(256) void k_irs(const DevBlock* __restrict__ blocks
     const Gas<double> gd{C.R, C.cp, C.cv, C.gam};
     const double c_rk = (C.n_rk == 1) ? 1.0 : (C.n_rk == 2 ? (irk == 1 ? 1.0 : 0.5)
                                                             : (irk == 1 ? 1.0 : (irk == 2 ? 0.25 : 2.0 / 3.0)));
-    const int stage_id = K.stage_count + irk;
+    const int stage_id = irk;
     const int x = item.x0 - 2 + lane;                  // lanes 0..63: columns x0-2 .. x0+61
     const int xc = min(max(x, 0), ni + 1);
     const bool own = lane >= 2 && lane < 2 + kOwn && x <= ni;
@@ -2386,7 +2390,6 @@ __global__ __launch_bounds__(kFinishThreads) void k_finish(DevCtrl* ctrl, const 
                         (C.res_drop > 0.0 && K.iter > 1 && K.residual[0] <= C.res_drop * K.res_first);
     K.todo = endsim ? 3 : 1;
     K.cur = (K.cur + 1 + ((C.n_rk - 1) & 1)) % 3;          // buffer written by the last stage
-    K.stage_count += 4;
     if (endsim || K.status != 0) K.active = 0;
   }
 }
@@ -2732,6 +2735,12 @@ int collect_timing(q2d_handle* h) {
   return reset_timing(h);
 }
 
+// Steps of this handle evaluate the next step's dt in their last stage (Euler cases on the fast stage
+// kernels); otherwise every step runs the dt pass.
+bool fused_handle(const q2d_handle* h) {
+  return !h->any_visc && !h->mixed_generic && (h->precision_mixed ? kFuseDtMx : kFuseDtFp64);
+}
+
 int launch_step(q2d_handle* h) {
   if (h->ncases == 0) return Q2D_OK;
   h->tslot = 0;
@@ -2742,8 +2751,7 @@ int launch_step(q2d_handle* h) {
   groups = std::max(groups, 1);
   const int cthreads = 64, cgroups = (h->ncases + cthreads - 1) / cthreads;
   const int bgroups = (h->nbc + 63) / 64;            // one wavefront per group: spread over every CU
-  const int use_next =
-      (h->dt_fused_valid && !h->any_visc && !h->mixed_generic && (h->precision_mixed ? kFuseDtMx : kFuseDtFp64)) ? 1 : 0;
+  const int use_next = (h->dt_fused_valid && fused_handle(h)) ? 1 : 0;
   // with the dt already reduced, the first boundary launch also does k_begin's work (one launch fewer)
   const bool fold_begin = use_next && h->nbc > 0;
   if (!fold_begin)
@@ -2788,11 +2796,13 @@ int launch_step(q2d_handle* h) {
                          h->d_items, n_items, n_items + n_irs, h->d_bc, h->d_bcface, h->d_partials, irk);
     }
   }
-  const int fused_dt = (!h->any_visc && !h->mixed_generic && (h->precision_mixed ? kFuseDtMx : kFuseDtFp64)) ? 1 : 0;
+  const int fused_dt = fused_handle(h) ? 1 : 0;
   hipLaunchKernelGGL(k_finish, dim3(h->ncases), dim3(kFinishThreads), 0, h->stream, h->d_ctrl, h->d_cases, h->d_partials, h->ncases,
                      h->timing ? h->d_tstamp : nullptr, fused_dt);
   HIPCHK(hipGetLastError());
-  h->dt_fused_valid = true;
+  // the next step may take its dt from this step's last stage only if this step fused it (a handle that
+  // loses its last laminar or generic-kernel case switches to fused steps with a dt pass first)
+  h->dt_fused_valid = fused_dt != 0;
   return Q2D_OK;
 }
 
@@ -3319,7 +3329,7 @@ int q2d_gpu_run(q2d_handle* h, int32_t nsteps, q2d_step_out* out) {
   if (nsteps < 0) return fail(Q2D_E_ARG, "nsteps < 0");
   if (h->ncases == 0) return Q2D_OK;
   if (nsteps > 0) {
-    if (!h->dt_fused_valid) {                          // first step after create/upload: eager, with k_dt
+    if (fused_handle(h) && !h->dt_fused_valid) {      // first step after create/upload: eager, with k_dt
       int rc = launch_step(h);
       if (rc == Q2D_OK) rc = count_timed_step(h);
       if (rc != Q2D_OK) return rc;

@@ -24,13 +24,20 @@ from . import cases
 from .capi import GpuSolver
 
 
+def block_planes(spec):
+    """FP64 planes of one block of a case, as q2d_gpu_add_cases allocates them: 30 for an FP64 Euler block,
+    37 for a mixed-form Euler block (its FP32 geometry), 39 for a laminar block, plus 4 with residual
+    smoothing (the stage's scaled residual)."""
+    planes = 39 if spec.model > 0 else (37 if spec.precision == "mixed" else 30)
+    return planes + (4 if spec.irs_beta > 0 else 0)
+
+
 def footprint_bytes(problem):
-    """Device bytes the blocks of `problem` occupy (SoA planes, 30 per Euler block, 39 per laminar block)."""
+    """Device bytes the blocks of `problem` occupy: planes of (nj + 4) rows of round16(ni + 4) doubles."""
     total = 0
     for b in problem.blocks:
-        planes = 39 if problem.cases[b.case_id].model > 0 else 30
         pitch = (b.ni + 4 + 15) // 16 * 16
-        total += 8 * planes * pitch * (b.nj + 4)
+        total += 8 * block_planes(problem.cases[b.case_id]) * pitch * (b.nj + 4)
     return total
 
 
@@ -144,13 +151,18 @@ class Server:
                             f"done at {tnow:.2f} s, {o['iter']} steps, latency {tnow - r.arrival:.2f} s")
         makespan = clock() - t0
         results.sort(key=lambda x: x.rid)
-        lat = np.array([x.latency for x in results])
+        # time-to-converged percentiles are over the requests that converged; a request retired at its
+        # iteration cap (or failed) is counted separately and does not enter them
+        lat = np.array([x.latency for x in results if x.converged])
+        lat_all = np.array([x.latency for x in results])
         stats = {
             "requests": len(results),
             "converged": int(sum(x.converged for x in results)),
+            "stalled": [x.rid for x in results if not x.converged],
             "p50_s": float(np.percentile(lat, 50)) if len(lat) else math.nan,
             "p99_s": float(np.percentile(lat, 99)) if len(lat) else math.nan,
             "mean_s": float(lat.mean()) if len(lat) else math.nan,
+            "p99_all_s": float(np.percentile(lat_all, 99)) if len(lat_all) else math.nan,
             "makespan_s": makespan,
             "cell_updates": int(cell_updates),
             "cell_updates_per_s": cell_updates / makespan if makespan > 0 else 0.0,

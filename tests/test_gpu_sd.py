@@ -25,11 +25,13 @@ def run_both(oracle_mod, pb, steps):
 
 
 def test_sd_ramp_reference_setting(oracle_mod):
-    """test/2D/euler/ramp-channel/input.ini: MUSCL-SD, minmod, HLLE, SSP-RK3, local dt, CFL 0.9, irs-beta 0.5."""
-    pb = cases.ramp_channel(ni=96, nj=48, wedge_deg=12.0, mach=2.5, cfl=0.9, itermax=80)
+    """test/2D/euler/ramp-channel/input.ini: MUSCL-SD, minmod, HLLE, SSP-RK3, local dt, CFL 0.9,
+    cfl-rise-threshold 100, irs-beta 0.5; run past the end of the CFL ramp."""
+    pb = cases.ramp_channel(ni=96, nj=48, wedge_deg=12.0, mach=2.5, cfl=0.9, itermax=120)
     pb.cases[0].space_sd = True
     pb.cases[0].irs_beta = 0.5
-    g, o = run_both(oracle_mod, pb, 80)
+    pb.cases[0].ramp_iter = 100
+    g, o = run_both(oracle_mod, pb, 120)
     assert np.array_equal(g.dtlocal(0), o.dtlocal(0))
 
 

@@ -68,3 +68,50 @@ def test_served_results_equal_alone():
         assert rs[0].iters == x.iters and rs[0].status == x.status == 0
         for a, b in zip(rs[0].state, x.state):
             assert np.array_equal(a, b)
+
+
+def test_removing_unfused_cases_keeps_the_survivors_dt():
+    """A handle whose last laminar case (FP64) or last generic-kernel case (mixed: HLLEM) is removed switches
+    to steps that fuse the next step's dt: the first such step must run the dt pass, not take a dt that no
+    stage reduced.  The survivor matches its solo run (FP64: bit for bit; mixed: the generic kernel ran its
+    first steps, so within the form's tolerance)."""
+    for prec in ("fp64", "mixed"):
+        a = cases.riemann_planar(seed=5, ni=200, nj=8, steps=10 ** 6, precision=prec)
+        others = [cases.axisymmetric_pipe(seed=1, ni=64, nj=24, first_cell=5e-4, reynolds=1e4, precision=prec,
+                                          steps=10 ** 6)]
+        if prec == "mixed":
+            others.append(cases.ramp_channel(ni=64, nj=32, riemann="HLLEM", precision=prec, itermax=10 ** 6))
+        alone = GpuSolver(a)
+        solo = alone.run(20)[0]
+        g = GpuSolver(cases.Problem())
+        ca, _ = g.add_cases(a)
+        ids = [g.add_cases(p)[0] for p in others]
+        g.run(10)
+        for c in ids:
+            g.remove_case(c)
+        out = g.run(10)[ca]
+        assert out["iter"] == 20
+        if prec == "fp64":
+            assert out["dtglobal"] == solo["dtglobal"] and out["time"] == solo["time"]
+            assert np.array_equal(g.state(0)[2:-2, 2:-2], alone.state(0)[2:-2, 2:-2])
+        else:
+            assert abs(out["time"] - solo["time"]) <= 1e-6 * solo["time"]
+            assert abs(out["dtglobal"] - solo["dtglobal"]) <= 1e-4 * solo["dtglobal"]
+            x, y = g.state(0)[2:-2, 2:-2, 0], alone.state(0)[2:-2, 2:-2, 0]
+            assert np.abs(x - y).sum() / np.abs(y).sum() <= 1e-5
+
+
+def test_footprint_equals_device_bytes():
+    """The serving loop's HBM accounting equals what add_cases allocates (FP64, mixed, IRS, laminar)."""
+    for prec in ("fp64", "mixed"):
+        g = GpuSolver(cases.Problem())
+        for pb in (cases.ramp_channel(ni=72, nj=40, precision=prec),
+                   cases.axisymmetric_pipe(seed=1, ni=64, nj=24, first_cell=5e-4, reynolds=1e4, precision=prec),
+                   cases.planar_shock(seed=1, n=40, precision=prec)):
+            for irs in (0.0, 0.5):
+                for c in pb.cases:
+                    c.irs_beta = irs
+                before = g.device_bytes()
+                g.add_cases(pb)
+                assert g.device_bytes() - before == serving.footprint_bytes(pb), (prec, pb.meta.get("name"), irs)
+        g.close()
