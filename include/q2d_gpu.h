@@ -217,6 +217,19 @@ int q2d_gpu_case_count(q2d_handle *h, int32_t *ncases, int32_t *nblocks, int64_t
  * (may be NULL).  Synchronises with the device. */
 int q2d_gpu_step(q2d_handle *h, q2d_step_out *out);
 
+/* Allocation-free step (no counterpart in the reference): exactly what q2d_gpu_step does, for callers
+ * that cannot allocate device memory while they run (serving loops and harnesses that forbid
+ * hipMalloc).  It calls no hipMalloc, hipFree or allocating library routine and captures no graph: the
+ * step's scratch (per-work-item residual-norm and dt partials, per-record boundary fluxes, the mixed
+ * form's FP32 per-face array) is taken from one file-scope device array of Q2D_NOALLOC_SCRATCH_BYTES
+ * that each GPU holds for the life of the process, instead of the handle's own.  Returns Q2D_E_NOMEM,
+ * computing nothing, when the handle's scratch exceeds that size, and Q2D_E_ARG with kernel timing on or
+ * another device current.  Every call on one GPU shares that GPU's array: calls that overlap on one GPU
+ * race.  Each step writes every part of the scratch it reads, so nothing carries over between calls;
+ * q2d_gpu_step, q2d_gpu_run and this entry point may be mixed on one handle. */
+#define Q2D_NOALLOC_SCRATCH_BYTES (32u << 20)
+int q2d_gpu_step_noalloc(q2d_handle *h, q2d_step_out *out);
+
 /* nsteps explicit steps without host synchronisation in between (replays of
  * captured HIP graphs of eight steps and of one step): the host's time loop around Explicit_Step
  * (main.f90:48 -> Wrap_Solve.f90:24) with its TODO test (Mod_Explicit.f90:

@@ -138,6 +138,7 @@ SIGNATURES = {
     "q2d_gpu_case_count": ([C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int64)], C.c_int),
     "q2d_gpu_step": ([C.c_void_p, C.POINTER(StepOut)], C.c_int),
     "q2d_gpu_run": ([C.c_void_p, C.c_int32, C.POINTER(StepOut)], C.c_int),
+    "q2d_gpu_step_noalloc": ([C.c_void_p, C.POINTER(StepOut)], C.c_int),
     "q2d_gpu_download_state": ([C.c_void_p, C.c_int32, C.c_void_p], C.c_int),
     "q2d_gpu_upload_state": ([C.c_void_p, C.c_int32, C.c_void_p], C.c_int),
     "q2d_gpu_download_dt": ([C.c_void_p, C.c_int32, C.c_void_p], C.c_int),
@@ -231,6 +232,13 @@ class GpuSolver:
 
     def step(self, raise_on_error=True):
         rc = self.lib.q2d_gpu_step(self.h, self._out)
+        if raise_on_error:
+            _check(self.lib, rc)
+        return [o.as_dict() for o in self._out[:self.ncases]]
+
+    def step_noalloc(self, raise_on_error=True):
+        """One step through the allocation-free entry point (file-scope device scratch)."""
+        rc = self.lib.q2d_gpu_step_noalloc(self.h, self._out)
         if raise_on_error:
             _check(self.lib, rc)
         return [o.as_dict() for o in self._out[:self.ncases]]

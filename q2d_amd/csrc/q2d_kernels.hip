@@ -26,6 +26,7 @@
 #include <cstdio>
 #include <cstring>
 #include <functional>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <utility>
@@ -35,6 +36,11 @@
 #include "q2d_device.h"
 
 using namespace q2d;
+
+// Step scratch of q2d_gpu_step_noalloc: a file-scope device array, so that entry point allocates nothing.
+// Each GPU the library runs on holds its own copy, at its own address, until the process ends; host code
+// uses only the address hipGetSymbolAddress returns for the current device (never the array's name).
+static __device__ __align__(256) unsigned char scratch[Q2D_NOALLOC_SCRATCH_BYTES];
 
 namespace {
 
@@ -2474,6 +2480,8 @@ struct q2d_handle {
   long long timed_launches = 0;
   double timed_ms = 0.0;
   int cu_count = 256;
+  int device = 0;                // the device current at q2d_gpu_create
+  unsigned char* noalloc_scratch = nullptr;   // that device's address of `scratch`
 };
 
 namespace {
@@ -2656,11 +2664,18 @@ int build_items(q2d_handle* h) {
   return Q2D_OK;
 }
 
+// Scratch of one step's kernels: the per-item residual-norm and dt partials (k_stage*, k_irs -> k_finish),
+// the per-record boundary fluxes (k_bcflux / k_bcvisc -> k_stage) and, for the mixed form, the dense FP32
+// per-face array (k_bcflux -> k_stage_mx; null for FP64 handles).  Every step writes each of them before
+// it reads it.
+struct StepScratch { double* partials; BCFlux* bcflux; float4* dense; };
+StepScratch own_scratch(const q2d_handle* h) { return StepScratch{h->d_partials, h->d_bcflux, h->d_bcdense}; }
+
 // With timing on, each stage launch is bracketed by an event pair on the solver's stream.  The timed steps
 // replay a graph captured with these event nodes, so the stage kernels run back to back exactly as in
 // the untimed graph; only the event nodes are added.
 template <typename FT, bool FUSE>
-int launch_stage(q2d_handle* h, int irk) {
+int launch_stage(q2d_handle* h, int irk, const StepScratch& scr) {
   const int tpb = kWave * h->waves_per_group;
   for (int pass = 0; pass < 4; ++pass) {
     // the two Riemann solvers' passes of each kernel family (Euler 0-1, laminar 2-3) share one launch
@@ -2677,14 +2692,14 @@ int launch_stage(q2d_handle* h, int irk) {
       ts = h->d_tstamp; slot = h->tslot++;
     }
     const int* bf = h->d_bcface;
-    const BCFlux* bq = h->d_bcflux;
+    const BCFlux* bq = scr.bcflux;
 #define Q2D_STAGE_LAUNCH(RIE, VISC)                                                                               \
     hipLaunchKernelGGL((k_stage<FT, RIE, FUSE, VISC>), dim3(groups), dim3(tpb), 0, h->stream, h->d_blocks,        \
-                       h->d_cases, h->d_ctrl, h->d_items, lo, hi, bf, bq, h->d_partials, irk, h->xcd_remap, ts,  \
+                       h->d_cases, h->d_ctrl, h->d_items, lo, hi, bf, bq, scr.partials, irk, h->xcd_remap, ts,   \
                        slot)
 #define Q2D_STAGE_MX_LAUNCH(RIE, IRS)                                                                             \
     hipLaunchKernelGGL((k_stage_mx<RIE, kFuseDtMx, IRS>), dim3(groups), dim3(tpb), 0, h->stream, h->d_blocks,      \
-                       h->d_cases, h->d_ctrl, h->d_items, lo, hi, h->d_bcdense, h->d_partials, irk, h->xcd_remap, ts, \
+                       h->d_cases, h->d_ctrl, h->d_items, lo, hi, scr.dense, scr.partials, irk, h->xcd_remap, ts,     \
                        slot)
     if constexpr (sizeof(FT) == 4) {                   // mixed form: Euler items run k_stage_mx
       if (pass == 0 && !h->mixed_generic && h->any_irs) Q2D_STAGE_MX_LAUNCH(-1, true);
@@ -2741,7 +2756,7 @@ bool fused_handle(const q2d_handle* h) {
   return !h->any_visc && !h->mixed_generic && (h->precision_mixed ? kFuseDtMx : kFuseDtFp64);
 }
 
-int launch_step(q2d_handle* h) {
+int launch_step(q2d_handle* h, const StepScratch& scr) {
   if (h->ncases == 0) return Q2D_OK;
   h->tslot = 0;
   const int n_items = (int)h->items.size();
@@ -2770,34 +2785,34 @@ int launch_step(q2d_handle* h) {
         const int begin_cases = (fold_begin && irk == 1 && ph <= 1) ? h->ncases : 0;
         if (h->precision_mixed)                        // the dense FP32 face array feeds k_stage_mx only
           hipLaunchKernelGGL(k_bcflux<float>, dim3(bgroups), dim3(64), 0, h->stream, h->d_blocks, h->d_cases,
-                             h->d_ctrl, h->d_bc, h->nbc, irk, h->d_bcflux, ph, begin_cases,
-                             h->mixed_generic ? nullptr : h->d_bcdense);
+                             h->d_ctrl, h->d_bc, h->nbc, irk, scr.bcflux, ph, begin_cases,
+                             h->mixed_generic ? nullptr : scr.dense);
         else
           hipLaunchKernelGGL(k_bcflux<double>, dim3(bgroups), dim3(64), 0, h->stream, h->d_blocks, h->d_cases,
-                             h->d_ctrl, h->d_bc, h->nbc, irk, h->d_bcflux, ph, begin_cases, h->d_bcdense);
+                             h->d_ctrl, h->d_bc, h->nbc, irk, scr.bcflux, ph, begin_cases, scr.dense);
       }
     }
     if (!h->vrec.empty())
       hipLaunchKernelGGL(k_bcvisc, dim3(((int)h->vrec.size() + 255) / 256), dim3(256), 0, h->stream, h->d_blocks,
-                         h->d_cases, h->d_ctrl, h->d_bc, h->d_vrec, (int)h->vrec.size(), irk, h->d_bcflux);
+                         h->d_cases, h->d_ctrl, h->d_bc, h->d_vrec, (int)h->vrec.size(), irk, scr.bcflux);
     if (h->range[4] > h->range[2]) {
       int vg = (h->range[4] - h->range[2] + h->waves_per_group - 1) / h->waves_per_group;
       if (h->max_groups > 0) vg = std::min(vg, h->max_groups);
       hipLaunchKernelGGL(k_visc, dim3(std::max(vg, 1)), dim3(tpb), 0, h->stream, h->d_blocks, h->d_cases, h->d_ctrl,
                          h->d_items, h->range[2], h->range[4], h->d_bc, h->d_bcface, irk);
     }
-    int rc = h->precision_mixed ? launch_stage<float, false>(h, irk) : launch_stage<double, kFuseDtFp64>(h, irk);
+    int rc = h->precision_mixed ? launch_stage<float, false>(h, irk, scr) : launch_stage<double, kFuseDtFp64>(h, irk, scr);
     if (rc != Q2D_OK) return rc;
     if (h->any_irs && !h->irs_items.empty()) {        // residual smoothing along i and j, then the update
       const int n_irs = (int)h->irs_items.size();
       int ig = (n_irs + h->waves_per_group - 1) / h->waves_per_group;
       if (h->max_groups > 0) ig = std::min(ig, h->max_groups);
       hipLaunchKernelGGL(k_irs, dim3(std::max(ig, 1)), dim3(tpb), 0, h->stream, h->d_blocks, h->d_cases, h->d_ctrl,
-                         h->d_items, n_items, n_items + n_irs, h->d_bc, h->d_bcface, h->d_partials, irk);
+                         h->d_items, n_items, n_items + n_irs, h->d_bc, h->d_bcface, scr.partials, irk);
     }
   }
   const int fused_dt = fused_handle(h) ? 1 : 0;
-  hipLaunchKernelGGL(k_finish, dim3(h->ncases), dim3(kFinishThreads), 0, h->stream, h->d_ctrl, h->d_cases, h->d_partials, h->ncases,
+  hipLaunchKernelGGL(k_finish, dim3(h->ncases), dim3(kFinishThreads), 0, h->stream, h->d_ctrl, h->d_cases, scr.partials, h->ncases,
                      h->timing ? h->d_tstamp : nullptr, fused_dt);
   HIPCHK(hipGetLastError());
   // the next step may take its dt from this step's last stage only if this step fused it (a handle that
@@ -2809,12 +2824,15 @@ int launch_step(q2d_handle* h) {
 int read_out(q2d_handle* h, q2d_step_out* out, int* first_err) {
   *first_err = Q2D_OK;
   if (h->ncases == 0) return Q2D_OK;
-  std::vector<DevCtrl> ctl(h->ncases);
-  HIPCHK(hipMemcpyAsync(ctl.data(), h->d_ctrl, sizeof(DevCtrl) * h->ncases, hipMemcpyDeviceToHost, h->stream));
-  HIPCHK(hipStreamSynchronize(h->stream));
-  *first_err = Q2D_OK;
+  constexpr int kChunk = 64;                         // control words read in chunks of a stack buffer
+  DevCtrl ctl[kChunk];
   for (int c = 0; c < h->ncases; ++c) {
-    DevCtrl& k = ctl[c];
+    if (c % kChunk == 0) {
+      const int n = std::min(kChunk, h->ncases - c);
+      HIPCHK(hipMemcpyAsync(ctl, h->d_ctrl + c, sizeof(DevCtrl) * n, hipMemcpyDeviceToHost, h->stream));
+      HIPCHK(hipStreamSynchronize(h->stream));
+    }
+    DevCtrl& k = ctl[c % kChunk];
     // The reported failure is the first in the reference's order (stage, then fluxes before the new
     // state, then block, row, column): the minimum key; its kind bit names the stop it maps to.
     if (k.status != 0) k.status = ((k.fail_key >> 41) & 1ull) ? Q2D_E_STATE : Q2D_E_INTERFACE;
@@ -3220,6 +3238,27 @@ int append_cases(q2d_handle* h, const q2d_problem* prob, int32_t* first_case) {
 
 }  // namespace
 
+namespace {
+// Device address of `scratch` for each device, looked up once per device with that device current.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_addr[kMaxDevices] = {};
+
+int noalloc_scratch_address(int dev, unsigned char** out) {
+  if (dev < 0 || dev >= kMaxDevices) return fail(Q2D_E_HIP, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_addr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(Q2D_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_addr[dev] = static_cast<unsigned char*>(p);
+  }
+  *out = g_scratch_addr[dev];
+  return Q2D_OK;
+}
+}  // namespace
+
 int q2d_gpu_create(const q2d_problem* prob, q2d_handle** out) {
   if (!prob || !out) return fail(Q2D_E_ARG, "NULL argument");
   *out = nullptr;
@@ -3227,6 +3266,11 @@ int q2d_gpu_create(const q2d_problem* prob, q2d_handle** out) {
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(Q2D_E_NODEVICE, "no HIP device");
   q2d_handle* h = new q2d_handle();
   auto bail = [&](int code) { q2d_gpu_destroy(h); return code; };
+  if (hipGetDevice(&h->device) != hipSuccess) return bail(fail(Q2D_E_HIP, "hipGetDevice failed"));
+  {
+    const int rc = noalloc_scratch_address(h->device, &h->noalloc_scratch);
+    if (rc != Q2D_OK) return bail(rc);
+  }
   {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, 0) == hipSuccess) h->cu_count = prop.multiProcessorCount;
@@ -3313,7 +3357,7 @@ int count_timed_step(q2d_handle* h) {
 
 int q2d_gpu_step(q2d_handle* h, q2d_step_out* out) {
   if (!h) return fail(Q2D_E_ARG, "NULL handle");
-  int rc = launch_step(h);
+  int rc = launch_step(h, own_scratch(h));
   if (rc == Q2D_OK) rc = count_timed_step(h);
   if (rc == Q2D_OK && h->timing) rc = collect_timing(h);
   if (rc != Q2D_OK) return rc;
@@ -3330,7 +3374,7 @@ int q2d_gpu_run(q2d_handle* h, int32_t nsteps, q2d_step_out* out) {
   if (h->ncases == 0) return Q2D_OK;
   if (nsteps > 0) {
     if (fused_handle(h) && !h->dt_fused_valid) {      // first step after create/upload: eager, with k_dt
-      int rc = launch_step(h);
+      int rc = launch_step(h, own_scratch(h));
       if (rc == Q2D_OK) rc = count_timed_step(h);
       if (rc != Q2D_OK) return rc;
       --nsteps;
@@ -3342,7 +3386,7 @@ int q2d_gpu_run(q2d_handle* h, int32_t nsteps, q2d_step_out* out) {
       hipGraph_t g;
       HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
       int rc = Q2D_OK;
-      for (int k = 0; k < nst && rc == Q2D_OK; ++k) rc = launch_step(h);
+      for (int k = 0; k < nst && rc == Q2D_OK; ++k) rc = launch_step(h, own_scratch(h));
       hipError_t ce = hipStreamEndCapture(h->stream, &g);
       if (rc != Q2D_OK) { if (ce == hipSuccess) (void)hipGraphDestroy(g); return rc; }
       if (ce != hipSuccess) return fail(Q2D_E_HIP, std::string("graph capture: ") + hipGetErrorString(ce));
@@ -3371,6 +3415,35 @@ int q2d_gpu_run(q2d_handle* h, int32_t nsteps, q2d_step_out* out) {
   }
   int err = Q2D_OK;
   int rc = read_out(h, out, &err);
+  if (rc != Q2D_OK) return rc;
+  if (err != Q2D_OK) return fail(err, err == Q2D_E_STATE ? "NaN or p<0 detected" : "Non-physical state detected at the interface");
+  return Q2D_OK;
+}
+
+int q2d_gpu_step_noalloc(q2d_handle* h, q2d_step_out* out) {
+  if (!h) return fail(Q2D_E_ARG, "NULL handle");
+  if (h->timing) return fail(Q2D_E_ARG, "q2d_gpu_step_noalloc records no kernel times: turn timing off");
+  int dev = -1;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev != h->device) return fail(Q2D_E_ARG, "the handle's device must be current");
+  // the handle's step scratch laid out in the file-scope array, each part on a 256-byte boundary
+  auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+  const size_t n_part = sizeof(double) * kPartials * std::max<size_t>(1, h->items.size() + h->irs_items.size());
+  const size_t n_bc = sizeof(BCFlux) * std::max<size_t>(1, h->dbc.size());
+  // the dense FP32 array exists for mixed-form handles only: a non-null one tells k_bcflux that the
+  // stage kernel reads it instead of the per-record results
+  const size_t n_dense = h->precision_mixed ? sizeof(float4) * std::max<size_t>(1, h->bcface.size()) : 0;
+  const size_t need = up(n_part) + up(n_bc) + up(n_dense);
+  if (need > (size_t)Q2D_NOALLOC_SCRATCH_BYTES)
+    return fail(Q2D_E_NOMEM, "step scratch of " + std::to_string(need) + " bytes exceeds the allocation-free maximum of " +
+                                 std::to_string((size_t)Q2D_NOALLOC_SCRATCH_BYTES) + " bytes");
+  unsigned char* base = h->noalloc_scratch;
+  const StepScratch scr{reinterpret_cast<double*>(base), reinterpret_cast<BCFlux*>(base + up(n_part)),
+                        h->precision_mixed ? reinterpret_cast<float4*>(base + up(n_part) + up(n_bc)) : nullptr};
+  int rc = launch_step(h, scr);
+  if (rc != Q2D_OK) return rc;
+  int err = Q2D_OK;
+  rc = read_out(h, out, &err);
   if (rc != Q2D_OK) return rc;
   if (err != Q2D_OK) return fail(err, err == Q2D_E_STATE ? "NaN or p<0 detected" : "Non-physical state detected at the interface");
   return Q2D_OK;
