@@ -10,7 +10,10 @@ BC 400; block ghosts exchanged on the device.  Both arithmetic forms run from th
 step, all RK stages); the plain form, the speed-up and the L1/Linf density error of the mixed form
 against the plain FP64 form are reported beside it.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 1024] [--no-cpu]
+The same line carries C3 (laminar axisymmetric 2048x512) and C4 (64 uneven cases) under "configs",
+the serving stream (C5) under "serving", and the CPU oracle's rate on the host under "cpu_baseline".
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 1024] [--no-cpu] [--no-configs] [--no-serve]
 """
 import argparse
 import json
@@ -59,9 +62,16 @@ def cpu_info():
     return "unknown"
 
 
-def time_form(problem_fn, precision, steps, warmup, tune=False):
+def with_precision(pb, precision):
+    """The same problem (shared block arrays) in another arithmetic form."""
+    from dataclasses import replace
+    return replace(pb, cases=[replace(c, precision=precision) for c in pb.cases])
+
+
+def time_form(pb, steps, warmup, tune=False):
+    """Time `steps` graph-replayed steps of `pb` after `warmup` steps; every case must still be running at
+    the end (status 0, iteration warmup + steps), or the throughput would count frozen cases."""
     from q2d_amd import GpuSolver
-    pb = problem_fn(precision)
     g = GpuSolver(pb)
     ncells = pb.ncells()
     if tune:
@@ -77,7 +87,9 @@ def time_form(problem_fn, precision, steps, warmup, tune=False):
     t1 = time.perf_counter()
     nlaunch, ms = g.timing_stats()
     g.set_timing(False)
-    rho = [g.state(b)[2:-2, 2:-2, 0].copy() for b in range(len(pb.blocks))]
+    bad = [k for k, o in enumerate(out) if o["status"] != 0 or o["iter"] != warmup + steps]
+    if bad:
+        raise RuntimeError(f"timed cases stopped early or failed: {[(k, out[k]['iter'], out[k]['status']) for k in bad[:5]]}")
     n_rk = pb.cases[0].n_rk
     sec = t1 - t0
     res = {
@@ -92,61 +104,145 @@ def time_form(problem_fn, precision, steps, warmup, tune=False):
         "work_items": g.info()[1],
     }
     g.close()
-    return res, rho
+    return res
 
 
-def cpu_baseline(problem_fn, steps):
+def accuracy(pb64, pbmx, steps):
+    """Relative L1 / Linf density error of the mixed form against the plain FP64 form after the workload's
+    own step count (its own runs, untimed, from the initial state)."""
+    from q2d_amd import GpuSolver
+    rho = []
+    for pb in (pb64, pbmx):
+        g = GpuSolver(pb)
+        out = g.run(steps)
+        assert all(o["status"] == 0 and o["iter"] == steps for o in out), "accuracy run stopped early"
+        rho.append([g.state(b)[2:-2, 2:-2, 0].copy() for b in range(len(pb.blocks))])
+        g.close()
+    a, b = rho[1], rho[0]
+    num = sum(np.abs(x - y).sum() for x, y in zip(a, b))
+    den = sum(np.abs(y).sum() for y in b)
+    linf = max(np.abs(x - y).max() for x, y in zip(a, b)) / max(np.abs(y).max() for y in b)
+    return {"rel_L1_rho": num / den, "rel_Linf_rho": linf, "steps": steps, "tolerance_rel_L1": 1e-5}
+
+
+def physical_cores():
+    """Physical cores lscpu reports (cores per socket x sockets)."""
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        kv = dict(line.split(":", 1) for line in out.splitlines() if ":" in line)
+        return int(kv["Core(s) per socket"].strip()) * int(kv["Socket(s)"].strip())
+    except Exception:
+        return None
+
+
+def cpu_baseline(pb_c2, c2_steps):
+    """The oracle (the C restatement of the reference loop structure, OpenMP over rows and columns) on
+    the host cores: C2 (4 x 1024^2, SSP-RK3) and C1 (512 x 8, SSP-RK2, 1000 steps), on every core the
+    process may use (at most 16, the host share of one GPU) and on one thread.  Built on the host that
+    runs it with -O3 -march=native when a compiler is there (the portable -march=x86-64-v3 build
+    otherwise); wall time of the step loop only, as the reference's "Time of operation"."""
+    # the CPUs this process may use, read before the OpenMP runtime starts (with OMP_PROC_BIND set it
+    # binds the calling thread to one place, after which the affinity mask shows that place only)
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
     import oracle
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
-    pb = problem_fn("fp64")
-    o = oracle.Oracle(pb, threads=threads)
-    t0 = time.perf_counter()
-    o.run(steps, stop_on_todo=False)
-    sec = time.perf_counter() - t0
-    return {"value": pb.ncells() * steps / sec, "unit": "cell-updates/s", "cores": oracle.load().q2do_get_threads(),
-            "kind": "port",
-            "sample": f"CPU oracle (C restatement, OpenMP) on the same 4x{int(round((pb.ncells() / 4) ** 0.5))}^2 "
-                      f"workload, {steps} SSP-RK3 steps ({sec:.1f} s); CPU: {cpu_info()}"}
+    from q2d_amd import cases
+    build = "native (-O3 -march=native)"
+    try:
+        subprocess.run(["make", "-s", "-C", str(ROOT / "oracle"), "native"], check=True, capture_output=True,
+                       timeout=120)
+        lib = oracle.load(ROOT / "oracle" / "build" / "libq2d_oracle_native.so")
+    except Exception:
+        build = "portable (-O3 -march=x86-64-v3)"
+        lib = oracle.load()
+    phys = physical_cores()
+    env_cap = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or 16
+    threads = max(1, min(usable, env_cap, 16, phys or usable))
+
+    def rate(pb, steps, nthreads):
+        o = oracle.Oracle(pb, threads=nthreads, lib=lib)
+        t0 = time.perf_counter()
+        rc, _ = o.run(steps, stop_on_todo=False)
+        sec = time.perf_counter() - t0
+        o.close()
+        return pb.ncells() * steps / sec, sec
+
+    c2_all, s2a = rate(pb_c2, c2_steps, threads)
+    c2_one, s2o = rate(pb_c2, max(1, c2_steps // 10), 1)
+    c1 = cases.riemann_planar(seed=0)
+    c1_all, s1a = rate(c1, 1000, threads)
+    c1_one, s1o = rate(c1, 1000, 1)
+    return {"value": c2_all, "unit": "cell-updates/s", "cores": lib.q2do_set_threads(threads), "kind": "port",
+            "sample": f"CPU oracle on the C2 workload (4x1024^2, SSP-RK3), {c2_steps} steps on {threads} threads "
+                      f"({s2a:.1f} s); {build}; CPU: {cpu_info()}, {phys} physical cores",
+            "single_thread": {"value": c2_one, "steps": max(1, c2_steps // 10), "seconds": round(s2o, 2)},
+            "c1_512x8_rk2_1000_steps": {"threads": threads, "value": c1_all, "seconds": round(s1a, 2),
+                                        "single_thread": c1_one, "single_thread_seconds": round(s1o, 2)},
+            "physical_cores": phys, "usable_cpus": usable, "omp": "OMP_PROC_BIND=close OMP_PLACES=cores"}
+
+
+def config_line(name, pb64, steps, warmup, acc_steps, bytes_per_update, tune=False, desc=""):
+    """A configuration measured in both forms: throughput, the mixed form's error at the workload's own step
+    count, and the step's bytes against the HBM roofline."""
+    pbmx = with_precision(pb64, "mixed")
+    plain = time_form(pb64, steps, warmup, tune)
+    mixed = time_form(pbmx, steps, warmup, tune)
+    log(f"bench: {name}: plain {plain['cell_updates_per_s'] / 1e9:.3f} G, mixed {mixed['cell_updates_per_s'] / 1e9:.3f} G "
+        f"cell-updates/s")
+
+    def roof(r):
+        ach = r["cell_updates_per_s"] * bytes_per_update / 1e9
+        return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4), "scope": "whole step, algorithmic bytes",
+                "bytes_per_cell_update": bytes_per_update, "stage_kernel_ms": round(r["stage_kernel_ms"], 4)}
+    return {"workload": desc, "cells": plain["ncells"], "value": mixed["cell_updates_per_s"],
+            "unit": "cell-updates/s", "dtype": "f32 flux / f64 state", "ms_per_step": mixed["ms_per_step"],
+            "plain_fp64": {"value": plain["cell_updates_per_s"], "ms_per_step": plain["ms_per_step"],
+                           "roofline": roof(plain), "tiling": plain["tiling"]},
+            "mixed_vs_plain": round(mixed["cell_updates_per_s"] / plain["cell_updates_per_s"], 3),
+            "mixed_err_vs_fp64": accuracy(pb64, pbmx, acc_steps), "roofline": roof(mixed), "tiling": mixed["tiling"]}
+
+
+def c3_c4(args):
+    """C3 (laminar axisymmetric 2048x512, SSP-RK3, local dt; 2000 steps for the error) and C4 (64 uneven
+    planar / quasi-2D cases, SSP-RK2, autotuned tiles; 200 steps for the error)."""
+    from q2d_amd import cases
+    out = {}
+    axi = cases.axisymmetric_pipe(seed=args.seed, precision="fp64", steps=1 << 30)
+    # 3 stages x (168 + M 32 + axis r and 1/r 16) + dt pass 80 + local dt 8
+    out["c3_axisymmetric_laminar"] = config_line(
+        "C3", axi, args.steps, args.warmup, 2000, 3 * 216 + 88,
+        desc="axisymmetric laminar Navier-Stokes pipe, 2048x512, tanh wall clustering (first cell 1e-5 L), Re_L 1e5, "
+             "Sutherland, Pr 0.72, isothermal wall, axis BC, MUSCL vanleer, HLLC, SSP-RK3, local dt, CFL 0.5, VNN 0.2")
+    del axi
+    batch = cases.batch_uneven(seed=args.seed, ncases=64, steps=1 << 30, precision="fp64")
+    quasi = sum(b.ni * b.nj for b in batch.blocks if batch.cases[b.case_id].source != "none")
+    frac_q = quasi / batch.ncells()
+    # SSP-RK2: 2 stages x (168, + G 16 on quasi-2D cases) + dt pass 80
+    bpu = int(round(2 * (168 + 16 * frac_q) + 80))
+    out["c4_batch_64_uneven"] = config_line(
+        "C4", batch, args.steps, args.warmup, 200, bpu, tune=True,
+        desc="64 independent planar / quasi-2D cases, 128x64 .. 2048x1024 log-uniform, Riemann or uniform inflow, "
+             "HLL or HLLC per case, SSP-RK2, one launch per stage for all cases, autotuned tiles")
+    return out
 
 
 def other_workload(args):
-    """--workload axi | batch: one JSON line for the laminar axisymmetric case (2048x512) or the batch of
-    64 uneven planar / quasi-2D cases, both arithmetic forms from the same initial state."""
+    """--workload axi | batch: the C3 or C4 configuration alone, on its own JSON line."""
     from q2d_amd import cases
     if args.workload == "axi":
-        def problem_fn(precision):
-            return cases.axisymmetric_pipe(seed=args.seed, precision=precision, steps=1 << 30)
-        desc = ("axisymmetric laminar Navier-Stokes pipe, 2048x512, tanh wall clustering (first cell 1e-5 L), "
-                "Re_L 1e5, Sutherland, Pr 0.72, isothermal wall, axis BC, MUSCL vanleer, HLLC, SSP-RK3, local dt, "
-                "CFL 0.5, VNN 0.2")
-        tune = False
+        pb = cases.axisymmetric_pipe(seed=args.seed, precision="fp64", steps=1 << 30)
+        r = config_line("C3", pb, args.steps, args.warmup, 2000, 3 * 216 + 88, desc="C3 axisymmetric laminar 2048x512")
     else:
-        def problem_fn(precision):
-            return cases.batch_uneven(seed=args.seed, ncases=64, steps=1 << 30, precision=precision)
-        desc = ("64 independent planar / quasi-2D cases, 128x64 .. 2048x1024 log-uniform, Riemann or uniform "
-                "inflow, HLL or HLLC per case, SSP-RK2, one persistent launch per stage, autotuned tiles")
-        tune = True
-    log(f"bench: {args.workload}: warmup {args.warmup}, steps {args.steps}")
-    plain, rho64 = time_form(problem_fn, "fp64", args.steps, args.warmup, tune)
-    log(f"bench: plain fp64  {plain['cell_updates_per_s'] / 1e9:.3f} G cell-updates/s  tiling {plain['tiling']}")
-    mixed, rho32 = time_form(problem_fn, "mixed", args.steps, args.warmup, tune)
-    log(f"bench: mixed       {mixed['cell_updates_per_s'] / 1e9:.3f} G cell-updates/s  tiling {mixed['tiling']}")
-    num = sum(np.abs(a - b).sum() for a, b in zip(rho32, rho64))
-    den = sum(np.abs(b).sum() for b in rho64)
-    linf = max(np.abs(a - b).max() for a, b in zip(rho32, rho64)) / max(np.abs(b).max() for b in rho64)
-    print(json.dumps({
-        "metric": "cell-updates/s batched-mixed vs plain FP64 at stated err",
-        "value": mixed["cell_updates_per_s"], "unit": "cell-updates/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": mixed["ms_per_step"],
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32 flux / f64 state",
-        "data": "synthetic (seeded)",
-        "config": {"workload": desc, "cells": plain["ncells"], "parallelism": "single GPU"},
-        "plain_fp64": {"value": plain["cell_updates_per_s"], "ms_per_step": plain["ms_per_step"],
-                       "stage_kernel_ms": plain["stage_kernel_ms"], "tiling": plain["tiling"]},
-        "mixed_stage_kernel_ms": mixed["stage_kernel_ms"], "mixed_tiling": mixed["tiling"],
-        "mixed_vs_plain": round(mixed["cell_updates_per_s"] / plain["cell_updates_per_s"], 3),
-        "mixed_err_vs_fp64": {"rel_L1_rho": num / den, "rel_Linf_rho": linf, "steps": args.warmup + args.steps},
-    }))
+        pb = cases.batch_uneven(seed=args.seed, ncases=64, steps=1 << 30, precision="fp64")
+        r = config_line("C4", pb, args.steps, args.warmup, 200, 416, tune=True, desc="C4 64 uneven cases")
+    print(json.dumps({"metric": "cell-updates/s batched-mixed vs plain FP64 at stated err", "value": r["value"],
+                      "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+                      "ms_per_step": r["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+                      "vs_baseline": None, "dtype": "f32 flux / f64 state", "data": "synthetic (seeded)",
+                      "config": {"workload": r["workload"], "cells": r["cells"], "parallelism": "single GPU"},
+                      "detail": r}))
 
 
 def irs_workload(args):
@@ -166,7 +262,7 @@ def irs_workload(args):
     thr = {}
     for prec in ("fp64", "mixed"):
         for beta in (0.0, 0.5):
-            r, _ = time_form(lambda p, b=beta: ramp(p, b), prec, args.steps, args.warmup)
+            r = time_form(ramp(prec, beta), args.steps, args.warmup)
             thr[f"{prec}_beta{beta}"] = {"cell_updates_per_s": r["cell_updates_per_s"], "ms_per_step": r["ms_per_step"],
                                          "stage_kernel_ms": round(r["stage_kernel_ms"], 4)}
             log(f"bench: irs {prec} beta {beta}: {r['cell_updates_per_s'] / 1e9:.3f} G cell-updates/s, "
@@ -238,6 +334,7 @@ def main():
                     help="profiling aid: run only one arithmetic form (no JSON line)")
     ap.add_argument("--requests", type=int, default=24, help="serving: requests in the Poisson stream")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
+    ap.add_argument("--no-configs", action="store_true", help="skip the C3 and C4 configurations")
     ap.add_argument("--no-conv", action="store_true", help="irs: skip the convergence runs")
     ap.add_argument("--workload", choices=["shock", "axi", "batch", "serve", "irs"], default="shock",
                     help="shock: the headline workload (default); axi, batch: the laminar axisymmetric case and "
@@ -270,22 +367,19 @@ def main():
 
     from q2d_amd import cases
 
-    def problem_fn(precision):
-        return cases.planar_shock(seed=args.seed, n=args.n, steps=1 << 30, precision=precision)
-
+    pb64 = cases.planar_shock(seed=args.seed, n=args.n, steps=1 << 30, precision="fp64")
+    pbmx = with_precision(pb64, "mixed")
     log(f"bench: planar shock 4 x {args.n}^2, SSP-RK3, warmup {args.warmup}, steps {args.steps}")
     if args.form != "both":
-        r, _ = time_form(problem_fn, args.form, args.steps, args.warmup)
+        r = time_form(pbmx if args.form == "mixed" else pb64, args.steps, args.warmup)
         log(f"bench: {args.form} {r['cell_updates_per_s'] / 1e9:.3f} G cell-updates/s, stage kernel {r['stage_kernel_ms']:.4f} ms")
         return
-    plain, rho64 = time_form(problem_fn, "fp64", args.steps, args.warmup)
+    plain = time_form(pb64, args.steps, args.warmup)
     log(f"bench: plain fp64  {plain['cell_updates_per_s'] / 1e9:.3f} G cell-updates/s")
-    mixed, rho32 = time_form(problem_fn, "mixed", args.steps, args.warmup)
+    mixed = time_form(pbmx, args.steps, args.warmup)
     log(f"bench: mixed       {mixed['cell_updates_per_s'] / 1e9:.3f} G cell-updates/s")
-    num = sum(np.abs(a - b).sum() for a, b in zip(rho32, rho64))
-    den = sum(np.abs(b).sum() for b in rho64)
-    l1 = num / den
-    linf = max(np.abs(a - b).max() for a, b in zip(rho32, rho64)) / max(np.abs(b).max() for b in rho64)
+    acc = accuracy(pb64, pbmx, 500)                  # C2's own 500 steps, whatever --steps is
+    log(f"bench: mixed vs fp64 after 500 steps: rel L1(rho) {acc['rel_L1_rho']:.2e}, rel Linf {acc['rel_Linf_rho']:.2e}")
 
     n_rk = plain["n_rk"]
     bytes_update = n_rk * BYTES_PER_CELL_STAGE + BYTES_DT_PER_CELL
@@ -319,19 +413,23 @@ def main():
         "plain_fp64": {"value": plain["cell_updates_per_s"], "ms_per_step": plain["ms_per_step"],
                        "roofline": roofline(plain, "fp64"), "dtype": "f64"},
         "mixed_vs_plain": round(mixed["cell_updates_per_s"] / plain["cell_updates_per_s"], 3),
-        "mixed_err_vs_fp64": {"rel_L1_rho": l1, "rel_Linf_rho": linf, "steps": args.warmup + args.steps},
+        "mixed_err_vs_fp64": acc,
         "alg_bytes_per_cell_update": bytes_update,
         "effective_GBps": {"plain": plain["cell_updates_per_s"] * bytes_update / 1e9,
                            "mixed": mixed["cell_updates_per_s"] * bytes_update / 1e9},
         "roofline": roofline(mixed, "mixed"),
     }
+    if not args.no_configs:
+        log("bench: C3 and C4 ...")
+        line["configs"] = c3_c4(args)
     if not args.no_serve:
         log("bench: serving stream ...")
         line["serving"] = serve_workload(args.requests, args.seed)
-        log(f"bench: serving p50 {line['serving']['p50_s']:.2f} s, p99 {line['serving']['p99_s']:.2f} s")
+        log(f"bench: serving p50 {line['serving']['p50_s']:.2f} s, p99 {line['serving']['p99_s']:.2f} s "
+            f"({line['serving']['converged']}/{line['serving']['requests']} converged)")
     if not args.no_cpu:
         log("bench: CPU baseline ...")
-        line["cpu_baseline"] = cpu_baseline(problem_fn, args.cpu_steps)
+        line["cpu_baseline"] = cpu_baseline(pb64, args.cpu_steps)
     print(json.dumps(line))
 
 

@@ -95,6 +95,10 @@ extern "C" {
 #define Q2D_BC_SUPERSONIC 405
 #define Q2D_BC_OUTFLOW    406
 
+/* integration-variables */
+#define Q2D_INTVARS_CONS  0
+#define Q2D_INTVARS_PRIM  1
+
 /* Numerical scheme of one case: the [Q2D-Numerics]/[Q2D-Physics]/[Q2D-Sources]
  * keys of input.ini after Assign_Setup (Assign_Setup.f90:11-62). */
 typedef struct q2d_scheme {
@@ -115,7 +119,10 @@ typedef struct q2d_scheme {
   int32_t space_sd;        /* 1: space-reconstruction = 'MUSCL-SD' (obj_space_scheme%SD, Assign_Setup.f90:
                               49-53): Tramel's pressure detector (Lib_Shock_Detector.f90:14-38) scales the
                               MUSCL slopes of every interior and block-connection face; 0: 'MUSCL'. */
-  int32_t reserved_sd;
+  int32_t integration_vars;  /* integration-variables (Register_Numerics.f90:42, bound by
+                              Mod_Newstate.f90:28-36): Q2D_INTVARS_CONS ('cons', Newstate_Conservative);
+                              Q2D_INTVARS_PRIM ('prim', Newstate_Primitive, Lib_Newstate.f90:167-310) is
+                              rejected with Q2D_E_UNSUPPORTED */
 } q2d_scheme;
 
 /* One structured block, in the reference's allocation layout (see top). */
@@ -161,6 +168,10 @@ typedef struct q2d_case_desc {
   double     time_threshold;
   double     res_drop;      /* > 0: also stop when residual(1) <= res_drop * residual(1) of the case's
                                first step (a relative drop; serving); 0: off, as in the reference */
+  double     time0;         /* domain%time when the run starts (Wrap_Setup.f90:129-152): 0 for a new
+                               time-accurate run, the solution's SOLUTIONTIME on a restart of one, -1 for
+                               a steady run; time-accurate steps add dtglobal to it, and the
+                               time-threshold test reads it */
 } q2d_case_desc;
 
 typedef struct q2d_problem {

@@ -22,11 +22,13 @@ def build():
     return lib_path()
 
 
-def load():
+def load(path=None):
+    """The oracle library (the portable build; `path`: another build of the same source, such as the
+    CPU-baseline timing build)."""
     global _LIB
-    if _LIB is not None:
+    if _LIB is not None and path is None:
         return _LIB
-    p = lib_path()
+    p = Path(path) if path else lib_path()
     if not p.exists():
         build()
     lib = C.CDLL(str(p))
@@ -64,7 +66,8 @@ def load():
     lib.q2do_bc_inflow_flux.restype = C.c_int
     lib.q2do_bc_outflow_flux.argtypes = [C.c_double, C.c_double, C.c_int, dp] + [C.c_double] * 5 + [dp]
     lib.q2do_bc_outflow_flux.restype = C.c_int
-    _LIB = lib
+    if path is None:
+        _LIB = lib
     return lib
 
 
@@ -114,9 +117,9 @@ def outflow_flux(R, cp, face, Bp, n1, n2, int_l, p_exit, rel_fac):
 class Oracle:
     """The CPU restatement run on a cases.Problem."""
 
-    def __init__(self, problem, threads=None):
+    def __init__(self, problem, threads=None, lib=None):
         from q2d_amd.capi import StepOut, to_problem_c
-        self.lib = load()
+        self.lib = lib if lib is not None else load()
         if threads:
             self.lib.q2do_set_threads(int(threads))
         self.problem = problem

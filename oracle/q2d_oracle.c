@@ -550,7 +550,7 @@ q2do *q2do_create(const q2d_problem *prob) {
     cs->gas.mu_ref = cs->d.scheme.mu_ref; cs->gas.T_ref = cs->d.scheme.T_ref;
     cs->gas.S = cs->d.scheme.S_suth; cs->gas.Pr = cs->d.scheme.Pr;
     cs->first_block = -1; cs->first_bc = -1;
-    cs->dtglobal = 0.0; cs->time = 0.0;
+    cs->dtglobal = 0.0; cs->time = cs->d.time0;      /* Wrap_Setup.f90:129-152 */
   }
   for (int b = 0; b < o->nblocks; b++) {
     ocase *cs = &o->cs[o->blk[b].case_id];
@@ -1612,6 +1612,12 @@ static int newstate(q2do *o, ocase *cs, int irk, int *fb, int *fi, int *fj) {
   }
   for (int bb = 0; bb < cs->nblocks; bb++) {
     oblk *b = &o->blk[cs->first_block + bb];
+    /* rows in parallel (the reference's !$omp do collapse(2), Lib_Newstate.f90:71-84); each row keeps its
+     * first failing column, and the first failing row is taken after the loop: the serial order's first */
+    int *row_fail = (int *)calloc((size_t)b->nj + 1, sizeof(int));
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
     for (int j = 1; j <= b->nj; j++) {
       for (int i = 1; i <= b->ni; i++) {
         double dt = b->dtlocal[IN(b, i, j)], sr = SUR(b, i, j);
@@ -1629,12 +1635,15 @@ static int newstate(q2do *o, ocase *cs, int irk, int *fb, int *fi, int *fj) {
         cons2prim(&cs->gas, cons, prim);
         /* Check_And_Fix_State — Lib_Newstate.f90:316-350 */
         if (isnan(prim[0]) || isnan(prim[1]) || isnan(prim[2]) || isnan(prim[3]) || prim[3] < 0.0) {
-          if (!err) { err = Q2D_E_STATE; *fb = bb + 1; *fi = i; *fj = j; }
+          if (!row_fail[j]) row_fail[j] = i;
         }
         if (prim[0] < 1e-20) prim[0] = 1e-20;
         for (int v = 0; v < NP; v++) PV(b, v, i, j) = prim[v];
       }
     }
+    for (int j = 1; j <= b->nj && !err; j++)
+      if (row_fail[j]) { err = Q2D_E_STATE; *fb = bb + 1; *fi = row_fail[j]; *fj = j; }
+    free(row_fail);
   }
   return err;
 }
@@ -1645,7 +1654,11 @@ static void residual(q2do *o, ocase *cs, double *tot) {
   for (int bb = 0; bb < cs->nblocks; bb++) {
     oblk *b = &o->blk[cs->first_block + bb];
     double blk[4] = {0, 0, 0, 0};
-    for (int j = 1; j <= b->nj; j++) {
+    double *rows = (double *)malloc(sizeof(double) * 4 * (size_t)(b->nj + 1));
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+    for (int j = 1; j <= b->nj; j++) {                 /* row partials in parallel ... */
       double row[4] = {0, 0, 0, 0};
       for (int i = 1; i <= b->ni; i++) {
         for (int v = 0; v < 4; v++) {
@@ -1653,8 +1666,11 @@ static void residual(q2do *o, ocase *cs, double *tot) {
           row[v] = row[v] + r * r;
         }
       }
-      for (int v = 0; v < 4; v++) blk[v] = blk[v] + row[v];
+      for (int v = 0; v < 4; v++) rows[4 * j + v] = row[v];
     }
+    for (int j = 1; j <= b->nj; j++)                   /* ... summed in row order (the same bits) */
+      for (int v = 0; v < 4; v++) blk[v] = blk[v] + rows[4 * j + v];
+    free(rows);
     for (int v = 0; v < 4; v++) tot[v] = tot[v] + blk[v];
   }
   for (int v = 0; v < 4; v++) tot[v] = sqrt(tot[v]);
@@ -1676,6 +1692,9 @@ static int step_case(q2do *o, ocase *cs, q2d_step_out *out) {
   }
   for (int bb = 0; bb < cs->nblocks; bb++) {          /* Copy_State :177-197 */
     oblk *b = &o->blk[cs->first_block + bb];
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
     for (int j = 1; j <= b->nj; j++)
       for (int i = 1; i <= b->ni; i++)
         for (int v = 0; v < NP; v++) POV(b, v, i, j) = PV(b, v, i, j);

@@ -26,6 +26,7 @@ LIMITERS = {"first-order": 0, "iord": 0, "minmod": 1, "vanleer": 2, "vanalbada":
 RIEMANN = {"HLLC": 0, "HLLE": 1, "HLLEM": 2, "HLLE++": 3, "HLLC+": 4}
 SOURCES = {"none": 0, "axisymmetric": 1, "area_variation": 2}
 PRECISION = {"fp64": 0, "mixed": 1}
+INTVARS = {"cons": 0, "prim": 1}
 
 
 class Q2DError(RuntimeError):
@@ -39,7 +40,7 @@ class Scheme(C.Structure):
                 ("model", C.c_int32), ("source", C.c_int32), ("precision", C.c_int32), ("ramp_iter", C.c_int32),
                 ("cfl", C.c_double), ("vnn", C.c_double), ("gas_R", C.c_double), ("gas_cp", C.c_double),
                 ("mu_ref", C.c_double), ("T_ref", C.c_double), ("S_suth", C.c_double), ("Pr", C.c_double),
-                ("irs_beta", C.c_double), ("space_sd", C.c_int32), ("reserved_sd", C.c_int32)]
+                ("irs_beta", C.c_double), ("space_sd", C.c_int32), ("integration_vars", C.c_int32)]
 
 
 class BlockDesc(C.Structure):
@@ -60,7 +61,8 @@ class BCEntry(C.Structure):
 
 class CaseDesc(C.Structure):
     _fields_ = [("scheme", Scheme), ("itermax", C.c_int32), ("reserved", C.c_int32),
-                ("res_threshold", C.c_double), ("time_threshold", C.c_double), ("res_drop", C.c_double)]
+                ("res_threshold", C.c_double), ("time_threshold", C.c_double), ("res_drop", C.c_double),
+                ("time0", C.c_double)]
 
 
 class ProblemC(C.Structure):
@@ -113,8 +115,10 @@ def to_problem_c(problem):
         s.mu_ref, s.T_ref, s.S_suth, s.Pr = c.mu_ref, c.T_ref, c.S_suth, c.Pr
         s.irs_beta = c.irs_beta
         s.space_sd = int(c.space_sd)
+        s.integration_vars = INTVARS[c.integration_vars]
         cases[k].itermax, cases[k].res_threshold, cases[k].time_threshold = c.itermax, c.res_threshold, c.time_threshold
         cases[k].res_drop = c.res_drop
+        cases[k].time0 = c.time0
     pc = ProblemC()
     pc.nblocks, pc.nbc, pc.ncases = nb, nbc, nc
     pc.blocks = C.cast(blocks, C.POINTER(BlockDesc))
@@ -199,6 +203,7 @@ class GpuSolver:
         self.ncases = len(problem.cases)
         self.blocks = list(problem.blocks)        # blocks by global index (grows with add_cases)
         self._out = (StepOut * max(1, self.ncases))()
+        self._cases = [problem.case(k) for k in range(self.ncases)]   # per case, None once removed
 
     def add_cases(self, problem):
         """Append the cases of `problem` to the live solver; returns (first case index, first block index)."""
@@ -208,11 +213,22 @@ class GpuSolver:
         _check(self.lib, self.lib.q2d_gpu_add_cases(self.h, C.byref(pc), C.byref(first)))
         self.blocks.extend(problem.blocks)
         self.ncases += len(problem.cases)
+        self._cases.extend(problem.case(k) for k in range(len(problem.cases)))
         self._out = (StepOut * max(1, self.ncases))()
         return first.value, nb0
 
     def remove_case(self, case):
         _check(self.lib, self.lib.q2d_gpu_remove_case(self.h, int(case)))
+        self._cases[int(case)] = None
+
+    def live_problem(self):
+        """The cases the handle holds now (added ones included, removed ones not), as one Problem."""
+        from .cases import Problem
+        pb = Problem()
+        for c in self._cases:
+            if c is not None:
+                pb.add(c)
+        return pb
 
     def device_bytes(self):
         nc, nb, by = C.c_int32(0), C.c_int32(0), C.c_int64(0)
@@ -280,7 +296,7 @@ class GpuSolver:
     def autotune(self, steps=4, use_cache=True, verbose=False):
         """Pick the stage-kernel tiling for this problem (q2d_amd.autotune) and apply it."""
         from .autotune import tune
-        r, w, _ = tune(self.problem, steps=steps, use_cache=use_cache, verbose=verbose)
+        r, w, _ = tune(self.live_problem(), steps=steps, use_cache=use_cache, verbose=verbose)
         self.set_tiling(r, w)
         return r, w
 

@@ -1,0 +1,213 @@
+"""The case-directory contract of the Q2D host, for Python hosts and for checking the Fortran host.
+
+The same files and rules as q2d_amd/fortran/q2d_case_io.f90 (which a Fortran host uses), written
+independently: input.ini keys (Register_Sim_Param.f90:26-29, Register_Physics.f90:24,
+Register_Numerics.f90:31-75, Register_Sources.f90:33) bound as Assign_Setup.f90:43-62 binds them,
+INPUT/phase.txt and INPUT/thermo.dat (one species, constant cp), INPUT/bc.txt (IO_BC.f90:103-420),
+INPUT/ic.tec (or OUTPUT/field.tec on a restart) as Tecplot ASCII BLOCK zones, and the writers of
+OUTPUT/field.tec (with SOLUTIONTIME) and OUTPUT/residual-history.dat.  load_case() returns a
+cases.Problem whose geometry comes from q2d_amd.geometry (Setup_Metrics, Mod_Metrics.f90:11-148).
+"""
+from __future__ import annotations
+
+import re
+from pathlib import Path
+
+import numpy as np
+
+from . import cases
+from .geometry import BlockGeometry, axisymmetric_geometry
+
+R_UNIVERSAL = 8314.51
+# codes with one property line that this path does not build (IO_BC.f90:119-141)
+_ONE_LINE_REJECTED = set(range(301, 310)) - {302} | {103, 401, 402, 403, 404, 407, 420, 501, 502}
+
+
+class CaseError(ValueError):
+    pass
+
+
+def read_ini(path):
+    """{section: {key: value}} of an input.ini ('[section]', 'key = value', ';' / '#' comments)."""
+    out, sec = {}, ""
+    for raw in Path(path).read_text().splitlines():
+        line = re.split(r"[;#]", raw, 1)[0].strip()
+        if not line:
+            continue
+        if line.startswith("["):
+            sec = line[1:line.index("]")]
+            out.setdefault(sec, {})
+            continue
+        if "=" in line:
+            k, v = line.split("=", 1)
+            out.setdefault(sec, {})[k.strip()] = v.strip()
+    return out
+
+
+def _logical(v):
+    return v.strip().lower().lstrip(".").startswith("t")
+
+
+def read_gas(case_dir):
+    """(R, cp) of the one species of INPUT/phase.txt with the constant-cp table INPUT/thermo.dat."""
+    d = Path(case_dir) / "INPUT"
+    lines = [x for x in (d / "phase.txt").read_text().splitlines() if x.strip()]
+    if len(lines) != 2:
+        raise CaseError("INPUT/phase.txt: one species only on this path")
+    W = float(lines[1].split()[1])
+    rows = []
+    for x in (d / "thermo.dat").read_text().splitlines():
+        x = x.strip()
+        if x and (x[0].isdigit() or x[0] in ".-"):
+            rows.append([float(v) for v in x.split()[:3]])
+    t = np.array(rows)
+    cp = t[0, 1]
+    if np.any(np.abs(t[:, 1] - cp) > 1e-12 * abs(cp)) or np.any(np.abs(t[:, 2] - cp * t[:, 0]) > 1e-9 * np.abs(cp * t[:, 0])):
+        raise CaseError("INPUT/thermo.dat: cp varies with T (table-mode thermodynamics is not on this path)")
+    return R_UNIVERSAL / W, cp
+
+
+def _dir_token(tok):
+    if "normal" in tok.lower():
+        return float(np.finfo(np.float64).max)
+    try:
+        return float(tok)
+    except ValueError:
+        return 0.0
+
+
+def read_bc(path, nrec):
+    """The nrec records of INPUT/bc.txt as cases-style dicts, ordered by block, then face (stable)."""
+    toks = Path(path).read_text().splitlines()
+    recs, k = [], 0
+    for _ in range(nrec):
+        while not toks[k].strip():
+            k += 1
+        b, i, j, f, t = (int(v) for v in toks[k].split()[:5])
+        k += 1
+        r = {"b": b, "i": i, "j": j, "f": f, "type": t, "case_id": 0}
+        if t in (101, 201):
+            v = [int(x) for x in toks[k].split()[:8]]
+            r.update(bs=v[0], js=v[2], fs=v[3], d11=v[4], d12=v[5], d21=v[6], d22=v[7])
+            r["is"] = v[1]
+            k += 1
+        elif t == 302:
+            r["T"] = float(toks[k].split()[0])
+            k += 1
+        elif t == 405:
+            v = toks[k].split()
+            r.update(mach=float(v[0]), T=float(v[1]), p=float(v[2]), alpha=_dir_token(v[3]), rel_fac=float(v[5]))
+            k += 1
+        elif t == 406:
+            v = toks[k].split()
+            r.update(p=float(v[0]), rel_fac=float(v[1]))
+            k += 1
+        elif t in (200, 300, 400):
+            pass
+        elif t in _ONE_LINE_REJECTED:
+            k += 1
+        else:
+            raise CaseError(f"INPUT/bc.txt: unsupported record code {t}")
+        recs.append(r)
+    recs.sort(key=lambda r: (r["b"], r["f"]))              # Python's sort is stable
+    return recs
+
+
+def read_tec(path):
+    """Zones of a Tecplot ASCII BLOCK file: (SOLUTIONTIME, [(node (nj+1, ni+1, 2), cell vars (nv, nj, ni))], names)."""
+    text = Path(path).read_text().splitlines()
+    names = re.findall(r'"([^"]+)"', text[0])
+    nv = len(names) - 2
+    zones, k, soltime = [], 1, 0.0
+    while k < len(text):
+        z = text[k]
+        if "ZONE" not in z:
+            k += 1
+            continue
+        ni1 = int(re.search(r"\bI=\s*(\d+)", z).group(1))
+        nj1 = int(re.search(r"\bJ=\s*(\d+)", z).group(1))
+        m = re.search(r"SOLUTIONTIME=\s*([-+0-9.EeDd]+)", z)
+        if m:
+            soltime = float(m.group(1).replace("D", "E").replace("d", "e"))
+        need = 2 * ni1 * nj1 + nv * (ni1 - 1) * (nj1 - 1)
+        vals = []
+        k += 1
+        while len(vals) < need:
+            vals.extend(float(v) for v in text[k].split())
+            k += 1
+        a = np.array(vals)
+        nn = ni1 * nj1
+        node = np.stack([a[:nn].reshape(nj1, ni1), a[nn:2 * nn].reshape(nj1, ni1)], -1)
+        cv = a[2 * nn:].reshape(nv, nj1 - 1, ni1 - 1)
+        zones.append((node, cv))
+    return soltime, zones, names
+
+
+def load_case(case_dir, precision="fp64"):
+    """A Q2D case directory as a cases.Problem (one case), with its SOLUTIONTIME-based start time."""
+    d = Path(case_dir)
+    ini = read_ini(d / "input.ini")
+    par, phy, num, src = (ini.get(s, {}) for s in ("Q2D-Parameters", "Q2D-Physics", "Q2D-Numerics", "Q2D-Sources"))
+    if phy.get("equations", "").lower() != "euler":
+        raise CaseError("this host runs the Euler equations")
+    source = src.get("active", "none")
+    if source not in ("none", "axisymmetric"):
+        raise CaseError(f"[Q2D-Sources] active = {source} is not read by this host")
+    R, cp = read_gas(d)
+    newrun = _logical(par.get("newrun", "true"))
+    soltime, zones, _ = read_tec(d / ("INPUT/ic.tec" if newrun else "OUTPUT/field.tec"))
+    blocks, geoms = [], []
+    for node, cv in zones:
+        nj, ni = node.shape[0] - 1, node.shape[1] - 1
+        geo = BlockGeometry(node)
+        P = np.zeros((nj + 4, ni + 4, 4))
+        P[2:-2, 2:-2] = np.moveaxis(cv[:4], 0, -1)
+        cases.fill_initial_ghosts(P)
+        blocks.append(cases.Block(geo, P))
+        geoms.append(geo)
+    nrec = sum(2 * b.ni + 2 * b.nj for b in blocks)
+    recs = read_bc(d / "INPUT/bc.txt", nrec)
+    ts = num.get("time-scheme", "euler")
+    space = num.get("space-reconstruction", "")
+    lim = "first-order" if space == "first-order" else num.get("flux-limiter", "")
+    if lim not in cases_limiters():
+        lim = "first-order"                                   # unknown limiter -> IORD (Lib_Limiters.f90:52-53)
+    ta = _logical(num.get("time-accurate", ".false."))
+    spec = cases.CaseSpec(
+        n_rk=1 if ts == "euler" else int(ts[2]), limiter=lim, riemann=num.get("riemann-solver", "HLLC"),
+        time_accurate=ta, source=source, precision=precision, ramp_iter=int(num.get("cfl-rise-threshold", "0")),
+        cfl=float(num.get("cfl", "0.5")), vnn=float(num.get("vnn", "0.3")), gas_R=R, gas_cp=cp,
+        itermax=int(float(par.get("iter-threshold", "1000000000"))), res_threshold=float(par.get("res-threshold", "1e-10")),
+        time_threshold=float(par.get("time-threshold", "1e30")), irs_beta=float(num.get("irs-beta", "0.0")),
+        space_sd=space == "MUSCL-SD", integration_vars=num.get("integration-variables", "cons"),
+        time0=(0.0 if newrun else soltime) if ta else -1.0)
+    pb = cases.Problem(blocks=blocks, bc=recs, cases=[spec], meta={"name": d.name, "soltime": soltime})
+    cases.finish(pb, geoms)
+    if source == "axisymmetric":
+        for b in blocks:
+            b.src_a, b.src_b = axisymmetric_geometry(b.geo)
+    return pb
+
+
+def cases_limiters():
+    from .capi import LIMITERS
+    return LIMITERS
+
+
+def write_field(path, problem, states, soltime):
+    """OUTPUT/field.tec of the blocks' interior states (x, y nodal; rho(1), u, v, p, T, g, R cell-centred)."""
+    R, cp = problem.cases[0].gas_R, problem.cases[0].gas_cp
+    with open(path, "w") as f:
+        f.write(' VARIABLES ="x" "y" "rho(1)" "u" "v" "p" "T" "g" "R"\n')
+        for k, (b, P) in enumerate(zip(problem.blocks, states)):
+            f.write(f" ZONE  T = Block{k + 1}, I={b.ni + 1}, J={b.nj + 1}, K=1, DATAPACKING=BLOCK, "
+                    f"VARLOCATION=([1-2]=NODAL,[3-9]=CELLCENTERED), SOLUTIONTIME={soltime:.15E}\n")
+            cols = [b.geo.node[..., 0], b.geo.node[..., 1], P[..., 0], P[..., 1], P[..., 2], P[..., 3],
+                    P[..., 3] / (P[..., 0] * R), np.full(P.shape[:2], cp / (cp - R)), np.full(P.shape[:2], R)]
+            for c in cols:
+                f.write("\n".join(f"{v: .15E}" for v in np.asarray(c).ravel()) + "\n")
+
+
+def read_residual_history(path):
+    """OUTPUT/residual-history.dat rows: iteration, time, residual(1:4)."""
+    return np.loadtxt(path, ndmin=2)

@@ -56,6 +56,8 @@ class CaseSpec:
     res_drop: float = 0.0
     irs_beta: float = 0.0         # irs-beta: > 0 enables implicit residual smoothing
     space_sd: bool = False        # space-reconstruction = MUSCL-SD (Tramel's shock detector on the slopes)
+    integration_vars: str = "cons"  # integration-variables ('prim' is rejected by the library)
+    time0: float = 0.0            # domain%time at the start: 0 (new run), SOLUTIONTIME (restart), -1 (steady)
 
 
 @dataclass

@@ -3014,6 +3014,10 @@ int append_cases(q2d_handle* h, const q2d_problem* prob, int32_t* first_case) {
     if (prob->cases[c].res_drop < 0 || prob->cases[c].res_drop >= 1) return fail(Q2D_E_ARG, "res_drop must be in [0, 1)");
     if (!(s.irs_beta >= 0) || !std::isfinite(s.irs_beta)) return fail(Q2D_E_ARG, "irs_beta must be finite and >= 0");
     if (s.space_sd != 0 && s.space_sd != 1) return fail(Q2D_E_ARG, "space_sd must be 0 or 1");
+    if (s.integration_vars == Q2D_INTVARS_PRIM)
+      return fail(Q2D_E_UNSUPPORTED, "integration-variables = prim (Newstate_Primitive) is not available on this path");
+    if (s.integration_vars != Q2D_INTVARS_CONS) return fail(Q2D_E_ARG, "unknown integration_vars");
+    if (!std::isfinite(prob->cases[c].time0)) return fail(Q2D_E_ARG, "time0 must be finite");
 
     for (int k = 0; k < nblk[c]; ++k) {
       const q2d_block_desc& d = prob->blocks[first_block[c] + k];
@@ -3114,7 +3118,11 @@ int append_cases(q2d_handle* h, const q2d_problem* prob, int32_t* first_case) {
       HIPCHK(hipMemcpy(nc, h->d_ctrl, sizeof(DevCtrl) * h->ncases, hipMemcpyDeviceToDevice));
     }
     std::vector<DevCtrl> ctl(prob->ncases);
-    for (auto& k : ctl) { memset(&k, 0, sizeof(k)); k.active = 1; k.todo = 1; k.fail_key = ~0ull; }
+    for (int c = 0; c < prob->ncases; ++c) {
+      DevCtrl& k = ctl[c];
+      memset(&k, 0, sizeof(k)); k.active = 1; k.todo = 1; k.fail_key = ~0ull;
+      k.time = prob->cases[c].time0;                   // domain%time at set-up (Wrap_Setup.f90:129-152)
+    }
     HIPCHK(hipMemcpy(nc + h->ncases, ctl.data(), sizeof(DevCtrl) * prob->ncases, hipMemcpyHostToDevice));
     (void)hipFree(h->d_ctrl);
     h->d_ctrl = nc;

@@ -7,7 +7,9 @@
 !
 ! usage: q2d_fdriver <case file> <output file>
 !   case file (native byte order, stream access):
-!     int32  magic (= 1362248753), nblocks, nbc, ncases, nsteps
+!     int32  magic (= 1362248754), nblocks, nbc, ncases, nsteps, and the byte sizes of the C structs
+!            q2d_case_desc, q2d_bc_entry and q2d_step_out the writer used (checked against this host's
+!            bind(C) types before anything else is read)
 !     ncases x q2d_case_desc, nbc x q2d_bc_entry           (the C structs' bytes)
 !     per block: int32 ni, nj, case_id, flags (1 node, 2 src_a, 4 src_b), then
 !       P(4,ni+4,nj+4), dl(2,ni+4,nj+4), M(4,ni+4,nj+4), sur(ni+4,nj+4),
@@ -24,7 +26,7 @@ program q2d_fdriver
     real(c_double), allocatable :: node(:,:,:), sa(:,:), sb(:,:)
   end type
 
-  integer(c_int32_t) :: hdr(5)
+  integer(c_int32_t) :: hdr(8)
   type(block_arrays), allocatable, target :: blk(:)
   type(q2d_block_desc), allocatable, target :: desc(:)
   type(q2d_bc_entry), allocatable, target :: bc(:)
@@ -34,12 +36,23 @@ program q2d_fdriver
   type(c_ptr) :: h
   character(len=1024) :: fin, fout
   integer :: u, b, rc, ndev, dims(4)
+  type(q2d_case_desc) :: cs1
+  type(q2d_bc_entry) :: bc1
+  type(q2d_step_out) :: out1
 
   call get_command_argument(1, fin)
   call get_command_argument(2, fout)
   open(newunit=u, file=trim(fin), access='stream', form='unformatted', status='old')
   read(u) hdr
-  if (hdr(1) /= 1362248753) stop 'q2d_fdriver: bad case file'
+  if (hdr(1) /= 1362248754) then
+    print '(a)', 'q2d_fdriver: not a case file of this layout version (magic mismatch)'
+    error stop 2
+  end if
+  if ((hdr(4) > 0 .and. hdr(6) /= c_sizeof(cs1)) .or. hdr(7) /= c_sizeof(bc1) .or. hdr(8) /= c_sizeof(out1)) then
+    print '(a,3(1x,i0),a,3(1x,i0))', 'q2d_fdriver: descriptor layout mismatch: file has struct sizes', hdr(6:8), &
+      ', this host', c_sizeof(cs1), c_sizeof(bc1), c_sizeof(out1)
+    error stop 2
+  end if
   allocate(blk(hdr(2)), desc(hdr(2)), bc(max(1, hdr(3))), cs(hdr(4)), out(hdr(4)))
   read(u) cs
   if (hdr(3) > 0) read(u) bc(1:hdr(3))

@@ -27,7 +27,7 @@ module q2d_gpu
     real(c_double)     :: gas_R, gas_cp
     real(c_double)     :: mu_ref, T_ref, S_suth, Pr
     real(c_double)     :: irs_beta
-    integer(c_int32_t) :: space_sd, reserved_sd
+    integer(c_int32_t) :: space_sd, integration_vars
   end type
 
   type, bind(C), public :: q2d_block_desc
@@ -48,6 +48,7 @@ module q2d_gpu
     integer(c_int32_t) :: itermax, reserved
     real(c_double)     :: res_threshold, time_threshold
     real(c_double)     :: res_drop
+    real(c_double)     :: time0
   end type
 
   type, bind(C), public :: q2d_problem

@@ -15,7 +15,10 @@ import numpy as np
 
 from .capi import StepOut, to_problem_c
 
-MAGIC = 1362248753
+MAGIC = 1362248754
+# the header carries the sizes of the descriptor structs written as raw bytes, so that a host built
+# against another layout of include/q2d_gpu.h reports the mismatch instead of misreading the file
+HEADER = "<8i"
 DRIVER = Path(__file__).resolve().parent / "fortran" / "build" / "q2d_fdriver"
 
 
@@ -24,7 +27,8 @@ def write_case_file(problem, path, nsteps):
     blocks, bcs, cases, _ = keep
     nb, nbc, nc = len(problem.blocks), len(problem.bc), len(problem.cases)
     with open(path, "wb") as f:
-        f.write(struct.pack("<5i", MAGIC, nb, nbc, nc, int(nsteps)))
+        f.write(struct.pack(HEADER, MAGIC, nb, nbc, nc, int(nsteps), C.sizeof(cases[0]) if nc else 0,
+                            C.sizeof(bcs[0]), C.sizeof(StepOut)))
         f.write(C.string_at(C.addressof(cases), C.sizeof(cases)))
         if nbc:
             f.write(C.string_at(C.addressof(bcs), C.sizeof(bcs[0]) * nbc))
@@ -53,3 +57,43 @@ def read_output(path, problem):
         outs.append(so.as_dict())
         off += C.sizeof(StepOut)
     return states, outs
+
+
+def read_case_file(path):
+    """A case file (as write_case_file writes it, or q2d_fcase --dump): header, case and record structs,
+    and per block its arrays in the reference layout."""
+    from .capi import BCEntry, CaseDesc
+    raw = Path(path).read_bytes()
+    hdr = struct.unpack_from(HEADER, raw, 0)
+    if hdr[0] != MAGIC:
+        raise ValueError("not a case file of this layout version")
+    _, nb, nbc, nc, nsteps, s_case, s_bc, s_out = hdr
+    if (nc and s_case != C.sizeof(CaseDesc)) or s_bc != C.sizeof(BCEntry) or s_out != C.sizeof(StepOut):
+        raise ValueError("descriptor layout mismatch")
+    off = struct.calcsize(HEADER)
+    cs = [CaseDesc.from_buffer_copy(raw[off + k * s_case: off + (k + 1) * s_case]) for k in range(nc)]
+    off += nc * s_case
+    bcs = [BCEntry.from_buffer_copy(raw[off + k * s_bc: off + (k + 1) * s_bc]) for k in range(nbc)]
+    off += nbc * s_bc
+    blocks = []
+    for _ in range(nb):
+        ni, nj, cid, flags = struct.unpack_from("<4i", raw, off)
+        off += 16
+
+        def take(shape):
+            nonlocal off
+            n = int(np.prod(shape))
+            a = np.frombuffer(raw, dtype=np.float64, count=n, offset=off).reshape(shape).copy()
+            off += 8 * n
+            return a
+        blk = {"ni": ni, "nj": nj, "case_id": cid, "P": take((nj + 4, ni + 4, 4)), "dl": take((nj + 4, ni + 4, 2)),
+               "M": take((nj + 4, ni + 4, 4)), "sur": take((nj + 4, ni + 4)), "fi": take((nj, ni + 1, 3)),
+               "fj": take((nj + 1, ni, 3))}
+        if flags & 1:
+            blk["node"] = take((nj + 1, ni + 1, 2))
+        if flags & 2:
+            blk["src_a"] = take((nj, ni))
+        if flags & 4:
+            blk["src_b"] = take((nj, ni))
+        blocks.append(blk)
+    return {"nsteps": nsteps, "cases": cs, "bc": bcs, "blocks": blocks}
