@@ -15,12 +15,17 @@ namespace q2d {
 template <typename T> struct Gas { T R, cp, cv, gam; };
 
 // ---- arithmetic primitives ----------------------------------------------
+// FP64 division and square root: the compiler's correctly rounded IEEE operations (the FP64 form's
+// results must equal the reference's bit for bit, subnormal round-off included).
+__device__ __forceinline__ double ddiv_rn(double a, double b) { return a / b; }
+__device__ __forceinline__ double dsqrt_rn(double x) { return __builtin_sqrt(x); }
+
 template <typename T> __device__ __forceinline__ T qdiv(T a, T b);
-template <> __device__ __forceinline__ double qdiv<double>(double a, double b) { return a / b; }
+template <> __device__ __forceinline__ double qdiv<double>(double a, double b) { return ddiv_rn(a, b); }
 template <> __device__ __forceinline__ float qdiv<float>(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 
 template <typename T> __device__ __forceinline__ T qsqrt(T a);
-template <> __device__ __forceinline__ double qsqrt<double>(double a) { return __builtin_sqrt(a); }
+template <> __device__ __forceinline__ double qsqrt<double>(double a) { return dsqrt_rn(a); }
 template <> __device__ __forceinline__ float qsqrt<float>(float a) { return __builtin_amdgcn_sqrtf(a); }
 
 template <typename T> __device__ __forceinline__ T qabs(T a) { return __builtin_fabs(a); }
@@ -332,7 +337,7 @@ __device__ __forceinline__ double rcp64(double a) { return 1.0 / a; }
 
 // ---- conservative <-> primitive (FP64 always) ------------------------------
 __device__ __forceinline__ void prim2cons(const Gas<double>& g, double rho, double u, double v, double p, double U[4]) {
-  double T = p / (rho * g.R);
+  double T = ddiv_rn(p, rho * g.R);
   U[0] = rho;
   U[1] = rho * u;
   U[2] = rho * v;
@@ -340,9 +345,9 @@ __device__ __forceinline__ void prim2cons(const Gas<double>& g, double rho, doub
 }
 __device__ __forceinline__ void cons2prim(const Gas<double>& g, const double U[4], double q[4]) {
   double rho = U[0];
-  double vx = U[1] / rho, vy = U[2] / rho;
-  double e = U[3] / rho - 0.5 * (vx * vx + vy * vy);
-  double T = e / g.cv;
+  double vx = ddiv_rn(U[1], rho), vy = ddiv_rn(U[2], rho);
+  double e = ddiv_rn(U[3], rho) - 0.5 * (vx * vx + vy * vy);
+  double T = ddiv_rn(e, g.cv);
   q[0] = rho; q[1] = vx; q[2] = vy; q[3] = rho * g.R * T;
 }
 

@@ -1223,8 +1223,11 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
       constexpr int RIE = decltype(riec)::value;
       const bool last = (irk == C.n_rk);
       const bool irs = C.irs_beta > 0.0;                 // residual to B.irs[0]; k_irs smooths and updates
-      const bool sd = C.space_sd != 0;                   // MUSCL-SD: faces take their cell's detector value
-      const bool det = sd || C.riemann >= Q2D_RIEMANN_HLLEPP;   // the detector is computed (Assign_Setup.f90:49-53)
+      // MUSCL-SD: faces take their cell's detector value; the detector is computed with MUSCL-SD or an
+      // HLLE++ / HLLC+ solver (Assign_Setup.f90:49-53).  RIE_T == -3 (HLLC / HLLE launches of handles
+      // without the detector) compiles neither.
+      const bool sd = RIE_T != -3 && C.space_sd != 0;
+      const bool det = RIE_T != -3 && (sd || C.riemann >= Q2D_RIEMANN_HLLEPP);
       const int ni = B.ni, nj = B.nj, pitch = B.pitch;
       const long long pl = B.plane;
       // the case's control words, read once per item (wave-uniform; a read inside the row loop would be a
@@ -1470,8 +1473,8 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
             const Gas<double> gd{C.R, C.cp, C.cv, C.gam};
             if (C.source != Q2D_SRC_NONE) {
               const double rho = qA[0], u = qA[1], v = qA[2], p = qA[3];
-              const double mod_v = sqrt(u * u + v * v);
-              const double E0 = gd.cv * (p / (rho * gd.R)) + 0.5 * mod_v * mod_v;
+              const double mod_v = dsqrt_rn(u * u + v * v);
+              const double E0 = gd.cv * ddiv_rn(p, rho * gd.R) + 0.5 * mod_v * mod_v;
               double S[4];
               if (C.source == Q2D_SRC_AXI) {
                 const double invr = gp(B.sb)[oA];
@@ -1499,7 +1502,7 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
               GPW rs = gpw(B.irs[0]);
               if (sizeof(FT) == 8) {
 #pragma unroll
-                for (int v = 0; v < 4; ++v) rs[v * pl + oA] = -res[v] / sr * dt;
+                for (int v = 0; v < 4; ++v) rs[v * pl + oA] = ddiv_rn(-res[v], sr) * dt;
               } else {
                 const double dts = dt * rcp64(sr);
 #pragma unroll
@@ -1517,7 +1520,7 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
             if (sizeof(FT) == 8) {
 #pragma unroll
               for (int v = 0; v < 4; ++v) {
-                const double rr = -res[v] / sr * dt;
+                const double rr = ddiv_rn(-res[v], sr) * dt;
                 U[v] = Uo[v] + (U[v] - Uo[v] + rr) * c_rk;   // RK_stage — Lib_RK.f90:15-27
               }
               cons2prim(gd, U, qn2);
@@ -1545,23 +1548,23 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
             if (last && FUSE) {
               // Compute_dt of the next step (Mod_dt.f90:59-104) on the state just written; the
               // next step's Compute_dt would read exactly these values.
-              const double snd = sqrt(C.gam * qn2[3] / qn2[0]);
+              const double snd = dsqrt_rn(ddiv_rn(C.gam * qn2[3], qn2[0]));
               double dtmin = 1e8;
               const double* mm = gcur.mm;                 // c11, c21, c12, c22
               {
                 const double m1 = mm[0], m2 = mm[2];
-                const double nrm = sqrt(m1 * m1 + m2 * m2);
-                const double v1 = m1 / nrm, v2 = m2 / nrm;
+                const double nrm = dsqrt_rn(m1 * m1 + m2 * m2);
+                const double v1 = ddiv_rn(m1, nrm), v2 = ddiv_rn(m2, nrm);
                 const double lambda = fabs(qn2[1] * v1 + qn2[2] * v2) + snd;
-                const double dtd = dlx / lambda * C.cfl;
+                const double dtd = ddiv_rn(dlx, lambda) * C.cfl;
                 if (dtd < dtmin) dtmin = dtd;
               }
               if (C.ndir == 2) {
                 const double m1 = mm[1], m2 = mm[3];
-                const double nrm = sqrt(m1 * m1 + m2 * m2);
-                const double v1 = m1 / nrm, v2 = m2 / nrm;
+                const double nrm = dsqrt_rn(m1 * m1 + m2 * m2);
+                const double v1 = ddiv_rn(m1, nrm), v2 = ddiv_rn(m2, nrm);
                 const double lambda = fabs(qn2[1] * v1 + qn2[2] * v2) + snd;
-                const double dtd = dlyA / lambda * C.cfl;
+                const double dtd = ddiv_rn(dlyA, lambda) * C.cfl;
                 if (dtd < dtmin) dtmin = dtd;
               }
               const int iter_next = kiter + 1;
@@ -1603,7 +1606,10 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
         if (lane == 0) partials[(long long)it * kPartials + 4] = dtnext;
       }
     };
-    if constexpr (RIE_T < 0) {
+    if constexpr (RIE_T == -3) {                       // HLLC and HLLE only
+      if (C.riemann == Q2D_RIEMANN_HLLE) item_body(std::integral_constant<int, 1>{});
+      else item_body(std::integral_constant<int, 0>{});
+    } else if constexpr (RIE_T < 0) {
       if (C.riemann == Q2D_RIEMANN_HLLE) item_body(std::integral_constant<int, 1>{});
       else if (C.riemann == Q2D_RIEMANN_HLLEM) item_body(std::integral_constant<int, 2>{});
       else if (RIE_T == -2 && C.riemann == Q2D_RIEMANN_HLLEPP) item_body(std::integral_constant<int, 3>{});
@@ -2456,6 +2462,7 @@ struct q2d_handle {
   bool any_visc = false;
   bool any_irs = false;          // a live case smooths its residual (k_irs after each stage)
   bool any_hllepp = false;       // a live case uses HLLE++ or HLLC+: the stage kernel's RIE_T = -2 instantiation
+  bool any_det_or_hllem = false; // a live case uses HLLEM or the detector: not the RIE_T = -3 instantiation
   bool mixed_generic = false;    // mixed form with a MUSCL-SD / HLLEM / HLLE++ / HLLC+ case: Euler items run the
                                  // generic FP32-face kernel k_stage<float> (k_stage_mx has no room for them)
   bool bc_two_phase = false;     // a live block is one cell wide: ghost fill and BC fluxes in two launches
@@ -2493,9 +2500,16 @@ const void* stage_kernel(const q2d_handle* h, int pass) {
                     : reinterpret_cast<const void*>(&k_stage<float, -1, false, true>);
   if (h->precision_mixed)
     return pass < 2 ? reinterpret_cast<const void*>(&k_stage_mx<-1, kFuseDtMx, false>)
-                    : reinterpret_cast<const void*>(&k_stage<float, -1, false, true>);
-  return pass < 2 ? reinterpret_cast<const void*>(&k_stage<double, -1, kFuseDtFp64, false>)
-                  : reinterpret_cast<const void*>(&k_stage<double, -1, kFuseDtFp64, true>);
+                    : (h->any_det_or_hllem ? reinterpret_cast<const void*>(&k_stage<float, -1, false, true>)
+                                           : reinterpret_cast<const void*>(&k_stage<float, -3, false, true>));
+  if (h->any_hllepp)
+    return pass < 2 ? reinterpret_cast<const void*>(&k_stage<double, -2, kFuseDtFp64, false>)
+                    : reinterpret_cast<const void*>(&k_stage<double, -2, kFuseDtFp64, true>);
+  if (h->any_det_or_hllem)
+    return pass < 2 ? reinterpret_cast<const void*>(&k_stage<double, -1, kFuseDtFp64, false>)
+                    : reinterpret_cast<const void*>(&k_stage<double, -1, kFuseDtFp64, true>);
+  return pass < 2 ? reinterpret_cast<const void*>(&k_stage<double, -3, kFuseDtFp64, false>)
+                  : reinterpret_cast<const void*>(&k_stage<double, -3, kFuseDtFp64, true>);
 }
 
 // The captured step graphs hold the launch arguments; they are dropped whenever those change.
@@ -2706,14 +2720,18 @@ int launch_stage(q2d_handle* h, int irk, const StepScratch& scr) {
       else if (pass == 0 && !h->mixed_generic) Q2D_STAGE_MX_LAUNCH(-1, false);
       else if (h->any_hllepp && pass == 0) Q2D_STAGE_LAUNCH(-2, false);
       else if (h->any_hllepp) Q2D_STAGE_LAUNCH(-2, true);
-      else if (pass == 0) Q2D_STAGE_LAUNCH(-1, false);
-      else Q2D_STAGE_LAUNCH(-1, true);
+      else if (h->any_det_or_hllem && pass == 0) Q2D_STAGE_LAUNCH(-1, false);
+      else if (h->any_det_or_hllem) Q2D_STAGE_LAUNCH(-1, true);
+      else Q2D_STAGE_LAUNCH(-3, true);                 // laminar HLLC / HLLE cases
     } else if (h->any_hllepp) {                        // HLLE++ / HLLC+ (plain FP64 form only)
       if (pass == 0) Q2D_STAGE_LAUNCH(-2, false);
       else Q2D_STAGE_LAUNCH(-2, true);
-    } else {
+    } else if (h->any_det_or_hllem) {                  // HLLEM or the detector
       if (pass == 0) Q2D_STAGE_LAUNCH(-1, false);
       else Q2D_STAGE_LAUNCH(-1, true);
+    } else {                                           // HLLC and HLLE only
+      if (pass == 0) Q2D_STAGE_LAUNCH(-3, false);
+      else Q2D_STAGE_LAUNCH(-3, true);
     }
 #undef Q2D_STAGE_LAUNCH
 #undef Q2D_STAGE_MX_LAUNCH
@@ -2940,6 +2958,7 @@ int rebuild_descriptors(q2d_handle* h) {
   h->any_visc = false;
   h->any_irs = false;
   h->any_hllepp = false;
+  h->any_det_or_hllem = false;
   h->mixed_generic = false;
   h->bc_two_phase = false;
   for (int b = 0; b < h->nblocks; ++b)
@@ -2950,6 +2969,7 @@ int rebuild_descriptors(q2d_handle* h) {
     if (h->dc[c].model > 0) h->any_visc = true;
     if (h->dc[c].irs_beta > 0.0) h->any_irs = true;
     if (h->dc[c].riemann >= Q2D_RIEMANN_HLLEPP) h->any_hllepp = true;
+    if (h->dc[c].riemann >= Q2D_RIEMANN_HLLEM || h->dc[c].space_sd) h->any_det_or_hllem = true;
     if (h->precision_mixed && (h->dc[c].space_sd || h->dc[c].riemann >= Q2D_RIEMANN_HLLEM)) h->mixed_generic = true;
     if (h->dc[c].space_sd || h->dc[c].riemann >= Q2D_RIEMANN_HLLEPP)
       h->bc_two_phase = true;                          // the connection faces' detector reads other records' ghosts
