@@ -469,14 +469,20 @@ __device__ __forceinline__ double sd_tramel(double p, double p1, double p2, doub
 
 struct BCFlux { double F[4]; double V[4]; int mask; int vmask; };
 
+// Cases of a mixed-form handle whose Euler items run k_stage_mx: MUSCL-SD and the HLLEM / HLLE++ / HLLC+
+// solvers are not in that kernel, so such cases run the generic FP32-face kernel k_stage<float>.
+__host__ __device__ __forceinline__ bool mx_kernel_case(int model, int space_sd, int riemann) {
+  return model == 0 && !space_sd && riemann < Q2D_RIEMANN_HLLEM;
+}
+
 // A record's result, stored at the record's boundary cell (DevBC::dpos): the BCFlux struct (read by the
-// FP64 and laminar stage kernels, which add the faces in record order) and,
-// for the mixed Euler stage kernel, its convective flux with the mask applied, in FP32 (the precision of
-// that form's face fluxes), in the dense per-face array (a float4 per boundary cell, indexed like the
-// per-face record index), so that kernel reads a boundary cell's flux with one load and no indirection.
+// FP64, generic FP32-face and laminar stage kernels, which add the faces in record order) and, for the
+// mixed Euler stage kernel, its convective flux with the mask applied, in FP32 (the precision of that
+// form's face fluxes), in the dense per-face array (a float4 per boundary cell, indexed like the per-face
+// record index), so that kernel reads a boundary cell's flux with one load and no indirection.
 __device__ __forceinline__ void emit_bc(BCFlux* __restrict__ out, float4* __restrict__ dense, int dpos,
-                                        const BCFlux& res, bool euler) {
-  if (!(dense && euler)) out[dpos] = res;             // the mixed Euler stage kernel reads only the dense array
+                                        const BCFlux& res, bool mx) {
+  if (!(dense && mx)) out[dpos] = res;                // k_stage_mx reads only the dense array
   if (dense) {
     float f[4];
 #pragma unroll
@@ -620,7 +626,7 @@ __global__ __launch_bounds__(256) void k_bcflux(const DevBlock* __restrict__ blo
     // one species: Sel_L*rho_L/rho_L - Sel_R*rho_R/rho_R == 1, so Flux(1) = F_r*int_l
 #pragma unroll
     for (int v = 0; v < 4; ++v) res.F[v] = ((double)F[v] * il) * sgn;
-    emit_bc(out, dense, E.dpos, res, C.model == 0);
+    emit_bc(out, dense, E.dpos, res, mx_kernel_case(C.model, C.space_sd, C.riemann));
     return;
   }
 
@@ -638,7 +644,7 @@ __global__ __launch_bounds__(256) void k_bcflux(const DevBlock* __restrict__ blo
     Fb[3] = Fmass * (gd.cp * Tt + 0.5 * vmag * vmag);
 #pragma unroll
     for (int k = 0; k < 4; ++k) res.F[k] = sgn * Fb[k];
-    emit_bc(out, dense, E.dpos, res, C.model == 0);
+    emit_bc(out, dense, E.dpos, res, mx_kernel_case(C.model, C.space_sd, C.riemann));
     return;
   }
   if (ty == 406) {
@@ -682,7 +688,7 @@ __global__ __launch_bounds__(256) void k_bcflux(const DevBlock* __restrict__ blo
       Fb[3] = mdot * (0.5 * (Fp[1] * Fp[1] + Fp[2] * Fp[2]) + Hf);
 #pragma unroll
       for (int k = 0; k < 4; ++k) res.F[k] = sgn * Fb[k];
-      emit_bc(out, dense, E.dpos, res, C.model == 0);
+      emit_bc(out, dense, E.dpos, res, mx_kernel_case(C.model, C.space_sd, C.riemann));
       return;
     }
   }
@@ -750,7 +756,7 @@ __global__ __launch_bounds__(256) void k_bcflux(const DevBlock* __restrict__ blo
 #pragma unroll
   for (int v = 0; v < 4; ++v) res.F[v] = sgn * ((double)F[v] * il);
   if (res.mask == 0x6) { res.F[0] = 0.0; res.F[3] = 0.0; }
-  emit_bc(out, dense, E.dpos, res, C.model == 0);
+  emit_bc(out, dense, E.dpos, res, mx_kernel_case(C.model, C.space_sd, C.riemann));
 }
 
 // ---------------------------------------------------------------------------
@@ -2455,7 +2461,7 @@ struct q2d_handle {
   double* d_partials = nullptr;
   BCFlux* d_bcflux = nullptr;
   float4* d_bcdense = nullptr;   // per-face convective boundary fluxes (mixed form), FP32, one float4 per cell
-  int range[5] = {0, 0, 0, 0, 0}; // items sorted into [range[k], range[k+1]): k = 2*viscous + riemann
+  int range[5] = {0, 0, 0, 0, 0}; // items sorted into [range[k], range[k+1]) by item_pass()
   std::vector<DtRange> dtr;      // dt launch: units of up to kDtCells cells of one live block
   DtRange* d_dtr = nullptr;
   int dt_total = 0;
@@ -2463,8 +2469,9 @@ struct q2d_handle {
   bool any_irs = false;          // a live case smooths its residual (k_irs after each stage)
   bool any_hllepp = false;       // a live case uses HLLE++ or HLLC+: the stage kernel's RIE_T = -2 instantiation
   bool any_det_or_hllem = false; // a live case uses HLLEM or the detector: not the RIE_T = -3 instantiation
-  bool mixed_generic = false;    // mixed form with a MUSCL-SD / HLLEM / HLLE++ / HLLC+ case: Euler items run the
-                                 // generic FP32-face kernel k_stage<float> (k_stage_mx has no room for them)
+  bool mixed_generic = false;    // mixed form with a MUSCL-SD / HLLEM / HLLE++ / HLLC+ case: those cases' Euler
+                                 // items run the generic FP32-face kernel k_stage<float> (k_stage_mx has no
+                                 // room for them), the others keep k_stage_mx; steps run the dt pass
   bool bc_two_phase = false;     // a live block is one cell wide: ghost fill and BC fluxes in two launches
   std::vector<int> vrec;         // boundary records with a viscous flux (BC 300/302 of model-1 cases)
   int* d_vrec = nullptr;
@@ -2493,15 +2500,26 @@ struct q2d_handle {
 
 namespace {
 
-// The stage-kernel instantiation that runs the items of `pass` (2 * viscous + riemann).
+// The item group of case c: laminar cases 2 + (riemann != 0); Euler cases of a mixed-form handle 0 when they
+// run k_stage_mx, 1 when they need the generic FP32-face kernel; Euler cases of an FP64 handle riemann != 0.
+// FP64 Euler groups 0 and 1 share one launch, as do groups 2 and 3.
+int item_pass(const q2d_handle* h, int c) {
+  const DevCase& C = h->dc[c];
+  if (C.model > 0) return 2 + (C.riemann != 0 ? 1 : 0);
+  if (h->precision_mixed) return mx_kernel_case(C.model, C.space_sd, C.riemann) ? 0 : 1;
+  return C.riemann != 0 ? 1 : 0;
+}
+
+// The stage-kernel instantiation that runs the items of group `pass` (item_pass()).
 const void* stage_kernel(const q2d_handle* h, int pass) {
-  if (h->precision_mixed && h->mixed_generic)
-    return pass < 2 ? reinterpret_cast<const void*>(&k_stage<float, -1, false, false>)
-                    : reinterpret_cast<const void*>(&k_stage<float, -1, false, true>);
-  if (h->precision_mixed)
-    return pass < 2 ? reinterpret_cast<const void*>(&k_stage_mx<-1, kFuseDtMx, false>)
-                    : (h->any_det_or_hllem ? reinterpret_cast<const void*>(&k_stage<float, -1, false, true>)
-                                           : reinterpret_cast<const void*>(&k_stage<float, -3, false, true>));
+  if (h->precision_mixed) {
+    if (pass == 0) return reinterpret_cast<const void*>(&k_stage_mx<-1, kFuseDtMx, false>);
+    if (pass == 1) return h->any_hllepp ? reinterpret_cast<const void*>(&k_stage<float, -2, false, false>)
+                                        : reinterpret_cast<const void*>(&k_stage<float, -1, false, false>);
+    if (h->any_hllepp) return reinterpret_cast<const void*>(&k_stage<float, -2, false, true>);
+    return h->any_det_or_hllem ? reinterpret_cast<const void*>(&k_stage<float, -1, false, true>)
+                               : reinterpret_cast<const void*>(&k_stage<float, -3, false, true>);
+  }
   if (h->any_hllepp)
     return pass < 2 ? reinterpret_cast<const void*>(&k_stage<double, -2, kFuseDtFp64, false>)
                     : reinterpret_cast<const void*>(&k_stage<double, -2, kFuseDtFp64, true>);
@@ -2577,7 +2595,7 @@ int balanced_rows(const std::vector<std::pair<int, int>>& strips, long long slot
 
 int build_items(q2d_handle* h) {
   h->items.clear();
-  // items grouped by (viscous, Riemann solver) — one kernel instantiation each — then by case
+  // items grouped by item_pass() — one kernel instantiation each — then by case
   for (int pass = 0; pass < 4; ++pass) {
     h->range[pass] = (int)h->items.size();
     // Form defaults, measured on 4 x 1024^2: the FP64 stage kernel (2 waves per SIMD, issue-bound)
@@ -2589,7 +2607,7 @@ int build_items(q2d_handle* h) {
     if (H <= 0) {                                  // balanced for this launch on this device
       std::vector<std::pair<int, int>> strips;
       for (int c = 0; c < h->ncases; ++c) {
-        if (!h->case_live[c] || (h->dc[c].riemann != 0) != ((pass & 1) != 0) || (h->dc[c].model > 0) != (pass >= 2)) continue;
+        if (!h->case_live[c] || item_pass(h, c) != pass) continue;
         for (int b = 0; b < h->nblocks; ++b)
           if (h->hb[b].case_id == c) strips.emplace_back((h->hb[b].ni + kOwn - 1) / kOwn, h->hb[b].nj);
       }
@@ -2606,7 +2624,7 @@ int build_items(q2d_handle* h) {
     }
     h->rows_used[pass] = H;
     for (int c = 0; c < h->ncases; ++c) {
-      if (!h->case_live[c] || (h->dc[c].riemann != 0) != ((pass & 1) != 0) || (h->dc[c].model > 0) != (pass >= 2)) continue;
+      if (!h->case_live[c] || item_pass(h, c) != pass) continue;
       h->dc[c].item_begin = (int)h->items.size();
       for (int b = 0; b < h->nblocks; ++b) {
         if (h->hb[b].case_id != c) continue;
@@ -2692,9 +2710,11 @@ template <typename FT, bool FUSE>
 int launch_stage(q2d_handle* h, int irk, const StepScratch& scr) {
   const int tpb = kWave * h->waves_per_group;
   for (int pass = 0; pass < 4; ++pass) {
-    // the two Riemann solvers' passes of each kernel family (Euler 0-1, laminar 2-3) share one launch
-    if (pass == 1 || pass == 3) continue;
-    const int lo = h->range[pass], hi = h->range[pass + 2];
+    // groups 0-1 (FP64 Euler) and 2-3 (laminar) share one launch; the mixed form's Euler groups 0
+    // (k_stage_mx) and 1 (generic FP32-face kernel) launch separately
+    const bool split = sizeof(FT) == 4 && pass < 2;
+    if ((pass == 1 && !split) || pass == 3) continue;
+    const int lo = h->range[pass], hi = split ? h->range[pass + 1] : h->range[pass + 2];
     if (hi <= lo) continue;
     int groups = (hi - lo + h->waves_per_group - 1) / h->waves_per_group;
     if (h->max_groups > 0) groups = std::min(groups, h->max_groups);
@@ -2715,12 +2735,12 @@ int launch_stage(q2d_handle* h, int irk, const StepScratch& scr) {
     hipLaunchKernelGGL((k_stage_mx<RIE, kFuseDtMx, IRS>), dim3(groups), dim3(tpb), 0, h->stream, h->d_blocks,      \
                        h->d_cases, h->d_ctrl, h->d_items, lo, hi, scr.dense, scr.partials, irk, h->xcd_remap, ts,     \
                        slot)
-    if constexpr (sizeof(FT) == 4) {                   // mixed form: Euler items run k_stage_mx
-      if (pass == 0 && !h->mixed_generic && h->any_irs) Q2D_STAGE_MX_LAUNCH(-1, true);
-      else if (pass == 0 && !h->mixed_generic) Q2D_STAGE_MX_LAUNCH(-1, false);
-      else if (h->any_hllepp && pass == 0) Q2D_STAGE_LAUNCH(-2, false);
+    if constexpr (sizeof(FT) == 4) {                   // mixed form: plain Euler items run k_stage_mx
+      if (pass == 0 && h->any_irs) Q2D_STAGE_MX_LAUNCH(-1, true);
+      else if (pass == 0) Q2D_STAGE_MX_LAUNCH(-1, false);
+      else if (h->any_hllepp && pass == 1) Q2D_STAGE_LAUNCH(-2, false);
       else if (h->any_hllepp) Q2D_STAGE_LAUNCH(-2, true);
-      else if (h->any_det_or_hllem && pass == 0) Q2D_STAGE_LAUNCH(-1, false);
+      else if (pass == 1) Q2D_STAGE_LAUNCH(-1, false);
       else if (h->any_det_or_hllem) Q2D_STAGE_LAUNCH(-1, true);
       else Q2D_STAGE_LAUNCH(-3, true);                 // laminar HLLC / HLLE cases
     } else if (h->any_hllepp) {                        // HLLE++ / HLLC+ (plain FP64 form only)
@@ -2801,10 +2821,9 @@ int launch_step(q2d_handle* h, const StepScratch& scr) {
     if (h->nbc > 0) {
       for (int ph = h->bc_two_phase ? 1 : 0; ph <= (h->bc_two_phase ? 2 : 0); ++ph) {
         const int begin_cases = (fold_begin && irk == 1 && ph <= 1) ? h->ncases : 0;
-        if (h->precision_mixed)                        // the dense FP32 face array feeds k_stage_mx only
+        if (h->precision_mixed)                        // the dense FP32 face array feeds k_stage_mx
           hipLaunchKernelGGL(k_bcflux<float>, dim3(bgroups), dim3(64), 0, h->stream, h->d_blocks, h->d_cases,
-                             h->d_ctrl, h->d_bc, h->nbc, irk, scr.bcflux, ph, begin_cases,
-                             h->mixed_generic ? nullptr : scr.dense);
+                             h->d_ctrl, h->d_bc, h->nbc, irk, scr.bcflux, ph, begin_cases, scr.dense);
         else
           hipLaunchKernelGGL(k_bcflux<double>, dim3(bgroups), dim3(64), 0, h->stream, h->d_blocks, h->d_cases,
                              h->d_ctrl, h->d_bc, h->nbc, irk, scr.bcflux, ph, begin_cases, scr.dense);

@@ -99,3 +99,24 @@ def test_mixed_form_within_tolerance(oracle_mod, rie, sd, irs):
     a, r = g.state(0)[2:-2, 2:-2, 0], o.state(0)[2:-2, 2:-2, 0]
     err = np.abs(a - r).sum() / np.abs(r).sum()
     assert 0.0 < err <= 1e-5, err
+
+
+@pytest.mark.parametrize("rie,sd", [("HLLC", True), ("HLLC+", False)])
+def test_mixed_2x2_blocks_detector_within_tolerance(oracle_mod, rie, sd):
+    """Mixed form on 2x2 blocks (BC 101 connections, y-periodic BC 201): the detector of MUSCL-SD and HLLC+
+    reads across block connections in the FP32 boundary-flux kernel; relative L1 density <= 1e-5 of the
+    FP64 oracle on every block."""
+    def make(prec):
+        pb = cases.planar_shock(seed=4, n=48, riemann=rie, precision=prec, steps=100)
+        pb.cases[0].space_sd = sd
+        return pb
+    print(f"mixed 2x2 detector case: {rie} sd={sd}", flush=True)
+    g = GpuSolver(make("mixed"))
+    go = g.run(100)
+    o = oracle_mod.Oracle(make("fp64"))
+    rc, oo = o.run(100, stop_on_todo=False)
+    assert rc == 0 and go[0]["iter"] == oo[0]["iter"] == 100
+    for b in range(4):
+        a, r = g.state(b)[2:-2, 2:-2, 0], o.state(b)[2:-2, 2:-2, 0]
+        err = np.abs(a - r).sum() / np.abs(r).sum()
+        assert 0.0 < err <= 1e-5, (b, err)
