@@ -68,9 +68,17 @@ class Result:
 
 def ramp_request(seed, smin=0.0, smax=1.0, precision="fp64", res_drop=1e-6, itermax=60000):
     """One compression-ramp solve: wedge ~ U[5, 15] deg, inflow Mach ~ U[1.5, 4], grid log-uniform on the
-    diagonal from 64x64 (s = 0) to 4096x1024 (s = 1), local dt, SSP-RK3, run to a `res_drop` drop of
-    residual(1) (capped at `itermax` steps).  FP64 by default: with FP32 face fluxes the residual of a
-    steady solve stalls on the flux rounding well before a 1e-6 drop."""
+    diagonal from 64x64 (s = 0) to 4096x1024 (s = 1), run to a `res_drop` drop of residual(1) (capped at
+    `itermax` steps).  Numerics of the reference's steady ramp-channel case
+    (test/2D/euler/ramp-channel/input.ini) except its shock detector: MUSCL with minmod, HLLE, SSP-RK3,
+    local dt, CFL 0.9 reached by a linear rise over the first 100 steps, implicit residual smoothing with
+    beta 0.5.  Measured on the 24-request stream of seed 0 (30,000-step cap): these settings converge 23 of
+    24 to the 1e-6 drop; plain MUSCL at CFL 0.5 without smoothing 22 (one stays in a limit cycle at a 1e-3
+    drop, and the others take up to 1.4x the steps); MUSCL-SD, with or without smoothing, 7-9 (the
+    detector's switching keeps the residual at 1e-3 to 1: the reference's own ramp-channel case, with
+    MUSCL-SD, ends at 2.5e-4).  The one that never converges is a channel that unstarts (stall_report).
+    FP64 by default: with FP32 face fluxes the residual of a steady solve stalls on the flux rounding well
+    before a 1e-6 drop."""
     rng = np.random.default_rng(seed)
     s = rng.uniform(smin, smax)
     wedge = rng.uniform(5.0, 15.0)
@@ -78,10 +86,62 @@ def ramp_request(seed, smin=0.0, smax=1.0, precision="fp64", res_drop=1e-6, iter
     ni = max(64, int(round(64 * 64 ** s / 8.0)) * 8)
     nj = max(64, int(round(64 * 16 ** s / 8.0)) * 8)
     pb = cases.ramp_channel(ni=ni, nj=nj, wedge_deg=wedge, mach=mach, precision=precision, n_rk=3,
-                            limiter="minmod", riemann="HLLE", itermax=itermax)
-    pb.cases[0] = replace(pb.cases[0], res_drop=res_drop)
+                            limiter="minmod", riemann="HLLE", cfl=0.9, itermax=itermax)
+    pb.cases[0] = replace(pb.cases[0], res_drop=res_drop, irs_beta=0.5, ramp_iter=100)
     pb.meta.update({"ni": ni, "nj": nj, "wedge": wedge, "mach": mach})
     return pb
+
+
+def oblique_shock(mach, wedge_deg, gamma=1.4):
+    """Weak attached oblique shock on a wedge: (shock angle deg, downstream Mach), or None when the wedge
+    exceeds the detachment angle.  From the theta-beta-Mach relation
+    tan(theta) = 2 cot(beta) (M^2 sin^2 beta - 1) / (M^2 (gamma + cos 2 beta) + 2), solved for the smaller
+    root by bisection between the Mach angle and the angle of maximum deflection."""
+    th = math.radians(wedge_deg)
+
+    def defl(b):
+        return math.atan(2.0 / math.tan(b) * (mach * mach * math.sin(b) ** 2 - 1.0)
+                         / (mach * mach * (gamma + math.cos(2.0 * b)) + 2.0))
+    lo, hi = math.asin(1.0 / mach), 0.5 * math.pi
+    for _ in range(200):                          # golden-section search for the maximum deflection
+        a, b = hi - 0.618034 * (hi - lo), lo + 0.618034 * (hi - lo)
+        if defl(a) < defl(b):
+            lo = a
+        else:
+            hi = b
+    bmax = 0.5 * (lo + hi)
+    if th > defl(bmax):
+        return None
+    lo, hi = math.asin(1.0 / mach), bmax
+    for _ in range(200):
+        mid = 0.5 * (lo + hi)
+        if defl(mid) < th:
+            lo = mid
+        else:
+            hi = mid
+    beta = 0.5 * (lo + hi)
+    mn1 = mach * math.sin(beta)
+    mn2 = math.sqrt((mn1 * mn1 + 2.0 / (gamma - 1.0)) / (2.0 * gamma / (gamma - 1.0) * mn1 * mn1 - 1.0))
+    return math.degrees(beta), mn2 / math.sin(beta - th)
+
+
+def stall_report(request, result, gamma=1.4):
+    """Why a ramp request did not converge: the weak-shock downstream Mach of the theta-beta-Mach
+    relation, and the retired state's Mach numbers.  A subsonic first column means the shock left the
+    domain through the supersonic inflow: the channel unstarted, and with extrapolated (BC 400) outflow
+    and top boundaries a subsonic channel has no steady state to converge to."""
+    m = request.problem.meta
+    th = oblique_shock(m["mach"], m["wedge"], gamma)
+    rep = {"rid": request.rid, "iters": result.iters, "status": result.status, "mach": m["mach"],
+           "wedge_deg": m["wedge"], "theory_M2": None if th is None else th[1]}
+    if result.state:
+        P = result.state[0]
+        M = np.sqrt(P[..., 1] ** 2 + P[..., 2] ** 2) / np.sqrt(gamma * P[..., 3] / P[..., 0])
+        rep.update(min_mach=float(M.min()), subsonic_fraction=float((M < 1.0).mean()),
+                   inflow_column_subsonic=bool((M[:, 0] < 1.0).any()))
+        rep["cause"] = ("unstarted: the shock left through the inflow and the channel is subsonic"
+                        if rep["inflow_column_subsonic"] else "not converged within the iteration cap")
+    return rep
 
 
 def poisson_stream(n, rate, seed=0, **kw):
@@ -107,6 +167,8 @@ class Server:
         self.g.close()
 
     def serve(self, requests, keep_state=True, clock=time.perf_counter, log=None):
+        """Serve `requests`; returns (results by rid, stats).  keep_state: True keeps every retired state,
+        "stalled" only those of requests that did not converge, False none."""
         pending = sorted(requests, key=lambda r: r.arrival)
         for r in pending:
             if footprint_bytes(r.problem) > self.budget:
@@ -139,8 +201,9 @@ class Server:
                 cell_updates += r.problem.ncells() * (o["iter"] - last_iter)
                 active[c] = (r, tadm, b0, nb, fp, o["iter"])
                 if o["todo"] == 3 or o["status"] != 0:
-                    st = [self.g.state(b0 + k)[2:-2, 2:-2].copy() for k in range(nb)] if keep_state else []
                     conv = o["status"] == 0 and o["iter"] < r.problem.cases[0].itermax
+                    keep = keep_state is True or (keep_state == "stalled" and not conv)
+                    st = [self.g.state(b0 + k)[2:-2, 2:-2].copy() for k in range(nb)] if keep else []
                     results.append(Result(r.rid, r.arrival, tadm, tnow, o["iter"], o["status"], conv,
                                           r.problem.ncells(), st))
                     self.g.remove_case(c)

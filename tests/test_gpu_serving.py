@@ -54,8 +54,13 @@ def test_relative_drop_stop_matches_oracle(oracle_mod):
     assert np.array_equal(g.state(0)[2:-2, 2:-2], o.state(0)[2:-2, 2:-2])
 
 
-def test_served_results_equal_alone():
-    reqs = serving.poisson_stream(6, rate=40.0, seed=3, smax=0.3, res_drop=1e-3, itermax=4000, precision="mixed")
+@pytest.mark.parametrize("precision,res_drop", [("fp64", 1e-6), ("mixed", 1e-3)])
+def test_served_results_equal_alone(precision, res_drop):
+    """The served configuration (FP64 to a 1e-6 drop, MUSCL-SD, residual smoothing, CFL rise) and the mixed
+    form: every request's iteration count and state are the same alone and in the mix, bit for bit."""
+    print(f"serving alone == mix: {precision} drop {res_drop}", flush=True)
+    reqs = serving.poisson_stream(6, rate=40.0, seed=3, smax=0.3, res_drop=res_drop, itermax=4000,
+                                  precision=precision)
     budget = 2 * max(serving.footprint_bytes(r.problem) for r in reqs)     # forces queueing
     srv = serving.Server(budget_bytes=budget, chunk_steps=40)
     res, stats = srv.serve(reqs)
@@ -73,8 +78,8 @@ def test_served_results_equal_alone():
 def test_removing_unfused_cases_keeps_the_survivors_dt():
     """A handle whose last laminar case (FP64) or last generic-kernel case (mixed: HLLEM) is removed switches
     to steps that fuse the next step's dt: the first such step must run the dt pass, not take a dt that no
-    stage reduced.  The survivor matches its solo run (FP64: bit for bit; mixed: the generic kernel ran its
-    first steps, so within the form's tolerance)."""
+    stage reduced.  The survivor matches its solo run bit for bit in both forms (in the mixed form it ran
+    k_stage_mx throughout, with its dt from the dt pass while the others were there)."""
     for prec in ("fp64", "mixed"):
         a = cases.riemann_planar(seed=5, ni=200, nj=8, steps=10 ** 6, precision=prec)
         others = [cases.axisymmetric_pipe(seed=1, ni=64, nj=24, first_cell=5e-4, reynolds=1e4, precision=prec,
@@ -91,14 +96,39 @@ def test_removing_unfused_cases_keeps_the_survivors_dt():
             g.remove_case(c)
         out = g.run(10)[ca]
         assert out["iter"] == 20
-        if prec == "fp64":
-            assert out["dtglobal"] == solo["dtglobal"] and out["time"] == solo["time"]
-            assert np.array_equal(g.state(0)[2:-2, 2:-2], alone.state(0)[2:-2, 2:-2])
-        else:
-            assert abs(out["time"] - solo["time"]) <= 1e-6 * solo["time"]
-            assert abs(out["dtglobal"] - solo["dtglobal"]) <= 1e-4 * solo["dtglobal"]
-            x, y = g.state(0)[2:-2, 2:-2, 0], alone.state(0)[2:-2, 2:-2, 0]
-            assert np.abs(x - y).sum() / np.abs(y).sum() <= 1e-5
+        assert out["dtglobal"] == solo["dtglobal"] and out["time"] == solo["time"], prec
+        assert np.array_equal(g.state(0)[2:-2, 2:-2], alone.state(0)[2:-2, 2:-2]), prec
+
+
+def test_mixed_plain_case_independent_of_generic_batch_mates():
+    """In the mixed form, cases with MUSCL-SD, HLLEM or HLLC+ run the generic FP32-face stage kernel and the
+    others keep k_stage_mx in the same step (their boundary fluxes go to the dense FP32 array, the others'
+    to the per-record array), so a plain case gives the same bits alone and beside them: the handle then
+    runs the dt pass every step, whose formula is the fused dt's.  The generic cases stay within the form's
+    tolerance of their solo runs (their kernel instantiation follows the handle's solver set)."""
+    plain = [cases.riemann_planar(seed=11, ni=200, nj=8, steps=10 ** 6, precision="mixed"),
+             cases.planar_shock(seed=2, n=40, steps=10 ** 6, precision="mixed")]
+    sd = cases.ramp_channel(ni=96, nj=48, precision="mixed", cfl=0.9, itermax=10 ** 6)
+    sd.cases[0].space_sd, sd.cases[0].irs_beta, sd.cases[0].ramp_iter = True, 0.5, 100
+    generic = [sd, cases.ramp_channel(ni=64, nj=32, riemann="HLLC+", precision="mixed", itermax=10 ** 6),
+               cases.ramp_channel(ni=64, nj=40, riemann="HLLEM", precision="mixed", itermax=10 ** 6)]
+    g = GpuSolver(cases.Problem())
+    ids = [g.add_cases(p) for p in plain + generic]
+    out = g.run(40)
+    for n, ((c, b0), p) in enumerate(zip(ids, plain + generic)):
+        solo = GpuSolver(p)
+        so = solo.run(40)[0]
+        assert out[c]["iter"] == so["iter"] == 40 and out[c]["status"] == 0
+        for k in range(len(p.blocks)):
+            x, y = g.state(b0 + k)[2:-2, 2:-2], solo.state(k)[2:-2, 2:-2]
+            if n < len(plain):
+                assert np.array_equal(x, y), (p.meta.get("name"), k)
+                assert out[c]["dtglobal"] == so["dtglobal"] and out[c]["time"] == so["time"]
+            else:
+                err = np.abs(x[..., 0] - y[..., 0]).sum() / np.abs(y[..., 0]).sum()
+                assert err <= 1e-5, (p.meta.get("name"), err)
+        solo.close()
+    g.close()
 
 
 def test_footprint_equals_device_bytes():
