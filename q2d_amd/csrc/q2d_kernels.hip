@@ -253,6 +253,21 @@ __global__ void k_dt_finish(DevCtrl* ctrl, const DevCase* cases, int ncases) {
 // ---------------------------------------------------------------------------
 // Compute_dt — Mod_dt.f90:10-104 (one wavefront per work item)
 // ---------------------------------------------------------------------------
+// The VNN limit of Compute_dt (Mod_dt.f90:88-102; laminar: mie = 0), after the convective limits.  The
+// dt pass and the fused dt of the stage kernels' last stage both apply exactly this.
+__device__ __forceinline__ double vnn_limit(const DevCase& C, double dtmin, double rho, double p, double dlx,
+                                            double dly) {
+  const double mil = mu_lam(C, p / (rho * C.R));
+  const double mi = 0.0 + mil;
+  double dt = rho * (dlx * dlx) * C.vnn / mi;
+  if (dt < dtmin) dtmin = dt;
+  if (C.ndir == 2) {
+    dt = rho * (dly * dly) * C.vnn / mi;
+    if (dt < dtmin) dtmin = dt;
+  }
+  return dtmin;
+}
+
 __device__ __forceinline__ double cell_dt(const DevCase& C, const DevCtrl& K, double rho, double u, double v, double p,
                                           double m11, double m21, double m12, double m22, double dlx, double dly) {
   const double snd = sqrt(C.gam * p / rho);
@@ -271,16 +286,7 @@ __device__ __forceinline__ double cell_dt(const DevCase& C, const DevCtrl& K, do
     const double dt = dly / lambda * C.cfl;
     if (dt < dtmin) dtmin = dt;
   }
-  if (C.model > 0) {                               // VNN limit — Mod_dt.f90:88-102 (laminar: mie = 0)
-    const double mil = mu_lam(C, p / (rho * C.R));
-    const double mi = 0.0 + mil;
-    double dt = rho * (dlx * dlx) * C.vnn / mi;
-    if (dt < dtmin) dtmin = dt;
-    if (C.ndir == 2) {
-      dt = rho * (dly * dly) * C.vnn / mi;
-      if (dt < dtmin) dtmin = dt;
-    }
-  }
+  if (C.model > 0) dtmin = vnn_limit(C, dtmin, rho, p, dlx, dly);
   if (K.iter < C.ramp_iter) dtmin = dtmin * (double)K.iter / (double)C.ramp_iter;   // Mod_dt.f90:43
   return dtmin;
 }
@@ -1573,6 +1579,7 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
                 const double dtd = ddiv_rn(dlyA, lambda) * C.cfl;
                 if (dtd < dtmin) dtmin = dtd;
               }
+              if (VISC && C.model > 0) dtmin = vnn_limit(C, dtmin, qn2[0], qn2[3], dlx, dlyA);
               const int iter_next = kiter + 1;
               if (iter_next < C.ramp_iter) dtmin = dtmin * (double)iter_next / (double)C.ramp_iter;
               if (!tacc) gpw(B.dtl[kdtsel ^ 1])[oA] = dtmin;
@@ -2120,8 +2127,9 @@ __device__ __forceinline__ void irs_i_at(const DevBlock* __restrict__ blocks, co
   }
 }
 
-// Compute_dt (Mod_dt.f90:59-86) of an Euler cell: the form the stage kernel fuses into its last stage.
-__device__ __forceinline__ double cell_dt_euler(const DevCase& C, int iter, const double q[4], const double mm[4],
+// Compute_dt (Mod_dt.f90:59-104) of a cell of an FP64-geometry block: the form the stage kernel fuses
+// into its last stage (with the VNN limit for laminar cases).
+__device__ __forceinline__ double cell_dt_fused(const DevCase& C, int iter, const double q[4], const double mm[4],
                                                 double dlx, double dly) {
   const double snd = sqrt(C.gam * q[3] / q[0]);
   double dtmin = 1e8;
@@ -2141,6 +2149,7 @@ __device__ __forceinline__ double cell_dt_euler(const DevCase& C, int iter, cons
     const double dtd = dly / lambda * C.cfl;
     if (dtd < dtmin) dtmin = dtd;
   }
+  if (C.model > 0) dtmin = vnn_limit(C, dtmin, q[0], q[3], dlx, dly);
   if (iter < C.ramp_iter) dtmin = dtmin * (double)iter / (double)C.ramp_iter;
   return dtmin;
 }
@@ -2315,7 +2324,7 @@ __global__ __launch_bounds__(256) void k_irs(const DevBlock* __restrict__ blocks
                                g4.z, g4.w);
           } else {
             const double mm[4] = {gp(B.m11)[o], gp(B.m11)[pl + o], gp(B.m11)[2 * pl + o], gp(B.m11)[3 * pl + o]};
-            dtmin = cell_dt_euler(C, kiter + 1, qn2, mm, gp(B.dlx)[o], gp(B.dly)[o]);
+            dtmin = cell_dt_fused(C, kiter + 1, qn2, mm, gp(B.dlx)[o], gp(B.dly)[o]);
           }
           if (!tacc) gpw(B.dtl[kdtsel ^ 1])[o] = dtmin;
           if (dtmin < dtnext) dtnext = dtmin;
@@ -2516,9 +2525,9 @@ const void* stage_kernel(const q2d_handle* h, int pass) {
     if (pass == 0) return reinterpret_cast<const void*>(&k_stage_mx<-1, kFuseDtMx, false>);
     if (pass == 1) return h->any_hllepp ? reinterpret_cast<const void*>(&k_stage<float, -2, false, false>)
                                         : reinterpret_cast<const void*>(&k_stage<float, -1, false, false>);
-    if (h->any_hllepp) return reinterpret_cast<const void*>(&k_stage<float, -2, false, true>);
-    return h->any_det_or_hllem ? reinterpret_cast<const void*>(&k_stage<float, -1, false, true>)
-                               : reinterpret_cast<const void*>(&k_stage<float, -3, false, true>);
+    if (h->any_hllepp) return reinterpret_cast<const void*>(&k_stage<float, -2, kFuseDtMx, true>);
+    return h->any_det_or_hllem ? reinterpret_cast<const void*>(&k_stage<float, -1, kFuseDtMx, true>)
+                               : reinterpret_cast<const void*>(&k_stage<float, -3, kFuseDtMx, true>);
   }
   if (h->any_hllepp)
     return pass < 2 ? reinterpret_cast<const void*>(&k_stage<double, -2, kFuseDtFp64, false>)
@@ -2731,6 +2740,10 @@ int launch_stage(q2d_handle* h, int irk, const StepScratch& scr) {
     hipLaunchKernelGGL((k_stage<FT, RIE, FUSE, VISC>), dim3(groups), dim3(tpb), 0, h->stream, h->d_blocks,        \
                        h->d_cases, h->d_ctrl, h->d_items, lo, hi, bf, bq, scr.partials, irk, h->xcd_remap, ts,   \
                        slot)
+#define Q2D_STAGE_LAM_MX_LAUNCH(RIE)                                                                              \
+    hipLaunchKernelGGL((k_stage<float, RIE, kFuseDtMx, true>), dim3(groups), dim3(tpb), 0, h->stream, h->d_blocks,  \
+                       h->d_cases, h->d_ctrl, h->d_items, lo, hi, bf, bq, scr.partials, irk, h->xcd_remap, ts,   \
+                       slot)
 #define Q2D_STAGE_MX_LAUNCH(RIE, IRS)                                                                             \
     hipLaunchKernelGGL((k_stage_mx<RIE, kFuseDtMx, IRS>), dim3(groups), dim3(tpb), 0, h->stream, h->d_blocks,      \
                        h->d_cases, h->d_ctrl, h->d_items, lo, hi, scr.dense, scr.partials, irk, h->xcd_remap, ts,     \
@@ -2739,10 +2752,10 @@ int launch_stage(q2d_handle* h, int irk, const StepScratch& scr) {
       if (pass == 0 && h->any_irs) Q2D_STAGE_MX_LAUNCH(-1, true);
       else if (pass == 0) Q2D_STAGE_MX_LAUNCH(-1, false);
       else if (h->any_hllepp && pass == 1) Q2D_STAGE_LAUNCH(-2, false);
-      else if (h->any_hllepp) Q2D_STAGE_LAUNCH(-2, true);
+      else if (h->any_hllepp) Q2D_STAGE_LAM_MX_LAUNCH(-2);
       else if (pass == 1) Q2D_STAGE_LAUNCH(-1, false);
-      else if (h->any_det_or_hllem) Q2D_STAGE_LAUNCH(-1, true);
-      else Q2D_STAGE_LAUNCH(-3, true);                 // laminar HLLC / HLLE cases
+      else if (h->any_det_or_hllem) Q2D_STAGE_LAM_MX_LAUNCH(-1);
+      else Q2D_STAGE_LAM_MX_LAUNCH(-3);                // laminar HLLC / HLLE cases
     } else if (h->any_hllepp) {                        // HLLE++ / HLLC+ (plain FP64 form only)
       if (pass == 0) Q2D_STAGE_LAUNCH(-2, false);
       else Q2D_STAGE_LAUNCH(-2, true);
@@ -2754,6 +2767,7 @@ int launch_stage(q2d_handle* h, int irk, const StepScratch& scr) {
       else Q2D_STAGE_LAUNCH(-3, true);
     }
 #undef Q2D_STAGE_LAUNCH
+#undef Q2D_STAGE_LAM_MX_LAUNCH
 #undef Q2D_STAGE_MX_LAUNCH
   }
   return Q2D_OK;
@@ -2788,10 +2802,12 @@ int collect_timing(q2d_handle* h) {
   return reset_timing(h);
 }
 
-// Steps of this handle evaluate the next step's dt in their last stage (Euler cases on the fast stage
-// kernels); otherwise every step runs the dt pass.
+// Steps of this handle evaluate the next step's dt in their last stage; otherwise every step runs the dt
+// pass.  Every FP64 stage kernel fuses it (the laminar ones with the VNN limit); in the mixed form
+// k_stage_mx and the laminar k_stage<float> do, the generic Euler k_stage<float> does not (its cases'
+// dt formula is k_stage_mx's, from FP32 geometry).
 bool fused_handle(const q2d_handle* h) {
-  return !h->any_visc && !h->mixed_generic && (h->precision_mixed ? kFuseDtMx : kFuseDtFp64);
+  return h->precision_mixed ? (kFuseDtMx && !h->mixed_generic) : kFuseDtFp64;
 }
 
 int launch_step(q2d_handle* h, const StepScratch& scr) {
