@@ -172,6 +172,18 @@ typedef struct q2d_case_desc {
                                time-accurate run, the solution's SOLUTIONTIME on a restart of one, -1 for
                                a steady run; time-accurate steps add dtglobal to it, and the
                                time-threshold test reads it */
+  const double *thermo;     /* NULL: calorically perfect gas (gas_cp constant, the closed forms).  Else
+                               the one species' property tables on integer kelvins, FLINT's native
+                               tabulated mode (Load_ThermoTransport.f90:15-71 reads INPUT/thermo.dat and
+                               transport.dat): four planes of thermo_n doubles, node k at T = k K, k = 0 ..
+                               thermo_n - 1: enthalpy h, cp, viscosity mu, conductivity k (mu and k are
+                               read by laminar cases only).  Properties are interpolated linearly, from
+                               node int(T) (Lib_Riemann_HLL.f90:257-282); T beyond the last interval
+                               extrapolates the end interval.  gas_R stays the species' R = Runiv / W;
+                               gas_cp is not read.  The plain FP64 form only: the mixed form rejects it
+                               with Q2D_E_UNSUPPORTED.  Copied at create / add_cases. */
+  int32_t    thermo_n;      /* nodes per plane (>= 2) when thermo != NULL */
+  int32_t    reserved2;
 } q2d_case_desc;
 
 typedef struct q2d_problem {

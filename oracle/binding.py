@@ -66,6 +66,11 @@ def load(path=None):
     lib.q2do_bc_inflow_flux.restype = C.c_int
     lib.q2do_bc_outflow_flux.argtypes = [C.c_double, C.c_double, C.c_int, dp] + [C.c_double] * 5 + [dp]
     lib.q2do_bc_outflow_flux.restype = C.c_int
+    lib.q2do_unit_tables.argtypes = [dp, C.c_int]
+    lib.q2do_thermo_eval.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double]
+    lib.q2do_thermo_eval.restype = C.c_double
+    lib.q2do_prim2cons.argtypes = [C.c_double, C.c_double, dp, dp]
+    lib.q2do_cons2prim.argtypes = [C.c_double, C.c_double, dp, C.c_double, dp]
     if path is None:
         _LIB = lib
     return lib
@@ -77,7 +82,7 @@ def _dp(a):
 
 # planted mistakes (q2d_oracle.h Q2DO_PLANT_*): a test shows it detects each one
 PLANTS = {"axi_sign": 1, "axi_energy": 2, "inflow_angle": 3, "inflow_rho": 4, "outflow_riem": 5,
-          "outflow_isen": 6, "ramp": 7}
+          "outflow_isen": 6, "ramp": 7, "tab_node": 8, "tab_tofe": 9}
 
 
 @contextmanager
@@ -89,6 +94,47 @@ def planted(name):
         yield
     finally:
         lib.q2do_plant(old)
+
+
+_UNIT_TAB = None
+
+
+@contextmanager
+def unit_tables(tab):
+    """Run the unit entry points (riemann, roe_averages, convective_flux, the BC fluxes, thermo) in table
+    mode with `tab` ((4, n) planes h, cp, mu, k), or in the closed form for None."""
+    global _UNIT_TAB
+    lib = load()
+    old = _UNIT_TAB
+    _UNIT_TAB = None if tab is None else np.ascontiguousarray(tab, dtype=np.float64)
+    lib.q2do_unit_tables(None if _UNIT_TAB is None else _dp(_UNIT_TAB), 0 if _UNIT_TAB is None else _UNIT_TAB.shape[1])
+    try:
+        yield
+    finally:
+        _UNIT_TAB = old
+        lib.q2do_unit_tables(None if old is None else _dp(old), 0 if old is None else old.shape[1])
+
+
+THERMO = {"h": 0, "cp": 1, "mu": 2, "k": 3, "gamma": 4, "e": 5, "T_of_e": 6, "sound": 7}
+
+
+def thermo(what, R, cp, a, b=0.0):
+    """One closure value (THERMO names), in the mode unit_tables set."""
+    return load().q2do_thermo_eval(THERMO[what], R, cp, float(a), float(b))
+
+
+def prim2cons(R, cp, prim):
+    prim = np.ascontiguousarray(prim, dtype=np.float64)
+    u = np.zeros(4)
+    load().q2do_prim2cons(R, cp, _dp(prim), _dp(u))
+    return u
+
+
+def cons2prim(R, cp, cons, T0):
+    cons = np.ascontiguousarray(cons, dtype=np.float64)
+    q = np.zeros(4)
+    load().q2do_cons2prim(R, cp, _dp(cons), float(T0), _dp(q))
+    return q
 
 
 def pow_(x, y):

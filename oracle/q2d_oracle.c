@@ -43,15 +43,63 @@
 /* ------------------------------------------------------------------------ */
 /* gas closure                                                               */
 /* ------------------------------------------------------------------------ */
-typedef struct { double R, cp, cv, gam, mu_ref, T_ref, S, Pr; } gas_t;
+/* tab: NULL for the calorically perfect closed forms; else the table mode's planes h, cp, mu, k of ntab
+ * nodes each (node k at T = k). */
+typedef struct { double R, cp, cv, gam, mu_ref, T_ref, S, Pr; const double *tab; int ntab; } gas_t;
 
 static gas_t mkgas(double R, double cp) {
   gas_t g; g.R = R; g.cp = cp; g.cv = cp - R; g.gam = cp / g.cv;
-  g.mu_ref = 0.0; g.T_ref = 1.0; g.S = 0.0; g.Pr = 0.72; return g;
+  g.mu_ref = 0.0; g.T_ref = 1.0; g.S = 0.0; g.Pr = 0.72; g.tab = NULL; g.ntab = 0; return g;
 }
 /* Planted mistakes (tests only, q2do_plant): each makes one restated formula wrong in a way a test of
  * that formula must detect.  0 = none. */
 static int g_plant = 0;
+
+/* Table mode — FLINT's native tabulated properties of one species (the library is not in the reference
+ * tree; docs/theory/thermo.md:39-41 states the linear interpolation, Lib_Riemann_HLL.f90:257-282 and
+ * Lib_Newstate.f90:267-282 show its arithmetic): a property at T is v(Ti) + (v(Ti+1) - v(Ti)) (T - Ti)
+ * with Ti = int(T).  Ti is clamped to the table's intervals (T beyond them extrapolates the end interval);
+ * T < 0 or NaN takes interval 0. */
+enum { TAB_H = 0, TAB_CP = 1, TAB_MU = 2, TAB_K = 3 };
+static int tab_node(const gas_t *g, double T) {
+  if (!(T >= 1.0)) return 0;
+  if (!(T < (double)(g->ntab - 1))) return g->ntab - 2;
+  return (int)T;
+}
+static double tab_interp(const gas_t *g, int q, int Ti, double dT) {
+  const double *v = g->tab + (size_t)q * (size_t)g->ntab;
+  if (g_plant == Q2DO_PLANT_TAB_NODE) return v[Ti + 1] + (v[Ti + 1] - v[Ti]) * dT;
+  return v[Ti] + (v[Ti + 1] - v[Ti]) * dT;
+}
+static double tabT(const gas_t *g, int q, double T) {
+  const int Ti = tab_node(g, T);
+  return tab_interp(g, q, Ti, T - (double)Ti);
+}
+/* f_gamma: cp / (cp - R) at T */
+static double f_gamma_T(const gas_t *g, double T) {
+  if (!g->tab) return g->gam;
+  const double cp = tabT(g, TAB_CP, T);
+  return cp / (cp - g->R);
+}
+/* cons2prim's temperature in table mode (FLINT's cons2prim, absent here; restated as the exact inverse of
+ * the interpolated e(T) = h(T) - R T): Newton steps from the caller's estimate T0 (the stage input's
+ * p / (rho R), Lib_Newstate.f90:119-123), each with the slope of the current interval, so a step that
+ * lands in its own interval is the root of the piecewise-linear e(T); at most 32 steps. */
+static double t_of_e(const gas_t *g, double e, double T0) {
+  const double *h = g->tab;
+  double T = T0;
+  for (int it = 0; it < 32; it++) {
+    const int Ti = tab_node(g, T);
+    const double e0 = h[Ti] - g->R * (double)Ti;
+    const double sl = (h[Ti + 1] - h[Ti]) - g->R;
+    const double Tn = (double)Ti + (e - e0) / sl;
+    if (g_plant == Q2DO_PLANT_TAB_TOFE) return Tn;
+    if (Tn >= (double)Ti && Tn <= (double)(Ti + 1)) return Tn;
+    if ((Ti == 0 && Tn < 0.0) || (Ti == g->ntab - 2 && Tn > (double)(Ti + 1))) return Tn;
+    T = Tn;
+  }
+  return T;
+}
 
 /* x^y for x > 0, from IEEE-754 binary64 +, -, *, /, fma, rint, frexp and ldexp only (each exactly
  * specified), so that any conforming evaluation of this sequence gives the same bits; the device
@@ -109,19 +157,33 @@ static double exact_pow(double x, double y) {
  * Sutherland mu = mu_ref * (T/T_ref)^1.5 * (T_ref + S) / (T + S), with (T/T_ref)^1.5 = x*sqrt(x);
  * k = mu * cp / Pr. */
 static double mu_lam(const gas_t *g, double T) {
+  if (g->tab) return tabT(g, TAB_MU, T);
   double x = T / g->T_ref;
   return g->mu_ref * (x * sqrt(x)) * (g->T_ref + g->S) / (T + g->S);
 }
-static double k_lam(const gas_t *g, double mu) { return mu * g->cp / g->Pr; }
-static double f_ss(const gas_t *g, double p, double rho) { return sqrt(g->gam * p / rho); }
+/* conductivity of viscosity mu at temperature T (table mode: the k plane) */
+static double k_lam(const gas_t *g, double mu, double T) {
+  if (g->tab) return tabT(g, TAB_K, T);
+  return mu * g->cp / g->Pr;
+}
 static double eos_T(const gas_t *g, double p, double rho) { return p / (rho * g->R); }
+static double f_ss(const gas_t *g, double p, double rho) {
+  if (g->tab) return sqrt(f_gamma_T(g, eos_T(g, p, rho)) * p / rho);
+  return sqrt(g->gam * p / rho);
+}
+/* specific internal energy e(T): cv T, or h(T) - R T in table mode */
+static double f_e(const gas_t *g, double T) {
+  if (g->tab) return tabT(g, TAB_H, T) - g->R * T;
+  return g->cv * T;
+}
 static double f_H0(const gas_t *g, double p, double rho, double vmag) {
   double T = eos_T(g, p, rho);
+  if (g->tab) return tabT(g, TAB_H, T) + 0.5 * vmag * vmag;
   return g->cp * T + 0.5 * vmag * vmag;
 }
 static double f_E0(const gas_t *g, double p, double rho, double vmag) {
   double T = eos_T(g, p, rho);
-  return g->cv * T + 0.5 * vmag * vmag;
+  return f_e(g, T) + 0.5 * vmag * vmag;
 }
 static void prim2cons(const gas_t *g, const double *pr, double *u) {
   double rho = pr[0];
@@ -129,13 +191,14 @@ static void prim2cons(const gas_t *g, const double *pr, double *u) {
   u[0] = rho;
   u[1] = rho * pr[1];
   u[2] = rho * pr[2];
-  u[3] = rho * (g->cv * T + 0.5 * (pr[1] * pr[1] + pr[2] * pr[2]));
+  u[3] = rho * (f_e(g, T) + 0.5 * (pr[1] * pr[1] + pr[2] * pr[2]));
 }
-static void cons2prim(const gas_t *g, const double *u, double *pr) {
+/* T0: the temperature estimate table mode starts from (unused by the closed form) */
+static void cons2prim(const gas_t *g, const double *u, double *pr, double T0) {
   double rho = u[0];
   double vx = u[1] / rho, vy = u[2] / rho;
   double e = u[3] / rho - 0.5 * (vx * vx + vy * vy);
-  double T = e / g->cv;
+  double T = g->tab ? t_of_e(g, e, T0) : e / g->cv;
   pr[0] = rho; pr[1] = vx; pr[2] = vy; pr[3] = rho * g->R * T;
 }
 /* check_gas_state: a reconstructed state is physical when rho>0 and p>0. */
@@ -216,7 +279,7 @@ static void reconstruct(int lim, double prev, double local, double next, double 
 /* ------------------------------------------------------------------------ */
 typedef struct { double h0l, h0r, rho, u, v, a, h0; } roe_t;
 
-/* roe_averages :229-317 (one species, constant-cp tables: h = cp*T, cp(T) = cp) */
+/* roe_averages :229-317 (one species): the closed form (h = cp T, cp(T) = cp) or the table arithmetic */
 static roe_t roe_averages(const gas_t *g, double dl, double dr, double dltot, double drtot,
                           double ul, double vl, double ur, double vr, double pl, double pr) {
   roe_t o;
@@ -234,7 +297,22 @@ static roe_t roe_averages(const gas_t *g, double dl, double dr, double dltot, do
   double R_roe = (srR * Rr + srL * Rl) * inv_sr;
   double T_roe = (srR * Tr + srL * Tl) * inv_sr;
   double h0l = 0.0, h0r = 0.0, cv_roe = 0.0, sum_ei = 0.0;
-  {
+  if (g->tab) {                                         /* the table arithmetic of :257-292 */
+    const int Til = tab_node(g, Tl), Tir = tab_node(g, Tr);
+    const double dTl = Tl - (double)Til, dTr = Tr - (double)Tir;
+    const double *cp = g->tab + (size_t)TAB_CP * (size_t)g->ntab;
+    double invW = g->R;
+    double hl = tab_interp(g, TAB_H, Til, dTl);
+    double hr = tab_interp(g, TAB_H, Tir, dTr);
+    double el = hl - invW * Tl;
+    double er = hr - invW * Tr;
+    h0l = h0l + hl * dl * inv_dltot;
+    h0r = h0r + hr * dr * inv_drtot;
+    double d_roe = (srR * dr * inv_drtot + srL * dl * inv_dltot) * inv_sr;
+    cv_roe = cv_roe + d_roe * (0.5 * (((cp[Til] + (cp[Til + 1] - cp[Til]) * dTl) + cp[Tir]) +
+                                      (cp[Tir + 1] - cp[Tir]) * dTr) - invW);
+    sum_ei = sum_ei + d_roe * (srR * er + srL * el) * inv_sr;
+  } else {
     double invW = g->R;
     double hl = g->cp * Tl;
     double hr = g->cp * Tr;
@@ -549,6 +627,11 @@ q2do *q2do_create(const q2d_problem *prob) {
     cs->gas = mkgas(cs->d.scheme.gas_R, cs->d.scheme.gas_cp);
     cs->gas.mu_ref = cs->d.scheme.mu_ref; cs->gas.T_ref = cs->d.scheme.T_ref;
     cs->gas.S = cs->d.scheme.S_suth; cs->gas.Pr = cs->d.scheme.Pr;
+    if (cs->d.thermo && cs->d.thermo_n >= 2) {         /* table mode: the case keeps its own copy */
+      cs->gas.tab = dupd(cs->d.thermo, (size_t)4 * (size_t)cs->d.thermo_n);
+      cs->gas.ntab = cs->d.thermo_n;
+    }
+    cs->d.thermo = NULL;
     cs->first_block = -1; cs->first_bc = -1;
     cs->dtglobal = 0.0; cs->time = cs->d.time0;      /* Wrap_Setup.f90:129-152 */
   }
@@ -589,6 +672,7 @@ void q2do_destroy(q2do *o) {
     free(k->P); free(k->PO); free(k->R); free(k->dl); free(k->M); free(k->sur);
     free(k->fi); free(k->fj); free(k->node); free(k->sa); free(k->sb); free(k->dtlocal); free(k->beta);
   }
+  for (int c = 0; c < o->ncases; c++) free((void *)o->cs[c].gas.tab);
   free(o->blk); free(o->bc); free(o->cs); free(o);
 }
 
@@ -817,7 +901,7 @@ static void compute_diffusive_flux(const ocase *cs, int axi, const double Pm[4],
   const gas_t *g = &cs->gas;
   double rho = Pm[0];
   double T = Pm[3] / (rho * g->R);
-  double mil = mu_lam(g, T), kl = k_lam(g, mil);
+  double mil = mu_lam(g, T), kl = k_lam(g, mil, T);
   double kappa = kl + 0.0;
   double gv[2][2] = {{Gr[1][0], Gr[1][1]}, {Gr[2][0], Gr[2][1]}};
   double S1, S2;
@@ -1131,7 +1215,7 @@ static int outflow_flux(const gas_t *g, int Fm, const double *Bp, double n1, dou
   if (BC_Sign * Un < 0.0) return 1;
   double Bound_rho = Bp[0], Bound_Rgas = g->R;
   double Bound_Sound = f_ss(g, Bp[3], Bound_rho);
-  double Bound_Gamma = g->gam;
+  double Bound_Gamma = f_gamma_T(g, eos_T(g, Bp[3], Bound_rho));
   double Riem = g_plant == Q2DO_PLANT_OUTFLOW_RIEM ? Bound_Sound - BC_Sign * 0.5 * (Bound_Gamma - 1.0) * Un
                                                    : Bound_Sound + BC_Sign * 0.5 * (Bound_Gamma - 1.0) * Un;
   double VN1 = Un * n1, VN2 = Un * n2;
@@ -1160,7 +1244,7 @@ static int outflow_flux(const gas_t *g, int Fm, const double *Bp, double n1, dou
     Face_Un = Un; Face_rho = Bound_rho; Face_Rgas = Bound_Rgas;
   }
   double Face_T = Fp[3] / (Face_Rgas * Face_rho);
-  double Face_Enthalpy = 0.0 + Fp[0] / Face_rho * (g->cp * Face_T);
+  double Face_Enthalpy = 0.0 + Fp[0] / Face_rho * (g->tab ? tabT(g, TAB_H, Face_T) : g->cp * Face_T);
   Flux[0] = Fp[0] * int_l * Face_Un;
   double mdot = Face_rho * int_l * Face_Un;
   Flux[1] = Fp[1] * mdot + Fp[3] * int_l * n1;
@@ -1306,7 +1390,7 @@ static void bc_wall_temperature(oblk *b, const ocase *cs, int Im, int Jm, int Fm
   const double *M = &MM(b, 1, 1, Im, Jm);
   double rho = PV(b, 0, Im, Jm), p = PV(b, 3, Im, Jm);
   double T = p / (rho * g->R);
-  double mil = mu_lam(g, T_wall), kl = k_lam(g, mil);
+  double mil = mu_lam(g, T_wall), kl = k_lam(g, mil, T_wall);
   double Gr[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
   int d = dir - 1;
   Gr[1][d] = PV(b, 1, Im, Jm) * (double)modfm3;
@@ -1388,7 +1472,7 @@ static void source_axi_visc(oblk *b, const ocase *cs) {
       double T2 = PV(b, 3, i - 1, j) / (rho2 * g->R);
       double T3 = PV(b, 3, i, j + 1) / (rho3 * g->R);
       double T4 = PV(b, 3, i, j - 1) / (rho4 * g->R);
-      double mil = mu_lam(g, T), kl = k_lam(g, mil);
+      double mil = mu_lam(g, T), kl = k_lam(g, mil, T);
       double Gr[4][2];
       Gr[0][0] = (PV(b, 0, i + 1, j) / rho1 - PV(b, 0, i - 1, j) / rho2) * 0.5;
       Gr[0][1] = (PV(b, 0, i, j + 1) / rho3 - PV(b, 0, i, j - 1) / rho4) * 0.5;
@@ -1441,7 +1525,7 @@ static void source_area_visc(oblk *b, const ocase *cs) {
       double T2 = PV(b, 3, i - 1, j) / (rho2 * g->R);
       double T3 = PV(b, 3, i, j + 1) / (rho3 * g->R);
       double T4 = PV(b, 3, i, j - 1) / (rho4 * g->R);
-      double mil = mu_lam(g, T), kl = k_lam(g, mil);
+      double mil = mu_lam(g, T), kl = k_lam(g, mil, T);
       double Gr[4][2];
       Gr[0][0] = (PV(b, 0, i + 1, j) / rho1 - PV(b, 0, i - 1, j) / rho2) * 0.5;
       Gr[0][1] = (PV(b, 0, i, j + 1) / rho3 - PV(b, 0, i, j - 1) / rho4) * 0.5;
@@ -1632,7 +1716,7 @@ static int newstate(q2do *o, ocase *cs, int irk, int *fb, int *fi, int *fj) {
           double r_ = (cons[v] - consold[v] + res[v]) * c;
           cons[v] = consold[v] + r_;
         }
-        cons2prim(&cs->gas, cons, prim);
+        cons2prim(&cs->gas, cons, prim, eos_T(&cs->gas, prim[3], prim[0]));   /* :119-123 */
         /* Check_And_Fix_State — Lib_Newstate.f90:316-350 */
         if (isnan(prim[0]) || isnan(prim[1]) || isnan(prim[2]) || isnan(prim[3]) || prim[3] < 0.0) {
           if (!row_fail[j]) row_fail[j] = i;
@@ -1780,22 +1864,55 @@ void q2do_reconstruct(int lim, double prev, double local, double next, double ne
   reconstruct(lim, prev, local, next, next2, dl0, dl1, dl2, dll, dlr, beta, priml, primr);
 }
 
+/* The unit entry points' gas: closed form, or table mode with the tables of q2do_unit_tables. */
+static const double *g_unit_tab = NULL;
+static int g_unit_ntab = 0;
+static gas_t unit_gas(double R, double cp) {
+  gas_t g = mkgas(R, cp);
+  if (g_unit_tab) { g.tab = g_unit_tab; g.ntab = g_unit_ntab; }
+  return g;
+}
+void q2do_unit_tables(const double *tab, int n) { g_unit_tab = (tab && n >= 2) ? tab : NULL; g_unit_ntab = n; }
+
+double q2do_thermo_eval(int what, double gas_R, double gas_cp, double a, double b) {
+  gas_t g = unit_gas(gas_R, gas_cp);
+  switch (what) {
+    case 0: return g.tab ? tabT(&g, TAB_H, a) : g.cp * a;
+    case 1: return g.tab ? tabT(&g, TAB_CP, a) : g.cp;
+    case 2: return mu_lam(&g, a);
+    case 3: return k_lam(&g, mu_lam(&g, a), a);
+    case 4: return f_gamma_T(&g, a);
+    case 5: return f_e(&g, a);
+    case 6: return g.tab ? t_of_e(&g, a, b) : a / g.cv;
+    case 7: return f_ss(&g, a, b);
+    default: return NAN;
+  }
+}
+void q2do_prim2cons(double gas_R, double gas_cp, const double *prim, double *cons) {
+  gas_t g = unit_gas(gas_R, gas_cp);
+  prim2cons(&g, prim, cons);
+}
+void q2do_cons2prim(double gas_R, double gas_cp, const double *cons, double T0, double *prim) {
+  gas_t g = unit_gas(gas_R, gas_cp);
+  cons2prim(&g, cons, prim, T0);
+}
+
 void q2do_riemann(int id, double gas_R, double gas_cp, const double *L, const double *Rr,
                   double beta, double nx, double ny, double *F) {
-  gas_t g = mkgas(gas_R, gas_cp);
+  gas_t g = unit_gas(gas_R, gas_cp);
   double aL = f_ss(&g, L[3], L[0]), aR = f_ss(&g, Rr[3], Rr[0]);
   riemann(id, &g, L[0], L[1], L[2], L[3], aL, L[0], Rr[0], Rr[1], Rr[2], Rr[3], aR, Rr[0], beta, nx, ny, F);
 }
 
 void q2do_roe_averages(double gas_R, double gas_cp, const double *L, const double *Rr, double *out) {
-  gas_t g = mkgas(gas_R, gas_cp);
+  gas_t g = unit_gas(gas_R, gas_cp);
   roe_t r = roe_averages(&g, L[0], Rr[0], L[0], Rr[0], L[1], L[2], Rr[1], Rr[2], L[3], Rr[3]);
   out[0] = r.h0l; out[1] = r.h0r; out[2] = r.rho; out[3] = r.u; out[4] = r.v; out[5] = r.a; out[6] = r.h0;
 }
 
 int q2do_convective_flux(int lim, int rid, double gas_R, double gas_cp, const double *dl,
                          double nx, double ny, double int_l, const double *prim, double *flux) {
-  gas_t g = mkgas(gas_R, gas_cp);
+  gas_t g = unit_gas(gas_R, gas_cp);
   return convective_flux(lim, rid, &g, dl, nx, ny, int_l, (const double(*)[NP])prim, 1.0, 1.0, flux);
 }
 
@@ -1826,12 +1943,12 @@ int q2do_plant(int which) { int old = g_plant; g_plant = which; return old; }
 int q2do_bc_inflow_flux(int limiter_id, int riemann_id, double gas_R, double gas_cp, int face, const double *Bp,
                         const double *Ip, double n1, double n2, double int_l, double mach, double T, double p,
                         double rel_fac, double alpha, double *F) {
-  gas_t g = mkgas(gas_R, gas_cp);
+  gas_t g = unit_gas(gas_R, gas_cp);
   return inflow_flux(&g, limiter_id, riemann_id, face, Bp, Ip, n1, n2, int_l, mach, T, p, rel_fac, cos(alpha),
                      sin(alpha), F);
 }
 int q2do_bc_outflow_flux(double gas_R, double gas_cp, int face, const double *Bp, double n1, double n2,
                          double int_l, double p_exit, double rel_fac, double *F) {
-  gas_t g = mkgas(gas_R, gas_cp);
+  gas_t g = unit_gas(gas_R, gas_cp);
   return outflow_flux(&g, face, Bp, n1, n2, int_l, p_exit, rel_fac, F);
 }

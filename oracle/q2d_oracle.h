@@ -83,7 +83,18 @@ int q2do_bc_outflow_flux(double gas_R, double gas_cp, int face, const double *Bp
 #define Q2DO_PLANT_OUTFLOW_RIEM  5   /* Outflow Riemann invariant with the sign of the Un term reversed */
 #define Q2DO_PLANT_OUTFLOW_ISEN  6   /* Outflow isentropic exponent (g-1)/g instead of (g-1)/(2g) */
 #define Q2DO_PLANT_RAMP          7   /* CFL ramp with iter - 1 in place of iter */
+#define Q2DO_PLANT_TAB_NODE      8   /* table interpolation from node Ti + 1 instead of Ti */
+#define Q2DO_PLANT_TAB_TOFE      9   /* table-mode cons2prim: one Newton step, whatever its interval */
 int q2do_plant(int which);
+/* Table mode for the unit entry points (riemann, roe_averages, convective_flux, the BC fluxes and the
+ * functions below): planes h, cp, mu, k of n nodes (node k at T = k), or NULL for the closed form.  The
+ * oracle keeps the pointer, not a copy. */
+void q2do_unit_tables(const double *tab, int n);
+/* what: 0 h(T=a), 1 cp(T=a), 2 mu(T=a), 3 k(T=a), 4 gamma(T=a), 5 e(T=a), 6 T of e=a from estimate b,
+ * 7 speed of sound at p=a, rho=b. */
+double q2do_thermo_eval(int what, double gas_R, double gas_cp, double a, double b);
+void q2do_prim2cons(double gas_R, double gas_cp, const double *prim, double *cons);
+void q2do_cons2prim(double gas_R, double gas_cp, const double *cons, double T0, double *prim);
 
 #ifdef __cplusplus
 }

@@ -62,7 +62,7 @@ class BCEntry(C.Structure):
 class CaseDesc(C.Structure):
     _fields_ = [("scheme", Scheme), ("itermax", C.c_int32), ("reserved", C.c_int32),
                 ("res_threshold", C.c_double), ("time_threshold", C.c_double), ("res_drop", C.c_double),
-                ("time0", C.c_double)]
+                ("time0", C.c_double), ("thermo", C.c_void_p), ("thermo_n", C.c_int32), ("reserved2", C.c_int32)]
 
 
 class ProblemC(C.Structure):
@@ -119,6 +119,11 @@ def to_problem_c(problem):
         cases[k].itermax, cases[k].res_threshold, cases[k].time_threshold = c.itermax, c.res_threshold, c.time_threshold
         cases[k].res_drop = c.res_drop
         cases[k].time0 = c.time0
+        if c.thermo is not None:
+            if c.thermo.dtype != np.float64 or c.thermo.ndim != 2 or c.thermo.shape[0] != 4 or \
+                    not c.thermo.flags.c_contiguous:
+                raise ValueError("CaseSpec.thermo must be a C-contiguous float64 array of shape (4, n)")
+            cases[k].thermo, cases[k].thermo_n = _ptr(c.thermo), c.thermo.shape[1]
     pc = ProblemC()
     pc.nblocks, pc.nbc, pc.ncases = nb, nbc, nc
     pc.blocks = C.cast(blocks, C.POINTER(BlockDesc))

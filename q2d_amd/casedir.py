@@ -15,7 +15,7 @@ from pathlib import Path
 
 import numpy as np
 
-from . import cases
+from . import cases, thermo
 from .geometry import BlockGeometry, axisymmetric_geometry
 
 R_UNIVERSAL = 8314.51
@@ -48,23 +48,22 @@ def _logical(v):
     return v.strip().lower().lstrip(".").startswith("t")
 
 
-def read_gas(case_dir):
-    """(R, cp) of the one species of INPUT/phase.txt with the constant-cp table INPUT/thermo.dat."""
+def read_gas(case_dir, allow_varying_cp=False):
+    """(R, cp, constant) of the one species of INPUT/phase.txt and its table INPUT/thermo.dat: cp of the
+    first node, and whether the table is that of a constant cp (h = cp T).  A varying cp raises unless
+    allow_varying_cp (the table mode reads the table itself)."""
     d = Path(case_dir) / "INPUT"
     lines = [x for x in (d / "phase.txt").read_text().splitlines() if x.strip()]
     if len(lines) != 2:
         raise CaseError("INPUT/phase.txt: one species only on this path")
     W = float(lines[1].split()[1])
-    rows = []
-    for x in (d / "thermo.dat").read_text().splitlines():
-        x = x.strip()
-        if x and (x[0].isdigit() or x[0] in ".-"):
-            rows.append([float(v) for v in x.split()[:3]])
-    t = np.array(rows)
+    t = thermo.read_point_table(d / "thermo.dat")[:, :3]
     cp = t[0, 1]
-    if np.any(np.abs(t[:, 1] - cp) > 1e-12 * abs(cp)) or np.any(np.abs(t[:, 2] - cp * t[:, 0]) > 1e-9 * np.abs(cp * t[:, 0])):
-        raise CaseError("INPUT/thermo.dat: cp varies with T (table-mode thermodynamics is not on this path)")
-    return R_UNIVERSAL / W, cp
+    const = not (np.any(np.abs(t[:, 1] - cp) > 1e-12 * abs(cp)) or
+                 np.any(np.abs(t[:, 2] - cp * t[:, 0]) > 1e-9 * np.abs(cp * t[:, 0])))
+    if not const and not allow_varying_cp:
+        raise CaseError("INPUT/thermo.dat: cp varies with T (the closed form needs a constant cp; use table mode)")
+    return R_UNIVERSAL / W, cp, const
 
 
 def _dir_token(tok):
@@ -143,8 +142,10 @@ def read_tec(path):
     return soltime, zones, names
 
 
-def load_case(case_dir, precision="fp64"):
-    """A Q2D case directory as a cases.Problem (one case), with its SOLUTIONTIME-based start time."""
+def load_case(case_dir, precision="fp64", thermo_mode="auto"):
+    """A Q2D case directory as a cases.Problem (one case), with its SOLUTIONTIME-based start time.
+    thermo_mode: "closed" (constant-cp closed forms; a varying cp raises), "table" (FLINT's tabulated mode:
+    the planes of INPUT/thermo.dat and INPUT/transport.dat), "auto" (closed when cp is constant)."""
     d = Path(case_dir)
     ini = read_ini(d / "input.ini")
     par, phy, num, src = (ini.get(s, {}) for s in ("Q2D-Parameters", "Q2D-Physics", "Q2D-Numerics", "Q2D-Sources"))
@@ -153,7 +154,10 @@ def load_case(case_dir, precision="fp64"):
     source = src.get("active", "none")
     if source not in ("none", "axisymmetric"):
         raise CaseError(f"[Q2D-Sources] active = {source} is not read by this host")
-    R, cp = read_gas(d)
+    R, cp, const = read_gas(d, allow_varying_cp=thermo_mode != "closed")
+    tab = None
+    if thermo_mode == "table" or (thermo_mode == "auto" and not const):
+        tab = thermo.load_tables(d / "INPUT" / "thermo.dat", d / "INPUT" / "transport.dat")
     newrun = _logical(par.get("newrun", "true"))
     soltime, zones, _ = read_tec(d / ("INPUT/ic.tec" if newrun else "OUTPUT/field.tec"))
     blocks, geoms = [], []
@@ -180,7 +184,7 @@ def load_case(case_dir, precision="fp64"):
         itermax=int(float(par.get("iter-threshold", "1000000000"))), res_threshold=float(par.get("res-threshold", "1e-10")),
         time_threshold=float(par.get("time-threshold", "1e30")), irs_beta=float(num.get("irs-beta", "0.0")),
         space_sd=space == "MUSCL-SD", integration_vars=num.get("integration-variables", "cons"),
-        time0=(0.0 if newrun else soltime) if ta else -1.0)
+        time0=(0.0 if newrun else soltime) if ta else -1.0, thermo=tab)
     pb = cases.Problem(blocks=blocks, bc=recs, cases=[spec], meta={"name": d.name, "soltime": soltime})
     cases.finish(pb, geoms)
     if source == "axisymmetric":

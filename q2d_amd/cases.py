@@ -58,6 +58,7 @@ class CaseSpec:
     space_sd: bool = False        # space-reconstruction = MUSCL-SD (Tramel's shock detector on the slopes)
     integration_vars: str = "cons"  # integration-variables ('prim' is rejected by the library)
     time0: float = 0.0            # domain%time at the start: 0 (new run), SOLUTIONTIME (restart), -1 (steady)
+    thermo: object = None         # table mode: (4, n) float64 planes h, cp, mu, k on T = 0..n-1 K (q2d_amd.thermo)
 
 
 @dataclass

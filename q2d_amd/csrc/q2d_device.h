@@ -12,7 +12,13 @@
 
 namespace q2d {
 
-template <typename T> struct Gas { T R, cp, cv, gam; };
+// Gas closure of one case: R, cp, cv, gamma of the calorically perfect closed forms, or, with tab set
+// (FP64 only), FLINT's native tabulated mode: planes h, cp, mu, k of ntab nodes each, node k at T = k.
+template <typename T> struct Gas {
+  T R, cp, cv, gam;
+  const double* tab = nullptr;
+  int ntab = 0;
+};
 
 // ---- arithmetic primitives ----------------------------------------------
 // FP64 division and square root: the compiler's correctly rounded IEEE operations (the FP64 form's
@@ -37,9 +43,68 @@ template <> __device__ __forceinline__ float qcopysign<float>(float a, float b) 
 template <typename T> __device__ __forceinline__ T min3(T a, T b, T c) { T m = a; if (b < m) m = b; if (c < m) m = c; return m; }
 template <typename T> __device__ __forceinline__ T max3(T a, T b, T c) { T m = a; if (b > m) m = b; if (c > m) m = c; return m; }
 
-// ---- gas closure (single species, constant cp) ---------------------------
-// a = sqrt(gamma*p/rho)  (f_ss)
-template <typename T> __device__ __forceinline__ T sound(const Gas<T>& g, T p, T rho) { return qsqrt<T>(qdiv<T>(g.gam * p, rho)); }
+// ---- property tables (FLINT's native tabulated mode, one species) ---------
+// A property at T: v(Ti) + (v(Ti+1) - v(Ti)) (T - Ti), Ti = int(T) (docs/theory/thermo.md:39-41; the
+// arithmetic of Lib_Riemann_HLL.f90:257-282), Ti clamped to the table's intervals (T beyond them
+// extrapolates the end interval; T < 1 or NaN takes interval 0).
+enum { kTabH = 0, kTabCp = 1, kTabMu = 2, kTabK = 3 };
+__device__ __forceinline__ int tab_node(int ntab, double T) {
+  if (!(T >= 1.0)) return 0;
+  if (!(T < (double)(ntab - 1))) return ntab - 2;
+  return (int)T;
+}
+__device__ __forceinline__ double tab_interp(const double* tab, int ntab, int q, int Ti, double dT) {
+  const double* v = tab + (long long)q * ntab;
+  const double a = v[Ti], b = v[Ti + 1];
+  return a + (b - a) * dT;
+}
+__device__ __forceinline__ double tab_at(const double* tab, int ntab, int q, double T) {
+  const int Ti = tab_node(ntab, T);
+  return tab_interp(tab, ntab, q, Ti, T - (double)Ti);
+}
+// cp / (cp - R) at T (f_gamma)
+__device__ __forceinline__ double tab_gamma(const double* tab, int ntab, double R, double T) {
+  const double cp = tab_at(tab, ntab, kTabCp, T);
+  return ddiv_rn(cp, cp - R);
+}
+// cons2prim's temperature: the exact inverse of the interpolated e(T) = h(T) - R T, by Newton steps from
+// the estimate T0 (the stage input's p / (rho R), Lib_Newstate.f90:119-123), each with the slope of its
+// interval (a step that lands in its own interval is the root); at most 32 steps.
+__device__ __forceinline__ double tab_t_of_e(const double* tab, int ntab, double R, double e, double T0) {
+  double T = T0;
+  for (int it = 0; it < 32; ++it) {
+    const int Ti = tab_node(ntab, T);
+    const double h0 = tab[Ti], h1 = tab[Ti + 1];
+    const double e0 = h0 - R * (double)Ti;
+    const double sl = (h1 - h0) - R;
+    const double Tn = (double)Ti + ddiv_rn(e - e0, sl);
+    if (Tn >= (double)Ti && Tn <= (double)(Ti + 1)) return Tn;
+    if ((Ti == 0 && Tn < 0.0) || (Ti == ntab - 2 && Tn > (double)(Ti + 1))) return Tn;
+    T = Tn;
+  }
+  return T;
+}
+
+// ---- gas closure (single species) ------------------------------------------
+// a = sqrt(gamma*p/rho)  (f_ss); in table mode gamma at T = p / (rho R)
+template <typename T> __device__ __forceinline__ T sound(const Gas<T>& g, T p, T rho) {
+  if constexpr (sizeof(T) == 8) {
+    if (g.tab) return dsqrt_rn(ddiv_rn(tab_gamma(g.tab, g.ntab, g.R, ddiv_rn(p, rho * g.R)) * p, rho));
+  }
+  return qsqrt<T>(qdiv<T>(g.gam * p, rho));
+}
+// specific enthalpy h = cp T (closed form) or h(T) (table), T = p / (rho R)
+template <typename T> __device__ __forceinline__ T enthalpy(const Gas<T>& g, T p, T rho) {
+  if constexpr (sizeof(T) == 8) {
+    if (g.tab) return tab_at(g.tab, g.ntab, kTabH, ddiv_rn(p, rho * g.R));
+  }
+  return g.cp * qdiv<T>(p, rho * g.R);
+}
+// specific internal energy at temperature T: cv T (closed form) or h(T) - R T (table)
+__device__ __forceinline__ double int_energy(const Gas<double>& g, double T) {
+  if (g.tab) return tab_at(g.tab, g.ntab, kTabH, T) - g.R * T;
+  return g.cv * T;
+}
 
 // ---- x^y from exactly specified operations --------------------------------
 // The `**` of the characteristic outflow (Lib_BC_Fluxes_Outflow.f90:125-134) for x > 0, built from
@@ -141,7 +206,7 @@ template <typename T> __device__ __forceinline__ T limiter(int id, T x, T y) {
 }
 
 // ---- Riemann solvers — Lib_Riemann_HLL.f90 --------------------------------
-// roe_averages :229-317, one species, h = cp*T, cp(T) = cp.
+// roe_averages :229-317, one species: h = cp*T, cp(T) = cp, or the table arithmetic.
 template <typename T>
 __device__ __forceinline__ void roe_averages(const Gas<T>& g, T dltot, T drtot, T ul, T vl, T ur, T vr, T pl, T pr,
                                              T& h0l_o, T& h0r_o, T& u_roe, T& v_roe, T& a_roe) {
@@ -159,14 +224,28 @@ __device__ __forceinline__ void roe_averages(const Gas<T>& g, T dltot, T drtot, 
   T R_roe = (srR * Rr + srL * Rl) * inv_sr;
   T T_roe = (srR * Tr + srL * Tl) * inv_sr;
   T invW = g.R;
-  T hl = g.cp * Tl;
-  T hr = g.cp * Tr;
+  T hl, hr, cpsum;
+  if constexpr (sizeof(T) == 8) {
+    if (g.tab) {                                  // the table arithmetic of :257-292
+      const int Til = tab_node(g.ntab, Tl), Tir = tab_node(g.ntab, Tr);
+      const T dTl = Tl - (T)Til, dTr = Tr - (T)Tir;
+      hl = tab_interp(g.tab, g.ntab, kTabH, Til, dTl);
+      hr = tab_interp(g.tab, g.ntab, kTabH, Tir, dTr);
+      const double* cp = g.tab + (long long)kTabCp * g.ntab;
+      const T cl0 = cp[Til], cl1 = cp[Til + 1], cr0 = cp[Tir], cr1 = cp[Tir + 1];
+      cpsum = ((cl0 + (cl1 - cl0) * dTl) + cr0) + (cr1 - cr0) * dTr;
+    } else {
+      hl = g.cp * Tl; hr = g.cp * Tr; cpsum = g.cp + g.cp;
+    }
+  } else {
+    hl = g.cp * Tl; hr = g.cp * Tr; cpsum = g.cp + g.cp;
+  }
   T el = hl - invW * Tl;
   T er = hr - invW * Tr;
   T h0l = T(0) + hl * dl * inv_dltot;
   T h0r = T(0) + hr * dr * inv_drtot;
   T d_roe = (srR * dr * inv_drtot + srL * dl * inv_dltot) * inv_sr;
-  T cv_roe = T(0) + d_roe * (T(0.5) * (g.cp + g.cp) - invW);
+  T cv_roe = T(0) + d_roe * (T(0.5) * cpsum - invW);
   T sum_ei = T(0) + d_roe * (srR * er + srL * el) * inv_sr;
   h0l = h0l + T(0.5) * (ul * ul + vl * vl);
   h0r = h0r + T(0.5) * (ur * ur + vr * vr);
@@ -224,7 +303,7 @@ __device__ __forceinline__ void riemann(const Gas<T>& g, T ul, T vl, T pl, T al,
       Fo[1] = Fo[0] * u + p * nx;
       Fo[2] = Fo[0] * v + p * ny;
       const T vmag = qsqrt<T>(u * u + v * v);
-      Fo[3] = Fo[0] * (g.cp * qdiv<T>(p, r * g.R) + T(0.5) * vmag * vmag);
+      Fo[3] = Fo[0] * (enthalpy<T>(g, p, r) + T(0.5) * vmag * vmag);
     };
     T FL[4], FR[4];
     phys(pl, dltot, ul, vl, FL);
@@ -255,7 +334,7 @@ __device__ __forceinline__ void riemann(const Gas<T>& g, T ul, T vl, T pl, T al,
       Fo[1] = Fo[0] * u + p * nx;
       Fo[2] = Fo[0] * v + p * ny;
       const T vmag = qsqrt<T>(u * u + v * v);
-      Fo[3] = Fo[0] * (g.cp * qdiv<T>(p, r * g.R) + T(0.5) * vmag * vmag);
+      Fo[3] = Fo[0] * (enthalpy<T>(g, p, r) + T(0.5) * vmag * vmag);
     };
     T FL[4], FR[4];
     phys(pl, dltot, ul, vl, FL);
@@ -341,13 +420,14 @@ __device__ __forceinline__ void prim2cons(const Gas<double>& g, double rho, doub
   U[0] = rho;
   U[1] = rho * u;
   U[2] = rho * v;
-  U[3] = rho * (g.cv * T + 0.5 * (u * u + v * v));
+  U[3] = rho * (int_energy(g, T) + 0.5 * (u * u + v * v));
 }
-__device__ __forceinline__ void cons2prim(const Gas<double>& g, const double U[4], double q[4]) {
+// T0: the temperature estimate of the table mode's inversion (the closed form does not read it)
+__device__ __forceinline__ void cons2prim(const Gas<double>& g, const double U[4], double q[4], double T0) {
   double rho = U[0];
   double vx = ddiv_rn(U[1], rho), vy = ddiv_rn(U[2], rho);
   double e = ddiv_rn(U[3], rho) - 0.5 * (vx * vx + vy * vy);
-  double T = ddiv_rn(e, g.cv);
+  double T = g.tab ? tab_t_of_e(g.tab, g.ntab, g.R, e, T0) : ddiv_rn(e, g.cv);
   q[0] = rho; q[1] = vx; q[2] = vy; q[3] = rho * g.R * T;
 }
 

@@ -90,7 +90,8 @@ struct DevCase {
   double irs_beta;                      // > 0: implicit residual smoothing (k_irs)
   int irs_begin, irs_end;               // the case's k_irs work items (their partials replace the stage's)
   int space_sd;                         // MUSCL-SD: Tramel's detector scales the slopes (sd_tramel)
-  int pad_irs;
+  int ntab;                             // table mode: nodes per property plane (0: closed forms)
+  const double* tab;                    // table mode: h, cp, mu, k planes (device copy), else null
 };
 
 struct DevCtrl {
@@ -142,12 +143,23 @@ struct Item { int blk, x0, y0, y1; };
 constexpr int kPartials = 5;
 
 // Laminar transport of one species: Sutherland viscosity and Fourier conductivity k = mu cp / Pr, the
-// closed form behind the reference's f_laminarViscosity / co_k_mi_lam_Wilke calls (FP64 in both forms).
+// closed form behind the reference's f_laminarViscosity / co_k_mi_lam_Wilke calls (FP64 in both forms),
+// or in table mode the mu and k planes at T (Wilke's rule of one species is the species' value).
 __device__ __forceinline__ double mu_lam(const DevCase& C, double T) {
+  if (C.tab) return tab_at(C.tab, C.ntab, kTabMu, T);
   const double x = T / C.T_ref;
   return C.mu_ref * (x * sqrt(x)) * (C.T_ref + C.S_suth) / (T + C.S_suth);
 }
-__device__ __forceinline__ double k_lam(const DevCase& C, double mu) { return mu * C.cp / C.Pr; }
+__device__ __forceinline__ double k_lam(const DevCase& C, double mu, double T) {
+  if (C.tab) return tab_at(C.tab, C.ntab, kTabK, T);
+  return mu * C.cp / C.Pr;
+}
+// the case's FP64 gas closure (with its tables in table mode)
+__device__ __forceinline__ Gas<double> case_gas(const DevCase& C) {
+  Gas<double> g{C.R, C.cp, C.cv, C.gam};
+  g.tab = C.tab; g.ntab = C.ntab;
+  return g;
+}
 
 __device__ __forceinline__ int stage_in(int cur, int irk) { return irk == 1 ? cur : (cur + 1 + ((irk - 2) & 1)) % 3; }
 __device__ __forceinline__ int stage_out(int cur, int irk) { return (cur + 1 + ((irk - 1) & 1)) % 3; }
@@ -270,7 +282,7 @@ __device__ __forceinline__ double vnn_limit(const DevCase& C, double dtmin, doub
 
 __device__ __forceinline__ double cell_dt(const DevCase& C, const DevCtrl& K, double rho, double u, double v, double p,
                                           double m11, double m21, double m12, double m22, double dlx, double dly) {
-  const double snd = sqrt(C.gam * p / rho);
+  const double snd = sound<double>(case_gas(C), p, rho);
   double dtmin = 1e8;
   {
     const double nrm = sqrt(m11 * m11 + m12 * m12);
@@ -563,8 +575,9 @@ __global__ __launch_bounds__(256) void k_bcflux(const DevBlock* __restrict__ blo
     }
   }
   const int lim = C.limiter, rid = C.riemann;
-  const Gas<FT> gf{(FT)C.R, (FT)C.cp, (FT)C.cv, (FT)C.gam};
-  const Gas<double> gd{C.R, C.cp, C.cv, C.gam};
+  Gas<FT> gf{(FT)C.R, (FT)C.cp, (FT)C.cv, (FT)C.gam};
+  if constexpr (sizeof(FT) == 8) { gf.tab = C.tab; gf.ntab = C.ntab; }
+  const Gas<double> gd = case_gas(C);
   // Face_Index — Lib_BC_Fluxes.f90:34-50
   const bool idir = Fm <= 2;
   const long long ofc = idir ? off(Im - modfm, Jm) : off(Im, Jm - modfm);
@@ -647,7 +660,7 @@ __global__ __launch_bounds__(256) void k_bcflux(const DevBlock* __restrict__ blo
     Fb[2] = Fmass * v + p * il * n2;
     const double vmag = sqrt(u * u + v * v);
     const double Tt = p / (rho * gd.R);
-    Fb[3] = Fmass * (gd.cp * Tt + 0.5 * vmag * vmag);
+    Fb[3] = Fmass * ((gd.tab ? tab_at(gd.tab, gd.ntab, kTabH, Tt) : gd.cp * Tt) + 0.5 * vmag * vmag);
 #pragma unroll
     for (int k = 0; k < 4; ++k) res.F[k] = sgn * Fb[k];
     emit_bc(out, dense, E.dpos, res, mx_kernel_case(C.model, C.space_sd, C.riemann));
@@ -659,8 +672,8 @@ __global__ __launch_bounds__(256) void k_bcflux(const DevBlock* __restrict__ blo
     if (sgn * Un < 0.0) {
       ty = 300;                                          // error=1 -> BC_Symmetry_Eul (Mod_BC_Fluxes.f90:227)
     } else {
-      const double Bound_Sound = sqrt(gd.gam * Bp[3] / Bp[0]);
-      const double G = gd.gam;
+      const double Bound_Sound = sound<double>(gd, Bp[3], Bp[0]);
+      const double G = gd.tab ? tab_gamma(gd.tab, gd.ntab, gd.R, Bp[3] / (Bp[0] * gd.R)) : gd.gam;
       const double Riem = Bound_Sound + sgn * 0.5 * (G - 1.0) * Un;
       const double VT1 = Bp[1] - Un * n1, VT2 = Bp[2] - Un * n2;
       const double Mach_N = fabs(Un / Bound_Sound);
@@ -685,7 +698,7 @@ __global__ __launch_bounds__(256) void k_bcflux(const DevBlock* __restrict__ blo
         Face_Un = Un; Face_rho = Bp[0];
       }
       const double Face_T = Fp[3] / (gd.R * Face_rho);
-      const double Hf = 0.0 + Fp[0] / Face_rho * (gd.cp * Face_T);
+      const double Hf = 0.0 + Fp[0] / Face_rho * (gd.tab ? tab_at(gd.tab, gd.ntab, kTabH, Face_T) : gd.cp * Face_T);
       double Fb[4];
       Fb[0] = Fp[0] * il * Face_Un;
       const double mdot = Face_rho * il * Face_Un;
@@ -706,14 +719,14 @@ __global__ __launch_bounds__(256) void k_bcflux(const DevBlock* __restrict__ blo
       ty = 300;                                          // error=1 -> BC_Symmetry_Eul (Mod_BC_Fluxes.f90:221)
     } else {
       const double dm = (double)modfm, dm1 = (double)modfm1;
-      const double flow_Sound = sqrt(gd.gam * Bp[3] / Bp[0]);
+      const double flow_Sound = sound<double>(gd, Bp[3], Bp[0]);
       const double flow_Mach = sqrt(Bp[1] * Bp[1] + Bp[2] * Bp[2]) / flow_Sound;
       const double Mach_l = E.rel_fac * E.mach + (1.0 - E.rel_fac) * flow_Mach;
       double Sup[4];
       Sup[3] = E.rel_fac * E.p + (1.0 - E.rel_fac) * Bp[3];
       const double Sup_rho = Sup[3] / (1.0 * gd.R * E.T);
       Sup[0] = Sup_rho * 1.0;
-      const double Sup_Sound = sqrt(gd.gam * Sup[3] / Sup_rho);
+      const double Sup_Sound = sound<double>(gd, Sup[3], Sup_rho);
       Sup[1] = Mach_l * Sup_Sound * E.cos_a;
       Sup[2] = Mach_l * Sup_Sound * E.sin_a;
 #pragma unroll
@@ -816,7 +829,7 @@ __device__ __forceinline__ void diff_flux(const DevCase& C, bool axi, double pm0
                                           double n1, double n2, double il, double r_ave, double VG, double D[3]) {
   const double T = pm3 / (pm0 * C.R);
   const double mil = mu_lam(C, T);
-  const double kappa = k_lam(C, mil) + 0.0;
+  const double kappa = k_lam(C, mil, T) + 0.0;
   double S1, S2;
   stress_vector(axi, gUx, gUy, gVx, gVy, n1, n2, mil, r_ave, pm2, VG, S1, S2);
   D[0] = il * S1;
@@ -944,7 +957,7 @@ __global__ __launch_bounds__(256) void k_visc(const DevBlock* __restrict__ block
         if (r == 1 && wall3) qS = refl;
         if (r == nj && wall4) qN = refl;
         const double mil = mu_lam(C, cC.T);
-        const double kl = k_lam(C, mil);
+        const double kl = k_lam(C, mil, cC.T);
         double gUx, gUy, gVx, gVy, gTx, gTy;
         grad_m((qE.u - qW.u) * 0.5, (qN.u - qS.u) * 0.5, MC, gUx, gUy);
         grad_m((qE.v - qW.v) * 0.5, (qN.v - qS.v) * 0.5, MC, gVx, gVy);
@@ -1115,7 +1128,7 @@ __global__ __launch_bounds__(256) void k_bcvisc(const DevBlock* __restrict__ blo
     return;
   } else {                                           // 302
     const double T = p / (rho * C.R);
-    const double mil = mu_lam(C, E.T), kl = k_lam(C, mil);
+    const double mil = mu_lam(C, E.T), kl = k_lam(C, mil, E.T);
     const double m3 = (double)modfm3;
     double gU0 = 0.0, gU1 = 0.0, gV0 = 0.0, gV1 = 0.0, gT0 = 0.0, gT1 = 0.0;
     if (idir) { gU0 = u * m3; gV0 = v * m3; gT0 = (T - E.T) * m3; }
@@ -1209,6 +1222,9 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
     tslotp = tstamp + 2 + 2 * ((long long)launch * kStampLanes + (blockIdx.x % kStampLanes));
     if (threadIdx.x == 0) atomicMin(&tslotp[0], (unsigned long long)wall_clock64());
   }
+  // RIE_T == -4: every solver of RIE_T == -2 and the cases' property tables (table mode, FP64 only)
+  constexpr bool TAB = RIE_T == -4;
+  static_assert(!TAB || sizeof(FT) == 8, "table mode is FP64 only");
   const int lane = threadIdx.x & (kWave - 1);
   const int wpg = blockDim.x / kWave;
   __shared__ BCFlux sbq[4][2][16];                     // per wavefront: face 1 and 2 boundary results, 16 rows
@@ -1255,7 +1271,8 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
       GP m11p = gp(B.m11);
       const int lim = C.limiter;
       const bool tacc = C.time_accurate != 0;
-      const Gas<FT> gf{(FT)C.R, (FT)C.cp, (FT)C.cv, (FT)C.gam};
+      Gas<FT> gf{(FT)C.R, (FT)C.cp, (FT)C.cv, (FT)C.gam};
+      if constexpr (TAB) { gf.tab = C.tab; gf.ntab = C.ntab; }
       const double c_rk = (C.n_rk == 1) ? 1.0 : (C.n_rk == 2 ? (irk == 1 ? 1.0 : 0.5)
                                                               : (irk == 1 ? 1.0 : (irk == 2 ? 0.25 : 2.0 / 3.0)));
       const double dtg = K.dtglobal;
@@ -1482,11 +1499,12 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
             if (r == nj && B.bcf[3] >= 0) add_bc(bcflux[B.bcf[3] + x - 1]);
             // ---- geometric sources — Lib_Source_Axisymmetry.f90:40-78, Lib_Source_Areavariation.f90:104-144
             const double sr = gcur.sur;
-            const Gas<double> gd{C.R, C.cp, C.cv, C.gam};
+            Gas<double> gd{C.R, C.cp, C.cv, C.gam};
+            if constexpr (TAB) { gd.tab = C.tab; gd.ntab = C.ntab; }
             if (C.source != Q2D_SRC_NONE) {
               const double rho = qA[0], u = qA[1], v = qA[2], p = qA[3];
               const double mod_v = dsqrt_rn(u * u + v * v);
-              const double E0 = gd.cv * ddiv_rn(p, rho * gd.R) + 0.5 * mod_v * mod_v;
+              const double E0 = int_energy(gd, ddiv_rn(p, rho * gd.R)) + 0.5 * mod_v * mod_v;
               double S[4];
               if (C.source == Q2D_SRC_AXI) {
                 const double invr = gp(B.sb)[oA];
@@ -1535,7 +1553,7 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
                 const double rr = ddiv_rn(-res[v], sr) * dt;
                 U[v] = Uo[v] + (U[v] - Uo[v] + rr) * c_rk;   // RK_stage — Lib_RK.f90:15-27
               }
-              cons2prim(gd, U, qn2);
+              cons2prim(gd, U, qn2, TAB ? ddiv_rn(qA[3], qA[0] * gd.R) : 0.0);   // estimate: Lib_Newstate.f90:151-154
             } else {
               // mixed form: FP64 update with one reciprocal of sur and of rho
               const double dts = dt * rcp64(sr);
@@ -1560,7 +1578,7 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
             if (last && FUSE) {
               // Compute_dt of the next step (Mod_dt.f90:59-104) on the state just written; the
               // next step's Compute_dt would read exactly these values.
-              const double snd = dsqrt_rn(ddiv_rn(C.gam * qn2[3], qn2[0]));
+              const double snd = TAB ? sound<double>(gd, qn2[3], qn2[0]) : dsqrt_rn(ddiv_rn(C.gam * qn2[3], qn2[0]));
               double dtmin = 1e8;
               const double* mm = gcur.mm;                 // c11, c21, c12, c22
               {
@@ -1625,8 +1643,8 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
     } else if constexpr (RIE_T < 0) {
       if (C.riemann == Q2D_RIEMANN_HLLE) item_body(std::integral_constant<int, 1>{});
       else if (C.riemann == Q2D_RIEMANN_HLLEM) item_body(std::integral_constant<int, 2>{});
-      else if (RIE_T == -2 && C.riemann == Q2D_RIEMANN_HLLEPP) item_body(std::integral_constant<int, 3>{});
-      else if (RIE_T == -2 && C.riemann == Q2D_RIEMANN_HLLCP) item_body(std::integral_constant<int, 4>{});
+      else if ((RIE_T == -2 || TAB) && C.riemann == Q2D_RIEMANN_HLLEPP) item_body(std::integral_constant<int, 3>{});
+      else if ((RIE_T == -2 || TAB) && C.riemann == Q2D_RIEMANN_HLLCP) item_body(std::integral_constant<int, 4>{});
       else item_body(std::integral_constant<int, 0>{});
     } else {
       item_body(std::integral_constant<int, RIE_T>{});
@@ -2131,7 +2149,7 @@ __device__ __forceinline__ void irs_i_at(const DevBlock* __restrict__ blocks, co
 // into its last stage (with the VNN limit for laminar cases).
 __device__ __forceinline__ double cell_dt_fused(const DevCase& C, int iter, const double q[4], const double mm[4],
                                                 double dlx, double dly) {
-  const double snd = sqrt(C.gam * q[3] / q[0]);
+  const double snd = sound<double>(case_gas(C), q[3], q[0]);
   double dtmin = 1e8;
   {
     const double m1 = mm[0], m2 = mm[2];
@@ -2181,7 +2199,7 @@ __global__ __launch_bounds__(256) void k_irs(const DevBlock* __restrict__ blocks
     GP Rs = gp(B.irs[0]);
     const bool tacc = C.time_accurate != 0;
     const bool mixed = C.precision == Q2D_PREC_MIXED;
-    const Gas<double> gd{C.R, C.cp, C.cv, C.gam};
+    const Gas<double> gd = case_gas(C);
     const double c_rk = (C.n_rk == 1) ? 1.0 : (C.n_rk == 2 ? (irk == 1 ? 1.0 : 0.5)
                                                             : (irk == 1 ? 1.0 : (irk == 2 ? 0.25 : 2.0 / 3.0)));
     const int stage_id = irk;
@@ -2298,7 +2316,7 @@ __global__ __launch_bounds__(256) void k_irs(const DevBlock* __restrict__ blocks
 #pragma unroll
         for (int v = 0; v < 4; ++v) U[v] = Uo[v] + (U[v] - Uo[v] + s2[v]) * c_rk;   // RK_stage — Lib_RK.f90:15-27
         if (!mixed) {
-          cons2prim(gd, U, qn2);
+          cons2prim(gd, U, qn2, ddiv_rn(q[3], q[0] * gd.R));   // estimate: Lib_Newstate.f90:119-123
         } else {
           const double ir = rcp64(U[0]);
           const double vx = U[1] * ir, vy = U[2] * ir;
@@ -2477,6 +2495,8 @@ struct q2d_handle {
   bool any_visc = false;
   bool any_irs = false;          // a live case smooths its residual (k_irs after each stage)
   bool any_hllepp = false;       // a live case uses HLLE++ or HLLC+: the stage kernel's RIE_T = -2 instantiation
+  bool any_table = false;        // a live case uses property tables: the FP64 stage kernel's RIE_T = -4 one
+  std::vector<double*> case_tab; // per case: its tables on the device (table mode), else null
   bool any_det_or_hllem = false; // a live case uses HLLEM or the detector: not the RIE_T = -3 instantiation
   bool mixed_generic = false;    // mixed form with a MUSCL-SD / HLLEM / HLLE++ / HLLC+ case: those cases' Euler
                                  // items run the generic FP32-face kernel k_stage<float> (k_stage_mx has no
@@ -2529,6 +2549,9 @@ const void* stage_kernel(const q2d_handle* h, int pass) {
     return h->any_det_or_hllem ? reinterpret_cast<const void*>(&k_stage<float, -1, kFuseDtMx, true>)
                                : reinterpret_cast<const void*>(&k_stage<float, -3, kFuseDtMx, true>);
   }
+  if (h->any_table)
+    return pass < 2 ? reinterpret_cast<const void*>(&k_stage<double, -4, kFuseDtFp64, false>)
+                    : reinterpret_cast<const void*>(&k_stage<double, -4, kFuseDtFp64, true>);
   if (h->any_hllepp)
     return pass < 2 ? reinterpret_cast<const void*>(&k_stage<double, -2, kFuseDtFp64, false>)
                     : reinterpret_cast<const void*>(&k_stage<double, -2, kFuseDtFp64, true>);
@@ -2756,6 +2779,9 @@ int launch_stage(q2d_handle* h, int irk, const StepScratch& scr) {
       else if (pass == 1) Q2D_STAGE_LAUNCH(-1, false);
       else if (h->any_det_or_hllem) Q2D_STAGE_LAM_MX_LAUNCH(-1);
       else Q2D_STAGE_LAM_MX_LAUNCH(-3);                // laminar HLLC / HLLE cases
+    } else if (h->any_table) {                         // property tables (plain FP64 form only)
+      if (pass == 0) Q2D_STAGE_LAUNCH(-4, false);
+      else Q2D_STAGE_LAUNCH(-4, true);
     } else if (h->any_hllepp) {                        // HLLE++ / HLLC+ (plain FP64 form only)
       if (pass == 0) Q2D_STAGE_LAUNCH(-2, false);
       else Q2D_STAGE_LAUNCH(-2, true);
@@ -2943,6 +2969,7 @@ int q2d_gpu_destroy(q2d_handle* h) {
   (void)hipFree(h->d_bcdense);
   (void)hipFree(h->d_tstamp);
   (void)hipFree(h->d_vrec); (void)hipFree(h->d_dtr);
+  for (double* t : h->case_tab) if (t) (void)hipFree(t);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
   return Q2D_OK;
@@ -2993,6 +3020,7 @@ int rebuild_descriptors(q2d_handle* h) {
   h->any_visc = false;
   h->any_irs = false;
   h->any_hllepp = false;
+  h->any_table = false;
   h->any_det_or_hllem = false;
   h->mixed_generic = false;
   h->bc_two_phase = false;
@@ -3004,6 +3032,7 @@ int rebuild_descriptors(q2d_handle* h) {
     if (h->dc[c].model > 0) h->any_visc = true;
     if (h->dc[c].irs_beta > 0.0) h->any_irs = true;
     if (h->dc[c].riemann >= Q2D_RIEMANN_HLLEPP) h->any_hllepp = true;
+    if (h->dc[c].tab) h->any_table = true;
     if (h->dc[c].riemann >= Q2D_RIEMANN_HLLEM || h->dc[c].space_sd) h->any_det_or_hllem = true;
     if (h->precision_mixed && (h->dc[c].space_sd || h->dc[c].riemann >= Q2D_RIEMANN_HLLEM)) h->mixed_generic = true;
     if (h->dc[c].space_sd || h->dc[c].riemann >= Q2D_RIEMANN_HLLEPP)
@@ -3064,7 +3093,18 @@ int append_cases(q2d_handle* h, const q2d_problem* prob, int32_t* first_case) {
       return fail(Q2D_E_ARG, "laminar model needs T_ref > 0, S >= 0, Pr > 0");
     if (s.source < 0 || s.source > 2) return fail(Q2D_E_ARG, "unknown source");
     if (s.precision < 0 || s.precision > 1) return fail(Q2D_E_ARG, "unknown precision");
-    if (!(s.gas_R > 0) || !(s.gas_cp > s.gas_R)) return fail(Q2D_E_ARG, "gas constants invalid");
+    const q2d_case_desc& cdv = prob->cases[c];
+    if (!(s.gas_R > 0) || (!cdv.thermo && !(s.gas_cp > s.gas_R))) return fail(Q2D_E_ARG, "gas constants invalid");
+    if (cdv.thermo) {                                  // table mode: e(T) = h(T) - R T must increase
+      if (cdv.thermo_n < 2) return fail(Q2D_E_ARG, "thermo tables need thermo_n >= 2");
+      if (s.precision == Q2D_PREC_MIXED)
+        return fail(Q2D_E_UNSUPPORTED, "table-mode thermodynamics is available in the plain FP64 form only");
+      for (int k = 0; k < 4 * cdv.thermo_n; ++k)
+        if (!std::isfinite(cdv.thermo[k])) return fail(Q2D_E_ARG, "thermo tables hold a non-finite value");
+      for (int k = 0; k + 1 < cdv.thermo_n; ++k)
+        if (!((cdv.thermo[k + 1] - cdv.thermo[k]) - s.gas_R > 0.0))
+          return fail(Q2D_E_ARG, "thermo tables: h(T) - R T must increase on every interval");
+    }
     if (s.precision != prec) return fail(Q2D_E_ARG, "all cases of a handle must use one precision");
     if (prob->cases[c].res_drop < 0 || prob->cases[c].res_drop >= 1) return fail(Q2D_E_ARG, "res_drop must be in [0, 1)");
     if (!(s.irs_beta >= 0) || !std::isfinite(s.irs_beta)) return fail(Q2D_E_ARG, "irs_beta must be finite and >= 0");
@@ -3158,6 +3198,17 @@ int append_cases(q2d_handle* h, const q2d_problem* prob, int32_t* first_case) {
     k.ndir = (prob->blocks[first_block[c]].nj == 1) ? 1 : 2;     // Check_Mesh_Type (Lib_Metrics.f90:24-37)
     k.itermax = cd.itermax; k.cfl = s.cfl; k.vnn = s.vnn;
     k.R = s.gas_R; k.cp = s.gas_cp; k.cv = s.gas_cp - s.gas_R; k.gam = s.gas_cp / k.cv;
+    double* dtab = nullptr;
+    if (cd.thermo) {                                   // table mode: the case's device copy of its tables
+      const size_t tb = sizeof(double) * 4 * (size_t)cd.thermo_n;
+      HIPCHK(hipMalloc(&dtab, tb));
+      HIPCHK(hipMemcpy(dtab, cd.thermo, tb, hipMemcpyHostToDevice));
+      k.tab = dtab; k.ntab = cd.thermo_n;
+      if (!(s.gas_cp > s.gas_R)) {                     // closed-form fields unused; kept finite
+        k.cp = cd.thermo[cd.thermo_n + 1]; k.cv = k.cp - k.R; k.gam = k.cp / k.cv;
+      }
+    }
+    h->case_tab.push_back(dtab);
     k.res_threshold = cd.res_threshold; k.time_threshold = cd.time_threshold; k.res_drop = cd.res_drop;
     k.mu_ref = s.mu_ref; k.T_ref = s.T_ref; k.S_suth = s.S_suth; k.Pr = s.Pr;
     k.irs_beta = s.irs_beta;
@@ -3367,13 +3418,15 @@ int q2d_gpu_add_cases(q2d_handle* h, const q2d_problem* prob, int32_t* first_cas
   // the next step runs the dt pass: the added cases have no dt from a previous last stage (the dt pass
   // and the fused dt evaluate the same formula, so the cases already running are not affected)
   h->dt_fused_valid = false;
-  if (rc != Q2D_OK && (h->nblocks != nb0 || h->ncases != nc0)) {
+  if (rc != Q2D_OK && (h->nblocks != nb0 || h->ncases != nc0 || h->dc.size() != (size_t)nc0 ||
+                       h->case_tab.size() != (size_t)nc0)) {
     // roll back a partial append: free what was allocated and rebuild from the previous set
     std::string msg = g_err;
     for (int b = nb0; b < h->nblocks; ++b) if (h->hb[b].arena) { (void)hipFree(h->hb[b].arena); h->arena_bytes -= h->hb[b].arena_bytes; }
+    for (size_t c = nc0; c < h->case_tab.size(); ++c) if (h->case_tab[c]) (void)hipFree(h->case_tab[c]);
     h->hb.resize(nb0); h->db.resize(nb0); h->nblocks = nb0;
     h->dbc.resize(nbc0);
-    h->dc.resize(nc0); h->case_live.resize(nc0); h->ncases = nc0;
+    h->dc.resize(nc0); h->case_live.resize(nc0); h->case_tab.resize(nc0); h->ncases = nc0;
     (void)rebuild_descriptors(h);
     g_err = msg;
   }
@@ -3385,6 +3438,11 @@ int q2d_gpu_remove_case(q2d_handle* h, int32_t case_id) {
   if (!h->case_live[case_id]) return fail(Q2D_E_ARG, "case already removed");
   HIPCHK(hipStreamSynchronize(h->stream));
   h->case_live[case_id] = 0;
+  if (h->case_tab[case_id]) {                          // its tables (the stream is idle)
+    (void)hipFree(h->case_tab[case_id]);
+    h->case_tab[case_id] = nullptr;
+    h->dc[case_id].tab = nullptr; h->dc[case_id].ntab = 0;
+  }
   const int zero = 0;
   HIPCHK(hipMemcpy((char*)(h->d_ctrl + case_id) + offsetof(DevCtrl, active), &zero, sizeof(int), hipMemcpyHostToDevice));
   for (int b = 0; b < h->nblocks; ++b) {

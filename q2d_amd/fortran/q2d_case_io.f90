@@ -49,7 +49,7 @@ module q2d_case_io
     real(c_double), allocatable :: sa(:,:), sb(:,:)  ! (1:ni, 1:nj) axis r_ave, 1/r
   end type
 
-  public :: read_input_ini, read_gas, read_bc_file, read_tec_field, setup_geometry, fill_initial_ghosts
+  public :: read_input_ini, read_gas, read_tables, read_bc_file, read_tec_field, setup_geometry, fill_initial_ghosts
   public :: write_field, lower, fail_stop
 
 contains
@@ -140,10 +140,12 @@ contains
     end if
   end subroutine read_input_ini
 
-  ! INPUT/phase.txt (one species) and INPUT/thermo.dat; returns R and cp of a constant-cp table.
-  subroutine read_gas(dir, R, cp)
+  ! INPUT/phase.txt (one species) and INPUT/thermo.dat; returns R and cp of a constant-cp table.  With
+  ! tables = .true. a varying cp is accepted (table mode reads the table itself, read_tables).
+  subroutine read_gas(dir, R, cp, tables)
     character(len=*), intent(in) :: dir
     real(c_double), intent(out) :: R, cp
+    logical, intent(in) :: tables
     integer :: u, ios, n
     character(len=256) :: line, name
     real(c_double) :: W, T, c, h, cp0
@@ -169,14 +171,80 @@ contains
       if (ios /= 0) cycle
       n = n + 1
       if (n == 1) cp0 = c
-      if (abs(c - cp0) > 1d-12 * abs(cp0) .or. abs(h - cp0 * T) > 1d-9 * abs(cp0 * T)) &
-        call fail_stop('INPUT/thermo.dat: cp varies with T (table-mode thermodynamics is not on this path)')
+      if (.not. tables .and. (abs(c - cp0) > 1d-12 * abs(cp0) .or. abs(h - cp0 * T) > 1d-9 * abs(cp0 * T))) &
+        call fail_stop('INPUT/thermo.dat: cp varies with T (the closed form needs a constant cp; run with --tables)')
     end do
     close(u)
     if (n == 0) call fail_stop('INPUT/thermo.dat: no table rows')
     R = R_UNIVERSAL / W
     cp = cp0
   end subroutine read_gas
+
+  ! The property planes of table mode (q2d_case_desc%thermo): h, cp, mu, k with node k at T = k K, k = 0 ..
+  ! n-1, from the rows T = 1, 2, ... of INPUT/thermo.dat (T, Cp, Enthalpy, ...) and, where it exists,
+  ! INPUT/transport.dat (T, Viscosity, Conductivity; the planes cover the shorter file); node 0 is
+  ! extrapolated linearly from nodes 1 and 2.  Plane q occupies tab(q*n+1 : q*n+n).
+  subroutine read_tables(dir, tab, n)
+    character(len=*), intent(in) :: dir
+    real(c_double), allocatable, intent(out) :: tab(:)
+    integer, intent(out) :: n
+    real(c_double), allocatable :: th(:,:), tr(:,:)
+    integer :: nth, ntr, m, q
+    logical :: have_tr
+    call read_rows(dir//'/INPUT/thermo.dat', 3, th, nth)
+    inquire(file=dir//'/INPUT/transport.dat', exist=have_tr)
+    m = nth
+    if (have_tr) then
+      call read_rows(dir//'/INPUT/transport.dat', 3, tr, ntr)
+      m = min(nth, ntr)
+      if (any(tr(1, 1:m) /= th(1, 1:m))) call fail_stop('thermo.dat and transport.dat list different temperatures')
+    end if
+    if (m < 2) call fail_stop('INPUT/thermo.dat: fewer than two table rows')
+    do q = 1, m
+      if (th(1, q) /= real(q, c_double)) call fail_stop('property tables must list T = 1, 2, 3, ... K')
+    end do
+    n = m + 1
+    allocate(tab(4 * n))
+    tab = 0d0
+    tab(2:n) = th(3, 1:m)                          ! h
+    tab(n + 2:2 * n) = th(2, 1:m)                  ! cp
+    if (have_tr) then
+      tab(2 * n + 2:3 * n) = tr(2, 1:m)            ! mu
+      tab(3 * n + 2:4 * n) = tr(3, 1:m)            ! k
+    end if
+    do q = 0, 3
+      tab(q * n + 1) = 2d0 * tab(q * n + 2) - tab(q * n + 3)
+    end do
+  end subroutine read_tables
+
+  ! the numeric rows of a Tecplot POINT table: the first ncol values of each, as columns of v(ncol, nrow)
+  subroutine read_rows(path, ncol, v, nrow)
+    character(len=*), intent(in) :: path
+    integer, intent(in) :: ncol
+    real(c_double), allocatable, intent(out) :: v(:,:)
+    integer, intent(out) :: nrow
+    integer :: u, ios, pass
+    character(len=512) :: line
+    real(c_double) :: x(8)
+    do pass = 1, 2
+      open(newunit=u, file=path, status='old', action='read', iostat=ios)
+      if (ios /= 0) call fail_stop('cannot open '//path)
+      nrow = 0
+      do
+        read(u, '(a)', iostat=ios) line
+        if (ios /= 0) exit
+        line = adjustl(line)
+        if (len_trim(line) == 0) cycle
+        if (scan(line(1:1), '0123456789.-+') == 0) cycle
+        read(line, *, iostat=ios) x(1:ncol)
+        if (ios /= 0) cycle
+        nrow = nrow + 1
+        if (pass == 2) v(:, nrow) = x(1:ncol)
+      end do
+      close(u)
+      if (pass == 1) allocate(v(ncol, max(1, nrow)))
+    end do
+  end subroutine read_rows
 
   ! a direction token of a property line: 'normal' -> huge (Compute_Inflow_Direction), else a number
   real(c_double) function dir_token(tok)

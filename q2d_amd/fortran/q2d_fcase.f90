@@ -7,8 +7,11 @@
 ! TODO test ends the run (iter-, res- or time-threshold), appending residual-history.dat every res-diter
 ! steps, and write OUTPUT/field.tec with its SOLUTIONTIME.
 !
-! usage: q2d_fcase <case directory> [--mixed] [--max-steps N] [--dump <file>]
+! usage: q2d_fcase <case directory> [--mixed] [--tables] [--max-steps N] [--dump <file>]
 !   --mixed      the mixed arithmetic form (FP32 faces, FP64 state) instead of plain FP64
+!   --tables     table-mode thermodynamics: the property planes of INPUT/thermo.dat (and transport.dat),
+!                interpolated from node int(T), as FLINT's native tabulated mode (plain FP64 form only);
+!                without it the closed forms of a constant-cp gas (a varying cp is refused)
 !   --max-steps  stop after N steps even if no threshold is reached
 !   --dump       write the problem the host built (case-file format of q2d_fdriver) and stop before the
 !                device is touched
@@ -29,17 +32,20 @@ program q2d_fcase
   character(len=1024) :: dir, arg, dump
   real(c_double) :: R, cp, soltime, t0, t1
   integer :: nb, b, nrec, rc, ndev, k, u, iter_general, max_steps, nsteps, chunk
-  logical :: mixed
+  logical :: mixed, tables
   integer(c_int64_t) :: c0, c1, crate
+  real(c_double), allocatable, target :: tab(:)
+  integer :: ntab
 
   call get_command_argument(1, dir)
   if (len_trim(dir) == 0) call fail_stop('usage: q2d_fcase <case directory> [--mixed] [--max-steps N] [--dump file]')
-  mixed = .false.; max_steps = huge(1); dump = ''
+  mixed = .false.; tables = .false.; max_steps = huge(1); dump = ''
   k = 2
   do while (k <= command_argument_count())
     call get_command_argument(k, arg)
     select case (trim(arg))
     case ('--mixed'); mixed = .true.
+    case ('--tables'); tables = .true.
     case ('--max-steps'); k = k + 1; call get_command_argument(k, arg); read(arg, *) max_steps
     case ('--dump'); k = k + 1; call get_command_argument(k, dump)
     case default; call fail_stop('unknown option '//trim(arg))
@@ -53,7 +59,9 @@ program q2d_fcase
     call fail_stop('equations = '//trim(cfg%equations)//': this host runs the Euler equations (transport tables are not read)')
   if (trim(cfg%source) /= 'none' .and. trim(cfg%source) /= 'axisymmetric') &
     call fail_stop('[Q2D-Sources] active = '//trim(cfg%source)//' is not read by this host')
-  call read_gas(trim(dir), R, cp)
+  call read_gas(trim(dir), R, cp, tables)
+  ntab = 0
+  if (tables) call read_tables(trim(dir), tab, ntab)
   if (cfg%newrun) then
     call read_tec_field(trim(dir)//'/INPUT/ic.tec', blk, soltime)
     soltime = 0d0
@@ -87,6 +95,8 @@ program q2d_fcase
   cs(1)%itermax = cfg%iter_threshold; cs(1)%reserved = 0
   cs(1)%res_threshold = cfg%res_threshold; cs(1)%time_threshold = cfg%time_threshold
   cs(1)%res_drop = 0d0
+  cs(1)%thermo = 0; cs(1)%thermo_n = ntab; cs(1)%reserved2 = 0
+  if (tables) cs(1)%thermo = transfer(c_loc(tab), cs(1)%thermo)
   ! domain%time at set-up (Wrap_Setup.f90:129-152)
   iter_general = 0
   if (cfg%time_accurate) then
@@ -205,10 +215,13 @@ contains
   subroutine write_dump(path)
     character(len=*), intent(in) :: path
     integer :: v, q, flags
+    type(q2d_case_desc) :: cw(1)
+    cw = cs
+    cw(1)%thermo = 0                                   ! the tables follow the blocks
     open(newunit=v, file=path, access='stream', form='unformatted', status='replace')
     write(v) int([1362248754, nb, nrec, 1, 0], c_int32_t), int(c_sizeof(cs(1)), c_int32_t), &
              int(c_sizeof(recs(1)), c_int32_t), int(c_sizeof(out(1)), c_int32_t)
-    write(v) cs
+    write(v) cw
     write(v) recs
     do q = 1, nb
       flags = 1
@@ -219,6 +232,7 @@ contains
       if (allocated(blk(q)%sa)) write(v) blk(q)%sa
       if (allocated(blk(q)%sb)) write(v) blk(q)%sb
     end do
+    if (tables) write(v) tab
     close(v)
   end subroutine write_dump
 

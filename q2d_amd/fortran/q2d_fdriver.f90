@@ -14,6 +14,8 @@
 !     per block: int32 ni, nj, case_id, flags (1 node, 2 src_a, 4 src_b), then
 !       P(4,ni+4,nj+4), dl(2,ni+4,nj+4), M(4,ni+4,nj+4), sur(ni+4,nj+4),
 !       face_i(3,ni+1,nj), face_j(3,ni,nj+1) [, node(2,ni+1,nj+1)] [, src_a(ni,nj)] [, src_b(ni,nj)]
+!     per case with thermo_n > 0 (table mode): its property planes, 4 x thermo_n doubles (the file's
+!       thermo pointers are null; this host points them at its own copies)
 !   output: per block P(4,ni+4,nj+4); then ncases x q2d_step_out
 program q2d_fdriver
   use iso_c_binding
@@ -25,6 +27,11 @@ program q2d_fdriver
     real(c_double), allocatable :: P(:,:,:), dl(:,:,:), M(:,:,:), sur(:,:), fi(:,:,:), fj(:,:,:)
     real(c_double), allocatable :: node(:,:,:), sa(:,:), sb(:,:)
   end type
+  type :: case_tables
+    real(c_double), allocatable :: tab(:)
+  end type
+  type(case_tables), allocatable, target :: tabs(:)
+  integer :: c
 
   integer(c_int32_t) :: hdr(8)
   type(block_arrays), allocatable, target :: blk(:)
@@ -73,6 +80,15 @@ program q2d_fdriver
         allocate(x%sb(x%ni, x%nj)); read(u) x%sb
       end if
     end associate
+  end do
+  allocate(tabs(hdr(4)))
+  do c = 1, hdr(4)
+    cs(c)%thermo = 0
+    if (cs(c)%thermo_n > 0) then
+      allocate(tabs(c)%tab(4 * cs(c)%thermo_n))
+      read(u) tabs(c)%tab
+      cs(c)%thermo = transfer(c_loc(tabs(c)%tab), cs(c)%thermo)
+    end if
   end do
   close(u)
 

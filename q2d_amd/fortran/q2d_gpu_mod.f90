@@ -49,6 +49,10 @@ module q2d_gpu
     real(c_double)     :: res_threshold, time_threshold
     real(c_double)     :: res_drop
     real(c_double)     :: time0
+    integer(c_intptr_t) :: thermo             ! table mode: address of planes h, cp, mu, k (4 x thermo_n), else 0
+                                              ! (an address integer, so that the type stays readable by
+                                              ! unformatted I/O: transfer(c_loc(tab), 0_c_intptr_t))
+    integer(c_int32_t) :: thermo_n, reserved2
   end type
 
   type, bind(C), public :: q2d_problem

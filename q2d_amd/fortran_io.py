@@ -26,6 +26,9 @@ def write_case_file(problem, path, nsteps):
     pc, keep = to_problem_c(problem)
     blocks, bcs, cases, _ = keep
     nb, nbc, nc = len(problem.blocks), len(problem.bc), len(problem.cases)
+    ptrs = [cases[k].thermo for k in range(nc)]
+    for k in range(nc):                     # pointers of this process mean nothing in the file
+        cases[k].thermo = None
     with open(path, "wb") as f:
         f.write(struct.pack(HEADER, MAGIC, nb, nbc, nc, int(nsteps), C.sizeof(cases[0]) if nc else 0,
                             C.sizeof(bcs[0]), C.sizeof(StepOut)))
@@ -41,6 +44,11 @@ def write_case_file(problem, path, nsteps):
             for a in (g.node, b.src_a, b.src_b):
                 if a is not None:
                     f.write(np.ascontiguousarray(a, dtype=np.float64).tobytes())
+        for c in problem.cases:             # table mode: the property planes of each such case
+            if c.thermo is not None:
+                f.write(np.ascontiguousarray(c.thermo, dtype=np.float64).tobytes())
+    for k in range(nc):
+        cases[k].thermo = ptrs[k]
 
 
 def read_output(path, problem):
@@ -96,4 +104,12 @@ def read_case_file(path):
         if flags & 4:
             blk["src_b"] = take((nj, ni))
         blocks.append(blk)
-    return {"nsteps": nsteps, "cases": cs, "bc": bcs, "blocks": blocks}
+    tables = []
+    for c in cs:
+        n = int(c.thermo_n)
+        if n > 0:
+            tables.append(np.frombuffer(raw, dtype=np.float64, count=4 * n, offset=off).reshape(4, n).copy())
+            off += 32 * n
+        else:
+            tables.append(None)
+    return {"nsteps": nsteps, "cases": cs, "bc": bcs, "blocks": blocks, "tables": tables}

@@ -1,0 +1,109 @@
+"""Table-mode thermodynamics on the device (needs the GPU): FLINT's native tabulated properties of one
+species (q2d_case_desc.thermo), in the plain FP64 form.  Bit-identical to the oracle's table mode on a
+moving shock of a gas whose cp varies with T (every Riemann solver), a 2D ramp with the shock detector
+and residual smoothing, the characteristic outflow BC 406, an axisymmetric laminar pipe with viscosity and
+conductivity tables, and the reference's Shu-Osher case directory with its Air table.  A closed-form case
+batched beside table cases keeps its solo bits; the mixed form and a table whose e(T) does not increase
+are refused."""
+import math
+from dataclasses import replace
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from q2d_amd import GpuSolver, Q2DError, casedir, cases, thermo as th
+
+pytestmark = pytest.mark.gpu
+CASE = Path(__file__).resolve().parent / "golden" / "cases" / "ShuOsher-LIMO3"
+
+
+def both(oracle_mod, pb, steps):
+    print(f"table-mode case: {pb.meta.get('name')} {[(b.ni, b.nj) for b in pb.blocks]} {steps} steps", flush=True)
+    o = oracle_mod.Oracle(pb)
+    rc, oo = o.run(steps, stop_on_todo=False)
+    assert rc == 0
+    g = GpuSolver(pb)
+    go = g.run(steps)
+    assert [x["iter"] for x in go] == [x["iter"] for x in oo]
+    for b in range(len(pb.blocks)):
+        Pg, Po = g.state(b)[2:-2, 2:-2], o.state(b)[2:-2, 2:-2]
+        assert np.array_equal(Pg, Po), (b, np.abs(Pg - Po).max())
+    for k in range(len(pb.cases)):
+        assert go[k]["dtglobal"] == oo[k]["dtglobal"] and go[k]["time"] == oo[k]["time"]
+    return g, go
+
+
+def nondim_table(n=64):
+    """cp(T) = 3 + T / 2 (R = 1): gamma = 1.4 at T = 1, the generators' free stream."""
+    return th.linear_cp(3.0, 0.5, n)
+
+
+@pytest.mark.parametrize("riemann", ["HLLC", "HLLE", "HLLEM", "HLLE++", "HLLC+"])
+def test_moving_shock_varying_cp(oracle_mod, riemann):
+    pb = cases.riemann_planar(seed=3, ni=300, nj=4, steps=10 ** 6, riemann=riemann, limiter="vanleer", n_rk=3)
+    pb.cases[0] = replace(pb.cases[0], thermo=nondim_table())
+    P = pb.blocks[0].P
+    P[2:-2, 2:-2, 3] *= 4.0                             # T from 0.4 to 4: several table intervals
+    cases.fill_initial_ghosts(P)
+    both(oracle_mod, pb, 200)
+
+
+def test_ramp_detector_and_smoothing(oracle_mod):
+    pb = cases.ramp_channel(ni=96, nj=48, wedge_deg=12.0, mach=2.5, cfl=0.9, itermax=10 ** 6)
+    pb.cases[0] = replace(pb.cases[0], thermo=nondim_table(), space_sd=True, irs_beta=0.5, ramp_iter=100)
+    both(oracle_mod, pb, 150)
+
+
+def test_outflow_406(oracle_mod):
+    pb = cases.outflow_channel(mach=0.5, p_exit=1.2, rel_fac=0.5)
+    pb.add(cases.outflow_channel(mach=2.5, p_exit=1.0, rel_fac=1.0))
+    for k in range(len(pb.cases)):
+        pb.cases[k] = replace(pb.cases[k], thermo=nondim_table())
+    both(oracle_mod, pb, 100)
+
+
+def test_laminar_pipe_transport_tables(oracle_mod):
+    pb = cases.axisymmetric_pipe(seed=1, ni=64, nj=24, first_cell=5e-4, reynolds=1e4, precision="fp64", steps=10 ** 6)
+    c = pb.cases[0]
+    tab = nondim_table()
+    T = np.arange(tab.shape[1], dtype=np.float64)
+    tab[2] = c.mu_ref * (0.3 + 0.7 * T)                  # mu(T), k(T) = mu cp / Pr at the nodes
+    tab[3] = tab[2] * tab[1] / c.Pr
+    pb.cases[0] = replace(c, thermo=tab)
+    both(oracle_mod, pb, 60)
+
+
+def test_shu_osher_case_directory(oracle_mod):
+    pb = casedir.load_case(CASE, thermo_mode="table")
+    g, go = both(oracle_mod, pb, 300)
+    assert go[0]["status"] == 0
+
+
+def test_closed_form_case_beside_table_cases_keeps_its_bits():
+    a = cases.riemann_planar(seed=11, ni=200, nj=8, steps=10 ** 6)
+    solo = GpuSolver(a)
+    so = solo.run(40)[0]
+    t = cases.riemann_planar(seed=12, ni=100, nj=8, steps=10 ** 6)
+    t.cases[0] = replace(t.cases[0], thermo=nondim_table())
+    g = GpuSolver(cases.Problem())
+    ca, _ = g.add_cases(a)
+    g.add_cases(t)
+    out = g.run(40)[ca]
+    assert out["iter"] == 40 and out["dtglobal"] == so["dtglobal"] and out["time"] == so["time"]
+    assert np.array_equal(g.state(0)[2:-2, 2:-2], solo.state(0)[2:-2, 2:-2])
+
+
+def test_refusals():
+    pb = cases.riemann_planar(seed=1, ni=64, nj=4, steps=10, precision="mixed")
+    pb.cases[0] = replace(pb.cases[0], thermo=nondim_table())
+    with pytest.raises(Q2DError) as e:
+        GpuSolver(pb)
+    assert e.value.code == -6 and "FP64" in str(e.value)
+    bad = nondim_table()
+    bad[0, 10] = bad[0, 9]                               # h flat on one interval: e(T) decreases
+    pb = cases.riemann_planar(seed=1, ni=64, nj=4, steps=10)
+    pb.cases[0] = replace(pb.cases[0], thermo=bad)
+    with pytest.raises(Q2DError) as e:
+        GpuSolver(pb)
+    assert "increase" in str(e.value)
