@@ -85,3 +85,15 @@ def test_case_directory_reproduces_the_limiter_baselines(oracle_mod, lim):
     ref = baseline(lim)
     rho = o.state(0)[2, 2:-2, 0]
     assert (np.abs(rho - ref[:, 1]) / ref[:, 1]).max() < 1e-10
+
+
+@pytest.mark.skipif(not FCASE.exists(), reason="Fortran host not built (make -C q2d_amd/fortran)")
+def test_fortran_host_reads_the_tables(tmp_path):
+    """q2d_fcase --tables builds the same planes as q2d_amd.thermo from INPUT/thermo.dat."""
+    r = subprocess.run([str(FCASE), str(CASE), "--tables", "--dump", str(tmp_path / "case.bin")],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    d = read_case_file(tmp_path / "case.bin")
+    pb = casedir.load_case(CASE, thermo_mode="table")
+    assert d["cases"][0].thermo_n == pb.cases[0].thermo.shape[1]
+    assert np.array_equal(d["tables"][0], pb.cases[0].thermo)

@@ -88,3 +88,19 @@ def test_restart_from_the_written_field(tmp_path):
     assert np.abs(cv[0, 0] - cv_full[0, 0]).max() / cv_full[0, 0].max() < 1e-9
     hist = casedir.read_residual_history(d / "OUTPUT" / "residual-history.dat")
     assert hist[0, 0] == 1 and np.all(np.diff(hist[:, 1]) > 0)
+
+
+@needs_host
+def test_table_mode_case_directory(oracle_mod, tmp_path):
+    """q2d_fcase --tables: the case's INPUT/thermo.dat as table-mode thermodynamics.  The written field
+    equals the oracle's table-mode run to the printed digits and passes the LIMO3 gates."""
+    d = copy_case(tmp_path)
+    soltime, xc, cv = run_case(d, "--tables")
+    assert limo3_gates(xc, cv[0, 0]) == []
+    pb = casedir.load_case(CASE, thermo_mode="table")
+    o = oracle_mod.Oracle(pb)
+    rc, out = o.run(100000)
+    assert rc == 0 and soltime == float(f"{out[0]['time']:.15E}")
+    Po = o.state(0)[2, 2:-2]
+    for k in range(4):
+        assert np.allclose(cv[k, 0], Po[:, k], rtol=2e-15, atol=1e-300), k
