@@ -231,6 +231,12 @@ def c3_c4(args):
 def other_workload(args):
     """--workload axi | batch: the C3 or C4 configuration alone, on its own JSON line."""
     from q2d_amd import cases
+    if args.workload == "axi" and args.form != "both":   # profiling aid: one form, no JSON line
+        pb = cases.axisymmetric_pipe(seed=args.seed, precision="fp64", steps=1 << 30)
+        r = time_form(pb if args.form == "fp64" else with_precision(pb, "mixed"), args.steps, args.warmup)
+        log(f"bench: C3 {args.form} {r['cell_updates_per_s'] / 1e9:.3f} G cell-updates/s, "
+            f"stage kernel {r['stage_kernel_ms']:.4f} ms")
+        return
     if args.workload == "axi":
         pb = cases.axisymmetric_pipe(seed=args.seed, precision="fp64", steps=1 << 30)
         r = config_line("C3", pb, args.steps, args.warmup, 2000, 3 * 216 + 88, desc="C3 axisymmetric laminar 2048x512")

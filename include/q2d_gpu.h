@@ -33,10 +33,12 @@
  * exception crosses the boundary.  q2d_last_error() returns a message for the
  * last failing call on the calling thread.
  *
- * Gas model: single species, calorically perfect (constant cp, gas constant R).
- * This is the closed form of the FLINT thermodynamic calls the reference makes
- * (co_rotot_Rtot, f_ss, H0, E0, prim2cons, cons2prim); FLINT itself is not part
- * of the reference tree.
+ * Gas model: one species, either calorically perfect (constant cp, gas constant
+ * R: the closed forms of the FLINT calls the reference makes, co_rotot_Rtot,
+ * f_ss, f_gamma, H0, E0, prim2cons, cons2prim) or, with q2d_case_desc.thermo,
+ * FLINT's native tabulated mode: h, cp, mu and k interpolated linearly from
+ * integer-kelvin tables (plain FP64 form).  FLINT itself is not part of the
+ * reference tree.
  */
 #ifndef Q2D_GPU_H
 #define Q2D_GPU_H
@@ -111,7 +113,7 @@ typedef struct q2d_scheme {
   int32_t precision;       /* Q2D_PREC_* */
   int32_t ramp_iter;       /* cfl-rise-threshold */
   double  cfl, vnn;
-  double  gas_R, gas_cp;   /* gas constant and constant cp */
+  double  gas_R, gas_cp;   /* gas constant and constant cp (table mode: gas_R only) */
   double  mu_ref, T_ref, S_suth, Pr;   /* Sutherland law mu = mu_ref (T/T_ref)^1.5 (T_ref+S)/(T+S); k = mu cp / Pr */
   double  irs_beta;        /* irs-beta (Register_Numerics.f90:53): > 0 enables implicit residual smoothing
                               (Assign_Setup.f90:61; Lib_IRS.f90:17-177 from Newstate_Conservative,
@@ -234,6 +236,25 @@ int q2d_gpu_remove_case(q2d_handle *h, int32_t case_id);
 /* Case and block counts of the handle (removed ones included) and the bytes of
  * device memory its blocks hold. */
 int q2d_gpu_case_count(q2d_handle *h, int32_t *ncases, int32_t *nblocks, int64_t *device_bytes);
+
+/* The keys of one case that the reference's host may change while it runs: every ini-diter steps
+ * Q2D_postprocess re-reads input.ini (Read_Inifile_Runtime, Read_Ini.f90:65-88, called from
+ * Wrap_Postprocess.f90:288-291), so the next step's Compute_dt and stop test see the new values.  The
+ * next q2d_gpu_step / q2d_gpu_run after q2d_gpu_set_runtime evaluates its dt with them (its dt pass runs
+ * again; a fused dt from the previous step is discarded).  Keys that would change the handle's memory
+ * (irs-beta, the model, the number of RK stages, the arithmetic form) are not runtime keys here. */
+typedef struct q2d_runtime {
+  double  cfl, vnn;
+  int32_t ramp_iter;        /* cfl-rise-threshold */
+  int32_t itermax;          /* iter-threshold */
+  double  res_threshold, time_threshold, res_drop;
+  int32_t limiter;          /* Q2D_LIM_*  (flux-limiter / space-reconstruction first-order) */
+  int32_t riemann;          /* Q2D_RIEMANN_* */
+  int32_t space_sd;         /* MUSCL-SD on (1) or off (0) */
+  int32_t reserved;
+} q2d_runtime;
+int q2d_gpu_set_runtime(q2d_handle *h, int32_t case_id, const q2d_runtime *rt);
+int q2d_gpu_get_runtime(q2d_handle *h, int32_t case_id, q2d_runtime *rt);
 
 /* Exactly one Explicit_Step (Mod_Explicit.f90:10-174, called from Q2D_solve,
  * Wrap_Solve.f90:19-32) for every active case; out[] has ncases entries

@@ -66,6 +66,9 @@ def load(path=None):
     lib.q2do_bc_inflow_flux.restype = C.c_int
     lib.q2do_bc_outflow_flux.argtypes = [C.c_double, C.c_double, C.c_int, dp] + [C.c_double] * 5 + [dp]
     lib.q2do_bc_outflow_flux.restype = C.c_int
+    from q2d_amd.capi import Runtime
+    lib.q2do_set_runtime.argtypes = [C.c_void_p, C.c_int, C.POINTER(Runtime)]
+    lib.q2do_set_runtime.restype = C.c_int
     lib.q2do_unit_tables.argtypes = [dp, C.c_int]
     lib.q2do_thermo_eval.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double]
     lib.q2do_thermo_eval.restype = C.c_double
@@ -189,6 +192,21 @@ class Oracle:
     def step(self):
         rc = self.lib.q2do_step(self.h, self._out)
         return rc, [o.as_dict() for o in self._out]
+
+    def set_runtime(self, case, **changes):
+        """Change runtime keys of `case` between steps, as GpuSolver.set_runtime."""
+        from q2d_amd.capi import LIMITERS, RIEMANN, Runtime, runtime_struct
+        if not hasattr(self, "_rt"):
+            self._rt = {}
+        c = self.problem.cases[case]
+        base = self._rt.get(case) or Runtime(
+            cfl=c.cfl, vnn=c.vnn, ramp_iter=c.ramp_iter, itermax=c.itermax, res_threshold=c.res_threshold,
+            time_threshold=c.time_threshold, res_drop=c.res_drop, limiter=LIMITERS[c.limiter],
+            riemann=RIEMANN[c.riemann], space_sd=int(c.space_sd))
+        rt = runtime_struct(base, **changes)
+        self._rt[case] = rt
+        if self.lib.q2do_set_runtime(self.h, int(case), C.byref(rt)) != 0:
+            raise ValueError("q2do_set_runtime failed")
 
     def run(self, nsteps, stop_on_todo=True):
         outs = None

@@ -1937,6 +1937,19 @@ double q2do_sd_tramel(double p, double p_im, double p_ip, double p_jm, double p_
   return sd_tramel(p, p_im, p_ip, p_jm, p_jp);
 }
 
+/* Read_Inifile_Runtime's keys of one case (Read_Ini.f90:65-88): the next step's Compute_dt and stop test
+ * read them. */
+int q2do_set_runtime(q2do *o, int case_id, const q2d_runtime *rt) {
+  if (!o || !rt || case_id < 0 || case_id >= o->ncases) return Q2D_E_ARG;
+  ocase *cs = &o->cs[case_id];
+  q2d_scheme *s = &cs->d.scheme;
+  s->cfl = rt->cfl; s->vnn = rt->vnn; s->ramp_iter = rt->ramp_iter;
+  s->limiter = rt->limiter; s->riemann = rt->riemann; s->space_sd = rt->space_sd;
+  cs->d.itermax = rt->itermax; cs->d.res_threshold = rt->res_threshold;
+  cs->d.time_threshold = rt->time_threshold; cs->d.res_drop = rt->res_drop;
+  return Q2D_OK;
+}
+
 double q2do_pow(double x, double y) { return exact_pow(x, y); }
 int q2do_plant(int which) { int old = g_plant; g_plant = which; return old; }
 

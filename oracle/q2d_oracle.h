@@ -86,6 +86,8 @@ int q2do_bc_outflow_flux(double gas_R, double gas_cp, int face, const double *Bp
 #define Q2DO_PLANT_TAB_NODE      8   /* table interpolation from node Ti + 1 instead of Ti */
 #define Q2DO_PLANT_TAB_TOFE      9   /* table-mode cons2prim: one Newton step, whatever its interval */
 int q2do_plant(int which);
+/* The runtime keys of one case (as q2d_gpu_set_runtime). */
+int q2do_set_runtime(q2do *o, int case_id, const q2d_runtime *rt);
 /* Table mode for the unit entry points (riemann, roe_averages, convective_flux, the BC fluxes and the
  * functions below): planes h, cp, mu, k of n nodes (node k at T = k), or NULL for the closed form.  The
  * oracle keeps the pointer, not a copy. */

@@ -65,6 +65,28 @@ class CaseDesc(C.Structure):
                 ("time0", C.c_double), ("thermo", C.c_void_p), ("thermo_n", C.c_int32), ("reserved2", C.c_int32)]
 
 
+class Runtime(C.Structure):
+    _fields_ = [("cfl", C.c_double), ("vnn", C.c_double), ("ramp_iter", C.c_int32), ("itermax", C.c_int32),
+                ("res_threshold", C.c_double), ("time_threshold", C.c_double), ("res_drop", C.c_double),
+                ("limiter", C.c_int32), ("riemann", C.c_int32), ("space_sd", C.c_int32), ("reserved", C.c_int32)]
+
+
+def runtime_struct(base, **changes):
+    """A Runtime from `base` (a Runtime) with fields changed; limiter and riemann by name or id."""
+    rt = Runtime.from_buffer_copy(base)
+    for k, v in changes.items():
+        if k == "limiter" and isinstance(v, str):
+            v = LIMITERS[v]
+        if k == "riemann" and isinstance(v, str):
+            v = RIEMANN[v]
+        if k == "space_sd":
+            v = int(v)
+        if not hasattr(rt, k):
+            raise KeyError(f"{k} is not a runtime key")
+        setattr(rt, k, v)
+    return rt
+
+
 class ProblemC(C.Structure):
     _fields_ = [("nblocks", C.c_int32), ("nbc", C.c_int32), ("ncases", C.c_int32), ("reserved", C.c_int32),
                 ("blocks", C.POINTER(BlockDesc)), ("bc", C.POINTER(BCEntry)), ("cases", C.POINTER(CaseDesc))]
@@ -148,6 +170,8 @@ SIGNATURES = {
     "q2d_gpu_step": ([C.c_void_p, C.POINTER(StepOut)], C.c_int),
     "q2d_gpu_run": ([C.c_void_p, C.c_int32, C.POINTER(StepOut)], C.c_int),
     "q2d_gpu_step_noalloc": ([C.c_void_p, C.POINTER(StepOut)], C.c_int),
+    "q2d_gpu_set_runtime": ([C.c_void_p, C.c_int32, C.POINTER(Runtime)], C.c_int),
+    "q2d_gpu_get_runtime": ([C.c_void_p, C.c_int32, C.POINTER(Runtime)], C.c_int),
     "q2d_gpu_download_state": ([C.c_void_p, C.c_int32, C.c_void_p], C.c_int),
     "q2d_gpu_upload_state": ([C.c_void_p, C.c_int32, C.c_void_p], C.c_int),
     "q2d_gpu_download_dt": ([C.c_void_p, C.c_int32, C.c_void_p], C.c_int),
@@ -263,6 +287,18 @@ class GpuSolver:
         if raise_on_error:
             _check(self.lib, rc)
         return [o.as_dict() for o in self._out[:self.ncases]]
+
+    def runtime(self, case):
+        """The runtime keys of `case` (a Runtime)."""
+        rt = Runtime()
+        _check(self.lib, self.lib.q2d_gpu_get_runtime(self.h, int(case), C.byref(rt)))
+        return rt
+
+    def set_runtime(self, case, **changes):
+        """Change runtime keys of `case` between steps (input.ini re-read: cfl, vnn, ramp_iter, itermax,
+        res_threshold, time_threshold, res_drop, limiter, riemann, space_sd)."""
+        rt = runtime_struct(self.runtime(case), **changes)
+        _check(self.lib, self.lib.q2d_gpu_set_runtime(self.h, int(case), C.byref(rt)))
 
     def run(self, nsteps, raise_on_error=True):
         rc = self.lib.q2d_gpu_run(self.h, int(nsteps), self._out)

@@ -3463,6 +3463,40 @@ int q2d_gpu_remove_case(q2d_handle* h, int32_t case_id) {
   return rebuild_descriptors(h);
 }
 
+int q2d_gpu_get_runtime(q2d_handle* h, int32_t case_id, q2d_runtime* rt) {
+  if (!h || !rt) return fail(Q2D_E_ARG, "NULL argument");
+  if (case_id < 0 || case_id >= h->ncases || !h->case_live[case_id]) return fail(Q2D_E_ARG, "bad case index");
+  const DevCase& k = h->dc[case_id];
+  memset(rt, 0, sizeof(*rt));
+  rt->cfl = k.cfl; rt->vnn = k.vnn; rt->ramp_iter = k.ramp_iter; rt->itermax = k.itermax;
+  rt->res_threshold = k.res_threshold; rt->time_threshold = k.time_threshold; rt->res_drop = k.res_drop;
+  rt->limiter = k.limiter; rt->riemann = k.riemann; rt->space_sd = k.space_sd;
+  return Q2D_OK;
+}
+
+int q2d_gpu_set_runtime(q2d_handle* h, int32_t case_id, const q2d_runtime* rt) {
+  if (!h || !rt) return fail(Q2D_E_ARG, "NULL argument");
+  if (case_id < 0 || case_id >= h->ncases || !h->case_live[case_id]) return fail(Q2D_E_ARG, "bad case index");
+  if (!(rt->cfl > 0) || !std::isfinite(rt->cfl) || !(rt->vnn > 0) || !std::isfinite(rt->vnn))
+    return fail(Q2D_E_ARG, "cfl and vnn must be finite and > 0");
+  if (rt->ramp_iter < 0 || rt->itermax < 0) return fail(Q2D_E_ARG, "ramp_iter and itermax must be >= 0");
+  if (rt->res_drop < 0 || rt->res_drop >= 1) return fail(Q2D_E_ARG, "res_drop must be in [0, 1)");
+  if (rt->limiter < 0 || rt->limiter > 6) return fail(Q2D_E_ARG, "unknown limiter");
+  if (rt->riemann < 0 || rt->riemann > 4) return fail(Q2D_E_ARG, "unknown Riemann solver");
+  if (rt->space_sd != 0 && rt->space_sd != 1) return fail(Q2D_E_ARG, "space_sd must be 0 or 1");
+  HIPCHK(hipStreamSynchronize(h->stream));
+  DevCase& k = h->dc[case_id];
+  const bool regroup = k.riemann != rt->riemann || k.space_sd != rt->space_sd;
+  k.cfl = rt->cfl; k.vnn = rt->vnn; k.ramp_iter = rt->ramp_iter; k.itermax = rt->itermax;
+  k.res_threshold = rt->res_threshold; k.time_threshold = rt->time_threshold; k.res_drop = rt->res_drop;
+  k.limiter = rt->limiter; k.riemann = rt->riemann; k.space_sd = rt->space_sd;
+  // the next step runs the dt pass with the new keys
+  h->dt_fused_valid = false;
+  if (regroup) return rebuild_descriptors(h);          // solver set and kernel grouping follow the case
+  HIPCHK(hipMemcpy(h->d_cases + case_id, &k, sizeof(DevCase), hipMemcpyHostToDevice));
+  return Q2D_OK;
+}
+
 int q2d_gpu_case_count(q2d_handle* h, int32_t* ncases, int32_t* nblocks, int64_t* device_bytes) {
   if (!h || !ncases || !nblocks || !device_bytes) return fail(Q2D_E_ARG, "NULL argument");
   *ncases = h->ncases; *nblocks = h->nblocks; *device_bytes = (int64_t)h->arena_bytes;

@@ -34,7 +34,7 @@ module q2d_case_io
 
   type, public :: case_config
     real(c_double) :: time_threshold = 1d30, res_threshold = 1d-10, cfl = 0.5d0, vnn = 0.3d0, irs_beta = 0d0
-    integer :: iter_threshold = 1000000000, ramp = 0, n_rk = 1, res_diter = 1
+    integer :: iter_threshold = 1000000000, ramp = 0, n_rk = 1, res_diter = 1, ini_diter = 10000
     logical :: time_accurate = .false., newrun = .true.
     character(len=64) :: equations = '', space = '', limiter = '', riemann = 'HLLC', intvars = 'cons'
     character(len=64) :: source = 'none', time_scheme = 'euler'
@@ -128,6 +128,7 @@ contains
         if (trim(key) == 'active') cfg%source = val
       case ('Q2D-IO')
         if (trim(key) == 'res-diter') read(val, *) cfg%res_diter
+        if (trim(key) == 'ini-diter') read(val, *) cfg%ini_diter
       end select
     end do
     close(u)

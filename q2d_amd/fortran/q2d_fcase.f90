@@ -147,6 +147,8 @@ contains
   call system_clock(c0, crate)
   do
     k = min(chunk, max_steps - nsteps)
+    ! stop at the next multiple of ini-diter, where the reference re-reads input.ini
+    k = min(k, max(1, cfg%ini_diter) - mod(nsteps, max(1, cfg%ini_diter)))
     if (k == 1) then
       rc = q2d_gpu_step(h, out)
     else
@@ -159,8 +161,10 @@ contains
       error stop 5
     end if
     nsteps = out(1)%iter
-    write(u, '(I8,E20.10,4E20.10)') iter_general + out(1)%iter, out(1)%time, out(1)%residual
+    if (mod(nsteps, max(1, cfg%res_diter)) == 0 .or. out(1)%todo == 3 .or. nsteps >= max_steps) &
+      write(u, '(I8,E20.10,4E20.10)') iter_general + out(1)%iter, out(1)%time, out(1)%residual
     if (out(1)%todo == 3 .or. nsteps >= max_steps) exit
+    if (mod(nsteps, max(1, cfg%ini_diter)) == 0) call reread_runtime()   ! Wrap_Postprocess.f90:288-291
   end do
   call system_clock(c1)
   close(u)
@@ -178,6 +182,25 @@ contains
   t0 = real(c1 - c0, c_double) / real(crate, c_double)
   print '(a,i0,a,es14.6,a,f10.3,a)', 'q2d_fcase: ', out(1)%iter, ' steps, time ', out(1)%time, ', ', t0, ' s'
   end subroutine run_on_device
+
+  ! Read_Inifile_Runtime (Read_Ini.f90:65-88): input.ini again, its runtime keys to the device
+  subroutine reread_runtime()
+    type(case_config) :: c2
+    type(q2d_runtime) :: rt
+    call read_input_ini(trim(dir)//'/input.ini', c2)
+    rc = q2d_gpu_get_runtime(h, 0_c_int32_t, rt)
+    if (rc /= Q2D_OK) error stop 7
+    rt%cfl = c2%cfl; rt%vnn = c2%vnn; rt%ramp_iter = c2%ramp; rt%itermax = c2%iter_threshold
+    rt%res_threshold = c2%res_threshold; rt%time_threshold = c2%time_threshold
+    rt%limiter = limiter_id(c2); rt%riemann = riemann_id(c2%riemann)
+    rt%space_sd = merge(1, 0, trim(c2%space) == 'MUSCL-SD')
+    rc = q2d_gpu_set_runtime(h, 0_c_int32_t, rt)
+    if (rc /= Q2D_OK) then
+      print '(a,i0,a)', 'q2d_fcase: runtime keys refused (', rc, '): '//q2d_error_message()
+      error stop 7
+    end if
+    cfg%res_diter = c2%res_diter; cfg%ini_diter = c2%ini_diter
+  end subroutine reread_runtime
 
   integer function limiter_id(c)
     type(case_config), intent(in) :: c

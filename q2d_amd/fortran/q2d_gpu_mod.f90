@@ -68,8 +68,17 @@ module q2d_gpu
     integer(c_int32_t) :: reserved(2)
   end type
 
+  ! runtime keys of one case (input.ini re-read every ini-diter steps, Read_Ini.f90:65-88)
+  type, bind(C), public :: q2d_runtime
+    real(c_double)     :: cfl, vnn
+    integer(c_int32_t) :: ramp_iter, itermax
+    real(c_double)     :: res_threshold, time_threshold, res_drop
+    integer(c_int32_t) :: limiter, riemann, space_sd, reserved
+  end type
+
   public :: q2d_version, q2d_last_error, q2d_device_count
   public :: q2d_gpu_create, q2d_gpu_destroy, q2d_gpu_step, q2d_gpu_run
+  public :: q2d_gpu_set_runtime, q2d_gpu_get_runtime
   public :: q2d_gpu_download_state, q2d_gpu_upload_state, q2d_gpu_download_dt
   public :: q2d_error_message
 
@@ -103,6 +112,18 @@ module q2d_gpu
       type(c_ptr), value :: handle
       integer(c_int32_t), value :: nsteps
       type(q2d_step_out), intent(out) :: out(*)
+    end function
+    integer(c_int) function q2d_gpu_set_runtime(handle, case_id, rt) bind(C, name="q2d_gpu_set_runtime")
+      import :: c_int, c_int32_t, c_ptr, q2d_runtime
+      type(c_ptr), value :: handle
+      integer(c_int32_t), value :: case_id
+      type(q2d_runtime), intent(in) :: rt
+    end function
+    integer(c_int) function q2d_gpu_get_runtime(handle, case_id, rt) bind(C, name="q2d_gpu_get_runtime")
+      import :: c_int, c_int32_t, c_ptr, q2d_runtime
+      type(c_ptr), value :: handle
+      integer(c_int32_t), value :: case_id
+      type(q2d_runtime), intent(out) :: rt
     end function
     integer(c_int) function q2d_gpu_download_state(handle, block, P) bind(C, name="q2d_gpu_download_state")
       import :: c_int, c_int32_t, c_ptr, c_double

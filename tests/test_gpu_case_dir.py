@@ -104,3 +104,20 @@ def test_table_mode_case_directory(oracle_mod, tmp_path):
     Po = o.state(0)[2, 2:-2]
     for k in range(4):
         assert np.allclose(cv[k, 0], Po[:, k], rtol=2e-15, atol=1e-300), k
+
+
+@needs_host
+def test_runtime_reread_of_input_ini(tmp_path):
+    """ini-diter = 50: the host re-reads input.ini every 50 steps and passes its runtime keys to the device
+    (q2d_gpu_set_runtime, whose next step runs the dt pass).  With the file unchanged the field is the one
+    of the run without re-reads, to every written digit."""
+    a = copy_case(tmp_path, "minmod")
+    _, _, cv_a = run_case(a)
+    b = tmp_path / "reread"
+    shutil.copytree(a, b)
+    ini = (b / "input.ini").read_text()
+    (b / "input.ini").write_text(ini.replace("[Q2D-IO]", "[Q2D-IO]\nini-diter = 50") if "[Q2D-IO]" in ini
+                                 else ini + "\n[Q2D-IO]\nini-diter = 50\n")
+    shutil.rmtree(b / "OUTPUT", ignore_errors=True)
+    _, _, cv_b = run_case(b)
+    assert np.array_equal(cv_a, cv_b)
