@@ -2118,9 +2118,11 @@ __device__ __forceinline__ void irs_ghost(const DevBlock* __restrict__ blocks, c
 
 // The i-smoothed residual of one cell (i, j) of block B, evaluated by one lane: the same expressions, in
 // the same order, as the lane-parallel form in k_irs.
+template <bool P2>
 __device__ __forceinline__ void irs_i_at(const DevBlock* __restrict__ blocks, const DevBlock& B,
                                          const DevBC* __restrict__ bc, const int* __restrict__ bcface, double beta,
-                                         double den, int i, int j, double out[4]) {
+                                         double den, double inv_den, int i, int j, double out[4]) {
+  auto idiv = [&](double x) { if constexpr (P2) return x * inv_den; else return x / den; };
   double q[5][4];
 #pragma unroll
   for (int d = 0; d < 5; ++d) {
@@ -2139,9 +2141,9 @@ __device__ __forceinline__ void irs_i_at(const DevBlock* __restrict__ blocks, co
   const bool inm = i - 1 >= 1, inp = i + 1 <= B.ni;
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
-    const double sm = inm ? (q[1][v] + beta * (q[0][v] + q[2][v])) / den : q[1][v];
-    const double sp = inp ? (q[3][v] + beta * (q[2][v] + q[4][v])) / den : q[3][v];
-    out[v] = (q[2][v] + beta * (sm + sp)) / den;
+    const double sm = inm ? idiv(q[1][v] + beta * (q[0][v] + q[2][v])) : q[1][v];
+    const double sp = inp ? idiv(q[3][v] + beta * (q[2][v] + q[4][v])) : q[3][v];
+    out[v] = idiv(q[2][v] + beta * (sm + sp));
   }
 }
 
@@ -2188,6 +2190,13 @@ __global__ __launch_bounds__(256) void k_irs(const DevBlock* __restrict__ blocks
     if (!K.active || irk > C.n_rk || !(C.irs_beta > 0.0)) continue;
     const bool last = irk == C.n_rk;
     const double beta = C.irs_beta, den = 1.0 + 2.0 * beta;
+    // den a power of two (irs-beta 0.5, every steady reference case: den = 2): x / den is exactly
+    // x * (1 / den) for every x (both are the correctly rounded x 2^-k), so the sweeps multiply
+    const double inv_den = 1.0 / den;
+    const bool den_pow2 = (__double_as_longlong(den) & 0x000FFFFFFFFFFFFFll) == 0;
+    auto item_body = [&](auto p2c) {
+    constexpr bool P2 = decltype(p2c)::value;
+    auto idiv = [&](double x) { if constexpr (P2) return x * inv_den; else return x / den; };
     const bool smooth_j = C.ndir == 2;                 // Residual_Smoothing's d = 1..ndir
     const int ni = B.ni, nj = B.nj, pitch = B.pitch;
     const long long pl = B.plane;
@@ -2224,12 +2233,12 @@ __global__ __launch_bounds__(256) void k_irs(const DevBlock* __restrict__ blocks
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const double qm = shup(q[v]), qp = shdn(q[v]);
-        s1[v] = inner ? (q[v] + beta * (qm + qp)) / den : q[v];
+        s1[v] = inner ? idiv(q[v] + beta * (qm + qp)) : q[v];
       }
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const double sm = shup(s1[v]), sp = shdn(s1[v]);
-        out[v] = (q[v] + beta * (sm + sp)) / den;
+        out[v] = idiv(q[v] + beta * (sm + sp));
       }
     };
     auto r1_interior = [&](int k, double out[4]) {
@@ -2248,13 +2257,13 @@ __global__ __launch_bounds__(256) void k_irs(const DevBlock* __restrict__ blocks
       r1_interior(k == 0 ? 1 : nj, out);              // zero-gradient: the boundary cell's own
       if (own) {
         const DevBC& e = bc[bcface[B.bcf[f - 1] + x - 1]];
-        if (e.type == Q2D_BC_CONNECT) irs_i_at(blocks, blocks[e.dblk], bc, bcface, beta, den, e.is, e.js, out);
+        if (e.type == Q2D_BC_CONNECT) irs_i_at<P2>(blocks, blocks[e.dblk], bc, bcface, beta, den, inv_den, e.is, e.js, out);
       }
     };
     auto sweep1 = [&](int k, const double a[4], const double b[4], const double c[4], double o[4]) {
       const bool in = k >= 1 && k <= nj;
 #pragma unroll
-      for (int v = 0; v < 4; ++v) o[v] = in ? (b[v] + beta * (a[v] + c[v])) / den : b[v];
+      for (int v = 0; v < 4; ++v) o[v] = in ? idiv(b[v] + beta * (a[v] + c[v])) : b[v];
     };
     double Rm[4], Rc[4], Rp[4], sA[4], sB[4];
     {
@@ -2301,7 +2310,7 @@ __global__ __launch_bounds__(256) void k_irs(const DevBlock* __restrict__ blocks
         }
         sweep1(r + 1, Rc, Rp, Rp2, sC);
 #pragma unroll
-        for (int v = 0; v < 4; ++v) s2[v] = (Rc[v] + beta * (sA[v] + sC[v])) / den;
+        for (int v = 0; v < 4; ++v) s2[v] = idiv(Rc[v] + beta * (sA[v] + sC[v]));
 #pragma unroll
         for (int v = 0; v < 4; ++v) { sA[v] = sB[v]; sB[v] = sC[v]; Rm[v] = Rc[v]; Rc[v] = Rp[v]; Rp[v] = Rp2[v]; }
       } else {
@@ -2362,6 +2371,9 @@ __global__ __launch_bounds__(256) void k_irs(const DevBlock* __restrict__ blocks
       }
       if (lane == 0) partials[(long long)it * kPartials + 4] = dtnext;
     }
+    };
+    if (den_pow2) item_body(std::integral_constant<bool, true>{});
+    else item_body(std::integral_constant<bool, false>{});
   }
 }
 
