@@ -254,6 +254,14 @@ typedef struct q2d_runtime {
   int32_t reserved;
 } q2d_runtime;
 int q2d_gpu_set_runtime(q2d_handle *h, int32_t case_id, const q2d_runtime *rt);
+
+/* Kernel fusion of the mixed form's laminar cases (no counterpart in the reference): visc = 1 (the
+ * default) evaluates the diffusive face fluxes and the axisymmetric viscous source inside the laminar
+ * stage kernel, in FP32 like the form's convective faces, for cases without the area-variation source and
+ * without faces that mix wall and other records; visc = 0 computes them in FP64 in a separate kernel
+ * first.  Both are held to the mixed form's tolerance against the FP64 form; the FP64 form is not
+ * affected. */
+int q2d_gpu_set_fusion(q2d_handle *h, int32_t visc);
 int q2d_gpu_get_runtime(q2d_handle *h, int32_t case_id, q2d_runtime *rt);
 
 /* Exactly one Explicit_Step (Mod_Explicit.f90:10-174, called from Q2D_solve,

@@ -171,6 +171,7 @@ SIGNATURES = {
     "q2d_gpu_run": ([C.c_void_p, C.c_int32, C.POINTER(StepOut)], C.c_int),
     "q2d_gpu_step_noalloc": ([C.c_void_p, C.POINTER(StepOut)], C.c_int),
     "q2d_gpu_set_runtime": ([C.c_void_p, C.c_int32, C.POINTER(Runtime)], C.c_int),
+    "q2d_gpu_set_fusion": ([C.c_void_p, C.c_int32], C.c_int),
     "q2d_gpu_get_runtime": ([C.c_void_p, C.c_int32, C.POINTER(Runtime)], C.c_int),
     "q2d_gpu_download_state": ([C.c_void_p, C.c_int32, C.c_void_p], C.c_int),
     "q2d_gpu_upload_state": ([C.c_void_p, C.c_int32, C.c_void_p], C.c_int),
@@ -287,6 +288,10 @@ class GpuSolver:
         if raise_on_error:
             _check(self.lib, rc)
         return [o.as_dict() for o in self._out[:self.ncases]]
+
+    def set_fusion(self, visc=True):
+        """Mixed laminar cases: viscous terms inside the stage kernel (True) or in k_visc (False)."""
+        _check(self.lib, self.lib.q2d_gpu_set_fusion(self.h, int(bool(visc))))
 
     def runtime(self, case):
         """The runtime keys of `case` (a Runtime)."""

@@ -91,6 +91,8 @@ struct DevCase {
   int irs_begin, irs_end;               // the case's k_irs work items (their partials replace the stage's)
   int space_sd;                         // MUSCL-SD: Tramel's detector scales the slopes (sd_tramel)
   int ntab;                             // table mode: nodes per property plane (0: closed forms)
+  int fvisc;                            // mixed laminar case whose stage kernel evaluates the viscous terms
+  int pad_fv;
   const double* tab;                    // table mode: h, cp, mu, k planes (device copy), else null
 };
 
@@ -859,6 +861,115 @@ __device__ __forceinline__ void visc_face(const DevCase& C, bool axi, int a, dou
   diff_flux(C, axi, pm0, pm1, pm2, pm3, gUx, gUy, gVx, gVy, gTx, gTy, n1, n2, il, r_ave, VG, D);
 }
 
+// Source_Axi_Visc_Blk (Lib_Source_Axisymmetry.f90:125-236) of cell c with its neighbours E, W, N, S (wall
+// ghosts already reflected), the cell's metric M, r_ave and 1/r; out = (S(nu), S(nv), S(np)) per unit
+// volume.  One species: the species and enthalpy-diffusion terms vanish.
+__device__ __forceinline__ void visc_source_axi(const DevCase& C, const VC& cC, const VC& qE, const VC& qW,
+                                                const VC& qN, const VC& qS, const double MC[4], double sa,
+                                                double sb, double out[3]) {
+  const double mil = mu_lam(C, cC.T);
+  const double kl = k_lam(C, mil, cC.T);
+  double gUx, gUy, gVx, gVy, gTx, gTy;
+  grad_m((qE.u - qW.u) * 0.5, (qN.u - qS.u) * 0.5, MC, gUx, gUy);
+  grad_m((qE.v - qW.v) * 0.5, (qN.v - qS.v) * 0.5, MC, gVx, gVy);
+  grad_m((qE.T - qW.T) * 0.5, (qN.T - qS.T) * 0.5, MC, gTx, gTy);
+  (void)gTx;
+  const double kappa = kl + 0.0;
+  const double mi = mil + 0.0;
+  const double lam = kNegTwoThird * mi;
+  // Stress_Tensor_Axi — Lib_Fluid.f90:166-198
+  const double vr = cC.v;
+  const double div = gUx + gVy + vr / sa;
+  const double ld2 = lam * div;
+  const double tyy = ld2 + 2.0 * mi * gVy;
+  const double txy = mi * (gUy + gVx);
+  const double tzz = 2.0 * mi * vr / sa + ld2;
+  const double qr = -kappa * gTy;
+  out[0] = txy * sb;
+  out[1] = (tyy - tzz) * sb;
+  out[2] = (-qr + cC.u * txy + cC.v * tyy + 0.0) * sb;
+}
+
+// ---- the mixed form's FP32 viscous terms (fused into its laminar stage kernel) ------------------------
+// The formulas of visc_face / diff_flux / stress_vector / visc_source_axi (Lib_Diffusive.f90:10-199,
+// Lib_Fluid.f90:26-76, 166-198, Lib_Source_Axisymmetry.f90:125-236), in FP32 with the hardware reciprocal
+// and square root: the mixed form is held to its tolerance against the FP64 oracle, not to the
+// reference's operation order.  Table mode is FP64 only, so the closed-form transport applies.
+struct VCf { float r, u, v, p, T; };
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ VCf vcf_of(const double q[4], float R) {
+  VCf c{(float)q[0], (float)q[1], (float)q[2], (float)q[3], 0.f};
+  c.T = c.p * frcp(c.r * R);
+  return c;
+}
+__device__ __forceinline__ VCf vcf_shdn(const VCf& c) { return VCf{shdn(c.r), shdn(c.u), shdn(c.v), shdn(c.p), shdn(c.T)}; }
+__device__ __forceinline__ VCf vcf_shup(const VCf& c) { return VCf{shup(c.r), shup(c.u), shup(c.v), shup(c.p), shup(c.T)}; }
+struct TransF { float mu_ref, inv_Tref, Tref_S, S, cp_Pr; };   // Sutherland constants, cp / Pr
+__device__ __forceinline__ TransF trans_f(const DevCase& C) {
+  return TransF{(float)C.mu_ref, (float)(1.0 / C.T_ref), (float)(C.T_ref + C.S_suth), (float)C.S_suth,
+                (float)(C.cp / C.Pr)};
+}
+__device__ __forceinline__ float mu_lam_f(const TransF& t, float T) {
+  const float x = T * t.inv_Tref;
+  return t.mu_ref * (x * __builtin_amdgcn_sqrtf(x)) * t.Tref_S * frcp(T + t.S);
+}
+__device__ __forceinline__ void grad_mf(float g0, float g1, const float M[4], float& o0, float& o1) {
+  o0 = g0 * M[0] + g1 * M[1];
+  o1 = g0 * M[2] + g1 * M[3];
+}
+// diffusive flux of one interior face (a: 0 i-face, 1 j-face), the stencil as visc_face's
+__device__ __forceinline__ void visc_face_f(const TransF& t, float R, bool axi, int a, float n1, float n2, float il,
+                                            const VCf& c1, const VCf& c2, const VCf& c3, const VCf& c4,
+                                            const VCf& c5, const VCf& c6, const float M[4], float r_ave,
+                                            double D[3]) {
+  const float nU = c2.u - c1.u, nV = c2.v - c1.v, nT = c2.T - c1.T;
+  const float tU = (c4.u - c3.u + c6.u - c5.u) * 0.25f;
+  const float tV = (c4.v - c3.v + c6.v - c5.v) * 0.25f;
+  const float tT = (c4.T - c3.T + c6.T - c5.T) * 0.25f;
+  float gUx, gUy, gVx, gVy, gTx, gTy;
+  if (a == 0) {
+    grad_mf(nU, tU, M, gUx, gUy); grad_mf(nV, tV, M, gVx, gVy); grad_mf(nT, tT, M, gTx, gTy);
+  } else {
+    grad_mf(tU, nU, M, gUx, gUy); grad_mf(tV, nV, M, gVx, gVy); grad_mf(tT, nT, M, gTx, gTy);
+  }
+  const float pm0 = 0.5f * (c1.r + c2.r), pm1 = 0.5f * (c1.u + c2.u);
+  const float pm2 = 0.5f * (c1.v + c2.v), pm3 = 0.5f * (c1.p + c2.p);
+  const float T = pm3 * frcp(pm0 * R);
+  const float mi = mu_lam_f(t, T);
+  const float kappa = mi * t.cp_Pr;
+  const float lam = -2.0f / 3.0f * mi;
+  float div;
+  if (axi) div = (r_ave <= 1.0e-6f) ? (gUx + gVy + gVy) : (gUx + gVy + pm2 * frcp(r_ave));
+  else div = gUx + gVy;
+  const float ld = lam * div;
+  const float txx = ld + 2.0f * mi * gUx, tyy = ld + 2.0f * mi * gVy, txy = mi * (gUy + gVx);
+  const float S1 = txx * n1 + txy * n2, S2 = txy * n1 + tyy * n2;
+  D[0] = (double)(il * S1);
+  D[1] = (double)(il * S2);
+  D[2] = (double)(il * ((S1 * pm1 + S2 * pm2) + kappa * (gTx * n1 + gTy * n2)));
+}
+__device__ __forceinline__ void visc_source_axi_f(const TransF& t, const VCf& cC, const VCf& qE, const VCf& qW,
+                                                  const VCf& qN, const VCf& qS, const float MC[4], float sa,
+                                                  float sb, double out[3]) {
+  const float mi = mu_lam_f(t, cC.T);
+  const float kappa = mi * t.cp_Pr;
+  float gUx, gUy, gVx, gVy, gTx, gTy;
+  grad_mf((qE.u - qW.u) * 0.5f, (qN.u - qS.u) * 0.5f, MC, gUx, gUy);
+  grad_mf((qE.v - qW.v) * 0.5f, (qN.v - qS.v) * 0.5f, MC, gVx, gVy);
+  grad_mf((qE.T - qW.T) * 0.5f, (qN.T - qS.T) * 0.5f, MC, gTx, gTy);
+  (void)gTx;
+  const float lam = -2.0f / 3.0f * mi;
+  const float vr = cC.v, ir = frcp(sa);
+  const float ld2 = lam * (gUx + gVy + vr * ir);
+  const float tyy = ld2 + 2.0f * mi * gVy;
+  const float txy = mi * (gUy + gVx);
+  const float tzz = 2.0f * mi * vr * ir + ld2;
+  const float qr = -kappa * gTy;
+  out[0] = (double)(txy * sb);
+  out[1] = (double)((tyy - tzz) * sb);
+  out[2] = (double)((-qr + cC.u * txy + cC.v * tyy) * sb);
+}
+
 // Interior diffusive face fluxes and the axisymmetric viscous source of every cell of the viscous work
 // items.  Cell (x,r) stores the flux of its i-face (x|x+1) (Mod_Fluxes.f90:160-183), of its j-face
 // (r|r+1) (:207-230) and its Source_Axi_Visc_Blk term (Lib_Source_Axisymmetry.f90:81-247); the stage
@@ -881,7 +992,7 @@ __global__ __launch_bounds__(256) void k_visc(const DevBlock* __restrict__ block
     const DevBlock& B = blocks[item.blk];
     const DevCtrl& K = ctrl[B.case_id];
     const DevCase& C = cases[B.case_id];
-    if (!K.active || irk > C.n_rk) continue;
+    if (!K.active || irk > C.n_rk || C.fvisc) continue;   // fvisc: the stage kernel evaluates these terms
     const int ni = B.ni, nj = B.nj, pitch = B.pitch;
     const long long pl = B.plane;
     const bool axi = C.source == Q2D_SRC_AXI, area = C.source == Q2D_SRC_AREA;
@@ -956,30 +1067,22 @@ __global__ __launch_bounds__(256) void k_visc(const DevBlock* __restrict__ block
         if (x == ni && is_wall(1, r)) qE = refl;
         if (r == 1 && wall3) qS = refl;
         if (r == nj && wall4) qN = refl;
+        if (axi) {
+          double Sv[3];
+          visc_source_axi(C, cC, qE, qW, qN, qS, MC, sa, sb, Sv);
+#pragma unroll
+          for (int k = 0; k < 3; ++k) vsv[k * pl + o] = Sv[k];
+        } else {
         const double mil = mu_lam(C, cC.T);
         const double kl = k_lam(C, mil, cC.T);
         double gUx, gUy, gVx, gVy, gTx, gTy;
         grad_m((qE.u - qW.u) * 0.5, (qN.u - qS.u) * 0.5, MC, gUx, gUy);
         grad_m((qE.v - qW.v) * 0.5, (qN.v - qS.v) * 0.5, MC, gVx, gVy);
         grad_m((qE.T - qW.T) * 0.5, (qN.T - qS.T) * 0.5, MC, gTx, gTy);
-        (void)gTx;
         const double kappa = kl + 0.0;
         const double mi = mil + 0.0;
         const double lam = kNegTwoThird * mi;
-        if (axi) {
-          // Stress_Tensor_Axi — Lib_Fluid.f90:166-198
-          (void)gTx;
-          const double vr = cC.v;
-          const double div = gUx + gVy + vr / sa;
-          const double ld2 = lam * div;
-          const double tyy = ld2 + 2.0 * mi * gVy;
-          const double txy = mi * (gUy + gVx);
-          const double tzz = 2.0 * mi * vr / sa + ld2;
-          const double qr = -kappa * gTy;
-          vsv[o] = txy * sb;
-          vsv[pl + o] = (tyy - tzz) * sb;
-          vsv[2 * pl + o] = (-qr + cC.u * txy + cC.v * tyy + 0.0) * sb;
-        } else {
+        {
           // Source_Area_Visc_Blk — Lib_Source_Areavariation.f90:147-349 (one species: the species,
           // enthalpy-diffusion and Hirschfelder terms are 0) with Stress_Tensor_Std and V.G (:128-163)
           const double Gx = sa, Gy = sb;
@@ -994,6 +1097,7 @@ __global__ __launch_bounds__(256) void k_visc(const DevBlock* __restrict__ block
           vsv[o] = Gx * (t33 - t11) - Gy * t12;
           vsv[pl + o] = -Gx * t12 + Gy * (t33 - t22);
           vsv[2 * pl + o] = -cC.u * (t11 * Gx + t12 * Gy) - cC.v * (t12 * Gx + t22 * Gy) + Gx * qx + Gy * qy;
+        }
         }
       }
       // roll the rows; issue row r+3
@@ -1330,6 +1434,31 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
       double acc[4] = {0.0, 0.0, 0.0, 0.0};
       double dtnext = 1e300;
       double Eprev[3] = {0.0, 0.0, 0.0};                 // diffusive flux of j-face r-1 (VISC)
+      // Mixed laminar cases with fvisc: the diffusive face fluxes and the viscous source are evaluated here
+      // in FP32 (the mixed form's face arithmetic) instead of read back from k_visc's FP64 planes: rows
+      // r-1, r, r+1 with their temperature, M of rows r .. r+2 and the node radius of rows r-1 .. r+1 roll
+      // with the row march.
+      constexpr bool kVM = VISC && sizeof(FT) == 4;
+      const bool fv = kVM && C.fvisc != 0;
+      const bool vaxi = C.source == Q2D_SRC_AXI;
+      VCf vP{0.f, 0.f, 0.f, 0.f, 0.f}, vA{0.f, 0.f, 0.f, 0.f, 0.f}, vB{0.f, 0.f, 0.f, 0.f, 0.f};
+      float MC[4] = {0.f, 0.f, 0.f, 0.f}, MN[4] = {0.f, 0.f, 0.f, 0.f}, MF[4] = {0.f, 0.f, 0.f, 0.f};
+      float nodS = 0.f, nodC = 0.f, nodN = 0.f;
+      const float Rf = (float)C.R;
+      const TransF trf = trans_f(C);
+      auto ldM = [&](int rr, float M[4]) {
+        const long long o = at(rr);
+        M[0] = (float)gp(B.m11)[o]; M[1] = (float)gp(B.m21)[o]; M[2] = (float)gp(B.m12)[o]; M[3] = (float)gp(B.m22)[o];
+      };
+      auto is_wall1 = [&](int f) { return B.bcf[f] >= 0 && B.wallf[f] == 1; };   // fvisc: no mixed faces
+      if constexpr (kVM) {
+        if (fv) {
+          vA = vcf_of(qA, Rf);                          // row y0-1
+          vB = vcf_of(qB, Rf);                          // row y0
+          ldM(item.y0 - 1, MC); ldM(item.y0, MN);
+          nodC = (float)gp(B.nodr)[at(item.y0 - 1)];
+        }
+      }
 
       for (int r = item.y0 - 1; r <= item.y1; ++r) {
         if ((lface || rface) && r >= item.y0 && ((r - item.y0) & 15) == 0 && lane < 32) {
@@ -1386,6 +1515,9 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
           }
           const long long og = at(min(r + 1, rmax));
           fjxN = fjxp[og]; fjyN = fjyp[og]; fjlN = fjlp[og];
+          if constexpr (kVM) {
+            if (fv) { ldM(r + 2, MF); nodN = (float)gp(B.nodr)[at(r + 1)]; }   // consumed by the next iteration
+          }
         }
         FT sBC[4], phiB[4], Gcur[4];
         {
@@ -1402,7 +1534,7 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
           bt = sd ? brie : FT(1);
         }
         double Ecur[3] = {0.0, 0.0, 0.0};
-        if (VISC && own && r >= 1 && r <= nj - 1) {
+        if (VISC && !fv && own && r >= 1 && r <= nj - 1) {
           const long long oE = at(r);
 #pragma unroll
           for (int k = 0; k < 3; ++k) Ecur[k] = gp(B.vey)[k * pl + oE];
@@ -1415,6 +1547,19 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
         } else {
 #pragma unroll
           for (int v = 0; v < 4; ++v) Gcur[v] = FT(0);
+        }
+        if constexpr (kVM) {
+          if (fv) {                                      // diffusive flux of j-face (r | r+1), FP32
+            const VCf vAW = vcf_shup(vA), vAE = vcf_shdn(vA), vBW = vcf_shup(vB), vBE = vcf_shdn(vB);
+            const float nodW = shup(nodC);               // the lane shifts before the lane-dependent branch
+            if (own && r >= 1 && r <= nj - 1) {
+              float Ma[4];
+#pragma unroll
+              for (int k = 0; k < 4; ++k) Ma[k] = 0.5f * (MC[k] + MN[k]);
+              visc_face_f(trf, Rf, vaxi, 1, (float)gcur.fjx, (float)gcur.fjy, (float)gcur.fjl, vA, vB, vAW, vAE,
+                          vBW, vBE, Ma, 0.5f * (nodC + nodW), Ecur);
+            }
+          }
         }
 
         if (r >= item.y0) {
@@ -1449,10 +1594,42 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
           for (int v = 0; v < 4; ++v) Fl[v] = shup(Fx[v]);
 
           double Dx[3] = {0.0, 0.0, 0.0}, Dl[3] = {0.0, 0.0, 0.0};
+          double Sv[3] = {0.0, 0.0, 0.0};                // viscous source of cell (x, r) (fvisc)
           if (VISC) {                                    // diffusive flux of i-face x, and of x-1 from the left lane
-            const long long oD = at(r);
+            bool done = false;
+            if constexpr (kVM) {
+              if (fv) {
+                // i-face (x | x+1) and the viscous source, as k_visc; lane 1 evaluates lane 2's face x-1.  The
+                // lane shifts come before the lane-dependent branches.
+                const VCf vE = vcf_shdn(vA), vW = vcf_shup(vA), vSE = vcf_shdn(vP), vNE = vcf_shdn(vB);
+                float ME[4];
 #pragma unroll
-            for (int k = 0; k < 3; ++k) Dx[k] = gp(B.vdx)[k * pl + oD];
+                for (int k = 0; k < 4; ++k) ME[k] = shdn(MC[k]);
+                if (lane >= 1 && lane <= 61 && x >= 1 && x <= ni - 1) {
+                  float Ma[4];
+#pragma unroll
+                  for (int k = 0; k < 4; ++k) Ma[k] = 0.5f * (MC[k] + ME[k]);
+                  visc_face_f(trf, Rf, vaxi, 0, (float)gcur.fix, (float)gcur.fiy, (float)gcur.fil, vA, vE, vP, vB,
+                              vSE, vNE, Ma, 0.5f * (nodC + nodS), Dx);
+                }
+                if (own && vaxi) {                       // Source_Axi_Visc_Blk :125-236, wall ghosts reflected
+                  VCf qE = vE, qW = vW, qS = vP, qN = vB;
+                  const VCf refl{vA.r, -vA.u, -vA.v, vA.p, vA.T};
+                  if (x == 1 && is_wall1(0)) qW = refl;
+                  if (x == ni && is_wall1(1)) qE = refl;
+                  if (r == 1 && is_wall1(2)) qS = refl;
+                  if (r == nj && is_wall1(3)) qN = refl;
+                  const long long oS = at(r);
+                  visc_source_axi_f(trf, vA, qE, qW, qN, qS, MC, (float)gp(B.sa)[oS], (float)gp(B.sb)[oS], Sv);
+                }
+                done = true;
+              }
+            }
+            if (!done) {
+              const long long oD = at(r);
+#pragma unroll
+              for (int k = 0; k < 3; ++k) Dx[k] = gp(B.vdx)[k * pl + oD];
+            }
 #pragma unroll
             for (int k = 0; k < 3; ++k) Dl[k] = shup(Dx[k]);
           }
@@ -1522,8 +1699,12 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
 #pragma unroll
               for (int k = 0; k < 4; ++k) res[k] = res[k] - S[k] * sr;
               if (VISC) {                                 // Source_Axi_Visc_Blk / Source_Area_Visc_Blk, after the Euler source
+                if (!fv) {
 #pragma unroll
-                for (int k = 1; k < 4; ++k) res[k] = res[k] - gp(B.vsv)[(k - 1) * pl + oA] * sr;
+                  for (int k = 0; k < 3; ++k) Sv[k] = gp(B.vsv)[k * pl + oA];
+                }
+#pragma unroll
+                for (int k = 1; k < 4; ++k) res[k] = res[k] - Sv[k - 1] * sr;
               }
             }
             const double dt = tacc ? dtg : gcur.dtl;
@@ -1616,6 +1797,15 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
         if (VISC) {
 #pragma unroll
           for (int k = 0; k < 3; ++k) Eprev[k] = Ecur[k];
+        }
+        if constexpr (kVM) {
+          if (fv) {
+            vP = vA; vA = vB;
+            vB = vcf_of(qB, Rf);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) { MC[k] = MN[k]; MN[k] = MF[k]; }
+            nodS = nodC; nodC = nodN;
+          }
         }
         dlyA = dlyB; dlyB = dlyC;
       }
@@ -2508,6 +2698,8 @@ struct q2d_handle {
   bool any_irs = false;          // a live case smooths its residual (k_irs after each stage)
   bool any_hllepp = false;       // a live case uses HLLE++ or HLLC+: the stage kernel's RIE_T = -2 instantiation
   bool any_table = false;        // a live case uses property tables: the FP64 stage kernel's RIE_T = -4 one
+  bool fuse_visc = true;         // mixed laminar cases: viscous terms in the stage kernel where possible
+  bool any_unfused_visc = false; // a live laminar case needs k_visc
   std::vector<double*> case_tab; // per case: its tables on the device (table mode), else null
   bool any_det_or_hllem = false; // a live case uses HLLEM or the detector: not the RIE_T = -3 instantiation
   bool mixed_generic = false;    // mixed form with a MUSCL-SD / HLLEM / HLLE++ / HLLC+ case: those cases' Euler
@@ -2886,7 +3078,7 @@ int launch_step(q2d_handle* h, const StepScratch& scr) {
     if (!h->vrec.empty())
       hipLaunchKernelGGL(k_bcvisc, dim3(((int)h->vrec.size() + 255) / 256), dim3(256), 0, h->stream, h->d_blocks,
                          h->d_cases, h->d_ctrl, h->d_bc, h->d_vrec, (int)h->vrec.size(), irk, scr.bcflux);
-    if (h->range[4] > h->range[2]) {
+    if (h->range[4] > h->range[2] && h->any_unfused_visc) {
       int vg = (h->range[4] - h->range[2] + h->waves_per_group - 1) / h->waves_per_group;
       if (h->max_groups > 0) vg = std::min(vg, h->max_groups);
       hipLaunchKernelGGL(k_visc, dim3(std::max(vg, 1)), dim3(tpb), 0, h->stream, h->d_blocks, h->d_cases, h->d_ctrl,
@@ -3028,6 +3220,18 @@ int rebuild_descriptors(q2d_handle* h) {
       h->db[b].wallf[f] = nwall == 0 ? 0 : (nwall == (int)fl.size() ? 1 : 2);
     }
   for (DevBC& r : h->dbc) r.dpos = h->db[r.blk].bcf[r.f - 1] + (r.f <= 2 ? r.j - 1 : r.i - 1);
+  // mixed laminar cases whose stage kernel evaluates the viscous terms itself: no area-variation source
+  // (its V.G terms stay in k_visc) and no face with both wall and other records
+  h->any_unfused_visc = false;
+  for (int c = 0; c < h->ncases; ++c) {
+    DevCase& k = h->dc[c];
+    bool fv = h->fuse_visc && h->precision_mixed && k.model > 0 && k.source != Q2D_SRC_AREA;
+    for (int b = 0; b < h->nblocks && fv; ++b)
+      if (h->hb[b].case_id == c)
+        for (int f = 0; f < 4; ++f) fv = fv && h->db[b].wallf[f] != 2;
+    k.fvisc = fv ? 1 : 0;
+    if (h->case_live[c] && k.model > 0 && !fv) h->any_unfused_visc = true;
+  }
   h->max_rk = 1;
   h->any_visc = false;
   h->any_irs = false;
@@ -3472,6 +3676,14 @@ int q2d_gpu_remove_case(q2d_handle* h, int32_t case_id) {
   keep.reserve(h->dbc.size());
   for (const DevBC& r : h->dbc) if (h->hb[r.blk].case_id != case_id) keep.push_back(r);
   h->dbc.swap(keep);
+  return rebuild_descriptors(h);
+}
+
+int q2d_gpu_set_fusion(q2d_handle* h, int32_t visc) {
+  if (!h) return fail(Q2D_E_ARG, "NULL argument");
+  if (visc != 0 && visc != 1) return fail(Q2D_E_ARG, "visc must be 0 or 1");
+  HIPCHK(hipStreamSynchronize(h->stream));
+  h->fuse_visc = visc != 0;
   return rebuild_descriptors(h);
 }
 
