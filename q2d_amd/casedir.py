@@ -75,6 +75,11 @@ def _dir_token(tok):
         return 0.0
 
 
+def _fields(line):
+    """The values of a bc.txt line: list-directed, separated by blanks and/or commas."""
+    return [t for t in re.split(r"[,\s]+", line.strip()) if t]
+
+
 def read_bc(path, nrec):
     """The nrec records of INPUT/bc.txt as cases-style dicts, ordered by block, then face (stable)."""
     toks = Path(path).read_text().splitlines()
@@ -82,23 +87,23 @@ def read_bc(path, nrec):
     for _ in range(nrec):
         while not toks[k].strip():
             k += 1
-        b, i, j, f, t = (int(v) for v in toks[k].split()[:5])
+        b, i, j, f, t = (int(v) for v in _fields(toks[k])[:5])
         k += 1
         r = {"b": b, "i": i, "j": j, "f": f, "type": t, "case_id": 0}
         if t in (101, 201):
-            v = [int(x) for x in toks[k].split()[:8]]
+            v = [int(x) for x in _fields(toks[k])[:8]]
             r.update(bs=v[0], js=v[2], fs=v[3], d11=v[4], d12=v[5], d21=v[6], d22=v[7])
             r["is"] = v[1]
             k += 1
         elif t == 302:
-            r["T"] = float(toks[k].split()[0])
+            r["T"] = float(_fields(toks[k])[0])
             k += 1
         elif t == 405:
-            v = toks[k].split()
+            v = _fields(toks[k])
             r.update(mach=float(v[0]), T=float(v[1]), p=float(v[2]), alpha=_dir_token(v[3]), rel_fac=float(v[5]))
             k += 1
         elif t == 406:
-            v = toks[k].split()
+            v = _fields(toks[k])
             r.update(p=float(v[0]), rel_fac=float(v[1]))
             k += 1
         elif t in (200, 300, 400):
