@@ -92,7 +92,10 @@ struct DevCase {
   int space_sd;                         // MUSCL-SD: Tramel's detector scales the slopes (sd_tramel)
   int ntab;                             // table mode: nodes per property plane (0: closed forms)
   int fvisc;                            // mixed laminar case whose stage kernel evaluates the viscous terms
-  int pad_fv;
+  int mxk;                              // the case's items run k_stage_mx (mixed form: plain MUSCL, HLLC or
+                                        // HLLE; laminar ones with fvisc and no residual smoothing)
+  int dtmx;                             // its dt is cell_dt_mx's, from FP32 geometry (mixed Euler, mxk laminar)
+  int pad_mx;
   const double* tab;                    // table mode: h, cp, mu, k planes (device copy), else null
 };
 
@@ -128,6 +131,8 @@ struct DevBlock {
   const float4* mgd;
   const float* mhy;
   const float2* msrc;
+  const float4* mmm;             // mixed-form laminar blocks: M (c11, c21, c12, c22) in FP32 and the node
+  const float* mnod;             // radius node(i,j)%c(2) in FP32, for the viscous terms of k_stage_mx
   double* irs[1];                // IRS cases: the stage's scaled residual (4 planes)
 };
 
@@ -305,9 +310,11 @@ __device__ __forceinline__ double cell_dt(const DevCase& C, const DevCtrl& K, do
   return dtmin;
 }
 
-// Compute_dt of a cell of a mixed-form Euler block, from its FP32 geometry: half lengths hx, hy and the
-// normalised rows of M (gd); FP64 arithmetic.  The dt pass and the fused dt of k_stage_mx's last stage
-// both evaluate exactly this, so a step may take its dt from either.  iter: the iteration being started.
+// Compute_dt of a cell of a k_stage_mx case, from its FP32 geometry: half lengths hx, hy and the
+// normalised rows of M (gd); FP64 arithmetic, and for laminar cases (VISC) the VNN limit with the lengths
+// 2 hx, 2 hy.  The dt pass and the fused dt of k_stage_mx's last stage both evaluate exactly this, so a
+// step may take its dt from either.  iter: the iteration being started.
+template <bool VISC>
 __device__ __forceinline__ double cell_dt_mx(const DevCase& C, int iter, double rho, double u, double v, double p,
                                              float hx, float hy, float g1x, float g1y, float g2x, float g2y) {
   const double snd = sqrt(C.gam * p * rcp_nr(rho));
@@ -322,6 +329,7 @@ __device__ __forceinline__ double cell_dt_mx(const DevCase& C, int iter, double 
     const double dtd = 2.0 * (double)hy * rcp_nr(lambda) * C.cfl;
     if (dtd < dtmin) dtmin = dtd;
   }
+  if (VISC && C.model > 0) dtmin = vnn_limit(C, dtmin, rho, p, 2.0 * (double)hx, 2.0 * (double)hy);
   if (iter < C.ramp_iter) dtmin = dtmin * (double)iter / (double)C.ramp_iter;   // Mod_dt.f90:43
   return dtmin;
 }
@@ -366,11 +374,11 @@ __global__ __launch_bounds__(256) void k_dt(const DevBlock* __restrict__ blocks,
         o[k] = (long long)(j + 2) * pitch + (i + 2);
       }
       double d[4];
-      if (B.mgd) {                                   // mixed-form Euler block: the fused dt's formula
+      if (C.dtmx) {                                  // a k_stage_mx case: that kernel's fused dt formula
         const __attribute__((address_space(1))) float* gd = (const __attribute__((address_space(1))) float*)B.mgd;
 #pragma unroll
         for (int k = 0; k < 4; ++k)
-          d[k] = cell_dt_mx(C, K.iter, P[o[k]], P[pl + o[k]], P[2 * pl + o[k]], P[3 * pl + o[k]],
+          d[k] = cell_dt_mx<true>(C, K.iter, P[o[k]], P[pl + o[k]], P[2 * pl + o[k]], P[3 * pl + o[k]],
                             (float)(0.5 * dlx[o[k]]), (float)(0.5 * dly[o[k]]), gd[4 * o[k]], gd[4 * o[k] + 1],
                             gd[4 * o[k] + 2], gd[4 * o[k] + 3]);
       } else {
@@ -489,17 +497,21 @@ __device__ __forceinline__ double sd_tramel(double p, double p1, double p2, doub
 
 struct BCFlux { double F[4]; double V[4]; int mask; int vmask; };
 
-// Cases of a mixed-form handle whose Euler items run k_stage_mx: MUSCL-SD and the HLLEM / HLLE++ / HLLC+
-// solvers are not in that kernel, so such cases run the generic FP32-face kernel k_stage<float>.
-__host__ __device__ __forceinline__ bool mx_kernel_case(int model, int space_sd, int riemann) {
-  return model == 0 && !space_sd && riemann < Q2D_RIEMANN_HLLEM;
+// Cases of a mixed-form handle whose items run k_stage_mx: MUSCL-SD and the HLLEM / HLLE++ / HLLC+
+// solvers are not in that kernel, so such cases run the generic FP32-face kernel k_stage<float>; a laminar
+// case also needs its viscous terms in the stage kernel (fvisc) and no residual smoothing.
+__host__ __device__ __forceinline__ bool mx_kernel_case(int model, int space_sd, int riemann, int fvisc,
+                                                        double irs_beta) {
+  if (space_sd || riemann >= Q2D_RIEMANN_HLLEM) return false;
+  return model == 0 || (fvisc != 0 && !(irs_beta > 0.0));
 }
 
 // A record's result, stored at the record's boundary cell (DevBC::dpos): the BCFlux struct (read by the
 // FP64, generic FP32-face and laminar stage kernels, which add the faces in record order) and, for the
-// mixed Euler stage kernel, its convective flux with the mask applied, in FP32 (the precision of that
-// form's face fluxes), in the dense per-face array (a float4 per boundary cell, indexed like the per-face
-// record index), so that kernel reads a boundary cell's flux with one load and no indirection.
+// cases of k_stage_mx, its convective flux with the mask applied, in FP32 (the precision of that form's
+// face fluxes), in the dense per-face array (a float4 per boundary cell, indexed like the per-face record
+// index), so that kernel reads a boundary cell's flux with one load and no indirection; k_bcvisc adds a
+// laminar record's viscous flux to the same element.
 __device__ __forceinline__ void emit_bc(BCFlux* __restrict__ out, float4* __restrict__ dense, int dpos,
                                         const BCFlux& res, bool mx) {
   if (!(dense && mx)) out[dpos] = res;                // k_stage_mx reads only the dense array
@@ -583,9 +595,10 @@ __global__ __launch_bounds__(256) void k_bcflux(const DevBlock* __restrict__ blo
   // Face_Index — Lib_BC_Fluxes.f90:34-50
   const bool idir = Fm <= 2;
   const long long ofc = idir ? off(Im - modfm, Jm) : off(Im, Jm - modfm);
-  // mixed-form Euler blocks read the compact FP32 geometry (face length-scaled normal, half cell sizes):
-  // fewer cache lines per record, which matters for i-face records (one line per lane per plane)
-  const bool mxg = sizeof(FT) == 4 && B.mgn != nullptr;
+  // mixed-form Euler blocks and the blocks of laminar k_stage_mx cases read the compact FP32 geometry (face
+  // length-scaled normal, half cell sizes): fewer cache lines per record, which matters for i-face records
+  // (one line per lane per plane)
+  const bool mxg = sizeof(FT) == 4 && B.mgn != nullptr && (C.model == 0 || C.mxk != 0);
   double n1, n2, il;
   if (mxg) {
     const float4 g = B.mgn[ofc];
@@ -647,7 +660,7 @@ __global__ __launch_bounds__(256) void k_bcflux(const DevBlock* __restrict__ blo
     // one species: Sel_L*rho_L/rho_L - Sel_R*rho_R/rho_R == 1, so Flux(1) = F_r*int_l
 #pragma unroll
     for (int v = 0; v < 4; ++v) res.F[v] = ((double)F[v] * il) * sgn;
-    emit_bc(out, dense, E.dpos, res, mx_kernel_case(C.model, C.space_sd, C.riemann));
+    emit_bc(out, dense, E.dpos, res, C.mxk != 0);
     return;
   }
 
@@ -665,7 +678,7 @@ __global__ __launch_bounds__(256) void k_bcflux(const DevBlock* __restrict__ blo
     Fb[3] = Fmass * ((gd.tab ? tab_at(gd.tab, gd.ntab, kTabH, Tt) : gd.cp * Tt) + 0.5 * vmag * vmag);
 #pragma unroll
     for (int k = 0; k < 4; ++k) res.F[k] = sgn * Fb[k];
-    emit_bc(out, dense, E.dpos, res, mx_kernel_case(C.model, C.space_sd, C.riemann));
+    emit_bc(out, dense, E.dpos, res, C.mxk != 0);
     return;
   }
   if (ty == 406) {
@@ -709,7 +722,7 @@ __global__ __launch_bounds__(256) void k_bcflux(const DevBlock* __restrict__ blo
       Fb[3] = mdot * (0.5 * (Fp[1] * Fp[1] + Fp[2] * Fp[2]) + Hf);
 #pragma unroll
       for (int k = 0; k < 4; ++k) res.F[k] = sgn * Fb[k];
-      emit_bc(out, dense, E.dpos, res, mx_kernel_case(C.model, C.space_sd, C.riemann));
+      emit_bc(out, dense, E.dpos, res, C.mxk != 0);
       return;
     }
   }
@@ -777,7 +790,7 @@ __global__ __launch_bounds__(256) void k_bcflux(const DevBlock* __restrict__ blo
 #pragma unroll
   for (int v = 0; v < 4; ++v) res.F[v] = sgn * ((double)F[v] * il);
   if (res.mask == 0x6) { res.F[0] = 0.0; res.F[3] = 0.0; }
-  emit_bc(out, dense, E.dpos, res, mx_kernel_case(C.model, C.space_sd, C.riemann));
+  emit_bc(out, dense, E.dpos, res, C.mxk != 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -1115,10 +1128,30 @@ __global__ __launch_bounds__(256) void k_visc(const DevBlock* __restrict__ block
 // (Lib_BC_Fluxes_Wall_Temperature.f90:20-106, static wall) for the boundary records of viscous cases.
 // Runs after every ghost cell of the stage is filled: at a corner the symmetry stencil reads the
 // neighbouring face's ghost.  One thread per listed record; the result goes to BCFlux::V.
+// A record's viscous flux: into its BCFlux (added after the convective one by the stage kernels that read
+// the struct) or, for the cases of k_stage_mx, added in FP32 to the record's element of the dense per-face
+// array, which k_bcflux wrote earlier in the stage (one record per element: no other writer).
+__device__ __forceinline__ void emit_bcvisc(BCFlux* __restrict__ out, float4* __restrict__ dense, int dpos,
+                                            const double V[4], int vmask, bool mx) {
+  if (dense && mx) {
+    float4 q = dense[dpos];
+    if (vmask & 1) q.x += (float)V[0];
+    if (vmask & 2) q.y += (float)V[1];
+    if (vmask & 4) q.z += (float)V[2];
+    if (vmask & 8) q.w += (float)V[3];
+    dense[dpos] = q;
+    return;
+  }
+  BCFlux& q = out[dpos];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) q.V[v] = V[v];
+  q.vmask = vmask;
+}
+
 __global__ __launch_bounds__(256) void k_bcvisc(const DevBlock* __restrict__ blocks, const DevCase* __restrict__ cases,
                                                 const DevCtrl* __restrict__ ctrl, const DevBC* __restrict__ bcs,
                                                 const int* __restrict__ vrec, int nvrec, int irk,
-                                                BCFlux* __restrict__ out) {
+                                                BCFlux* __restrict__ out, float4* __restrict__ dense) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nvrec) return;
   const int k = vrec[t];
@@ -1224,11 +1257,11 @@ __global__ __launch_bounds__(256) void k_bcvisc(const DevBlock* __restrict__ blo
     double Dl[3], Dg[3];
     diff_flux(C, axi, pm0, pm1, pm2, pm3, a[0][0], a[0][1], a[1][0], a[1][1], a[2][0], a[2][1], n1, n2, il, r_ave, VG, Dl);
     diff_flux(C, axi, pm0, pm1, pm2, pm3, b[0][0], b[0][1], b[1][0], b[1][1], b[2][0], b[2][1], n1, n2, il, r_ave, VG, Dg);
-    BCFlux& q = out[E.dpos];
-    q.V[0] = 0.0;
+    double V[4];
+    V[0] = 0.0;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) q.V[c + 1] = -(0.5 * m2 * (Dl[c] + Dg[c]));
-    q.vmask = 0xE;
+    for (int c = 0; c < 3; ++c) V[c + 1] = -(0.5 * m2 * (Dl[c] + Dg[c]));
+    emit_bcvisc(out, dense, E.dpos, V, 0xE, C.mxk != 0);
     return;
   } else {                                           // 302
     const double T = p / (rho * C.R);
@@ -1244,10 +1277,10 @@ __global__ __launch_bounds__(256) void k_bcvisc(const DevBlock* __restrict__ blo
     F1 = S1 * il; F2 = S2 * il;
     F3 = il * kl * (gTx * n1 + gTy * n2);
   }
-  BCFlux& q = out[E.dpos];
-  q.V[0] = -(sgn * 0.0);                             // Flux(1) = 0; BC 302 updates R(:), BC 300 R(nu:np)
-  q.V[1] = -(sgn * F1); q.V[2] = -(sgn * F2); q.V[3] = -(sgn * F3);
-  q.vmask = E.type == 302 ? 0xF : 0xE;
+  double V[4];
+  V[0] = -(sgn * 0.0);                               // Flux(1) = 0; BC 302 updates R(:), BC 300 R(nu:np)
+  V[1] = -(sgn * F1); V[2] = -(sgn * F2); V[3] = -(sgn * F3);
+  emit_bcvisc(out, dense, E.dpos, V, E.type == 302 ? 0xF : 0xE, C.mxk != 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -1866,6 +1899,10 @@ typedef const __attribute__((address_space(1))) f4v* GP4;
 #ifndef Q2D_STAGE_MX_MIN_WAVES
 #define Q2D_STAGE_MX_MIN_WAVES 3
 #endif
+// The laminar instantiation (VISC) holds the viscous stencil rows as well: 2 waves per SIMD.
+#ifndef Q2D_STAGE_MXV_MIN_WAVES
+#define Q2D_STAGE_MXV_MIN_WAVES 2
+#endif
 // Streaming (non-temporal) access for the data each mixed stage touches once: the new state it writes,
 // the RK base state and the i-face, j-face and dt geometry it reads; the L2 keeps the state rows that
 // neighbouring items re-read (their halos).  Measured on 4 x 1024^2, SSP-RK3: 0.516 -> 0.508 ms per
@@ -1898,8 +1935,8 @@ __device__ __forceinline__ float2 mx_stream_load2(GPF2 p) {
 }
 // IRS: the launch holds cases with implicit residual smoothing (a separate instantiation, so that the
 // register allocation of the plain kernel is not affected by the residual-store branch).
-template <int RIE, bool FUSE, bool IRS>
-__global__ __launch_bounds__(256, Q2D_STAGE_MX_MIN_WAVES) void k_stage_mx(const DevBlock* __restrict__ blocks, const DevCase* __restrict__ cases,
+template <int RIE, bool FUSE, bool IRS, bool VISC>
+__global__ __launch_bounds__(256, VISC ? Q2D_STAGE_MXV_MIN_WAVES : Q2D_STAGE_MX_MIN_WAVES) void k_stage_mx(const DevBlock* __restrict__ blocks, const DevCase* __restrict__ cases,
                                                   DevCtrl* __restrict__ ctrl, const Item* __restrict__ items, int item_lo,
                                                   int item_hi, const float4* __restrict__ bcdense, double* __restrict__ partials,
                                                   int irk, int xcd_remap, unsigned long long* __restrict__ tstamp,
@@ -1968,6 +2005,22 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MX_MIN_WAVES) void k_stage_mx(const 
       const bool rface = B.bcf[1] >= 0 && ni < item.x0 + kOwn;
       const int rmax = nj + 2;
       auto at = [&](int r) { return (long long)(r + 1) * pitch + (xc + 1); };
+      // Laminar cases (VISC): the diffusive face fluxes and the axisymmetric viscous source in FP32, with the
+      // formulas of the generic laminar kernel (visc_face_f, visc_source_axi_f).  The temperatures of rows r
+      // and r+1, (u, v, T) of row r-1, M of rows r and r+1 (r+2 in flight) and the node radius of rows r-1
+      // and r (r+1 in flight) roll with the stencil; the i-neighbours come through DPP.  A face's diffusive
+      // flux D joins its convective flux in the face's slot as F - D (its density component is 0), so the
+      // residual, the boundary faces (k_bcvisc adds the records' viscous fluxes to the dense array) and the
+      // update are the Euler kernel's; the viscous source joins the Euler source.
+      GP4 mmp = (GP4)B.mmm;
+      GPF nodp = (GPF)B.mnod;
+      const float Rf = (float)C.R;
+      TransF trf{0.f, 0.f, 0.f, 0.f, 0.f};
+      if constexpr (VISC) trf = trans_f(C);
+      const bool vaxi = C.source == Q2D_SRC_AXI;
+      float TA = 0.f, TB = 0.f, TC = 0.f, Pu = 0.f, Pv = 0.f, PT = 0.f;
+      f4v MA = {0.f, 0.f, 0.f, 0.f}, MB = {0.f, 0.f, 0.f, 0.f}, MF = {0.f, 0.f, 0.f, 0.f};
+      float nodS = 0.f, nodA = 0.f, nodN = 0.f;
 
       // ---- pipeline warm-up: rows y0-2 .. y0+1 ------------------------------
       double qA[4], qB[4], qN[4];                        // FP64 rows r, r+1 and r+2 (in flight)
@@ -1996,6 +2049,13 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MX_MIN_WAVES) void k_stage_mx(const 
 #pragma unroll
         for (int v = 0; v < 4; ++v) phiA[v] = limiter_mxt<LIM>(sAB[v], s0[v]);
       }
+      if constexpr (VISC) {                              // rows y0-1 (r) and y0 (r+1) of the first j-face
+        TA = fA[3] * frcp(fA[0] * Rf);
+        TB = fB[3] * frcp(fB[0] * Rf);
+        MA = mmp[at(item.y0 - 1)];
+        MB = mmp[at(item.y0)];
+        nodA = nodp[at(item.y0 - 1)];
+      }
       double acc[4] = {0.0, 0.0, 0.0, 0.0};
       double dtnext = 1e300;
       // Failures are kept per lane (the first row of each kind) and recorded once after the rows, so
@@ -2020,10 +2080,28 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MX_MIN_WAVES) void k_stage_mx(const 
         const bool bad = jdo && !(L[0] > 0.f && L[3] > 0.f && R[0] > 0.f && R[3] > 0.f);
         fail_if_r = bad ? min(fail_if_r, r) : fail_if_r;
         flux_mxs<RV>(gf, L, R, gn.z, gn.w, G);
+        if constexpr (VISC) {
+          // diffusive flux of j-face (r | r+1) (Mod_Fluxes.f90:207-230): cells (x, r), (x, r+1) and their
+          // i-neighbours, M averaged over the two cells, the face radius from nodes (x-1, r) and (x, r); the
+          // scaled normal carries int_l
+          TC = fC[3] * frcp(fC[0] * Rf);
+          const VCf a{fA[0], fA[1], fA[2], fA[3], TA}, b{fB[0], fB[1], fB[2], fB[3], TB};
+          const VCf aW = vcf_shup(a), aE = vcf_shdn(a), bW = vcf_shup(b), bE = vcf_shdn(b);
+          const float nodW = shup(nodA);
+          const float Ma[4] = {0.5f * (MA.x + MB.x), 0.5f * (MA.y + MB.y), 0.5f * (MA.z + MB.z), 0.5f * (MA.w + MB.w)};
+          double D[3];
+          visc_face_f(trf, Rf, vaxi, 1, gn.z, gn.w, 1.0f, a, b, aW, aE, bW, bE, Ma, 0.5f * (nodA + nodW), D);
+#pragma unroll
+          for (int k = 0; k < 3; ++k) G[k + 1] = G[k + 1] - (float)D[k];
+        }
 #pragma unroll
         for (int v = 0; v < 4; ++v) Gcur[v] = jdo ? G[v] : 0.f;
       };
       auto roll = [&](const double (&qC)[4], float hyC) {
+        if constexpr (VISC) {
+          Pu = fA[1]; Pv = fA[2]; PT = TA; TA = TB; TB = TC;
+          MA = MB; MB = MF; nodS = nodA; nodA = nodN;
+        }
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           qA[v] = qB[v]; qB[v] = qC[v]; fA[v] = fB[v]; fB[v] = fC[v];
@@ -2044,6 +2122,7 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MX_MIN_WAVES) void k_stage_mx(const 
         for (int v = 0; v < 4; ++v) qN[v] = Pin[v * pl + oN];
         hyN = hyp[oN];
         gnN = mx_stream_load(&gnp[at(min(r + 1, rmax))]);
+        if constexpr (VISC) { MF = mmp[at(min(r + 2, rmax))]; nodN = nodp[at(min(r + 1, rmax))]; }
         jface(r, qC, hyC, gn);
         roll(qC, hyC);
       }
@@ -2089,6 +2168,7 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MX_MIN_WAVES) void k_stage_mx(const 
           for (int v = 0; v < 4; ++v) qN[v] = Pin[v * pl + oN];
           hyN = hyp[oN];
           gnN = mx_stream_load(&gnp[at(min(r + 1, rmax))]);
+          if constexpr (VISC) { MF = mmp[at(min(r + 2, rmax))]; nodN = nodp[at(min(r + 1, rmax))]; }
         }
         jface(r, qC, hyC, gn);
 
@@ -2113,6 +2193,20 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MX_MIN_WAVES) void k_stage_mx(const 
           const bool bad = xdo && !(L[0] > 0.f && L[3] > 0.f && R[0] > 0.f && R[3] > 0.f);
           fail_if_r = bad ? min(fail_if_r, r) : fail_if_r;
           flux_mxs<RV>(gf, L, R, gn.x, gn.y, F);
+          if constexpr (VISC) {
+            // diffusive flux of i-face (x | x+1) (Mod_Fluxes.f90:160-183): cells (x, r), (x+1, r) and rows
+            // r-1, r+1 with their x+1 neighbours, M averaged over the two cells, the face radius from nodes
+            // (x, r-1) and (x, r)
+            const VCf a{fA[0], fA[1], fA[2], fA[3], TA}, e = vcf_shdn(a);
+            const VCf sS{0.f, Pu, Pv, 0.f, PT}, sN{fB[0], fB[1], fB[2], fB[3], TB};
+            const VCf sSE = vcf_shdn(sS), sNE = vcf_shdn(sN);
+            const f4v ME = {shdn(MA.x), shdn(MA.y), shdn(MA.z), shdn(MA.w)};
+            const float Ma[4] = {0.5f * (MA.x + ME.x), 0.5f * (MA.y + ME.y), 0.5f * (MA.z + ME.z), 0.5f * (MA.w + ME.w)};
+            double D[3];
+            visc_face_f(trf, Rf, vaxi, 0, gn.x, gn.y, 1.0f, a, e, sS, sN, sSE, sNE, Ma, 0.5f * (nodA + nodS), D);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) F[k + 1] = F[k + 1] - (float)D[k];
+          }
 #pragma unroll
           for (int v = 0; v < 4; ++v) Fx[v] = xdo ? F[v] : 0.f;
         }
@@ -2162,6 +2256,24 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MX_MIN_WAVES) void k_stage_mx(const 
             S[1] = -(rho * u * v * invr);
             S[2] = -(rho * v * v * invr);
             S[3] = -((rhoE0 + p) * v * invr);
+            if constexpr (VISC) {
+              // Source_Axi_Visc_Blk (Lib_Source_Axisymmetry.f90:125-236), after the Euler source; a wall face's
+              // ghost is the boundary cell with its velocity reversed (Ghost_Wall_Extrapolation,
+              // Lib_Ghost.f90:366-399)
+              const VCf a{fA[0], fA[1], fA[2], fA[3], TA};
+              VCf qE = vcf_shdn(a), qW = vcf_shup(a), qS{0.f, Pu, Pv, 0.f, PT}, qN{fB[0], fB[1], fB[2], fB[3], TB};
+              const VCf refl{a.r, -a.u, -a.v, a.p, a.T};
+              auto wall = [&](int f) { return B.bcf[f] >= 0 && B.wallf[f] == 1; };
+              if (x == 1 && wall(0)) qW = refl;
+              if (x == ni && wall(1)) qE = refl;
+              if (r == 1 && wall(2)) qS = refl;
+              if (r == nj && wall(3)) qN = refl;
+              const float Mc[4] = {MA.x, MA.y, MA.z, MA.w};
+              double Sv[3];
+              visc_source_axi_f(trf, a, qE, qW, qN, qS, Mc, sg.x, sg.y, Sv);
+#pragma unroll
+              for (int k = 0; k < 3; ++k) S[k + 1] = S[k + 1] + Sv[k];
+            }
           } else {
             const double VdG = u * (double)sg.x + v * (double)sg.y;
             S[0] = rho * VdG;
@@ -2213,7 +2325,7 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MX_MIN_WAVES) void k_stage_mx(const 
         double dtmin = 0.0;
         if (last && FUSE) {
           // next step's Compute_dt on the state just written (Mod_dt.f90:59-104), FP64
-          dtmin = cell_dt_mx(C, kiter + 1, qn2[0], qn2[1], qn2[2], qn2[3], hx, hyA, gd.x, gd.y, gd.z, gd.w);
+          dtmin = cell_dt_mx<VISC>(C, kiter + 1, qn2[0], qn2[1], qn2[2], qn2[3], hx, hyA, gd.x, gd.y, gd.z, gd.w);
           dtnext = (own && dtmin < dtnext) ? dtmin : dtnext;
         }
         if (own) {
@@ -2534,10 +2646,10 @@ __global__ __launch_bounds__(256) void k_irs(const DevBlock* __restrict__ blocks
           for (int v = 0; v < 4; ++v) { const double d = fabs(qn2[v] - po[v]); acc[v] = acc[v] + d * d; }
           // the next step's Compute_dt on the state just written (as the stage kernel's fused form)
           double dtmin;
-          if (B.mgd) {                                 // mixed-form Euler block: its FP32 dt geometry
+          if (C.dtmx) {                                // mixed-form Euler block: its FP32 dt geometry
             const float2 gs = mx_load2((GPF2)&B.mgs[o]);
             const f4v g4 = mx_load4(&B.mgd[o]);
-            dtmin = cell_dt_mx(C, kiter + 1, qn2[0], qn2[1], qn2[2], qn2[3], gs.x, ((GPF)B.mhy)[o], g4.x, g4.y,
+            dtmin = cell_dt_mx<true>(C, kiter + 1, qn2[0], qn2[1], qn2[2], qn2[3], gs.x, ((GPF)B.mhy)[o], g4.x, g4.y,
                                g4.z, g4.w);
           } else {
             const double mm[4] = {gp(B.m11)[o], gp(B.m11)[pl + o], gp(B.m11)[2 * pl + o], gp(B.m11)[3 * pl + o]};
@@ -2733,22 +2845,24 @@ struct q2d_handle {
 
 namespace {
 
-// The item group of case c: laminar cases 2 + (riemann != 0); Euler cases of a mixed-form handle 0 when they
-// run k_stage_mx, 1 when they need the generic FP32-face kernel; Euler cases of an FP64 handle riemann != 0.
-// FP64 Euler groups 0 and 1 share one launch, as do groups 2 and 3.
+// The item group of case c.  Mixed form: Euler cases 0 when they run k_stage_mx, 1 when they need the generic
+// FP32-face kernel; laminar cases 2 (k_stage_mx's laminar instantiation) or 3 (the generic laminar kernel);
+// each group is its own launch.  FP64 form: Euler cases riemann != 0, laminar 2 + (riemann != 0); groups 0
+// and 1 share one launch, as do groups 2 and 3.
 int item_pass(const q2d_handle* h, int c) {
   const DevCase& C = h->dc[c];
+  if (h->precision_mixed) return (C.model > 0 ? 2 : 0) + (C.mxk ? 0 : 1);
   if (C.model > 0) return 2 + (C.riemann != 0 ? 1 : 0);
-  if (h->precision_mixed) return mx_kernel_case(C.model, C.space_sd, C.riemann) ? 0 : 1;
   return C.riemann != 0 ? 1 : 0;
 }
 
 // The stage-kernel instantiation that runs the items of group `pass` (item_pass()).
 const void* stage_kernel(const q2d_handle* h, int pass) {
   if (h->precision_mixed) {
-    if (pass == 0) return reinterpret_cast<const void*>(&k_stage_mx<-1, kFuseDtMx, false>);
+    if (pass == 0) return reinterpret_cast<const void*>(&k_stage_mx<-1, kFuseDtMx, false, false>);
     if (pass == 1) return h->any_hllepp ? reinterpret_cast<const void*>(&k_stage<float, -2, false, false>)
                                         : reinterpret_cast<const void*>(&k_stage<float, -1, false, false>);
+    if (pass == 2) return reinterpret_cast<const void*>(&k_stage_mx<-1, kFuseDtMx, false, true>);
     if (h->any_hllepp) return reinterpret_cast<const void*>(&k_stage<float, -2, kFuseDtMx, true>);
     return h->any_det_or_hllem ? reinterpret_cast<const void*>(&k_stage<float, -1, kFuseDtMx, true>)
                                : reinterpret_cast<const void*>(&k_stage<float, -3, kFuseDtMx, true>);
@@ -2946,10 +3060,10 @@ template <typename FT, bool FUSE>
 int launch_stage(q2d_handle* h, int irk, const StepScratch& scr) {
   const int tpb = kWave * h->waves_per_group;
   for (int pass = 0; pass < 4; ++pass) {
-    // groups 0-1 (FP64 Euler) and 2-3 (laminar) share one launch; the mixed form's Euler groups 0
-    // (k_stage_mx) and 1 (generic FP32-face kernel) launch separately
-    const bool split = sizeof(FT) == 4 && pass < 2;
-    if ((pass == 1 && !split) || pass == 3) continue;
+    // FP64: groups 0-1 (Euler) and 2-3 (laminar) share one launch; mixed form: every group is its own
+    // launch (0, 2: k_stage_mx; 1, 3: the generic FP32-face kernels)
+    const bool split = sizeof(FT) == 4;
+    if (!split && (pass == 1 || pass == 3)) continue;
     const int lo = h->range[pass], hi = split ? h->range[pass + 1] : h->range[pass + 2];
     if (hi <= lo) continue;
     int groups = (hi - lo + h->waves_per_group - 1) / h->waves_per_group;
@@ -2971,13 +3085,14 @@ int launch_stage(q2d_handle* h, int irk, const StepScratch& scr) {
     hipLaunchKernelGGL((k_stage<float, RIE, kFuseDtMx, true>), dim3(groups), dim3(tpb), 0, h->stream, h->d_blocks,  \
                        h->d_cases, h->d_ctrl, h->d_items, lo, hi, bf, bq, scr.partials, irk, h->xcd_remap, ts,   \
                        slot)
-#define Q2D_STAGE_MX_LAUNCH(RIE, IRS)                                                                             \
-    hipLaunchKernelGGL((k_stage_mx<RIE, kFuseDtMx, IRS>), dim3(groups), dim3(tpb), 0, h->stream, h->d_blocks,      \
+#define Q2D_STAGE_MX_LAUNCH(RIE, IRS, VISC)                                                                       \
+    hipLaunchKernelGGL((k_stage_mx<RIE, kFuseDtMx, IRS, VISC>), dim3(groups), dim3(tpb), 0, h->stream, h->d_blocks,\
                        h->d_cases, h->d_ctrl, h->d_items, lo, hi, scr.dense, scr.partials, irk, h->xcd_remap, ts,     \
                        slot)
     if constexpr (sizeof(FT) == 4) {                   // mixed form: plain Euler items run k_stage_mx
-      if (pass == 0 && h->any_irs) Q2D_STAGE_MX_LAUNCH(-1, true);
-      else if (pass == 0) Q2D_STAGE_MX_LAUNCH(-1, false);
+      if (pass == 0 && h->any_irs) Q2D_STAGE_MX_LAUNCH(-1, true, false);
+      else if (pass == 0) Q2D_STAGE_MX_LAUNCH(-1, false, false);
+      else if (pass == 2) Q2D_STAGE_MX_LAUNCH(-1, false, true);   // laminar k_stage_mx cases (no smoothing)
       else if (h->any_hllepp && pass == 1) Q2D_STAGE_LAUNCH(-2, false);
       else if (h->any_hllepp) Q2D_STAGE_LAM_MX_LAUNCH(-2);
       else if (pass == 1) Q2D_STAGE_LAUNCH(-1, false);
@@ -3077,7 +3192,7 @@ int launch_step(q2d_handle* h, const StepScratch& scr) {
     }
     if (!h->vrec.empty())
       hipLaunchKernelGGL(k_bcvisc, dim3(((int)h->vrec.size() + 255) / 256), dim3(256), 0, h->stream, h->d_blocks,
-                         h->d_cases, h->d_ctrl, h->d_bc, h->d_vrec, (int)h->vrec.size(), irk, scr.bcflux);
+                         h->d_cases, h->d_ctrl, h->d_bc, h->d_vrec, (int)h->vrec.size(), irk, scr.bcflux, scr.dense);
     if (h->range[4] > h->range[2] && h->any_unfused_visc) {
       int vg = (h->range[4] - h->range[2] + h->waves_per_group - 1) / h->waves_per_group;
       if (h->max_groups > 0) vg = std::min(vg, h->max_groups);
@@ -3231,6 +3346,8 @@ int rebuild_descriptors(q2d_handle* h) {
         for (int f = 0; f < 4; ++f) fv = fv && h->db[b].wallf[f] != 2;
     k.fvisc = fv ? 1 : 0;
     if (h->case_live[c] && k.model > 0 && !fv) h->any_unfused_visc = true;
+    k.mxk = h->precision_mixed && mx_kernel_case(k.model, k.space_sd, k.riemann, k.fvisc, k.irs_beta) ? 1 : 0;
+    k.dtmx = h->precision_mixed && (k.model == 0 || k.mxk) ? 1 : 0;
   }
   h->max_rk = 1;
   h->any_visc = false;
@@ -3454,13 +3571,17 @@ int append_cases(q2d_handle* h, const q2d_problem* prob, int32_t* first_case) {
   // ---- blocks: SoA planes in one arena per block ----
   // planes 0-11: 3 state buffers x 4; 12-24 geometry; 25, 28 local dt; 26, 27 source terms; 29 node radius;
   // viscous blocks add 30-38: diffusive fluxes of i- and j-faces and the viscous source, 3 planes each;
-  // blocks of IRS cases add 4 after those: the stage's scaled residual
+  // mixed-form blocks add 7 planes of FP32 geometry after those (30-36 Euler, 39-45 laminar) and laminar
+  // ones 3 more (46-47: M, 48: node radius, FP32); blocks of IRS cases add 4 after all: the stage's scaled
+  // residual
   for (int lb = 0; lb < prob->nblocks; ++lb) {
     const q2d_block_desc& d = prob->blocks[lb];
     const bool visc = prob->cases[d.case_id].scheme.model > 0;
-    const bool mxgeo = prec == Q2D_PREC_MIXED && !visc;   // FP32 geometry planes of k_stage_mx
+    const bool mxgeo = prec == Q2D_PREC_MIXED;            // FP32 geometry planes of k_stage_mx
+    const bool mxvis = mxgeo && visc;                     // and its laminar instantiation's M and node radius
+    const int g0 = visc ? 39 : 30;                        // first FP32 geometry plane
     const bool irs = prob->cases[d.case_id].scheme.irs_beta > 0;
-    const int irs_plane = visc ? 39 : (mxgeo ? 37 : 30);
+    const int irs_plane = mxvis ? 49 : (visc ? 39 : (mxgeo ? 37 : 30));
     const int kPlanes = irs_plane + (irs ? 4 : 0);
     h->hb.emplace_back();
     h->db.emplace_back();
@@ -3517,14 +3638,16 @@ int append_cases(q2d_handle* h, const q2d_problem* prob, int32_t* first_case) {
         for (int i = 0; i <= d.ni; ++i)
           plane(29)[(size_t)(j + 1) * pitch + (i + 1)] = d.node[((size_t)j * (d.ni + 1) + i) * 2 + 1];
     if (mxgeo) {
-      // planes 30-31: GN (float4: int_l * N of the i-face x|x+1 and of the j-face r|r+1), 32: GS (float2:
-      // dl_x / 2, 1 / sur), 33-34: GD (float4), 35: HY (float), 36: SRC (float2: source geometry), over the
-      // ghost extent
-      float* gn = reinterpret_cast<float*>(plane(30));
-      float* gs = reinterpret_cast<float*>(plane(32));
-      float* gd = reinterpret_cast<float*>(plane(33));
-      float* hy = reinterpret_cast<float*>(plane(35));
-      float* sg = reinterpret_cast<float*>(plane(36));
+      // planes g0, g0+1: GN (float4: int_l * N of the i-face x|x+1 and of the j-face r|r+1), g0+2: GS (float2:
+      // dl_x / 2, 1 / sur), g0+3, g0+4: GD (float4), g0+5: HY (float), g0+6: SRC (float2: source geometry),
+      // over the ghost extent; laminar blocks: g0+7, g0+8: MM (float4: c11, c21, c12, c22), g0+9: NOD (float)
+      float* gn = reinterpret_cast<float*>(plane(g0));
+      float* gs = reinterpret_cast<float*>(plane(g0 + 2));
+      float* gd = reinterpret_cast<float*>(plane(g0 + 3));
+      float* hy = reinterpret_cast<float*>(plane(g0 + 5));
+      float* sg = reinterpret_cast<float*>(plane(g0 + 6));
+      float* mm = mxvis ? reinterpret_cast<float*>(plane(g0 + 7)) : nullptr;
+      float* nd = mxvis ? reinterpret_cast<float*>(plane(g0 + 9)) : nullptr;
       for (int j = 0; j < ngj; ++j)
         for (int i = 0; i < ngi; ++i) {
           const size_t c = (size_t)j * pitch + i;
@@ -3538,6 +3661,10 @@ int append_cases(q2d_handle* h, const q2d_problem* prob, int32_t* first_case) {
           gd[4 * c + 2] = n2 > 0.0 ? (float)(c21 / n2) : 0.f; gd[4 * c + 3] = n2 > 0.0 ? (float)(c22 / n2) : 0.f;
           hy[c] = (float)(0.5 * plane(13)[c]);
           sg[2 * c + 0] = (float)plane(26)[c]; sg[2 * c + 1] = (float)plane(27)[c];
+          if (mxvis) {
+            mm[4 * c + 0] = (float)c11; mm[4 * c + 1] = (float)c21; mm[4 * c + 2] = (float)c12; mm[4 * c + 3] = (float)c22;
+            nd[c] = (float)plane(29)[c];
+          }
         }
     }
     HIPCHK(hipMemcpy(hb.arena, host.data(), hb.arena_bytes, hipMemcpyHostToDevice));
@@ -3555,9 +3682,13 @@ int append_cases(q2d_handle* h, const q2d_problem* prob, int32_t* first_case) {
     db.nodr = a + 29 * pl;
     if (visc) { db.vdx = a + 30 * pl; db.vey = a + 33 * pl; db.vsv = a + 36 * pl; }
     if (mxgeo) {
-      db.mgn = reinterpret_cast<const float4*>(a + 30 * pl); db.mgs = reinterpret_cast<const float2*>(a + 32 * pl);
-      db.mgd = reinterpret_cast<const float4*>(a + 33 * pl); db.mhy = reinterpret_cast<const float*>(a + 35 * pl);
-      db.msrc = reinterpret_cast<const float2*>(a + 36 * pl);
+      db.mgn = reinterpret_cast<const float4*>(a + g0 * pl); db.mgs = reinterpret_cast<const float2*>(a + (g0 + 2) * pl);
+      db.mgd = reinterpret_cast<const float4*>(a + (g0 + 3) * pl); db.mhy = reinterpret_cast<const float*>(a + (g0 + 5) * pl);
+      db.msrc = reinterpret_cast<const float2*>(a + (g0 + 6) * pl);
+    }
+    if (mxvis) {
+      db.mmm = reinterpret_cast<const float4*>(a + (g0 + 7) * pl);
+      db.mnod = reinterpret_cast<const float*>(a + (g0 + 9) * pl);
     }
     if (irs) db.irs[0] = a + (size_t)irs_plane * pl;
     for (int f = 0; f < 4; ++f) db.bcf[f] = -1;

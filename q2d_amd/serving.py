@@ -26,9 +26,14 @@ from .capi import GpuSolver
 
 def block_planes(spec):
     """FP64 planes of one block of a case, as q2d_gpu_add_cases allocates them: 30 for an FP64 Euler block,
-    37 for a mixed-form Euler block (its FP32 geometry), 39 for a laminar block, plus 4 with residual
-    smoothing (the stage's scaled residual)."""
-    planes = 39 if spec.model > 0 else (37 if spec.precision == "mixed" else 30)
+    37 for a mixed-form Euler block (its FP32 geometry), 39 for an FP64 laminar block and 49 for a
+    mixed-form one (FP32 geometry, M and node radius), plus 4 with residual smoothing (the stage's scaled
+    residual)."""
+    mixed = spec.precision == "mixed"
+    if spec.model > 0:
+        planes = 49 if mixed else 39
+    else:
+        planes = 37 if mixed else 30
     return planes + (4 if spec.irs_beta > 0 else 0)
 
 

@@ -2,8 +2,10 @@
 
 FP64 form: bit-identical state and local time step (interior diffusive fluxes, viscous symmetry and
 isothermal-wall fluxes, axisymmetric viscous source with reflected wall ghosts, VNN limit).
-Mixed form (FP32 convective face flux, FP64 viscous terms and state): relative L1 density error
-<= 1e-5 against the FP64 oracle."""
+Mixed form (FP32 convective and diffusive face fluxes, FP64 state and update): relative L1 density error
+<= 1e-5 against the FP64 oracle.  The full-size axisymmetric pipe (2048x512, first cell 1e-5 L, Re_L 1e5)
+is checked for a few steps in both forms, and a mixed laminar case on k_stage_mx keeps its bits beside
+cases of the generic laminar kernel."""
 import math
 
 import numpy as np
@@ -87,6 +89,55 @@ def test_axisymmetric_pipe_mixed_within_tolerance(oracle_mod):
     a, r = g.state(0)[2:-2, 2:-2, 0], o.state(0)[2:-2, 2:-2, 0]
     err = np.abs(a - r).sum() / np.abs(r).sum()
     assert 0 < err <= TOL_L1, err
+
+
+def test_axisymmetric_pipe_full_size(oracle_mod):
+    """The laminar workload at its full size (2048x512, tanh wall clustering with a first cell of 1e-5 L,
+    Re_L 1e5, SSP-RK3, local dt): 8 steps, FP64 bit-identical to the oracle (state and local dt), the mixed
+    form within its tolerance of the same oracle run."""
+    steps = 8
+    make = lambda prec: cases.axisymmetric_pipe(seed=0, precision=prec, steps=steps)
+    pb = make("fp64")
+    print(f"full-size pipe {pb.blocks[0].ni}x{pb.blocks[0].nj}, {steps} steps", flush=True)
+    assert (pb.blocks[0].ni, pb.blocks[0].nj) == (2048, 512)
+    g, go, o, oo = _run_both(oracle_mod, pb, steps)
+    assert go[0]["iter"] == oo[0]["iter"] == steps
+    ref = o.state(0)[2:-2, 2:-2]
+    assert np.array_equal(g.state(0)[2:-2, 2:-2], ref), np.abs(g.state(0)[2:-2, 2:-2] - ref).max()
+    assert np.array_equal(g.dtlocal(0), o.dtlocal(0))
+    g.close()
+    gm = GpuSolver(make("mixed"))
+    gmo = gm.run(steps)
+    assert gmo[0]["iter"] == steps and gmo[0]["status"] == 0
+    a, r = gm.state(0)[2:-2, 2:-2, 0], ref[..., 0]
+    err = np.abs(a - r).sum() / np.abs(r).sum()
+    assert 0 < err <= TOL_L1, err
+    gm.close()
+
+
+def test_mixed_laminar_case_keeps_its_bits_beside_generic_cases():
+    """A mixed laminar case that runs k_stage_mx (fused viscous terms and dt) gives the same state and local
+    dt alone and in a batch with an HLLEM laminar case (the generic laminar kernel; the handle then runs the
+    dt pass every step, evaluating the same formula) and an area-variation case (k_visc)."""
+    steps = 40
+    pipe = lambda: cases.axisymmetric_pipe(seed=4, ni=96, nj=32, first_cell=5e-4, reynolds=1e4, precision="mixed",
+                                           steps=10 ** 6)
+    solo = GpuSolver(pipe())
+    so = solo.run(steps)
+    assert so[0]["status"] == 0 and so[0]["iter"] == steps
+    batch = pipe()
+    hllem = cases.axisymmetric_pipe(seed=5, ni=64, nj=24, first_cell=5e-4, reynolds=1e4, precision="mixed",
+                                    steps=10 ** 6)
+    hllem.cases[0].riemann = "HLLEM"
+    batch.add(hllem)
+    batch.add(cases.laminar_area_channel(seed=6, ni=64, nj=24, area_a=0.4, precision="mixed", steps=10 ** 6))
+    g = GpuSolver(batch)
+    go = g.run(steps)
+    assert all(o["status"] == 0 and o["iter"] == steps for o in go)
+    assert np.array_equal(g.state(0), solo.state(0)), np.abs(g.state(0) - solo.state(0)).max()
+    assert np.array_equal(g.dtlocal(0), solo.dtlocal(0))
+    g.close()
+    solo.close()
 
 
 def test_laminar_area_variation_fp64_bit_exact(oracle_mod):

@@ -50,3 +50,7 @@ def test_block_planes_rule(precision):
     assert serving.block_planes(spec) == (37 if precision == "mixed" else 30)
     spec.irs_beta = 0.5
     assert serving.block_planes(spec) == (41 if precision == "mixed" else 34)
+    lam = cases.CaseSpec(precision=precision, model=1)
+    assert serving.block_planes(lam) == (49 if precision == "mixed" else 39)
+    lam.irs_beta = 0.5
+    assert serving.block_planes(lam) == (53 if precision == "mixed" else 43)
