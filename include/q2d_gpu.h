@@ -289,6 +289,15 @@ int q2d_gpu_step_noalloc(q2d_handle *h, q2d_step_out *out);
  * out[] as for step. */
 int q2d_gpu_run(q2d_handle *h, int32_t nsteps, q2d_step_out *out);
 
+/* Capture the step graphs that q2d_gpu_run replays (eight steps and one step) now, so that the next
+ * q2d_gpu_run spends no host time on capture and instantiation; no step runs and no device memory is
+ * allocated.  No counterpart in the reference (for time loops and serving loops that keep graph capture
+ * out of their timed or latency-critical region).  Before the handle's first step after create,
+ * upload_state, add_cases or set_runtime (which runs eagerly with its dt pass) the call does nothing; any
+ * change of the handle's launches (those calls, remove_case, tiling, timing, fusion) drops the graphs
+ * again. */
+int q2d_gpu_capture(q2d_handle *h);
+
 /* Copy block `block` (0-based global index) state to/from P in the reference
  * layout P(nprim,1-gc:ni+gc,1-gc:nj+gc): what the host needs for a solution
  * write or a restart (Wrap_Postprocess.f90:67-113; domain%blk(b)%P). */

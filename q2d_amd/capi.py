@@ -169,6 +169,7 @@ SIGNATURES = {
     "q2d_gpu_case_count": ([C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int64)], C.c_int),
     "q2d_gpu_step": ([C.c_void_p, C.POINTER(StepOut)], C.c_int),
     "q2d_gpu_run": ([C.c_void_p, C.c_int32, C.POINTER(StepOut)], C.c_int),
+    "q2d_gpu_capture": ([C.c_void_p], C.c_int),
     "q2d_gpu_step_noalloc": ([C.c_void_p, C.POINTER(StepOut)], C.c_int),
     "q2d_gpu_set_runtime": ([C.c_void_p, C.c_int32, C.POINTER(Runtime)], C.c_int),
     "q2d_gpu_set_fusion": ([C.c_void_p, C.c_int32], C.c_int),
@@ -310,6 +311,10 @@ class GpuSolver:
         if raise_on_error:
             _check(self.lib, rc)
         return [o.as_dict() for o in self._out[:self.ncases]]
+
+    def capture(self):
+        """Capture the step graphs now (q2d_gpu_capture): the next run() spends no host time on capture."""
+        _check(self.lib, self.lib.q2d_gpu_capture(self.h))
 
     def state(self, block):
         b = self.blocks[block]

@@ -3878,6 +3878,34 @@ int q2d_gpu_step(q2d_handle* h, q2d_step_out* out) {
   return Q2D_OK;
 }
 
+namespace {
+// Capture `nst` back-to-back steps of the handle's launch sequence into an executable graph.
+int capture_steps(q2d_handle* h, int nst, hipGraphExec_t* out) {
+  hipGraph_t g;
+  HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+  int rc = Q2D_OK;
+  for (int k = 0; k < nst && rc == Q2D_OK; ++k) rc = launch_step(h, own_scratch(h));
+  hipError_t ce = hipStreamEndCapture(h->stream, &g);
+  if (rc != Q2D_OK) { if (ce == hipSuccess) (void)hipGraphDestroy(g); return rc; }
+  if (ce != hipSuccess) return fail(Q2D_E_HIP, std::string("graph capture: ") + hipGetErrorString(ce));
+  hipError_t ie = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (ie != hipSuccess) return fail(Q2D_E_HIP, std::string("graph instantiate: ") + hipGetErrorString(ie));
+  return Q2D_OK;
+}
+}  // namespace
+
+int q2d_gpu_capture(q2d_handle* h) {
+  if (!h) return fail(Q2D_E_ARG, "NULL handle");
+  if (h->ncases == 0) return Q2D_OK;
+  // the first step after create / upload / add_cases / set_runtime runs eagerly with its dt pass, and the
+  // graphs hold the steps after it
+  if (fused_handle(h) && !h->dt_fused_valid) return Q2D_OK;
+  if (!h->graph_multi) { int rc = capture_steps(h, kGraphSteps, &h->graph_multi); if (rc != Q2D_OK) return rc; }
+  if (!h->graph) { int rc = capture_steps(h, 1, &h->graph); if (rc != Q2D_OK) return rc; }
+  return Q2D_OK;
+}
+
 int q2d_gpu_run(q2d_handle* h, int32_t nsteps, q2d_step_out* out) {
   if (!h) return fail(Q2D_E_ARG, "NULL handle");
   if (nsteps < 0) return fail(Q2D_E_ARG, "nsteps < 0");
@@ -3892,19 +3920,7 @@ int q2d_gpu_run(q2d_handle* h, int32_t nsteps, q2d_step_out* out) {
     // Steps replay captured graphs: kGraphSteps steps per launch while that many remain (the replays of
     // one graph run back to back; each graph launch costs ~8 us of idle device time), then single steps.
     // Every kernel checks its case's active flag, so steps after a case stops are no-ops for it.
-    auto capture = [h](int nst, hipGraphExec_t* out) -> int {
-      hipGraph_t g;
-      HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-      int rc = Q2D_OK;
-      for (int k = 0; k < nst && rc == Q2D_OK; ++k) rc = launch_step(h, own_scratch(h));
-      hipError_t ce = hipStreamEndCapture(h->stream, &g);
-      if (rc != Q2D_OK) { if (ce == hipSuccess) (void)hipGraphDestroy(g); return rc; }
-      if (ce != hipSuccess) return fail(Q2D_E_HIP, std::string("graph capture: ") + hipGetErrorString(ce));
-      hipError_t ie = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
-      (void)hipGraphDestroy(g);
-      if (ie != hipSuccess) return fail(Q2D_E_HIP, std::string("graph instantiate: ") + hipGetErrorString(ie));
-      return Q2D_OK;
-    };
+    auto capture = [h](int nst, hipGraphExec_t* out) { return capture_steps(h, nst, out); };
     while (nsteps >= kGraphSteps) {
       if (!h->graph_multi) { int rc = capture(kGraphSteps, &h->graph_multi); if (rc != Q2D_OK) return rc; }
       if (h->timing && h->timed_steps_pending + kGraphSteps > kTimedSteps) {

@@ -87,6 +87,25 @@ def test_step_equals_graph_run(oracle_mod):
     assert np.array_equal(g1.state(0), g2.state(0))
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_graphs_captured_ahead_equal_lazy_capture(precision):
+    """q2d_gpu_capture before the first step does nothing; after it, the graphs it captures replay the same
+    steps as graphs captured inside q2d_gpu_run (state, dt, time and residual bit for bit)."""
+    make = lambda: cases.planar_shock(seed=2, n=48, steps=10 ** 6, precision=precision)
+    a, b = GpuSolver(make()), GpuSolver(make())
+    a.capture()                                    # first step still pending: nothing to capture
+    oa, ob = a.run(3), b.run(3)
+    a.capture()
+    oa, ob = a.run(21), b.run(21)
+    for k in range(len(a.blocks)):
+        assert np.array_equal(a.state(k), b.state(k))
+    for x, y in zip(oa, ob):
+        assert x["iter"] == y["iter"] == 24 and x["dtglobal"] == y["dtglobal"] and x["time"] == y["time"]
+        assert x["residual"] == y["residual"]
+    a.close()
+    b.close()
+
+
 def test_multi_step_graph_stops_at_itermax(oracle_mod):
     """A case whose itermax falls inside a multi-step graph replay stops there, like the oracle."""
     pb = cases.riemann_planar(seed=5, ni=150, nj=10, steps=21)
