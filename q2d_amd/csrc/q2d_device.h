@@ -446,7 +446,7 @@ __device__ __forceinline__ float fsqrt(float a) { return __builtin_amdgcn_sqrtf(
 // rlimiter(x, y) — Lib_Limiters.f90:26-112.  van Leer in its product form: for r = y/x > 0,
 // x (r + |r|) / (1 + |r|) = 2 x y / (x + y); 0 otherwise.  The others as in limiter<float>.
 __device__ __forceinline__ float limiter_mx(int id, float x, float y) {
-#pragma clang fp contract(fast)
+#pragma clang fp contract(on)
   if (id == 1) {                                          // minmod
     if (x * y <= 0.f) return 0.f;
     return __builtin_fabsf(x) > __builtin_fabsf(y) ? y : x;
@@ -461,7 +461,7 @@ __device__ __forceinline__ float limiter_mx(int id, float x, float y) {
 // The same with the limiter fixed at compile time (no branch on the limiter id).
 template <int LIM>
 __device__ __forceinline__ float limiter_mxt(float x, float y) {
-#pragma clang fp contract(fast)
+#pragma clang fp contract(on)
   if constexpr (LIM == 1) {                               // minmod
     if (x * y <= 0.f) return 0.f;
     return __builtin_fabsf(x) > __builtin_fabsf(y) ? y : x;
@@ -480,7 +480,7 @@ __device__ __forceinline__ float limiter_mxt(float x, float y) {
 template <int RIE>
 __device__ __forceinline__ void flux_mxs(const GasF& g, const float L[4], const float R[4], float nx, float ny,
                                          float F[4]) {
-#pragma clang fp contract(fast)
+#pragma clang fp contract(on)
   const float l2 = nx * nx + ny * ny;
   const float len = fsqrt(l2);                            // int_l
   const float rsL = frsq(L[0]), rsR = frsq(R[0]);         // 1 / sqrt(rho)
@@ -533,7 +533,7 @@ __device__ __forceinline__ void flux_mxs(const GasF& g, const float L[4], const 
 template <int RIE>
 __device__ __forceinline__ void flux_mx(const GasF& g, const float L[4], const float R[4], float nx, float ny,
                                         float il, float F[4]) {
-#pragma clang fp contract(fast)
+#pragma clang fp contract(on)
   const float rsL = frsq(L[0]), rsR = frsq(R[0]);         // 1 / sqrt(rho)
   const float sL = L[0] * rsL, sR = R[0] * rsR;           // sqrt(rho)
   const float irL = rsL * rsL, irR = rsR * rsR;           // 1 / rho

@@ -145,7 +145,7 @@ struct DevBC {
   double mach, T, p, rel_fac, cos_a, sin_a;
 };
 
-struct Item { int blk, x0, y0, y1; };
+struct Item { int blk, x0, y0, y1; };   // rows y0..y1; k_stage_mx items with y0 > y1 march down from row y0
 // Per-item results of a step's last stage: residual-norm partials (4) and the fused next-step dt (1).
 constexpr int kPartials = 5;
 
@@ -1968,9 +1968,20 @@ __global__ __launch_bounds__(256, VISC ? Q2D_STAGE_MXV_MIN_WAVES : Q2D_STAGE_MX_
     if (!K.active || irk > C.n_rk) continue;
     // The item body is instantiated per limiter (a uniform switch per item), so the row loop has no
     // limiter branches: straight-line code that the compiler can schedule across the whole row.
-    auto item_body = [&](auto limc, auto riec) {
+    // Direction of the row march (DIR): +1 up from row y0, -1 down from row y1 (Euler items whose Item has
+    // y0 > y1, every other row band of a strip).  A band and the next march towards (or away from) their
+    // common edge at the same time, so the two halo rows each reads of the other are read by both within
+    // a few microseconds and come from the L2.  Every face is evaluated by the same expressions in either
+    // direction (left state from its lower row, slopes as (upper - lower) / (h_lower + h_upper), limiter
+    // arguments in row order), and each cell adds its faces in the same order, so the state does not
+    // depend on the direction.
+    auto item_body = [&](auto limc, auto riec, auto dirc) {
       constexpr int LIM = decltype(limc)::value;
       constexpr int RV = decltype(riec)::value;         // Riemann solver of this item's case
+      constexpr int DIR = decltype(dirc)::value;
+      constexpr bool UP = DIR > 0;
+      static_assert(UP || !VISC, "the laminar instantiation marches up only");
+      const int y0 = min(item.y0, item.y1), y1 = max(item.y0, item.y1);
       const bool last = (irk == C.n_rk);
       const bool irs = IRS && C.irs_beta > 0.0;          // residual to B.irs[0]; k_irs smooths and updates
       const int ni = B.ni, nj = B.nj, pitch = B.pitch;
@@ -2022,15 +2033,18 @@ __global__ __launch_bounds__(256, VISC ? Q2D_STAGE_MXV_MIN_WAVES : Q2D_STAGE_MX_
       f4v MA = {0.f, 0.f, 0.f, 0.f}, MB = {0.f, 0.f, 0.f, 0.f}, MF = {0.f, 0.f, 0.f, 0.f};
       float nodS = 0.f, nodA = 0.f, nodN = 0.f;
 
-      // ---- pipeline warm-up: rows y0-2 .. y0+1 ------------------------------
-      double qA[4], qB[4], qN[4];                        // FP64 rows r, r+1 and r+2 (in flight)
-      float fA[4], fB[4];                                // their FP32 copies (rows r, r+1)
+      // ---- pipeline warm-up: rows ys-2D .. ys+D, ys = y0 - 1 (up) or y1 + 1 (down) ---------------
+      // A, B: the march's rows r and r+DIR; N: row r+2 DIR (in flight).  A face is named by its lower row k:
+      // (k | k+1); the face of iteration r is (r | r+1) up and (r-1 | r) down.
+      const int ys = UP ? y0 - 1 : y1 + 1;               // the warm-up iteration's row
+      double qA[4], qB[4], qN[4];                        // FP64 rows r, r+DIR and r+2 DIR (in flight)
+      float fA[4], fB[4];                                // their FP32 copies (rows r, r+DIR)
       float hyA, hyB, hyN;
       float sAB[4], phiA[4], Gprev[4];
-      f4v gnN = gnp[at(item.y0 - 1)];                    // face normals, one row ahead
+      f4v gnN = gnp[at(UP ? ys : ys - 1)];               // face normals, one face ahead
       {
         double qm[4];
-        const long long o0 = at(item.y0 - 2), o1 = at(item.y0 - 1), o2 = at(item.y0), o3 = at(item.y0 + 1);
+        const long long o0 = at(ys - DIR), o1 = at(ys), o2 = at(ys + DIR), o3 = at(ys + 2 * DIR);
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           qm[v] = Pin[v * pl + o0]; qA[v] = Pin[v * pl + o1]; qB[v] = Pin[v * pl + o2]; qN[v] = Pin[v * pl + o3];
@@ -2042,19 +2056,20 @@ __global__ __launch_bounds__(256, VISC ? Q2D_STAGE_MXV_MIN_WAVES : Q2D_STAGE_MX_
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           fA[v] = (float)qA[v]; fB[v] = (float)qB[v];
-          s0[v] = (fA[v] - (float)qm[v]) * i0;
-          sAB[v] = (fB[v] - fA[v]) * i1;
+          const float fm = (float)qm[v];
+          s0[v] = UP ? (fA[v] - fm) * i0 : (fm - fA[v]) * i0;          // slope between rows ys-DIR and ys
+          sAB[v] = UP ? (fB[v] - fA[v]) * i1 : (fA[v] - fB[v]) * i1;   // slope between rows ys and ys+DIR
           Gprev[v] = 0.f;
         }
 #pragma unroll
-        for (int v = 0; v < 4; ++v) phiA[v] = limiter_mxt<LIM>(sAB[v], s0[v]);
+        for (int v = 0; v < 4; ++v) phiA[v] = UP ? limiter_mxt<LIM>(sAB[v], s0[v]) : limiter_mxt<LIM>(s0[v], sAB[v]);
       }
       if constexpr (VISC) {                              // rows y0-1 (r) and y0 (r+1) of the first j-face
         TA = fA[3] * frcp(fA[0] * Rf);
         TB = fB[3] * frcp(fB[0] * Rf);
-        MA = mmp[at(item.y0 - 1)];
-        MB = mmp[at(item.y0)];
-        nodA = nodp[at(item.y0 - 1)];
+        MA = mmp[at(y0 - 1)];
+        MB = mmp[at(y0)];
+        nodA = nodp[at(y0 - 1)];
       }
       double acc[4] = {0.0, 0.0, 0.0, 0.0};
       double dtnext = 1e300;
@@ -2063,22 +2078,29 @@ __global__ __launch_bounds__(256, VISC ? Q2D_STAGE_MXV_MIN_WAVES : Q2D_STAGE_MX_
       constexpr int kNoRow = 0x7fffffff;
       int fail_if_r = kNoRow, fail_st_r = kNoRow;
 
-      // One j-face (r | r+1) of the march: the FP32 copy and slopes of row r+2 (qC), the limited slopes
-      // of row r+1 and the face flux from rows r, r+1 (fA, fB) — Mod_Fluxes.f90:187-232.  The flux is
-      // evaluated on every lane; lanes that own no face of this row discard it.
+      // One j-face of the march, between rows r and r+DIR: the FP32 copy and slopes of row r+2 DIR (qC), the
+      // limited slopes of row r+DIR and the face flux from rows r, r+DIR (fA, fB) — Mod_Fluxes.f90:187-232.
+      // The flux is evaluated on every lane; lanes that own no face of this row discard it.
       float fC[4], sBC[4], phiB[4], Gcur[4];
       auto jface = [&](int r, const double (&qC)[4], float hyC, const f4v& gn) {
         const float iBC = frcp(hyB + hyC);
 #pragma unroll
-        for (int v = 0; v < 4; ++v) { fC[v] = (float)qC[v]; sBC[v] = (fC[v] - fB[v]) * iBC; }
+        for (int v = 0; v < 4; ++v) {
+          fC[v] = (float)qC[v];
+          sBC[v] = UP ? (fC[v] - fB[v]) * iBC : (fB[v] - fC[v]) * iBC;
+        }
 #pragma unroll
-        for (int v = 0; v < 4; ++v) phiB[v] = limiter_mxt<LIM>(sBC[v], sAB[v]);
-        const bool jdo = own && r >= 1 && r <= nj - 1;
+        for (int v = 0; v < 4; ++v) phiB[v] = UP ? limiter_mxt<LIM>(sBC[v], sAB[v]) : limiter_mxt<LIM>(sAB[v], sBC[v]);
+        const int kf = UP ? r : r - 1;                   // the face's lower row
+        const bool jdo = own && kf >= 1 && kf <= nj - 1;
         float L[4], R[4], G[4];
 #pragma unroll
-        for (int v = 0; v < 4; ++v) { L[v] = fA[v] + phiA[v] * hyA; R[v] = fB[v] - phiB[v] * hyB; }
+        for (int v = 0; v < 4; ++v) {
+          if (UP) { L[v] = fA[v] + phiA[v] * hyA; R[v] = fB[v] - phiB[v] * hyB; }
+          else { L[v] = fB[v] + phiB[v] * hyB; R[v] = fA[v] - phiA[v] * hyA; }
+        }
         const bool bad = jdo && !(L[0] > 0.f && L[3] > 0.f && R[0] > 0.f && R[3] > 0.f);
-        fail_if_r = bad ? min(fail_if_r, r) : fail_if_r;
+        fail_if_r = bad ? min(fail_if_r, kf) : fail_if_r;
         flux_mxs<RV>(gf, L, R, gn.z, gn.w, G);
         if constexpr (VISC) {
           // diffusive flux of j-face (r | r+1) (Mod_Fluxes.f90:207-230): cells (x, r), (x, r+1) and their
@@ -2110,33 +2132,41 @@ __global__ __launch_bounds__(256, VISC ? Q2D_STAGE_MXV_MIN_WAVES : Q2D_STAGE_MX_
         hyA = hyB; hyB = hyC;
       };
 
-      {                                                  // row y0-1: only its j-face (y0-1 | y0)
-        const int r = item.y0 - 1;
+      // rows of the march beyond row r+2 DIR (clamped to the ghost rows), and the lower row of the face one
+      // iteration ahead
+      auto far_row = [&](int r) { return UP ? min(r + 3, rmax) : max(r - 3, -1); };
+      auto next_face = [&](int r) { return UP ? min(r + 1, rmax) : max(r - 2, -1); };
+      {                                                  // row ys: only its j-face (y0-1 | y0) or (y1 | y1+1)
+        const int r = ys;
         double qC[4];
 #pragma unroll
         for (int v = 0; v < 4; ++v) qC[v] = qN[v];
         const float hyC = hyN;
         const f4v gn = gnN;
-        const long long oN = at(min(r + 3, rmax));
+        const long long oN = at(far_row(r));
 #pragma unroll
         for (int v = 0; v < 4; ++v) qN[v] = Pin[v * pl + oN];
         hyN = hyp[oN];
-        gnN = mx_stream_load(&gnp[at(min(r + 1, rmax))]);
+        gnN = mx_stream_load(&gnp[at(next_face(r))]);
         if constexpr (VISC) { MF = mmp[at(min(r + 2, rmax))]; nodN = nodp[at(min(r + 1, rmax))]; }
         jface(r, qC, hyC, gn);
         roll(qC, hyC);
       }
 
-      for (int r0 = item.y0; r0 <= item.y1; r0 += 16) {
+      // chunks of 16 rows in march order; the edge lanes' boundary fluxes of a chunk are staged in LDS
+      for (int c0 = 0; c0 <= y1 - y0; c0 += 16) {
+      const int rlo = UP ? y0 + c0 : max(y0, y1 - c0 - 15);
+      const int rhi = UP ? min(y1, y0 + c0 + 15) : y1 - c0;
       if ((lface || rface) && lane < 32) {
-        // lanes 0-15: face 1, lanes 16-31: face 2, rows r0 .. r0+15 (zero past the item)
-        const int k = lane & 15, fsel = lane >> 4, row = r0 + k;
-        const bool use = (fsel == 0 ? lface : rface) && row <= item.y1;
+        // lanes 0-15: face 1, lanes 16-31: face 2, rows rlo .. rlo+15 (zero past the chunk)
+        const int k = lane & 15, fsel = lane >> 4, row = rlo + k;
+        const bool use = (fsel == 0 ? lface : rface) && row <= rhi;
         f4v q = {0.f, 0.f, 0.f, 0.f};
         if (use) q = mx_load4(&bcdense[B.bcf[fsel] + row - 1]);
         sbc[wslot][fsel][k] = q;
       }
-      for (int r = r0; r <= min(r0 + 15, item.y1); ++r) {
+      for (int n = 0; n <= rhi - rlo; ++n) {
+        const int r = UP ? rlo + n : rhi - n;
         // Loads of this iteration, in the order they are consumed (vector-memory waits are in issue
         // order): first row r's own data (i-face geometry, RK base state, local dt, the fused dt's
         // directions; first used after the j-face flux), then row r+3 and row r+1's j-face geometry for the
@@ -2163,11 +2193,11 @@ __global__ __launch_bounds__(256, VISC ? Q2D_STAGE_MXV_MIN_WAVES : Q2D_STAGE_MX_
         if (!tacc) dtl = dtlp[oc];
         if (FUSE && last && !irs) gd = mx_stream_load(&gdp[oc]);
         {
-          const long long oN = at(min(r + 3, rmax));
+          const long long oN = at(far_row(r));
 #pragma unroll
           for (int v = 0; v < 4; ++v) qN[v] = Pin[v * pl + oN];
           hyN = hyp[oN];
-          gnN = mx_stream_load(&gnp[at(min(r + 1, rmax))]);
+          gnN = mx_stream_load(&gnp[at(next_face(r))]);
           if constexpr (VISC) { MF = mmp[at(min(r + 2, rmax))]; nodN = nodp[at(min(r + 1, rmax))]; }
         }
         jface(r, qC, hyC, gn);
@@ -2215,21 +2245,24 @@ __global__ __launch_bounds__(256, VISC ? Q2D_STAGE_MXV_MIN_WAVES : Q2D_STAGE_MX_
         for (int v = 0; v < 4; ++v) Fl[v] = shup(Fx[v]);
 
         // ---- boundary faces (BC_Fluxes, Mod_BC_Fluxes.f90:49-271, from k_bcflux): each enters the cell's
-        // residual through the face-flux slot of its face (Fl, Fx, Gprev, Gcur), carrying the face sign
+        // residual through the face-flux slot of its face (Fl, Fx, Glo, Ghi), carrying the face sign
         const bool bl = lface && x == 1, br = rface && x == ni;
         const bool bb = r == 1 && B.bcf[2] >= 0, bt = r == nj && B.bcf[3] >= 0;
+        float Glo[4], Ghi[4];                            // j-faces (r-1 | r) and (r | r+1) of row r
+#pragma unroll
+        for (int v = 0; v < 4; ++v) { Glo[v] = UP ? Gprev[v] : Gcur[v]; Ghi[v] = UP ? Gcur[v] : Gprev[v]; }
         if (lface || rface) {                            // edge strip: faces 1 / 2 from the staged rows
-          const f4v q1 = sbc[wslot][0][r - r0], q2 = sbc[wslot][1][r - r0];
+          const f4v q1 = sbc[wslot][0][r - rlo], q2 = sbc[wslot][1][r - rlo];
           Fl[0] = bl ? -q1.x : Fl[0]; Fl[1] = bl ? -q1.y : Fl[1]; Fl[2] = bl ? -q1.z : Fl[2]; Fl[3] = bl ? -q1.w : Fl[3];
           Fx[0] = br ? q2.x : Fx[0]; Fx[1] = br ? q2.y : Fx[1]; Fx[2] = br ? q2.z : Fx[2]; Fx[3] = br ? q2.w : Fx[3];
         }
         if (bb) {                                        // first row: face 3
           const f4v q = mx_load4(&bcdense[B.bcf[2] + min(max(x, 1), ni) - 1]);
-          Gprev[0] = -q.x; Gprev[1] = -q.y; Gprev[2] = -q.z; Gprev[3] = -q.w;
+          Glo[0] = -q.x; Glo[1] = -q.y; Glo[2] = -q.z; Glo[3] = -q.w;
         }
         if (bt) {                                        // last row: face 4 (also the first in a one-row block)
           const f4v q = mx_load4(&bcdense[B.bcf[3] + min(max(x, 1), ni) - 1]);
-          Gcur[0] = q.x; Gcur[1] = q.y; Gcur[2] = q.z; Gcur[3] = q.w;
+          Ghi[0] = q.x; Ghi[1] = q.y; Ghi[2] = q.z; Ghi[3] = q.w;
         }
 
         // ---- residual of the cell (every lane; owners store) ----------
@@ -2239,8 +2272,8 @@ __global__ __launch_bounds__(256, VISC ? Q2D_STAGE_MXV_MIN_WAVES : Q2D_STAGE_MX_
           double rv = 0.0;
           if (x >= 2 || bl) rv = rv - (double)Fl[v];
           if (x <= ni - 1 || br) rv = rv + (double)Fx[v];
-          if (r >= 2 || bb) rv = rv - (double)Gprev[v];
-          if (r <= nj - 1 || bt) rv = rv + (double)Gcur[v];
+          if (r >= 2 || bb) rv = rv - (double)Glo[v];
+          if (r <= nj - 1 || bt) rv = rv + (double)Ghi[v];
           res[v] = rv;
         }
         // geometric sources (Lib_Source_Axisymmetry.f90:40-78, Lib_Source_Areavariation.f90:104-144),
@@ -2362,15 +2395,20 @@ __global__ __launch_bounds__(256, VISC ? Q2D_STAGE_MXV_MIN_WAVES : Q2D_STAGE_MX_
       }
     };
     // RIE < 0: HLLC and HLLE cases share the launch, the solver picked per item (uniform switch)
+    auto by_dir = [&](auto limc, auto riec) {
+      if constexpr (VISC) item_body(limc, riec, std::integral_constant<int, 1>{});
+      else if (item.y0 > item.y1) item_body(limc, riec, std::integral_constant<int, -1>{});
+      else item_body(limc, riec, std::integral_constant<int, 1>{});
+    };
     auto by_limiter = [&](auto riec) {
       switch (C.limiter) {
-        case 1: item_body(std::integral_constant<int, 1>{}, riec); break;
-        case 2: item_body(std::integral_constant<int, 2>{}, riec); break;
-        case 3: item_body(std::integral_constant<int, 3>{}, riec); break;
-        case 4: item_body(std::integral_constant<int, 4>{}, riec); break;
-        case 5: item_body(std::integral_constant<int, 5>{}, riec); break;
-        case 6: item_body(std::integral_constant<int, 6>{}, riec); break;
-        default: item_body(std::integral_constant<int, 0>{}, riec); break;
+        case 1: by_dir(std::integral_constant<int, 1>{}, riec); break;
+        case 2: by_dir(std::integral_constant<int, 2>{}, riec); break;
+        case 3: by_dir(std::integral_constant<int, 3>{}, riec); break;
+        case 4: by_dir(std::integral_constant<int, 4>{}, riec); break;
+        case 5: by_dir(std::integral_constant<int, 5>{}, riec); break;
+        case 6: by_dir(std::integral_constant<int, 6>{}, riec); break;
+        default: by_dir(std::integral_constant<int, 0>{}, riec); break;
       }
     };
     if constexpr (RIE < 0) {
@@ -2976,11 +3014,17 @@ int build_items(q2d_handle* h) {
     for (int c = 0; c < h->ncases; ++c) {
       if (!h->case_live[c] || item_pass(h, c) != pass) continue;
       h->dc[c].item_begin = (int)h->items.size();
+      // k_stage_mx's Euler items march every other row band of a strip down (stored with y0 > y1), so that
+      // neighbouring bands read their shared halo rows at the same time
+      const bool alt = h->precision_mixed && pass == 0;
       for (int b = 0; b < h->nblocks; ++b) {
         if (h->hb[b].case_id != c) continue;
         const int ni = h->hb[b].ni, nj = h->hb[b].nj;
-        for (int y0 = 1; y0 <= nj; y0 += H)
-          for (int x0 = 1; x0 <= ni; x0 += kOwn) h->items.push_back(Item{b, x0, y0, std::min(nj, y0 + H - 1)});
+        for (int y0 = 1, band = 0; y0 <= nj; y0 += H, ++band) {
+          const int y1 = std::min(nj, y0 + H - 1);
+          const bool down = alt && (band & 1);
+          for (int x0 = 1; x0 <= ni; x0 += kOwn) h->items.push_back(down ? Item{b, x0, y1, y0} : Item{b, x0, y0, y1});
+        }
       }
       h->dc[c].item_end = (int)h->items.size();
     }
