@@ -68,11 +68,12 @@ def with_precision(pb, precision):
     return replace(pb, cases=[replace(c, precision=precision) for c in pb.cases])
 
 
-def time_form(pb, steps, warmup, tune=False):
-    """Time `steps` graph-replayed steps of `pb` after `warmup` steps; every case must still be running at
-    the end (status 0, iteration warmup + steps), or the throughput would count frozen cases."""
+def time_form(pb, steps, warmup, tune=False, solver=None):
+    """Time `steps` graph-replayed steps of `pb` after `warmup` steps (on `solver` when given: a solver of
+    `pb` built earlier, which has not stepped); every case must still be running at the end (status 0,
+    iteration warmup + steps), or the throughput would count frozen cases."""
     from q2d_amd import GpuSolver
-    g = GpuSolver(pb)
+    g = solver if solver is not None else GpuSolver(pb)
     ncells = pb.ncells()
     if tune:
         g.autotune()                               # tile shape measured on a scratch solver (cached)
@@ -108,22 +109,35 @@ def time_form(pb, steps, warmup, tune=False):
     return res
 
 
-def accuracy(pb64, pbmx, steps):
-    """Relative L1 / Linf density error of the mixed form against the plain FP64 form after the workload's
-    own step count (its own runs, untimed, from the initial state)."""
+def accuracy_runs(pb64, pbmx, steps):
+    """The two untimed runs of the accuracy leg (FP64 then mixed, `steps` steps each from the initial
+    state), their solvers built before either runs; accuracy_result() reads them."""
     from q2d_amd import GpuSolver
-    rho = []
-    for pb in (pb64, pbmx):
-        g = GpuSolver(pb)
+    solvers = [GpuSolver(pb) for pb in (pb64, pbmx)]
+    for g in solvers[::-1]:                            # mixed first, so that FP64 work precedes the FP64 timed leg
         out = g.run(steps)
         assert all(o["status"] == 0 and o["iter"] == steps for o in out), "accuracy run stopped early"
-        rho.append([g.state(b)[2:-2, 2:-2, 0].copy() for b in range(len(pb.blocks))])
+    return solvers, steps
+
+
+def accuracy_result(runs):
+    """Relative L1 / Linf density error of the mixed run against the FP64 run of accuracy_runs()."""
+    solvers, steps = runs
+    rho = []
+    for g in solvers:
+        rho.append([g.state(b)[2:-2, 2:-2, 0].copy() for b in range(len(g.blocks))])
         g.close()
     a, b = rho[1], rho[0]
     num = sum(np.abs(x - y).sum() for x, y in zip(a, b))
     den = sum(np.abs(y).sum() for y in b)
     linf = max(np.abs(x - y).max() for x, y in zip(a, b)) / max(np.abs(y).max() for y in b)
     return {"rel_L1_rho": num / den, "rel_Linf_rho": linf, "steps": steps, "tolerance_rel_L1": 1e-5}
+
+
+def accuracy(pb64, pbmx, steps):
+    """Relative L1 / Linf density error of the mixed form against the plain FP64 form after the workload's
+    own step count (its own runs, untimed, from the initial state)."""
+    return accuracy_result(accuracy_runs(pb64, pbmx, steps))
 
 
 def physical_cores():
@@ -384,16 +398,20 @@ def main():
         r = time_form(pbmx if args.form == "mixed" else pb64, args.steps, args.warmup)
         log(f"bench: {args.form} {r['cell_updates_per_s'] / 1e9:.3f} G cell-updates/s, stage kernel {r['stage_kernel_ms']:.4f} ms")
         return
-    # The accuracy leg first (C2's own 500 steps in both forms, whatever --steps is): the timed legs then
-    # start on a device at the clocks a running time loop holds; a device fresh from idle runs the first
-    # ~100 steps of this workload ~5 % slower (stage kernel 0.127-0.129 ms after 5-50 steps, 0.120 ms
-    # after 200), which a 20-step timed run would otherwise mostly measure.
-    acc = accuracy(pb64, pbmx, 500)
-    log(f"bench: mixed vs fp64 after 500 steps: rel L1(rho) {acc['rel_L1_rho']:.2e}, rel Linf {acc['rel_Linf_rho']:.2e}")
-    plain = time_form(pb64, args.steps, args.warmup)
+    # Every solver is built first and the accuracy leg's runs (C2's own 500 steps in both forms, whatever
+    # --steps is) go next, so that the timed legs follow GPU work instead of host set-up: after the device
+    # idles its clocks drop, and the first ~50 steps of this workload then run up to ~10 % slower (stage
+    # kernel 0.133 ms right after 0.5 s idle, 0.120 ms once ramped), which a 20-step timed run would
+    # otherwise mostly measure.  The accuracy leg's states are read after the timed legs.
+    from q2d_amd import GpuSolver
+    g64, gmx = GpuSolver(pb64), GpuSolver(pbmx)
+    acc_runs = accuracy_runs(pb64, pbmx, 500)
+    plain = time_form(pb64, args.steps, args.warmup, solver=g64)
     log(f"bench: plain fp64  {plain['cell_updates_per_s'] / 1e9:.3f} G cell-updates/s")
-    mixed = time_form(pbmx, args.steps, args.warmup)
+    mixed = time_form(pbmx, args.steps, args.warmup, solver=gmx)
     log(f"bench: mixed       {mixed['cell_updates_per_s'] / 1e9:.3f} G cell-updates/s")
+    acc = accuracy_result(acc_runs)
+    log(f"bench: mixed vs fp64 after 500 steps: rel L1(rho) {acc['rel_L1_rho']:.2e}, rel Linf {acc['rel_Linf_rho']:.2e}")
 
     n_rk = plain["n_rk"]
     bytes_update = n_rk * BYTES_PER_CELL_STAGE + BYTES_DT_PER_CELL

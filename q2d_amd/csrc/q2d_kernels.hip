@@ -3162,10 +3162,12 @@ int launch_stage(q2d_handle* h, int irk, const StepScratch& scr) {
   return Q2D_OK;
 }
 
-// Clear the launch-timing slots and step counter (a kernel, so it is ordered on the stream).
-int reset_timing(q2d_handle* h) {
-  hipLaunchKernelGGL(k_tstamp_init, dim3((kTimedSlots * kStampLanes + 255) / 256), dim3(256), 0, h->stream,
-                     h->d_tstamp, kTimedSlots * kStampLanes);
+// Clear the launch-timing slots of the first `steps` steps of the ring (all by default) and the step
+// counter (a kernel, so it is ordered on the stream).
+int reset_timing(q2d_handle* h, int steps = kTimedSteps) {
+  const int nsub = std::min(steps, kTimedSteps) * kLaunchesPerStep * kStampLanes;
+  hipLaunchKernelGGL(k_tstamp_init, dim3(std::max(1, (nsub + 255) / 256)), dim3(256), 0, h->stream,
+                     h->d_tstamp, nsub);
   HIPCHK(hipGetLastError());
   h->timed_steps_pending = 0;
   return Q2D_OK;
@@ -3175,7 +3177,9 @@ int reset_timing(q2d_handle* h) {
 int collect_timing(q2d_handle* h) {
   const int nsteps = h->timed_steps_pending;
   if (nsteps == 0) return Q2D_OK;
-  std::vector<unsigned long long> ts(2 + 2 * (size_t)kTimedSlots * kStampLanes);
+  // only the slots of the pending steps were written since the last reset: copy and clear those
+  const int nst = std::min(nsteps, kTimedSteps);
+  std::vector<unsigned long long> ts(2 + 2 * (size_t)nst * kLaunchesPerStep * kStampLanes);
   HIPCHK(hipMemcpyAsync(ts.data(), h->d_tstamp, sizeof(unsigned long long) * ts.size(), hipMemcpyDeviceToHost, h->stream));
   HIPCHK(hipStreamSynchronize(h->stream));
   for (int k = 0; k < std::min(nsteps, kTimedSteps) * kLaunchesPerStep; ++k) {
@@ -3188,7 +3192,7 @@ int collect_timing(q2d_handle* h) {
     h->timed_ms += (double)(t1 - t0) / h->wallclock_khz;
     h->timed_launches += 1;
   }
-  return reset_timing(h);
+  return reset_timing(h, nst);
 }
 
 // Steps of this handle evaluate the next step's dt in their last stage; otherwise every step runs the dt
