@@ -68,20 +68,34 @@ def with_precision(pb, precision):
     return replace(pb, cases=[replace(c, precision=precision) for c in pb.cases])
 
 
-def time_form(pb, steps, warmup, tune=False, solver=None):
-    """Time `steps` graph-replayed steps of `pb` after `warmup` steps (on `solver` when given: a solver of
-    `pb` built earlier, which has not stepped); every case must still be running at the end (status 0,
-    iteration warmup + steps), or the throughput would count frozen cases."""
+def prepare_form(pb, warmup):
+    """A solver of `pb` with kernel timing on, its first warm-up step run (the eager one, with the dt pass)
+    and its step graphs captured, so that what remains of the warm-up and the timed steps replay graphs
+    with no host-side capture between them (time_form(..., prepared=...) continues from here)."""
     from q2d_amd import GpuSolver
-    g = solver if solver is not None else GpuSolver(pb)
+    g = GpuSolver(pb)
+    g.set_timing(True)
+    if warmup > 0:
+        g.run(1)
+        g.capture()
+    return g, min(warmup, 1)
+
+
+def time_form(pb, steps, warmup, tune=False, prepared=None):
+    """Time `steps` graph-replayed steps of `pb` after `warmup` steps (on the solver of `prepared`, the
+    pair prepare_form() returns, when given: its first warm-up steps ran there); every case must still be
+    running at the end (status 0, iteration warmup + steps), or the throughput would count frozen cases."""
+    from q2d_amd import GpuSolver
+    g, done = prepared if prepared is not None else (GpuSolver(pb), 0)
     ncells = pb.ncells()
     if tune:
         g.autotune()                               # tile shape measured on a scratch solver (cached)
     # Stage-kernel durations are measured on the timed steps themselves: with timing on, every stage
     # launch stamps its first workgroup start and last wavefront end on the device wall clock (two
     # atomics per workgroup); the graph replays are otherwise unchanged.
-    g.set_timing(True)
-    g.run(warmup)                                  # warm-up steps (synchronised)
+    if prepared is None:
+        g.set_timing(True)
+    g.run(warmup - done)                           # (the rest of) the warm-up steps, synchronised
     g.set_timing(True)                             # clear the warm-up launches from the statistics
     g.capture()                                    # the step graphs the timed run replays, captured untimed
     t0 = time.perf_counter()
@@ -398,17 +412,18 @@ def main():
         r = time_form(pbmx if args.form == "mixed" else pb64, args.steps, args.warmup)
         log(f"bench: {args.form} {r['cell_updates_per_s'] / 1e9:.3f} G cell-updates/s, stage kernel {r['stage_kernel_ms']:.4f} ms")
         return
-    # Every solver is built first and the accuracy leg's runs (C2's own 500 steps in both forms, whatever
-    # --steps is) go next, so that the timed legs follow GPU work instead of host set-up: after the device
-    # idles its clocks drop, and the first ~50 steps of this workload then run up to ~10 % slower (stage
-    # kernel 0.133 ms right after 0.5 s idle, 0.120 ms once ramped), which a 20-step timed run would
-    # otherwise mostly measure.  The accuracy leg's states are read after the timed legs.
-    from q2d_amd import GpuSolver
-    g64, gmx = GpuSolver(pb64), GpuSolver(pbmx)
+    # The timed solvers are built, run their first warm-up step and capture their step graphs first, and
+    # the accuracy leg's runs (C2's own 500 steps in both forms, whatever --steps is) go next, so that the
+    # timed legs follow GPU work with no host set-up or graph capture in between: after the device idles
+    # its clocks drop, and the next ~25-50 steps of this workload run up to 10-14 % slower (mixed stage
+    # kernel 0.133 ms right after 0.5 s idle, 0.120 ms once ramped; FP64 0.313 against 0.276 ms), which a
+    # 20-step timed run would otherwise mostly measure.  The accuracy leg's states are read after the
+    # timed legs.
+    p64, pmx = prepare_form(pb64, args.warmup), prepare_form(pbmx, args.warmup)
     acc_runs = accuracy_runs(pb64, pbmx, 500)
-    plain = time_form(pb64, args.steps, args.warmup, solver=g64)
+    plain = time_form(pb64, args.steps, args.warmup, prepared=p64)
     log(f"bench: plain fp64  {plain['cell_updates_per_s'] / 1e9:.3f} G cell-updates/s")
-    mixed = time_form(pbmx, args.steps, args.warmup, solver=gmx)
+    mixed = time_form(pbmx, args.steps, args.warmup, prepared=pmx)
     log(f"bench: mixed       {mixed['cell_updates_per_s'] / 1e9:.3f} G cell-updates/s")
     acc = accuracy_result(acc_runs)
     log(f"bench: mixed vs fp64 after 500 steps: rel L1(rho) {acc['rel_L1_rho']:.2e}, rel Linf {acc['rel_Linf_rho']:.2e}")
