@@ -177,6 +177,9 @@ template <typename T> __device__ __forceinline__ T limiter(int id, T x, T y) {
     }
     case 3: {                                   // vanalbada :61-70
       T ru = qdiv<T>(y, x + tiny);
+      // FP32 (mixed form): |r| held below 1e18 so that r^2 stays finite; where x vanishes r ~ y / 1e-30, and
+      // FP64 represents r^2 while FP32 would overflow to inf / inf.  phi(1e18) is 1 to FP32 precision.
+      if constexpr (sizeof(T) == 4) ru = __builtin_fminf(__builtin_fmaxf(ru, -1e18f), 1e18f);
       T phi = qdiv<T>(ru * ru + ru, T(1) + ru * ru);
       return phi * x;
     }
