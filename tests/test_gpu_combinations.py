@@ -67,7 +67,7 @@ def random_batch(seed, precision, ncases=6):
 
 # seeds whose draws all run the 12 steps (seed 1 draws a LIMO3 / HLLC / SSP-RK2 shock at CFL 0.5 that the
 # oracle stops with NaN or p < 0; the failure path has its own test)
-SEEDS = [0, 2, 3, 4, 5, 6, 7, 8]
+SEEDS = [s for s in range(24) if s != 1]
 
 
 @pytest.mark.parametrize("seed", SEEDS)
@@ -92,7 +92,7 @@ def test_random_batch_fp64_bit_exact(oracle_mod, seed):
     g.close()
 
 
-@pytest.mark.parametrize("seed", SEEDS[:4])
+@pytest.mark.parametrize("seed", SEEDS)
 def test_random_batch_mixed_within_tolerance(oracle_mod, seed):
     pb, desc = random_batch(seed, "mixed")
     print(f"combinations (mixed) seed {seed}: {desc}", flush=True)
