@@ -161,8 +161,8 @@ def finish(problem, geoms):
 
 # ---------------------------------------------------------------------------
 def riemann_planar(seed=0, ni=512, nj=8, steps=1000, limiter="minmod", riemann="HLLC", n_rk=2, cfl=0.5,
-                   precision="fp64", bc_y=400):
-    """Two-state Riemann problem on x in [0,1], diaphragm at x = 0.5.
+                   precision="fp64", bc_y=400, length=1.0):
+    """Two-state Riemann problem on x in [0,length], diaphragm at x = length/2.
 
     rho_L, p_L, rho_R, p_R ~ U[0.1, 1] drawn in that order from default_rng(seed);
     u = v = 0, gamma = 1.4 (R = 1, cp = 3.5); square cells; BC 400 on all faces
@@ -170,11 +170,11 @@ def riemann_planar(seed=0, ni=512, nj=8, steps=1000, limiter="minmod", riemann="
     """
     rng = np.random.default_rng(seed)
     rl, pl, rr, pr = rng.uniform(0.1, 1.0, 4)
-    node = cartesian_nodes(ni, nj, 0.0, 1.0, 0.0, nj / ni)
+    node = cartesian_nodes(ni, nj, 0.0, length, 0.0, length * nj / ni)
     geo = BlockGeometry(node)
     P = np.zeros((nj + 4, ni + 4, 4))
     xc = 0.5 * (node[0, 1:, 0] + node[0, :-1, 0])
-    left = xc < 0.5
+    left = xc < 0.5 * length
     P[2:-2, 2:-2, 0] = np.where(left, rl, rr)[None, :]
     P[2:-2, 2:-2, 3] = np.where(left, pl, pr)[None, :]
     fill_initial_ghosts(P)

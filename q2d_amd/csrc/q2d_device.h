@@ -14,10 +14,13 @@ namespace q2d {
 
 // Gas closure of one case: R, cp, cv, gamma of the calorically perfect closed forms, or, with tab set
 // (FP64 only), FLINT's native tabulated mode: planes h, cp, mu, k of ntab nodes each, node k at T = k.
-template <typename T> struct Gas {
+// DV selects the FP64 division of the functions that take the closure (gdiv): 0 the compiler's IEEE
+// division, 1 ddiv_fast with the binary exponents of every numerator and quotient folded into ex[0..1].
+template <typename T, int DV = 0> struct Gas {
   T R, cp, cv, gam;
   const double* tab = nullptr;
   int ntab = 0;
+  int* ex = nullptr;
 };
 
 // ---- arithmetic primitives ----------------------------------------------
@@ -26,6 +29,26 @@ template <typename T> struct Gas {
 __device__ __forceinline__ double ddiv_rn(double a, double b) { return a / b; }
 __device__ __forceinline__ double dsqrt_rn(double x) { return __builtin_sqrt(x); }
 
+// a / b by the hardware reciprocal refined by two Newton steps, the quotient and one remainder
+// correction (Markstein), then v_div_fixup_f64 for zero, infinite and NaN operands (signed zeros and
+// the numerator's NaN included).  This is the compiler's IEEE f64 division without its two
+// v_div_scale_f64 steps and with a plain fma for v_div_fmas_f64; those steps rescale operands near
+// the ends of the exponent range and are the identity when a = 0 or 2^-969 <= |a|, 2^-1021 <= |b| <=
+// 2^1021 (or b = 0) and 2^-1022 <= |a / b| < 2^1023, where both give the IEEE quotient bit for bit.
+// Without the scaling steps the sequence does not pass a flag through VCC (one register, which
+// serialises the compiler's divisions) and divisions by one denominator share the refined reciprocal.
+__device__ __forceinline__ double ddiv_fast(double a, double b) {
+  double r = __builtin_amdgcn_rcp(b);
+  double e = __builtin_fma(-b, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  e = __builtin_fma(-b, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  double q = a * r;
+  const double rem = __builtin_fma(-b, q, a);
+  q = __builtin_fma(rem, r, q);
+  return __builtin_amdgcn_div_fixup(q, b, a);
+}
+
 template <typename T> __device__ __forceinline__ T qdiv(T a, T b);
 template <> __device__ __forceinline__ double qdiv<double>(double a, double b) { return ddiv_rn(a, b); }
 template <> __device__ __forceinline__ float qdiv<float>(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
@@ -33,6 +56,24 @@ template <> __device__ __forceinline__ float qdiv<float>(float a, float b) { ret
 template <typename T> __device__ __forceinline__ T qsqrt(T a);
 template <> __device__ __forceinline__ double qsqrt<double>(double a) { return dsqrt_rn(a); }
 template <> __device__ __forceinline__ float qsqrt<float>(float a) { return __builtin_amdgcn_sqrtf(a); }
+
+// Division of the functions that take a gas closure: FP32 as qdiv; FP64 with DV = 0 the IEEE division,
+// with DV = 1 ddiv_fast, the exponents (frexp: x = m 2^e, 1/2 <= |m| < 1; 0 for x = 0) of the
+// numerator and of the quotient folded into the running minima g.ex[0] and g.ex[1], which the caller
+// checks: |a| >= 2^-969 while g.ex[0] >= -968, |q| >= 2^-1020 while g.ex[1] >= -1019.
+template <typename T, int DV> __device__ __forceinline__ T gdiv(const Gas<T, DV>& g, T a, T b) {
+  if constexpr (sizeof(T) == 4) {
+    return qdiv<T>(a, b);
+  } else if constexpr (DV == 0) {
+    return ddiv_rn(a, b);
+  } else {
+    const double q = ddiv_fast(a, b);
+    const int ea = __builtin_amdgcn_frexp_exp(a), eq = __builtin_amdgcn_frexp_exp(q);
+    g.ex[0] = ea < g.ex[0] ? ea : g.ex[0];
+    g.ex[1] = eq < g.ex[1] ? eq : g.ex[1];
+    return q;
+  }
+}
 
 template <typename T> __device__ __forceinline__ T qabs(T a) { return __builtin_fabs(a); }
 template <> __device__ __forceinline__ float qabs<float>(float a) { return __builtin_fabsf(a); }
@@ -63,21 +104,26 @@ __device__ __forceinline__ double tab_at(const double* tab, int ntab, int q, dou
   return tab_interp(tab, ntab, q, Ti, T - (double)Ti);
 }
 // cp / (cp - R) at T (f_gamma)
-__device__ __forceinline__ double tab_gamma(const double* tab, int ntab, double R, double T) {
-  const double cp = tab_at(tab, ntab, kTabCp, T);
-  return ddiv_rn(cp, cp - R);
+template <int DV>
+__device__ __forceinline__ double tab_gamma(const Gas<double, DV>& g, double T) {
+  const double cp = tab_at(g.tab, g.ntab, kTabCp, T);
+  return gdiv(g, cp, cp - g.R);
 }
 // cons2prim's temperature: the exact inverse of the interpolated e(T) = h(T) - R T, by Newton steps from
 // the estimate T0 (the stage input's p / (rho R), Lib_Newstate.f90:119-123), each with the slope of its
 // interval (a step that lands in its own interval is the root); at most 32 steps.
-__device__ __forceinline__ double tab_t_of_e(const double* tab, int ntab, double R, double e, double T0) {
+template <int DV>
+__device__ __forceinline__ double tab_t_of_e(const Gas<double, DV>& g, double e, double T0) {
+  const double* tab = g.tab;
+  const int ntab = g.ntab;
+  const double R = g.R;
   double T = T0;
   for (int it = 0; it < 32; ++it) {
     const int Ti = tab_node(ntab, T);
     const double h0 = tab[Ti], h1 = tab[Ti + 1];
     const double e0 = h0 - R * (double)Ti;
     const double sl = (h1 - h0) - R;
-    const double Tn = (double)Ti + ddiv_rn(e - e0, sl);
+    const double Tn = (double)Ti + gdiv(g, e - e0, sl);
     if (Tn >= (double)Ti && Tn <= (double)(Ti + 1)) return Tn;
     if ((Ti == 0 && Tn < 0.0) || (Ti == ntab - 2 && Tn > (double)(Ti + 1))) return Tn;
     T = Tn;
@@ -87,21 +133,21 @@ __device__ __forceinline__ double tab_t_of_e(const double* tab, int ntab, double
 
 // ---- gas closure (single species) ------------------------------------------
 // a = sqrt(gamma*p/rho)  (f_ss); in table mode gamma at T = p / (rho R)
-template <typename T> __device__ __forceinline__ T sound(const Gas<T>& g, T p, T rho) {
+template <typename T, int DV> __device__ __forceinline__ T sound(const Gas<T, DV>& g, T p, T rho) {
   if constexpr (sizeof(T) == 8) {
-    if (g.tab) return dsqrt_rn(ddiv_rn(tab_gamma(g.tab, g.ntab, g.R, ddiv_rn(p, rho * g.R)) * p, rho));
+    if (g.tab) return dsqrt_rn(gdiv(g, tab_gamma(g, gdiv(g, p, rho * g.R)) * p, rho));
   }
-  return qsqrt<T>(qdiv<T>(g.gam * p, rho));
+  return qsqrt<T>(gdiv(g, g.gam * p, rho));
 }
 // specific enthalpy h = cp T (closed form) or h(T) (table), T = p / (rho R)
-template <typename T> __device__ __forceinline__ T enthalpy(const Gas<T>& g, T p, T rho) {
+template <typename T, int DV> __device__ __forceinline__ T enthalpy(const Gas<T, DV>& g, T p, T rho) {
   if constexpr (sizeof(T) == 8) {
-    if (g.tab) return tab_at(g.tab, g.ntab, kTabH, ddiv_rn(p, rho * g.R));
+    if (g.tab) return tab_at(g.tab, g.ntab, kTabH, gdiv(g, p, rho * g.R));
   }
-  return g.cp * qdiv<T>(p, rho * g.R);
+  return g.cp * gdiv(g, p, rho * g.R);
 }
 // specific internal energy at temperature T: cv T (closed form) or h(T) - R T (table)
-__device__ __forceinline__ double int_energy(const Gas<double>& g, double T) {
+template <int DV> __device__ __forceinline__ double int_energy(const Gas<double, DV>& g, double T) {
   if (g.tab) return tab_at(g.tab, g.ntab, kTabH, T) - g.R * T;
   return g.cv * T;
 }
@@ -160,7 +206,8 @@ __device__ __forceinline__ double exact_pow(double x, double y) {
 }
 
 // ---- limiters — Lib_Limiters.f90:26-112 ----------------------------------
-template <typename T> __device__ __forceinline__ T limiter(int id, T x, T y) {
+// g: the closure whose division the limiter uses (gdiv)
+template <typename T, int DV> __device__ __forceinline__ T limiter(const Gas<T, DV>& g, int id, T x, T y) {
   const T tiny = T(1e-30);
   switch (id) {
     case 1: {                                   // minmod :36-46
@@ -171,33 +218,33 @@ template <typename T> __device__ __forceinline__ T limiter(int id, T x, T y) {
       return r;
     }
     case 2: {                                   // vanleer :49-58
-      T ru = qdiv<T>(y, x + tiny);
-      T phi = qdiv<T>(ru + qabs<T>(ru), T(1) + qabs<T>(ru));
+      T ru = gdiv(g, y, x + tiny);
+      T phi = gdiv(g, ru + qabs<T>(ru), T(1) + qabs<T>(ru));
       return phi * x;
     }
     case 3: {                                   // vanalbada :61-70
-      T ru = qdiv<T>(y, x + tiny);
+      T ru = gdiv(g, y, x + tiny);
       // FP32 (mixed form): |r| held below 1e18 so that r^2 stays finite; where x vanishes r ~ y / 1e-30, and
       // FP64 represents r^2 while FP32 would overflow to inf / inf.  phi(1e18) is 1 to FP32 precision.
       if constexpr (sizeof(T) == 4) ru = __builtin_fminf(__builtin_fmaxf(ru, -1e18f), 1e18f);
-      T phi = qdiv<T>(ru * ru + ru, T(1) + ru * ru);
+      T phi = gdiv(g, ru * ru + ru, T(1) + ru * ru);
       return phi * x;
     }
     case 4: {                                   // MC :73-82
-      T ru = qdiv<T>(y, x + tiny);
+      T ru = gdiv(g, y, x + tiny);
       T m = min3<T>(T(2) * ru, T(0.5) * (T(1) + ru), T(2));
       T phi = T(0); if (m > phi) phi = m;
       return phi * x;
     }
     case 5: {                                   // superbee :85-94
-      T ru = qdiv<T>(y, x + tiny);
+      T ru = gdiv(g, y, x + tiny);
       T a = T(2) * ru; if (T(1) < a) a = T(1);
       T b = ru; if (T(2) < b) b = T(2);
       return max3<T>(T(0), a, b) * x;
     }
     case 6: {                                   // LIMO3 :98-110
-      T ru = qdiv<T>(y, x + tiny);
-      T t = qdiv<T>(T(2) + ru, T(3));
+      T ru = gdiv(g, y, x + tiny);
+      T t = gdiv(g, T(2) + ru, T(3));
       T inner = min3<T>(T(2) * ru, t, T(1.6));
       T mid = max3<T>(T(-0.5) * ru, T(0), inner);
       T outer = t; if (mid < outer) outer = mid;
@@ -208,20 +255,24 @@ template <typename T> __device__ __forceinline__ T limiter(int id, T x, T y) {
   }
 }
 
+template <typename T> __device__ __forceinline__ T limiter(int id, T x, T y) {
+  return limiter(Gas<T>{}, id, x, y);
+}
+
 // ---- Riemann solvers — Lib_Riemann_HLL.f90 --------------------------------
 // roe_averages :229-317, one species: h = cp*T, cp(T) = cp, or the table arithmetic.
-template <typename T>
-__device__ __forceinline__ void roe_averages(const Gas<T>& g, T dltot, T drtot, T ul, T vl, T ur, T vr, T pl, T pr,
+template <typename T, int DV>
+__device__ __forceinline__ void roe_averages(const Gas<T, DV>& g, T dltot, T drtot, T ul, T vl, T ur, T vr, T pl, T pr,
                                              T& h0l_o, T& h0r_o, T& u_roe, T& v_roe, T& a_roe) {
   const T dl = dltot, dr = drtot;
   T Rl = g.R, Rr = g.R;
-  T inv_dltot = qdiv<T>(T(1), dltot);
-  T inv_drtot = qdiv<T>(T(1), drtot);
-  T Tl = qdiv<T>(pl * inv_dltot, Rl);
-  T Tr = qdiv<T>(pr * inv_drtot, Rr);
+  T inv_dltot = gdiv(g, T(1), dltot);
+  T inv_drtot = gdiv(g, T(1), drtot);
+  T Tl = gdiv(g, pl * inv_dltot, Rl);
+  T Tr = gdiv(g, pr * inv_drtot, Rr);
   T srL = qsqrt<T>(dltot);
   T srR = qsqrt<T>(drtot);
-  T inv_sr = qdiv<T>(T(1), srL + srR);
+  T inv_sr = gdiv(g, T(1), srL + srR);
   u_roe = (srR * ur + srL * ul) * inv_sr;
   v_roe = (srR * vr + srL * vl) * inv_sr;
   T R_roe = (srR * Rr + srL * Rl) * inv_sr;
@@ -253,15 +304,15 @@ __device__ __forceinline__ void roe_averages(const Gas<T>& g, T dltot, T drtot, 
   h0l = h0l + T(0.5) * (ul * ul + vl * vl);
   h0r = h0r + T(0.5) * (ur * ur + vr * vr);
   T h0_roe = (srR * h0r + srL * h0l) * inv_sr;
-  T gam_roe = T(1) + qdiv<T>(R_roe, cv_roe);
+  T gam_roe = T(1) + gdiv(g, R_roe, cv_roe);
   T vel2 = u_roe * u_roe + v_roe * v_roe;
   a_roe = qsqrt<T>((gam_roe - T(1)) * (h0_roe - T(0.5) * vel2 + cv_roe * T_roe - sum_ei));
   h0l_o = h0l; h0r_o = h0r;
 }
 
 // riemann_HLLC :34-123, riemann_HLLE :126-169 and riemann_HLLEM :355-432.  F = (F_r, F_u, F_v, F_E).
-template <typename T, int RIE>
-__device__ __forceinline__ void riemann(const Gas<T>& g, T ul, T vl, T pl, T al, T dltot,
+template <typename T, int RIE, int DV>
+__device__ __forceinline__ void riemann(const Gas<T, DV>& g, T ul, T vl, T pl, T al, T dltot,
                                         T ur, T vr, T pr, T ar, T drtot, T nx, T ny, T F[4], T beta = T(1)) {
   T unl = ul * nx + vl * ny;
   T unr = ur * nx + vr * ny;
@@ -285,17 +336,17 @@ __device__ __forceinline__ void riemann(const Gas<T>& g, T ul, T vl, T pl, T al,
   if (RIE == 3) {                               // HLLE++ :435-530: Roe waves, speeds blended HLLE <-> Roe by beta
     const T srL = qsqrt<T>(dltot), srR = qsqrt<T>(drtot);
     const T rho_roe = srL * srR;
-    const T h0_roe = (srR * h0r + srL * h0l) * qdiv<T>(T(1), srL + srR);
+    const T h0_roe = (srR * h0r + srL * h0l) * gdiv(g, T(1), srL + srR);
     const T Sm = min3<T>(T(0), unl - al, un_roe - a_roe);
     const T Sp = max3<T>(T(0), unr + ar, un_roe + a_roe);
-    const T inv_denom = qdiv<T>(T(1), Sp - Sm);
+    const T inv_denom = gdiv(g, T(1), Sp - Sm);
     const T l1h = fabs((Sp + Sm) * un_roe * inv_denom - T(2) * Sp * Sm * inv_denom);
     const T l2h = fabs((Sp + Sm) * (un_roe - a_roe) * inv_denom - T(2) * Sp * Sm * inv_denom);
     const T l3h = fabs((Sp + Sm) * (un_roe + a_roe) * inv_denom - T(2) * Sp * Sm * inv_denom);
     const T eps = T(0.3) * (fabs(un_roe) + a_roe);
     auto hh = [&](T v) {                        // hartenHyman :320-331
       const T av = fabs(v);
-      return av <= eps ? qdiv<T>(v * v + eps * eps, T(2) * eps) : av;
+      return av <= eps ? gdiv(g, v * v + eps * eps, T(2) * eps) : av;
     };
     const T l1r = fabs(un_roe), l2r = hh(un_roe - a_roe), l3r = hh(un_roe + a_roe);
     const T l1 = beta * l1r + (T(1) - beta) * l1h;
@@ -313,10 +364,10 @@ __device__ __forceinline__ void riemann(const Gas<T>& g, T ul, T vl, T pl, T al,
     phys(pr, drtot, ur, vr, FR);
     const T drho = drtot - dltot, dp = pr - pl, du = ur - ul, dv = vr - vl, dun = unr - unl;
     const T tx = -ny, ty = nx;
-    const T alpha_m = qdiv<T>(dp - rho_roe * a_roe * dun, T(2) * a_roe * a_roe);
-    const T alpha_c = drho - qdiv<T>(dp, a_roe * a_roe);
+    const T alpha_m = gdiv(g, dp - rho_roe * a_roe * dun, T(2) * a_roe * a_roe);
+    const T alpha_c = drho - gdiv(g, dp, a_roe * a_roe);
     const T alpha_s = rho_roe * (du * tx + dv * ty);
-    const T alpha_p = qdiv<T>(dp + rho_roe * a_roe * dun, T(2) * a_roe * a_roe);
+    const T alpha_p = gdiv(g, dp + rho_roe * a_roe * dun, T(2) * a_roe * a_roe);
     F[0] = T(0.5) * (FL[0] + FR[0]) - T(0.5) * (alpha_c * l1 + alpha_m * l2 + alpha_p * l3);
     F[1] = T(0.5) * (FL[1] + FR[1]) - T(0.5) * (alpha_c * l1 * u_roe + alpha_s * l1 * tx + alpha_m * l2 * (u_roe - a_roe * nx) +
                                                 alpha_p * l3 * (u_roe + a_roe * nx));
@@ -342,9 +393,9 @@ __device__ __forceinline__ void riemann(const Gas<T>& g, T ul, T vl, T pl, T al,
     T FL[4], FR[4];
     phys(pl, dltot, ul, vl, FL);
     phys(pr, drtot, ur, vr, FR);
-    T e0l = h0l - qdiv<T>(pl, dltot);
-    T e0r = h0r - qdiv<T>(pr, drtot);
-    T inv_denom = qdiv<T>(T(1), S4 - S1);
+    T e0l = h0l - gdiv(g, pl, dltot);
+    T e0r = h0r - gdiv(g, pr, drtot);
+    T inv_denom = gdiv(g, T(1), S4 - S1);
     F[0] = (S4 * FL[0] - S1 * FR[0] + S1 * S4 * (drtot - dltot)) * inv_denom;
     F[1] = (S4 * FL[1] - S1 * FR[1] + S1 * S4 * (drtot * ur - dltot * ul)) * inv_denom;
     F[2] = (S4 * FL[2] - S1 * FR[2] + S1 * S4 * (drtot * vr - dltot * vl)) * inv_denom;
@@ -353,9 +404,9 @@ __device__ __forceinline__ void riemann(const Gas<T>& g, T ul, T vl, T pl, T al,
       const T rho_roe = qsqrt<T>(dltot) * qsqrt<T>(drtot);
       const T drho = drtot - dltot, dp = pr - pl, du = ur - ul, dv = vr - vl;
       const T tx = -ny, ty = nx;
-      const T alpha_c = drho - qdiv<T>(dp, a_roe * a_roe);
+      const T alpha_c = drho - gdiv(g, dp, a_roe * a_roe);
       const T alpha_s = rho_roe * (du * tx + dv * ty);
-      const T delta = qdiv<T>(a_roe, a_roe + fabs(T(0.5) * (S1 + S4)));
+      const T delta = gdiv(g, a_roe, a_roe + fabs(T(0.5) * (S1 + S4)));
       const T C = (S1 * S4) * inv_denom;
       F[0] = F[0] - C * delta * alpha_c;
       F[1] = F[1] - C * delta * (alpha_c * u_roe + alpha_s * tx);
@@ -367,7 +418,7 @@ __device__ __forceinline__ void riemann(const Gas<T>& g, T ul, T vl, T pl, T al,
   if (RIE == 1) {                               // HLLE
     T S1 = min3<T>(T(0), unl - al, un_roe - a_roe);
     T S4 = max3<T>(T(0), unr + ar, un_roe + a_roe);
-    T inv_denom = qdiv<T>(T(1), S4 - S1);
+    T inv_denom = gdiv(g, T(1), S4 - S1);
     T FrL = dltot * unl;
     T FrR = drtot * unr;
     T S1S4 = S1 * S4;
@@ -375,14 +426,14 @@ __device__ __forceinline__ void riemann(const Gas<T>& g, T ul, T vl, T pl, T al,
     F[1] = (S4 * (FrL * ul + pl * nx) - S1 * (FrR * ur + pr * nx) + S1S4 * (drtot * ur - dltot * ul)) * inv_denom;
     F[2] = (S4 * (FrL * vl + pl * ny) - S1 * (FrR * vr + pr * ny) + S1S4 * (drtot * vr - dltot * vl)) * inv_denom;
     F[3] = (S4 * (FrL * h0l) - S1 * (FrR * h0r)
-            + S1S4 * (drtot * (h0r - qdiv<T>(pr, drtot)) - dltot * (h0l - qdiv<T>(pl, dltot)))) * inv_denom;
+            + S1S4 * (drtot * (h0r - gdiv(g, pr, drtot)) - dltot * (h0l - gdiv(g, pl, dltot)))) * inv_denom;
     return;
   }
-  T e0l = h0l - qdiv<T>(pl, dltot);
-  T e0r = h0r - qdiv<T>(pr, drtot);
+  T e0l = h0l - gdiv(g, pl, dltot);
+  T e0r = h0r - gdiv(g, pr, drtot);
   T S1 = min3<T>(T(0), unl - al, un_roe - a_roe);
   T S4 = max3<T>(T(0), unr + ar, un_roe + a_roe);
-  T inv_denom = qdiv<T>(T(1), dltot * (S1 - unl) - drtot * (S4 - unr));
+  T inv_denom = gdiv(g, T(1), dltot * (S1 - unl) - drtot * (S4 - unr));
   T Sstar = (pr - pl + dltot * unl * (S1 - unl) - drtot * unr * (S4 - unr)) * inv_denom;
   T pstar = dltot * (unl - S1) * (unl - Sstar) + pl;
   // The two star branches share one code path: side s = L when Sstar >= 0, else R.
@@ -401,7 +452,7 @@ __device__ __forceinline__ void riemann(const Gas<T>& g, T ul, T vl, T pl, T al,
   T F_v = F_r * vk + pk * ny;
   T F_E = F_r * h0k;
   if (star) {
-    T inv_d = qdiv<T>(T(1), Sk - Sstar);
+    T inv_d = gdiv(g, T(1), Sk - Sstar);
     T U1S = dk * (Sk - unk) * inv_d;
     T U2S = ((Sk - unk) * dk * uk + (pstar - pk) * nx) * inv_d;
     T U3S = ((Sk - unk) * dk * vk + (pstar - pk) * ny) * inv_d;
@@ -418,19 +469,21 @@ __device__ __forceinline__ void riemann(const Gas<T>& g, T ul, T vl, T pl, T al,
 __device__ __forceinline__ double rcp64(double a) { return 1.0 / a; }
 
 // ---- conservative <-> primitive (FP64 always) ------------------------------
-__device__ __forceinline__ void prim2cons(const Gas<double>& g, double rho, double u, double v, double p, double U[4]) {
-  double T = ddiv_rn(p, rho * g.R);
+template <int DV>
+__device__ __forceinline__ void prim2cons(const Gas<double, DV>& g, double rho, double u, double v, double p, double U[4]) {
+  double T = gdiv(g, p, rho * g.R);
   U[0] = rho;
   U[1] = rho * u;
   U[2] = rho * v;
   U[3] = rho * (int_energy(g, T) + 0.5 * (u * u + v * v));
 }
 // T0: the temperature estimate of the table mode's inversion (the closed form does not read it)
-__device__ __forceinline__ void cons2prim(const Gas<double>& g, const double U[4], double q[4], double T0) {
+template <int DV>
+__device__ __forceinline__ void cons2prim(const Gas<double, DV>& g, const double U[4], double q[4], double T0) {
   double rho = U[0];
-  double vx = ddiv_rn(U[1], rho), vy = ddiv_rn(U[2], rho);
-  double e = ddiv_rn(U[3], rho) - 0.5 * (vx * vx + vy * vy);
-  double T = g.tab ? tab_t_of_e(g.tab, g.ntab, g.R, e, T0) : ddiv_rn(e, g.cv);
+  double vx = gdiv(g, U[1], rho), vy = gdiv(g, U[2], rho);
+  double e = gdiv(g, U[3], rho) - 0.5 * (vx * vx + vy * vy);
+  double T = g.tab ? tab_t_of_e(g, e, T0) : gdiv(g, e, g.cv);
   q[0] = rho; q[1] = vx; q[2] = vy; q[3] = rho * g.R * T;
 }
 

@@ -75,6 +75,12 @@ constexpr bool kFuseDtMx = Q2D_FUSE_DT_MX != 0;
 #ifndef Q2D_STAGE_MIN_WAVES
 #define Q2D_STAGE_MIN_WAVES 1
 #endif
+// FP64 stage kernel: each work item runs a fast-division pass (ddiv_fast, with checks) first and the
+// IEEE division where a check fails.  Measured on 4 x 1024^2, SSP-RK3: 279 -> 240 us per launch when
+// every item passes its checks.
+#ifndef Q2D_FASTDIV
+#define Q2D_FASTDIV 1
+#endif
 
 // ---------------------------------------------------------------------------
 // device-side descriptors
@@ -112,7 +118,8 @@ struct DevCtrl {
 
 struct DevBlock {
   int ni, nj, pitch, case_id;
-  int blk_in_case, pad0;
+  int blk_in_case;
+  int fastdiv;                   // FP64: geometry within the fast-division range (k_stage's fast pass)
   int bcf[4];                    // offset of the per-cell BC-record index list of faces 1..4 (-1: none)
   int wallf[4];                  // faces 1..4: 0 no wall record (301-304), 1 every record a wall, 2 mixed
   long long plane;               // doubles per plane
@@ -485,14 +492,20 @@ __device__ __forceinline__ void ghost_fill(const DevBlock* blocks, const DevBC& 
 // SD_Tramel — Lib_Shock_Detector.f90:14-38 (DELTA = 20): p at the cell, p1 / p2 at i-1 / i+1, p3 / p4 at
 // j-1 / j+1.  tanh(x) for x = 10 phi^3 <= 1.6e-7 is x - x^3/3 (the next term of its series is below
 // 1e-28 x): tanh rounded to nearest, the same expression as the oracle's.
-__device__ __forceinline__ double sd_tramel(double p, double p1, double p2, double p3, double p4) {
-  const double kp1 = fabs((p2 - 2.0 * p + p1) / (p2 + 2.0 * p + p1));
-  const double kp2 = fabs((p4 - 2.0 * p + p3) / (p4 + 2.0 * p + p3));
+// g: the closure whose division it uses (gdiv)
+template <int DV>
+__device__ __forceinline__ double sd_tramel(const Gas<double, DV>& g, double p, double p1, double p2, double p3,
+                                            double p4) {
+  const double kp1 = fabs(gdiv(g, p2 - 2.0 * p + p1, p2 + 2.0 * p + p1));
+  const double kp2 = fabs(gdiv(g, p4 - 2.0 * p + p3, p4 + 2.0 * p + p3));
   const double sw = kp1 > kp2 ? kp1 : kp2;
   if (!(sw < 1.0 / 20.0)) return 0.0;
-  const double phi = sw / 20.0;
+  const double phi = gdiv(g, sw, 20.0);
   const double x = 10.0 * phi * phi * phi;
-  return 1.0 - (x - x * x * x / 3.0);
+  return 1.0 - (x - gdiv(g, x * x * x, 3.0));
+}
+__device__ __forceinline__ double sd_tramel(double p, double p1, double p2, double p3, double p4) {
+  return sd_tramel(Gas<double>{}, p, p1, p2, p3, p4);
 }
 
 struct BCFlux { double F[4]; double V[4]; int mask; int vmask; };
@@ -688,7 +701,7 @@ __global__ __launch_bounds__(256) void k_bcflux(const DevBlock* __restrict__ blo
       ty = 300;                                          // error=1 -> BC_Symmetry_Eul (Mod_BC_Fluxes.f90:227)
     } else {
       const double Bound_Sound = sound<double>(gd, Bp[3], Bp[0]);
-      const double G = gd.tab ? tab_gamma(gd.tab, gd.ntab, gd.R, Bp[3] / (Bp[0] * gd.R)) : gd.gam;
+      const double G = gd.tab ? tab_gamma(gd, Bp[3] / (Bp[0] * gd.R)) : gd.gam;
       const double Riem = Bound_Sound + sgn * 0.5 * (G - 1.0) * Un;
       const double VT1 = Bp[1] - Un * n1, VT2 = Bp[2] - Un * n2;
       const double Mach_N = fabs(Un / Bound_Sound);
@@ -1286,32 +1299,32 @@ __global__ __launch_bounds__(256) void k_bcvisc(const DevBlock* __restrict__ blo
 // ---------------------------------------------------------------------------
 // the fused RK-stage kernel
 // ---------------------------------------------------------------------------
-template <typename FT>
-__device__ __forceinline__ void limiter4(int lim, const FT x[4], const FT y[4], FT o[4]) {
+template <typename FT, int DV>
+__device__ __forceinline__ void limiter4(const Gas<FT, DV>& g, int lim, const FT x[4], const FT y[4], FT o[4]) {
   switch (lim) {
     case 1:
 #pragma unroll
-      for (int v = 0; v < 4; ++v) o[v] = limiter<FT>(1, x[v], y[v]);
+      for (int v = 0; v < 4; ++v) o[v] = limiter(g, 1, x[v], y[v]);
       break;
     case 2:
 #pragma unroll
-      for (int v = 0; v < 4; ++v) o[v] = limiter<FT>(2, x[v], y[v]);
+      for (int v = 0; v < 4; ++v) o[v] = limiter(g, 2, x[v], y[v]);
       break;
     case 3:
 #pragma unroll
-      for (int v = 0; v < 4; ++v) o[v] = limiter<FT>(3, x[v], y[v]);
+      for (int v = 0; v < 4; ++v) o[v] = limiter(g, 3, x[v], y[v]);
       break;
     case 4:
 #pragma unroll
-      for (int v = 0; v < 4; ++v) o[v] = limiter<FT>(4, x[v], y[v]);
+      for (int v = 0; v < 4; ++v) o[v] = limiter(g, 4, x[v], y[v]);
       break;
     case 5:
 #pragma unroll
-      for (int v = 0; v < 4; ++v) o[v] = limiter<FT>(5, x[v], y[v]);
+      for (int v = 0; v < 4; ++v) o[v] = limiter(g, 5, x[v], y[v]);
       break;
     case 6:
 #pragma unroll
-      for (int v = 0; v < 4; ++v) o[v] = limiter<FT>(6, x[v], y[v]);
+      for (int v = 0; v < 4; ++v) o[v] = limiter(g, 6, x[v], y[v]);
       break;
     default:
 #pragma unroll
@@ -1323,8 +1336,8 @@ __device__ __forceinline__ void limiter4(int lim, const FT x[4], const FT y[4], 
 // Convective_Flux — Lib_Convective.f90:46-80 for one face whose left/right
 // cell states and limited slopes are known.  Returns true on a non-physical
 // interface state (check_gas_state).
-template <typename FT, int RIE>
-__device__ __forceinline__ bool face_flux(const Gas<FT>& g, const double qa[4], const FT pa[4], FT lm, const double qb[4],
+template <typename FT, int RIE, int DV>
+__device__ __forceinline__ bool face_flux(const Gas<FT, DV>& g, const double qa[4], const FT pa[4], FT lm, const double qb[4],
                                           const FT pb[4], FT lp, FT bt, FT brie, FT nx, FT ny, FT il, FT out[4]) {
   FT qL[4], qR[4];
 #pragma unroll
@@ -1384,8 +1397,14 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
     // RIE_T < 0: the cases' solvers share the launch; the item's case picks one (uniform switch).  HLLE++ is
     // instantiated only with RIE_T == -2, launched for handles that hold an HLLE++ case, so that the other
     // launches keep their register allocation (2 waves per SIMD)
-    auto item_body = [&](auto riec) {
+    // dvc = 1: the FP64 fast-division pass (ddiv_fast).  It checks every row it computes and returns false
+    // at the first row that fails a check, having committed no failure; the item is then run again with
+    // the IEEE division (dvc = 0).  Rows stored before that are the IEEE pass's values already.
+    auto item_body = [&](auto riec, auto dvc) -> bool {
       constexpr int RIE = decltype(riec)::value;
+      constexpr int DV = decltype(dvc)::value;
+      constexpr bool FD = DV != 0;
+      static_assert(!FD || sizeof(FT) == 8, "the fast division is an FP64 pass");
       const bool last = (irk == C.n_rk);
       const bool irs = C.irs_beta > 0.0;                 // residual to B.irs[0]; k_irs smooths and updates
       // MUSCL-SD: faces take their cell's detector value; the detector is computed with MUSCL-SD or an
@@ -1408,8 +1427,42 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
       GP m11p = gp(B.m11);
       const int lim = C.limiter;
       const bool tacc = C.time_accurate != 0;
-      Gas<FT> gf{(FT)C.R, (FT)C.cp, (FT)C.cv, (FT)C.gam};
+      // Fast pass checks (running minima / maxima of frexp exponents, 0 for 0):
+      //  ex_div[0]  every numerator:  a = 0 or |a| >= 2^-969
+      //  ex_div[1]  every quotient:   q = 0 or |q| >= 2^-1020 (so the exact quotient is normal)
+      //  ex_lo / ex_hi  every state value read (stage input rows, RK base state): rho and p 0 or at
+      //                 least 2^-100 in magnitude, every component below 2^100
+      // With the block's geometry 0 or within [2^-100, 2^100) (checked at upload, B.fastdiv), every
+      // denominator of the path is 0 or within [2^-300, 2^300] (densities, pressures, sound speeds,
+      // lengths, their sums, products and cancellations, x + 1e-30 of the limiters) except the
+      // limiters' 1 + |r| and 1 + r^2, whose numerators grow with them, and no quotient reaches 2^1023:
+      // within these bounds ddiv_fast equals the IEEE division bit for bit.  The next step's dt (the
+      // division of values just written) uses the IEEE division in both passes.
+      int ex_div[2] = {0, 0}, ex_lo = 0, ex_hi = 0;
+      unsigned long long fkey = ~0ull;                  // failures of the fast pass, committed when it holds
+      int fcode = 0;
+      auto fail = [&](int code, unsigned long long key) {
+        if constexpr (FD) {
+          fkey = key < fkey ? key : fkey;
+          fcode = code < fcode ? code : fcode;
+        } else {
+          record_fail(&K, code, key);
+        }
+      };
+      auto chk_in = [&](const double* q) {
+        if constexpr (FD) {
+          const int e0 = __builtin_amdgcn_frexp_exp(q[0]), e1 = __builtin_amdgcn_frexp_exp(q[1]);
+          const int e2 = __builtin_amdgcn_frexp_exp(q[2]), e3 = __builtin_amdgcn_frexp_exp(q[3]);
+          ex_lo = min(ex_lo, min(e0, e3));
+          ex_hi = max(ex_hi, max(max(e0, e1), max(e2, e3)));
+        }
+      };
+      Gas<FT, DV> gf{(FT)C.R, (FT)C.cp, (FT)C.cv, (FT)C.gam};
       if constexpr (TAB) { gf.tab = C.tab; gf.ntab = C.ntab; }
+      gf.ex = ex_div;
+      Gas<double, DV> gd{C.R, C.cp, C.cv, C.gam};
+      if constexpr (TAB) { gd.tab = C.tab; gd.ntab = C.ntab; }
+      gd.ex = ex_div;
       const double c_rk = (C.n_rk == 1) ? 1.0 : (C.n_rk == 2 ? (irk == 1 ? 1.0 : 0.5)
                                                               : (irk == 1 ? 1.0 : (irk == 2 ? 0.25 : 2.0 / 3.0)));
       const double dtg = K.dtglobal;
@@ -1451,6 +1504,7 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
           dlyN2 = dlyp[o4];
         }
         const double dlm = dlyp[o0];
+        chk_in(qm); chk_in(qA); chk_in(qB);
         pPrev = qm[3];
         dlyA = dlyp[o1]; dlyB = dlyp[o2]; dlyN = dlyp[o3];
         { const long long og = at(item.y0 - 1); fjxN = fjxp[og]; fjyN = fjyp[og]; fjlN = fjlp[og]; }
@@ -1458,11 +1512,11 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
         FT s0[4];
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-          s0[v] = qdiv<FT>((FT)(qA[v] - qm[v]), l0);
-          sAB[v] = qdiv<FT>((FT)(qB[v] - qA[v]), l1);
+          s0[v] = gdiv(gf, (FT)(qA[v] - qm[v]), l0);
+          sAB[v] = gdiv(gf, (FT)(qB[v] - qA[v]), l1);
           Gprev[v] = FT(0);
         }
-        limiter4<FT>(lim, sAB, s0, phiA);
+        limiter4(gf, lim, sAB, s0, phiA);
       }
       double acc[4] = {0.0, 0.0, 0.0, 0.0};
       double dtnext = 1e300;
@@ -1514,6 +1568,7 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
         // flux, which hides its latency without holding a second row of it in registers.
 #pragma unroll
         for (int v = 0; v < 4; ++v) qC[v] = qN[v];
+        chk_in(qC);
         dlyC = dlyN;
         gcur.fjx = fjxN; gcur.fjy = fjyN; gcur.fjl = fjlN;
         {
@@ -1556,14 +1611,14 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
         {
           const FT lBC = (FT)(0.5 * (dlyB + dlyC));
 #pragma unroll
-          for (int v = 0; v < 4; ++v) sBC[v] = qdiv<FT>((FT)(qC[v] - qB[v]), lBC);
-          limiter4<FT>(lim, sBC, sAB, phiB);            // phi_{r+1} = rlimiter(s(r+1|r+2), s(r|r+1))
+          for (int v = 0; v < 4; ++v) sBC[v] = gdiv(gf, (FT)(qC[v] - qB[v]), lBC);
+          limiter4(gf, lim, sBC, sAB, phiB);            // phi_{r+1} = rlimiter(s(r+1|r+2), s(r|r+1))
         }
         // Tramel's detector of cell (x, r) (Mod_Fluxes.f90:129-138): its i-face x|x+1 and j-face r|r+1
         FT bt = FT(1), brie = FT(1);                     // the reconstruction's and the Riemann solver's
         if (det) {
           const double pc = qA[3], pim = shup(qA[3]), pip = shdn(qA[3]);
-          brie = (FT)sd_tramel(pc, pim, pip, pPrev, qB[3]);
+          brie = (FT)sd_tramel(gd, pc, pim, pip, pPrev, qB[3]);
           bt = sd ? brie : FT(1);
         }
         double Ecur[3] = {0.0, 0.0, 0.0};
@@ -1576,7 +1631,7 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
         if (own && r >= 1 && r <= nj - 1) {
           if (face_flux<FT, RIE>(gf, qA, phiA, (FT)(0.5 * dlyA), qB, phiB, (FT)(0.5 * dlyB), bt, brie, (FT)gcur.fjx,
                                  (FT)gcur.fjy, (FT)gcur.fjl, Gcur))
-            record_fail(&K, Q2D_E_INTERFACE, fail_key(stage_id, 0, B.blk_in_case, r, x));
+            fail(Q2D_E_INTERFACE, fail_key(stage_id, 0, B.blk_in_case, r, x));
         } else {
 #pragma unroll
           for (int v = 0; v < 4; ++v) Gcur[v] = FT(0);
@@ -1606,18 +1661,18 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
           {
             const FT l1 = (FT)(0.5 * (dlx + dlxn));
 #pragma unroll
-            for (int v = 0; v < 4; ++v) sx[v] = qdiv<FT>((FT)(qn[v] - qA[v]), l1);
+            for (int v = 0; v < 4; ++v) sx[v] = gdiv(gf, (FT)(qn[v] - qA[v]), l1);
           }
 #pragma unroll
           for (int v = 0; v < 4; ++v) sxp[v] = shup(sx[v]);
-          limiter4<FT>(lim, sx, sxp, phx);
+          limiter4(gf, lim, sx, sxp, phx);
 #pragma unroll
           for (int v = 0; v < 4; ++v) phxn[v] = shdn(phx[v]);
           FT Fx[4];
           if (xdo) {                                     // interior i-face (x|x+1) — Mod_Fluxes.f90:140-185
             if (face_flux<FT, RIE>(gf, qA, phx, (FT)(0.5 * dlx), qn, phxn, (FT)(0.5 * dlxn), bt, brie, (FT)gcur.fix,
                                    (FT)gcur.fiy, (FT)gcur.fil, Fx))
-              record_fail(&K, Q2D_E_INTERFACE, fail_key(stage_id, 0, B.blk_in_case, r, x));
+              fail(Q2D_E_INTERFACE, fail_key(stage_id, 0, B.blk_in_case, r, x));
           } else {
 #pragma unroll
             for (int v = 0; v < 4; ++v) Fx[v] = FT(0);
@@ -1709,12 +1764,10 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
             if (r == nj && B.bcf[3] >= 0) add_bc(bcflux[B.bcf[3] + x - 1]);
             // ---- geometric sources — Lib_Source_Axisymmetry.f90:40-78, Lib_Source_Areavariation.f90:104-144
             const double sr = gcur.sur;
-            Gas<double> gd{C.R, C.cp, C.cv, C.gam};
-            if constexpr (TAB) { gd.tab = C.tab; gd.ntab = C.ntab; }
             if (C.source != Q2D_SRC_NONE) {
               const double rho = qA[0], u = qA[1], v = qA[2], p = qA[3];
               const double mod_v = dsqrt_rn(u * u + v * v);
-              const double E0 = int_energy(gd, ddiv_rn(p, rho * gd.R)) + 0.5 * mod_v * mod_v;
+              const double E0 = int_energy(gd, gdiv(gd, p, rho * gd.R)) + 0.5 * mod_v * mod_v;
               double S[4];
               if (C.source == Q2D_SRC_AXI) {
                 const double invr = gp(B.sb)[oA];
@@ -1746,7 +1799,7 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
               GPW rs = gpw(B.irs[0]);
               if (sizeof(FT) == 8) {
 #pragma unroll
-                for (int v = 0; v < 4; ++v) rs[v * pl + oA] = ddiv_rn(-res[v], sr) * dt;
+                for (int v = 0; v < 4; ++v) rs[v * pl + oA] = gdiv(gd, -res[v], sr) * dt;
               } else {
                 const double dts = dt * rcp64(sr);
 #pragma unroll
@@ -1757,6 +1810,8 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
             if (irk == 1) {                              // RK base state == stage input on stage 1
 #pragma unroll
               for (int v = 0; v < 4; ++v) gcur.po[v] = qA[v];
+            } else {
+              chk_in(gcur.po);
             }
             double Uo[4], U[4], qn2[4];
             prim2cons(gd, gcur.po[0], gcur.po[1], gcur.po[2], gcur.po[3], Uo);
@@ -1764,10 +1819,10 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
             if (sizeof(FT) == 8) {
 #pragma unroll
               for (int v = 0; v < 4; ++v) {
-                const double rr = ddiv_rn(-res[v], sr) * dt;
+                const double rr = gdiv(gd, -res[v], sr) * dt;
                 U[v] = Uo[v] + (U[v] - Uo[v] + rr) * c_rk;   // RK_stage — Lib_RK.f90:15-27
               }
-              cons2prim(gd, U, qn2, TAB ? ddiv_rn(qA[3], qA[0] * gd.R) : 0.0);   // estimate: Lib_Newstate.f90:151-154
+              cons2prim(gd, U, qn2, TAB ? gdiv(gd, qA[3], qA[0] * gd.R) : 0.0);   // estimate: Lib_Newstate.f90:151-154
             } else {
               // mixed form: FP64 update with one reciprocal of sur and of rho
               const double dts = dt * rcp64(sr);
@@ -1781,7 +1836,7 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
             // Check_And_Fix_State — Lib_Newstate.f90:316-350
             if (__builtin_isnan(qn2[0]) || __builtin_isnan(qn2[1]) || __builtin_isnan(qn2[2]) ||
                 __builtin_isnan(qn2[3]) || qn2[3] < 0.0)
-              record_fail(&K, Q2D_E_STATE, fail_key(stage_id, 1, B.blk_in_case, r, x));
+              fail(Q2D_E_STATE, fail_key(stage_id, 1, B.blk_in_case, r, x));
             if (qn2[0] < 1e-20) qn2[0] = 1e-20;
 #pragma unroll
             for (int v = 0; v < 4; ++v) Pout[v * pl + oA] = qn2[v];
@@ -1792,7 +1847,9 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
             if (last && FUSE) {
               // Compute_dt of the next step (Mod_dt.f90:59-104) on the state just written; the
               // next step's Compute_dt would read exactly these values.
-              const double snd = TAB ? sound<double>(gd, qn2[3], qn2[0]) : dsqrt_rn(ddiv_rn(C.gam * qn2[3], qn2[0]));
+              Gas<double> gdt{C.R, C.cp, C.cv, C.gam};        // IEEE division: dt of the state just written
+              if constexpr (TAB) { gdt.tab = C.tab; gdt.ntab = C.ntab; }
+              const double snd = TAB ? sound<double>(gdt, qn2[3], qn2[0]) : dsqrt_rn(ddiv_rn(C.gam * qn2[3], qn2[0]));
               double dtmin = 1e8;
               const double* mm = gcur.mm;                 // c11, c21, c12, c22
               {
@@ -1841,6 +1898,13 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
           }
         }
         dlyA = dlyB; dlyB = dlyC;
+        if constexpr (FD) {
+          const bool bad = ex_div[0] < -968 || ex_div[1] < -1019 || ex_lo < -99 || ex_hi > 100;
+          if (__builtin_amdgcn_ballot_w64(bad) != 0ull) return false;
+        }
+      }
+      if constexpr (FD) {
+        if (fcode < 0) record_fail(&K, fcode, fkey);
       }
       if (last && !irs) {
         // fixed-pattern wavefront sum -> one partial per work item (deterministic)
@@ -1859,18 +1923,29 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
         }
         if (lane == 0) partials[(long long)it * kPartials + 4] = dtnext;
       }
+      return true;
+    };
+    // FP64 items of blocks whose geometry is in the fast-division range run the fast pass first.  Only the
+    // Euler instantiations without HLLE++ and table mode hold both passes within 256 VGPRs (2 waves per
+    // SIMD); the laminar, HLLE++ and table-mode ones run the IEEE pass alone.
+    constexpr bool kFastInst = sizeof(FT) == 8 && Q2D_FASTDIV && !VISC && !TAB && RIE_T != -2;
+    auto run_item = [&](auto riec) {
+      if constexpr (kFastInst) {
+        if (B.fastdiv && item_body(riec, std::integral_constant<int, 1>{})) return;
+      }
+      item_body(riec, std::integral_constant<int, 0>{});
     };
     if constexpr (RIE_T == -3) {                       // HLLC and HLLE only
-      if (C.riemann == Q2D_RIEMANN_HLLE) item_body(std::integral_constant<int, 1>{});
-      else item_body(std::integral_constant<int, 0>{});
+      if (C.riemann == Q2D_RIEMANN_HLLE) run_item(std::integral_constant<int, 1>{});
+      else run_item(std::integral_constant<int, 0>{});
     } else if constexpr (RIE_T < 0) {
-      if (C.riemann == Q2D_RIEMANN_HLLE) item_body(std::integral_constant<int, 1>{});
-      else if (C.riemann == Q2D_RIEMANN_HLLEM) item_body(std::integral_constant<int, 2>{});
-      else if ((RIE_T == -2 || TAB) && C.riemann == Q2D_RIEMANN_HLLEPP) item_body(std::integral_constant<int, 3>{});
-      else if ((RIE_T == -2 || TAB) && C.riemann == Q2D_RIEMANN_HLLCP) item_body(std::integral_constant<int, 4>{});
-      else item_body(std::integral_constant<int, 0>{});
+      if (C.riemann == Q2D_RIEMANN_HLLE) run_item(std::integral_constant<int, 1>{});
+      else if (C.riemann == Q2D_RIEMANN_HLLEM) run_item(std::integral_constant<int, 2>{});
+      else if ((RIE_T == -2 || TAB) && C.riemann == Q2D_RIEMANN_HLLEPP) run_item(std::integral_constant<int, 3>{});
+      else if ((RIE_T == -2 || TAB) && C.riemann == Q2D_RIEMANN_HLLCP) run_item(std::integral_constant<int, 4>{});
+      else run_item(std::integral_constant<int, 0>{});
     } else {
-      item_body(std::integral_constant<int, RIE_T>{});
+      run_item(std::integral_constant<int, RIE_T>{});
     }
   }
   if (tslotp && lane == 0) atomicMax(&tslotp[1], (unsigned long long)wall_clock64());
@@ -3715,9 +3790,20 @@ int append_cases(q2d_handle* h, const q2d_problem* prob, int32_t* first_case) {
           }
         }
     }
+    // the FP64 stage kernel's fast-division pass: every geometry value it reads (lengths, area, face
+    // normals and lengths, source geometry) is 0 or within [2^-100, 2^100) in magnitude
+    bool geo_fast = true;
+    for (int k : {12, 13, 14, 19, 20, 21, 22, 23, 24, 26, 27}) {
+      const double* g = plane(k);
+      for (long long c = 0; c < pl && geo_fast; ++c) {
+        const double a = fabs(g[c]);
+        if (!(a == 0.0 || (a >= 0x1p-100 && a < 0x1p100))) geo_fast = false;
+      }
+    }
     HIPCHK(hipMemcpy(hb.arena, host.data(), hb.arena_bytes, hipMemcpyHostToDevice));
     DevBlock& db = h->db.back();
     memset(&db, 0, sizeof(db));
+    db.fastdiv = geo_fast ? 1 : 0;
     db.ni = d.ni; db.nj = d.nj; db.pitch = pitch; db.case_id = hb.case_id; db.blk_in_case = hb.blk_in_case;
     db.plane = pl;
     for (int sb = 0; sb < 3; ++sb) db.P[sb] = hb.arena + (size_t)4 * sb * pl;
