@@ -182,8 +182,9 @@ typedef struct q2d_case_desc {
                                read by laminar cases only).  Properties are interpolated linearly, from
                                node int(T) (Lib_Riemann_HLL.f90:257-282); T beyond the last interval
                                extrapolates the end interval.  gas_R stays the species' R = Runiv / W;
-                               gas_cp is not read.  The plain FP64 form only: the mixed form rejects it
-                               with Q2D_E_UNSUPPORTED.  Copied at create / add_cases. */
+                               gas_cp is not read.  Both forms: the mixed form's FP32 faces interpolate
+                               the tables in FP32 and its FP64 update inverts e(T) on them.  Copied at
+                               create / add_cases. */
   int32_t    thermo_n;      /* nodes per plane (>= 2) when thermo != NULL */
   int32_t    reserved2;
 } q2d_case_desc;

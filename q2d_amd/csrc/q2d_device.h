@@ -99,6 +99,16 @@ __device__ __forceinline__ double tab_interp(const double* tab, int ntab, int q,
   const double a = v[Ti], b = v[Ti + 1];
   return a + (b - a) * dT;
 }
+// the same in T's arithmetic (FP32: the mixed form's face arithmetic on the FP64 tables' nodes)
+template <typename T> __device__ __forceinline__ T tab_interp_t(const double* tab, int ntab, int q, int Ti, T dT) {
+  const double* v = tab + (long long)q * ntab;
+  const T a = (T)v[Ti], b = (T)v[Ti + 1];
+  return a + (b - a) * dT;
+}
+template <typename T> __device__ __forceinline__ T tab_at_t(const double* tab, int ntab, int q, T Tk) {
+  const int Ti = tab_node(ntab, (double)Tk);
+  return tab_interp_t<T>(tab, ntab, q, Ti, Tk - (T)Ti);
+}
 __device__ __forceinline__ double tab_at(const double* tab, int ntab, int q, double T) {
   const int Ti = tab_node(ntab, T);
   return tab_interp(tab, ntab, q, Ti, T - (double)Ti);
@@ -136,6 +146,11 @@ __device__ __forceinline__ double tab_t_of_e(const Gas<double, DV>& g, double e,
 template <typename T, int DV> __device__ __forceinline__ T sound(const Gas<T, DV>& g, T p, T rho) {
   if constexpr (sizeof(T) == 8) {
     if (g.tab) return dsqrt_rn(gdiv(g, tab_gamma(g, gdiv(g, p, rho * g.R)) * p, rho));
+  } else {
+    if (g.tab) {                                  // FP32: gamma = cp / (cp - R) at T = p / (rho R)
+      const T cp = tab_at_t<T>(g.tab, g.ntab, kTabCp, gdiv(g, p, rho * g.R));
+      return qsqrt<T>(gdiv(g, gdiv(g, cp, cp - g.R) * p, rho));
+    }
   }
   return qsqrt<T>(gdiv(g, g.gam * p, rho));
 }
@@ -143,6 +158,8 @@ template <typename T, int DV> __device__ __forceinline__ T sound(const Gas<T, DV
 template <typename T, int DV> __device__ __forceinline__ T enthalpy(const Gas<T, DV>& g, T p, T rho) {
   if constexpr (sizeof(T) == 8) {
     if (g.tab) return tab_at(g.tab, g.ntab, kTabH, gdiv(g, p, rho * g.R));
+  } else {
+    if (g.tab) return tab_at_t<T>(g.tab, g.ntab, kTabH, gdiv(g, p, rho * g.R));
   }
   return g.cp * gdiv(g, p, rho * g.R);
 }
@@ -279,18 +296,14 @@ __device__ __forceinline__ void roe_averages(const Gas<T, DV>& g, T dltot, T drt
   T T_roe = (srR * Tr + srL * Tl) * inv_sr;
   T invW = g.R;
   T hl, hr, cpsum;
-  if constexpr (sizeof(T) == 8) {
-    if (g.tab) {                                  // the table arithmetic of :257-292
-      const int Til = tab_node(g.ntab, Tl), Tir = tab_node(g.ntab, Tr);
-      const T dTl = Tl - (T)Til, dTr = Tr - (T)Tir;
-      hl = tab_interp(g.tab, g.ntab, kTabH, Til, dTl);
-      hr = tab_interp(g.tab, g.ntab, kTabH, Tir, dTr);
-      const double* cp = g.tab + (long long)kTabCp * g.ntab;
-      const T cl0 = cp[Til], cl1 = cp[Til + 1], cr0 = cp[Tir], cr1 = cp[Tir + 1];
-      cpsum = ((cl0 + (cl1 - cl0) * dTl) + cr0) + (cr1 - cr0) * dTr;
-    } else {
-      hl = g.cp * Tl; hr = g.cp * Tr; cpsum = g.cp + g.cp;
-    }
+  if (g.tab) {                                    // the table arithmetic of :257-292 (FP32: on the FP64 nodes)
+    const int Til = tab_node(g.ntab, (double)Tl), Tir = tab_node(g.ntab, (double)Tr);
+    const T dTl = Tl - (T)Til, dTr = Tr - (T)Tir;
+    hl = tab_interp_t<T>(g.tab, g.ntab, kTabH, Til, dTl);
+    hr = tab_interp_t<T>(g.tab, g.ntab, kTabH, Tir, dTr);
+    const double* cp = g.tab + (long long)kTabCp * g.ntab;
+    const T cl0 = (T)cp[Til], cl1 = (T)cp[Til + 1], cr0 = (T)cp[Tir], cr1 = (T)cp[Tir + 1];
+    cpsum = ((cl0 + (cl1 - cl0) * dTl) + cr0) + (cr1 - cr0) * dTr;
   } else {
     hl = g.cp * Tl; hr = g.cp * Tr; cpsum = g.cp + g.cp;
   }

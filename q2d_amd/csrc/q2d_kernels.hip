@@ -603,7 +603,7 @@ __global__ __launch_bounds__(256) void k_bcflux(const DevBlock* __restrict__ blo
   }
   const int lim = C.limiter, rid = C.riemann;
   Gas<FT> gf{(FT)C.R, (FT)C.cp, (FT)C.cv, (FT)C.gam};
-  if constexpr (sizeof(FT) == 8) { gf.tab = C.tab; gf.ntab = C.ntab; }
+  gf.tab = C.tab; gf.ntab = C.ntab;
   const Gas<double> gd = case_gas(C);
   // Face_Index — Lib_BC_Fluxes.f90:34-50
   const bool idir = Fm <= 2;
@@ -1372,9 +1372,9 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
     tslotp = tstamp + 2 + 2 * ((long long)launch * kStampLanes + (blockIdx.x % kStampLanes));
     if (threadIdx.x == 0) atomicMin(&tslotp[0], (unsigned long long)wall_clock64());
   }
-  // RIE_T == -4: every solver of RIE_T == -2 and the cases' property tables (table mode, FP64 only)
+  // RIE_T == -4: every solver of RIE_T == -2 and the cases' property tables (table mode; with FT = float
+  // the faces interpolate the tables in FP32 and the update inverts e(T) in FP64, as the FP64 form does)
   constexpr bool TAB = RIE_T == -4;
-  static_assert(!TAB || sizeof(FT) == 8, "table mode is FP64 only");
   const int lane = threadIdx.x & (kWave - 1);
   const int wpg = blockDim.x / kWave;
   __shared__ BCFlux sbq[4][2][16];                     // per wavefront: face 1 and 2 boundary results, 16 rows
@@ -1816,7 +1816,7 @@ __global__ __launch_bounds__(256, Q2D_STAGE_MIN_WAVES) void k_stage(const DevBlo
             double Uo[4], U[4], qn2[4];
             prim2cons(gd, gcur.po[0], gcur.po[1], gcur.po[2], gcur.po[3], Uo);
             prim2cons(gd, qA[0], qA[1], qA[2], qA[3], U);
-            if (sizeof(FT) == 8) {
+            if (sizeof(FT) == 8 || TAB) {               // (table mode: e(T) inverted on the tables)
 #pragma unroll
               for (int v = 0; v < 4; ++v) {
                 const double rr = gdiv(gd, -res[v], sr) * dt;
@@ -2739,7 +2739,7 @@ __global__ __launch_bounds__(256) void k_irs(const DevBlock* __restrict__ blocks
         prim2cons(gd, q[0], q[1], q[2], q[3], U);
 #pragma unroll
         for (int v = 0; v < 4; ++v) U[v] = Uo[v] + (U[v] - Uo[v] + s2[v]) * c_rk;   // RK_stage — Lib_RK.f90:15-27
-        if (!mixed) {
+        if (!mixed || gd.tab) {                          // (table mode, both forms: e(T) inverted on the tables)
           cons2prim(gd, U, qn2, ddiv_rn(q[3], q[0] * gd.R));   // estimate: Lib_Newstate.f90:119-123
         } else {
           const double ir = rcp64(U[0]);
@@ -2973,9 +2973,11 @@ int item_pass(const q2d_handle* h, int c) {
 const void* stage_kernel(const q2d_handle* h, int pass) {
   if (h->precision_mixed) {
     if (pass == 0) return reinterpret_cast<const void*>(&k_stage_mx<-1, kFuseDtMx, false, false>);
-    if (pass == 1) return h->any_hllepp ? reinterpret_cast<const void*>(&k_stage<float, -2, false, false>)
-                                        : reinterpret_cast<const void*>(&k_stage<float, -1, false, false>);
+    if (pass == 1) return h->any_table ? reinterpret_cast<const void*>(&k_stage<float, -4, false, false>)
+                          : h->any_hllepp ? reinterpret_cast<const void*>(&k_stage<float, -2, false, false>)
+                                          : reinterpret_cast<const void*>(&k_stage<float, -1, false, false>);
     if (pass == 2) return reinterpret_cast<const void*>(&k_stage_mx<-1, kFuseDtMx, false, true>);
+    if (h->any_table) return reinterpret_cast<const void*>(&k_stage<float, -4, kFuseDtMx, true>);
     if (h->any_hllepp) return reinterpret_cast<const void*>(&k_stage<float, -2, kFuseDtMx, true>);
     return h->any_det_or_hllem ? reinterpret_cast<const void*>(&k_stage<float, -1, kFuseDtMx, true>)
                                : reinterpret_cast<const void*>(&k_stage<float, -3, kFuseDtMx, true>);
@@ -3212,12 +3214,14 @@ int launch_stage(q2d_handle* h, int irk, const StepScratch& scr) {
       if (pass == 0 && h->any_irs) Q2D_STAGE_MX_LAUNCH(-1, true, false);
       else if (pass == 0) Q2D_STAGE_MX_LAUNCH(-1, false, false);
       else if (pass == 2) Q2D_STAGE_MX_LAUNCH(-1, false, true);   // laminar k_stage_mx cases (no smoothing)
+      else if (h->any_table && pass == 1) Q2D_STAGE_LAUNCH(-4, false);   // property tables
+      else if (h->any_table) Q2D_STAGE_LAM_MX_LAUNCH(-4);
       else if (h->any_hllepp && pass == 1) Q2D_STAGE_LAUNCH(-2, false);
       else if (h->any_hllepp) Q2D_STAGE_LAM_MX_LAUNCH(-2);
       else if (pass == 1) Q2D_STAGE_LAUNCH(-1, false);
       else if (h->any_det_or_hllem) Q2D_STAGE_LAM_MX_LAUNCH(-1);
       else Q2D_STAGE_LAM_MX_LAUNCH(-3);                // laminar HLLC / HLLE cases
-    } else if (h->any_table) {                         // property tables (plain FP64 form only)
+    } else if (h->any_table) {                         // property tables
       if (pass == 0) Q2D_STAGE_LAUNCH(-4, false);
       else Q2D_STAGE_LAUNCH(-4, true);
     } else if (h->any_hllepp) {                        // HLLE++ / HLLC+ (plain FP64 form only)
@@ -3467,10 +3471,13 @@ int rebuild_descriptors(q2d_handle* h) {
     for (int b = 0; b < h->nblocks && fv; ++b)
       if (h->hb[b].case_id == c)
         for (int f = 0; f < 4; ++f) fv = fv && h->db[b].wallf[f] != 2;
+    fv = fv && k.tab == nullptr;                       // table transport: k_visc
     k.fvisc = fv ? 1 : 0;
     if (h->case_live[c] && k.model > 0 && !fv) h->any_unfused_visc = true;
-    k.mxk = h->precision_mixed && mx_kernel_case(k.model, k.space_sd, k.riemann, k.fvisc, k.irs_beta) ? 1 : 0;
-    k.dtmx = h->precision_mixed && (k.model == 0 || k.mxk) ? 1 : 0;
+    // table-mode cases run the generic FP32-face kernels, with the dt pass's FP64 table formula
+    k.mxk = h->precision_mixed && k.tab == nullptr &&
+            mx_kernel_case(k.model, k.space_sd, k.riemann, k.fvisc, k.irs_beta) ? 1 : 0;
+    k.dtmx = h->precision_mixed && k.tab == nullptr && (k.model == 0 || k.mxk) ? 1 : 0;
   }
   h->max_rk = 1;
   h->any_visc = false;
@@ -3490,7 +3497,8 @@ int rebuild_descriptors(q2d_handle* h) {
     if (h->dc[c].riemann >= Q2D_RIEMANN_HLLEPP) h->any_hllepp = true;
     if (h->dc[c].tab) h->any_table = true;
     if (h->dc[c].riemann >= Q2D_RIEMANN_HLLEM || h->dc[c].space_sd) h->any_det_or_hllem = true;
-    if (h->precision_mixed && (h->dc[c].space_sd || h->dc[c].riemann >= Q2D_RIEMANN_HLLEM)) h->mixed_generic = true;
+    if (h->precision_mixed && (h->dc[c].space_sd || h->dc[c].riemann >= Q2D_RIEMANN_HLLEM || h->dc[c].tab))
+      h->mixed_generic = true;
     if (h->dc[c].space_sd || h->dc[c].riemann >= Q2D_RIEMANN_HLLEPP)
       h->bc_two_phase = true;                          // the connection faces' detector reads other records' ghosts
   }
@@ -3553,8 +3561,6 @@ int append_cases(q2d_handle* h, const q2d_problem* prob, int32_t* first_case) {
     if (!(s.gas_R > 0) || (!cdv.thermo && !(s.gas_cp > s.gas_R))) return fail(Q2D_E_ARG, "gas constants invalid");
     if (cdv.thermo) {                                  // table mode: e(T) = h(T) - R T must increase
       if (cdv.thermo_n < 2) return fail(Q2D_E_ARG, "thermo tables need thermo_n >= 2");
-      if (s.precision == Q2D_PREC_MIXED)
-        return fail(Q2D_E_UNSUPPORTED, "table-mode thermodynamics is available in the plain FP64 form only");
       for (int k = 0; k < 4 * cdv.thermo_n; ++k)
         if (!std::isfinite(cdv.thermo[k])) return fail(Q2D_E_ARG, "thermo tables hold a non-finite value");
       for (int k = 0; k + 1 < cdv.thermo_n; ++k)

@@ -1,10 +1,11 @@
 """Table-mode thermodynamics on the device (needs the GPU): FLINT's native tabulated properties of one
-species (q2d_case_desc.thermo), in the plain FP64 form.  Bit-identical to the oracle's table mode on a
-moving shock of a gas whose cp varies with T (every Riemann solver), a 2D ramp with the shock detector
-and residual smoothing, the characteristic outflow BC 406, an axisymmetric laminar pipe with viscosity and
-conductivity tables, and the reference's Shu-Osher case directory with its Air table.  A closed-form case
-batched beside table cases keeps its solo bits; the mixed form and a table whose e(T) does not increase
-are refused."""
+species (q2d_case_desc.thermo).  Plain FP64 form: bit-identical to the oracle's table mode on a moving
+shock of a gas whose cp varies with T (every Riemann solver), a 2D ramp with the shock detector and
+residual smoothing, the characteristic outflow BC 406, an axisymmetric laminar pipe with viscosity and
+conductivity tables, and the reference's Shu-Osher case directory with its Air table.  Mixed form (FP32
+faces interpolating the tables, FP64 update inverting e(T) on them): the same cases within the mixed
+form's tolerance of the oracle.  A closed-form case batched beside table cases keeps its solo bits; a
+table whose e(T) does not increase is refused."""
 import math
 from dataclasses import replace
 from pathlib import Path
@@ -34,36 +35,59 @@ def both(oracle_mod, pb, steps):
     return g, go
 
 
+def mixed_close(oracle_mod, pb, steps, tol_l1=1e-5):
+    """The mixed form of `pb` against the FP64 oracle: relative L1(rho) of every case within tol_l1."""
+    print(f"table-mode case (mixed): {pb.meta.get('name')} {[(b.ni, b.nj) for b in pb.blocks]} {steps} steps",
+          flush=True)
+    pb64 = replace(pb, cases=[replace(c, precision="fp64") for c in pb.cases])
+    pbmx = replace(pb, cases=[replace(c, precision="mixed") for c in pb.cases])
+    o = oracle_mod.Oracle(pb64)
+    rc, oo = o.run(steps, stop_on_todo=False)
+    assert rc == 0
+    g = GpuSolver(pbmx)
+    go = g.run(steps)
+    errs = []
+    for k in range(len(pb.cases)):
+        assert go[k]["status"] == 0 and go[k]["iter"] == oo[k]["iter"], (k, go[k]["status"])
+        blocks = [b for b in range(len(pb.blocks)) if pb.blocks[b].case_id == k]
+        num = sum(np.abs(g.state(b)[2:-2, 2:-2, 0] - o.state(b)[2:-2, 2:-2, 0]).sum() for b in blocks)
+        den = sum(np.abs(o.state(b)[2:-2, 2:-2, 0]).sum() for b in blocks)
+        linf = max(np.abs(g.state(b)[2:-2, 2:-2, 0] - o.state(b)[2:-2, 2:-2, 0]).max() for b in blocks)
+        errs.append((num / den, linf))
+        assert num / den <= tol_l1, (k, num / den)
+    print(f"  rel L1(rho), Linf(rho) per case: {errs}", flush=True)
+    return g
+
+
 def nondim_table(n=64):
     """cp(T) = 3 + T / 2 (R = 1): gamma = 1.4 at T = 1, the generators' free stream."""
     return th.linear_cp(3.0, 0.5, n)
 
 
-@pytest.mark.parametrize("riemann", ["HLLC", "HLLE", "HLLEM", "HLLE++", "HLLC+"])
-def test_moving_shock_varying_cp(oracle_mod, riemann):
+def moving_shock(riemann):
     pb = cases.riemann_planar(seed=3, ni=300, nj=4, steps=10 ** 6, riemann=riemann, limiter="vanleer", n_rk=3)
     pb.cases[0] = replace(pb.cases[0], thermo=nondim_table())
     P = pb.blocks[0].P
     P[2:-2, 2:-2, 3] *= 4.0                             # T from 0.4 to 4: several table intervals
     cases.fill_initial_ghosts(P)
-    both(oracle_mod, pb, 200)
+    return pb
 
 
-def test_ramp_detector_and_smoothing(oracle_mod):
+def ramp():
     pb = cases.ramp_channel(ni=96, nj=48, wedge_deg=12.0, mach=2.5, cfl=0.9, itermax=10 ** 6)
     pb.cases[0] = replace(pb.cases[0], thermo=nondim_table(), space_sd=True, irs_beta=0.5, ramp_iter=100)
-    both(oracle_mod, pb, 150)
+    return pb
 
 
-def test_outflow_406(oracle_mod):
+def outflow():
     pb = cases.outflow_channel(mach=0.5, p_exit=1.2, rel_fac=0.5)
     pb.add(cases.outflow_channel(mach=2.5, p_exit=1.0, rel_fac=1.0))
     for k in range(len(pb.cases)):
         pb.cases[k] = replace(pb.cases[k], thermo=nondim_table())
-    both(oracle_mod, pb, 100)
+    return pb
 
 
-def test_laminar_pipe_transport_tables(oracle_mod):
+def laminar_pipe():
     pb = cases.axisymmetric_pipe(seed=1, ni=64, nj=24, first_cell=5e-4, reynolds=1e4, precision="fp64", steps=10 ** 6)
     c = pb.cases[0]
     tab = nondim_table()
@@ -71,13 +95,69 @@ def test_laminar_pipe_transport_tables(oracle_mod):
     tab[2] = c.mu_ref * (0.3 + 0.7 * T)                  # mu(T), k(T) = mu cp / Pr at the nodes
     tab[3] = tab[2] * tab[1] / c.Pr
     pb.cases[0] = replace(c, thermo=tab)
-    both(oracle_mod, pb, 60)
+    return pb
+
+
+@pytest.mark.parametrize("riemann", ["HLLC", "HLLE", "HLLEM", "HLLE++", "HLLC+"])
+def test_moving_shock_varying_cp(oracle_mod, riemann):
+    both(oracle_mod, moving_shock(riemann), 200)
+
+
+def test_ramp_detector_and_smoothing(oracle_mod):
+    both(oracle_mod, ramp(), 150)
+
+
+def test_outflow_406(oracle_mod):
+    both(oracle_mod, outflow(), 100)
+
+
+def test_laminar_pipe_transport_tables(oracle_mod):
+    both(oracle_mod, laminar_pipe(), 60)
 
 
 def test_shu_osher_case_directory(oracle_mod):
     pb = casedir.load_case(CASE, thermo_mode="table")
     g, go = both(oracle_mod, pb, 300)
     assert go[0]["status"] == 0
+
+
+@pytest.mark.parametrize("riemann", ["HLLC", "HLLE", "HLLEM", "HLLE++", "HLLC+"])
+def test_mixed_moving_shock_varying_cp(oracle_mod, riemann):
+    mixed_close(oracle_mod, moving_shock(riemann), 200)
+
+
+def test_mixed_ramp_detector_and_smoothing(oracle_mod):
+    mixed_close(oracle_mod, ramp(), 150)
+
+
+def test_mixed_outflow_406(oracle_mod):
+    mixed_close(oracle_mod, outflow(), 100)
+
+
+def test_mixed_laminar_pipe_transport_tables(oracle_mod):
+    mixed_close(oracle_mod, laminar_pipe(), 60)
+
+
+def test_mixed_shu_osher_case_directory(oracle_mod):
+    mixed_close(oracle_mod, casedir.load_case(CASE, thermo_mode="table"), 300)
+
+
+def test_mixed_table_case_beside_closed_form_mixed_cases():
+    """A mixed table case (generic FP32-face kernel) batched beside a closed-form mixed case: both keep
+    their solo results (the closed-form case its k_stage_mx bits)."""
+    a = cases.riemann_planar(seed=11, ni=200, nj=8, steps=10 ** 6, precision="mixed")
+    t = moving_shock("HLLC")
+    t.cases[0] = replace(t.cases[0], precision="mixed")
+    solo_a, solo_t = GpuSolver(a), GpuSolver(t)
+    sa, st = solo_a.run(40)[0], solo_t.run(40)[0]
+    g = GpuSolver(cases.Problem())
+    ca, _ = g.add_cases(a)
+    ct, _ = g.add_cases(t)
+    out = g.run(40)
+    for c, so, solo in [(ca, sa, solo_a), (ct, st, solo_t)]:
+        assert out[c]["iter"] == 40 and out[c]["time"] == so["time"]
+    assert np.array_equal(g.state(0)[2:-2, 2:-2], solo_a.state(0)[2:-2, 2:-2])
+    assert np.array_equal(g.state(1)[2:-2, 2:-2], solo_t.state(0)[2:-2, 2:-2])
 
 
 def test_closed_form_case_beside_table_cases_keeps_its_bits():
@@ -95,11 +175,6 @@ def test_closed_form_case_beside_table_cases_keeps_its_bits():
 
 
 def test_refusals():
-    pb = cases.riemann_planar(seed=1, ni=64, nj=4, steps=10, precision="mixed")
-    pb.cases[0] = replace(pb.cases[0], thermo=nondim_table())
-    with pytest.raises(Q2DError) as e:
-        GpuSolver(pb)
-    assert e.value.code == -6 and "FP64" in str(e.value)
     bad = nondim_table()
     bad[0, 10] = bad[0, 9]                               # h flat on one interval: e(T) decreases
     pb = cases.riemann_planar(seed=1, ni=64, nj=4, steps=10)
