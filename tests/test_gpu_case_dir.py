@@ -107,6 +107,27 @@ def test_table_mode_case_directory(oracle_mod, tmp_path):
 
 
 @needs_host
+def test_mixed_table_mode_case_directory(oracle_mod, tmp_path):
+    """q2d_fcase --mixed --tables: the mixed form with the case's Air table.  The written field at t = 1.8
+    passes the LIMO3 gates; after 300 steps its density is within the mixed form's tolerance (rel. L1 1e-5)
+    of the oracle's FP64 table-mode run.  (Over the whole run, 681 steps, the LIMO3 switches at the
+    shocklets take the mixed form to 2.1e-5 rel. L1 with the table and 2.8e-5 with the closed form.)"""
+    d = copy_case(tmp_path)
+    soltime, xc, cv = run_case(d, "--mixed", "--tables")
+    assert limo3_gates(xc, cv[0, 0]) == []
+    o = oracle_mod.Oracle(casedir.load_case(CASE, thermo_mode="table"))
+    rc, out = o.run(100000)
+    assert rc == 0 and abs(soltime - out[0]["time"]) <= 1e-6 * out[0]["time"]
+    d300 = tmp_path / "steps300"
+    shutil.copytree(CASE, d300)
+    _, _, cv300 = run_case(d300, "--mixed", "--tables", "--max-steps", "300")
+    o300 = oracle_mod.Oracle(casedir.load_case(CASE, thermo_mode="table"))
+    rc, _ = o300.run(300)
+    rho_o = o300.state(0)[2, 2:-2, 0]
+    assert rc == 0 and np.abs(cv300[0, 0] - rho_o).sum() / np.abs(rho_o).sum() <= 1e-5
+
+
+@needs_host
 def test_runtime_reread_of_input_ini(tmp_path):
     """ini-diter = 50: the host re-reads input.ini every 50 steps and passes its runtime keys to the device
     (q2d_gpu_set_runtime, whose next step runs the dt pass).  With the file unchanged the field is the one
